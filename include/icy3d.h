@@ -1,0 +1,175 @@
+/* icy3d — C ABI of the MI355X (gfx950) 3D ice/water coupled solver step.
+ *
+ * Drop-in boundary for the reference's Python solver class
+ * ``iceflow3d.simulator.IceFlow3D`` (iceflow3d/simulator.py:40-947).  The
+ * reference has no native FFI: its operator API is the Python class, whose
+ * methods each launch one or more Taichi kernels.  Every entry point below
+ * replaces one such method (cited per function) and is enqueued on the
+ * caller's HIP stream.  The Python host (``projecticy_amd.iceflow3d``) owns
+ * every device buffer (PyTorch-ROCm tensors); this library never allocates
+ * persistent memory and keeps no global state.
+ *
+ * Conventions
+ *   - Every device array is a plain pointer in ``icy3d_grid``.  Grid fields
+ *     are structure-of-arrays planes: component c of cell n lives at
+ *     ``ptr[c * N + n]`` with ``n = (i * ny + j) * nz + k`` (z fastest).
+ *   - Return codes: 0 = success, >0 = hipError_t of a failed launch,
+ *     <0 = ICY3D_ERR_* below.
+ *   - ``stream`` is a ``hipStream_t`` passed as ``void*`` (null = default).
+ */
+#ifndef ICY3D_H
+#define ICY3D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ICY3D_MAX_BODIES 32
+#define ICY3D_STATE_STRIDE 21   /* origin[3], rotation[9] row-major, centre[3], v[3], omega[3] */
+#define ICY3D_REDUCED_STRIDE 19 /* mass, first[3], second[9], lower[3], upper[3] */
+
+enum {
+  ICY3D_OK = 0,
+  ICY3D_ERR_ARGUMENT = -1,
+  ICY3D_ERR_LAUNCH = -2,
+  ICY3D_ERR_TRAVEL = -4,   /* rigid displacement exceeds half a cell */
+  ICY3D_ERR_CONTACT = -5,  /* contact projection did not converge */
+  ICY3D_ERR_MANIFOLD = -6  /* wall contact manifold infeasible */
+};
+
+/* Device control block: every scalar the reference keeps in 0-d Taichi
+ * fields plus the per-body loads and mass reductions, laid out so that one
+ * small device-to-host copy returns all of them. */
+typedef struct icy3d_control {
+  int32_t invalid_index;   /* simulator.py:133 */
+  int32_t thermal_invalid; /* thermal.py:51 */
+  int32_t openings;        /* simulator.py:113 */
+  int32_t covered;
+  int32_t remaining;       /* simulator.py:114 */
+  int32_t repair_count;
+  int32_t refill_lower[3], refill_upper[3];
+  int32_t geometry_lower[3], geometry_upper[3];
+  float maximum_speed, peak_speed, phase_clip_volume;
+  float remap_energy, discarded_heat, max_outflow;
+  float loads[ICY3D_MAX_BODIES][6];                     /* hydrodynamic force, torque */
+  float reduced[ICY3D_MAX_BODIES][ICY3D_REDUCED_STRIDE]; /* simulator.py:738-776 */
+} icy3d_control;
+
+typedef struct icy3d_grid {
+  int32_t nx, ny, nz, boundary, bodies, mx, my, mz;
+  int32_t water_width, water_height, floor_depth, check_stability;
+  /* lattice-unit physics */
+  float rho_air, gravity_y, sigma, width, mobility, normal_eps, bulk_rate;
+  float threshold, max_velocity, expansion, water_t0, air_t, flux_limit;
+  float rho_water_si, rho_air_si, mu_air, nu_scale;
+  /* SI thermal constants */
+  float tm, rho_cp, cp_ice, k_water, k_ice, latent, cell_volume, dx;
+  float velocity_m_s, initial_cell_mass, mass_kg;
+  /* grid fields */
+  float *home, *home_post;             /* [10][N] HOME moments */
+  float* h;                            /* [7][N] D3Q7 AA slots */
+  float *phi, *pressure, *old_pressure, *coverage; /* [N] */
+  float *u, *old_u, *force, *accel, *grad, *flux;  /* [3][N] */
+  int8_t *id, *prev_id, *cov_id, *flag;            /* [N] */
+  float *refill_state, *refill_next;               /* [5][N] */
+  int8_t *refill_valid, *refill_next_valid;        /* [N] */
+  float *ref_density, *ref_pressure;               /* [ny] */
+  float* body;                                     /* [bodies][ICY3D_STATE_STRIDE] */
+  int32_t* material_size;                          /* [bodies][3] */
+  float* body_t0;                                  /* [bodies] */
+  float *material_mass, *material_energy, *material_heat; /* [bodies][mx][my][mz] */
+  float *water_volume, *water_energy, *target_energy, *sampled_ice_t; /* [N] */
+  float *energy_flux, *volume_flux;                /* [3][N] */
+  icy3d_control* control;
+} icy3d_grid;
+
+/* Host rigid-body arrays (FP64, row-major, length body_count). */
+typedef struct icy3d_rigid_arrays {
+  int32_t body_count;
+  double *origin, *q, *velocity, *omega, *com, *mass, *inertia, *lower, *upper;
+} icy3d_rigid_arrays;
+
+/* ---- initialisation (simulator.py:143-167) ---------------------------- */
+int icy3d_initialize_material(const icy3d_grid* g, void* stream);       /* thermal.py:53-62 */
+int icy3d_initialize_phase(const icy3d_grid* g, void* stream);          /* simulator.py:325-347 */
+int icy3d_initialize_water_heat(const icy3d_grid* g, void* stream);     /* thermal.py:88-96 */
+int icy3d_seed_populations(const icy3d_grid* g, void* stream);          /* simulator.py:385-395 */
+/* Equilibrium from the current phase/velocity (optionally a constant
+ * pressure) at every fluid cell: the reference tests' seed kernels
+ * (test_numerics.py:37-46, test_mirror_gradient.py:49-55). */
+int icy3d_set_equilibrium_all(const icy3d_grid* g, int parity, int with_phase, int use_constant_pressure,
+                              float pressure, void* stream);
+int icy3d_phase_warmup(const icy3d_grid* g, int steps, int* parity, void* stream); /* simulator.py:149-152 */
+
+/* ---- flow step (simulator.py:823-842) ----------------------------------- */
+int icy3d_classify(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream); /* home.py:280-284 */
+int icy3d_gradients(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream); /* gradients.py:352-386 */
+int icy3d_refill_gradients(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream); /* gradients.py:402-431 */
+int icy3d_collide_phase(const icy3d_grid* g, int parity, int moving_walls, void* stream); /* phase.py:90-127 */
+int icy3d_recover_phase(const icy3d_grid* g, int parity, int recover_flow, void* stream); /* phase.py:129-161 */
+int icy3d_recover_pressure(const icy3d_grid* g, void* stream);          /* home.py:347-353 */
+int icy3d_scan_stability(const icy3d_grid* g, void* stream);            /* simulator.py:796-805 */
+int icy3d_forces(const icy3d_grid* g, int with_flux, int moment_pressure, void* stream); /* simulator.py:521-557 */
+int icy3d_collide_momentum(const icy3d_grid* g, void* stream);          /* home.py:199-211 */
+int icy3d_stream_home(const icy3d_grid* g, void* stream);               /* home.py:256-345 */
+/* Fused device part of IceFlow3D.step up to the stability check
+ * (simulator.py:826-838): [gradients] collide_phase, forces, collide,
+ * zero loads, stream, recover+scan, gradients. */
+int icy3d_flow_step(const icy3d_grid* g, int parity, int refresh_gradients, void* stream);
+int icy3d_repair_phase_layout(const icy3d_grid* g, int parity, void* stream); /* phase.py:163-184 */
+int icy3d_export_phase(const icy3d_grid* g, int parity, float* out, void* stream); /* phase.py:186-191 */
+
+/* ---- geometry and refill (simulator.py:288-323, 589-736) ---------------- */
+int icy3d_rasterize(const icy3d_grid* g, void* stream);
+int icy3d_find_refill_changes(const icy3d_grid* g, void* stream);
+int icy3d_seed_refill_region(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream);
+int icy3d_propagate_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream);
+int icy3d_accept_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream);
+int icy3d_commit_refill(const icy3d_grid* g, void* stream);
+int icy3d_project_refill(const icy3d_grid* g, int parity, const int32_t lo[3], const int32_t hi[3], void* stream);
+
+/* ---- thermal (thermal.py:98-440) ---------------------------------------- */
+int icy3d_synchronize_water(const icy3d_grid* g, void* stream);
+int icy3d_thermal_fluxes(const icy3d_grid* g, int advect, void* stream);
+int icy3d_apply_thermal_fluxes(const icy3d_grid* g, float dt, int advect, void* stream);
+int icy3d_measure_outflow(const icy3d_grid* g, void* stream);
+int icy3d_body_conduction(const icy3d_grid* g, float dt, void* stream);
+int icy3d_exchange_heat(const icy3d_grid* g, float dt, const int32_t lo[3], const int32_t hi[3], void* stream);
+int icy3d_apply_melting(const icy3d_grid* g, void* stream);
+int icy3d_reduce_mass(const icy3d_grid* g, void* stream);              /* simulator.py:738-776 */
+int icy3d_sample_ice_temperature(const icy3d_grid* g, void* stream);
+int icy3d_check_heat(const icy3d_grid* g, void* stream);
+
+/* ---- diagnostics and probes (the reference tests' probe kernels) -------- */
+int icy3d_capillary_stress(const icy3d_grid* g, float* out, void* stream);  /* [9][N], gradients.py:147-158 */
+int icy3d_generic_forces(const icy3d_grid* g, float* out, void* stream);    /* [6][N], test_home_phase.py:69-78 */
+int icy3d_write_gradient_all(const icy3d_grid* g, void* stream);            /* test_home_phase.py:81-84 */
+int icy3d_probe_home(const float* data, int rows, float* populations, float* collided, void* stream);
+int icy3d_probe_phase(const float* data, int rows, float mobility, float width, float* out, void* stream);
+int icy3d_probe_viscosity(const float* phi, int n_phi, const float* temps, int n_t, float rho_w, float rho_a,
+                          float mu_air, float* out, void* stream);
+int icy3d_probe_stress_velocity(const icy3d_grid* g, const int32_t cell[3], const int32_t c[3], float* out,
+                                void* stream); /* simulator.py:559-587 */
+int icy3d_probe_mirror(const icy3d_grid* g, const int32_t cell[3], const int32_t c[3], float* out,
+                       void* stream); /* gradients.py:60-113 */
+
+/* ---- host rigid bodies (rigid.py:121-404), FP64 ------------------------- */
+int icy3d_rigid_integrate(const icy3d_rigid_arrays* s, const float* loads, const double gravity[3],
+                          double linear_damping, double angular_damping, int* failed_body);
+int icy3d_rigid_resolve_contacts(const icy3d_rigid_arrays* s, const int shape[3], int boundary, int iterations,
+                                 double residual[2]);
+int icy3d_rigid_contact_residuals(const icy3d_rigid_arrays* s, const int shape[3], int boundary,
+                                  double residual[2]);
+int icy3d_rigid_update_mass(const icy3d_rigid_arrays* s, const float* reduced);
+int icy3d_rigid_pack_state(const icy3d_rigid_arrays* s, float* out);
+
+/* Size of the structs this library was built with (ABI check). */
+int icy3d_abi_sizes(int32_t* grid_bytes, int32_t* control_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ICY3D_H */

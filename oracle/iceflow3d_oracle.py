@@ -1,0 +1,725 @@
+"""Oracle (test infrastructure only): FP64 CPU restatement of the reference
+3D coupled step ``IceFlow3D`` (``iceflow3d/simulator.py:40-947``).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+may import this module.  It is the checker, never the product: the product
+path (``projecticy_amd``) does not import it and fails when its HIP library
+is missing.
+
+Layout: every per-cell kernel lives in ``oracle3d.c`` (sequential FP64 C,
+compiled by ``build_oracle()`` with gcc); this module owns the NumPy arrays,
+restates the host logic of the reference (hydrostatic reference, refill
+loop, multi-rate thermal schedule) and drives the kernels in the reference's
+order.  The rigid bodies use ``rigid3d_oracle.OracleRigid``.
+
+Parity pinning (see tests/test_oracle3d_*.py): the reference's closed-form
+and invariant tests (HOME moments, third Hermite moments, D3Q7 7x7 central
+matrix, AA parity against a two-buffer push, quadrature, linear/mirror
+gradients, cut-wall stress, heat-exchange balance, conservative transport,
+column mass reduction, donor-flux momentum/energy budget, pressure product
+rule, refill locality) plus the reference's rigid fixtures.  Full multi-step
+trajectories have no reference-held vectors: "parity unpinned" beyond those
+invariants.
+"""
+
+import ctypes
+import math
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from .rigid3d_oracle import OracleRigid
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle3d.so"
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+_I8 = ctypes.POINTER(ctypes.c_int8)
+
+
+class O3D(ctypes.Structure):
+    """Mirror of ``struct o3d`` in oracle3d.h (field order matters)."""
+
+    _fields_ = (
+        [(n, ctypes.c_int) for n in ("nx", "ny", "nz", "boundary", "bodies", "mx", "my", "mz")]
+        + [(n, ctypes.c_int) for n in ("parity", "check_stability", "water_width", "water_height", "floor_depth")]
+        + [
+            (n, ctypes.c_double)
+            for n in (
+                "rho_air", "gravity_y", "sigma", "width", "mobility", "normal_eps", "bulk_rate",
+                "threshold", "max_velocity", "expansion", "water_t0", "air_t", "flux_limit",
+                "rho_water_si", "rho_air_si", "mu_air", "nu_scale",
+                "tm", "rho_cp", "cp_ice", "k_water", "k_ice", "latent", "cell_volume", "dx",
+                "velocity_m_s", "initial_cell_mass", "mass_kg", "rho_ice",
+            )
+        ]
+        + [(n, _D) for n in ("home", "home_post", "h", "phi", "pressure", "old_pressure", "coverage",
+                             "u", "old_u", "force", "accel", "grad", "flux", "stress")]
+        + [(n, _I32) for n in ("id", "prev_id", "cov_id")]
+        + [("flag", _I8), ("refill_state", _D), ("refill_next", _D), ("refill_valid", _I8),
+           ("refill_next_valid", _I8), ("ref_density", _D), ("ref_pressure", _D)]
+        + [(n, _D) for n in ("body_origin", "body_rot", "body_center", "body_v", "body_w",
+                             "hydro_force", "hydro_torque")]
+        + [("material_size", _I32), ("body_t0", _D)]
+        + [(n, _D) for n in ("material_mass", "material_energy", "material_heat", "water_volume",
+                             "water_energy", "target_energy", "sampled_ice_t", "energy_flux",
+                             "volume_flux", "reduced")]
+        + [(n, ctypes.c_int32) for n in ("invalid_index", "thermal_invalid", "openings", "covered", "remaining")]
+        + [(n, ctypes.c_int32 * 3) for n in ("refill_lower", "refill_upper", "geo_lower", "geo_upper")]
+        + [(n, ctypes.c_double) for n in ("maximum_speed", "peak_speed", "phase_clip_volume",
+                                          "remap_energy", "discarded_heat", "max_outflow")]
+    )
+
+
+def build_oracle(force=False):
+    """Compile oracle3d.c with gcc into oracle/_build (no reference build)."""
+    source = HERE / "oracle3d.c"
+    LIB_PATH.parent.mkdir(exist_ok=True)
+    if force or not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < source.stat().st_mtime:
+        tmp = LIB_PATH.with_suffix(f".so.tmp{os.getpid()}")
+        subprocess.run(
+            ["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-fno-fast-math", str(source), "-o", str(tmp), "-lm"],
+            check=True,
+        )
+        os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+_LIB = None
+
+
+def oracle_lib():
+    global _LIB
+    if _LIB is None:
+        if not LIB_PATH.exists():
+            build_oracle()
+        lib = ctypes.CDLL(str(LIB_PATH))
+        ptr = ctypes.POINTER(O3D)
+        for name in (
+            "rasterize", "classify", "collide_momentum", "stream_home", "recover_pressure",
+            "scan_stability", "repair_phase_layout", "find_refill_changes", "seed_refill_region",
+            "propagate_refill", "accept_refill", "commit_refill", "refill_gradients", "project_refill",
+            "synchronize_water", "initialize_water_heat", "measure_outflow", "initialize_material",
+            "apply_melting", "reduce_mass", "sample_ice_temperature", "check_heat", "initialize_phase",
+            "seed_populations",
+        ):
+            fn = getattr(lib, "o3d_" + name)
+            fn.argtypes = [ptr]
+            fn.restype = None
+        for name, extra in (
+            ("gradients", [ctypes.c_int]), ("collide_phase", [ctypes.c_int]),
+            ("recover_phase", [ctypes.c_int]), ("forces", [ctypes.c_int, ctypes.c_int]),
+            ("thermal_fluxes", [ctypes.c_int]), ("apply_thermal_fluxes", [ctypes.c_double, ctypes.c_int]),
+            ("body_conduction", [ctypes.c_double]), ("exchange_heat", [ctypes.c_double]),
+            ("export_phase", [_D]), ("set_equilibrium_all", [ctypes.c_int]),
+        ):
+            fn = getattr(lib, "o3d_" + name)
+            fn.argtypes = [ptr] + extra
+            fn.restype = None
+        lib.o3d_probe_stress_velocity.argtypes = [ptr, _I32, _I32, _D]
+        lib.o3d_probe_mirror.argtypes = [ptr, _I32, _I32, _D]
+        lib.o3d_exchange_heat_cell.argtypes = [ptr, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        lib.o3d_write_gradient.argtypes = [ptr, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.o3d_reconstruct.argtypes = [ctypes.c_int, _D]
+        lib.o3d_reconstruct.restype = ctypes.c_double
+        lib.o3d_central_collision.argtypes = [_D, _D, ctypes.c_double, ctypes.c_double, _D]
+        lib.o3d_phase_collision.argtypes = [_D, ctypes.c_double, _D, _D, ctypes.c_double, ctypes.c_double, _D, _D]
+        lib.o3d_mixture_nu.argtypes = [ptr, ctypes.c_double, ctypes.c_double]
+        lib.o3d_mixture_nu.restype = ctypes.c_double
+        lib.o3d_force_at.argtypes = [ptr, _I32, ctypes.c_int, ctypes.c_int, _D]
+        lib.o3d_acceleration_at.argtypes = [ptr, _I32, _D, ctypes.c_int, ctypes.c_int, _D]
+        _LIB = lib
+    return _LIB
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+# -- pointwise primitives (used directly by the closed-form tests) ----------
+
+def reconstruct(q, m):
+    """home.py:30-53: population q of the HOME moment vector m (FP64)."""
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    return oracle_lib().o3d_reconstruct(int(q), _dptr(m))
+
+
+def central_collision(m, acceleration, rate, bulk):
+    """home.py:61-78."""
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    a = np.ascontiguousarray(acceleration, dtype=np.float64)
+    out = np.zeros(10)
+    oracle_lib().o3d_central_collision(_dptr(m), _dptr(a), rate, bulk, _dptr(out))
+    return out
+
+
+def phase_collision(h, phi, u, normal, mobility=0.1, width=5.0):
+    """phase.py:38-60: returns (post[7], flux[3])."""
+    h = np.ascontiguousarray(h, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    n = np.ascontiguousarray(normal, dtype=np.float64)
+    post, flux = np.zeros(7), np.zeros(3)
+    oracle_lib().o3d_phase_collision(_dptr(h), phi, _dptr(u), _dptr(n), mobility, width, _dptr(post), _dptr(flux))
+    return post, flux
+
+
+
+
+class OField:
+    """Reference-style field view (``to_numpy``/``from_numpy``/``fill``)
+    over an oracle NumPy array whose buffer stays fixed."""
+
+    def __init__(self, array):
+        self.a = array
+
+    @property
+    def shape(self):
+        return self.a.shape
+
+    @property
+    def dtype(self):
+        return self.a.dtype
+
+    def to_numpy(self):
+        return self.a.copy()
+
+    def from_numpy(self, value):
+        self.a[...] = value
+
+    def fill(self, value):
+        self.a[...] = value
+
+    def __getitem__(self, key):
+        return self.a[key]
+
+    def __setitem__(self, key, value):
+        self.a[key] = value
+
+
+class OScalar:
+    """A 0-d reference field (``x[None]``) stored in the C struct."""
+
+    def __init__(self, struct, name):
+        self.s, self.name = struct, name
+
+    def _get(self):
+        value = getattr(self.s, self.name)
+        return np.array(list(value)) if hasattr(value, "__len__") else value
+
+    def __getitem__(self, key):
+        return self._get()
+
+    def __setitem__(self, key, value):
+        current = getattr(self.s, self.name)
+        if hasattr(current, "__len__"):
+            for i, v in enumerate(value):
+                current[i] = int(v)
+        else:
+            setattr(self.s, self.name, type(current)(value))
+
+    def to_numpy(self):
+        return np.array(self._get())
+
+    def from_numpy(self, value):
+        self[None] = value.tolist() if np.ndim(value) else value.item()
+
+    def fill(self, value):
+        self[None] = value
+
+
+# oracle C member -> reference attribute name
+FIELD_NAMES = {
+    "home": "home", "home_post": "home_post", "h": "h", "phi": "water_phase",
+    "pressure": "pressure_lattice", "old_pressure": "old_pressure", "coverage": "coverage",
+    "u": "velocity_lattice", "old_u": "old_velocity", "force": "force_density",
+    "accel": "home_acceleration", "grad": "phase_gradient_lattice", "flux": "phase_mass_flux",
+    "stress": "capillary_stress", "id": "solid_body_id", "prev_id": "previous_body_id",
+    "cov_id": "coverage_body_id", "flag": "_home_boundary", "refill_state": "refill_state",
+    "refill_next": "refill_next", "refill_valid": "refill_valid", "refill_next_valid": "refill_next_valid",
+    "ref_density": "reference_density", "ref_pressure": "reference_pressure",
+    "body_origin": "body_origin", "body_rot": "body_rotation", "body_center": "body_center",
+    "body_v": "body_velocity", "body_w": "body_angular_velocity", "hydro_force": "hydrodynamic_force",
+    "hydro_torque": "hydrodynamic_torque", "material_size": "material_size",
+    "body_t0": "initial_body_temperature", "material_mass": "material_mass",
+    "material_energy": "material_energy", "material_heat": "material_heat",
+    "water_volume": "water_volume", "water_energy": "water_energy", "target_energy": "target_energy",
+    "sampled_ice_t": "sampled_ice_temperature", "energy_flux": "energy_flux",
+    "volume_flux": "volume_flux", "reduced": "reduced_state",
+}
+SCALAR_NAMES = {
+    "parity": "phase_parity", "invalid_index": "invalid_index", "maximum_speed": "maximum_speed",
+    "peak_speed": "peak_speed", "phase_clip_volume": "phase_clip_volume", "remap_energy": "remap_energy",
+    "discarded_heat": "discarded_heat", "max_outflow": "max_outflow", "thermal_invalid": "thermal_invalid",
+    "openings": "openings", "remaining": "remaining", "refill_lower": "refill_lower",
+    "refill_upper": "refill_upper", "geo_lower": "_gradient_geometry_lower",
+    "geo_upper": "_gradient_geometry_upper",
+}
+# Scratch fields that are not restart state (simulator.py:42-45, :82-84).
+TRANSIENT = frozenset(("_gradient_geometry_lower", "_gradient_geometry_upper", "_home_boundary"))
+
+
+class Oracle3D:
+    """FP64 restatement of ``IceFlow3D`` (simulator.py:47-167, 823-856).
+
+    Attributes carry the reference's field names and logical shapes and
+    behave like Taichi fields (``to_numpy``, ``from_numpy``, ``fill``,
+    ``x[None]`` for scalars).
+    """
+
+    def __init__(self, config, *, warmup_steps=None):
+        from projecticy_amd.iceflow3d.config import LatticeScales
+        from projecticy_amd.iceflow3d.viscosity import air_dynamic_viscosity_pa_s
+
+        self.config = config
+        self.scales = LatticeScales.from_iceflow_config(config)
+        self.shape = (config.nx, config.ny, config.nz)
+        self.nx, self.ny, self.nz = self.shape
+        self.steps = 0
+        self.body_count = B = len(config.bodies)
+        self._thermal_pending = 0
+        self.pending_cells = 0
+        th, props = config.thermal, config.thermal.properties
+        sizes = np.array([b.size_cells for b in config.bodies], dtype=np.int32)
+        mshape = (B, *map(int, sizes.max(axis=0)))
+        N = self.shape
+        arrays = {
+            "home": np.zeros((*N, 10)), "home_post": np.zeros((*N, 10)), "h": np.zeros((*N, 7)),
+            "stress": np.zeros((*N, 3, 3)), "refill_state": np.zeros((*N, 5)), "refill_next": np.zeros((*N, 5)),
+            "id": np.zeros(N, np.int32), "prev_id": np.zeros(N, np.int32), "cov_id": np.zeros(N, np.int32),
+            "flag": np.zeros(N, np.int8), "refill_valid": np.zeros(N, np.int8),
+            "refill_next_valid": np.zeros(N, np.int8),
+            "ref_density": np.zeros(self.ny), "ref_pressure": np.zeros(self.ny),
+            "body_rot": np.zeros((B, 3, 3)), "material_size": np.ascontiguousarray(sizes),
+            "body_t0": np.array(
+                [th.initial_ice_temperature_c if b.temperature_c is None else b.temperature_c
+                 for b in config.bodies], dtype=float),
+            "material_mass": np.zeros(mshape), "material_energy": np.zeros(mshape),
+            "material_heat": np.zeros(mshape), "reduced": np.zeros((B, 19)),
+        }
+        for name in ("phi", "pressure", "old_pressure", "coverage", "water_volume", "water_energy",
+                     "target_energy", "sampled_ice_t"):
+            arrays[name] = np.zeros(N)
+        for name in ("u", "old_u", "force", "accel", "grad", "flux", "energy_flux", "volume_flux"):
+            arrays[name] = np.zeros((*N, 3))
+        for name in ("body_origin", "body_center", "body_v", "body_w", "hydro_force", "hydro_torque"):
+            arrays[name] = np.zeros((B, 3))
+        self._arrays = arrays
+
+        dx = self.scales.dx_m
+        self.cell_volume = dx**3
+        self.rho_air = config.rho_air / config.rho_water
+        self.sigma = config.surface_tension_n_m * self.scales.dt_s**2 / (config.rho_water * dx**3)
+        self.mu_air = air_dynamic_viscosity_pa_s(th.initial_air_temperature_c)
+        self.rho_cp = config.rho_water * props.specific_heat_water_j_kg_k
+        self.tm = props.melting_temperature_c
+        self.cp_ice = props.specific_heat_ice_j_kg_k
+        self.latent = props.latent_heat_j_kg
+        self.initial_cell_mass = config.rho_ice * self.cell_volume
+        s = O3D()
+        s.nx, s.ny, s.nz = self.shape
+        s.boundary, s.bodies = config.boundary_cells, B
+        s.mx, s.my, s.mz = mshape[1:]
+        s.parity = 0
+        s.check_stability = int(config.check_stability)
+        s.water_width, s.water_height = config.water_width, config.water_height
+        s.floor_depth = config.floor_water_depth_cells
+        s.rho_air = self.rho_air
+        s.gravity_y = config.gravity[1] * self.scales.dt_s**2 / dx
+        s.sigma = self.sigma
+        s.width, s.mobility = config.interface_width, config.mobility
+        s.normal_eps, s.bulk_rate = config.phase_normal_epsilon, config.cm_acoustic_relaxation_rate
+        s.threshold, s.max_velocity = th.solid_liquid_threshold, config.maximum_velocity_lattice
+        s.expansion, s.water_t0 = th.thermal_expansion_water_1_k, th.initial_water_temperature_c
+        s.air_t, s.flux_limit = th.initial_air_temperature_c, config.phase_flux_acceleration_limit
+        s.rho_water_si, s.rho_air_si, s.mu_air = config.rho_water, config.rho_air, self.mu_air
+        s.nu_scale = self.scales.dt_s / dx**2
+        s.tm, s.rho_cp, s.cp_ice = self.tm, self.rho_cp, self.cp_ice
+        s.k_water, s.k_ice, s.latent = props.conductivity_water_w_m_k, props.conductivity_ice_w_m_k, self.latent
+        s.cell_volume, s.dx = self.cell_volume, dx
+        s.velocity_m_s, s.initial_cell_mass = self.scales.velocity_m_s, self.initial_cell_mass
+        s.mass_kg, s.rho_ice = self.scales.mass_kg, config.rho_ice
+        self._s = s
+        for cname, array in arrays.items():
+            setattr(s, cname, array.ctypes.data_as(dict(O3D._fields_)[cname]))
+            setattr(self, FIELD_NAMES[cname], OField(array))
+        for cname, attr in SCALAR_NAMES.items():
+            setattr(self, attr, OScalar(s, cname))
+        self.gravity = (0.0, s.gravity_y, 0.0)
+
+        self.rigid = OracleRigid(self)
+        self.call("initialize_material")
+        self.rigid.update_mass()
+        self.call("rasterize")
+        self.call("initialize_phase")
+        warm = config.phase_warmup_steps if warmup_steps is None else warmup_steps
+        for _ in range(warm):
+            self._gradients()
+            self._collide_phase(moving_walls=False)
+            self._recover_phase()
+        self.call("initialize_water_heat")
+        self._build_reference()
+        self._gradients()
+        self._forces(False)
+        self._seed_populations()
+        self.initial_ice_mass_kg = self.rigid.mass.copy() * self.scales.mass_kg
+        self.initial_water_volume_m3 = float(self._arrays["water_volume"].sum())
+        self.initial_phase_volume_m3 = float(self._arrays["phi"].sum() * self.cell_volume)
+        self._gradient_cache_valid = True
+
+    # -- plumbing ----------------------------------------------------------
+    def call(self, name, *args):
+        getattr(oracle_lib(), "o3d_" + name)(ctypes.byref(self._s), *args)
+
+    def fields(self):
+        """Restart-state fields by reference name (checkpoint._fields)."""
+        out = {}
+        for cname in self._arrays:
+            name = FIELD_NAMES[cname]
+            if name not in TRANSIENT and name != "reduced_state":
+                out[name] = getattr(self, name)
+        for name in SCALAR_NAMES.values():
+            if name not in TRANSIENT:
+                out[name] = getattr(self, name)
+        return out
+
+    def boundary_flags(self):
+        """Current HOME link classes (-1 solid, 0 bulk, 1 tank, 2 ice)."""
+        return self._home_boundary.to_numpy()
+
+    @property
+    def physical_time_s(self):
+        return self.steps * self.scales.dt_s
+
+    def invalidate_gradient_cache(self):
+        self._gradient_cache_valid = False
+
+    # -- reference kernel entry points ---------------------------------------
+    def _classify_home(self):
+        self.call("classify")
+
+    def _gradients(self, classified=False):
+        self.call("gradients", int(classified))
+
+    def _collide_phase(self, moving_walls=True):
+        self.call("collide_phase", int(moving_walls))
+
+    def _recover_phase(self, recover_flow=False):
+        self.call("recover_phase", int(recover_flow))
+
+    def _recover_pressure(self):
+        self.call("recover_pressure")
+
+    def _scan_stability(self):
+        self.call("scan_stability")
+
+    def _forces(self, with_flux, moment_pressure=False):
+        self.call("forces", int(with_flux), int(moment_pressure))
+
+    def _collide_momentum(self):
+        self.call("collide_momentum")
+
+    def _stream_home(self, classified=False):
+        if not classified:
+            self.call("classify")
+        self.call("stream_home")
+
+    def _seed_populations(self):
+        self.call("seed_populations")
+
+    def _rasterize(self):
+        self.call("rasterize")
+
+    def _repair_phase_layout(self):
+        self.call("repair_phase_layout")
+
+    def _project_refill(self):
+        self.call("project_refill")
+
+    def _refill_gradients(self):
+        self.call("refill_gradients")
+
+    def _synchronize_water(self):
+        self.call("synchronize_water")
+
+    def _thermal_fluxes(self, advect):
+        self.call("thermal_fluxes", int(advect))
+
+    def _apply_thermal_fluxes(self, dt, advect):
+        self.call("apply_thermal_fluxes", float(dt), int(advect))
+
+    def _measure_outflow(self):
+        self.call("measure_outflow")
+
+    def _body_conduction(self, dt):
+        self.call("body_conduction", float(dt))
+
+    def _exchange_heat(self, dt):
+        self.call("exchange_heat", float(dt))
+
+    def _exchange_heat_full(self, dt):
+        """The reference test's ``full_heat`` probe: every cell, no box."""
+        for i in range(self.nx):
+            for j in range(self.ny):
+                for k in range(self.nz):
+                    oracle_lib().o3d_exchange_heat_cell(ctypes.byref(self._s), i, j, k, float(dt))
+
+    def _apply_melting(self):
+        self.call("apply_melting")
+
+    def _reduce_mass(self):
+        self.call("reduce_mass")
+
+    def _sample_ice_temperature(self):
+        self.call("sample_ice_temperature")
+
+    def _check_heat(self):
+        self.call("check_heat")
+
+    def _write_gradient_all(self):
+        """The reference test's generic gradient probe (every cell)."""
+        for i in range(self.nx):
+            for j in range(self.ny):
+                for k in range(self.nz):
+                    oracle_lib().o3d_write_gradient(ctypes.byref(self._s), i, j, k)
+
+    def generic_forces(self, with_flux=True, moment_pressure=True):
+        """The reference test's generic force probe: [*shape, 6]."""
+        out = np.zeros((*self.shape, 6))
+        f, a = np.zeros(3), np.zeros(3)
+        ids = self._arrays["id"]
+        for I in zip(*np.nonzero(ids == -1)):
+            idx = np.array(I, dtype=np.int32)
+            oracle_lib().o3d_force_at(ctypes.byref(self._s), idx.ctypes.data_as(_I32), int(with_flux), 0, _dptr(f))
+            oracle_lib().o3d_acceleration_at(
+                ctypes.byref(self._s), idx.ctypes.data_as(_I32), _dptr(f), 0, int(moment_pressure), _dptr(a)
+            )
+            out[I] = np.r_[f, a]
+        return out
+
+    def _stress_neighbor_velocity(self, I, c):
+        out = np.zeros(3)
+        I = np.asarray(I, dtype=np.int32)
+        c = np.asarray(c, dtype=np.int32)
+        lib = oracle_lib()
+        lib.o3d_probe_stress_velocity(ctypes.byref(self._s), I.ctypes.data_as(_I32), c.ctypes.data_as(_I32), _dptr(out))
+        return out
+
+    def mirror_probe(self, I, c):
+        out = np.zeros(4)
+        I = np.asarray(I, dtype=np.int32)
+        c = np.asarray(c, dtype=np.int32)
+        oracle_lib().o3d_probe_mirror(ctypes.byref(self._s), I.ctypes.data_as(_I32), c.ctypes.data_as(_I32), _dptr(out))
+        return out
+
+    def _set_equilibrium_all(self, velocity=None, pressure=None, with_phase=True):
+        """The reference tests' seed kernels (``seed_velocity``,
+        ``seed_cached_source``): equilibrium at every fluid cell from the
+        current phase, pressure (or a constant) and velocity (or a constant)."""
+        ids = self._arrays["id"]
+        if pressure is not None:
+            self._arrays["pressure"][ids == -1] = pressure
+        if velocity is not None:
+            self._arrays["u"][ids == -1] = velocity
+        self.call("set_equilibrium_all", int(with_phase))
+
+    # -- host logic ---------------------------------------------------------
+    def _build_reference(self):
+        """simulator.py:349-383: frozen hydrostatic profile (host NumPy)."""
+        cfg = self.config
+        phase, ids = self._arrays["phi"], self._arrays["id"]
+        ny = self.ny
+        ref = (np.arange(ny) < cfg.boundary_cells + cfg.floor_water_depth_cells).astype(float)
+        full_width = cfg.water_width == self.nx - cfg.boundary_cells
+        mask = ids == -1
+        if full_width:
+            ref = (np.arange(ny) < cfg.water_height).astype(float)
+        else:
+            mask = mask.copy()
+            mask[: cfg.water_width] = False
+        if cfg.floor_water_depth_cells or full_width:
+            counts = mask.sum(axis=(0, 2))
+            sums = np.where(mask, phase, 0).sum(axis=(0, 2))
+            ref = np.where(counts > 0, sums / np.maximum(counts, 1), ref)
+        g = self._s.gravity_y
+        rho = self.rho_air + (1 - self.rho_air) * ref
+        p = -g * (np.cumsum(rho[::-1])[::-1] - 0.5 * rho)
+        excess = np.where(ids == -1, (phase - ref[None, :, None]) * (1 - self.rho_air), 0.0)
+        dynamic = -g * (np.cumsum(excess[:, ::-1, :], axis=1)[:, ::-1, :] - 0.5 * excess)
+        self._arrays["ref_density"][:] = rho
+        self._arrays["ref_pressure"][:] = p
+        self._arrays["pressure"][...] = dynamic
+
+    def check_stability(self, scanned=False):
+        """simulator.py:807-821."""
+        if not scanned:
+            self._scan_stability()
+        index = self._s.invalid_index
+        if index < math.prod(self.shape):
+            cell = tuple(int(v) for v in np.unravel_index(index, self.shape))
+            raise RuntimeError(f"unstable fluid at step {self.steps + 1}, cell {cell}")
+        return float(self._s.maximum_speed)
+
+    def _seed_refill(self):
+        self.call("find_refill_changes")
+        counts = (self._s.openings, self._s.covered)
+        if counts[0] > 0:
+            self.call("seed_refill_region")
+        return counts
+
+    def _refresh_geometry(self, phase_changed=True):
+        """simulator.py:708-736."""
+        self._rasterize()
+        self._repair_phase_layout()
+        remaining, covered = self._seed_refill()
+        has_openings = remaining > 0
+        self.pending_cells = 0
+        while remaining:
+            self.call("propagate_refill")
+            new = self._s.remaining
+            self.call("accept_refill")
+            if new == remaining:
+                self.pending_cells = new
+                break
+            remaining = new
+        self.call("commit_refill")
+        self._refill_gradients()
+        if has_openings or covered:
+            self._project_refill()
+        if phase_changed or has_openings or covered:
+            self._synchronize_water()
+
+    def _advance_advection(self, speed_bound=None):
+        """thermal.py:208-226."""
+        th = self.config.thermal
+        if not th.advection_enabled:
+            return
+        rate = 3 * speed_bound if speed_bound is not None else math.inf
+        if rate > th.max_courant_number:
+            self._measure_outflow()
+            rate = float(self._s.max_outflow)
+        if not math.isfinite(rate):
+            raise RuntimeError(f"nonfinite thermal velocity at step {self.steps + 1}")
+        substeps = max(1, math.ceil(rate / th.max_courant_number))
+        if substeps > th.max_substeps_per_update:
+            raise RuntimeError(f"thermal update needs {substeps} substeps at step {self.steps + 1}")
+        for _ in range(substeps):
+            self._thermal_fluxes(True)
+            self._apply_thermal_fluxes(self.scales.dt_s / substeps, True)
+            self._synchronize_water()
+
+    def _advance_conduction(self, dt):
+        """thermal.py:234-258."""
+        th, props = self.config.thermal, self.config.thermal.properties
+        alpha = max(
+            props.conductivity_water_w_m_k / self.rho_cp,
+            props.conductivity_ice_w_m_k / (self.config.rho_ice * props.specific_heat_ice_j_kg_k),
+        )
+        substeps = max(1, math.ceil(alpha * dt / (self.scales.dx_m**2 * th.max_fourier_number)))
+        if substeps > th.max_substeps_per_update:
+            raise RuntimeError(f"thermal update needs {substeps} substeps at step {self.steps + 1}")
+        for _ in range(substeps):
+            self._body_conduction(dt / substeps)
+            self._thermal_fluxes(False)
+            self._apply_thermal_fluxes(dt / substeps, False)
+            self._exchange_heat(dt / substeps)
+            self._apply_melting()
+            self.rigid.update_mass(upload=False)
+            self.rigid.resolve_contacts()
+            self.rigid.upload()
+            self._refresh_geometry(phase_changed=False)
+        if self.config.check_stability:
+            self._check_heat()
+            if self._s.thermal_invalid:
+                raise RuntimeError(f"invalid thermal state at step {self.steps + 1}")
+
+    def step(self, num_steps=1):
+        """simulator.py:823-850."""
+        for _ in range(int(num_steps)):
+            if not self._gradient_cache_valid:
+                self._gradients()
+                self._gradient_cache_valid = True
+            self._collide_phase()
+            self._forces(True, moment_pressure=True)
+            self._collide_momentum()
+            self._arrays["hydro_force"][...] = 0
+            self._arrays["hydro_torque"][...] = 0
+            self._stream_home(classified=True)
+            self._recover_phase(recover_flow=True)
+            self._gradients(classified=True)
+            speed_bound = self.check_stability(scanned=True) if self.config.check_stability else None
+            self.rigid.advance()
+            self._refresh_geometry()
+            self._advance_advection(speed_bound)
+            self._thermal_pending += 1
+            if self._thermal_pending >= self.config.thermal.update_interval_lbm_steps:
+                self._advance_conduction(self._thermal_pending * self.scales.dt_s)
+                self._thermal_pending = 0
+            self.steps += 1
+
+    def flush_thermal(self):
+        if self._thermal_pending:
+            self._advance_conduction(self._thermal_pending * self.scales.dt_s)
+            self._thermal_pending = 0
+
+    def export_phase_populations(self, out=None):
+        result = np.zeros((*self.shape, 7))
+        self.call("export_phase", _dptr(result))
+        if out is not None:
+            out[...] = result
+        return result
+
+    def snapshot(self):
+        """simulator.py:858-889."""
+        th = self.config.thermal
+        ids = self.solid_body_id.to_numpy()
+        volume, energy = self.water_volume.to_numpy(), self.water_energy.to_numpy()
+        cp = self.config.rho_water * th.properties.specific_heat_water_j_kg_k
+        temp = np.full(self.shape, th.initial_air_temperature_c)
+        wet = volume > 1e-30
+        temp[wet] = energy[wet] / (cp * volume[wet]) + th.properties.melting_temperature_c
+        self._sample_ice_temperature()
+        ice = self.sampled_ice_temperature.to_numpy()
+        temp[ids >= 0] = ice[ids >= 0]
+        return {
+            "water_phase": self.water_phase.to_numpy(),
+            "solid_body_id": ids,
+            "ice_fraction": self.coverage.to_numpy(),
+            "velocity_m_s": self.velocity_lattice.to_numpy() * self.scales.velocity_m_s,
+            "pressure_pa": (self.pressure_lattice.to_numpy() + self.reference_pressure.to_numpy()[None, :, None])
+            * self.scales.pressure_pa,
+            "temperature_c": temp,
+        }
+
+    def diagnostics(self):
+        """simulator.py:891-947 (the subset with oracle counterparts)."""
+        self._scan_stability()
+        volume = self.water_volume.to_numpy()
+        rigid = self.rigid
+        centers = self.body_center.to_numpy() * self.scales.dx_m
+        result = {
+            "step": self.steps,
+            "time_s": self.physical_time_s,
+            "water_volume_m3": float(volume.sum()),
+            "water_energy_j": float(self.water_energy.to_numpy().sum()),
+            "ice_sensible_energy_j": float(self.material_energy.to_numpy().sum()),
+            "maximum_fluid_speed_lattice": float(self._s.maximum_speed),
+            "phase_volume_change_ratio": float(
+                self.water_phase.to_numpy().sum() * self.cell_volume / self.initial_phase_volume_m3 - 1
+            ),
+            "ice_mass_kg": float(rigid.mass.sum() * self.scales.mass_kg),
+            "melted_mass_kg": float(self.initial_ice_mass_kg.sum() - rigid.mass.sum() * self.scales.mass_kg),
+            "phase_clip_volume_m3": float(self._s.phase_clip_volume) * self.scales.dx_m**3,
+            "thermal_remap_energy_j": float(self._s.remap_energy),
+            "discarded_melt_heat_j": float(self._s.discarded_heat),
+            "pending_refill_cells": self.pending_cells,
+        }
+        for b in range(self.body_count):
+            result[f"ice_{b}_mass_kg"] = float(rigid.mass[b] * self.scales.mass_kg)
+            for a, label in enumerate("xyz"):
+                result[f"ice_{b}_{label}_m"] = float(centers[b, a])
+            for a, label in enumerate("wxyz"):
+                result[f"ice_{b}_q{label}"] = float(rigid.q[b, a])
+        return result

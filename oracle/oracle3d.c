@@ -1,0 +1,1489 @@
+/* Oracle (test infrastructure only): FP64 restatement of the reference 3D
+ * kernels.  Each function cites the reference file:line it restates.  The
+ * loops are sequential; every reference kernel is race-free apart from its
+ * atomics (AA writes the seven addresses it read, pull streaming writes only
+ * its own cell), so sequential order reproduces the parallel semantics and
+ * the atomics become exact FP64 sums. */
+#include "oracle3d.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define Q27 27
+#define REST 13
+
+static const double W_ORDER[4] = {8.0 / 27.0, 2.0 / 27.0, 1.0 / 54.0, 1.0 / 216.0};
+static const int C7[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+static const int OPP7[7] = {0, 2, 1, 4, 3, 6, 5};
+static const int FACES[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+static const double WATER_TERMS[4][2] = {{280.68, -1.9}, {511.45, -7.7}, {61.131, -19.6}, {0.45903, -40.0}};
+
+/* lattice.py:28-46 */
+static void dir27(int q, int c[3]) {
+  c[0] = q / 9 - 1;
+  c[1] = q / 3 % 3 - 1;
+  c[2] = q % 3 - 1;
+}
+static double w27(int q) {
+  int c[3];
+  dir27(q, c);
+  return W_ORDER[abs(c[0]) + abs(c[1]) + abs(c[2])];
+}
+static double clamp01(double v) { return fmin(1.0, fmax(0.0, v)); }
+static double density(const o3d* s, double phi) { return s->rho_air + (1 - s->rho_air) * clamp01(phi); }
+
+static long idx(const o3d* s, int i, int j, int k) { return ((long)i * s->ny + j) * s->nz + k; }
+static int inside(const o3d* s, const int I[3]) {
+  return I[0] >= 0 && I[0] < s->nx && I[1] >= 0 && I[1] < s->ny && I[2] >= 0 && I[2] < s->nz;
+}
+static long lin(const o3d* s, const int I[3]) { return idx(s, I[0], I[1], I[2]); }
+static int fluid(const o3d* s, const int I[3]) { return inside(s, I) && s->id[lin(s, I)] == -1; }
+static long cells(const o3d* s) { return (long)s->nx * s->ny * s->nz; }
+
+/* lattice.py:131-137 and 49-55 */
+static double water_mu(double t_c) {
+  double t = (t_c + 273.15) / 300.0, mu = 0;
+  for (int i = 0; i < 4; ++i) mu += WATER_TERMS[i][0] * pow(t, WATER_TERMS[i][1]);
+  return mu * 1e-6;
+}
+double o3d_mixture_nu(const o3d* s, double phi, double t_c) {
+  double f = clamp01(phi);
+  double mu = f * water_mu(t_c) + (1 - f) * s->mu_air;
+  double rho = f * s->rho_water_si + (1 - f) * s->rho_air_si;
+  return mu / rho;
+}
+
+/* thermal.py:71-86 */
+static double water_t(const o3d* s, long n) {
+  double t = s->air_t;
+  if (s->water_volume[n] > 1e-30) t = s->tm + s->water_energy[n] / (s->rho_cp * s->water_volume[n]);
+  return t;
+}
+static long mat(const o3d* s, int b, int i, int j, int k) { return (((long)b * s->mx + i) * s->my + j) * s->mz + k; }
+static int mat_inside(const o3d* s, int b, const int J[3]) {
+  const int* m = s->material_size + 3 * b;
+  return J[0] >= 0 && J[0] < m[0] && J[1] >= 0 && J[1] < m[1] && J[2] >= 0 && J[2] < m[2];
+}
+static double ice_t(const o3d* s, int b, const int J[3]) {
+  long n = mat(s, b, J[0], J[1], J[2]);
+  double m = s->material_mass[n], t = s->tm;
+  if (m > 1e-30) t += s->material_energy[n] / (s->cp_ice * m);
+  return t;
+}
+static double nu_lattice(const o3d* s, long n) {
+  return o3d_mixture_nu(s, s->phi[n], water_t(s, n)) * s->nu_scale;
+}
+
+/* ---------------------------------------------------------------- HOME */
+/* home.py:30-53 */
+double o3d_reconstruct(int q, const double m[10]) {
+  double c[3];
+  int ci[3];
+  dir27(q, ci);
+  for (int a = 0; a < 3; ++a) c[a] = ci[a];
+  double p = m[0], u[3] = {m[1], m[2], m[3]};
+  double S[3][3] = {{m[4], m[7], m[8]}, {m[7], m[5], m[9]}, {m[8], m[9], m[6]}};
+  double cSc = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) cSc += c[a] * S[a][b] * c[b];
+  double value = p + 3 * (c[0] * u[0] + c[1] * u[1] + c[2] * u[2]) + 4.5 * (cSc - (S[0][0] + S[1][1] + S[2][2]) / 3);
+  double third = 0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      if (a != b) third += (c[a] * c[a] - 1.0 / 3) * c[b] * (S[a][a] * u[b] + 2 * S[a][b] * u[a] - 2 * u[a] * u[a] * u[b]);
+  third += 2 * c[0] * c[1] * c[2] * (S[0][2] * u[1] + S[1][2] * u[0] + S[0][1] * u[2] - 2 * u[0] * u[1] * u[2]);
+  return w27(q) * (value + 13.5 * third);
+}
+static void pack(double out[10], double p, const double u[3], double S[3][3]) {
+  out[0] = p;
+  out[1] = u[0];
+  out[2] = u[1];
+  out[3] = u[2];
+  out[4] = S[0][0];
+  out[5] = S[1][1];
+  out[6] = S[2][2];
+  out[7] = S[0][1];
+  out[8] = S[0][2];
+  out[9] = S[1][2];
+}
+/* home.py:56-58 */
+static double equilibrium(int q, double p, const double u[3]) {
+  double S[3][3], m[10];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) S[a][b] = u[a] * u[b];
+  pack(m, p, u, S);
+  return o3d_reconstruct(q, m);
+}
+/* home.py:61-78 */
+void o3d_central_collision(const double m[10], const double acc[3], double rate, double bulk, double out[10]) {
+  double j[3] = {m[1], m[2], m[3]};
+  double S[3][3] = {{m[4], m[7], m[8]}, {m[7], m[5], m[9]}, {m[8], m[9], m[6]}};
+  double u[3], R[3][3], F[3][3], post[3][3];
+  for (int a = 0; a < 3; ++a) u[a] = j[a] + 0.5 * acc[a];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      R[a][b] = S[a][b] - u[a] * u[b];
+      F[a][b] = acc[a] * u[b] + u[a] * acc[b];
+    }
+  double trR = R[0][0] + R[1][1] + R[2][2], trF = F[0][0] + F[1][1] + F[2][2];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      double e = a == b ? 1.0 : 0.0;
+      post[a][b] = u[a] * u[b] + (1 - rate) * (R[a][b] - trR / 3 * e) + (1 - bulk) * trR / 3 * e +
+                   (1 - 0.5 * rate) * (F[a][b] - trF / 3 * e) + (1 - 0.5 * bulk) * trF / 3 * e;
+    }
+  double jn[3] = {j[0] + acc[0], j[1] + acc[1], j[2] + acc[2]};
+  pack(out, m[0], jn, post);
+}
+
+/* ----------------------------------------------------------------- D3Q7 */
+/* phase.py:38-60 */
+void o3d_phase_collision(const double h[7], double phi, const double u[3], const double n[3], double mobility,
+                         double width, double post[7], double flux[3]) {
+  double first[3] = {h[1] - h[2], h[3] - h[4], h[5] - h[6]};
+  double rate = 1 / (0.5 + 4 * mobility);
+  double j[3], second[3];
+  for (int a = 0; a < 3; ++a) {
+    double source = phi * (1 - phi) / width * n[a];
+    double delta = (1 - rate) * (first[a] - phi * u[a]) + (1 - 0.5 * rate) * source;
+    j[a] = phi * u[a] + delta;
+    second[a] = phi * 0.25 + 2 * u[a] * (delta - 0.5 * source);
+    post[2 * a + 1] = 0.5 * (second[a] + j[a]);
+    post[2 * a + 2] = 0.5 * (second[a] - j[a]);
+    flux[a] = 0.5 * (first[a] + j[a]) - phi * u[a];
+  }
+  post[0] = phi - (second[0] + second[1] + second[2]);
+}
+static double eq7(int q, double phi, const double u[3]) {
+  double w = q == 0 ? 0.25 : 0.125;
+  return w * phi * (1 + 4 * (C7[q][0] * u[0] + C7[q][1] * u[1] + C7[q][2] * u[2]));
+}
+/* phase.py:69-82: AA address of logical population q at cell I */
+static long phase_address(const o3d* s, const int I[3], int q) {
+  int J[3] = {I[0], I[1], I[2]}, slot = q;
+  if (s->parity != 0) {
+    int nb[3] = {I[0] - C7[q][0], I[1] - C7[q][1], I[2] - C7[q][2]};
+    if (fluid(s, nb)) {
+      J[0] = nb[0];
+      J[1] = nb[1];
+      J[2] = nb[2];
+      slot = OPP7[q];
+    }
+  }
+  return lin(s, J) * 7 + slot;
+}
+
+/* -------------------------------------------------------- geometry */
+/* simulator.py:217-257 */
+static void body_rot(const o3d* s, int b, double R[3][3]) {
+  for (int a = 0; a < 3; ++a)
+    for (int c = 0; c < 3; ++c) R[a][c] = s->body_rot[9 * b + 3 * a + c];
+}
+static void material_grid(const o3d* s, const double point[3], int b, double grid[3]) {
+  double R[3][3], d[3];
+  body_rot(s, b, R);
+  for (int a = 0; a < 3; ++a) d[a] = point[a] - s->body_origin[3 * b + a];
+  for (int a = 0; a < 3; ++a)
+    grid[a] = R[0][a] * d[0] + R[1][a] * d[1] + R[2][a] * d[2] + 0.5 * s->material_size[3 * b + a] - 0.5;
+}
+static double fraction_at(const o3d* s, const double point[3], int b) {
+  double grid[3], part[3];
+  int base[3];
+  material_grid(s, point, b, grid);
+  for (int a = 0; a < 3; ++a) {
+    base[a] = (int)floor(grid[a]);
+    part[a] = grid[a] - base[a];
+  }
+  double value = 0;
+  for (int a = 0; a < 2; ++a)
+    for (int c = 0; c < 2; ++c)
+      for (int d = 0; d < 2; ++d) {
+        int J[3] = {base[0] + a, base[1] + c, base[2] + d};
+        if (!mat_inside(s, b, J)) continue;
+        double w = (a ? part[0] : 1 - part[0]) * (c ? part[1] : 1 - part[1]) * (d ? part[2] : 1 - part[2]);
+        value += w * (s->material_mass[mat(s, b, J[0], J[1], J[2])] / s->initial_cell_mass);
+      }
+  return value;
+}
+static void body_velocity_at(const o3d* s, const double point[3], int b, double out[3]) {
+  const double *v = s->body_v + 3 * b, *w = s->body_w + 3 * b, *c = s->body_center + 3 * b;
+  double r[3] = {point[0] - c[0], point[1] - c[1], point[2] - c[2]};
+  out[0] = v[0] + w[1] * r[2] - w[2] * r[1];
+  out[1] = v[1] + w[2] * r[0] - w[0] * r[2];
+  out[2] = v[2] + w[0] * r[1] - w[1] * r[0];
+}
+
+/* simulator.py:288-323 */
+void o3d_rasterize(o3d* s) {
+  int b = s->boundary;
+  for (int a = 0; a < 3; ++a) {
+    s->geo_lower[a] = a == 0 ? s->nx : (a == 1 ? s->ny : s->nz);
+    s->geo_upper[a] = -1;
+  }
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        long n = idx(s, i, j, k);
+        s->prev_id[n] = s->id[n];
+        int wall = i < b || i >= s->nx - b || j < b || j >= s->ny - b || k < b || k >= s->nz - b;
+        double fraction = 0;
+        int owner = -1;
+        if (!wall)
+          for (int body = 0; body < s->bodies; ++body) {
+            double point[3] = {i + 0.5, j + 0.5, k + 0.5};
+            double f = fraction_at(s, point, body);
+            if (f > fraction) {
+              fraction = f;
+              owner = body;
+            }
+          }
+        int new_id = wall ? -2 : (fraction < s->threshold ? -1 : owner);
+        if (fraction != s->coverage[n] || new_id != s->id[n]) {
+          int I[3] = {i, j, k};
+          for (int a = 0; a < 3; ++a) {
+            if (I[a] < s->geo_lower[a]) s->geo_lower[a] = I[a];
+            if (I[a] > s->geo_upper[a]) s->geo_upper[a] = I[a];
+          }
+        }
+        s->coverage[n] = fraction;
+        s->cov_id[n] = owner;
+        s->id[n] = new_id;
+      }
+}
+
+/* gradients.py:22-33 */
+static void tank_mirror(const o3d* s, const int J[3], int K[3]) {
+  int shape[3] = {s->nx, s->ny, s->nz}, b = s->boundary;
+  for (int a = 0; a < 3; ++a) {
+    K[a] = J[a];
+    if (K[a] < b)
+      K[a] = 2 * b - 1 - K[a];
+    else if (K[a] >= shape[a] - b)
+      K[a] = 2 * (shape[a] - b) - 1 - K[a];
+  }
+}
+/* gradients.py:35-46 */
+static void coverage_normal(const o3d* s, const int I[3], double g[3]) {
+  for (int a = 0; a < 3; ++a) {
+    int lo[3] = {I[0], I[1], I[2]}, hi[3] = {I[0], I[1], I[2]}, ml[3], mh[3];
+    lo[a] -= 1;
+    hi[a] += 1;
+    tank_mirror(s, lo, ml);
+    tank_mirror(s, hi, mh);
+    g[a] = 0;
+    if (inside(s, ml) && inside(s, mh)) g[a] = 0.5 * (s->coverage[lin(s, mh)] - s->coverage[lin(s, ml)]);
+  }
+}
+/* gradients.py:48-58 */
+static double link_fraction(const o3d* s, const int I[3], const int J[3]) {
+  double ci = s->coverage[lin(s, I)], cj = s->coverage[lin(s, J)];
+  return fmin(0.95, fmax(0.05, (s->threshold - ci) / fmax(cj - ci, 1e-6)));
+}
+/* gradients.py:60-74 */
+static void ice_mirror_point(const o3d* s, const int I[3], const int J[3], double out[3]) {
+  double c[3] = {J[0] - I[0], J[1] - I[1], J[2] - I[2]};
+  double eta = link_fraction(s, I, J), gi[3], gj[3], n[3];
+  coverage_normal(s, I, gi);
+  coverage_normal(s, J, gj);
+  for (int a = 0; a < 3; ++a) n[a] = (1 - eta) * gi[a] + eta * gj[a];
+  double nc = n[0] * c[0] + n[1] * c[1] + n[2] * c[2];
+  double nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2], cc = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+  if (nc <= 1e-6 * sqrt(nn * cc) || nn < 1e-12)
+    for (int a = 0; a < 3; ++a) n[a] = c[a];
+  nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  double inv = 1 / sqrt(nn + 1e-30);
+  for (int a = 0; a < 3; ++a) n[a] *= inv;
+  nc = n[0] * c[0] + n[1] * c[1] + n[2] * c[2];
+  for (int a = 0; a < 3; ++a) out[a] = J[a] - 2 * (1 - eta) * nc * n[a];
+}
+/* gradients.py:76-101 */
+static double fluid_interpolate(const o3d* s, const double point[3], double fallback) {
+  int base[3];
+  double part[3];
+  for (int a = 0; a < 3; ++a) {
+    base[a] = (int)floor(point[a]);
+    part[a] = point[a] - base[a];
+  }
+  double value = 0, total = 0;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b)
+      for (int c = 0; c < 2; ++c) {
+        int raw[3] = {base[0] + a, base[1] + b, base[2] + c}, J[3];
+        tank_mirror(s, raw, J);
+        if (!fluid(s, J)) continue;
+        double w = (a ? part[0] : 1 - part[0]) * (b ? part[1] : 1 - part[1]) * (c ? part[2] : 1 - part[2]);
+        value += w * s->phi[lin(s, J)];
+        total += w;
+      }
+  return total > 1e-7 ? value / total : fallback;
+}
+/* gradients.py:103-113 */
+static double wall_phase_sample(const o3d* s, const int I[3], const int c[3]) {
+  int raw[3] = {I[0] + c[0], I[1] + c[1], I[2] + c[2]}, J[3];
+  tank_mirror(s, raw, J);
+  double value = s->phi[lin(s, I)];
+  if (fluid(s, J))
+    value = s->phi[lin(s, J)];
+  else if (inside(s, J)) {
+    double point[3];
+    ice_mirror_point(s, I, J, point);
+    value = fluid_interpolate(s, point, value);
+  }
+  return value;
+}
+/* gradients.py:147-158 */
+static void store_gradient(o3d* s, long n, const double g[3]) {
+  double gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2], k = 1.5 * s->sigma * s->width;
+  for (int a = 0; a < 3; ++a) {
+    s->grad[3 * n + a] = g[a];
+    for (int b = 0; b < 3; ++b) s->stress[9 * n + 3 * a + b] = k * ((a == b ? gg : 0.0) - g[a] * g[b]);
+  }
+}
+/* gradients.py:116-145 */
+void o3d_write_gradient(o3d* s, int i, int j, int k) {
+  int I[3] = {i, j, k};
+  long n = lin(s, I);
+  double g[3] = {0, 0, 0};
+  if (fluid(s, I)) {
+    double rhs[3] = {0, 0, 0};
+    int wall = 0;
+    for (int q = 0; q < Q27; ++q) {
+      if (q == REST) continue;
+      int c[3], J[3];
+      dir27(q, c);
+      for (int a = 0; a < 3; ++a) J[a] = I[a] + c[a];
+      if (fluid(s, J)) {
+        double d = s->phi[lin(s, J)] - s->phi[n];
+        for (int a = 0; a < 3; ++a) rhs[a] += w27(q) * c[a] * d;
+      } else
+        wall = 1;
+    }
+    if (wall)
+      for (int q = 0; q < Q27; ++q) {
+        if (q == REST) continue;
+        int c[3], J[3];
+        dir27(q, c);
+        for (int a = 0; a < 3; ++a) J[a] = I[a] + c[a];
+        if (fluid(s, J)) continue;
+        double d = wall_phase_sample(s, I, c) - s->phi[n];
+        for (int a = 0; a < 3; ++a) rhs[a] += w27(q) * c[a] * d;
+      }
+    for (int a = 0; a < 3; ++a) g[a] = 3 * rhs[a];
+  }
+  store_gradient(s, n, g);
+}
+/* gradients.py:167-181 */
+static void write_regular_gradient(o3d* s, const int I[3], int flag) {
+  double rhs[3] = {0, 0, 0};
+  long n = lin(s, I);
+  if (flag >= 0)
+    for (int q = 0; q < Q27; ++q) {
+      if (q == REST) continue;
+      int c[3], J[3], K[3];
+      dir27(q, c);
+      for (int a = 0; a < 3; ++a) J[a] = I[a] + c[a];
+      if (flag == 1) {
+        tank_mirror(s, J, K);
+      } else {
+        K[0] = J[0];
+        K[1] = J[1];
+        K[2] = J[2];
+      }
+      double d = s->phi[lin(s, K)] - s->phi[n];
+      for (int a = 0; a < 3; ++a) rhs[a] += w27(q) * c[a] * d;
+    }
+  double g[3] = {3 * rhs[0], 3 * rhs[1], 3 * rhs[2]};
+  store_gradient(s, n, g);
+}
+/* home.py:262-278 */
+static int boundary_flag(const o3d* s, const int I[3]) {
+  int flag = -1;
+  if (fluid(s, I)) {
+    flag = 0;
+    for (int q = 0; q < Q27; ++q) {
+      if (q == REST) continue;
+      int c[3], J[3];
+      dir27(q, c);
+      for (int a = 0; a < 3; ++a) J[a] = I[a] - c[a];
+      if (!fluid(s, J)) {
+        if (flag < 1) flag = 1;
+        if (inside(s, J)) {
+          int id = s->id[lin(s, J)];
+          if (id >= 0 || id == -3) flag = 2;
+        }
+      }
+    }
+  }
+  return flag;
+}
+static void box_bounds(const o3d* s, const int lo_in[3], const int hi_in[3], int lo[3], int hi[3]) {
+  (void)s;
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = lo_in[a];
+    hi[a] = hi_in[a];
+  }
+}
+/* home.py:280-284 */
+void o3d_classify_box(o3d* s, const int lo[3], const int hi[3]) {
+  for (int i = lo[0]; i < hi[0]; ++i)
+    for (int j = lo[1]; j < hi[1]; ++j)
+      for (int k = lo[2]; k < hi[2]; ++k) {
+        int I[3] = {i, j, k};
+        s->flag[lin(s, I)] = (int8_t)boundary_flag(s, I);
+      }
+}
+void o3d_classify(o3d* s) {
+  int lo[3] = {0, 0, 0}, hi[3] = {s->nx, s->ny, s->nz};
+  o3d_classify_box(s, lo, hi);
+}
+/* gradients.py:358-386 (and the refill box variants :402-431) */
+void o3d_gradients_box(o3d* s, const int lo_in[3], const int hi_in[3]) {
+  int lo[3], hi[3];
+  box_bounds(s, lo_in, hi_in, lo, hi);
+  for (int i = lo[0]; i < hi[0]; ++i)
+    for (int j = lo[1]; j < hi[1]; ++j)
+      for (int k = lo[2]; k < hi[2]; ++k) {
+        int I[3] = {i, j, k};
+        int flag = s->flag[lin(s, I)];
+        if (flag < 2) write_regular_gradient(s, I, flag);
+      }
+  for (int i = lo[0]; i < hi[0]; ++i)
+    for (int j = lo[1]; j < hi[1]; ++j)
+      for (int k = lo[2]; k < hi[2]; ++k) {
+        int I[3] = {i, j, k};
+        if (s->flag[lin(s, I)] == 2) o3d_write_gradient(s, i, j, k);
+      }
+}
+void o3d_gradients(o3d* s, int classified) {
+  if (!classified) o3d_classify(s);
+  int lo[3] = {0, 0, 0}, hi[3] = {s->nx, s->ny, s->nz};
+  o3d_gradients_box(s, lo, hi);
+}
+/* gradients.py:388-400 */
+void o3d_refill_gradients(o3d* s) {
+  int lo[3], hi[3], shape[3] = {s->nx, s->ny, s->nz}, changed = 1;
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = s->refill_lower[a] < s->geo_lower[a] ? s->refill_lower[a] : s->geo_lower[a];
+    hi[a] = s->refill_upper[a] > s->geo_upper[a] ? s->refill_upper[a] : s->geo_upper[a];
+    if (!(hi[a] >= lo[a])) changed = 0;
+  }
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = lo[a] - 6 > 0 ? lo[a] - 6 : 0;
+    hi[a] = hi[a] + 7 < shape[a] ? hi[a] + 7 : shape[a];
+    if (!changed) hi[a] = lo[a];
+  }
+  o3d_classify_box(s, lo, hi);
+  o3d_gradients_box(s, lo, hi);
+}
+
+/* ---------------------------------------------------------------- phase */
+/* phase.py:93-127 */
+void o3d_collide_phase(o3d* s, int moving_walls) {
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        int I[3] = {i, j, k};
+        if (!fluid(s, I)) continue;
+        long n = lin(s, I);
+        double phi = s->phi[n], *u = s->u + 3 * n, *g = s->grad + 3 * n;
+        double norm = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2] + s->normal_eps * s->normal_eps + 1e-28);
+        double nrm[3] = {g[0] / norm, g[1] / norm, g[2] / norm}, in[7], post[7], fl[3];
+        for (int q = 0; q < 7; ++q) in[q] = s->h[phase_address(s, I, q)];
+        o3d_phase_collision(in, phi, u, nrm, s->mobility, s->width, post, fl);
+        for (int a = 0; a < 3; ++a) s->flux[3 * n + a] = (1 - s->rho_air) * fl[a];
+        long target[7];
+        for (int q = 0; q < 7; ++q) target[q] = phase_address(s, I, OPP7[q]);
+        for (int q = 0; q < 7; ++q) {
+          double value = post[q];
+          int nb[3] = {i + C7[q][0], j + C7[q][1], k + C7[q][2]};
+          if (q != 0 && moving_walls && inside(s, nb) && s->id[lin(s, nb)] >= 0) {
+            double eta = link_fraction(s, I, nb), point[3], ub[3];
+            for (int a = 0; a < 3; ++a) point[a] = I[a] + 0.5 + eta * C7[q][a];
+            body_velocity_at(s, point, s->id[lin(s, nb)], ub);
+            value -= phi * (C7[q][0] * ub[0] + C7[q][1] * ub[1] + C7[q][2] * ub[2]);
+          }
+          s->h[target[q]] = value;
+        }
+      }
+  s->parity = 1 - s->parity;
+}
+/* simulator.py:778-794 */
+static void scan_stability_at(o3d* s, long n, double phi, double p) {
+  double *u = s->u + 3 * n, speed = fabs(u[0]) + fabs(u[1]) + fabs(u[2]);
+  if (speed > s->maximum_speed) s->maximum_speed = speed;
+  int invalid = isnan(speed) || isinf(speed) || speed > s->max_velocity || isnan(p) || isinf(p) || isnan(phi) ||
+                isinf(phi);
+  if (invalid && n < s->invalid_index) s->invalid_index = (int32_t)n;
+}
+/* phase.py:132-161 */
+void o3d_recover_phase(o3d* s, int recover_flow) {
+  if (recover_flow && s->check_stability) {
+    s->invalid_index = (int32_t)cells(s);
+    s->maximum_speed = 0;
+  }
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        int I[3] = {i, j, k};
+        long n = lin(s, I);
+        if (!fluid(s, I)) {
+          s->phi[n] = 0;
+          continue;
+        }
+        double total = s->h[7 * n];
+        for (int a = 0; a < 3; ++a) total += s->h[phase_address(s, I, 2 * a + 1)] + s->h[phase_address(s, I, 2 * a + 2)];
+        double bounded = total;
+        if (!isnan(total) && !isinf(total)) bounded = clamp01(total);
+        s->h[7 * n] += bounded - total;
+        s->phi[n] = bounded;
+        s->phase_clip_volume += bounded - total;
+        if (recover_flow) {
+          double p = s->home[10 * n] * density(s, bounded) / 3;
+          s->pressure[n] = p;
+          if (s->check_stability) scan_stability_at(s, n, bounded, p);
+        }
+      }
+  if (recover_flow && s->check_stability && s->maximum_speed > s->peak_speed) s->peak_speed = s->maximum_speed;
+}
+/* simulator.py:796-805 */
+void o3d_scan_stability(o3d* s) {
+  s->invalid_index = (int32_t)cells(s);
+  s->maximum_speed = 0;
+  for (long n = 0; n < cells(s); ++n)
+    if (s->id[n] == -1) scan_stability_at(s, n, s->phi[n], s->pressure[n]);
+  if (s->maximum_speed > s->peak_speed) s->peak_speed = s->maximum_speed;
+}
+/* phase.py:163-184 */
+void o3d_repair_phase_layout(o3d* s) {
+  if (s->parity == 0) return;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        int I[3] = {i, j, k};
+        long n = lin(s, I);
+        for (int a = 0; a < 3; ++a) {
+          int q = 2 * a + 1;
+          int J[3] = {i + C7[q][0], j + C7[q][1], k + C7[q][2]};
+          if (!inside(s, J)) continue;
+          long m = lin(s, J);
+          int before = s->prev_id[n] == -1 && s->prev_id[m] == -1;
+          int after = s->id[n] == -1 && s->id[m] == -1;
+          if (before != after) {
+            double v = s->h[7 * n + OPP7[q]];
+            s->h[7 * n + OPP7[q]] = s->h[7 * m + q];
+            s->h[7 * m + q] = v;
+          }
+        }
+      }
+}
+/* phase.py:186-191 */
+void o3d_export_phase(const o3d* s, double* out) {
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        int I[3] = {i, j, k};
+        long n = lin(s, I);
+        for (int q = 0; q < 7; ++q) out[7 * n + q] = s->h[phase_address(s, I, q)];
+      }
+}
+
+/* --------------------------------------------------------------- forces */
+/* simulator.py:193-215 */
+static int connected(const o3d* s, const int I[3], const int c[3]) {
+  int J[3] = {I[0] + c[0], I[1] + c[1], I[2] + c[2]};
+  if (!fluid(s, J)) return 0;
+  int distance = abs(c[0]) + abs(c[1]) + abs(c[2]);
+  if (distance <= 1) return 1;
+  int valid = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (c[a] == 0) continue;
+    int A[3] = {I[0], I[1], I[2]};
+    A[a] += c[a];
+    if (!fluid(s, A)) continue;
+    if (distance == 2)
+      valid = 1;
+    else
+      for (int b = 0; b < 3; ++b) {
+        if (b == a) continue;
+        int B[3] = {A[0], A[1], A[2]};
+        B[b] += c[b];
+        if (fluid(s, B)) valid = 1;
+      }
+  }
+  return valid;
+}
+/* simulator.py:397-416 */
+static void capillary_neighbor(const o3d* s, const int I[3], const int c[3], double out[9]) {
+  int J[3] = {I[0] + c[0], I[1] + c[1], I[2] + c[2]}, shape[3] = {s->nx, s->ny, s->nz}, b = s->boundary;
+  const double* src = s->stress + 9 * lin(s, I);
+  for (int t = 0; t < 9; ++t) out[t] = src[t];
+  if (fluid(s, J)) {
+    src = s->stress + 9 * lin(s, J);
+    for (int t = 0; t < 9; ++t) out[t] = src[t];
+  } else if (inside(s, J) && s->id[lin(s, J)] == -2) {
+    double R[3] = {1, 1, 1};
+    for (int a = 0; a < 3; ++a) {
+      if (J[a] < b) {
+        J[a] = 2 * b - 1 - J[a];
+        R[a] = -1;
+      } else if (J[a] >= shape[a] - b) {
+        J[a] = 2 * (shape[a] - b) - 1 - J[a];
+        R[a] = -1;
+      }
+    }
+    if (fluid(s, J)) {
+      src = s->stress + 9 * lin(s, J);
+      for (int a = 0; a < 3; ++a)
+        for (int d = 0; d < 3; ++d) out[3 * a + d] = R[a] * src[3 * a + d] * R[d];
+    }
+  }
+}
+enum { MODE_GENERAL = 0, MODE_INTERIOR = 1, MODE_TANK = 2 };
+/* home.py:116-125 */
+static int link_fluid(const o3d* s, const int J[3], int mode) {
+  if (mode == MODE_INTERIOR) return 1;
+  if (mode == MODE_TANK) {
+    int shape[3] = {s->nx, s->ny, s->nz}, b = s->boundary;
+    for (int a = 0; a < 3; ++a)
+      if (J[a] < b || J[a] >= shape[a] - b) return 0;
+    return 1;
+  }
+  return fluid(s, J);
+}
+/* simulator.py:418-519 */
+void o3d_force_at(const o3d* s, const int I[3], int with_flux, int mode, double force[3]) {
+  force[0] = force[1] = force[2] = 0;
+  if (!fluid(s, I)) return;
+  long n = lin(s, I);
+  double phi = s->phi[n], rho = density(s, phi);
+  double gvec[3] = {0, s->gravity_y, 0};
+  double tw = water_t(s, n);
+  for (int a = 0; a < 3; ++a) force[a] = (rho - s->ref_density[I[1]]) * gvec[a];
+  for (int a = 0; a < 3; ++a) force[a] -= fmax(0.0, 2 * phi - 1) * s->expansion * (tw - s->water_t0) * gvec[a];
+  if (s->sigma > 0) {
+    int wall = 0, shape[3] = {s->nx, s->ny, s->nz}, b = s->boundary;
+    for (int q = 0; q < Q27; ++q) {
+      if (q == REST) continue;
+      int c[3], J[3];
+      dir27(q, c);
+      for (int a = 0; a < 3; ++a) J[a] = I[a] + c[a];
+      double g[3] = {0, 0, 0};
+      int valid = 0;
+      if (mode == MODE_TANK) {
+        int K[3];
+        for (int a = 0; a < 3; ++a) {
+          K[a] = J[a];
+          if (K[a] > shape[a] - b - 1) K[a] = shape[a] - b - 1;
+          if (K[a] < b) K[a] = b;
+        }
+        for (int a = 0; a < 3; ++a) g[a] = s->grad[3 * lin(s, K) + a] * (J[a] != K[a] ? -1.0 : 1.0);
+        valid = 1;
+      } else if (link_fluid(s, J, mode)) {
+        for (int a = 0; a < 3; ++a) g[a] = s->grad[3 * lin(s, J) + a];
+        valid = 1;
+      } else
+        wall = 1;
+      if (valid) {
+        double gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2], gc = g[0] * c[0] + g[1] * c[1] + g[2] * c[2];
+        double coef = 4.5 * s->sigma * s->width * w27(q);
+        for (int a = 0; a < 3; ++a) force[a] += coef * (gg * c[a] - gc * g[a]);
+      }
+    }
+    if (mode == MODE_GENERAL && wall)
+      for (int q = 0; q < Q27; ++q) {
+        if (q == REST) continue;
+        int c[3], J[3];
+        dir27(q, c);
+        for (int a = 0; a < 3; ++a) J[a] = I[a] + c[a];
+        if (fluid(s, J)) continue;
+        double st[9];
+        capillary_neighbor(s, I, c, st);
+        for (int a = 0; a < 3; ++a) force[a] += 3 * w27(q) * (st[3 * a] * c[0] + st[3 * a + 1] * c[1] + st[3 * a + 2] * c[2]);
+      }
+  }
+  if (with_flux)
+    for (int a = 0; a < 3; ++a)
+      for (int sign = 0; sign < 2; ++sign) {
+        int J[3] = {I[0], I[1], I[2]}, dirn = 2 * sign - 1;
+        J[a] += dirn;
+        if (!link_fluid(s, J, mode)) continue;
+        long m = lin(s, J);
+        double mass_flux = 0.5 * (s->flux[3 * n + a] + s->flux[3 * m + a]);
+        const double* donor = s->u + 3 * m;
+        if (dirn * mass_flux >= 0) donor = s->u + 3 * n;
+        double fl[3];
+        for (int d = 0; d < 3; ++d) fl[d] = mass_flux * (donor[d] - s->u[3 * n + d]);
+        if (s->flux_limit > 0) {
+          double budget = s->flux_limit / 6 * fmin(rho, density(s, s->phi[m]));
+          double mag = fabs(fl[0]) + fabs(fl[1]) + fabs(fl[2]);
+          double scale = fmin(1.0, budget / (mag + 1e-20));
+          for (int d = 0; d < 3; ++d) fl[d] *= scale;
+        }
+        for (int d = 0; d < 3; ++d) force[d] -= dirn * fl[d];
+      }
+}
+/* simulator.py:559-587 */
+static void stress_neighbor_velocity(const o3d* s, const int I[3], const int c[3], double out[3]) {
+  long n = lin(s, I);
+  const double* center = s->u + 3 * n;
+  int J[3] = {I[0] + c[0], I[1] + c[1], I[2] + c[2]};
+  for (int a = 0; a < 3; ++a) out[a] = -center[a];
+  if (fluid(s, J)) {
+    for (int a = 0; a < 3; ++a) out[a] = s->u[3 * lin(s, J) + a];
+  } else if (inside(s, J) && s->id[lin(s, J)] >= 0) {
+    double eta = link_fraction(s, I, J), point[3], ub[3];
+    for (int a = 0; a < 3; ++a) point[a] = I[a] + 0.5 + eta * c[a];
+    body_velocity_at(s, point, s->id[lin(s, J)], ub);
+    int mc[3] = {-c[0], -c[1], -c[2]};
+    if (connected(s, I, mc)) {
+      int K[3] = {I[0] - c[0], I[1] - c[1], I[2] - c[2]};
+      for (int a = 0; a < 3; ++a) out[a] = (2 * ub[a] + (eta - 1) * s->u[3 * lin(s, K) + a]) / (1 + eta);
+    } else
+      for (int a = 0; a < 3; ++a) out[a] = 2 * ub[a] - center[a];
+  }
+}
+/* home.py:127-197 */
+void o3d_acceleration_at(const o3d* s, const int I[3], const double force[3], int mode, int moment_pressure,
+                         double acc[3]) {
+  long n = lin(s, I);
+  double rho = density(s, s->phi[n]);
+  for (int a = 0; a < 3; ++a) acc[a] = force[a] / rho;
+  for (int q = 0; q < Q27; ++q) {
+    if (q == REST) continue;
+    int c[3], J[3];
+    dir27(q, c);
+    for (int a = 0; a < 3; ++a) J[a] = I[a] + c[a];
+    if (!link_fluid(s, J, mode)) continue;
+    long m = lin(s, J);
+    double rj = density(s, s->phi[m]);
+    for (int a = 0; a < 3; ++a) {
+      if (moment_pressure)
+        acc[a] += w27(q) * c[a] * s->home[10 * m] * ((rho - rj) / rho);
+      else
+        acc[a] += 3 * w27(q) * c[a] * s->pressure[m] * (1 / rj - 1 / rho);
+    }
+  }
+  double gr[3];
+  for (int a = 0; a < 3; ++a) gr[a] = (1 - s->rho_air) * s->grad[3 * n + a];
+  if (gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2] > 0) {
+    double G[3][3];
+    for (int axis = 0; axis < 3; ++axis) {
+      int c[3] = {0, 0, 0};
+      c[axis] = 1;
+      double d[3];
+      if (mode == MODE_INTERIOR) {
+        int P[3] = {I[0] + c[0], I[1] + c[1], I[2] + c[2]}, M[3] = {I[0] - c[0], I[1] - c[1], I[2] - c[2]};
+        for (int a = 0; a < 3; ++a) d[a] = 0.5 * (s->u[3 * lin(s, P) + a] - s->u[3 * lin(s, M) + a]);
+      } else if (mode == MODE_TANK) {
+        double left[3], right[3];
+        int P[3] = {I[0] + c[0], I[1] + c[1], I[2] + c[2]}, M[3] = {I[0] - c[0], I[1] - c[1], I[2] - c[2]};
+        for (int a = 0; a < 3; ++a) left[a] = right[a] = -s->u[3 * n + a];
+        if (link_fluid(s, M, MODE_TANK))
+          for (int a = 0; a < 3; ++a) left[a] = s->u[3 * lin(s, M) + a];
+        if (link_fluid(s, P, MODE_TANK))
+          for (int a = 0; a < 3; ++a) right[a] = s->u[3 * lin(s, P) + a];
+        for (int a = 0; a < 3; ++a) d[a] = 0.5 * (right[a] - left[a]);
+      } else {
+        double plus[3], minus[3];
+        int mc[3] = {-c[0], -c[1], -c[2]};
+        stress_neighbor_velocity(s, I, c, plus);
+        stress_neighbor_velocity(s, I, mc, minus);
+        for (int a = 0; a < 3; ++a) d[a] = 0.5 * (plus[a] - minus[a]);
+      }
+      for (int comp = 0; comp < 3; ++comp) G[comp][axis] = d[comp];
+    }
+    double coef = nu_lattice(s, n) / rho;
+    for (int a = 0; a < 3; ++a) {
+      double v = 0;
+      for (int b = 0; b < 3; ++b) v += (G[a][b] + G[b][a]) * gr[b];
+      acc[a] += coef * v;
+    }
+  }
+}
+/* simulator.py:521-557 */
+void o3d_forces(o3d* s, int with_flux, int moment_pressure) {
+  o3d_classify(s);
+  for (int pass = 0; pass < 3; ++pass)
+    for (int i = 0; i < s->nx; ++i)
+      for (int j = 0; j < s->ny; ++j)
+        for (int k = 0; k < s->nz; ++k) {
+          int I[3] = {i, j, k};
+          long n = lin(s, I);
+          int flag = s->flag[n];
+          int want = pass == 0 ? 0 : (pass == 1 ? 1 : 2);
+          if (flag != want) continue;
+          int mode = flag == 0 ? MODE_INTERIOR : (flag == 1 ? MODE_TANK : MODE_GENERAL);
+          double f[3], acc[3];
+          o3d_force_at(s, I, with_flux, mode, f);
+          for (int a = 0; a < 3; ++a) s->force[3 * n + a] = f[a];
+          o3d_acceleration_at(s, I, f, mode, moment_pressure, acc);
+          for (int a = 0; a < 3; ++a) s->accel[3 * n + a] = acc[a];
+        }
+}
+/* home.py:199-211 */
+void o3d_collide_momentum(o3d* s) {
+  for (long n = 0; n < cells(s); ++n) {
+    if (s->id[n] != -1) continue;
+    for (int a = 0; a < 3; ++a) s->old_u[3 * n + a] = s->u[3 * n + a];
+    s->old_pressure[n] = s->pressure[n];
+    double rate = 1 / (0.5 + 3 * nu_lattice(s, n));
+    o3d_central_collision(s->home + 10 * n, s->accel + 3 * n, rate, s->bulk_rate, s->home_post + 10 * n);
+  }
+}
+/* home.py:213-254 */
+static double home_reflection(o3d* s, const int I[3], int outgoing) {
+  int c[3], J[3];
+  dir27(outgoing, c);
+  for (int a = 0; a < 3; ++a) J[a] = I[a] + c[a];
+  long n = lin(s, I);
+  double raw = o3d_reconstruct(outgoing, s->home_post + 10 * n), reflected = raw;
+  if (inside(s, J) && s->id[lin(s, J)] >= 0) {
+    int b = s->id[lin(s, J)];
+    double eta = link_fraction(s, I, J), point[3], ub[3];
+    for (int a = 0; a < 3; ++a) point[a] = I[a] + 0.5 + eta * c[a];
+    body_velocity_at(s, point, b, ub);
+    double p = s->home[10 * n];
+    double neq = o3d_reconstruct(outgoing, s->home + 10 * n) - equilibrium(outgoing, p, s->old_u + 3 * n);
+    reflected = (equilibrium(26 - outgoing, p, ub) + neq + eta * o3d_reconstruct(26 - outgoing, s->home_post + 10 * n)) /
+                (1 + eta);
+    double rho = density(s, s->phi[n]), impulse[3], cap[3], st[9];
+    double pref = (1 - eta) * s->ref_pressure[I[1]] + eta * s->ref_pressure[J[1]];
+    for (int a = 0; a < 3; ++a) impulse[a] = rho * ((c[a] - ub[a]) * raw - (-c[a] - ub[a]) * reflected);
+    for (int a = 0; a < 3; ++a) impulse[a] += 6 * w27(outgoing) * pref * c[a];
+    capillary_neighbor(s, I, c, st);
+    const double* own = s->stress + 9 * n;
+    for (int a = 0; a < 3; ++a) {
+      double v = 0;
+      for (int d = 0; d < 3; ++d) v += (own[3 * a + d] + st[3 * a + d]) * c[d];
+      cap[a] = -3 * w27(outgoing) * v;
+    }
+    double r1[3], r2[3];
+    const double* center = s->body_center + 3 * b;
+    for (int a = 0; a < 3; ++a) {
+      r1[a] = point[a] - center[a];
+      r2[a] = I[a] + 0.5 + 0.5 * c[a] - center[a];
+      s->hydro_force[3 * b + a] += impulse[a] + cap[a];
+    }
+    double t[3] = {r1[1] * impulse[2] - r1[2] * impulse[1], r1[2] * impulse[0] - r1[0] * impulse[2],
+                   r1[0] * impulse[1] - r1[1] * impulse[0]};
+    t[0] += r2[1] * cap[2] - r2[2] * cap[1];
+    t[1] += r2[2] * cap[0] - r2[0] * cap[2];
+    t[2] += r2[0] * cap[1] - r2[1] * cap[0];
+    for (int a = 0; a < 3; ++a) s->hydro_torque[3 * b + a] += t[a];
+  }
+  return reflected;
+}
+/* home.py:256-345 (bulk then boundary kernel) */
+void o3d_stream_home(o3d* s) {
+  for (int pass = 0; pass < 2; ++pass)
+    for (int i = 0; i < s->nx; ++i)
+      for (int j = 0; j < s->ny; ++j)
+        for (int k = 0; k < s->nz; ++k) {
+          int I[3] = {i, j, k};
+          long n = lin(s, I);
+          int flag = s->flag[n];
+          if (pass == 0 && flag < 0) {
+            for (int a = 0; a < 3; ++a) s->u[3 * n + a] = 0;
+            if (s->id[n] >= 0) {
+              double point[3] = {i + 0.5, j + 0.5, k + 0.5};
+              body_velocity_at(s, point, s->id[n], s->u + 3 * n);
+            }
+            continue;
+          }
+          if ((pass == 0 && flag >= 2) || (pass == 1 && flag != 2) || flag < 0) continue;
+          double P = 0, jv[3] = {0, 0, 0}, S[3][3] = {{0}};
+          for (int q = 0; q < Q27; ++q) {
+            int c[3], J[3];
+            dir27(q, c);
+            for (int a = 0; a < 3; ++a) J[a] = I[a] - c[a];
+            double value;
+            if (pass == 0) {
+              double m[10];
+              const double* src = s->home_post + 10 * n;
+              int neg = 1;
+              if (fluid(s, J)) {
+                src = s->home_post + 10 * lin(s, J);
+                neg = 0;
+              }
+              for (int t = 0; t < 10; ++t) m[t] = src[t];
+              if (neg)
+                for (int t = 1; t < 4; ++t) m[t] = -m[t];
+              value = o3d_reconstruct(q, m);
+            } else {
+              if (fluid(s, J))
+                value = o3d_reconstruct(q, s->home_post + 10 * lin(s, J));
+              else
+                value = home_reflection(s, I, 26 - q);
+            }
+            P += value;
+            for (int a = 0; a < 3; ++a) jv[a] += c[a] * value;
+            for (int a = 0; a < 3; ++a)
+              for (int b = 0; b < 3; ++b) S[a][b] += (c[a] * c[b] - (a == b ? 1.0 / 3 : 0.0)) * value;
+          }
+          pack(s->home + 10 * n, P, jv, S);
+          for (int a = 0; a < 3; ++a) s->u[3 * n + a] = jv[a] + 0.5 * s->accel[3 * n + a];
+        }
+}
+/* home.py:347-353 */
+void o3d_recover_pressure(o3d* s) {
+  for (long n = 0; n < cells(s); ++n)
+    if (s->id[n] == -1) s->pressure[n] = s->home[10 * n] * density(s, s->phi[n]) / 3;
+}
+
+/* ----------------------------------------------------------- refill */
+/* home.py:87-100 */
+static void set_momentum_equilibrium(o3d* s, long n, double phi, double p, const double u[3]) {
+  const double* a = s->accel + 3 * n;
+  double j[3], S[3][3];
+  for (int x = 0; x < 3; ++x) j[x] = u[x] - 0.5 * a[x];
+  for (int x = 0; x < 3; ++x)
+    for (int y = 0; y < 3; ++y) S[x][y] = u[x] * u[y] - 0.5 * (a[x] * u[y] + u[x] * a[y]);
+  pack(s->home + 10 * n, 3 * p / density(s, phi), j, S);
+  for (int x = 0; x < 3; ++x) s->u[3 * n + x] = u[x];
+}
+static void set_phase_equilibrium(o3d* s, const int I[3], double phi, const double u[3]) {
+  for (int q = 0; q < 7; ++q) s->h[phase_address(s, I, q)] = eq7(q, phi, u);
+}
+/* simulator.py:385-395 */
+void o3d_seed_populations(o3d* s) {
+  double zero[3] = {0, 0, 0};
+  for (long n = 0; n < cells(s); ++n)
+    if (s->id[n] == -1) set_momentum_equilibrium(s, n, s->phi[n], s->pressure[n], zero);
+}
+/* simulator.py:595-613 */
+void o3d_find_refill_changes(o3d* s) {
+  int shape[3] = {s->nx, s->ny, s->nz};
+  s->openings = 0;
+  s->covered = 0;
+  for (int a = 0; a < 3; ++a) {
+    s->refill_lower[a] = shape[a];
+    s->refill_upper[a] = -1;
+  }
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        int I[3] = {i, j, k};
+        long n = lin(s, I);
+        int is_fluid = s->id[n] == -1, valid = is_fluid && s->prev_id[n] == -1, touch = 0;
+        if (is_fluid && !valid) {
+          s->openings++;
+          touch = 1;
+        }
+        if (s->prev_id[n] == -1 && !is_fluid) {
+          s->covered++;
+          touch = 1;
+        }
+        if (touch)
+          for (int a = 0; a < 3; ++a) {
+            if (I[a] < s->refill_lower[a]) s->refill_lower[a] = I[a];
+            if (I[a] > s->refill_upper[a]) s->refill_upper[a] = I[a];
+          }
+      }
+}
+static void refill_box(const o3d* s, int lo[3], int hi[3]) {
+  int shape[3] = {s->nx, s->ny, s->nz};
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = s->refill_lower[a] - 1 > 0 ? s->refill_lower[a] - 1 : 0;
+    hi[a] = s->refill_upper[a] + 2 < shape[a] ? s->refill_upper[a] + 2 : shape[a];
+  }
+}
+/* simulator.py:615-629 */
+void o3d_seed_refill_region(o3d* s) {
+  int lo[3], hi[3];
+  refill_box(s, lo, hi);
+  for (int i = lo[0]; i < hi[0]; ++i)
+    for (int j = lo[1]; j < hi[1]; ++j)
+      for (int k = lo[2]; k < hi[2]; ++k) {
+        long n = idx(s, i, j, k);
+        double* st = s->refill_state + 5 * n;
+        st[0] = s->phi[n];
+        st[1] = s->pressure[n];
+        for (int a = 0; a < 3; ++a) st[2 + a] = s->u[3 * n + a];
+        s->refill_valid[n] = (int8_t)(s->id[n] == -1 && s->prev_id[n] == -1);
+      }
+}
+/* simulator.py:631-651 */
+void o3d_propagate_refill(o3d* s) {
+  int lo[3], hi[3];
+  refill_box(s, lo, hi);
+  s->remaining = 0;
+  for (int i = lo[0]; i < hi[0]; ++i)
+    for (int j = lo[1]; j < hi[1]; ++j)
+      for (int k = lo[2]; k < hi[2]; ++k) {
+        int I[3] = {i, j, k};
+        long n = lin(s, I);
+        int valid = s->refill_valid[n];
+        double value[5];
+        for (int t = 0; t < 5; ++t) value[t] = s->refill_state[5 * n + t];
+        if (fluid(s, I) && valid == 0) {
+          double total[5] = {0, 0, 0, 0, 0}, count = 0;
+          for (int q = 0; q < Q27; ++q) {
+            if (q == REST) continue;
+            int c[3];
+            dir27(q, c);
+            if (!connected(s, I, c)) continue;
+            int J[3] = {i + c[0], j + c[1], k + c[2]};
+            long m = lin(s, J);
+            if (!s->refill_valid[m]) continue;
+            for (int t = 0; t < 5; ++t) total[t] += w27(q) * s->refill_state[5 * m + t];
+            count += w27(q);
+          }
+          if (count > 0) {
+            for (int t = 0; t < 5; ++t) value[t] = total[t] / count;
+            valid = 1;
+          } else
+            s->remaining++;
+        }
+        for (int t = 0; t < 5; ++t) s->refill_next[5 * n + t] = value[t];
+        s->refill_next_valid[n] = (int8_t)valid;
+      }
+}
+/* simulator.py:653-660 */
+void o3d_accept_refill(o3d* s) {
+  int lo[3], hi[3];
+  refill_box(s, lo, hi);
+  for (int i = lo[0]; i < hi[0]; ++i)
+    for (int j = lo[1]; j < hi[1]; ++j)
+      for (int k = lo[2]; k < hi[2]; ++k) {
+        long n = idx(s, i, j, k);
+        for (int t = 0; t < 5; ++t) s->refill_state[5 * n + t] = s->refill_next[5 * n + t];
+        s->refill_valid[n] = s->refill_next_valid[n];
+      }
+}
+/* simulator.py:662-678 */
+void o3d_commit_refill(o3d* s) {
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        long n = idx(s, i, j, k);
+        if (s->id[n] == -1 && s->prev_id[n] != -1) {
+          if (s->refill_valid[n]) {
+            const double* st = s->refill_state + 5 * n;
+            s->phi[n] = st[0];
+            s->pressure[n] = st[1];
+            for (int a = 0; a < 3; ++a) s->u[3 * n + a] = st[2 + a];
+          } else
+            s->id[n] = -3;
+        }
+        if (s->id[n] != -1) {
+          s->phi[n] = 0;
+          for (int a = 0; a < 3; ++a) s->u[3 * n + a] = 0;
+          if (s->id[n] >= 0) {
+            double point[3] = {i + 0.5, j + 0.5, k + 0.5};
+            body_velocity_at(s, point, s->id[n], s->u + 3 * n);
+          }
+        }
+      }
+}
+/* simulator.py:680-706 */
+void o3d_project_refill(o3d* s) {
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = s->refill_lower[a];
+    hi[a] = s->refill_upper[a] + 1;
+  }
+  for (int i = lo[0]; i < hi[0]; ++i)
+    for (int j = lo[1]; j < hi[1]; ++j)
+      for (int k = lo[2]; k < hi[2]; ++k) {
+        int I[3] = {i, j, k};
+        long n = lin(s, I);
+        if (!(s->id[n] == -1 && s->prev_id[n] != -1)) continue;
+        double f[3], acc[3];
+        o3d_force_at(s, I, 0, MODE_GENERAL, f);
+        for (int a = 0; a < 3; ++a) s->force[3 * n + a] = f[a];
+        o3d_acceleration_at(s, I, f, MODE_GENERAL, 0, acc);
+        for (int a = 0; a < 3; ++a) s->accel[3 * n + a] = acc[a];
+      }
+  for (int i = lo[0]; i < hi[0]; ++i)
+    for (int j = lo[1]; j < hi[1]; ++j)
+      for (int k = lo[2]; k < hi[2]; ++k) {
+        int I[3] = {i, j, k};
+        long n = lin(s, I);
+        if (!(s->id[n] == -1 && s->prev_id[n] != -1)) continue;
+        double u[3] = {s->u[3 * n], s->u[3 * n + 1], s->u[3 * n + 2]};
+        set_momentum_equilibrium(s, n, s->phi[n], s->pressure[n], u);
+        set_phase_equilibrium(s, I, s->phi[n], u);
+      }
+}
+
+/* -------------------------------------------------------------- thermal */
+/* thermal.py:64-69 */
+static double water_target(const o3d* s, long n) {
+  if (s->id[n] == -1 && s->phi[n] > 1e-3) return s->phi[n] * s->cell_volume;
+  return 0;
+}
+/* thermal.py:98-131 */
+void o3d_synchronize_water(o3d* s) {
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        long n = idx(s, i, j, k);
+        double target = water_target(s, n), specific = 0;
+        if (target > 0) {
+          if (s->water_volume[n] > 1e-30)
+            specific = s->water_energy[n] / s->water_volume[n];
+          else {
+            double total = 0;
+            for (int a = -1; a <= 1; ++a)
+              for (int b = -1; b <= 1; ++b)
+                for (int c = -1; c <= 1; ++c) {
+                  int J[3] = {i + a, j + b, k + c};
+                  if (!inside(s, J)) continue;
+                  long m = lin(s, J);
+                  if (s->water_volume[m] > 1e-30) {
+                    specific += s->water_energy[m];
+                    total += s->water_volume[m];
+                  }
+                }
+            if (total > 1e-30) specific /= total;
+          }
+        }
+        s->target_energy[n] = target * specific;
+      }
+  for (long n = 0; n < cells(s); ++n) {
+    s->remap_energy += s->target_energy[n] - s->water_energy[n];
+    s->water_volume[n] = water_target(s, n);
+    s->water_energy[n] = s->target_energy[n];
+  }
+}
+/* thermal.py:88-96 */
+void o3d_initialize_water_heat(o3d* s) {
+  for (long n = 0; n < cells(s); ++n) {
+    s->water_volume[n] = water_target(s, n);
+    s->water_energy[n] = s->water_volume[n] * s->rho_cp * (s->water_t0 - s->tm);
+  }
+}
+/* thermal.py:133-169 */
+void o3d_thermal_fluxes(o3d* s, int advect) {
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        int I[3] = {i, j, k};
+        long n = lin(s, I);
+        double ef[3] = {0, 0, 0}, vf[3] = {0, 0, 0};
+        for (int a = 0; a < 3; ++a) {
+          int J[3] = {i, j, k};
+          J[a] += 1;
+          if (!(fluid(s, I) && fluid(s, J))) continue;
+          long m = lin(s, J);
+          double vi = s->water_volume[n], vj = s->water_volume[m];
+          if (!(vi > 1e-30 && vj > 1e-30)) continue;
+          if (advect) {
+            double speed = 0.5 * (s->u[3 * n + a] + s->u[3 * m + a]) * s->velocity_m_s;
+            double V = vi, E = s->water_energy[n];
+            if (speed < 0) {
+              V = vj;
+              E = s->water_energy[m];
+            }
+            vf[a] = speed * fmin(1.0, V / s->cell_volume) * s->dx * s->dx;
+            ef[a] = vf[a] * E / V;
+          } else {
+            double aperture = fmin(vi, vj) / s->cell_volume;
+            ef[a] = s->k_water * s->dx * aperture * (water_t(s, n) - water_t(s, m));
+          }
+        }
+        for (int a = 0; a < 3; ++a) {
+          s->energy_flux[3 * n + a] = ef[a];
+          s->volume_flux[3 * n + a] = vf[a];
+        }
+      }
+}
+/* thermal.py:171-184 */
+void o3d_apply_thermal_fluxes(o3d* s, double dt, int advect) {
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        long n = idx(s, i, j, k);
+        double de = 0, dv = 0;
+        for (int a = 0; a < 3; ++a) {
+          int J[3] = {i, j, k};
+          J[a] -= 1;
+          de += s->energy_flux[3 * n + a];
+          dv += s->volume_flux[3 * n + a];
+          if (inside(s, J)) {
+            long m = lin(s, J);
+            de -= s->energy_flux[3 * m + a];
+            dv -= s->volume_flux[3 * m + a];
+          }
+        }
+        s->water_energy[n] -= dt * de;
+        if (advect) s->water_volume[n] -= dt * dv;
+      }
+}
+/* thermal.py:186-206 */
+void o3d_measure_outflow(o3d* s) {
+  s->max_outflow = 0;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        int I[3] = {i, j, k};
+        long n = lin(s, I);
+        if (!(fluid(s, I) && s->water_volume[n] > 1e-30)) continue;
+        double rate = 0;
+        for (int a = 0; a < 3; ++a)
+          for (int sign = 0; sign < 2; ++sign) {
+            int J[3] = {i, j, k}, d = 2 * sign - 1;
+            J[a] += d;
+            if (fluid(s, J) && s->water_volume[lin(s, J)] > 1e-30)
+              rate += fmax(0.0, d * 0.5 * (s->u[3 * n + a] + s->u[3 * lin(s, J) + a]));
+          }
+        if (rate > s->max_outflow) s->max_outflow = rate;
+      }
+}
+/* thermal.py:53-62 */
+void o3d_initialize_material(o3d* s) {
+  for (int b = 0; b < s->bodies; ++b)
+    for (int i = 0; i < s->mx; ++i)
+      for (int j = 0; j < s->my; ++j)
+        for (int k = 0; k < s->mz; ++k) {
+          int J[3] = {i, j, k};
+          if (!mat_inside(s, b, J)) continue;
+          long n = mat(s, b, i, j, k);
+          s->material_mass[n] = s->initial_cell_mass;
+          s->material_energy[n] = s->initial_cell_mass * s->cp_ice * (s->body_t0[b] - s->tm);
+        }
+}
+/* thermal.py:260-281 */
+void o3d_body_conduction(o3d* s, double dt) {
+  for (int b = 0; b < s->bodies; ++b)
+    for (int i = 0; i < s->mx; ++i)
+      for (int j = 0; j < s->my; ++j)
+        for (int k = 0; k < s->mz; ++k) {
+          int I[3] = {i, j, k};
+          long n = mat(s, b, i, j, k);
+          double mass = s->material_mass[n], heat = 0;
+          if (mass > 0) {
+            double t = ice_t(s, b, I);
+            for (int f = 0; f < 6; ++f) {
+              int J[3] = {i + FACES[f][0], j + FACES[f][1], k + FACES[f][2]};
+              if (!mat_inside(s, b, J)) continue;
+              double other = s->material_mass[mat(s, b, J[0], J[1], J[2])];
+              if (other > 0) {
+                double aperture = fmin(mass, other) / s->initial_cell_mass;
+                heat += dt * s->k_ice * s->dx * aperture * (ice_t(s, b, J) - t);
+              }
+            }
+          }
+          s->material_heat[n] = heat;
+        }
+}
+/* thermal.py:283-293 */
+static int touches_thermal_ice(const o3d* s, const int I[3]) {
+  for (int f = 0; f < 7; ++f) {
+    int J[3] = {I[0], I[1], I[2]};
+    if (f < 6)
+      for (int a = 0; a < 3; ++a) J[a] += FACES[f][a];
+    if (inside(s, J)) {
+      long m = lin(s, J);
+      if (s->cov_id[m] >= 0 && s->coverage[m] > 1e-8) return 1;
+    }
+  }
+  return 0;
+}
+/* thermal.py:308-392 */
+void o3d_exchange_heat_cell(o3d* s, int i, int j, int k, double dt) {
+  int I[3] = {i, j, k};
+  long n = lin(s, I);
+  if (!(s->water_volume[n] > 1e-30 && fluid(s, I) && touches_thermal_ice(s, I))) return;
+  double heat[56] = {0}, requested = 0;
+  int body[7] = {0}, base[7][3] = {{0}};
+  double tw = water_t(s, n), aperture = s->water_volume[n] / s->cell_volume;
+  double kh = 2 * s->k_water * s->k_ice / (s->k_water + s->k_ice);
+  for (int f = 0; f < 7; ++f) {
+    int J[3] = {i, j, k};
+    if (f < 6)
+      for (int a = 0; a < 3; ++a) J[a] += FACES[f][a];
+    if (!inside(s, J)) continue;
+    long m = lin(s, J);
+    int b = s->cov_id[m];
+    double fraction = s->coverage[m];
+    if (!(b >= 0 && fraction > 1e-8)) continue;
+    double point[3] = {J[0] + 0.5, J[1] + 0.5, J[2] + 0.5}, grid[3], part[3];
+    material_grid(s, point, b, grid);
+    for (int a = 0; a < 3; ++a) {
+      base[f][a] = (int)floor(grid[a]);
+      part[a] = grid[a] - base[f][a];
+    }
+    body[f] = b;
+    double contact = fmin(aperture, fmin(1.0, fraction / s->threshold));
+    for (int a = 0; a < 2; ++a)
+      for (int c = 0; c < 2; ++c)
+        for (int d = 0; d < 2; ++d) {
+          int K[3] = {base[f][0] + a, base[f][1] + c, base[f][2] + d};
+          if (!mat_inside(s, b, K)) continue;
+          double mass = s->material_mass[mat(s, b, K[0], K[1], K[2])];
+          if (!(mass > 0)) continue;
+          double w = (a ? part[0] : 1 - part[0]) * (c ? part[1] : 1 - part[1]) * (d ? part[2] : 1 - part[2]);
+          double share = w * mass / (s->initial_cell_mass * fraction);
+          double value = dt * kh * s->dx * contact * share * fmax(0.0, tw - ice_t(s, b, K));
+          heat[8 * f + 4 * a + 2 * c + d] = value;
+          requested += value;
+        }
+  }
+  double scale = fmin(1.0, fmax(0.0, s->water_energy[n]) / fmax(requested, 1e-30));
+  for (int f = 0; f < 7; ++f)
+    for (int a = 0; a < 2; ++a)
+      for (int c = 0; c < 2; ++c)
+        for (int d = 0; d < 2; ++d) {
+          double value = scale * heat[8 * f + 4 * a + 2 * c + d];
+          if (value > 0)
+            s->material_heat[mat(s, body[f], base[f][0] + a, base[f][1] + c, base[f][2] + d)] += value;
+        }
+  s->water_energy[n] -= scale * requested;
+}
+/* thermal.py:295-306 */
+void o3d_thermal_ice_bounds(const o3d* s, int lo[3], int hi[3]) {
+  int shape[3] = {s->nx, s->ny, s->nz};
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = shape[a];
+    hi[a] = 0;
+  }
+  for (int b = 0; b < s->bodies; ++b)
+    for (int a = 0; a < 3; ++a) {
+      double extent = 0;
+      for (int c = 0; c < 3; ++c) extent += fabs(s->body_rot[9 * b + 3 * a + c]) * (0.5 * s->material_size[3 * b + c] + 1.0);
+      int lower = (int)floor(s->body_origin[3 * b + a] - extent) - 3;
+      int upper = (int)ceil(s->body_origin[3 * b + a] + extent) + 3;
+      if (lower < lo[a]) lo[a] = lower;
+      if (upper > hi[a]) hi[a] = upper;
+    }
+  for (int a = 0; a < 3; ++a) {
+    if (lo[a] < 0) lo[a] = 0;
+    if (hi[a] > shape[a]) hi[a] = shape[a];
+  }
+}
+/* thermal.py:394-399 */
+void o3d_exchange_heat(o3d* s, double dt) {
+  int lo[3], hi[3];
+  o3d_thermal_ice_bounds(s, lo, hi);
+  for (int i = lo[0]; i < hi[0]; ++i)
+    for (int j = lo[1]; j < hi[1]; ++j)
+      for (int k = lo[2]; k < hi[2]; ++k) o3d_exchange_heat_cell(s, i, j, k, dt);
+}
+/* thermal.py:401-415 */
+void o3d_apply_melting(o3d* s) {
+  long count = (long)s->bodies * s->mx * s->my * s->mz;
+  for (long n = 0; n < count; ++n) {
+    double mass = s->material_mass[n], energy = s->material_energy[n] + s->material_heat[n];
+    if (energy > 0) {
+      double melted = fmin(mass, energy / s->latent);
+      s->discarded_heat += energy - melted * s->latent;
+      mass -= melted;
+      energy = 0;
+    }
+    if (mass <= 1e-6 * s->initial_cell_mass) {
+      s->discarded_heat += energy - mass * s->latent;
+      mass = 0;
+      energy = 0;
+    }
+    s->material_mass[n] = mass;
+    s->material_energy[n] = energy;
+  }
+}
+/* simulator.py:738-776 */
+void o3d_reduce_mass(o3d* s) {
+  for (int b = 0; b < s->bodies; ++b) {
+    double* r = s->reduced + 19 * b;
+    for (int t = 0; t < 13; ++t) r[t] = 0;
+    for (int a = 0; a < 3; ++a) {
+      r[13 + a] = 1e9;
+      r[16 + a] = -1e9;
+    }
+    for (int i = 0; i < s->mx; ++i)
+      for (int j = 0; j < s->my; ++j)
+        for (int k = 0; k < s->mz; ++k) {
+          double m = s->material_mass[mat(s, b, i, j, k)] / s->mass_kg;
+          if (!(m > 0)) continue;
+          double p[3] = {i + 0.5 - 0.5 * s->material_size[3 * b], j + 0.5 - 0.5 * s->material_size[3 * b + 1],
+                         k + 0.5 - 0.5 * s->material_size[3 * b + 2]};
+          double pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+          r[0] += m;
+          for (int a = 0; a < 3; ++a) r[1 + a] += m * p[a];
+          for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) r[4 + 3 * a + c] += m * ((a == c ? pp + 1.0 / 6 : 0.0) - p[a] * p[c]);
+          for (int a = 0; a < 3; ++a) {
+            if (p[a] - 0.5 < r[13 + a]) r[13 + a] = p[a] - 0.5;
+            if (p[a] + 0.5 > r[16 + a]) r[16 + a] = p[a] + 0.5;
+          }
+        }
+  }
+}
+/* thermal.py:417-428 */
+void o3d_sample_ice_temperature(o3d* s) {
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        long n = idx(s, i, j, k);
+        double t = s->tm;
+        int b = s->id[n];
+        if (b >= 0) {
+          double point[3] = {i + 0.5, j + 0.5, k + 0.5}, grid[3];
+          int J[3];
+          material_grid(s, point, b, grid);
+          for (int a = 0; a < 3; ++a) {
+            J[a] = (int)floor(grid[a] + 0.5);
+            if (J[a] < 0) J[a] = 0;
+            if (J[a] > s->material_size[3 * b + a] - 1) J[a] = s->material_size[3 * b + a] - 1;
+          }
+          t = ice_t(s, b, J);
+        }
+        s->sampled_ice_t[n] = t;
+      }
+}
+/* thermal.py:430-440 */
+void o3d_check_heat(o3d* s) {
+  s->thermal_invalid = 0;
+  for (long n = 0; n < cells(s); ++n) {
+    double e = s->water_energy[n], v = s->water_volume[n];
+    if (isnan(e) || isinf(e) || e < -1e-10 || v < -1e-20) s->thermal_invalid++;
+  }
+  long count = (long)s->bodies * s->mx * s->my * s->mz;
+  for (long n = 0; n < count; ++n) {
+    double m = s->material_mass[n], e = s->material_energy[n];
+    if (isnan(m) || isnan(e) || isinf(e) || m < 0) s->thermal_invalid++;
+  }
+}
+/* simulator.py:325-347 */
+void o3d_initialize_phase(o3d* s) {
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        long n = idx(s, i, j, k);
+        double phi = 0;
+        if (s->id[n] == -1) {
+          double column = fmin(s->water_width - i - 0.5, s->water_height - j - 0.5), distance = column;
+          if (s->floor_depth > 0) distance = fmax(column, s->boundary + s->floor_depth - j - 0.5);
+          phi = 0.5 + 0.5 * tanh(2 * distance / s->width);
+        }
+        s->phi[n] = phi;
+        double zero[3] = {0, 0, 0};
+        for (int q = 0; q < 7; ++q) s->h[7 * n + q] = eq7(q, phi, zero);
+      }
+}
+
+/* Probes for the reference's own probe kernels (tests only). */
+void o3d_probe_stress_velocity(const o3d* s, const int32_t I[3], const int32_t c[3], double out[3]) {
+  int Ii[3] = {I[0], I[1], I[2]}, ci[3] = {c[0], c[1], c[2]};
+  stress_neighbor_velocity(s, Ii, ci, out);
+}
+void o3d_probe_mirror(const o3d* s, const int32_t I[3], const int32_t c[3], double out[4]) {
+  int Ii[3] = {I[0], I[1], I[2]}, ci[3] = {c[0], c[1], c[2]};
+  int J[3] = {I[0] + c[0], I[1] + c[1], I[2] + c[2]};
+  ice_mirror_point(s, Ii, J, out);
+  out[3] = wall_phase_sample(s, Ii, ci);
+}
+/* The reference tests' seed kernels (test_numerics.py:37-46,
+ * test_mirror_gradient.py:49-55): equilibrium from the current fields. */
+void o3d_set_equilibrium_all(o3d* s, int with_phase) {
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      for (int k = 0; k < s->nz; ++k) {
+        int I[3] = {i, j, k};
+        long n = lin(s, I);
+        if (s->id[n] != -1) continue;
+        double u[3] = {s->u[3 * n], s->u[3 * n + 1], s->u[3 * n + 2]};
+        set_momentum_equilibrium(s, n, s->phi[n], s->pressure[n], u);
+        if (with_phase) set_phase_equilibrium(s, I, s->phi[n], u);
+      }
+}

@@ -1,0 +1,146 @@
+"""ctypes binding of ``libicy3d.so`` (the C ABI declared in include/icy3d.h).
+
+The library is loaded after ``import torch`` so that it resolves the HIP
+runtime PyTorch already loaded (both carry the SONAME libamdhip64.so.7); a
+second runtime would not share streams or allocations.  A missing library is
+an error: there is no fallback path.
+"""
+
+import ctypes
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libicy3d.so"
+MAX_BODIES = 32
+STATE_STRIDE = 21
+REDUCED_STRIDE = 19
+
+_f = ctypes.c_float
+_i = ctypes.c_int32
+_p = ctypes.c_void_p
+
+GRID_INTS = ("nx", "ny", "nz", "boundary", "bodies", "mx", "my", "mz",
+             "water_width", "water_height", "floor_depth", "check_stability")
+GRID_FLOATS = (
+    "rho_air", "gravity_y", "sigma", "width", "mobility", "normal_eps", "bulk_rate",
+    "threshold", "max_velocity", "expansion", "water_t0", "air_t", "flux_limit",
+    "rho_water_si", "rho_air_si", "mu_air", "nu_scale",
+    "tm", "rho_cp", "cp_ice", "k_water", "k_ice", "latent", "cell_volume", "dx",
+    "velocity_m_s", "initial_cell_mass", "mass_kg",
+)
+GRID_POINTERS = (
+    "home", "home_post", "h", "phi", "pressure", "old_pressure", "coverage",
+    "u", "old_u", "force", "accel", "grad", "flux", "id", "prev_id", "cov_id", "flag",
+    "refill_state", "refill_next", "refill_valid", "refill_next_valid",
+    "ref_density", "ref_pressure", "body", "material_size", "body_t0",
+    "material_mass", "material_energy", "material_heat",
+    "water_volume", "water_energy", "target_energy", "sampled_ice_t",
+    "energy_flux", "volume_flux", "control",
+)
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [(n, _i) for n in GRID_INTS] + [(n, _f) for n in GRID_FLOATS] + [(n, _p) for n in GRID_POINTERS]
+
+
+class Control(ctypes.Structure):
+    _fields_ = [
+        ("invalid_index", _i), ("thermal_invalid", _i), ("openings", _i), ("covered", _i),
+        ("remaining", _i), ("repair_count", _i),
+        ("refill_lower", _i * 3), ("refill_upper", _i * 3),
+        ("geometry_lower", _i * 3), ("geometry_upper", _i * 3),
+        ("maximum_speed", _f), ("peak_speed", _f), ("phase_clip_volume", _f),
+        ("remap_energy", _f), ("discarded_heat", _f), ("max_outflow", _f),
+        ("loads", (_f * 6) * MAX_BODIES),
+        ("reduced", (_f * REDUCED_STRIDE) * MAX_BODIES),
+    ]
+
+
+class IcyError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what} failed with icy3d code {code}")
+        self.code = code
+
+
+def check(code, what="icy3d call"):
+    if code != 0:
+        raise IcyError(code, what)
+
+
+_GRID = ctypes.POINTER(Grid)
+_I3 = ctypes.POINTER(_i)
+
+# name -> argument types after the grid pointer (stream is always last).
+_GRID_CALLS = {
+    "initialize_material": [], "initialize_phase": [], "initialize_water_heat": [],
+    "seed_populations": [], "set_equilibrium_all": [_i, _i, _i, _f],
+    "phase_warmup": [_i, ctypes.POINTER(_i)],
+    "classify": [_I3, _I3], "gradients": [_I3, _I3], "refill_gradients": [_I3, _I3],
+    "collide_phase": [_i, _i], "recover_phase": [_i, _i], "recover_pressure": [],
+    "scan_stability": [], "forces": [_i, _i], "collide_momentum": [], "stream_home": [],
+    "flow_step": [_i, _i], "repair_phase_layout": [_i], "export_phase": [_i, _p],
+    "rasterize": [], "find_refill_changes": [], "seed_refill_region": [_I3, _I3],
+    "propagate_refill": [_I3, _I3], "accept_refill": [_I3, _I3], "commit_refill": [],
+    "project_refill": [_i, _I3, _I3],
+    "synchronize_water": [], "thermal_fluxes": [_i], "apply_thermal_fluxes": [_f, _i],
+    "measure_outflow": [], "body_conduction": [_f], "exchange_heat": [_f, _I3, _I3],
+    "apply_melting": [], "reduce_mass": [], "sample_ice_temperature": [], "check_heat": [],
+    "capillary_stress": [_p], "generic_forces": [_p], "write_gradient_all": [],
+    "probe_stress_velocity": [_I3, _I3, _p], "probe_mirror": [_I3, _I3, _p],
+}
+
+_LIB = None
+
+
+def library_path():
+    return LIB_PATH
+
+
+def lib():
+    """Load and type the native library (import torch first)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    import torch  # noqa: F401  -- share PyTorch's HIP runtime
+
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH.name} is not built; run `python -m projecticy_amd.build` "
+            "(or __graft_entry__.build()) first"
+        )
+    handle = ctypes.CDLL(str(LIB_PATH))
+    for name, args in _GRID_CALLS.items():
+        fn = getattr(handle, "icy3d_" + name)
+        fn.argtypes = [_GRID, *args, _p]
+        fn.restype = ctypes.c_int
+    handle.icy3d_probe_home.argtypes = [_p, _i, _p, _p, _p]
+    handle.icy3d_probe_phase.argtypes = [_p, _i, _f, _f, _p, _p]
+    handle.icy3d_probe_viscosity.argtypes = [_p, _i, _p, _i, _f, _f, _f, _p, _p]
+    for name in ("probe_home", "probe_phase", "probe_viscosity"):
+        getattr(handle, "icy3d_" + name).restype = ctypes.c_int
+    from .iceflow3d.rigid import _RigidArrays
+
+    rigid = ctypes.POINTER(_RigidArrays)
+    handle.icy3d_rigid_integrate.argtypes = [rigid, _p, _p, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_int)]
+    handle.icy3d_rigid_resolve_contacts.argtypes = [rigid, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
+                                                    ctypes.POINTER(ctypes.c_double)]
+    handle.icy3d_rigid_contact_residuals.argtypes = [rigid, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                                     ctypes.POINTER(ctypes.c_double)]
+    handle.icy3d_rigid_update_mass.argtypes = [rigid, _p]
+    handle.icy3d_rigid_pack_state.argtypes = [rigid, _p]
+    for name in ("rigid_integrate", "rigid_resolve_contacts", "rigid_contact_residuals", "rigid_update_mass",
+                 "rigid_pack_state"):
+        getattr(handle, "icy3d_" + name).restype = ctypes.c_int
+    sizes = (_i(), _i())
+    handle.icy3d_abi_sizes(ctypes.byref(sizes[0]), ctypes.byref(sizes[1]))
+    if (sizes[0].value, sizes[1].value) != (ctypes.sizeof(Grid), ctypes.sizeof(Control)):
+        raise RuntimeError(
+            f"icy3d ABI mismatch: library grid/control {sizes[0].value}/{sizes[1].value} bytes, "
+            f"binding {ctypes.sizeof(Grid)}/{ctypes.sizeof(Control)}"
+        )
+    _LIB = handle
+    return _LIB
+
+
+def int3(values):
+    return (_i * 3)(*[int(v) for v in values])

@@ -1,0 +1,290 @@
+// Device helpers shared by the icy3d HIP translation units (gfx950 only).
+//
+// Lattice conventions follow the reference (iceflow3d/lattice.py:11-46):
+// D3Q27 direction q = 9(cx+1) + 3(cy+1) + (cz+1), rest q = 13, weights
+// rounded once per order (8/27, 2/27, 1/54, 1/216); D3Q7 order
+// 0,+x,-x,+y,-y,+z,-z (phase.py:15-24).  Populations are never stored: the
+// HOME moments (p*, u, S) are reconstructed per direction in registers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/icy3d.h"
+
+namespace icy {
+
+constexpr int kBlock = 256;
+
+__host__ __device__ constexpr int cx(int q) { return q / 9 - 1; }
+__host__ __device__ constexpr int cy(int q) { return q / 3 % 3 - 1; }
+__host__ __device__ constexpr int cz(int q) { return q % 3 - 1; }
+__host__ __device__ constexpr int order(int q) {
+  return (cx(q) != 0) + (cy(q) != 0) + (cz(q) != 0);
+}
+__host__ __device__ constexpr float w27(int q) {
+  return order(q) == 0 ? 8.0f / 27.0f : (order(q) == 1 ? 2.0f / 27.0f : (order(q) == 2 ? 1.0f / 54.0f : 1.0f / 216.0f));
+}
+
+constexpr int kC7[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+__host__ __device__ constexpr int opp7(int q) { return q == 0 ? 0 : (q % 2 == 1 ? q + 1 : q - 1); }
+
+struct Cell {
+  int i, j, k, n;
+};
+
+__device__ __forceinline__ Cell cell_of(const icy3d_grid& g, int n) {
+  Cell c;
+  c.n = n;
+  c.k = n % g.nz;
+  int t = n / g.nz;
+  c.j = t % g.ny;
+  c.i = t / g.ny;
+  return c;
+}
+__device__ __forceinline__ int lin(const icy3d_grid& g, int i, int j, int k) { return (i * g.ny + j) * g.nz + k; }
+__device__ __forceinline__ bool inside(const icy3d_grid& g, int i, int j, int k) {
+  return (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny && (unsigned)k < (unsigned)g.nz;
+}
+__device__ __forceinline__ bool fluid_at(const icy3d_grid& g, int i, int j, int k) {
+  return inside(g, i, j, k) && g.id[lin(g, i, j, k)] == -1;
+}
+__device__ __forceinline__ size_t cells(const icy3d_grid& g) { return (size_t)g.nx * g.ny * g.nz; }
+
+__device__ __forceinline__ float clamp01(float v) { return fminf(1.0f, fmaxf(0.0f, v)); }
+__device__ __forceinline__ float density(const icy3d_grid& g, float phi) {
+  return g.rho_air + (1.0f - g.rho_air) * clamp01(phi);
+}
+
+// IAPWS-style water viscosity, lattice.py:131-137, as 1e-6 sum c t^e with
+// t^e evaluated as exp2(e log2 t) (one log, four exponentials).
+__device__ __forceinline__ float water_mu(float t_c) {
+  float lt = log2f((t_c + 273.15f) / 300.0f);
+  return 1e-6f * (280.68f * exp2f(-1.9f * lt) + 511.45f * exp2f(-7.7f * lt) + 61.131f * exp2f(-19.6f * lt) +
+                  0.45903f * exp2f(-40.0f * lt));
+}
+// lattice.py:49-55
+__device__ __forceinline__ float mixture_nu(float phi, float t_c, float rho_w, float rho_a, float mu_air) {
+  float f = clamp01(phi);
+  float mu = f * water_mu(t_c) + (1.0f - f) * mu_air;
+  float rho = f * rho_w + (1.0f - f) * rho_a;
+  return mu / rho;
+}
+// thermal.py:71-78
+__device__ __forceinline__ float water_temperature(const icy3d_grid& g, int n) {
+  float v = g.water_volume[n];
+  return v > 1e-30f ? g.tm + g.water_energy[n] / (g.rho_cp * v) : g.air_t;
+}
+__device__ __forceinline__ float lattice_nu(const icy3d_grid& g, int n) {
+  return mixture_nu(g.phi[n], water_temperature(g, n), g.rho_water_si, g.rho_air_si, g.mu_air) * g.nu_scale;
+}
+
+// HOME moment vector: p*, u(3), Sxx, Syy, Szz, Sxy, Sxz, Syz (home.py:16-27).
+struct Moments {
+  float m[10];
+};
+
+__device__ __forceinline__ Moments load_moments(const float* __restrict__ base, size_t stride, int n) {
+  Moments r;
+#pragma unroll
+  for (int t = 0; t < 10; ++t) r.m[t] = base[t * stride + n];
+  return r;
+}
+__device__ __forceinline__ void store_moments(float* __restrict__ base, size_t stride, int n, const Moments& r) {
+#pragma unroll
+  for (int t = 0; t < 10; ++t) base[t * stride + n] = r.m[t];
+}
+
+// Third-order filtered reconstruction of direction (x,y,z), home.py:30-53,
+// with the direction known at compile time so every c-product folds.
+template <int X, int Y, int Z>
+__device__ __forceinline__ float reconstruct_c(const Moments& M) {
+  const float p = M.m[0], ux = M.m[1], uy = M.m[2], uz = M.m[3];
+  const float sxx = M.m[4], syy = M.m[5], szz = M.m[6], sxy = M.m[7], sxz = M.m[8], syz = M.m[9];
+  constexpr float fx = X, fy = Y, fz = Z;
+  float cu = fx * ux + fy * uy + fz * uz;
+  float csc = fx * fx * sxx + fy * fy * syy + fz * fz * szz + 2.0f * (fx * fy * sxy + fx * fz * sxz + fy * fz * syz);
+  float value = p + 3.0f * cu + 4.5f * (csc - (sxx + syy + szz) * (1.0f / 3.0f));
+  // sum over ordered pairs a != b of (c_a^2 - 1/3) c_b (S_aa u_b + 2 S_ab u_a - 2 u_a^2 u_b)
+  float third = 0.0f;
+  constexpr float ax = fx * fx - 1.0f / 3.0f, ay = fy * fy - 1.0f / 3.0f, az = fz * fz - 1.0f / 3.0f;
+  if (Y != 0) third += ax * fy * (sxx * uy + 2.0f * sxy * ux - 2.0f * ux * ux * uy);
+  if (Z != 0) third += ax * fz * (sxx * uz + 2.0f * sxz * ux - 2.0f * ux * ux * uz);
+  if (X != 0) third += ay * fx * (syy * ux + 2.0f * sxy * uy - 2.0f * uy * uy * ux);
+  if (Z != 0) third += ay * fz * (syy * uz + 2.0f * syz * uy - 2.0f * uy * uy * uz);
+  if (X != 0) third += az * fx * (szz * ux + 2.0f * sxz * uz - 2.0f * uz * uz * ux);
+  if (Y != 0) third += az * fy * (szz * uy + 2.0f * syz * uz - 2.0f * uz * uz * uy);
+  if (X != 0 && Y != 0 && Z != 0)
+    third += 2.0f * fx * fy * fz * (sxz * uy + syz * ux + sxy * uz - 2.0f * ux * uy * uz);
+  constexpr float w = (X != 0) + (Y != 0) + (Z != 0) == 0   ? 8.0f / 27.0f
+                      : (X != 0) + (Y != 0) + (Z != 0) == 1 ? 2.0f / 27.0f
+                      : (X != 0) + (Y != 0) + (Z != 0) == 2 ? 1.0f / 54.0f
+                                                            : 1.0f / 216.0f;
+  return w * (value + 13.5f * third);
+}
+template <int Q>
+__device__ __forceinline__ float reconstruct(const Moments& M) {
+  return reconstruct_c<cx(Q), cy(Q), cz(Q)>(M);
+}
+
+// Runtime-direction reconstruction for the rare boundary paths.
+__device__ __noinline__ float reconstruct_rt(int q, const Moments& M);
+
+// home.py:56-58
+__device__ __forceinline__ Moments equilibrium_moments(float p, float ux, float uy, float uz) {
+  Moments r;
+  r.m[0] = p;
+  r.m[1] = ux;
+  r.m[2] = uy;
+  r.m[3] = uz;
+  r.m[4] = ux * ux;
+  r.m[5] = uy * uy;
+  r.m[6] = uz * uz;
+  r.m[7] = ux * uy;
+  r.m[8] = ux * uz;
+  r.m[9] = uy * uz;
+  return r;
+}
+
+// home.py:61-78: forced central-moment collision of the HOME moments.
+__device__ __forceinline__ Moments central_collision(const Moments& M, float ax, float ay, float az, float rate,
+                                                     float bulk) {
+  const float u[3] = {M.m[1] + 0.5f * ax, M.m[2] + 0.5f * ay, M.m[3] + 0.5f * az};
+  const float a[3] = {ax, ay, az};
+  // S in (xx, yy, zz, xy, xz, yz) order
+  const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+  float R[6], F[6], uu[6];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    uu[t] = u[ia[t]] * u[ib[t]];
+    R[t] = M.m[4 + t] - uu[t];
+    F[t] = a[ia[t]] * u[ib[t]] + u[ia[t]] * a[ib[t]];
+  }
+  const float tr_r = (R[0] + R[1] + R[2]) * (1.0f / 3.0f);
+  const float tr_f = (F[0] + F[1] + F[2]) * (1.0f / 3.0f);
+  const float keep_shear = 1.0f - rate, keep_bulk = 1.0f - bulk;
+  const float force_shear = 1.0f - 0.5f * rate, force_bulk = 1.0f - 0.5f * bulk;
+  Moments out;
+  out.m[0] = M.m[0];
+  out.m[1] = M.m[1] + ax;
+  out.m[2] = M.m[2] + ay;
+  out.m[3] = M.m[3] + az;
+#pragma unroll
+  for (int t = 0; t < 3; ++t)
+    out.m[4 + t] = uu[t] + keep_shear * (R[t] - tr_r) + keep_bulk * tr_r + force_shear * (F[t] - tr_f) +
+                   force_bulk * tr_f;
+#pragma unroll
+  for (int t = 3; t < 6; ++t) out.m[4 + t] = uu[t] + keep_shear * R[t] + force_shear * F[t];
+  return out;
+}
+
+// phase.py:38-60: D3Q7 CM-MRT Allen-Cahn collision.
+__device__ __forceinline__ void phase_collision(const float h[7], float phi, const float u[3], const float n[3],
+                                                float mobility, float width, float post[7], float flux[3]) {
+  const float rate = 1.0f / (0.5f + 4.0f * mobility);
+  float second_sum = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float first = h[2 * a + 1] - h[2 * a + 2];
+    float source = phi * (1.0f - phi) / width * n[a];
+    float delta = (1.0f - rate) * (first - phi * u[a]) + (1.0f - 0.5f * rate) * source;
+    float j = phi * u[a] + delta;
+    float second = phi * 0.25f + 2.0f * u[a] * (delta - 0.5f * source);
+    post[2 * a + 1] = 0.5f * (second + j);
+    post[2 * a + 2] = 0.5f * (second - j);
+    flux[a] = 0.5f * (first + j) - phi * u[a];
+    second_sum += second;
+  }
+  post[0] = phi - second_sum;
+}
+__device__ __forceinline__ float phase_equilibrium7(int q, float phi, float ux, float uy, float uz) {
+  float w = q == 0 ? 0.25f : 0.125f;
+  return w * phi * (1.0f + 4.0f * (kC7[q][0] * ux + kC7[q][1] * uy + kC7[q][2] * uz));
+}
+
+// Body state row: origin[3], R[9] (row-major), centre[3], v[3], omega[3].
+__device__ __forceinline__ const float* body_row(const icy3d_grid& g, int b) { return g.body + ICY3D_STATE_STRIDE * b; }
+// simulator.py:253-257
+__device__ __forceinline__ void body_velocity_at(const icy3d_grid& g, float px, float py, float pz, int b, float out[3]) {
+  const float* s = body_row(g, b);
+  float rx = px - s[12], ry = py - s[13], rz = pz - s[14];
+  const float* v = s + 15;
+  const float* w = s + 18;
+  out[0] = v[0] + (w[1] * rz - w[2] * ry);
+  out[1] = v[1] + (w[2] * rx - w[0] * rz);
+  out[2] = v[2] + (w[0] * ry - w[1] * rx);
+}
+// simulator.py:217-223: world point -> material-grid coordinates of body b.
+__device__ __forceinline__ void material_grid(const icy3d_grid& g, float px, float py, float pz, int b, float out[3]) {
+  const float* s = body_row(g, b);
+  float dx = px - s[0], dy = py - s[1], dz = pz - s[2];
+  const int* size = g.material_size + 3 * b;
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    out[a] = (s[3 + a] * dx + s[6 + a] * dy + s[9 + a] * dz) + 0.5f * (float)size[a] - 0.5f;
+}
+__device__ __forceinline__ size_t mat_index(const icy3d_grid& g, int b, int i, int j, int k) {
+  return (((size_t)b * g.mx + i) * g.my + j) * g.mz + k;
+}
+__device__ __forceinline__ bool mat_inside(const icy3d_grid& g, int b, int i, int j, int k) {
+  const int* s = g.material_size + 3 * b;
+  return (unsigned)i < (unsigned)s[0] && (unsigned)j < (unsigned)s[1] && (unsigned)k < (unsigned)s[2];
+}
+// thermal.py:80-86
+__device__ __forceinline__ float ice_temperature(const icy3d_grid& g, int b, int i, int j, int k) {
+  size_t m = mat_index(g, b, i, j, k);
+  float mass = g.material_mass[m];
+  float t = g.tm;
+  if (mass > 1e-30f) t += g.material_energy[m] / (g.cp_ice * mass);
+  return t;
+}
+
+// gradients.py:48-58 (shared by mirror sampling and population reflection)
+__device__ __forceinline__ float link_fraction(const icy3d_grid& g, int n_i, int n_j) {
+  float ci = g.coverage[n_i], cj = g.coverage[n_j];
+  return fminf(0.95f, fmaxf(0.05f, (g.threshold - ci) / fmaxf(cj - ci, 1e-6f)));
+}
+
+// ---- wave-level reductions (wave64) ---------------------------------------
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_min_int(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_sum_int(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ bool wave_leader() { return (threadIdx.x & 63) == 0; }
+// Non-negative float max through the ordered integer image.
+__device__ __forceinline__ void atomic_max_nonneg(float* address, float value) {
+  atomicMax(reinterpret_cast<int*>(address), __float_as_int(value));
+}
+
+}  // namespace icy
+
+#define ICY_LAUNCH_CHECK()                         \
+  do {                                             \
+    hipError_t _e = hipGetLastError();             \
+    if (_e != hipSuccess) return static_cast<int>(_e); \
+  } while (0)
+
+inline unsigned icy_blocks(size_t count) {
+  return static_cast<unsigned>((count + icy::kBlock - 1) / icy::kBlock);
+}

@@ -1,0 +1,1339 @@
+// Flow kernels of the 3D step: D3Q7 AA phase collision and recovery, phase
+// gradients with wall/ice mirrors, force densities, HOME central collision
+// and pull streaming with moving-ice reflection.
+//
+// Storage is SoA planes with z fastest, so a wave64 of consecutive cells
+// reads 256 contiguous bytes per plane.  Every kernel is one thread per
+// cell; the hot bulk paths unroll all 27 (or 26) directions at compile time
+// so that each c-product folds to an add or a negation.  The rare boundary
+// paths (ice links, mirrors) live in separate kernels that exit early on
+// bulk cells, which keeps register pressure of the hot kernels low.
+#include "icy3d_device.h"
+
+namespace icy {
+
+// ---------------------------------------------------------------------------
+// Runtime-direction reconstruction (boundary paths only).
+__device__ __noinline__ float reconstruct_rt(int q, const Moments& M) {
+  const float fx = (float)cx(q), fy = (float)cy(q), fz = (float)cz(q);
+  const float p = M.m[0], ux = M.m[1], uy = M.m[2], uz = M.m[3];
+  const float sxx = M.m[4], syy = M.m[5], szz = M.m[6], sxy = M.m[7], sxz = M.m[8], syz = M.m[9];
+  float cu = fx * ux + fy * uy + fz * uz;
+  float csc = fx * fx * sxx + fy * fy * syy + fz * fz * szz + 2.0f * (fx * fy * sxy + fx * fz * sxz + fy * fz * syz);
+  float value = p + 3.0f * cu + 4.5f * (csc - (sxx + syy + szz) * (1.0f / 3.0f));
+  float ax = fx * fx - 1.0f / 3.0f, ay = fy * fy - 1.0f / 3.0f, az = fz * fz - 1.0f / 3.0f;
+  float third = 0.0f;
+  third += ax * fy * (sxx * uy + 2.0f * sxy * ux - 2.0f * ux * ux * uy);
+  third += ax * fz * (sxx * uz + 2.0f * sxz * ux - 2.0f * ux * ux * uz);
+  third += ay * fx * (syy * ux + 2.0f * sxy * uy - 2.0f * uy * uy * ux);
+  third += ay * fz * (syy * uz + 2.0f * syz * uy - 2.0f * uy * uy * uz);
+  third += az * fx * (szz * ux + 2.0f * sxz * uz - 2.0f * uz * uz * ux);
+  third += az * fy * (szz * uy + 2.0f * syz * uz - 2.0f * uz * uz * uy);
+  third += 2.0f * fx * fy * fz * (sxz * uy + syz * ux + sxy * uz - 2.0f * ux * uy * uz);
+  return w27(q) * (value + 13.5f * third);
+}
+
+// ---------------------------------------------------------------------------
+// AA addressing of D3Q7 (phase.py:69-82): plane offset of logical slot q.
+__device__ __forceinline__ size_t phase_addr(const icy3d_grid& g, const Cell& c, int q, int parity) {
+  const size_t N = cells(g);
+  if (parity != 0 && q != 0) {
+    int i = c.i - kC7[q][0], j = c.j - kC7[q][1], k = c.k - kC7[q][2];
+    if (fluid_at(g, i, j, k)) return (size_t)opp7(q) * N + lin(g, i, j, k);
+  }
+  return (size_t)q * N + c.n;
+}
+
+// home.py:262-278
+__device__ __forceinline__ int boundary_flag(const icy3d_grid& g, const Cell& c) {
+  if (g.id[c.n] != -1) return -1;
+  int flag = 0;
+  for (int q = 0; q < 27; ++q) {
+    if (q == 13) continue;
+    int i = c.i - cx(q), j = c.j - cy(q), k = c.k - cz(q);
+    if (!fluid_at(g, i, j, k)) {
+      flag = max(flag, 1);
+      if (inside(g, i, j, k)) {
+        int id = g.id[lin(g, i, j, k)];
+        if (id >= 0 || id == -3) flag = 2;
+      }
+    }
+  }
+  return flag;
+}
+
+struct Box {
+  int lo[3], hi[3];
+};
+__device__ __forceinline__ bool box_cell(const icy3d_grid& g, const Box& b, int t, Cell& c) {
+  int ex = b.hi[0] - b.lo[0], ey = b.hi[1] - b.lo[1], ez = b.hi[2] - b.lo[2];
+  if (ex <= 0 || ey <= 0 || ez <= 0 || t >= ex * ey * ez) return false;
+  c.k = b.lo[2] + t % ez;
+  int r = t / ez;
+  c.j = b.lo[1] + r % ey;
+  c.i = b.lo[0] + r / ey;
+  c.n = lin(g, c.i, c.j, c.k);
+  return true;
+}
+
+__global__ void k_classify(icy3d_grid g, Box box) {
+  Cell c;
+  if (!box_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
+  g.flag[c.n] = (int8_t)boundary_flag(g, c);
+}
+
+// ---------------------------------------------------------------------------
+// Gradients (gradients.py).
+__device__ __forceinline__ void tank_mirror(const icy3d_grid& g, int& i, int& j, int& k) {
+  const int b = g.boundary;
+  if (i < b) i = 2 * b - 1 - i; else if (i >= g.nx - b) i = 2 * (g.nx - b) - 1 - i;
+  if (j < b) j = 2 * b - 1 - j; else if (j >= g.ny - b) j = 2 * (g.ny - b) - 1 - j;
+  if (k < b) k = 2 * b - 1 - k; else if (k >= g.nz - b) k = 2 * (g.nz - b) - 1 - k;
+}
+// gradients.py:35-46
+__device__ void coverage_normal(const icy3d_grid& g, int i, int j, int k, float out[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    int li = i - (a == 0), lj = j - (a == 1), lk = k - (a == 2);
+    int hi = i + (a == 0), hj = j + (a == 1), hk = k + (a == 2);
+    tank_mirror(g, li, lj, lk);
+    tank_mirror(g, hi, hj, hk);
+    out[a] = 0.0f;
+    if (inside(g, li, lj, lk) && inside(g, hi, hj, hk))
+      out[a] = 0.5f * (g.coverage[lin(g, hi, hj, hk)] - g.coverage[lin(g, li, lj, lk)]);
+  }
+}
+// gradients.py:60-74: reflect J about the coverage-isosurface tangent plane.
+__device__ void ice_mirror_point(const icy3d_grid& g, int i, int j, int k, int ji, int jj, int jk, float out[3]) {
+  const float c[3] = {(float)(ji - i), (float)(jj - j), (float)(jk - k)};
+  const float eta = link_fraction(g, lin(g, i, j, k), lin(g, ji, jj, jk));
+  float gi[3], gj[3], n[3];
+  coverage_normal(g, i, j, k, gi);
+  coverage_normal(g, ji, jj, jk, gj);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) n[a] = (1.0f - eta) * gi[a] + eta * gj[a];
+  float nc = n[0] * c[0] + n[1] * c[1] + n[2] * c[2];
+  float nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  float cc = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+  if (nc <= 1e-6f * sqrtf(nn * cc) || nn < 1e-12f) {
+    n[0] = c[0];
+    n[1] = c[1];
+    n[2] = c[2];
+  }
+  nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  float inv = 1.0f / sqrtf(nn + 1e-30f);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) n[a] *= inv;
+  nc = n[0] * c[0] + n[1] * c[1] + n[2] * c[2];
+  const float J[3] = {(float)ji, (float)jj, (float)jk};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) out[a] = J[a] - 2.0f * (1.0f - eta) * nc * n[a];
+}
+// gradients.py:76-101: bounded trilinear sample over fluid corners only.
+__device__ float fluid_interpolate(const icy3d_grid& g, const float p[3], float fallback) {
+  int base[3];
+  float part[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    base[a] = (int)floorf(p[a]);
+    part[a] = p[a] - (float)base[a];
+  }
+  float value = 0.0f, total = 0.0f;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    int a = t >> 2, b = (t >> 1) & 1, cc = t & 1;
+    int i = base[0] + a, j = base[1] + b, k = base[2] + cc;
+    tank_mirror(g, i, j, k);
+    if (!fluid_at(g, i, j, k)) continue;
+    float w = (a ? part[0] : 1.0f - part[0]) * (b ? part[1] : 1.0f - part[1]) * (cc ? part[2] : 1.0f - part[2]);
+    value += w * g.phi[lin(g, i, j, k)];
+    total += w;
+  }
+  return total > 1e-7f ? value / total : fallback;
+}
+// gradients.py:103-113
+__device__ float wall_phase_sample(const icy3d_grid& g, int i, int j, int k, int dx, int dy, int dz) {
+  int ji = i + dx, jj = j + dy, jk = k + dz;
+  tank_mirror(g, ji, jj, jk);
+  float value = g.phi[lin(g, i, j, k)];
+  if (fluid_at(g, ji, jj, jk)) {
+    value = g.phi[lin(g, ji, jj, jk)];
+  } else if (inside(g, ji, jj, jk)) {
+    float point[3];
+    ice_mirror_point(g, i, j, k, ji, jj, jk, point);
+    value = fluid_interpolate(g, point, value);
+  }
+  return value;
+}
+
+__device__ __forceinline__ void store_gradient(const icy3d_grid& g, int n, float gx, float gy, float gz) {
+  const size_t N = cells(g);
+  g.grad[n] = gx;
+  g.grad[N + n] = gy;
+  g.grad[2 * N + n] = gz;
+}
+
+// gradients.py:359-373 for flag 0 (all links fluid) and flag 1 (tank mirror).
+template <bool TANK>
+__device__ __forceinline__ void regular_gradient(const icy3d_grid& g, const Cell& c) {
+  const float centre = g.phi[c.n];
+  float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    if (q == 13) continue;
+    int i = c.i + cx(q), j = c.j + cy(q), k = c.k + cz(q);
+    if (TANK) tank_mirror(g, i, j, k);
+    float d = g.phi[lin(g, i, j, k)] - centre;
+    const float w = w27(q);
+    if (cx(q)) rx += w * (float)cx(q) * d;
+    if (cy(q)) ry += w * (float)cy(q) * d;
+    if (cz(q)) rz += w * (float)cz(q) * d;
+  }
+  store_gradient(g, c.n, 3.0f * rx, 3.0f * ry, 3.0f * rz);
+}
+
+// gradients.py:116-145: general gradient with wall and ice mirror samples.
+__device__ void general_gradient(const icy3d_grid& g, const Cell& c) {
+  float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+  if (g.id[c.n] == -1) {
+    const float centre = g.phi[c.n];
+    bool wall = false;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      if (q == 13) continue;
+      int i = c.i + cx(q), j = c.j + cy(q), k = c.k + cz(q);
+      if (fluid_at(g, i, j, k)) {
+        float d = g.phi[lin(g, i, j, k)] - centre;
+        const float w = w27(q);
+        if (cx(q)) rx += w * (float)cx(q) * d;
+        if (cy(q)) ry += w * (float)cy(q) * d;
+        if (cz(q)) rz += w * (float)cz(q) * d;
+      } else {
+        wall = true;
+      }
+    }
+    if (wall) {
+      for (int q = 0; q < 27; ++q) {
+        if (q == 13) continue;
+        int dx = cx(q), dy = cy(q), dz = cz(q);
+        if (fluid_at(g, c.i + dx, c.j + dy, c.k + dz)) continue;
+        float d = wall_phase_sample(g, c.i, c.j, c.k, dx, dy, dz) - centre;
+        const float w = w27(q);
+        rx += w * (float)dx * d;
+        ry += w * (float)dy * d;
+        rz += w * (float)dz * d;
+      }
+    }
+  }
+  store_gradient(g, c.n, 3.0f * rx, 3.0f * ry, 3.0f * rz);
+}
+
+__global__ void k_gradients_regular(icy3d_grid g, Box box) {
+  Cell c;
+  if (!box_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
+  int flag = g.flag[c.n];
+  if (flag == 0)
+    regular_gradient<false>(g, c);
+  else if (flag == 1)
+    regular_gradient<true>(g, c);
+  else if (flag < 0)
+    store_gradient(g, c.n, 0.0f, 0.0f, 0.0f);
+}
+__global__ void k_gradients_general(icy3d_grid g, Box box) {
+  Cell c;
+  if (!box_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
+  if (g.flag[c.n] == 2) general_gradient(g, c);
+}
+__global__ void k_write_gradient_all(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int)cells(g)) return;
+  general_gradient(g, cell_of(g, n));
+}
+
+// Capillary stress 1.5 sigma W (|g|^2 I - g g^T), gradients.py:147-158.
+__device__ __forceinline__ void stress_of(const icy3d_grid& g, float gx, float gy, float gz, float s[9]) {
+  const float k = 1.5f * g.sigma * g.width;
+  const float gg = gx * gx + gy * gy + gz * gz;
+  const float v[3] = {gx, gy, gz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) s[3 * a + b] = k * ((a == b ? gg : 0.0f) - v[a] * v[b]);
+}
+__device__ __forceinline__ void load_grad(const icy3d_grid& g, int n, float v[3]) {
+  const size_t N = cells(g);
+  v[0] = g.grad[n];
+  v[1] = g.grad[N + n];
+  v[2] = g.grad[2 * N + n];
+}
+__global__ void k_capillary_stress(icy3d_grid g, float* __restrict__ out) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  if (n >= (int)N) return;
+  float v[3], s[9];
+  load_grad(g, n, v);
+  stress_of(g, v[0], v[1], v[2], s);
+#pragma unroll
+  for (int t = 0; t < 9; ++t) out[t * N + n] = s[t];
+}
+
+// ---------------------------------------------------------------------------
+// D3Q7 AA collision (phase.py:93-127).
+__global__ void k_collide_phase(icy3d_grid g, int parity, int moving_walls) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  if (n >= (int)N || g.id[n] != -1) return;
+  const Cell c = cell_of(g, n);
+  const float phi = g.phi[n];
+  const float u[3] = {g.u[n], g.u[N + n], g.u[2 * N + n]};
+  float gr[3];
+  load_grad(g, n, gr);
+  const float norm = sqrtf(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2] + g.normal_eps * g.normal_eps + 1e-28f);
+  const float nrm[3] = {gr[0] / norm, gr[1] / norm, gr[2] / norm};
+  size_t addr[7];
+  float in[7], post[7], fl[3];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    addr[q] = phase_addr(g, c, q, parity);
+    in[q] = g.h[addr[q]];
+  }
+  phase_collision(in, phi, u, nrm, g.mobility, g.width, post, fl);
+  const float scale = 1.0f - g.rho_air;
+  g.flux[n] = scale * fl[0];
+  g.flux[N + n] = scale * fl[1];
+  g.flux[2 * N + n] = scale * fl[2];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) {
+    float value = post[q];
+    if (q != 0 && moving_walls) {
+      int i = c.i + kC7[q][0], j = c.j + kC7[q][1], k = c.k + kC7[q][2];
+      if (inside(g, i, j, k)) {
+        int m = lin(g, i, j, k);
+        int b = g.id[m];
+        if (b >= 0) {
+          float eta = link_fraction(g, n, m), ub[3];
+          body_velocity_at(g, c.i + 0.5f + eta * kC7[q][0], c.j + 0.5f + eta * kC7[q][1],
+                           c.k + 0.5f + eta * kC7[q][2], b, ub);
+          value -= phi * (kC7[q][0] * ub[0] + kC7[q][1] * ub[1] + kC7[q][2] * ub[2]);
+        }
+      }
+    }
+    // The slot that held logical opp(q) receives post[q] (AA swap).
+    g.h[addr[opp7(q)]] = value;
+  }
+}
+
+// simulator.py:778-794 (per-cell part).
+struct Stability {
+  float speed;
+  int invalid;
+};
+__device__ __forceinline__ Stability stability_at(const icy3d_grid& g, int n, float phi, float p) {
+  const size_t N = cells(g);
+  float speed = fabsf(g.u[n]) + fabsf(g.u[N + n]) + fabsf(g.u[2 * N + n]);
+  bool bad = !isfinite(speed) || speed > g.max_velocity || !isfinite(p) || !isfinite(phi);
+  Stability s;
+  s.speed = isnan(speed) ? 0.0f : speed;
+  s.invalid = bad ? n : INT32_MAX;
+  return s;
+}
+__device__ __forceinline__ void commit_stability(const icy3d_grid& g, Stability s) {
+  float speed = wave_max(s.speed);
+  int invalid = wave_min_int(s.invalid);
+  if (wave_leader()) {
+    if (speed > 0.0f) atomic_max_nonneg(&g.control->maximum_speed, speed);
+    if (invalid != INT32_MAX) atomicMin(&g.control->invalid_index, invalid);
+  }
+}
+
+// phase.py:132-161, fused pressure recovery and stability scan.
+__global__ void k_recover_phase(icy3d_grid g, int parity, int recover_flow) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  bool active = n < (int)N;
+  float clip = 0.0f;
+  Stability st = {0.0f, INT32_MAX};
+  if (active) {
+    if (g.id[n] != -1) {
+      g.phi[n] = 0.0f;
+    } else {
+      const Cell c = cell_of(g, n);
+      float total = g.h[n];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) total += g.h[phase_addr(g, c, 2 * a + 1, parity)] + g.h[phase_addr(g, c, 2 * a + 2, parity)];
+      float bounded = isfinite(total) ? clamp01(total) : total;
+      g.h[n] += bounded - total;
+      g.phi[n] = bounded;
+      clip = bounded - total;
+      if (recover_flow) {
+        float p = g.home[n] * density(g, bounded) / 3.0f;
+        g.pressure[n] = p;
+        if (g.check_stability) st = stability_at(g, n, bounded, p);
+      }
+    }
+  }
+  if (!isfinite(clip)) clip = 0.0f;
+  clip = wave_sum(clip);
+  if (wave_leader() && clip != 0.0f) atomicAdd(&g.control->phase_clip_volume, clip);
+  if (recover_flow && g.check_stability) commit_stability(g, st);
+}
+
+__global__ void k_recover_pressure(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int)cells(g) || g.id[n] != -1) return;
+  g.pressure[n] = g.home[n] * density(g, g.phi[n]) / 3.0f;
+}
+__global__ void k_scan_stability(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  Stability st = {0.0f, INT32_MAX};
+  if (n < (int)cells(g) && g.id[n] == -1) st = stability_at(g, n, g.phi[n], g.pressure[n]);
+  commit_stability(g, st);
+}
+__global__ void k_reset_stability(icy3d_grid g) {
+  g.control->invalid_index = (int32_t)cells(g);
+  g.control->maximum_speed = 0.0f;
+}
+__global__ void k_fold_peak(icy3d_grid g) {
+  g.control->peak_speed = fmaxf(g.control->peak_speed, g.control->maximum_speed);
+}
+
+// phase.py:163-184
+__global__ void k_repair_layout(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  if (n >= (int)N) return;
+  const Cell c = cell_of(g, n);
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int q = 2 * a + 1;
+    int i = c.i + kC7[q][0], j = c.j + kC7[q][1], k = c.k + kC7[q][2];
+    if (!inside(g, i, j, k)) continue;
+    int m = lin(g, i, j, k);
+    bool before = g.prev_id[n] == -1 && g.prev_id[m] == -1;
+    bool after = g.id[n] == -1 && g.id[m] == -1;
+    if (before != after) {
+      float v = g.h[(size_t)opp7(q) * N + n];
+      g.h[(size_t)opp7(q) * N + n] = g.h[(size_t)q * N + m];
+      g.h[(size_t)q * N + m] = v;
+    }
+  }
+}
+// phase.py:186-191 (out is [7][N] logical populations)
+__global__ void k_export_phase(icy3d_grid g, int parity, float* __restrict__ out) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  if (n >= (int)N) return;
+  const Cell c = cell_of(g, n);
+#pragma unroll
+  for (int q = 0; q < 7; ++q) out[(size_t)q * N + n] = g.h[phase_addr(g, c, q, parity)];
+}
+
+// ---------------------------------------------------------------------------
+// Forces (simulator.py:418-519) and HOME acceleration (home.py:127-197).
+enum Mode { kGeneral = 0, kInterior = 1, kTank = 2 };
+
+template <int MODE>
+__device__ __forceinline__ bool link_fluid(const icy3d_grid& g, int i, int j, int k) {
+  if (MODE == kInterior) return true;
+  if (MODE == kTank) {
+    const int b = g.boundary;
+    return i >= b && i < g.nx - b && j >= b && j < g.ny - b && k >= b && k < g.nz - b;
+  }
+  return fluid_at(g, i, j, k);
+}
+
+// simulator.py:193-215
+__device__ bool connected(const icy3d_grid& g, int i, int j, int k, int dx, int dy, int dz) {
+  if (!fluid_at(g, i + dx, j + dy, k + dz)) return false;
+  int distance = abs(dx) + abs(dy) + abs(dz);
+  if (distance <= 1) return true;
+  const int c[3] = {dx, dy, dz};
+  bool valid = false;
+  for (int a = 0; a < 3; ++a) {
+    if (c[a] == 0) continue;
+    int A[3] = {i, j, k};
+    A[a] += c[a];
+    if (!fluid_at(g, A[0], A[1], A[2])) continue;
+    if (distance == 2) {
+      valid = true;
+    } else {
+      for (int b = 0; b < 3; ++b) {
+        if (b == a) continue;
+        int B[3] = {A[0], A[1], A[2]};
+        B[b] += c[b];
+        if (fluid_at(g, B[0], B[1], B[2])) valid = true;
+      }
+    }
+  }
+  return valid;
+}
+
+// simulator.py:397-416: stress of the link neighbour (planar-wall mirror).
+__device__ void capillary_neighbor(const icy3d_grid& g, int i, int j, int k, int dx, int dy, int dz, float s[9]) {
+  float v[3];
+  load_grad(g, lin(g, i, j, k), v);
+  int ji = i + dx, jj = j + dy, jk = k + dz;
+  if (fluid_at(g, ji, jj, jk)) {
+    load_grad(g, lin(g, ji, jj, jk), v);
+  } else if (inside(g, ji, jj, jk) && g.id[lin(g, ji, jj, jk)] == -2) {
+    const int b = g.boundary;
+    const int shape[3] = {g.nx, g.ny, g.nz};
+    int J[3] = {ji, jj, jk};
+    float r[3] = {1.0f, 1.0f, 1.0f};
+    for (int a = 0; a < 3; ++a) {
+      if (J[a] < b) {
+        J[a] = 2 * b - 1 - J[a];
+        r[a] = -1.0f;
+      } else if (J[a] >= shape[a] - b) {
+        J[a] = 2 * (shape[a] - b) - 1 - J[a];
+        r[a] = -1.0f;
+      }
+    }
+    if (fluid_at(g, J[0], J[1], J[2])) {
+      load_grad(g, lin(g, J[0], J[1], J[2]), v);
+      // R S(g) R = S(R g) for a diagonal sign matrix R.
+      v[0] *= r[0];
+      v[1] *= r[1];
+      v[2] *= r[2];
+    }
+  }
+  stress_of(g, v[0], v[1], v[2], s);
+}
+
+template <int MODE>
+__device__ void force_at(const icy3d_grid& g, const Cell& c, bool with_flux, float f[3]) {
+  const size_t N = cells(g);
+  const int n = c.n;
+  const float phi = g.phi[n], rho = density(g, phi);
+  const float gy = g.gravity_y;
+  const float tw = water_temperature(g, n);
+  f[0] = 0.0f;
+  f[1] = (rho - g.ref_density[c.j]) * gy;
+  f[2] = 0.0f;
+  f[1] -= fmaxf(0.0f, 2.0f * phi - 1.0f) * g.expansion * (tw - g.water_t0) * gy;
+  if (g.sigma > 0.0f) {
+    bool wall = false;
+    const float kcap = 4.5f * g.sigma * g.width;
+#pragma unroll
+    for (int q = 0; q < 27; ++q) {
+      if (q == 13) continue;
+      int i = c.i + cx(q), j = c.j + cy(q), k = c.k + cz(q);
+      float v[3];
+      bool valid = false;
+      if (MODE == kTank) {
+        const int b = g.boundary;
+        int K[3] = {min(max(i, b), g.nx - b - 1), min(max(j, b), g.ny - b - 1), min(max(k, b), g.nz - b - 1)};
+        load_grad(g, lin(g, K[0], K[1], K[2]), v);
+        if (K[0] != i) v[0] = -v[0];
+        if (K[1] != j) v[1] = -v[1];
+        if (K[2] != k) v[2] = -v[2];
+        valid = true;
+      } else if (link_fluid<MODE>(g, i, j, k)) {
+        load_grad(g, lin(g, i, j, k), v);
+        valid = true;
+      } else {
+        wall = true;
+      }
+      if (valid) {
+        const float gg = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        const float gc = (float)cx(q) * v[0] + (float)cy(q) * v[1] + (float)cz(q) * v[2];
+        const float coef = kcap * w27(q);
+        f[0] += coef * (gg * (float)cx(q) - gc * v[0]);
+        f[1] += coef * (gg * (float)cy(q) - gc * v[1]);
+        f[2] += coef * (gg * (float)cz(q) - gc * v[2]);
+      }
+    }
+    if (MODE == kGeneral && wall) {
+      for (int q = 0; q < 27; ++q) {
+        if (q == 13) continue;
+        int dx = cx(q), dy = cy(q), dz = cz(q);
+        if (fluid_at(g, c.i + dx, c.j + dy, c.k + dz)) continue;
+        float s[9];
+        capillary_neighbor(g, c.i, c.j, c.k, dx, dy, dz, s);
+        const float w3 = 3.0f * w27(q);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) f[a] += w3 * (s[3 * a] * dx + s[3 * a + 1] * dy + s[3 * a + 2] * dz);
+      }
+    }
+  }
+  if (with_flux) {
+    const float ui[3] = {g.u[n], g.u[N + n], g.u[2 * N + n]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int sign = 0; sign < 2; ++sign) {
+        const int d = 2 * sign - 1;
+        int i = c.i + (a == 0) * d, j = c.j + (a == 1) * d, k = c.k + (a == 2) * d;
+        if (!link_fluid<MODE>(g, i, j, k)) continue;
+        const int m = lin(g, i, j, k);
+        const float mass_flux = 0.5f * (g.flux[a * N + n] + g.flux[a * N + m]);
+        const bool own = (float)d * mass_flux >= 0.0f;
+        float fl[3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          float donor = own ? ui[e] : g.u[e * N + m];
+          fl[e] = mass_flux * (donor - ui[e]);
+        }
+        if (g.flux_limit > 0.0f) {
+          float budget = g.flux_limit / 6.0f * fminf(rho, density(g, g.phi[m]));
+          float scale = fminf(1.0f, budget / (fabsf(fl[0]) + fabsf(fl[1]) + fabsf(fl[2]) + 1e-20f));
+          fl[0] *= scale;
+          fl[1] *= scale;
+          fl[2] *= scale;
+        }
+        f[0] -= (float)d * fl[0];
+        f[1] -= (float)d * fl[1];
+        f[2] -= (float)d * fl[2];
+      }
+  }
+}
+
+// simulator.py:559-587: neighbour velocity for the stress derivative.
+__device__ void stress_neighbor_velocity(const icy3d_grid& g, int i, int j, int k, int dx, int dy, int dz, float out[3]) {
+  const size_t N = cells(g);
+  const int n = lin(g, i, j, k);
+  const float centre[3] = {g.u[n], g.u[N + n], g.u[2 * N + n]};
+  out[0] = -centre[0];
+  out[1] = -centre[1];
+  out[2] = -centre[2];
+  int ji = i + dx, jj = j + dy, jk = k + dz;
+  if (fluid_at(g, ji, jj, jk)) {
+    int m = lin(g, ji, jj, jk);
+    out[0] = g.u[m];
+    out[1] = g.u[N + m];
+    out[2] = g.u[2 * N + m];
+  } else if (inside(g, ji, jj, jk) && g.id[lin(g, ji, jj, jk)] >= 0) {
+    int m = lin(g, ji, jj, jk);
+    float eta = link_fraction(g, n, m), ub[3];
+    body_velocity_at(g, i + 0.5f + eta * dx, j + 0.5f + eta * dy, k + 0.5f + eta * dz, g.id[m], ub);
+    if (connected(g, i, j, k, -dx, -dy, -dz)) {
+      int o = lin(g, i - dx, j - dy, k - dz);
+      const float other[3] = {g.u[o], g.u[N + o], g.u[2 * N + o]};
+#pragma unroll
+      for (int a = 0; a < 3; ++a) out[a] = (2.0f * ub[a] + (eta - 1.0f) * other[a]) / (1.0f + eta);
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) out[a] = 2.0f * ub[a] - centre[a];
+    }
+  }
+}
+
+template <int MODE>
+__device__ void acceleration_at(const icy3d_grid& g, const Cell& c, const float f[3], bool moment_pressure, float a[3]) {
+  const size_t N = cells(g);
+  const int n = c.n;
+  const float rho = density(g, g.phi[n]);
+  const float inv_rho = 1.0f / rho;
+  a[0] = f[0] / rho;
+  a[1] = f[1] / rho;
+  a[2] = f[2] / rho;
+#pragma unroll
+  for (int q = 0; q < 27; ++q) {
+    if (q == 13) continue;
+    int i = c.i + cx(q), j = c.j + cy(q), k = c.k + cz(q);
+    if (!link_fluid<MODE>(g, i, j, k)) continue;
+    const int m = lin(g, i, j, k);
+    const float rj = density(g, g.phi[m]);
+    float term;
+    if (moment_pressure)
+      term = w27(q) * g.home[m] * ((rho - rj) / rho);
+    else
+      term = 3.0f * w27(q) * g.pressure[m] * (1.0f / rj - inv_rho);
+    if (cx(q)) a[0] += (float)cx(q) * term;
+    if (cy(q)) a[1] += (float)cy(q) * term;
+    if (cz(q)) a[2] += (float)cz(q) * term;
+  }
+  float gr[3];
+  load_grad(g, n, gr);
+  const float dr = 1.0f - g.rho_air;
+  gr[0] *= dr;
+  gr[1] *= dr;
+  gr[2] *= dr;
+  if (gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2] > 0.0f) {
+    float G[3][3];
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+      const int ex = axis == 0, ey = axis == 1, ez = axis == 2;
+      float d[3];
+      if (MODE == kInterior) {
+        int p = lin(g, c.i + ex, c.j + ey, c.k + ez), m = lin(g, c.i - ex, c.j - ey, c.k - ez);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) d[e] = 0.5f * (g.u[e * N + p] - g.u[e * N + m]);
+      } else if (MODE == kTank) {
+        float left[3], right[3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) left[e] = right[e] = -g.u[e * N + n];
+        if (link_fluid<kTank>(g, c.i - ex, c.j - ey, c.k - ez)) {
+          int m = lin(g, c.i - ex, c.j - ey, c.k - ez);
+#pragma unroll
+          for (int e = 0; e < 3; ++e) left[e] = g.u[e * N + m];
+        }
+        if (link_fluid<kTank>(g, c.i + ex, c.j + ey, c.k + ez)) {
+          int p = lin(g, c.i + ex, c.j + ey, c.k + ez);
+#pragma unroll
+          for (int e = 0; e < 3; ++e) right[e] = g.u[e * N + p];
+        }
+#pragma unroll
+        for (int e = 0; e < 3; ++e) d[e] = 0.5f * (right[e] - left[e]);
+      } else {
+        float plus[3], minus[3];
+        stress_neighbor_velocity(g, c.i, c.j, c.k, ex, ey, ez, plus);
+        stress_neighbor_velocity(g, c.i, c.j, c.k, -ex, -ey, -ez, minus);
+#pragma unroll
+        for (int e = 0; e < 3; ++e) d[e] = 0.5f * (plus[e] - minus[e]);
+      }
+#pragma unroll
+      for (int e = 0; e < 3; ++e) G[e][axis] = d[e];
+    }
+    const float coef = lattice_nu(g, n) / rho;
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+      a[e] += coef * ((G[e][0] + G[0][e]) * gr[0] + (G[e][1] + G[1][e]) * gr[1] + (G[e][2] + G[2][e]) * gr[2]);
+  }
+}
+
+template <int MODE>
+__global__ void k_forces(icy3d_grid g, int with_flux, int moment_pressure) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  const int want = MODE == kInterior ? 0 : (MODE == kTank ? 1 : 2);
+  if (n >= (int)N || g.flag[n] != want) return;
+  const Cell c = cell_of(g, n);
+  float f[3], a[3];
+  force_at<MODE>(g, c, with_flux != 0, f);
+  acceleration_at<MODE>(g, c, f, moment_pressure != 0, a);
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    g.force[e * N + n] = f[e];
+    g.accel[e * N + n] = a[e];
+  }
+}
+
+// simulator.py:684-706: forces and equilibrium at newly exposed cells.
+__global__ void k_refill_forces(icy3d_grid g, Box box) {
+  Cell c;
+  if (!box_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
+  if (!(g.id[c.n] == -1 && g.prev_id[c.n] != -1)) return;
+  const size_t N = cells(g);
+  float f[3], a[3];
+  force_at<kGeneral>(g, c, false, f);
+  acceleration_at<kGeneral>(g, c, f, false, a);
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    g.force[e * N + c.n] = f[e];
+    g.accel[e * N + c.n] = a[e];
+  }
+}
+
+__global__ void k_generic_forces(icy3d_grid g, float* __restrict__ out) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  if (n >= (int)N) return;
+  float f[3] = {0, 0, 0}, a[3] = {0, 0, 0};
+  if (g.id[n] == -1) {
+    const Cell c = cell_of(g, n);
+    force_at<kGeneral>(g, c, true, f);
+    acceleration_at<kGeneral>(g, c, f, true, a);
+  }
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    out[e * N + n] = f[e];
+    out[(3 + e) * N + n] = a[e];
+  }
+}
+
+// home.py:87-100 and phase.py:84-88: equilibrium initialisation.
+__device__ __forceinline__ void set_momentum_equilibrium(const icy3d_grid& g, int n, float phi, float p, float ux, float uy,
+                                                         float uz) {
+  const size_t N = cells(g);
+  const float a[3] = {g.accel[n], g.accel[N + n], g.accel[2 * N + n]};
+  const float u[3] = {ux, uy, uz};
+  Moments m;
+  m.m[0] = 3.0f * p / density(g, phi);
+  m.m[1] = ux - 0.5f * a[0];
+  m.m[2] = uy - 0.5f * a[1];
+  m.m[3] = uz - 0.5f * a[2];
+  const int ia[6] = {0, 1, 2, 0, 0, 1}, ib[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+  for (int t = 0; t < 6; ++t) m.m[4 + t] = u[ia[t]] * u[ib[t]] - 0.5f * (a[ia[t]] * u[ib[t]] + u[ia[t]] * a[ib[t]]);
+  store_moments(g.home, N, n, m);
+  g.u[n] = ux;
+  g.u[N + n] = uy;
+  g.u[2 * N + n] = uz;
+}
+__device__ __forceinline__ void set_phase_equilibrium(const icy3d_grid& g, const Cell& c, int parity, float phi, float ux,
+                                                      float uy, float uz) {
+#pragma unroll
+  for (int q = 0; q < 7; ++q) g.h[phase_addr(g, c, q, parity)] = phase_equilibrium7(q, phi, ux, uy, uz);
+}
+__global__ void k_refill_equilibrium(icy3d_grid g, Box box, int parity) {
+  Cell c;
+  if (!box_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
+  if (!(g.id[c.n] == -1 && g.prev_id[c.n] != -1)) return;
+  const size_t N = cells(g);
+  const float ux = g.u[c.n], uy = g.u[N + c.n], uz = g.u[2 * N + c.n];
+  const float phi = g.phi[c.n];
+  set_momentum_equilibrium(g, c.n, phi, g.pressure[c.n], ux, uy, uz);
+  set_phase_equilibrium(g, c, parity, phi, ux, uy, uz);
+}
+// simulator.py:385-395 (zero velocity, momentum only) and the reference
+// tests' seed kernels (current velocity, optional constant pressure).
+__global__ void k_set_equilibrium(icy3d_grid g, int parity, int with_phase, int zero_velocity, int use_constant,
+                                  float pressure) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  if (n >= (int)N || g.id[n] != -1) return;
+  float ux = 0.0f, uy = 0.0f, uz = 0.0f;
+  if (!zero_velocity) {
+    ux = g.u[n];
+    uy = g.u[N + n];
+    uz = g.u[2 * N + n];
+  }
+  const float phi = g.phi[n];
+  set_momentum_equilibrium(g, n, phi, use_constant ? pressure : g.pressure[n], ux, uy, uz);
+  if (with_phase) set_phase_equilibrium(g, cell_of(g, n), parity, phi, ux, uy, uz);
+}
+
+// ---------------------------------------------------------------------------
+// HOME collision (home.py:199-211).
+__global__ void k_collide_momentum(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  if (n >= (int)N || g.id[n] != -1) return;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) g.old_u[e * N + n] = g.u[e * N + n];
+  g.old_pressure[n] = g.pressure[n];
+  const float rate = 1.0f / (0.5f + 3.0f * lattice_nu(g, n));
+  Moments m = load_moments(g.home, N, n);
+  Moments post = central_collision(m, g.accel[n], g.accel[N + n], g.accel[2 * N + n], rate, g.bulk_rate);
+  store_moments(g.home_post, N, n, post);
+}
+
+// ---------------------------------------------------------------------------
+// HOME pull streaming (home.py:286-321): flags 0/1 and solid velocities.
+struct MomentSums {
+  float p, jx, jy, jz, xx, yy, zz, xy, xz, yz;
+};
+template <int Q>
+__device__ __forceinline__ void accumulate(MomentSums& s, float v) {
+  constexpr int X = cx(Q), Y = cy(Q), Z = cz(Q);
+  s.p += v;
+  if (X) s.jx += (float)X * v;
+  if (Y) s.jy += (float)Y * v;
+  if (Z) s.jz += (float)Z * v;
+  if (X) s.xx += v;
+  if (Y) s.yy += v;
+  if (Z) s.zz += v;
+  if (X && Y) s.xy += (float)(X * Y) * v;
+  if (X && Z) s.xz += (float)(X * Z) * v;
+  if (Y && Z) s.yz += (float)(Y * Z) * v;
+}
+
+template <int Q>
+__device__ __forceinline__ void pull_bulk(const icy3d_grid& g, const Cell& c, const Moments& own, MomentSums& s) {
+  const size_t N = cells(g);
+  const int i = c.i - cx(Q), j = c.j - cy(Q), k = c.k - cz(Q);
+  float v;
+  if (Q != 13 && !fluid_at(g, i, j, k)) {
+    // Stationary halfway bounce-back: R_q(p,-u,S) = R_-q(p,u,S).
+    Moments m = own;
+    m.m[1] = -m.m[1];
+    m.m[2] = -m.m[2];
+    m.m[3] = -m.m[3];
+    v = reconstruct<Q>(m);
+  } else {
+    v = reconstruct<Q>(Q == 13 ? own : load_moments(g.home_post, N, lin(g, i, j, k)));
+  }
+  accumulate<Q>(s, v);
+}
+template <int Q>
+__device__ __forceinline__ void pull_all(const icy3d_grid& g, const Cell& c, const Moments& own, MomentSums& s) {
+  pull_bulk<Q>(g, c, own, s);
+  if constexpr (Q + 1 < 27) pull_all<Q + 1>(g, c, own, s);
+}
+
+__device__ __forceinline__ void store_streamed(const icy3d_grid& g, int n, const MomentSums& s) {
+  const size_t N = cells(g);
+  const float third = s.p * (1.0f / 3.0f);
+  Moments m;
+  m.m[0] = s.p;
+  m.m[1] = s.jx;
+  m.m[2] = s.jy;
+  m.m[3] = s.jz;
+  m.m[4] = s.xx - third;
+  m.m[5] = s.yy - third;
+  m.m[6] = s.zz - third;
+  m.m[7] = s.xy;
+  m.m[8] = s.xz;
+  m.m[9] = s.yz;
+  store_moments(g.home, N, n, m);
+  g.u[n] = s.jx + 0.5f * g.accel[n];
+  g.u[N + n] = s.jy + 0.5f * g.accel[N + n];
+  g.u[2 * N + n] = s.jz + 0.5f * g.accel[2 * N + n];
+}
+
+__global__ void __launch_bounds__(256) k_stream_bulk(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  if (n >= (int)N) return;
+  const int flag = g.flag[n];
+  if (flag == 2) return;
+  const Cell c = cell_of(g, n);
+  if (flag < 0) {
+    float v[3] = {0.0f, 0.0f, 0.0f};
+    int b = g.id[n];
+    if (b >= 0) body_velocity_at(g, c.i + 0.5f, c.j + 0.5f, c.k + 0.5f, b, v);
+    g.u[n] = v[0];
+    g.u[N + n] = v[1];
+    g.u[2 * N + n] = v[2];
+    return;
+  }
+  const Moments own = load_moments(g.home_post, N, n);
+  MomentSums s = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  pull_all<0>(g, c, own, s);
+  store_streamed(g, n, s);
+}
+
+// home.py:213-254: moving-ice reflection; accumulates body force/torque.
+__device__ float home_reflection(const icy3d_grid& g, const Cell& c, int outgoing, const Moments& post_i,
+                                 const Moments& home_i, float load[][6], int* load_body, int& loads) {
+  const size_t N = cells(g);
+  const int dx = cx(outgoing), dy = cy(outgoing), dz = cz(outgoing);
+  const int ji = c.i + dx, jj = c.j + dy, jk = c.k + dz;
+  const float raw = reconstruct_rt(outgoing, post_i);
+  float reflected = raw;
+  if (inside(g, ji, jj, jk)) {
+    const int m = lin(g, ji, jj, jk);
+    const int b = g.id[m];
+    if (b >= 0) {
+      const float eta = link_fraction(g, c.n, m);
+      const float px = c.i + 0.5f + eta * dx, py = c.j + 0.5f + eta * dy, pz = c.k + 0.5f + eta * dz;
+      float ub[3];
+      body_velocity_at(g, px, py, pz, b, ub);
+      const float p = home_i.m[0];
+      const float neq = reconstruct_rt(outgoing, home_i) -
+                        reconstruct_rt(outgoing, equilibrium_moments(p, g.old_u[c.n], g.old_u[N + c.n], g.old_u[2 * N + c.n]));
+      const int incoming = 26 - outgoing;
+      reflected = (reconstruct_rt(incoming, equilibrium_moments(p, ub[0], ub[1], ub[2])) + neq +
+                   eta * reconstruct_rt(incoming, post_i)) /
+                  (1.0f + eta);
+      const float rho = density(g, g.phi[c.n]);
+      const float cf[3] = {(float)dx, (float)dy, (float)dz};
+      float impulse[3];
+      const float pref = (1.0f - eta) * g.ref_pressure[c.j] + eta * g.ref_pressure[jj];
+      const float w = w27(outgoing);
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        impulse[a] = rho * ((cf[a] - ub[a]) * raw - (-cf[a] - ub[a]) * reflected);
+        impulse[a] += 6.0f * w * pref * cf[a];
+      }
+      float own_s[9], nb_s[9], gi[3];
+      load_grad(g, c.n, gi);
+      stress_of(g, gi[0], gi[1], gi[2], own_s);
+      capillary_neighbor(g, c.i, c.j, c.k, dx, dy, dz, nb_s);
+      float cap[3];
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        cap[a] = -3.0f * w *
+                 ((own_s[3 * a] + nb_s[3 * a]) * cf[0] + (own_s[3 * a + 1] + nb_s[3 * a + 1]) * cf[1] +
+                  (own_s[3 * a + 2] + nb_s[3 * a + 2]) * cf[2]);
+      const float* centre = body_row(g, b) + 12;
+      const float r1[3] = {px - centre[0], py - centre[1], pz - centre[2]};
+      const float r2[3] = {c.i + 0.5f + 0.5f * cf[0] - centre[0], c.j + 0.5f + 0.5f * cf[1] - centre[1],
+                           c.k + 0.5f + 0.5f * cf[2] - centre[2]};
+      float t[3] = {r1[1] * impulse[2] - r1[2] * impulse[1], r1[2] * impulse[0] - r1[0] * impulse[2],
+                    r1[0] * impulse[1] - r1[1] * impulse[0]};
+      t[0] += r2[1] * cap[2] - r2[2] * cap[1];
+      t[1] += r2[2] * cap[0] - r2[0] * cap[2];
+      t[2] += r2[0] * cap[1] - r2[1] * cap[0];
+      // Accumulate per body in registers; the kernel flushes with atomics.
+      int slot = -1;
+      for (int s = 0; s < loads; ++s)
+        if (load_body[s] == b) slot = s;
+      if (slot < 0 && loads < 4) {
+        slot = loads++;
+        load_body[slot] = b;
+        for (int e = 0; e < 6; ++e) load[slot][e] = 0.0f;
+      }
+      if (slot >= 0) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          load[slot][a] += impulse[a] + cap[a];
+          load[slot][3 + a] += t[a];
+        }
+      } else {
+        float* dst = g.control->loads[b];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          atomicAdd(dst + a, impulse[a] + cap[a]);
+          atomicAdd(dst + 3 + a, t[a]);
+        }
+      }
+    }
+  }
+  return reflected;
+}
+
+// home.py:323-345: flag-2 cells (ice or unresolved cavity neighbours).
+__global__ void k_stream_boundary(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  if (n >= (int)N || g.flag[n] != 2) return;
+  const Cell c = cell_of(g, n);
+  const Moments post_i = load_moments(g.home_post, N, n);
+  const Moments home_i = load_moments(g.home, N, n);
+  float load[4][6];
+  int load_body[4];
+  int loads = 0;
+  float P = 0, J[3] = {0, 0, 0}, S[6] = {0, 0, 0, 0, 0, 0};
+  for (int q = 0; q < 27; ++q) {
+    const int dx = cx(q), dy = cy(q), dz = cz(q);
+    const int i = c.i - dx, j = c.j - dy, k = c.k - dz;
+    float v;
+    if (fluid_at(g, i, j, k))
+      v = reconstruct_rt(q, q == 13 ? post_i : load_moments(g.home_post, N, lin(g, i, j, k)));
+    else
+      v = home_reflection(g, c, 26 - q, post_i, home_i, load, load_body, loads);
+    P += v;
+    J[0] += dx * v;
+    J[1] += dy * v;
+    J[2] += dz * v;
+    S[0] += (float)(dx * dx) * v;
+    S[1] += (float)(dy * dy) * v;
+    S[2] += (float)(dz * dz) * v;
+    S[3] += (float)(dx * dy) * v;
+    S[4] += (float)(dx * dz) * v;
+    S[5] += (float)(dy * dz) * v;
+  }
+  MomentSums s = {P, J[0], J[1], J[2], S[0], S[1], S[2], S[3], S[4], S[5]};
+  store_streamed(g, n, s);
+  for (int t = 0; t < loads; ++t) {
+    float* dst = g.control->loads[load_body[t]];
+    for (int e = 0; e < 6; ++e) atomicAdd(dst + e, load[t][e]);
+  }
+}
+
+__global__ void k_zero_loads(icy3d_grid g) {
+  int t = threadIdx.x;
+  if (t < ICY3D_MAX_BODIES * 6) (&g.control->loads[0][0])[t] = 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Probes (the reference tests' probe kernels).
+__global__ void k_probe_home(const float* __restrict__ data, int rows, float* __restrict__ pops, float* __restrict__ post) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const float* d = data + 15 * r;
+  Moments m;
+#pragma unroll
+  for (int t = 0; t < 10; ++t) m.m[t] = d[t];
+  for (int q = 0; q < 27; ++q) pops[27 * r + q] = reconstruct_rt(q, m);
+  Moments o = central_collision(m, d[10], d[11], d[12], d[13], d[14]);
+#pragma unroll
+  for (int t = 0; t < 10; ++t) post[10 * r + t] = o.m[t];
+}
+__global__ void k_probe_phase(const float* __restrict__ data, int rows, float mobility, float width, float* __restrict__ out) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const float* d = data + 14 * r;
+  float h[7], post[7], fl[3];
+#pragma unroll
+  for (int q = 0; q < 7; ++q) h[q] = d[q];
+  const float u[3] = {d[8], d[9], d[10]}, nrm[3] = {d[11], d[12], d[13]};
+  phase_collision(h, d[7], u, nrm, mobility, width, post, fl);
+#pragma unroll
+  for (int q = 0; q < 7; ++q) out[10 * r + q] = post[q];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) out[10 * r + 7 + a] = fl[a];
+}
+__global__ void k_probe_viscosity(const float* phi, int n_phi, const float* temps, int n_t, float rho_w, float rho_a,
+                                  float mu_air, float* out) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_phi * n_t) return;
+  int i = r / n_t, t = r % n_t;
+  out[r] = mixture_nu(phi[i], temps[t], rho_w, rho_a, mu_air);
+}
+__global__ void k_probe_stress_velocity(icy3d_grid g, int i, int j, int k, int dx, int dy, int dz, float* out) {
+  if (threadIdx.x != 0) return;
+  stress_neighbor_velocity(g, i, j, k, dx, dy, dz, out);
+}
+__global__ void k_probe_mirror(icy3d_grid g, int i, int j, int k, int dx, int dy, int dz, float* out) {
+  if (threadIdx.x != 0) return;
+  ice_mirror_point(g, i, j, k, i + dx, j + dy, k + dz, out);
+  out[3] = wall_phase_sample(g, i, j, k, dx, dy, dz);
+}
+
+}  // namespace icy
+
+// ===========================================================================
+// C ABI
+using namespace icy;
+
+namespace {
+Box full_box(const icy3d_grid& g) {
+  Box b = {{0, 0, 0}, {g.nx, g.ny, g.nz}};
+  return b;
+}
+Box make_box(const int32_t lo[3], const int32_t hi[3]) {
+  Box b;
+  for (int a = 0; a < 3; ++a) {
+    b.lo[a] = lo[a];
+    b.hi[a] = hi[a];
+  }
+  return b;
+}
+size_t box_cells(const Box& b) {
+  long ex = b.hi[0] - b.lo[0], ey = b.hi[1] - b.lo[1], ez = b.hi[2] - b.lo[2];
+  if (ex <= 0 || ey <= 0 || ez <= 0) return 0;
+  return (size_t)ex * ey * ez;
+}
+size_t grid_cells(const icy3d_grid& g) { return (size_t)g.nx * g.ny * g.nz; }
+bool valid(const icy3d_grid* g) {
+  return g && g->nx > 0 && g->ny > 0 && g->nz > 0 && g->bodies >= 0 && g->bodies <= ICY3D_MAX_BODIES &&
+         grid_cells(*g) < (size_t)INT32_MAX / 16 && g->control;
+}
+
+int launch_classify(const icy3d_grid& g, const Box& b, hipStream_t s) {
+  size_t n = box_cells(b);
+  if (n) k_classify<<<icy_blocks(n), kBlock, 0, s>>>(g, b);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int launch_gradients(const icy3d_grid& g, const Box& b, hipStream_t s) {
+  size_t n = box_cells(b);
+  if (!n) return 0;
+  k_gradients_regular<<<icy_blocks(n), kBlock, 0, s>>>(g, b);
+  k_gradients_general<<<icy_blocks(n), kBlock, 0, s>>>(g, b);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int launch_forces(const icy3d_grid& g, int with_flux, int moment_pressure, hipStream_t s) {
+  const unsigned blocks = icy_blocks(grid_cells(g));
+  k_forces<kInterior><<<blocks, kBlock, 0, s>>>(g, with_flux, moment_pressure);
+  k_forces<kTank><<<blocks, kBlock, 0, s>>>(g, with_flux, moment_pressure);
+  k_forces<kGeneral><<<blocks, kBlock, 0, s>>>(g, with_flux, moment_pressure);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int launch_recover(const icy3d_grid& g, int parity, int recover_flow, hipStream_t s) {
+  bool scan = recover_flow && g.check_stability;
+  if (scan) k_reset_stability<<<1, 1, 0, s>>>(g);
+  k_recover_phase<<<icy_blocks(grid_cells(g)), kBlock, 0, s>>>(g, parity, recover_flow);
+  if (scan) k_fold_peak<<<1, 1, 0, s>>>(g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int launch_stream(const icy3d_grid& g, hipStream_t s) {
+  const unsigned blocks = icy_blocks(grid_cells(g));
+  k_stream_bulk<<<blocks, kBlock, 0, s>>>(g);
+  k_stream_boundary<<<blocks, kBlock, 0, s>>>(g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int icy3d_abi_sizes(int32_t* grid_bytes, int32_t* control_bytes) {
+  *grid_bytes = (int32_t)sizeof(icy3d_grid);
+  *control_bytes = (int32_t)sizeof(icy3d_control);
+  return 0;
+}
+
+int icy3d_classify(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  return launch_classify(*g, lo ? make_box(lo, hi) : full_box(*g), (hipStream_t)stream);
+}
+int icy3d_gradients(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  return launch_gradients(*g, lo ? make_box(lo, hi) : full_box(*g), (hipStream_t)stream);
+}
+int icy3d_refill_gradients(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  Box b = make_box(lo, hi);
+  int rc = launch_classify(*g, b, (hipStream_t)stream);
+  return rc ? rc : launch_gradients(*g, b, (hipStream_t)stream);
+}
+int icy3d_collide_phase(const icy3d_grid* g, int parity, int moving_walls, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  k_collide_phase<<<icy_blocks(grid_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g, parity, moving_walls);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_recover_phase(const icy3d_grid* g, int parity, int recover_flow, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  return launch_recover(*g, parity, recover_flow, (hipStream_t)stream);
+}
+int icy3d_phase_warmup(const icy3d_grid* g, int steps, int* parity, void* stream) {
+  if (!valid(g) || !parity || steps < 0) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  Box b = full_box(*g);
+  const unsigned blocks = icy_blocks(grid_cells(*g));
+  for (int t = 0; t < steps; ++t) {
+    int rc = launch_classify(*g, b, s);
+    if (!rc) rc = launch_gradients(*g, b, s);
+    if (rc) return rc;
+    k_collide_phase<<<blocks, kBlock, 0, s>>>(*g, *parity, 0);
+    *parity = 1 - *parity;
+    rc = launch_recover(*g, *parity, 0, s);
+    if (rc) return rc;
+  }
+  return 0;
+}
+int icy3d_recover_pressure(const icy3d_grid* g, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  k_recover_pressure<<<icy_blocks(grid_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_scan_stability(const icy3d_grid* g, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  k_reset_stability<<<1, 1, 0, s>>>(*g);
+  k_scan_stability<<<icy_blocks(grid_cells(*g)), kBlock, 0, s>>>(*g);
+  k_fold_peak<<<1, 1, 0, s>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_forces(const icy3d_grid* g, int with_flux, int moment_pressure, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = launch_classify(*g, full_box(*g), s);
+  return rc ? rc : launch_forces(*g, with_flux, moment_pressure, s);
+}
+int icy3d_collide_momentum(const icy3d_grid* g, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  k_collide_momentum<<<icy_blocks(grid_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_stream_home(const icy3d_grid* g, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  return launch_stream(*g, (hipStream_t)stream);
+}
+int icy3d_flow_step(const icy3d_grid* g, int parity, int refresh_gradients, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  const icy3d_grid& G = *g;
+  const unsigned blocks = icy_blocks(grid_cells(G));
+  Box b = full_box(G);
+  int rc = 0;
+  if (refresh_gradients) {
+    rc = launch_classify(G, b, s);
+    if (!rc) rc = launch_gradients(G, b, s);
+    if (rc) return rc;
+  }
+  k_collide_phase<<<blocks, kBlock, 0, s>>>(G, parity, 1);
+  parity = 1 - parity;
+  rc = launch_classify(G, b, s);
+  if (!rc) rc = launch_forces(G, 1, 1, s);
+  if (rc) return rc;
+  k_collide_momentum<<<blocks, kBlock, 0, s>>>(G);
+  k_zero_loads<<<1, ICY3D_MAX_BODIES * 6, 0, s>>>(G);
+  rc = launch_stream(G, s);
+  if (!rc) rc = launch_recover(G, parity, 1, s);
+  if (!rc) rc = launch_gradients(G, b, s);
+  return rc;
+}
+int icy3d_repair_phase_layout(const icy3d_grid* g, int parity, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  if (!parity) return 0;
+  k_repair_layout<<<icy_blocks(grid_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_export_phase(const icy3d_grid* g, int parity, float* out, void* stream) {
+  if (!valid(g) || !out) return ICY3D_ERR_ARGUMENT;
+  k_export_phase<<<icy_blocks(grid_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g, parity, out);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_project_refill(const icy3d_grid* g, int parity, const int32_t lo[3], const int32_t hi[3], void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  Box b = make_box(lo, hi);
+  size_t n = box_cells(b);
+  if (!n) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  k_refill_forces<<<icy_blocks(n), kBlock, 0, s>>>(*g, b);
+  k_refill_equilibrium<<<icy_blocks(n), kBlock, 0, s>>>(*g, b, parity);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_seed_populations(const icy3d_grid* g, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  k_set_equilibrium<<<icy_blocks(grid_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g, 0, 0, 1, 0, 0.0f);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_set_equilibrium_all(const icy3d_grid* g, int parity, int with_phase, int use_constant_pressure,
+                              float pressure, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  k_set_equilibrium<<<icy_blocks(grid_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g, parity, with_phase, 0,
+                                                                                   use_constant_pressure, pressure);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_capillary_stress(const icy3d_grid* g, float* out, void* stream) {
+  if (!valid(g) || !out) return ICY3D_ERR_ARGUMENT;
+  k_capillary_stress<<<icy_blocks(grid_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g, out);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_generic_forces(const icy3d_grid* g, float* out, void* stream) {
+  if (!valid(g) || !out) return ICY3D_ERR_ARGUMENT;
+  k_generic_forces<<<icy_blocks(grid_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g, out);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_write_gradient_all(const icy3d_grid* g, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  k_write_gradient_all<<<icy_blocks(grid_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_probe_home(const float* data, int rows, float* populations, float* collided, void* stream) {
+  if (!data || rows < 0 || !populations || !collided) return ICY3D_ERR_ARGUMENT;
+  if (rows) k_probe_home<<<icy_blocks(rows), kBlock, 0, (hipStream_t)stream>>>(data, rows, populations, collided);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_probe_phase(const float* data, int rows, float mobility, float width, float* out, void* stream) {
+  if (!data || rows < 0 || !out) return ICY3D_ERR_ARGUMENT;
+  if (rows) k_probe_phase<<<icy_blocks(rows), kBlock, 0, (hipStream_t)stream>>>(data, rows, mobility, width, out);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_probe_viscosity(const float* phi, int n_phi, const float* temps, int n_t, float rho_w, float rho_a,
+                          float mu_air, float* out, void* stream) {
+  if (!phi || !temps || !out || n_phi < 0 || n_t < 0) return ICY3D_ERR_ARGUMENT;
+  int total = n_phi * n_t;
+  if (total)
+    k_probe_viscosity<<<icy_blocks(total), kBlock, 0, (hipStream_t)stream>>>(phi, n_phi, temps, n_t, rho_w, rho_a,
+                                                                             mu_air, out);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_probe_stress_velocity(const icy3d_grid* g, const int32_t cell[3], const int32_t c[3], float* out,
+                                void* stream) {
+  if (!valid(g) || !out) return ICY3D_ERR_ARGUMENT;
+  // The probe reads the cell and its two link neighbours: keep all inside.
+  for (int a = 0; a < 3; ++a) {
+    int extent = a == 0 ? g->nx : (a == 1 ? g->ny : g->nz);
+    if (cell[a] - 1 < 0 || cell[a] + 1 >= extent || c[a] < -1 || c[a] > 1) return ICY3D_ERR_ARGUMENT;
+  }
+  k_probe_stress_velocity<<<1, 64, 0, (hipStream_t)stream>>>(*g, cell[0], cell[1], cell[2], c[0], c[1], c[2], out);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_probe_mirror(const icy3d_grid* g, const int32_t cell[3], const int32_t c[3], float* out, void* stream) {
+  if (!valid(g) || !out) return ICY3D_ERR_ARGUMENT;
+  for (int a = 0; a < 3; ++a) {
+    int extent = a == 0 ? g->nx : (a == 1 ? g->ny : g->nz);
+    if (cell[a] - 1 < 0 || cell[a] + 1 >= extent || c[a] < -1 || c[a] > 1) return ICY3D_ERR_ARGUMENT;
+  }
+  k_probe_mirror<<<1, 64, 0, (hipStream_t)stream>>>(*g, cell[0], cell[1], cell[2], c[0], c[1], c[2], out);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,349 @@
+// Moving-ice geometry: trilinear material coverage rasterisation, change
+// detection and the Jacobi refill of newly exposed fluid nodes
+// (iceflow3d/simulator.py:288-323, 589-678).
+//
+// Bounding boxes and counters are reduced per wave64 first, so each wave
+// issues at most one atomic per quantity.
+#include "icy3d_device.h"
+
+namespace icy {
+
+struct GBox {
+  int lo[3], hi[3];
+};
+__device__ __forceinline__ bool gbox_cell(const icy3d_grid& g, const GBox& b, int t, Cell& c) {
+  int ex = b.hi[0] - b.lo[0], ey = b.hi[1] - b.lo[1], ez = b.hi[2] - b.lo[2];
+  if (ex <= 0 || ey <= 0 || ez <= 0 || t >= ex * ey * ez) return false;
+  c.k = b.lo[2] + t % ez;
+  int r = t / ez;
+  c.j = b.lo[1] + r % ey;
+  c.i = b.lo[0] + r / ey;
+  c.n = lin(g, c.i, c.j, c.k);
+  return true;
+}
+
+// simulator.py:233-251: trilinear material coverage of body b at a point.
+__device__ float body_fraction(const icy3d_grid& g, float px, float py, float pz, int b) {
+  float grid[3];
+  material_grid(g, px, py, pz, b, grid);
+  const int* size = g.material_size + 3 * b;
+  // Outside [-1, size] on any axis no trilinear corner is inside the body.
+  if (grid[0] <= -1.0f || grid[1] <= -1.0f || grid[2] <= -1.0f || grid[0] >= (float)size[0] ||
+      grid[1] >= (float)size[1] || grid[2] >= (float)size[2])
+    return 0.0f;
+  int base[3];
+  float part[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    base[a] = (int)floorf(grid[a]);
+    part[a] = grid[a] - (float)base[a];
+  }
+  float value = 0.0f;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    int a = t >> 2, c = (t >> 1) & 1, d = t & 1;
+    int i = base[0] + a, j = base[1] + c, k = base[2] + d;
+    if (!mat_inside(g, b, i, j, k)) continue;
+    float w = (a ? part[0] : 1.0f - part[0]) * (c ? part[1] : 1.0f - part[1]) * (d ? part[2] : 1.0f - part[2]);
+    value += w * (g.material_mass[mat_index(g, b, i, j, k)] / g.initial_cell_mass);
+  }
+  return value;
+}
+
+__device__ __forceinline__ void reduce_bounds(int32_t* lower, int32_t* upper, bool hit, const Cell& c) {
+  const int big = INT32_MAX, small = INT32_MIN;
+  int lo[3] = {hit ? c.i : big, hit ? c.j : big, hit ? c.k : big};
+  int hi[3] = {hit ? c.i : small, hit ? c.j : small, hit ? c.k : small};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = wave_min_int(lo[a]);
+    hi[a] = wave_max_int(hi[a]);
+  }
+  if (wave_leader() && lo[0] != big) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(lower + a, lo[a]);
+      atomicMax(upper + a, hi[a]);
+    }
+  }
+}
+
+__global__ void k_reset_geometry_bounds(icy3d_grid g) {
+  const int shape[3] = {g.nx, g.ny, g.nz};
+  for (int a = 0; a < 3; ++a) {
+    g.control->geometry_lower[a] = shape[a];
+    g.control->geometry_upper[a] = -1;
+  }
+}
+
+// simulator.py:288-323
+__global__ void k_rasterize(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  bool active = n < (int)cells(g);
+  Cell c = {0, 0, 0, 0};
+  bool changed = false;
+  if (active) {
+    c = cell_of(g, n);
+    const int b = g.boundary;
+    const int old_id = g.id[n];
+    g.prev_id[n] = (int8_t)old_id;
+    bool wall = c.i < b || c.i >= g.nx - b || c.j < b || c.j >= g.ny - b || c.k < b || c.k >= g.nz - b;
+    float fraction = 0.0f;
+    int owner = -1;
+    if (!wall)
+      for (int body = 0; body < g.bodies; ++body) {
+        float f = body_fraction(g, c.i + 0.5f, c.j + 0.5f, c.k + 0.5f, body);
+        if (f > fraction) {
+          fraction = f;
+          owner = body;
+        }
+      }
+    int new_id = wall ? -2 : (fraction < g.threshold ? -1 : owner);
+    changed = fraction != g.coverage[n] || new_id != old_id;
+    g.coverage[n] = fraction;
+    g.cov_id[n] = (int8_t)owner;
+    g.id[n] = (int8_t)new_id;
+  }
+  reduce_bounds(g.control->geometry_lower, g.control->geometry_upper, active && changed, c);
+}
+
+__global__ void k_reset_refill_counts(icy3d_grid g) {
+  icy3d_control* ctl = g.control;
+  const int shape[3] = {g.nx, g.ny, g.nz};
+  ctl->openings = 0;
+  ctl->covered = 0;
+  for (int a = 0; a < 3; ++a) {
+    ctl->refill_lower[a] = shape[a];
+    ctl->refill_upper[a] = -1;
+  }
+}
+
+// simulator.py:595-613
+__global__ void k_find_refill_changes(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  bool active = n < (int)cells(g);
+  bool opening = false, covered = false;
+  Cell c = {0, 0, 0, 0};
+  if (active) {
+    c = cell_of(g, n);
+    const bool is_fluid = g.id[n] == -1;
+    const bool was_fluid = g.prev_id[n] == -1;
+    opening = is_fluid && !was_fluid;
+    covered = was_fluid && !is_fluid;
+  }
+  int opens = wave_sum_int(opening ? 1 : 0), covers = wave_sum_int(covered ? 1 : 0);
+  if (wave_leader()) {
+    if (opens) atomicAdd(&g.control->openings, opens);
+    if (covers) atomicAdd(&g.control->covered, covers);
+  }
+  reduce_bounds(g.control->refill_lower, g.control->refill_upper, opening || covered, c);
+}
+
+// simulator.py:615-629
+__global__ void k_seed_refill(icy3d_grid g, GBox box) {
+  Cell c;
+  if (!gbox_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
+  const size_t N = cells(g);
+  const int n = c.n;
+  g.refill_state[n] = g.phi[n];
+  g.refill_state[N + n] = g.pressure[n];
+  g.refill_state[2 * N + n] = g.u[n];
+  g.refill_state[3 * N + n] = g.u[N + n];
+  g.refill_state[4 * N + n] = g.u[2 * N + n];
+  g.refill_valid[n] = (int8_t)(g.id[n] == -1 && g.prev_id[n] == -1);
+}
+
+// simulator.py:193-215 (duplicated here to keep translation units independent)
+__device__ bool connected_link(const icy3d_grid& g, int i, int j, int k, int dx, int dy, int dz) {
+  if (!fluid_at(g, i + dx, j + dy, k + dz)) return false;
+  int distance = abs(dx) + abs(dy) + abs(dz);
+  if (distance <= 1) return true;
+  const int c[3] = {dx, dy, dz};
+  bool valid = false;
+  for (int a = 0; a < 3; ++a) {
+    if (c[a] == 0) continue;
+    int A[3] = {i, j, k};
+    A[a] += c[a];
+    if (!fluid_at(g, A[0], A[1], A[2])) continue;
+    if (distance == 2) {
+      valid = true;
+    } else {
+      for (int b = 0; b < 3; ++b) {
+        if (b == a) continue;
+        int B[3] = {A[0], A[1], A[2]};
+        B[b] += c[b];
+        if (fluid_at(g, B[0], B[1], B[2])) valid = true;
+      }
+    }
+  }
+  return valid;
+}
+
+// simulator.py:631-651 (Jacobi sweep into refill_next)
+__global__ void k_propagate_refill(icy3d_grid g, GBox box) {
+  Cell c;
+  bool active = gbox_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c);
+  bool unresolved = false;
+  if (active) {
+    const size_t N = cells(g);
+    const int n = c.n;
+    int valid = g.refill_valid[n];
+    float value[5];
+#pragma unroll
+    for (int t = 0; t < 5; ++t) value[t] = g.refill_state[t * N + n];
+    if (g.id[n] == -1 && valid == 0) {
+      float total[5] = {0, 0, 0, 0, 0}, count = 0.0f;
+      for (int q = 0; q < 27; ++q) {
+        if (q == 13) continue;
+        const int dx = cx(q), dy = cy(q), dz = cz(q);
+        if (!connected_link(g, c.i, c.j, c.k, dx, dy, dz)) continue;
+        const int m = lin(g, c.i + dx, c.j + dy, c.k + dz);
+        if (!g.refill_valid[m]) continue;
+        const float w = w27(q);
+#pragma unroll
+        for (int t = 0; t < 5; ++t) total[t] += w * g.refill_state[t * N + m];
+        count += w;
+      }
+      if (count > 0.0f) {
+#pragma unroll
+        for (int t = 0; t < 5; ++t) value[t] = total[t] / count;
+        valid = 1;
+      } else {
+        unresolved = true;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 5; ++t) g.refill_next[t * N + n] = value[t];
+    g.refill_next_valid[n] = (int8_t)valid;
+  }
+  int left = wave_sum_int(unresolved ? 1 : 0);
+  if (wave_leader() && left) atomicAdd(&g.control->remaining, left);
+}
+__global__ void k_reset_remaining(icy3d_grid g) { g.control->remaining = 0; }
+
+// simulator.py:653-660
+__global__ void k_accept_refill(icy3d_grid g, GBox box) {
+  Cell c;
+  if (!gbox_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
+  const size_t N = cells(g);
+#pragma unroll
+  for (int t = 0; t < 5; ++t) g.refill_state[t * N + c.n] = g.refill_next[t * N + c.n];
+  g.refill_valid[c.n] = g.refill_next_valid[c.n];
+}
+
+// simulator.py:662-678
+__global__ void k_commit_refill(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  if (n >= (int)N) return;
+  int id = g.id[n];
+  if (id == -1 && g.prev_id[n] != -1) {
+    if (g.refill_valid[n]) {
+      g.phi[n] = g.refill_state[n];
+      g.pressure[n] = g.refill_state[N + n];
+      g.u[n] = g.refill_state[2 * N + n];
+      g.u[N + n] = g.refill_state[3 * N + n];
+      g.u[2 * N + n] = g.refill_state[4 * N + n];
+    } else {
+      id = -3;
+      g.id[n] = -3;
+    }
+  }
+  if (id != -1) {
+    float v[3] = {0.0f, 0.0f, 0.0f};
+    if (id >= 0) {
+      const Cell c = cell_of(g, n);
+      body_velocity_at(g, c.i + 0.5f, c.j + 0.5f, c.k + 0.5f, id, v);
+    }
+    g.phi[n] = 0.0f;
+    g.u[n] = v[0];
+    g.u[N + n] = v[1];
+    g.u[2 * N + n] = v[2];
+  }
+}
+
+}  // namespace icy
+
+using namespace icy;
+
+namespace {
+bool valid_grid(const icy3d_grid* g) {
+  return g && g->nx > 0 && g->ny > 0 && g->nz > 0 && g->bodies >= 0 && g->bodies <= ICY3D_MAX_BODIES && g->control &&
+         (size_t)g->nx * g->ny * g->nz < (size_t)INT32_MAX / 16;
+}
+size_t all_cells(const icy3d_grid& g) { return (size_t)g.nx * g.ny * g.nz; }
+GBox make_gbox(const int32_t lo[3], const int32_t hi[3]) {
+  GBox b;
+  for (int a = 0; a < 3; ++a) {
+    b.lo[a] = lo[a];
+    b.hi[a] = hi[a];
+  }
+  return b;
+}
+size_t gbox_cells(const GBox& b) {
+  long ex = b.hi[0] - b.lo[0], ey = b.hi[1] - b.lo[1], ez = b.hi[2] - b.lo[2];
+  return (ex <= 0 || ey <= 0 || ez <= 0) ? 0 : (size_t)ex * ey * ez;
+}
+bool gbox_inside(const icy3d_grid& g, const GBox& b) {
+  const int shape[3] = {g.nx, g.ny, g.nz};
+  for (int a = 0; a < 3; ++a)
+    if (b.lo[a] < 0 || b.hi[a] > shape[a]) return false;
+  return true;
+}
+}  // namespace
+
+extern "C" {
+
+int icy3d_rasterize(const icy3d_grid* g, void* stream) {
+  if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  k_reset_geometry_bounds<<<1, 1, 0, s>>>(*g);
+  k_rasterize<<<icy_blocks(all_cells(*g)), kBlock, 0, s>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_find_refill_changes(const icy3d_grid* g, void* stream) {
+  if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  k_reset_refill_counts<<<1, 1, 0, s>>>(*g);
+  k_find_refill_changes<<<icy_blocks(all_cells(*g)), kBlock, 0, s>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_seed_refill_region(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
+  if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
+  GBox b = make_gbox(lo, hi);
+  if (!gbox_inside(*g, b)) return ICY3D_ERR_ARGUMENT;
+  size_t n = gbox_cells(b);
+  if (n) k_seed_refill<<<icy_blocks(n), kBlock, 0, (hipStream_t)stream>>>(*g, b);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_propagate_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
+  if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
+  GBox b = make_gbox(lo, hi);
+  if (!gbox_inside(*g, b)) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  k_reset_remaining<<<1, 1, 0, s>>>(*g);
+  size_t n = gbox_cells(b);
+  if (n) k_propagate_refill<<<icy_blocks(n), kBlock, 0, s>>>(*g, b);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_accept_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
+  if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
+  GBox b = make_gbox(lo, hi);
+  if (!gbox_inside(*g, b)) return ICY3D_ERR_ARGUMENT;
+  size_t n = gbox_cells(b);
+  if (n) k_accept_refill<<<icy_blocks(n), kBlock, 0, (hipStream_t)stream>>>(*g, b);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_commit_refill(const icy3d_grid* g, void* stream) {
+  if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
+  k_commit_refill<<<icy_blocks(all_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // extern "C"

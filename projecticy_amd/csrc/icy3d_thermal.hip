@@ -1,0 +1,524 @@
+// Finite-volume heat transport and disappearing ice (iceflow3d/thermal.py),
+// material-grid mass reductions (simulator.py:738-776) and initialisation.
+//
+// Water stores volume [m^3] and sensible energy [J] per world cell; ice
+// stores mass [kg] and sensible energy [J] per material cell.  Scalar sums
+// (remap energy, discarded heat) are reduced per wave64 before one atomic.
+#include "icy3d_device.h"
+
+namespace icy {
+
+__device__ __forceinline__ float water_target(const icy3d_grid& g, int n) {
+  return (g.id[n] == -1 && g.phi[n] > 1e-3f) ? g.phi[n] * g.cell_volume : 0.0f;
+}
+
+// thermal.py:98-118
+__global__ void k_build_water_targets(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int)cells(g)) return;
+  const float target = water_target(g, n);
+  float specific = 0.0f;
+  if (target > 0.0f) {
+    const float v = g.water_volume[n];
+    if (v > 1e-30f) {
+      specific = g.water_energy[n] / v;
+    } else {
+      const Cell c = cell_of(g, n);
+      float total = 0.0f;
+      for (int a = -1; a <= 1; ++a)
+        for (int b = -1; b <= 1; ++b)
+          for (int d = -1; d <= 1; ++d) {
+            int i = c.i + a, j = c.j + b, k = c.k + d;
+            if (!inside(g, i, j, k)) continue;
+            int m = lin(g, i, j, k);
+            if (g.water_volume[m] > 1e-30f) {
+              specific += g.water_energy[m];
+              total += g.water_volume[m];
+            }
+          }
+      if (total > 1e-30f) specific /= total;
+    }
+  }
+  g.target_energy[n] = target * specific;
+}
+// thermal.py:120-127
+__global__ void k_commit_water_targets(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  float delta = 0.0f;
+  if (n < (int)cells(g)) {
+    const float e = g.target_energy[n];
+    delta = e - g.water_energy[n];
+    g.water_volume[n] = water_target(g, n);
+    g.water_energy[n] = e;
+  }
+  delta = wave_sum(delta);
+  if (wave_leader() && delta != 0.0f) atomicAdd(&g.control->remap_energy, delta);
+}
+// thermal.py:88-96
+__global__ void k_initialize_water_heat(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int)cells(g)) return;
+  const float v = water_target(g, n);
+  g.water_volume[n] = v;
+  g.water_energy[n] = v * g.rho_cp * (g.water_t0 - g.tm);
+}
+
+// thermal.py:133-169
+__global__ void k_thermal_fluxes(icy3d_grid g, int advect) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  if (n >= (int)N) return;
+  const Cell c = cell_of(g, n);
+  float ef[3] = {0.0f, 0.0f, 0.0f}, vf[3] = {0.0f, 0.0f, 0.0f};
+  if (g.id[n] == -1) {
+    const float vi = g.water_volume[n];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      int i = c.i + (a == 0), j = c.j + (a == 1), k = c.k + (a == 2);
+      if (!fluid_at(g, i, j, k)) continue;
+      const int m = lin(g, i, j, k);
+      const float vj = g.water_volume[m];
+      if (!(vi > 1e-30f && vj > 1e-30f)) continue;
+      if (advect) {
+        const float speed = 0.5f * (g.u[a * N + n] + g.u[a * N + m]) * g.velocity_m_s;
+        float V = vi, E = g.water_energy[n];
+        if (speed < 0.0f) {
+          V = vj;
+          E = g.water_energy[m];
+        }
+        vf[a] = speed * fminf(1.0f, V / g.cell_volume) * (g.dx * g.dx);
+        ef[a] = vf[a] * E / V;
+      } else {
+        const float aperture = fminf(vi, vj) / g.cell_volume;
+        ef[a] = g.k_water * g.dx * aperture * (water_temperature(g, n) - water_temperature(g, m));
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    g.energy_flux[a * N + n] = ef[a];
+    g.volume_flux[a * N + n] = vf[a];
+  }
+}
+// thermal.py:171-184
+__global__ void k_apply_thermal_fluxes(icy3d_grid g, float dt, int advect) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  if (n >= (int)N) return;
+  const Cell c = cell_of(g, n);
+  float de = 0.0f, dv = 0.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    de += g.energy_flux[a * N + n];
+    dv += g.volume_flux[a * N + n];
+    int i = c.i - (a == 0), j = c.j - (a == 1), k = c.k - (a == 2);
+    if (inside(g, i, j, k)) {
+      const int m = lin(g, i, j, k);
+      de -= g.energy_flux[a * N + m];
+      dv -= g.volume_flux[a * N + m];
+    }
+  }
+  g.water_energy[n] -= dt * de;
+  if (advect) g.water_volume[n] -= dt * dv;
+}
+// thermal.py:186-206
+__global__ void k_measure_outflow(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  float rate = 0.0f;
+  if (n < (int)N && g.id[n] == -1 && g.water_volume[n] > 1e-30f) {
+    const Cell c = cell_of(g, n);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int sign = 0; sign < 2; ++sign) {
+        const int d = 2 * sign - 1;
+        int i = c.i + (a == 0) * d, j = c.j + (a == 1) * d, k = c.k + (a == 2) * d;
+        if (!fluid_at(g, i, j, k)) continue;
+        const int m = lin(g, i, j, k);
+        if (g.water_volume[m] > 1e-30f) rate += fmaxf(0.0f, (float)d * 0.5f * (g.u[a * N + n] + g.u[a * N + m]));
+      }
+  }
+  if (isnan(rate)) rate = 0.0f;
+  rate = wave_max(rate);
+  if (wave_leader() && rate > 0.0f) atomic_max_nonneg(&g.control->max_outflow, rate);
+}
+__global__ void k_reset_outflow(icy3d_grid g) { g.control->max_outflow = 0.0f; }
+
+// thermal.py:53-62
+__global__ void k_initialize_material(icy3d_grid g) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)g.bodies * g.mx * g.my * g.mz;
+  if (t >= total) return;
+  int k = t % g.mz, j = (t / g.mz) % g.my, i = (t / ((size_t)g.mz * g.my)) % g.mx;
+  int b = t / ((size_t)g.mz * g.my * g.mx);
+  if (!mat_inside(g, b, i, j, k)) return;
+  g.material_mass[t] = g.initial_cell_mass;
+  g.material_energy[t] = g.initial_cell_mass * g.cp_ice * (g.body_t0[b] - g.tm);
+}
+// thermal.py:260-281
+__global__ void k_body_conduction(icy3d_grid g, float dt) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)g.bodies * g.mx * g.my * g.mz;
+  if (t >= total) return;
+  int k = t % g.mz, j = (t / g.mz) % g.my, i = (t / ((size_t)g.mz * g.my)) % g.mx;
+  int b = t / ((size_t)g.mz * g.my * g.mx);
+  const float mass = g.material_mass[t];
+  float heat = 0.0f;
+  if (mass > 0.0f) {
+    const float temp = ice_temperature(g, b, i, j, k);
+    const int faces[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      int fi = i + faces[f][0], fj = j + faces[f][1], fk = k + faces[f][2];
+      if (!mat_inside(g, b, fi, fj, fk)) continue;
+      const float other = g.material_mass[mat_index(g, b, fi, fj, fk)];
+      if (other > 0.0f) {
+        const float aperture = fminf(mass, other) / g.initial_cell_mass;
+        heat += dt * g.k_ice * g.dx * aperture * (ice_temperature(g, b, fi, fj, fk) - temp);
+      }
+    }
+  }
+  g.material_heat[t] = heat;
+}
+
+// thermal.py:283-293
+__device__ __forceinline__ bool touches_thermal_ice(const icy3d_grid& g, const Cell& c) {
+  const int offs[7][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}, {0, 0, 0}};
+#pragma unroll
+  for (int f = 0; f < 7; ++f) {
+    int i = c.i + offs[f][0], j = c.j + offs[f][1], k = c.k + offs[f][2];
+    if (!inside(g, i, j, k)) continue;
+    int m = lin(g, i, j, k);
+    if (g.cov_id[m] >= 0 && g.coverage[m] > 1e-8f) return true;
+  }
+  return false;
+}
+
+// thermal.py:308-392: seven world samples x eight trilinear material donors.
+__device__ void exchange_heat_cell(const icy3d_grid& g, const Cell& c, float dt) {
+  const int n = c.n;
+  if (!(g.water_volume[n] > 1e-30f && g.id[n] == -1 && touches_thermal_ice(g, c))) return;
+  const int offs[7][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}, {0, 0, 0}};
+  float heat[56];
+  int body[7], base[7][3];
+  float requested = 0.0f;
+  const float tw = water_temperature(g, n);
+  const float aperture = g.water_volume[n] / g.cell_volume;
+  const float kh = 2.0f * g.k_water * g.k_ice / (g.k_water + g.k_ice);
+  for (int f = 0; f < 7; ++f) {
+    body[f] = 0;
+    base[f][0] = base[f][1] = base[f][2] = 0;
+    for (int s = 0; s < 8; ++s) heat[8 * f + s] = 0.0f;
+    int ji = c.i + offs[f][0], jj = c.j + offs[f][1], jk = c.k + offs[f][2];
+    if (!inside(g, ji, jj, jk)) continue;
+    const int m = lin(g, ji, jj, jk);
+    const int b = g.cov_id[m];
+    const float fraction = g.coverage[m];
+    if (!(b >= 0 && fraction > 1e-8f)) continue;
+    float grid[3], part[3];
+    material_grid(g, ji + 0.5f, jj + 0.5f, jk + 0.5f, b, grid);
+    for (int a = 0; a < 3; ++a) {
+      base[f][a] = (int)floorf(grid[a]);
+      part[a] = grid[a] - (float)base[f][a];
+    }
+    body[f] = b;
+    const float contact = fminf(aperture, fminf(1.0f, fraction / g.threshold));
+    for (int s = 0; s < 8; ++s) {
+      int a = s >> 2, cc = (s >> 1) & 1, d = s & 1;
+      int ki = base[f][0] + a, kj = base[f][1] + cc, kk = base[f][2] + d;
+      if (!mat_inside(g, b, ki, kj, kk)) continue;
+      const float mass = g.material_mass[mat_index(g, b, ki, kj, kk)];
+      if (!(mass > 0.0f)) continue;
+      const float w = (a ? part[0] : 1.0f - part[0]) * (cc ? part[1] : 1.0f - part[1]) * (d ? part[2] : 1.0f - part[2]);
+      const float share = w * mass / (g.initial_cell_mass * fraction);
+      const float value = dt * kh * g.dx * contact * share * fmaxf(0.0f, tw - ice_temperature(g, b, ki, kj, kk));
+      heat[8 * f + s] = value;
+      requested += value;
+    }
+  }
+  const float scale = fminf(1.0f, fmaxf(0.0f, g.water_energy[n]) / fmaxf(requested, 1e-30f));
+  for (int f = 0; f < 7; ++f)
+    for (int s = 0; s < 8; ++s) {
+      const float value = scale * heat[8 * f + s];
+      if (value > 0.0f) {
+        int a = s >> 2, cc = (s >> 1) & 1, d = s & 1;
+        atomicAdd(&g.material_heat[mat_index(g, body[f], base[f][0] + a, base[f][1] + cc, base[f][2] + d)], value);
+      }
+    }
+  g.water_energy[n] -= scale * requested;
+}
+
+__global__ void k_exchange_heat(icy3d_grid g, int lo0, int lo1, int lo2, int ex, int ey, int ez, float dt) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ex * ey * ez) return;
+  Cell c;
+  c.k = lo2 + t % ez;
+  c.j = lo1 + (t / ez) % ey;
+  c.i = lo0 + t / (ez * ey);
+  c.n = lin(g, c.i, c.j, c.k);
+  exchange_heat_cell(g, c, dt);
+}
+
+// thermal.py:401-415
+__global__ void k_apply_melting(icy3d_grid g) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t total = (size_t)g.bodies * g.mx * g.my * g.mz;
+  float discarded = 0.0f;
+  if (t < total) {
+    float mass = g.material_mass[t];
+    float energy = g.material_energy[t] + g.material_heat[t];
+    if (energy > 0.0f) {
+      const float melted = fminf(mass, energy / g.latent);
+      discarded += energy - melted * g.latent;
+      mass -= melted;
+      energy = 0.0f;
+    }
+    if (mass <= 1e-6f * g.initial_cell_mass) {
+      discarded += energy - mass * g.latent;
+      mass = 0.0f;
+      energy = 0.0f;
+    }
+    g.material_mass[t] = mass;
+    g.material_energy[t] = energy;
+  }
+  discarded = wave_sum(discarded);
+  if (wave_leader() && discarded != 0.0f) atomicAdd(&g.control->discarded_heat, discarded);
+}
+
+// simulator.py:738-776: one thread per material column, FP32 sums.
+__global__ void k_reset_reduced(icy3d_grid g) {
+  int t = threadIdx.x;
+  if (t >= g.bodies) return;
+  float* r = g.control->reduced[t];
+  for (int e = 0; e < 13; ++e) r[e] = 0.0f;
+  for (int a = 0; a < 3; ++a) {
+    r[13 + a] = 1e9f;
+    r[16 + a] = -1e9f;
+  }
+}
+// Non-negative-or-any float min/max through CAS (bounds may be negative).
+__device__ __forceinline__ void atomic_min_float(float* address, float value) {
+  int* a = reinterpret_cast<int*>(address);
+  int old = *a;
+  while (__int_as_float(old) > value) {
+    int assumed = old;
+    old = atomicCAS(a, assumed, __float_as_int(value));
+    if (old == assumed) break;
+  }
+}
+__device__ __forceinline__ void atomic_max_float(float* address, float value) {
+  int* a = reinterpret_cast<int*>(address);
+  int old = *a;
+  while (__int_as_float(old) < value) {
+    int assumed = old;
+    old = atomicCAS(a, assumed, __float_as_int(value));
+    if (old == assumed) break;
+  }
+}
+__global__ void k_reduce_mass(icy3d_grid g) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.bodies * g.mx * g.my) return;
+  const int j = t % g.my, i = (t / g.my) % g.mx, b = t / (g.my * g.mx);
+  const int* size = g.material_size + 3 * b;
+  float total = 0.0f, first[3] = {0, 0, 0}, second[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  float lower[3] = {1e9f, 1e9f, 1e9f}, upper[3] = {-1e9f, -1e9f, -1e9f};
+  for (int k = 0; k < g.mz; ++k) {
+    const float mass = g.material_mass[mat_index(g, b, i, j, k)] / g.mass_kg;
+    if (!(mass > 0.0f)) continue;
+    const float p[3] = {(i + 0.5f) - 0.5f * size[0], (j + 0.5f) - 0.5f * size[1], (k + 0.5f) - 0.5f * size[2]};
+    const float pp = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+    total += mass;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) first[a] += mass * p[a];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) second[3 * a + c] += mass * ((a == c ? pp + 1.0f / 6.0f : 0.0f) - p[a] * p[c]);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      lower[a] = fminf(lower[a], p[a] - 0.5f);
+      upper[a] = fmaxf(upper[a], p[a] + 0.5f);
+    }
+  }
+  if (total > 0.0f) {
+    float* r = g.control->reduced[b];
+    atomicAdd(r, total);
+    for (int a = 0; a < 3; ++a) atomicAdd(r + 1 + a, first[a]);
+    for (int e = 0; e < 9; ++e) atomicAdd(r + 4 + e, second[e]);
+    for (int a = 0; a < 3; ++a) {
+      atomic_min_float(r + 13 + a, lower[a]);
+      atomic_max_float(r + 16 + a, upper[a]);
+    }
+  }
+}
+
+// thermal.py:417-428
+__global__ void k_sample_ice_temperature(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int)cells(g)) return;
+  float t = g.tm;
+  const int b = g.id[n];
+  if (b >= 0) {
+    const Cell c = cell_of(g, n);
+    float grid[3];
+    material_grid(g, c.i + 0.5f, c.j + 0.5f, c.k + 0.5f, b, grid);
+    const int* size = g.material_size + 3 * b;
+    int J[3];
+    for (int a = 0; a < 3; ++a) J[a] = min(size[a] - 1, max(0, (int)floorf(grid[a] + 0.5f)));
+    t = ice_temperature(g, b, J[0], J[1], J[2]);
+  }
+  g.sampled_ice_t[n] = t;
+}
+// thermal.py:430-440
+__global__ void k_check_heat(icy3d_grid g) {
+  size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  const size_t M = (size_t)g.bodies * g.mx * g.my * g.mz;
+  int bad = 0;
+  if (t < N) {
+    const float e = g.water_energy[t], v = g.water_volume[t];
+    bad += (!isfinite(e) || e < -1e-10f || v < -1e-20f) ? 1 : 0;
+  }
+  if (t < M) {
+    const float m = g.material_mass[t], e = g.material_energy[t];
+    bad += (isnan(m) || !isfinite(e) || m < 0.0f) ? 1 : 0;
+  }
+  bad = wave_sum_int(bad);
+  if (wave_leader() && bad) atomicAdd(&g.control->thermal_invalid, bad);
+}
+__global__ void k_reset_thermal_invalid(icy3d_grid g) { g.control->thermal_invalid = 0; }
+
+// simulator.py:325-347
+__global__ void k_initialize_phase(icy3d_grid g) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = cells(g);
+  if (n >= (int)N) return;
+  float phi = 0.0f;
+  if (g.id[n] == -1) {
+    const Cell c = cell_of(g, n);
+    float column = fminf((float)g.water_width - c.i - 0.5f, (float)g.water_height - c.j - 0.5f);
+    float distance = column;
+    if (g.floor_depth > 0) distance = fmaxf(column, (float)(g.boundary + g.floor_depth) - c.j - 0.5f);
+    phi = 0.5f + 0.5f * tanhf(2.0f * distance / g.width);
+  }
+  g.phi[n] = phi;
+#pragma unroll
+  for (int q = 0; q < 7; ++q) g.h[(size_t)q * N + n] = phase_equilibrium7(q, phi, 0.0f, 0.0f, 0.0f);
+}
+
+}  // namespace icy
+
+using namespace icy;
+
+namespace {
+bool ok_grid(const icy3d_grid* g) {
+  return g && g->nx > 0 && g->ny > 0 && g->nz > 0 && g->bodies >= 0 && g->bodies <= ICY3D_MAX_BODIES && g->control &&
+         (size_t)g->nx * g->ny * g->nz < (size_t)INT32_MAX / 16;
+}
+size_t n_cells(const icy3d_grid& g) { return (size_t)g.nx * g.ny * g.nz; }
+size_t n_material(const icy3d_grid& g) { return (size_t)g.bodies * g.mx * g.my * g.mz; }
+}  // namespace
+
+extern "C" {
+
+int icy3d_synchronize_water(const icy3d_grid* g, void* stream) {
+  if (!ok_grid(g)) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned blocks = icy_blocks(n_cells(*g));
+  k_build_water_targets<<<blocks, kBlock, 0, s>>>(*g);
+  k_commit_water_targets<<<blocks, kBlock, 0, s>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_initialize_water_heat(const icy3d_grid* g, void* stream) {
+  if (!ok_grid(g)) return ICY3D_ERR_ARGUMENT;
+  k_initialize_water_heat<<<icy_blocks(n_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_thermal_fluxes(const icy3d_grid* g, int advect, void* stream) {
+  if (!ok_grid(g)) return ICY3D_ERR_ARGUMENT;
+  k_thermal_fluxes<<<icy_blocks(n_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g, advect);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_apply_thermal_fluxes(const icy3d_grid* g, float dt, int advect, void* stream) {
+  if (!ok_grid(g)) return ICY3D_ERR_ARGUMENT;
+  k_apply_thermal_fluxes<<<icy_blocks(n_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g, dt, advect);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_measure_outflow(const icy3d_grid* g, void* stream) {
+  if (!ok_grid(g)) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  k_reset_outflow<<<1, 1, 0, s>>>(*g);
+  k_measure_outflow<<<icy_blocks(n_cells(*g)), kBlock, 0, s>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_initialize_material(const icy3d_grid* g, void* stream) {
+  if (!ok_grid(g)) return ICY3D_ERR_ARGUMENT;
+  size_t n = n_material(*g);
+  if (n) k_initialize_material<<<icy_blocks(n), kBlock, 0, (hipStream_t)stream>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_body_conduction(const icy3d_grid* g, float dt, void* stream) {
+  if (!ok_grid(g)) return ICY3D_ERR_ARGUMENT;
+  size_t n = n_material(*g);
+  if (n) k_body_conduction<<<icy_blocks(n), kBlock, 0, (hipStream_t)stream>>>(*g, dt);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_exchange_heat(const icy3d_grid* g, float dt, const int32_t lo[3], const int32_t hi[3], void* stream) {
+  if (!ok_grid(g)) return ICY3D_ERR_ARGUMENT;
+  const int shape[3] = {g->nx, g->ny, g->nz};
+  for (int a = 0; a < 3; ++a)
+    if (lo[a] < 0 || hi[a] > shape[a]) return ICY3D_ERR_ARGUMENT;
+  int ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+  if (ex <= 0 || ey <= 0 || ez <= 0) return 0;
+  k_exchange_heat<<<icy_blocks((size_t)ex * ey * ez), kBlock, 0, (hipStream_t)stream>>>(*g, lo[0], lo[1], lo[2], ex,
+                                                                                        ey, ez, dt);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_apply_melting(const icy3d_grid* g, void* stream) {
+  if (!ok_grid(g)) return ICY3D_ERR_ARGUMENT;
+  size_t n = n_material(*g);
+  if (n) k_apply_melting<<<icy_blocks(n), kBlock, 0, (hipStream_t)stream>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_reduce_mass(const icy3d_grid* g, void* stream) {
+  if (!ok_grid(g)) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  k_reset_reduced<<<1, ICY3D_MAX_BODIES, 0, s>>>(*g);
+  size_t columns = (size_t)g->bodies * g->mx * g->my;
+  if (columns) k_reduce_mass<<<(unsigned)((columns + 63) / 64), 64, 0, s>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_sample_ice_temperature(const icy3d_grid* g, void* stream) {
+  if (!ok_grid(g)) return ICY3D_ERR_ARGUMENT;
+  k_sample_ice_temperature<<<icy_blocks(n_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_check_heat(const icy3d_grid* g, void* stream) {
+  if (!ok_grid(g)) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  size_t n = n_cells(*g) > n_material(*g) ? n_cells(*g) : n_material(*g);
+  k_reset_thermal_invalid<<<1, 1, 0, s>>>(*g);
+  k_check_heat<<<icy_blocks(n), kBlock, 0, s>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_initialize_phase(const icy3d_grid* g, void* stream) {
+  if (!ok_grid(g)) return ICY3D_ERR_ARGUMENT;
+  k_initialize_phase<<<icy_blocks(n_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+
+}  // extern "C"

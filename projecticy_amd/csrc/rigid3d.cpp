@@ -1,0 +1,619 @@
+// Host FP64 six-DOF integration and OBB contact projection for ice bodies.
+//
+// Algorithm follows iceflow3d/rigid.py:14-404 (the reference keeps this on
+// the host in NumPy FP64): explicit velocity update with gravity and the
+// hydrodynamic loads, Euler's rigid-body equation for omega, exponential-map
+// quaternion update, then a sequential-impulse projection of box/wall and
+// box/box (separating-axis) contacts, with a simultaneous wall-manifold
+// fallback for low-inertia remnants.  Everything here is plain C++ so one
+// step costs microseconds instead of the ~0.5 ms of small-array NumPy calls.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "../../include/icy3d.h"
+
+namespace {
+
+struct V3 {
+  double x, y, z;
+  double& operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+  double operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+};
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator-(V3 a) { return {-a.x, -a.y, -a.z}; }
+inline V3 operator*(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline V3 operator*(V3 a, double s) { return {s * a.x, s * a.y, s * a.z}; }
+inline V3 operator/(V3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline double norm(V3 a) { return std::sqrt(dot(a, a)); }
+
+struct M3 {
+  double m[3][3];
+  V3 row(int i) const { return {m[i][0], m[i][1], m[i][2]}; }
+  V3 col(int j) const { return {m[0][j], m[1][j], m[2][j]}; }
+};
+inline V3 mul(const M3& a, V3 v) { return {dot(a.row(0), v), dot(a.row(1), v), dot(a.row(2), v)}; }
+inline V3 mulT(const M3& a, V3 v) { return {dot(a.col(0), v), dot(a.col(1), v), dot(a.col(2), v)}; }
+inline M3 matmul(const M3& a, const M3& b) {
+  M3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r.m[i][j] = a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j] + a.m[i][2] * b.m[2][j];
+  return r;
+}
+inline M3 transpose(const M3& a) {
+  M3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r.m[i][j] = a.m[j][i];
+  return r;
+}
+
+M3 rotation(const double* q) {
+  double n = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  double w = q[0] / n, x = q[1] / n, y = q[2] / n, z = q[3] / n;
+  M3 r;
+  r.m[0][0] = 1 - 2 * (y * y + z * z);
+  r.m[0][1] = 2 * (x * y - z * w);
+  r.m[0][2] = 2 * (x * z + y * w);
+  r.m[1][0] = 2 * (x * y + z * w);
+  r.m[1][1] = 1 - 2 * (x * x + z * z);
+  r.m[1][2] = 2 * (y * z - x * w);
+  r.m[2][0] = 2 * (x * z - y * w);
+  r.m[2][1] = 2 * (y * z + x * w);
+  r.m[2][2] = 1 - 2 * (x * x + y * y);
+  return r;
+}
+
+// Gauss-Jordan inverse with partial pivoting (3x3).
+M3 inverse(const M3& a) {
+  double w[3][6];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      w[i][j] = a.m[i][j];
+      w[i][j + 3] = (i == j) ? 1.0 : 0.0;
+    }
+  for (int c = 0; c < 3; ++c) {
+    int p = c;
+    for (int r = c + 1; r < 3; ++r)
+      if (std::fabs(w[r][c]) > std::fabs(w[p][c])) p = r;
+    if (p != c)
+      for (int j = 0; j < 6; ++j) std::swap(w[p][j], w[c][j]);
+    double d = w[c][c];
+    for (int j = 0; j < 6; ++j) w[c][j] /= d;
+    for (int r = 0; r < 3; ++r) {
+      if (r == c) continue;
+      double f = w[r][c];
+      for (int j = 0; j < 6; ++j) w[r][j] -= f * w[c][j];
+    }
+  }
+  M3 r;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) r.m[i][j] = w[i][j + 3];
+  return r;
+}
+
+V3 solve(const M3& a, V3 b) { return mul(inverse(a), b); }
+
+struct Body {
+  const icy3d_rigid_arrays* s;
+  int b;
+  V3 get(const double* base) const { return {base[3 * b], base[3 * b + 1], base[3 * b + 2]}; }
+  void set(double* base, V3 v) const {
+    base[3 * b] = v.x;
+    base[3 * b + 1] = v.y;
+    base[3 * b + 2] = v.z;
+  }
+  V3 origin() const { return get(s->origin); }
+  V3 velocity() const { return get(s->velocity); }
+  V3 omega() const { return get(s->omega); }
+  V3 com() const { return get(s->com); }
+  V3 lower() const { return get(s->lower); }
+  V3 upper() const { return get(s->upper); }
+  double mass() const { return s->mass[b]; }
+  M3 inertia() const {
+    M3 r;
+    std::memcpy(r.m, s->inertia + 9 * b, sizeof(r.m));
+    return r;
+  }
+};
+
+bool is_active(const Body& body) {
+  V3 lo = body.lower(), hi = body.upper();
+  return body.mass() > 1e-15 && hi.x > lo.x && hi.y > lo.y && hi.z > lo.z;
+}
+
+// Eight OBB corners in the order x-major, then y, then z (signs -1, +1).
+void corners(const Body& body, const M3& r, V3 out[8]) {
+  V3 o = body.origin(), lo = body.lower(), hi = body.upper();
+  int k = 0;
+  for (int sx = 0; sx < 2; ++sx)
+    for (int sy = 0; sy < 2; ++sy)
+      for (int sz = 0; sz < 2; ++sz) {
+        V3 local = {sx ? hi.x : lo.x, sy ? hi.y : lo.y, sz ? hi.z : lo.z};
+        // lower + (sign+1)/2 * (upper-lower): evaluate exactly as written.
+        local.x = lo.x + (sx ? 1.0 : 0.0) * (hi.x - lo.x);
+        local.y = lo.y + (sy ? 1.0 : 0.0) * (hi.y - lo.y);
+        local.z = lo.z + (sz ? 1.0 : 0.0) * (hi.z - lo.z);
+        out[k++] = o + mul(r, local);
+      }
+}
+
+double apply_impulse(V3& va, V3& wa, double ma, const M3& ia, V3 ra, V3 n, V3* vb, V3* wb, double mb,
+                     const M3* ib, V3 rb) {
+  V3 relative = va + cross(wa, ra);
+  double effective = 1.0 / ma + dot(n, cross(mul(ia, cross(ra, n)), ra));
+  if (vb) {
+    relative = relative - (*vb + cross(*wb, rb));
+    effective += 1.0 / mb + dot(n, cross(mul(*ib, cross(rb, n)), rb));
+  }
+  double closing = dot(relative, n);
+  if (closing < 0) {
+    V3 impulse = (-closing / effective) * n;
+    va = va + impulse / ma;
+    wa = wa + mul(ia, cross(ra, impulse));
+    if (vb) {
+      *vb = *vb - impulse / mb;
+      *wb = *wb - mul(*ib, cross(rb, impulse));
+    }
+  }
+  return std::max(0.0, -closing);
+}
+
+// Returns true and (normal, depth) on overlap; false when separated.
+bool separating_axis(V3 ca, const M3& ra, V3 ha, V3 cb, const M3& rb, V3 hb, V3* normal, double* depth) {
+  V3 delta = ca - cb;
+  for (int i = 0; i < 3; ++i) {
+    double reach = 0;
+    for (int j = 0; j < 3; ++j) reach += std::fabs(ra.m[i][j]) * ha[j];
+    double reach_b = 0;
+    for (int j = 0; j < 3; ++j) reach_b += std::fabs(rb.m[i][j]) * hb[j];
+    if (std::fabs(delta[i]) > reach + reach_b + 1e-9) return false;
+  }
+  V3 axes[15];
+  for (int i = 0; i < 3; ++i) axes[i] = ra.col(i);
+  for (int i = 0; i < 3; ++i) axes[3 + i] = rb.col(i);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) axes[6 + 3 * i + j] = cross(ra.col(i), rb.col(j));
+  double best = std::numeric_limits<double>::infinity();
+  V3 best_n = {0, 0, 0};
+  bool found = false;
+  for (int k = 0; k < 15; ++k) {
+    double length = norm(axes[k]);
+    if (length < 1e-9) continue;
+    V3 n = axes[k] / length;
+    V3 pa = mulT(ra, n), pb = mulT(rb, n);
+    double d = ha.x * std::fabs(pa.x) + ha.y * std::fabs(pa.y) + ha.z * std::fabs(pa.z) +
+               hb.x * std::fabs(pb.x) + hb.y * std::fabs(pb.y) + hb.z * std::fabs(pb.z) -
+               std::fabs(dot(delta, n));
+    if (d < -1e-9) return false;
+    if (d < best) {
+      best = d;
+      best_n = dot(delta, n) >= 0 ? n : -n;
+      found = true;
+    }
+  }
+  *normal = best_n;
+  *depth = found ? std::max(0.0, best) : best;
+  return true;
+}
+
+V3 support(V3 center, const M3& r, V3 half, V3 direction) {
+  V3 local = mulT(r, direction);
+  V3 signed_half;
+  for (int i = 0; i < 3; ++i) {
+    double s = std::fabs(local[i]) < 1e-9 ? 0.0 : (local[i] > 0 ? 1.0 : -1.0);
+    signed_half[i] = half[i] * s;
+  }
+  return center + mul(r, signed_half);
+}
+
+// Symmetric eigen-decomposition by cyclic Jacobi rotations (n <= 3).
+void symmetric_eigen(int n, double a[3][3], double values[3], double vectors[3][3]) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) vectors[i][j] = (i == j) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    double off = 0;
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) off += a[i][j] * a[i][j];
+    if (off < 1e-300) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        if (a[p][q] == 0.0) continue;
+        double theta = (a[q][q] - a[p][p]) / (2 * a[p][q]);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+        double c = 1 / std::sqrt(t * t + 1), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          double akp = a[k][p], akq = a[k][q];
+          a[k][p] = c * akp - s * akq;
+          a[k][q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {
+          double apk = a[p][k], aqk = a[q][k];
+          a[p][k] = c * apk - s * aqk;
+          a[q][k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          double vkp = vectors[k][p], vkq = vectors[k][q];
+          vectors[k][p] = c * vkp - s * vkq;
+          vectors[k][q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  for (int i = 0; i < n; ++i) values[i] = a[i][i];
+}
+
+// Minimum-norm least squares of a symmetric n x n system with the LAPACK
+// gelsd cutoff: singular values <= rcond * max singular value are dropped.
+void symmetric_lstsq(int n, const double block[3][3], const double rhs[3], double rcond, double x[3]) {
+  double a[3][3], values[3], vectors[3][3];
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) a[i][j] = block[i][j];
+  symmetric_eigen(n, a, values, vectors);
+  double smax = 0;
+  for (int i = 0; i < n; ++i) smax = std::max(smax, std::fabs(values[i]));
+  for (int i = 0; i < n; ++i) x[i] = 0;
+  for (int k = 0; k < n; ++k) {
+    if (std::fabs(values[k]) <= rcond * smax || values[k] == 0.0) continue;
+    double proj = 0;
+    for (int i = 0; i < n; ++i) proj += vectors[i][k] * rhs[i];
+    proj /= values[k];
+    for (int i = 0; i < n; ++i) x[i] += proj * vectors[i][k];
+  }
+}
+
+// Returns 0, or ICY3D_ERR_MANIFOLD when no feasible impulse set exists.
+int wall_manifold(V3& velocity, V3& omega, double mass, const M3& inverse_inertia, const V3* arms, int count,
+                  V3 normal) {
+  V3 angular[8];
+  double speed[8];
+  double peak = 0;
+  for (int i = 0; i < count; ++i) {
+    angular[i] = cross(arms[i], normal);
+    speed[i] = dot(velocity + cross(omega, arms[i]), normal);
+    peak = std::max(peak, std::fabs(speed[i]));
+  }
+  double tolerance = 1e-12 * std::max(1.0, peak);
+  double slowest = speed[0];
+  for (int i = 1; i < count; ++i) slowest = std::min(slowest, speed[i]);
+  if (slowest >= -tolerance) return 0;
+  double coupling[8][8];
+  for (int i = 0; i < count; ++i)
+    for (int j = 0; j < count; ++j) coupling[i][j] = 1.0 / mass + dot(angular[i], mul(inverse_inertia, angular[j]));
+  int maximum = std::min(3, count);
+  for (int size = 1; size <= maximum; ++size) {
+    // Lexicographic subsets of the given size (itertools.combinations order).
+    int idx[3];
+    for (int i = 0; i < size; ++i) idx[i] = i;
+    while (true) {
+      double block[3][3], rhs[3], impulse[3];
+      for (int i = 0; i < size; ++i) {
+        rhs[i] = -speed[idx[i]];
+        for (int j = 0; j < size; ++j) block[i][j] = coupling[idx[i]][idx[j]];
+      }
+      symmetric_lstsq(size, block, rhs, 1e-12, impulse);
+      bool ok = true;
+      for (int i = 0; i < size; ++i) ok = ok && !(impulse[i] < 0);
+      if (ok) {
+        for (int r = 0; r < count && ok; ++r) {
+          double after = speed[r];
+          for (int i = 0; i < size; ++i) after += coupling[r][idx[i]] * impulse[i];
+          if (after < -tolerance) ok = false;
+          for (int i = 0; i < size; ++i)
+            if (idx[i] == r && std::fabs(after) > tolerance) ok = false;
+        }
+      }
+      if (ok) {
+        double total = 0;
+        V3 torque = {0, 0, 0};
+        for (int i = 0; i < size; ++i) {
+          total += impulse[i];
+          torque = torque + impulse[i] * angular[idx[i]];
+        }
+        velocity = velocity + normal * (total / mass);
+        omega = omega + mul(inverse_inertia, torque);
+        return 0;
+      }
+      int k = size - 1;
+      while (k >= 0 && idx[k] == count - size + k) --k;
+      if (k < 0) break;
+      ++idx[k];
+      for (int i = k + 1; i < size; ++i) idx[i] = idx[i - 1] + 1;
+    }
+  }
+  return ICY3D_ERR_MANIFOLD;
+}
+
+struct Residual {
+  double penetration, closing;
+};
+
+Residual residuals(const icy3d_rigid_arrays* s, const int shape[3], int boundary) {
+  Residual out = {0.0, 0.0};
+  double low[3] = {double(boundary), double(boundary), double(boundary)};
+  double high[3] = {shape[0] - low[0], shape[1] - low[1], shape[2] - low[2]};
+  int active[ICY3D_MAX_BODIES];
+  int n_active = 0;
+  for (int b = 0; b < s->body_count; ++b)
+    if (is_active(Body{s, b})) active[n_active++] = b;
+  for (int k = 0; k < n_active; ++k) {
+    Body body{s, active[k]};
+    M3 r = rotation(s->q + 4 * active[k]);
+    V3 verts[8];
+    corners(body, r, verts);
+    V3 center = body.origin() + mul(r, body.com());
+    for (int axis = 0; axis < 3; ++axis)
+      for (int side = -1; side <= 1; side += 2) {
+        double extreme = verts[0][axis];
+        for (int i = 1; i < 8; ++i)
+          extreme = side == 1 ? std::min(extreme, verts[i][axis]) : std::max(extreme, verts[i][axis]);
+        double depth = side == 1 ? low[axis] - extreme : extreme - high[axis];
+        out.penetration = std::max(out.penetration, depth);
+        if (depth >= -1e-8) {
+          double fastest = -std::numeric_limits<double>::infinity();
+          for (int i = 0; i < 8; ++i) {
+            if (!(std::fabs(verts[i][axis] - extreme) < 1e-6)) continue;
+            V3 speed = body.velocity() + cross(body.omega(), verts[i] - center);
+            fastest = std::max(fastest, -side * speed[axis]);
+          }
+          out.closing = std::max(out.closing, fastest);
+        }
+      }
+  }
+  for (int ia = 0; ia < n_active; ++ia)
+    for (int ib = ia + 1; ib < n_active; ++ib) {
+      Body A{s, active[ia]}, B{s, active[ib]};
+      M3 ra = rotation(s->q + 4 * active[ia]), rb = rotation(s->q + 4 * active[ib]);
+      V3 ca = A.origin() + mul(ra, 0.5 * (A.lower() + A.upper()));
+      V3 cb = B.origin() + mul(rb, 0.5 * (B.lower() + B.upper()));
+      V3 ha = 0.5 * (A.upper() - A.lower()), hb = 0.5 * (B.upper() - B.lower());
+      V3 n;
+      double depth;
+      if (!separating_axis(ca, ra, ha, cb, rb, hb, &n, &depth)) continue;
+      V3 point = 0.5 * (support(ca, ra, ha, -n) + support(cb, rb, hb, n));
+      V3 relative = A.velocity() + cross(A.omega(), point - A.origin() - mul(ra, A.com())) - B.velocity() -
+                    cross(B.omega(), point - B.origin() - mul(rb, B.com()));
+      out.penetration = std::max(out.penetration, depth);
+      out.closing = std::max(out.closing, -dot(relative, n));
+    }
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+int icy3d_rigid_contact_residuals(const icy3d_rigid_arrays* s, const int shape[3], int boundary,
+                                  double residual[2]) {
+  if (!s || s->body_count > ICY3D_MAX_BODIES) return ICY3D_ERR_ARGUMENT;
+  Residual r = residuals(s, shape, boundary);
+  residual[0] = r.penetration;
+  residual[1] = r.closing;
+  return 0;
+}
+
+int icy3d_rigid_resolve_contacts(const icy3d_rigid_arrays* s, const int shape[3], int boundary, int iterations,
+                                 double residual[2]) {
+  if (!s || s->body_count > ICY3D_MAX_BODIES || iterations < 1) return ICY3D_ERR_ARGUMENT;
+  double low[3] = {double(boundary), double(boundary), double(boundary)};
+  double high[3] = {shape[0] - low[0], shape[1] - low[1], shape[2] - low[2]};
+  int active[ICY3D_MAX_BODIES];
+  int n_active = 0;
+  for (int b = 0; b < s->body_count; ++b)
+    if (is_active(Body{s, b})) active[n_active++] = b;
+  M3 rotations[ICY3D_MAX_BODIES], inverses[ICY3D_MAX_BODIES];
+  for (int it = 0; it < iterations; ++it) {
+    double maximum = 0.0, closing = 0.0;
+    for (int b = 0; b < s->body_count; ++b) rotations[b] = rotation(s->q + 4 * b);
+    for (int k = 0; k < n_active; ++k) {
+      int b = active[k];
+      Body body{s, b};
+      const M3& r = rotations[b];
+      inverses[b] = inverse(matmul(matmul(r, body.inertia()), transpose(r)));
+    }
+    for (int k = 0; k < n_active; ++k) {
+      int b = active[k];
+      Body body{s, b};
+      const M3& r = rotations[b];
+      V3 verts[8];
+      corners(body, r, verts);
+      for (int axis = 0; axis < 3; ++axis)
+        for (int side = -1; side <= 1; side += 2) {
+          double extreme = verts[0][axis];
+          for (int i = 1; i < 8; ++i)
+            extreme = side == 1 ? std::min(extreme, verts[i][axis]) : std::max(extreme, verts[i][axis]);
+          double depth = side == 1 ? low[axis] - extreme : extreme - high[axis];
+          if (!(depth >= -1e-8)) continue;
+          V3 n = {0, 0, 0};
+          n[axis] = side;
+          V3 correction = std::max(0.0, depth) * n;
+          body.set(s->origin, body.origin() + correction);
+          for (int i = 0; i < 8; ++i) verts[i] = verts[i] + correction;
+          maximum = std::max(maximum, depth);
+          double target = verts[0][axis];
+          for (int i = 1; i < 8; ++i)
+            target = side == 1 ? std::min(target, verts[i][axis]) : std::max(target, verts[i][axis]);
+          for (int i = 0; i < 8; ++i) {
+            if (!(std::fabs(verts[i][axis] - target) < 1e-6)) continue;
+            V3 arm = verts[i] - (body.origin() + mul(r, body.com()));
+            V3 v = body.velocity(), w = body.omega();
+            closing = std::max(closing, apply_impulse(v, w, body.mass(), inverses[b], arm, n, nullptr, nullptr,
+                                                      0.0, nullptr, V3{0, 0, 0}));
+            body.set(s->velocity, v);
+            body.set(s->omega, w);
+          }
+        }
+    }
+    for (int ia = 0; ia < n_active; ++ia)
+      for (int ib = ia + 1; ib < n_active; ++ib) {
+        int a = active[ia], b = active[ib];
+        Body A{s, a}, B{s, b};
+        const M3 &ra = rotations[a], &rb = rotations[b];
+        V3 ca = A.origin() + mul(ra, 0.5 * (A.lower() + A.upper()));
+        V3 cb = B.origin() + mul(rb, 0.5 * (B.lower() + B.upper()));
+        V3 ha = 0.5 * (A.upper() - A.lower()), hb = 0.5 * (B.upper() - B.lower());
+        V3 n;
+        double depth;
+        if (!separating_axis(ca, ra, ha, cb, rb, hb, &n, &depth)) continue;
+        V3 point = 0.5 * (support(ca, ra, ha, -n) + support(cb, rb, hb, n));
+        V3 va = A.velocity(), wa = A.omega(), vb = B.velocity(), wb = B.omega();
+        closing = std::max(closing, apply_impulse(va, wa, A.mass(), inverses[a], point - A.origin() - mul(ra, A.com()),
+                                                  n, &vb, &wb, B.mass(), &inverses[b],
+                                                  point - B.origin() - mul(rb, B.com())));
+        A.set(s->velocity, va);
+        A.set(s->omega, wa);
+        B.set(s->velocity, vb);
+        B.set(s->omega, wb);
+        double total = A.mass() + B.mass();
+        A.set(s->origin, A.origin() + ((depth + 1e-9) * n * B.mass()) / total);
+        B.set(s->origin, B.origin() - ((depth + 1e-9) * n * A.mass()) / total);
+        maximum = std::max(maximum, depth);
+      }
+    if (maximum < 1e-7 && closing < 1e-7) {
+      residual[0] = maximum;
+      residual[1] = closing;
+      return 0;
+    }
+  }
+  Residual r = residuals(s, shape, boundary);
+  if (r.penetration < 1e-7 && r.closing < 1e-7) {
+    residual[0] = r.penetration;
+    residual[1] = r.closing;
+    return 0;
+  }
+  if (r.penetration < 1e-7) {
+    for (int k = 0; k < n_active; ++k) {
+      int b = active[k];
+      Body body{s, b};
+      const M3& rot = rotations[b];
+      V3 center = body.origin() + mul(rot, body.com());
+      V3 verts[8];
+      corners(body, rot, verts);
+      for (int axis = 0; axis < 3; ++axis)
+        for (int side = -1; side <= 1; side += 2) {
+          double extreme = verts[0][axis];
+          for (int i = 1; i < 8; ++i)
+            extreme = side == 1 ? std::min(extreme, verts[i][axis]) : std::max(extreme, verts[i][axis]);
+          double depth = side == 1 ? low[axis] - extreme : extreme - high[axis];
+          if (!(depth >= -1e-8)) continue;
+          V3 arms[8];
+          int count = 0;
+          for (int i = 0; i < 8; ++i)
+            if (std::fabs(verts[i][axis] - extreme) < 1e-6) arms[count++] = verts[i] - center;
+          V3 n = {0, 0, 0};
+          n[axis] = side;
+          V3 v = body.velocity(), w = body.omega();
+          int code = wall_manifold(v, w, body.mass(), inverses[b], arms, count, n);
+          if (code) return code;
+          body.set(s->velocity, v);
+          body.set(s->omega, w);
+        }
+    }
+    r = residuals(s, shape, boundary);
+    if (r.penetration < 1e-7 && r.closing < 1e-7) {
+      residual[0] = r.penetration;
+      residual[1] = r.closing;
+      return 0;
+    }
+  }
+  residual[0] = r.penetration;
+  residual[1] = r.closing;
+  return ICY3D_ERR_CONTACT;
+}
+
+int icy3d_rigid_integrate(const icy3d_rigid_arrays* s, const float* loads, const double gravity[3],
+                          double linear_damping, double angular_damping, int* failed_body) {
+  if (!s || s->body_count > ICY3D_MAX_BODIES) return ICY3D_ERR_ARGUMENT;
+  V3 g = {gravity[0], gravity[1], gravity[2]};
+  for (int b = 0; b < s->body_count; ++b) {
+    Body body{s, b};
+    if (body.mass() <= 1e-15) continue;
+    M3 r = rotation(s->q + 4 * b);
+    M3 inertia = matmul(matmul(r, body.inertia()), transpose(r));
+    V3 center = body.origin() + mul(r, body.com());
+    V3 force = {double(loads[6 * b]), double(loads[6 * b + 1]), double(loads[6 * b + 2])};
+    V3 torque = {double(loads[6 * b + 3]), double(loads[6 * b + 4]), double(loads[6 * b + 5])};
+    V3 v = body.velocity() + (force / body.mass() + g);
+    V3 w = body.omega();
+    w = w + solve(inertia, torque - cross(w, mul(inertia, w)));
+    v = v * linear_damping;
+    w = w * angular_damping;
+    double travel = norm(v) + norm(w) * norm(body.upper() - body.lower()) / 2;
+    body.set(s->velocity, v);
+    body.set(s->omega, w);
+    if (!std::isfinite(travel) || travel > 0.5) {
+      if (failed_body) *failed_body = b;
+      return ICY3D_ERR_TRAVEL;
+    }
+    center = center + v;
+    // Exponential-map quaternion update, then renormalise.
+    double* q = s->q + 4 * b;
+    double rate = norm(w);
+    double angle = rate;
+    double scale = angle < 1e-12 ? 0.5 : std::sin(0.5 * angle) / rate;
+    double dw = std::cos(0.5 * angle);
+    V3 dv = scale * w;
+    double nq[4] = {dw * q[0] - dv.x * q[1] - dv.y * q[2] - dv.z * q[3],
+                    dw * q[1] + dv.x * q[0] + dv.y * q[3] - dv.z * q[2],
+                    dw * q[2] - dv.x * q[3] + dv.y * q[0] + dv.z * q[1],
+                    dw * q[3] + dv.x * q[2] - dv.y * q[1] + dv.z * q[0]};
+    double qn = std::sqrt(nq[0] * nq[0] + nq[1] * nq[1] + nq[2] * nq[2] + nq[3] * nq[3]);
+    for (int i = 0; i < 4; ++i) q[i] = nq[i] / qn;
+    body.set(s->origin, center - mul(rotation(q), body.com()));
+  }
+  return 0;
+}
+
+int icy3d_rigid_update_mass(const icy3d_rigid_arrays* s, const float* reduced) {
+  if (!s || s->body_count > ICY3D_MAX_BODIES) return ICY3D_ERR_ARGUMENT;
+  for (int b = 0; b < s->body_count; ++b) {
+    Body body{s, b};
+    const float* row = reduced + ICY3D_REDUCED_STRIDE * b;
+    double mass = row[0];
+    V3 first = {double(row[1]), double(row[2]), double(row[3])};
+    if (mass > 1e-15) {
+      V3 com = first / mass;
+      V3 shift = mul(rotation(s->q + 4 * b), com - body.com());
+      body.set(s->velocity, body.velocity() + cross(body.omega(), shift));
+      body.set(s->com, com);
+      double cc = dot(com, com);
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          s->inertia[9 * b + 3 * i + j] =
+              double(row[4 + 3 * i + j]) - mass * ((i == j ? cc : 0.0) - com[i] * com[j]);
+    } else {
+      body.set(s->velocity, V3{0, 0, 0});
+      body.set(s->omega, V3{0, 0, 0});
+    }
+    s->mass[b] = mass;
+    for (int a = 0; a < 3; ++a) {
+      s->lower[3 * b + a] = row[13 + a];
+      s->upper[3 * b + a] = row[16 + a];
+    }
+  }
+  return 0;
+}
+
+int icy3d_rigid_pack_state(const icy3d_rigid_arrays* s, float* out) {
+  if (!s || s->body_count > ICY3D_MAX_BODIES) return ICY3D_ERR_ARGUMENT;
+  for (int b = 0; b < s->body_count; ++b) {
+    Body body{s, b};
+    M3 r = rotation(s->q + 4 * b);
+    float* row = out + ICY3D_STATE_STRIDE * b;
+    V3 o = body.origin(), center = o + mul(r, body.com()), v = body.velocity(), w = body.omega();
+    for (int a = 0; a < 3; ++a) {
+      row[a] = float(o[a]);
+      for (int c = 0; c < 3; ++c) row[3 + 3 * a + c] = float(r.m[a][c]);
+      row[12 + a] = float(center[a]);
+      row[15 + a] = float(v[a]);
+      row[18 + a] = float(w[a]);
+    }
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,288 @@
+"""Six-DOF rigid ice bodies with frictionless, inelastic OBB contact.
+
+Host arithmetic is FP64 and runs in native C++ (``csrc/rigid3d.cpp``); the
+Python class only owns the state arrays so that callers can read and edit
+them exactly as with the reference's ``iceflow3d/rigid.py:121-404``
+(``origin``, ``q``, ``velocity``, ``omega``, ``com``, ``mass``, ``inertia``,
+``lower``, ``upper``).  The small free functions below are the public
+geometry helpers of the same module (``rigid.py:14-118``); configuration
+validation uses them without loading the native library.
+"""
+
+from itertools import combinations
+import ctypes
+
+import numpy as np
+
+_BOX_SIGNS = np.array(
+    [[sx, sy, sz] for sx in (-1.0, 1.0) for sy in (-1.0, 1.0) for sz in (-1.0, 1.0)]
+)
+
+
+def rotation_matrix(q):
+    """Rotation matrix of a (normalised) w,x,y,z quaternion."""
+    w, x, y, z = np.asarray(q, dtype=float) / np.linalg.norm(q)
+    xx, yy, zz = x * x, y * y, z * z
+    return np.array(
+        [
+            [1 - 2 * (yy + zz), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+            [2 * (x * y + z * w), 1 - 2 * (xx + zz), 2 * (y * z - x * w)],
+            [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (xx + yy)],
+        ]
+    )
+
+
+def quaternion_multiply(a, b):
+    """Hamilton product a*b of w,x,y,z quaternions."""
+    aw, av = a[0], np.asarray(a[1:], dtype=float)
+    bw, bv = b[0], np.asarray(b[1:], dtype=float)
+    return np.r_[aw * bw - av @ bv, aw * bv + bw * av + np.cross(av, bv)]
+
+
+def advance_quaternion(q, omega, dt=1.0):
+    """Exponential-map update q <- exp(omega dt / 2) q, renormalised."""
+    omega = np.asarray(omega, dtype=float)
+    rate = np.linalg.norm(omega)
+    angle = rate * dt
+    scale = 0.5 * dt if angle < 1e-12 else np.sin(0.5 * angle) / rate
+    result = quaternion_multiply(np.r_[np.cos(0.5 * angle), scale * omega], q)
+    return result / np.linalg.norm(result)
+
+
+def separating_axis(ca, ra, ha, cb, rb, hb):
+    """Separating-axis test of two oriented boxes.
+
+    Returns ``(normal, depth)`` with the minimum-translation normal pointing
+    from b to a, or ``None`` when the boxes are separated.
+    """
+    delta = np.asarray(ca, float) - np.asarray(cb, float)
+    reach = np.abs(ra) @ ha + np.abs(rb) @ hb
+    if np.any(np.abs(delta) > reach + 1e-9):
+        return None
+    axes = [ra[:, i] for i in range(3)] + [rb[:, i] for i in range(3)]
+    axes += [np.cross(ra[:, i], rb[:, j]) for i in range(3) for j in range(3)]
+    best_normal, best_depth = None, np.inf
+    for axis in axes:
+        length = np.linalg.norm(axis)
+        if length < 1e-9:
+            continue
+        n = axis / length
+        depth = ha @ np.abs(ra.T @ n) + hb @ np.abs(rb.T @ n) - abs(delta @ n)
+        if depth < -1e-9:
+            return None
+        if depth < best_depth:
+            best_normal = n if delta @ n >= 0 else -n
+            best_depth = max(0.0, depth)
+    return best_normal, best_depth
+
+
+def support(center, rotation, half, direction):
+    """Support point of a box; returns a face centre on flat faces."""
+    local = rotation.T @ direction
+    signs = np.where(np.abs(local) < 1e-9, 0.0, np.sign(local))
+    return center + rotation @ (half * signs)
+
+
+def apply_contact_impulse(va, wa, ma, ia, ra, normal, vb=None, wb=None, mb=None, ib=None, rb=None):
+    """Inelastic normal impulse; modifies velocities in place.
+
+    ``vb is None`` denotes a stationary, infinite-mass wall.  Returns the
+    pre-impulse closing speed (>= 0).
+    """
+    relative = va + np.cross(wa, ra)
+    effective = 1 / ma + normal @ np.cross(ia @ np.cross(ra, normal), ra)
+    if vb is not None:
+        relative = relative - (vb + np.cross(wb, rb))
+        effective += 1 / mb + normal @ np.cross(ib @ np.cross(rb, normal), rb)
+    closing = relative @ normal
+    if closing < 0:
+        impulse = (-closing / effective) * normal
+        va += impulse / ma
+        wa += ia @ np.cross(ra, impulse)
+        if vb is not None:
+            vb -= impulse / mb
+            wb -= ib @ np.cross(rb, impulse)
+    return max(0.0, -closing)
+
+
+def apply_wall_contact_manifold(velocity, omega, mass, inverse_inertia, arms, normal):
+    """Simultaneous non-negative impulses on all support points of one wall.
+
+    The normal-velocity Jacobian of a planar contact has rank <= 3, so the
+    active set is searched over subsets of at most three points.
+    """
+    arms = np.asarray(arms, dtype=float)
+    angular = np.cross(arms, normal)
+    speed = (velocity + np.cross(omega, arms)) @ normal
+    tolerance = 1e-12 * max(1.0, float(np.max(np.abs(speed))))
+    if speed.min() >= -tolerance:
+        return
+    coupling = 1 / mass + angular @ inverse_inertia @ angular.T
+    for size in range(1, min(3, len(arms)) + 1):
+        for subset in combinations(range(len(arms)), size):
+            idx = np.array(subset)
+            impulses = np.linalg.lstsq(coupling[np.ix_(idx, idx)], -speed[idx], rcond=1e-12)[0]
+            if np.any(impulses < 0):
+                continue
+            after = speed + coupling[:, idx] @ impulses
+            if np.any(after < -tolerance) or np.any(np.abs(after[idx]) > tolerance):
+                continue
+            velocity += normal * impulses.sum() / mass
+            omega += inverse_inertia @ (angular[idx].T @ impulses)
+            return
+    raise RuntimeError("wall contact manifold has no feasible nonnegative impulse")
+
+
+# ---------------------------------------------------------------------------
+# Rigid-body state and the native FP64 solver.
+# ---------------------------------------------------------------------------
+
+_RIGID_FIELDS = ("origin", "q", "velocity", "omega", "com", "mass", "inertia", "lower", "upper")
+
+
+class _RigidArrays(ctypes.Structure):
+    _fields_ = [("body_count", ctypes.c_int)] + [
+        (name, ctypes.POINTER(ctypes.c_double)) for name in _RIGID_FIELDS
+    ]
+
+
+def _contact_error(code, steps, residual):
+    from .._native import IcyError
+
+    if code == -6:
+        raise RuntimeError("wall contact manifold has no feasible nonnegative impulse")
+    if code == -5:
+        raise RuntimeError(
+            f"contact projection did not converge at step {steps + 1}: "
+            f"penetration={residual[0]:.9g} cells, closing_speed={residual[1]:.9g} LU"
+        )
+    raise IcyError(code, "rigid contact solver")
+
+
+class RigidBodies:
+    """Host FP64 rigid state of every ice body, integrated natively.
+
+    ``s`` is the owning simulation (or any object exposing ``shape``,
+    ``body_count``, ``steps`` and ``config.boundary_cells`` /
+    ``config.body_contact_iterations``; the reference's tests build such
+    namespaces around fixture states).
+    """
+
+    def __init__(self, simulation):
+        self.s = simulation
+        bodies = simulation.config.bodies
+        self.origin = np.array([b.center_cells for b in bodies], dtype=float)
+        self.q = np.array([b.quaternion_wxyz for b in bodies], dtype=float)
+        self.velocity = np.array([b.velocity_lattice for b in bodies], dtype=float)
+        self.omega = np.array([b.angular_velocity_lattice for b in bodies], dtype=float)
+        self.com = np.zeros((len(bodies), 3))
+        self.mass = np.zeros(len(bodies))
+        self.inertia = np.zeros((len(bodies), 3, 3))
+        self.lower = -0.5 * np.array([b.size_cells for b in bodies], dtype=float)
+        self.upper = -self.lower
+        self.upload()
+
+    # -- native plumbing -------------------------------------------------
+    def _arrays(self):
+        """Contiguous FP64 views of the state for the native solver."""
+        count = len(np.atleast_1d(self.mass))
+        shapes = {
+            "origin": (count, 3), "q": (count, 4), "velocity": (count, 3),
+            "omega": (count, 3), "com": (count, 3), "mass": (count,),
+            "inertia": (count, 3, 3), "lower": (count, 3), "upper": (count, 3),
+        }
+        for name in _RIGID_FIELDS:
+            value = getattr(self, name)
+            if (
+                not isinstance(value, np.ndarray)
+                or value.dtype != np.float64
+                or not value.flags.c_contiguous
+                or not value.flags.writeable
+                or value.shape != shapes[name]
+            ):
+                value = np.ascontiguousarray(np.array(value, dtype=np.float64).reshape(shapes[name]))
+                setattr(self, name, value)
+        arrays = _RigidArrays(count)
+        for name in _RIGID_FIELDS:
+            setattr(arrays, name, getattr(self, name).ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        return arrays
+
+    def _limits(self):
+        shape = (ctypes.c_int * 3)(*[int(n) for n in self.s.shape])
+        return shape, int(self.s.config.boundary_cells), int(self.s.config.body_contact_iterations)
+
+    # -- public API ------------------------------------------------------
+    def upload(self):
+        """Send the once-rounded FP32 pose/velocity state to the device."""
+        self.s._upload_rigid_state(self.device_state())
+
+    def device_state(self):
+        """[B, 21] float32: origin, rotation (row-major), centre, v, omega."""
+        from .._native import lib
+
+        arrays = self._arrays()
+        out = np.empty((arrays.body_count, 21), np.float32)
+        lib().icy3d_rigid_pack_state(ctypes.byref(arrays), out.ctypes.data_as(ctypes.c_void_p))
+        return out
+
+    def update_mass(self, *, upload=True):
+        """Reduce the material grids and refresh mass, COM, inertia, bounds."""
+        from .._native import lib
+
+        reduced = self.s._reduce_mass_state()
+        arrays = self._arrays()
+        lib().icy3d_rigid_update_mass(ctypes.byref(arrays), reduced.ctypes.data_as(ctypes.c_void_p))
+        if upload:
+            self.upload()
+
+    def advance(self):
+        """Integrate one LBM step with the downloaded hydrodynamic loads."""
+        forces = self.s._download_loads()
+        self.integrate(forces)
+        self.resolve_contacts()
+        self.upload()
+
+    def integrate(self, loads):
+        """Velocity/pose update from [B, 6] float32 force/torque loads."""
+        from .._native import lib, check
+
+        s = self.s
+        gravity = np.array(s.config.gravity, dtype=float) * s.scales.dt_s**2 / s.scales.dx_m
+        arrays = self._arrays()
+        loads = np.ascontiguousarray(loads, dtype=np.float32)
+        failed = ctypes.c_int(-1)
+        code = lib().icy3d_rigid_integrate(
+            ctypes.byref(arrays),
+            loads.ctypes.data_as(ctypes.c_void_p),
+            gravity.ctypes.data_as(ctypes.c_void_p),
+            ctypes.c_double(s.config.linear_damping),
+            ctypes.c_double(s.config.angular_damping),
+            ctypes.byref(failed),
+        )
+        if code == -4:
+            raise RuntimeError(
+                f"rigid displacement exceeds half a cell at step {s.steps + 1}, body {failed.value}"
+            )
+        check(code, "rigid integration")
+
+    def resolve_contacts(self):
+        from .._native import lib
+
+        arrays = self._arrays()
+        shape, boundary, iterations = self._limits()
+        residual = (ctypes.c_double * 2)()
+        code = lib().icy3d_rigid_resolve_contacts(
+            ctypes.byref(arrays), shape, boundary, iterations, residual
+        )
+        if code != 0:
+            _contact_error(code, self.s.steps, residual)
+
+    def contact_residuals(self):
+        """(maximum penetration, maximum closing speed); no mutation."""
+        from .._native import lib
+
+        arrays = self._arrays()
+        shape, boundary, _ = self._limits()
+        residual = (ctypes.c_double * 2)()
+        lib().icy3d_rigid_contact_residuals(ctypes.byref(arrays), shape, boundary, residual)
+        return float(residual[0]), float(residual[1])

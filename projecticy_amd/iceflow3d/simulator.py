@@ -1,0 +1,647 @@
+"""``IceFlow3D``: the reference's 3D coupled solver API on MI355X.
+
+Drop-in for ``iceflow3d.simulator.IceFlow3D`` (iceflow3d/simulator.py:40-947):
+same constructor, ``step(n)``, ``flush_thermal()``, ``snapshot()``,
+``diagnostics()``, ``check_stability()``, public field names and exception
+behaviour.  State lives in PyTorch-ROCm tensors on ``cuda:0`` in GPU-friendly
+layouts; every kernel is a hand-written gfx950 HIP kernel reached through the
+C ABI of include/icy3d.h.  Rigid bodies integrate on the host in FP64 native
+code.  There is no CPU path: without a HIP device the constructor raises.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from .. import _native
+from .._native import Control, Grid, check, int3, lib
+from .config import IceFlowConfig, LatticeScales, integer
+from .fields import ControlScalar, DerivedField, DeviceField, HostScalar
+from .rigid import RigidBodies
+from .viscosity import air_dynamic_viscosity_pa_s
+
+
+def _require_device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("IceFlow3D requires a HIP (ROCm) GPU; none is available")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+class IceFlow3D:
+    """HOME D3Q27 momentum, D3Q7 AA phase, moving and melting ice (3D)."""
+
+    def __init__(self, config: IceFlowConfig):
+        if not isinstance(config, IceFlowConfig):
+            raise TypeError("config must be IceFlowConfig")
+        if config.conserve_phase_volume:
+            raise ValueError(
+                "global phase-volume correction is commented out; set conserve_phase_volume=False"
+            )
+        if config.gas_state_repair:
+            raise ValueError("HOME does not use gas-state repair")
+        if len(config.bodies) > _native.MAX_BODIES:
+            raise ValueError(f"at most {_native.MAX_BODIES} ice bodies are supported")
+        self.device = _require_device()
+        self._lib = lib()
+        self.config, self.scales = config, LatticeScales.from_iceflow_config(config)
+        self.shape = (config.nx, config.ny, config.nz)
+        self.nx, self.ny, self.nz = self.shape
+        self.steps = 0
+        self.body_count = len(config.bodies)
+        self.rho_air = config.rho_air / config.rho_water
+        self.gravity = tuple(np.array(config.gravity) * self.scales.dt_s**2 / self.scales.dx_m)
+        dx = self.scales.dx_m
+        self.sigma = config.surface_tension_n_m * self.scales.dt_s**2 / (config.rho_water * dx**3)
+        self.mu_air = air_dynamic_viscosity_pa_s(config.thermal.initial_air_temperature_c)
+        self.nu_air = self.mu_air / config.rho_air * self.scales.dt_s / dx**2
+        props = config.thermal.properties
+        self.dx = dx
+        self.cell_volume = dx**3
+        self.initial_cell_mass = config.rho_ice * self.cell_volume
+        self.rho_cp = config.rho_water * props.specific_heat_water_j_kg_k
+        self.tm = props.melting_temperature_c
+        self.cp_ice = props.specific_heat_ice_j_kg_k
+        self.k_water, self.k_ice = props.conductivity_water_w_m_k, props.conductivity_ice_w_m_k
+        self.latent = props.latent_heat_j_kg
+        self.pending_cells = 0
+        self._parity = 0
+        self._allocate()
+        self.rigid = RigidBodies(self)
+        self._call("initialize_material")
+        self.rigid.update_mass()
+        self._rasterize()
+        self._call("initialize_phase")
+        if config.phase_warmup_steps:
+            parity = ctypes.c_int32(self._parity)
+            check(self._lib.icy3d_phase_warmup(self._gp, config.phase_warmup_steps, ctypes.byref(parity),
+                                               self._stream()), "phase warm-up")
+            self._parity = parity.value
+        self._call("initialize_water_heat")
+        self._build_reference()
+        self._gradients()
+        self._forces(False)
+        self._seed_populations()
+        self.initial_ice_mass_kg = self.rigid.mass.copy() * self.scales.mass_kg
+        self.initial_water_volume_m3 = float(self.water_volume.to_numpy().sum(dtype=float))
+        self.initial_phase_volume_m3 = float(self.water_phase.to_numpy().sum(dtype=np.float64) * self.cell_volume)
+        self._thermal_pending = 0
+        self._gradient_cache_valid = True
+
+    # ------------------------------------------------------------------ setup
+    def _allocate(self):
+        cfg, th, props = self.config, self.config.thermal, self.config.thermal.properties
+        dev, f32, i8 = self.device, torch.float32, torch.int8
+        N = self.shape
+        B = self.body_count
+        sizes = np.array([b.size_cells for b in cfg.bodies], dtype=np.int32)
+        mshape = (B, *map(int, sizes.max(axis=0)))
+        zeros = lambda *s, dtype=f32: torch.zeros(s, dtype=dtype, device=dev)  # noqa: E731
+        self._t = t = {
+            "home": zeros(10, *N), "home_post": zeros(10, *N), "h": zeros(7, *N),
+            "phi": zeros(*N), "pressure": zeros(*N), "old_pressure": zeros(*N), "coverage": zeros(*N),
+            "u": zeros(3, *N), "old_u": zeros(3, *N), "force": zeros(3, *N), "accel": zeros(3, *N),
+            "grad": zeros(3, *N), "flux": zeros(3, *N),
+            "id": zeros(*N, dtype=i8), "prev_id": zeros(*N, dtype=i8), "cov_id": zeros(*N, dtype=i8),
+            "flag": zeros(*N, dtype=i8),
+            "refill_state": zeros(5, *N), "refill_next": zeros(5, *N),
+            "refill_valid": zeros(*N, dtype=i8), "refill_next_valid": zeros(*N, dtype=i8),
+            "ref_density": zeros(self.ny), "ref_pressure": zeros(self.ny),
+            "body": zeros(B, _native.STATE_STRIDE),
+            "material_size": torch.from_numpy(sizes).to(dev),
+            "body_t0": torch.tensor(
+                [th.initial_ice_temperature_c if b.temperature_c is None else b.temperature_c for b in cfg.bodies],
+                dtype=f32, device=dev),
+            "material_mass": zeros(*mshape), "material_energy": zeros(*mshape), "material_heat": zeros(*mshape),
+            "water_volume": zeros(*N), "water_energy": zeros(*N), "target_energy": zeros(*N),
+            "sampled_ice_t": zeros(*N), "energy_flux": zeros(3, *N), "volume_flux": zeros(3, *N),
+            "control": torch.zeros(ctypes.sizeof(Control), dtype=torch.uint8, device=dev),
+        }
+        self._control_host = torch.zeros(ctypes.sizeof(Control), dtype=torch.uint8).pin_memory()
+        g = Grid()
+        g.nx, g.ny, g.nz = self.shape
+        g.boundary, g.bodies = cfg.boundary_cells, B
+        g.mx, g.my, g.mz = mshape[1:]
+        g.water_width, g.water_height = cfg.water_width, cfg.water_height
+        g.floor_depth, g.check_stability = cfg.floor_water_depth_cells, int(cfg.check_stability)
+        g.rho_air, g.gravity_y, g.sigma = self.rho_air, self.gravity[1], self.sigma
+        g.width, g.mobility = cfg.interface_width, cfg.mobility
+        g.normal_eps, g.bulk_rate = cfg.phase_normal_epsilon, cfg.cm_acoustic_relaxation_rate
+        g.threshold, g.max_velocity = th.solid_liquid_threshold, cfg.maximum_velocity_lattice
+        g.expansion, g.water_t0 = th.thermal_expansion_water_1_k, th.initial_water_temperature_c
+        g.air_t, g.flux_limit = th.initial_air_temperature_c, cfg.phase_flux_acceleration_limit
+        g.rho_water_si, g.rho_air_si, g.mu_air = cfg.rho_water, cfg.rho_air, self.mu_air
+        g.nu_scale = self.scales.dt_s / self.dx**2
+        g.tm, g.rho_cp, g.cp_ice = self.tm, self.rho_cp, self.cp_ice
+        g.k_water, g.k_ice, g.latent = self.k_water, self.k_ice, self.latent
+        g.cell_volume, g.dx = self.cell_volume, self.dx
+        g.velocity_m_s, g.initial_cell_mass, g.mass_kg = self.scales.velocity_m_s, self.initial_cell_mass, self.scales.mass_kg
+        for name in _native.GRID_POINTERS:
+            setattr(g, name, t[name].data_ptr())
+        self._grid = g
+        self._gp = ctypes.pointer(g)
+        self._body_state = np.zeros((B, _native.STATE_STRIDE), np.float32)
+        self._control_offsets()
+        self._make_fields()
+        # Stability limits start at "nothing invalid".
+        control = self._read_control()
+        control.invalid_index = math.prod(self.shape)
+        self._write_control(control)
+
+    def _control_offsets(self):
+        floats = self._t["control"].view(torch.float32)
+        loads0 = Control.loads.offset // 4
+        reduced0 = Control.reduced.offset // 4
+        B = self.body_count
+        self._loads = floats[loads0 : loads0 + 6 * _native.MAX_BODIES].view(_native.MAX_BODIES, 6)[:B]
+        self._reduced = floats[reduced0 : reduced0 + _native.REDUCED_STRIDE * _native.MAX_BODIES].view(
+            _native.MAX_BODIES, _native.REDUCED_STRIDE)[:B]
+
+    def _make_fields(self):
+        t = self._t
+        vec = lambda name: DeviceField(t[name], components=3)  # noqa: E731
+        self.home = DeviceField(t["home"], components=10)
+        self.home_post = DeviceField(t["home_post"], components=10)
+        self.h = DeviceField(t["h"], components=7)
+        self.water_phase = DeviceField(t["phi"])
+        self.pressure_lattice = DeviceField(t["pressure"])
+        self.old_pressure = DeviceField(t["old_pressure"])
+        self.coverage = DeviceField(t["coverage"])
+        self.velocity_lattice = vec("u")
+        self.old_velocity = vec("old_u")
+        self.force_density = vec("force")
+        self.home_acceleration = vec("accel")
+        self.phase_gradient_lattice = vec("grad")
+        self.phase_mass_flux = vec("flux")
+        self.capillary_stress = DerivedField(self._capillary_stress, (*self.shape, 3, 3))
+        self.solid_body_id = DeviceField(t["id"], logical_dtype=np.int32)
+        self.previous_body_id = DeviceField(t["prev_id"], logical_dtype=np.int32)
+        self.coverage_body_id = DeviceField(t["cov_id"], logical_dtype=np.int32)
+        self.refill_state = DeviceField(t["refill_state"], components=5)
+        self.refill_next = DeviceField(t["refill_next"], components=5)
+        self.refill_valid = DeviceField(t["refill_valid"])
+        self.refill_next_valid = DeviceField(t["refill_next_valid"])
+        self.reference_density = DeviceField(t["ref_density"])
+        self.reference_pressure = DeviceField(t["ref_pressure"])
+        body = t["body"]
+        self.body_origin = DeviceField(body[:, 0:3])
+        self.body_rotation = _MatrixRows(body[:, 3:12])
+        self.body_center = DeviceField(body[:, 12:15])
+        self.body_velocity = DeviceField(body[:, 15:18])
+        self.body_angular_velocity = DeviceField(body[:, 18:21])
+        self.hydrodynamic_force = DeviceField(self._loads[:, 0:3])
+        self.hydrodynamic_torque = DeviceField(self._loads[:, 3:6])
+        self.reduced_state = DeviceField(self._reduced)
+        self.material_size = DeviceField(t["material_size"])
+        self.initial_body_temperature = DeviceField(t["body_t0"])
+        self.material_mass = DeviceField(t["material_mass"])
+        self.material_energy = DeviceField(t["material_energy"])
+        self.material_heat = DeviceField(t["material_heat"])
+        self.water_volume = DeviceField(t["water_volume"])
+        self.water_energy = DeviceField(t["water_energy"])
+        self.target_energy = DeviceField(t["target_energy"])
+        self.sampled_ice_temperature = DeviceField(t["sampled_ice_t"])
+        self.energy_flux = vec("energy_flux")
+        self.volume_flux = vec("volume_flux")
+        self.phase_parity = HostScalar(self, "_parity")
+        scalars = {
+            "invalid_index": np.int32, "maximum_speed": np.float32, "peak_speed": np.float32,
+            "repair_count": np.int32, "phase_clip_volume": np.float32, "remap_energy": np.float32,
+            "discarded_heat": np.float32, "max_outflow": np.float32, "thermal_invalid": np.int32,
+            "openings": np.int32, "remaining": np.int32, "refill_lower": np.int32, "refill_upper": np.int32,
+        }
+        self._scalar_names = tuple(scalars)
+        for name, dtype in scalars.items():
+            setattr(self, name, ControlScalar(self, name, dtype))
+        self._gradient_geometry_lower = ControlScalar(self, "geometry_lower", np.int32)
+        self._gradient_geometry_upper = ControlScalar(self, "geometry_upper", np.int32)
+
+    _GRID_FIELDS = (
+        "home", "home_post", "h", "water_phase", "pressure_lattice", "old_pressure", "coverage",
+        "velocity_lattice", "old_velocity", "force_density", "home_acceleration", "phase_gradient_lattice",
+        "phase_mass_flux", "solid_body_id", "previous_body_id", "coverage_body_id", "refill_state",
+        "refill_next", "refill_valid", "refill_next_valid", "reference_density", "reference_pressure",
+        "body_origin", "body_rotation", "body_center", "body_velocity", "body_angular_velocity",
+        "hydrodynamic_force", "hydrodynamic_torque", "material_size", "initial_body_temperature",
+        "material_mass", "material_energy", "material_heat", "water_volume", "water_energy",
+        "target_energy", "sampled_ice_temperature", "energy_flux", "volume_flux", "capillary_stress",
+    )
+
+    def fields(self):
+        """Restart-state fields by reference name (checkpoint._fields)."""
+        out = {name: getattr(self, name) for name in self._GRID_FIELDS}
+        out["phase_parity"] = self.phase_parity
+        for name in self._scalar_names:
+            out[name] = getattr(self, name)
+        return out
+
+    # --------------------------------------------------------------- plumbing
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _call(self, name, *args):
+        check(getattr(self._lib, "icy3d_" + name)(self._gp, *args, self._stream()), name)
+
+    def _read_control(self):
+        self._control_host.copy_(self._t["control"])
+        return Control.from_buffer_copy(self._control_host.numpy().tobytes())
+
+    def _write_control(self, control):
+        raw = np.frombuffer(bytes(control), dtype=np.uint8)
+        self._t["control"].copy_(torch.from_numpy(raw.copy()))
+
+    def _upload_rigid_state(self, state):
+        """rigid.py:136-151 device side: one small host-to-device copy."""
+        state = np.ascontiguousarray(state, dtype=np.float32)
+        self._body_state[...] = state
+        self._t["body"].copy_(torch.from_numpy(state.copy()))
+
+    def _reduce_mass_state(self):
+        self._call("reduce_mass")
+        control = self._read_control()
+        return np.array(control.reduced, dtype=np.float32)[: self.body_count]
+
+    def _download_loads(self):
+        return np.array(self._read_control().loads, dtype=np.float32)[: self.body_count]
+
+    def _capillary_stress(self):
+        out = torch.empty((9, *self.shape), dtype=torch.float32, device=self.device)
+        self._call("capillary_stress", ctypes.c_void_p(out.data_ptr()))
+        return np.moveaxis(out.cpu().numpy(), 0, -1).reshape(*self.shape, 3, 3)
+
+    def invalidate_gradient_cache(self):
+        """Call after externally editing phase, coverage, or fluid/solid masks."""
+        self._gradient_cache_valid = False
+
+    @property
+    def physical_time_s(self):
+        return self.steps * self.scales.dt_s
+
+    # ------------------------------------------------- reference kernel methods
+    def _classify_home(self):
+        self._call("classify", None, None)
+
+    def _gradients(self, classified=False):
+        if not classified:
+            self._classify_home()
+        self._call("gradients", None, None)
+
+    def _collide_phase(self, moving_walls=True):
+        self._call("collide_phase", self._parity, int(moving_walls))
+        self._parity = 1 - self._parity
+
+    def _recover_phase(self, recover_flow=False):
+        self._call("recover_phase", self._parity, int(recover_flow))
+
+    def _recover_pressure(self):
+        self._call("recover_pressure")
+
+    def _scan_stability(self):
+        self._call("scan_stability")
+
+    def _forces(self, with_flux, moment_pressure=False):
+        self._call("forces", int(with_flux), int(moment_pressure))
+
+    def _collide_momentum(self):
+        self._call("collide_momentum")
+
+    def _stream_home(self, classified=False):
+        if not classified:
+            self._classify_home()
+        self._call("stream_home")
+
+    def _seed_populations(self):
+        self._call("seed_populations")
+
+    def _set_equilibrium_all(self, velocity=None, pressure=None, with_phase=True):
+        if velocity is not None:
+            self.velocity_lattice.from_numpy(
+                np.where((self.solid_body_id.to_numpy() == -1)[..., None], np.asarray(velocity, np.float32),
+                         self.velocity_lattice.to_numpy()))
+        use = pressure is not None
+        self._call("set_equilibrium_all", self._parity, int(with_phase), int(use), float(pressure or 0.0))
+
+    def _rasterize(self):
+        self._call("rasterize")
+
+    def _repair_phase_layout(self):
+        self._call("repair_phase_layout", self._parity)
+
+    def _synchronize_water(self):
+        self._call("synchronize_water")
+
+    def _thermal_fluxes(self, advect):
+        self._call("thermal_fluxes", int(advect))
+
+    def _apply_thermal_fluxes(self, dt, advect):
+        self._call("apply_thermal_fluxes", float(dt), int(advect))
+
+    def _measure_outflow(self):
+        self._call("measure_outflow")
+
+    def _body_conduction(self, dt):
+        self._call("body_conduction", float(dt))
+
+    def _thermal_ice_bounds(self):
+        """thermal.py:295-306 from the uploaded FP32 body state."""
+        state = self._body_state
+        shape = np.array(self.shape)
+        lo, hi = shape.copy(), np.zeros(3, dtype=int)
+        sizes = np.array([b.size_cells for b in self.config.bodies], dtype=np.float32)
+        for b in range(self.body_count):
+            half = np.float32(0.5) * sizes[b] + np.float32(1.0)
+            rot = np.abs(state[b, 3:12].reshape(3, 3))
+            extent = (rot @ half).astype(np.float32)
+            origin = state[b, 0:3]
+            lower = np.floor(origin - extent).astype(int) - 3
+            upper = np.ceil(origin + extent).astype(int) + 3
+            lo, hi = np.minimum(lo, lower), np.maximum(hi, upper)
+        return np.maximum(lo, 0), np.minimum(hi, shape)
+
+    def _exchange_heat(self, dt):
+        lo, hi = self._thermal_ice_bounds()
+        self._call("exchange_heat", float(dt), int3(lo), int3(hi))
+
+    def _exchange_heat_full(self, dt):
+        self._call("exchange_heat", float(dt), int3((0, 0, 0)), int3(self.shape))
+
+    def _apply_melting(self):
+        self._call("apply_melting")
+
+    def _reduce_mass(self):
+        self._call("reduce_mass")
+
+    def _sample_ice_temperature(self):
+        self._call("sample_ice_temperature")
+
+    def _check_heat(self):
+        self._call("check_heat")
+
+    def _project_refill(self):
+        control = self._read_control()
+        lo = np.array(control.refill_lower)
+        hi = np.array(control.refill_upper) + 1
+        self._call("project_refill", self._parity, int3(lo), int3(hi))
+
+    def _write_gradient_all(self):
+        self._call("write_gradient_all")
+
+    def boundary_flags(self):
+        self._classify_home()
+        return DeviceField(self._t["flag"]).to_numpy().astype(np.int32)
+
+    def generic_forces(self):
+        out = torch.empty((6, *self.shape), dtype=torch.float32, device=self.device)
+        self._call("generic_forces", ctypes.c_void_p(out.data_ptr()))
+        return np.moveaxis(out.cpu().numpy(), 0, -1)
+
+    def _stress_neighbor_velocity(self, cell, c):
+        out = torch.zeros(4, dtype=torch.float32, device=self.device)
+        self._call("probe_stress_velocity", int3(cell), int3(c), ctypes.c_void_p(out.data_ptr()))
+        return out[:3].cpu().numpy().astype(float)
+
+    def mirror_probe(self, cell, c):
+        out = torch.zeros(4, dtype=torch.float32, device=self.device)
+        self._call("probe_mirror", int3(cell), int3(c), ctypes.c_void_p(out.data_ptr()))
+        return out.cpu().numpy().astype(float)
+
+    def export_phase_populations(self, out=None):
+        """phase.py:186-191: logical D3Q7 populations [nx, ny, nz, 7]."""
+        buf = torch.empty((7, *self.shape), dtype=torch.float32, device=self.device)
+        self._call("export_phase", self._parity, ctypes.c_void_p(buf.data_ptr()))
+        result = np.moveaxis(buf.cpu().numpy(), 0, -1)
+        if out is not None:
+            out[...] = result
+        return result
+
+    # ------------------------------------------------------------ host logic
+    def _build_reference(self):
+        """simulator.py:349-383: frozen hydrostatic reference (host NumPy)."""
+        cfg = self.config
+        phase = self.water_phase.to_numpy()
+        ids = self.solid_body_id.to_numpy()
+        ref = (np.arange(self.ny) < cfg.boundary_cells + cfg.floor_water_depth_cells).astype(float)
+        full_width = cfg.water_width == self.nx - cfg.boundary_cells
+        mask = ids == -1
+        if full_width:
+            ref = (np.arange(self.ny) < cfg.water_height).astype(float)
+        else:
+            mask[: cfg.water_width] = False
+        if cfg.floor_water_depth_cells or full_width:
+            counts = mask.sum(axis=(0, 2))
+            sums = np.where(mask, phase, 0).sum(axis=(0, 2), dtype=float)
+            np.divide(sums, counts, out=ref, where=counts > 0)
+        rho = self.rho_air + (1 - self.rho_air) * ref
+        p = -self.gravity[1] * (np.cumsum(rho[::-1])[::-1] - 0.5 * rho)
+        excess = np.where(ids == -1, (phase - ref[None, :, None]) * (1 - self.rho_air), 0.0)
+        dynamic = -self.gravity[1] * (np.cumsum(excess[:, ::-1, :], axis=1)[:, ::-1, :] - 0.5 * excess)
+        self.reference_density.from_numpy(rho.astype(np.float32))
+        self.reference_pressure.from_numpy(p.astype(np.float32))
+        self.pressure_lattice.from_numpy(dynamic.astype(np.float32))
+
+    def _raise_unstable(self, index):
+        cell = (index // (self.ny * self.nz), index // self.nz % self.ny, index % self.nz)
+        phi = self.water_phase.to_numpy()[cell]
+        velocity = self.velocity_lattice.to_numpy()[cell]
+        raise RuntimeError(f"unstable fluid at step {self.steps + 1}, cell {cell}: phi={phi}, velocity={velocity}")
+
+    def check_stability(self, scanned=False):
+        """simulator.py:807-821."""
+        if not scanned:
+            self._scan_stability()
+        control = self._read_control()
+        if control.invalid_index < math.prod(self.shape):
+            self._raise_unstable(control.invalid_index)
+        return float(control.maximum_speed)
+
+    def _refresh_geometry(self, phase_changed=True):
+        """simulator.py:708-736; the refill loop reads one counter per sweep."""
+        self._rasterize()
+        self._repair_phase_layout()
+        self._call("find_refill_changes")
+        control = self._read_control()
+        remaining, covered = int(control.openings), int(control.covered)
+        shape = np.array(self.shape)
+        lower, upper = np.array(control.refill_lower), np.array(control.refill_upper)
+        box_lo, box_hi = np.maximum(lower - 1, 0), np.minimum(upper + 2, shape)
+        if remaining > 0:
+            self._call("seed_refill_region", int3(box_lo), int3(box_hi))
+        has_openings = remaining > 0
+        self.pending_cells = 0
+        while remaining:
+            self._call("propagate_refill", int3(box_lo), int3(box_hi))
+            new = int(self._read_control().remaining)
+            self._call("accept_refill", int3(box_lo), int3(box_hi))
+            if new == remaining:
+                self.pending_cells = new
+                break
+            remaining = new
+        self._call("commit_refill")
+        # gradients.py:388-400: changed box plus the mirror dependency radius.
+        geo_lo, geo_hi = np.array(control.geometry_lower), np.array(control.geometry_upper)
+        lo, hi = np.minimum(lower, geo_lo), np.maximum(upper, geo_hi)
+        if np.all(hi >= lo):
+            lo, hi = np.maximum(lo - 6, 0), np.minimum(hi + 7, shape)
+            self._call("refill_gradients", int3(lo), int3(hi))
+        if has_openings or covered:
+            self._call("project_refill", self._parity, int3(lower), int3(upper + 1))
+        if phase_changed or has_openings or covered:
+            self._synchronize_water()
+
+    def _advance_advection(self, speed_bound=None):
+        """thermal.py:208-226."""
+        th = self.config.thermal
+        if not th.advection_enabled:
+            return
+        rate = 3 * speed_bound if speed_bound is not None else math.inf
+        if rate > th.max_courant_number:
+            self._measure_outflow()
+            rate = float(self._read_control().max_outflow)
+        if not math.isfinite(rate):
+            raise RuntimeError(f"nonfinite thermal velocity at step {self.steps + 1}")
+        substeps = max(1, math.ceil(rate / th.max_courant_number))
+        self._check_substeps(substeps)
+        for _ in range(substeps):
+            self._thermal_fluxes(True)
+            self._apply_thermal_fluxes(self.scales.dt_s / substeps, True)
+            self._synchronize_water()
+
+    def _check_substeps(self, count):
+        if count > self.config.thermal.max_substeps_per_update:
+            raise RuntimeError(f"thermal update needs {count} substeps at step {self.steps + 1}")
+
+    def _advance_conduction(self, dt):
+        """thermal.py:234-258."""
+        alpha = max(self.k_water / self.rho_cp, self.k_ice / (self.config.rho_ice * self.cp_ice))
+        substeps = max(1, math.ceil(alpha * dt / (self.dx**2 * self.config.thermal.max_fourier_number)))
+        self._check_substeps(substeps)
+        for _ in range(substeps):
+            self._body_conduction(dt / substeps)
+            self._thermal_fluxes(False)
+            self._apply_thermal_fluxes(dt / substeps, False)
+            self._exchange_heat(dt / substeps)
+            self._apply_melting()
+            self.rigid.update_mass(upload=False)
+            self.rigid.resolve_contacts()
+            self.rigid.upload()
+            self._refresh_geometry(phase_changed=False)
+        if self.config.check_stability:
+            self._check_heat()
+            if int(self._read_control().thermal_invalid):
+                raise RuntimeError(f"invalid thermal state at step {self.steps + 1}")
+
+    def step(self, num_steps=1):
+        """simulator.py:823-850: one fused device launch sequence for the
+        flow part, then host rigid bodies, geometry refresh and heat."""
+        num_steps = integer("num_steps", num_steps, 0)
+        for _ in range(num_steps):
+            refresh = not self._gradient_cache_valid
+            self._call("flow_step", self._parity, int(refresh))
+            self._gradient_cache_valid = True
+            self._parity = 1 - self._parity
+            control = self._read_control()
+            speed_bound = None
+            if self.config.check_stability:
+                if control.invalid_index < math.prod(self.shape):
+                    self._raise_unstable(control.invalid_index)
+                speed_bound = float(control.maximum_speed)
+            loads = np.array(control.loads, dtype=np.float32)[: self.body_count]
+            self.rigid.integrate(loads)
+            self.rigid.resolve_contacts()
+            self.rigid.upload()
+            self._refresh_geometry()
+            self._advance_advection(speed_bound)
+            self._thermal_pending += 1
+            if self._thermal_pending >= self.config.thermal.update_interval_lbm_steps:
+                self._advance_conduction(self._thermal_pending * self.scales.dt_s)
+                self._thermal_pending = 0
+            self.steps += 1
+
+    def flush_thermal(self):
+        """Resolve the final partial thermal interval at the end of a run."""
+        if self._thermal_pending:
+            self._advance_conduction(self._thermal_pending * self.scales.dt_s)
+            self._thermal_pending = 0
+
+    def snapshot(self):
+        """simulator.py:858-889."""
+        th = self.config.thermal
+        ids = self.solid_body_id.to_numpy()
+        phi = self.water_phase.to_numpy()
+        volume, energy = self.water_volume.to_numpy(), self.water_energy.to_numpy()
+        cp = self.config.rho_water * th.properties.specific_heat_water_j_kg_k
+        temp = np.full(self.shape, th.initial_air_temperature_c, np.float32)
+        wet = volume > 1e-30
+        temp[wet] = (energy[wet] / (cp * volume[wet]) + th.properties.melting_temperature_c).astype(np.float32)
+        self._sample_ice_temperature()
+        ice = self.sampled_ice_temperature.to_numpy()
+        temp[ids >= 0] = ice[ids >= 0]
+        return {
+            "water_phase": phi,
+            "solid_body_id": ids,
+            "ice_fraction": self.coverage.to_numpy(),
+            "velocity_m_s": self.velocity_lattice.to_numpy() * self.scales.velocity_m_s,
+            "pressure_pa": (self.pressure_lattice.to_numpy() + self.reference_pressure.to_numpy()[None, :, None])
+            * self.scales.pressure_pa,
+            "temperature_c": temp,
+        }
+
+    def diagnostics(self):
+        """simulator.py:891-947."""
+        self._scan_stability()
+        control = self._read_control()
+        volume = self.water_volume.to_numpy()
+        rigid = self.rigid
+        centers = self.body_center.to_numpy() * self.scales.dx_m
+        result = {
+            "step": self.steps,
+            "time_s": self.physical_time_s,
+            "water_volume_m3": float(volume.sum(dtype=float)),
+            "water_volume_change_ratio": (
+                float(volume.sum(dtype=float) / self.initial_water_volume_m3 - 1)
+                if self.initial_water_volume_m3 > 0 else None
+            ),
+            "water_energy_j": float(self.water_energy.to_numpy().sum(dtype=float)),
+            "ice_sensible_energy_j": float(self.material_energy.to_numpy().sum(dtype=float)),
+            "maximum_fluid_speed_lattice": float(control.maximum_speed),
+            "peak_fluid_speed_lattice": float(control.peak_speed),
+            "phase_volume_change_ratio": float(
+                self.water_phase.to_numpy().sum(dtype=np.float64) * self.cell_volume / self.initial_phase_volume_m3 - 1
+            ),
+            "phase_volume_projection_m3": 0.0,
+            "phase_projection_momentum_x_n_s": 0.0,
+            "phase_projection_momentum_y_n_s": 0.0,
+            "phase_projection_momentum_z_n_s": 0.0,
+            "ice_mass_kg": float(rigid.mass.sum() * self.scales.mass_kg),
+            "melted_mass_kg": float(self.initial_ice_mass_kg.sum() - rigid.mass.sum() * self.scales.mass_kg),
+            "gas_repair_count": int(control.repair_count),
+            "phase_clip_volume_m3": float(control.phase_clip_volume) * self.scales.dx_m**3,
+            "thermal_remap_energy_j": float(control.remap_energy),
+            "discarded_melt_heat_j": float(control.discarded_heat),
+            "pending_refill_cells": self.pending_cells,
+        }
+        for b in range(self.body_count):
+            result[f"ice_{b}_mass_kg"] = float(rigid.mass[b] * self.scales.mass_kg)
+            for a, label in enumerate("xyz"):
+                result[f"ice_{b}_{label}_m"] = float(centers[b, a])
+                result[f"ice_{b}_v{label}_m_s"] = float(rigid.velocity[b, a] * self.scales.velocity_m_s)
+                result[f"ice_{b}_omega_{label}_s_1"] = float(rigid.omega[b, a] / self.scales.dt_s)
+            for a, label in enumerate("wxyz"):
+                result[f"ice_{b}_q{label}"] = float(rigid.q[b, a])
+        return result
+
+
+class _MatrixRows(DeviceField):
+    """[B, 9] device rows exposed as [B, 3, 3] matrices."""
+
+    @property
+    def shape(self):
+        return (self.t.shape[0], 3, 3)
+
+    def to_numpy(self):
+        return self.t.detach().cpu().numpy().reshape(-1, 3, 3).copy()
+
+    def from_numpy(self, value):
+        value = np.broadcast_to(np.asarray(value, dtype=np.float32), self.shape)
+        self.t.copy_(torch.from_numpy(np.ascontiguousarray(value.reshape(-1, 9))))

@@ -1,0 +1,194 @@
+"""HIP product against the FP64 oracle on identical seeded inputs.
+
+Each kernel test copies the oracle's state into the product (rounded once to
+FP32), perturbs it with seeded noise so that every term is active, runs the
+same reference method on both and compares every output field.  The step
+tests run both solvers from the same configuration for N coupled steps.
+
+Tolerances: FP32 kernels against FP64 arithmetic.  Single-kernel outputs
+must agree to a few FP32 ulps of the field scale (``rtol_scale``); after N
+coupled steps the HOME moments and phase must agree within 1e-5 of the field
+scale and the body pose/mass within 1e-4 (the north-star parity bar).
+"""
+
+import numpy as np
+import pytest
+
+from tests.cases3d import four_body_config, numerics_config, small_config
+
+pytestmark = pytest.mark.gpu
+
+STEP_FIELDS = ("home", "water_phase", "velocity_lattice", "pressure_lattice", "h", "water_volume", "water_energy",
+               "material_mass", "material_energy", "coverage")
+
+
+def scaled_error(actual, expected):
+    actual, expected = np.asarray(actual, float), np.asarray(expected, float)
+    scale = max(float(np.abs(expected).max()), 1e-30)
+    return float(np.abs(actual - expected).max()) / scale
+
+
+def assert_close(actual, expected, rtol_scale, name):
+    err = scaled_error(actual, expected)
+    assert err <= rtol_scale, f"{name}: max error {err:.3e} of field scale (limit {rtol_scale:.1e})"
+
+
+def pair(factory, **kwargs):
+    from oracle.iceflow3d_oracle import Oracle3D
+    from projecticy_amd.iceflow3d import IceFlow3D
+
+    cfg = factory(**kwargs)
+    return IceFlow3D(cfg), Oracle3D(cfg)
+
+
+def copy_state(dst, src):
+    """Oracle -> product, every restart field, rigid state and parity."""
+    fields = dst.fields()
+    for name, field in src.fields().items():
+        if name in fields and name != "capillary_stress":
+            fields[name].from_numpy(np.asarray(field.to_numpy()))
+    for name in ("origin", "q", "velocity", "omega", "com", "mass", "inertia", "lower", "upper"):
+        setattr(dst.rigid, name, getattr(src.rigid, name).copy())
+    dst.phase_parity[None] = src.phase_parity[None]
+    dst._body_state = np.concatenate(
+        [src.body_origin.to_numpy(), src.body_rotation.to_numpy().reshape(-1, 9), src.body_center.to_numpy(),
+         src.body_velocity.to_numpy(), src.body_angular_velocity.to_numpy()], axis=1).astype(np.float32)
+    dst.body_origin.from_numpy(src.body_origin.to_numpy())
+    dst.body_rotation.from_numpy(src.body_rotation.to_numpy())
+
+
+def perturb(sim, seed):
+    """Seeded noise on the fluid state, applied identically to both."""
+    rng = np.random.default_rng(seed)
+    ids = sim.solid_body_id.to_numpy()
+    fluid = ids == -1
+    u = sim.velocity_lattice.to_numpy()
+    u[fluid] += rng.uniform(-0.01, 0.01, (fluid.sum(), 3))
+    phi = sim.water_phase.to_numpy()
+    phi[fluid] = np.clip(phi[fluid] + rng.uniform(-0.05, 0.05, fluid.sum()), 0, 1)
+    home = sim.home.to_numpy()
+    home[fluid] += rng.normal(0, 1e-3, (fluid.sum(), 10))
+    flux = rng.uniform(-1e-4, 1e-4, (*sim.shape, 3))
+    return {"velocity_lattice": u, "water_phase": phi, "home": home, "phase_mass_flux": flux,
+            "body_velocity": rng.uniform(-2e-3, 2e-3, (sim.body_count, 3)),
+            "body_angular_velocity": rng.uniform(-2e-4, 2e-4, (sim.body_count, 3))}
+
+
+def prepared(factory, seed, **kwargs):
+    sim, ref = pair(factory, **kwargs)
+    noise = perturb(ref, seed)
+    for name, value in noise.items():
+        getattr(ref, name).from_numpy(value)
+    ref._gradients()
+    copy_state(sim, ref)
+    return sim, ref
+
+
+def compare(sim, ref, names, rtol_scale):
+    for name in names:
+        assert_close(getattr(sim, name).to_numpy(), getattr(ref, name).to_numpy(), rtol_scale, name)
+
+
+# ---------------------------------------------------------------- init
+@pytest.mark.parametrize("factory", [small_config, numerics_config, four_body_config])
+def test_initial_state_matches_oracle(factory):
+    sim, ref = pair(factory, phase_warmup_steps=25)
+    assert sim.phase_parity[None] == ref.phase_parity[None]
+    np.testing.assert_array_equal(sim.solid_body_id.to_numpy(), ref.solid_body_id.to_numpy())
+    np.testing.assert_array_equal(sim.boundary_flags(), ref.boundary_flags())
+    compare(sim, ref, ("water_phase", "home", "h", "phase_gradient_lattice", "coverage", "water_volume",
+                       "water_energy", "reference_pressure", "pressure_lattice", "home_acceleration",
+                       "force_density"), 2e-6)
+    np.testing.assert_allclose(sim.rigid.mass, ref.rigid.mass, rtol=1e-6)
+    np.testing.assert_allclose(sim.rigid.inertia, ref.rigid.inertia, rtol=1e-5, atol=1e-5)
+
+
+# ------------------------------------------------------------ kernels
+KERNELS = {
+    "gradients": (lambda s: s._gradients(), ("phase_gradient_lattice", "capillary_stress")),
+    "collide_phase": (lambda s: s._collide_phase(), ("h", "phase_mass_flux")),
+    "forces": (lambda s: s._forces(True, moment_pressure=True), ("force_density", "home_acceleration")),
+    "forces_macro": (lambda s: s._forces(False), ("force_density", "home_acceleration")),
+    "collide_momentum": (lambda s: s._collide_momentum(), ("home_post", "old_velocity", "old_pressure")),
+    "stream": (lambda s: (s._forces(True, True), s._collide_momentum(), s.hydrodynamic_force.fill(0),
+                          s.hydrodynamic_torque.fill(0), s._stream_home()),
+               ("home", "velocity_lattice", "hydrodynamic_force", "hydrodynamic_torque")),
+    "recover": (lambda s: s._recover_phase(recover_flow=True), ("water_phase", "h", "pressure_lattice")),
+    "synchronize_water": (lambda s: s._synchronize_water(), ("water_volume", "water_energy")),
+    "advection": (lambda s: (s._thermal_fluxes(True), s._apply_thermal_fluxes(s.scales.dt_s, True)),
+                  ("water_volume", "water_energy", "energy_flux", "volume_flux")),
+    "conduction": (lambda s: (s._body_conduction(s.scales.dt_s * 8), s._thermal_fluxes(False),
+                              s._apply_thermal_fluxes(s.scales.dt_s * 8, False), s._exchange_heat(s.scales.dt_s * 8),
+                              s._apply_melting()),
+                   ("material_heat", "material_mass", "material_energy", "water_energy")),
+}
+
+
+@pytest.mark.parametrize("kernel", list(KERNELS))
+@pytest.mark.parametrize("factory", [small_config, four_body_config])
+def test_kernel_matches_oracle(kernel, factory):
+    run, outputs = KERNELS[kernel]
+    sim, ref = prepared(factory, seed=91 + len(kernel))
+    run(ref)
+    run(sim)
+    for name in outputs:
+        assert_close(getattr(sim, name).to_numpy(), getattr(ref, name).to_numpy(), 2e-5, f"{kernel}:{name}")
+
+
+@pytest.mark.parametrize("factory", [small_config, four_body_config])
+def test_geometry_refresh_matches_oracle(factory):
+    sim, ref = prepared(factory, seed=17)
+    for s in (sim, ref):
+        s.rigid.origin[0] += [0.8, 0.35, 0.25]
+        s.rigid.upload()
+        s._refresh_geometry()
+    np.testing.assert_array_equal(sim.solid_body_id.to_numpy(), ref.solid_body_id.to_numpy())
+    np.testing.assert_array_equal(sim.previous_body_id.to_numpy(), ref.previous_body_id.to_numpy())
+    assert sim.pending_cells == ref.pending_cells
+    compare(sim, ref, ("coverage", "water_phase", "velocity_lattice", "pressure_lattice", "home", "h",
+                       "phase_gradient_lattice", "water_volume", "water_energy"), 2e-5)
+
+
+def test_mass_reduction_matches_oracle():
+    sim, ref = prepared(four_body_config, seed=5)
+    rng = np.random.default_rng(8)
+    mass = ref.material_mass.to_numpy() * rng.uniform(0.2, 1.0, ref.material_mass.shape)
+    for s in (sim, ref):
+        s.material_mass.from_numpy(mass)
+        s.rigid.update_mass()
+    np.testing.assert_allclose(sim.rigid.mass, ref.rigid.mass, rtol=2e-6)
+    np.testing.assert_allclose(sim.rigid.com, ref.rigid.com, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(sim.rigid.inertia, ref.rigid.inertia, rtol=1e-5, atol=1e-4)
+
+
+# --------------------------------------------------------------- steps
+@pytest.mark.parametrize("steps", [1, 4, 17])
+@pytest.mark.parametrize("factory", [small_config, numerics_config, four_body_config])
+def test_coupled_steps_match_oracle(factory, steps):
+    """N coupled steps from the same configuration: HOME moments and phase
+    within 1e-5 of scale, body pose and mass within 1e-4."""
+    sim, ref = pair(factory, phase_warmup_steps=50)
+    sim.step(steps)
+    ref.step(steps)
+    assert sim.steps == ref.steps == steps
+    np.testing.assert_array_equal(sim.solid_body_id.to_numpy(), ref.solid_body_id.to_numpy())
+    for name in ("home", "water_phase", "h"):
+        assert_close(getattr(sim, name).to_numpy(), getattr(ref, name).to_numpy(), 1e-5, name)
+    for name in ("velocity_lattice", "pressure_lattice", "water_energy", "water_volume", "material_mass"):
+        assert_close(getattr(sim, name).to_numpy(), getattr(ref, name).to_numpy(), 1e-4, name)
+    for name in ("origin", "velocity", "q", "omega", "mass"):
+        a, b = getattr(sim.rigid, name), getattr(ref.rigid, name)
+        np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-4 * max(1e-8, float(np.abs(b).max())), err_msg=name)
+
+
+def test_step_argument_validation_leaves_state_unchanged():
+    from projecticy_amd.iceflow3d import IceFlow3D
+
+    sim = IceFlow3D(small_config())
+    before = sim.home.to_numpy()
+    for bad in (-1, 1.5, True, "2"):
+        with pytest.raises(ValueError):
+            sim.step(bad)
+    sim.step(0)
+    np.testing.assert_array_equal(sim.home.to_numpy(), before)
+    assert sim.steps == 0

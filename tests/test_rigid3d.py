@@ -1,0 +1,267 @@
+"""Rigid bodies: the reference's fixture-driven contact tests
+(iceflow3d/tests/test_config_and_rigid.py:99-308) run against the product's
+native FP64 solver and the oracle's NumPy restatement, plus a direct
+cross-check of the two on the same states."""
+
+import json
+from pathlib import Path
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+
+from projecticy_amd.iceflow3d import IceFlowConfig
+from projecticy_amd.iceflow3d.rigid import (
+    RigidBodies,
+    advance_quaternion,
+    apply_contact_impulse,
+    apply_wall_contact_manifold,
+    rotation_matrix,
+    separating_axis,
+)
+from oracle.rigid3d_oracle import OracleRigid
+
+GOLDEN = Path(__file__).parent / "golden"
+SOLVERS = {"native": RigidBodies, "oracle": OracleRigid}
+
+
+def fine_grid(kind, budget):
+    """Four-body/wall state at fine-grid step 15963 (reference fixture)."""
+    state = json.loads((GOLDEN / "fine_grid_coupled_contact.json").read_text())
+    r = SOLVERS[kind].__new__(SOLVERS[kind])
+    r.s = SimpleNamespace(
+        shape=tuple(state["shape"]), body_count=state["body_count"], steps=state["source_step"] - 1,
+        config=SimpleNamespace(boundary_cells=state["boundary_cells"], body_contact_iterations=budget),
+    )
+    for name, value in state["arrays"].items():
+        setattr(r, name, np.array(value, dtype=float))
+    return r
+
+
+def late_floor(kind, iterations):
+    """One melted remnant before contact resolution in a late restart."""
+    state = json.loads((GOLDEN / "late_floor_contact.json").read_text())
+    r = SOLVERS[kind].__new__(SOLVERS[kind])
+    r.s = SimpleNamespace(
+        shape=tuple(state.pop("shape")), body_count=1, steps=state.pop("source_step") - 1,
+        config=SimpleNamespace(boundary_cells=1, body_contact_iterations=iterations),
+    )
+    for name, value in state.items():
+        setattr(r, name, np.array(value, dtype=float))
+    return r
+
+
+def energy(r):
+    total = 0.0
+    for b in range(len(r.mass)):
+        rot = rotation_matrix(r.q[b])
+        inertia = rot @ r.inertia[b] @ rot.T
+        total += (r.mass[b] * r.velocity[b] @ r.velocity[b] + r.omega[b] @ inertia @ r.omega[b]) / 2
+    return total
+
+
+@pytest.mark.parametrize("kind", list(SOLVERS))
+def test_fine_grid_coupled_contact_converges_with_default_budget(kind):
+    """test_config_and_rigid.py:100-141."""
+    old = fine_grid(kind, 32)
+    with pytest.raises(RuntimeError, match="contact projection did not converge"):
+        old.resolve_contacts()
+    repaired = fine_grid(kind, IceFlowConfig().body_contact_iterations)
+    before = energy(repaired)
+    repaired.resolve_contacts()
+    assert repaired.contact_residuals()[0] < 1e-7
+    assert repaired.contact_residuals()[1] < 1e-7
+    assert energy(repaired) <= before
+    larger = fine_grid(kind, 256)
+    larger.resolve_contacts()
+    for name in ("origin", "q", "velocity", "omega"):
+        np.testing.assert_array_equal(getattr(repaired, name), getattr(larger, name))
+
+
+@pytest.mark.parametrize("kind", list(SOLVERS))
+def test_final_contact_sweep_uses_actual_constraint_residual(kind):
+    """test_config_and_rigid.py:161-185."""
+    r = late_floor(kind, 32)
+    before_v, before_w = r.velocity.copy(), r.omega.copy()
+    rot = rotation_matrix(r.q[0])
+    inertia = rot @ r.inertia[0] @ rot.T
+    e0 = (r.mass[0] * before_v[0] @ before_v[0] + before_w[0] @ inertia @ before_w[0]) / 2
+    r.resolve_contacts()
+    penetration, closing = r.contact_residuals()
+    assert penetration < 1e-7 and closing < 1e-7
+    assert (r.mass[0] * r.velocity[0] @ r.velocity[0] + r.omega[0] @ inertia @ r.omega[0]) / 2 < e0
+    np.testing.assert_array_equal(r.velocity[:, [0, 2]], before_v[:, [0, 2]])
+    before = {k: v.copy() for k, v in vars(r).items() if isinstance(v, np.ndarray)}
+    r.contact_residuals()
+    for name, value in before.items():
+        np.testing.assert_array_equal(getattr(r, name), value)
+
+
+@pytest.mark.parametrize("kind", list(SOLVERS))
+def test_contact_budget_still_rejects_unresolved_penetration(kind):
+    """test_config_and_rigid.py:187-192."""
+    r = late_floor(kind, 1)
+    r.s.shape = (150, 8, 30)
+    with pytest.raises(RuntimeError, match="closing_speed="):
+        r.resolve_contacts()
+    assert r.contact_residuals()[0] > 1e-7
+
+
+@pytest.mark.parametrize("kind", list(SOLVERS))
+def test_low_inertia_remnant_uses_coupled_wall_impulses(kind):
+    """test_config_and_rigid.py:194-201."""
+    r = late_floor(kind, 32)
+    r.velocity *= 10
+    r.omega *= 10
+    r.resolve_contacts()
+    penetration, closing = r.contact_residuals()
+    assert penetration < 1e-7
+    assert closing < 1e-12
+
+
+@pytest.mark.parametrize("kind", list(SOLVERS))
+def test_joint_floor_pair_and_corner_constraints(kind):
+    """test_config_and_rigid.py:221-244."""
+    r = SOLVERS[kind].__new__(SOLVERS[kind])
+    r.s = SimpleNamespace(shape=(20, 20, 20), body_count=2, steps=0,
+                          config=SimpleNamespace(boundary_cells=1, body_contact_iterations=32))
+    r.origin = np.array([[8.0, 1.9, 10.0], [9.8, 1.9, 10.0]])
+    r.q = np.array([[1.0, 0.0, 0.0, 0.0]] * 2)
+    r.com = np.zeros((2, 3))
+    r.mass = np.ones(2)
+    r.inertia = np.array([np.eye(3) * 2 / 3] * 2)
+    r.velocity = np.array([[0.1, -0.1, 0.0], [-0.1, -0.1, 0.0]])
+    r.omega = np.array([[0.01, 0.02, 0.03], [-0.01, -0.02, -0.03]])
+    r.lower, r.upper = -np.ones((2, 3)), np.ones((2, 3))
+    r.resolve_contacts()
+    assert np.min(r.origin[:, 1]) >= 2 - 1e-8
+    assert r.origin[1, 0] - r.origin[0, 0] >= 2 - 1e-8
+    for b in range(2):
+        for x in (-1.0, 1.0):
+            for z in (-1.0, 1.0):
+                assert (r.velocity[b] + np.cross(r.omega[b], [x, -1.0, z]))[1] >= -1e-7
+
+
+@pytest.mark.parametrize("case", ["fine", "late", "late_fast"])
+def test_native_solver_matches_oracle_restatement(case):
+    """The C++ solver and the NumPy restatement agree on the same states."""
+    results = {}
+    for kind in SOLVERS:
+        if case == "fine":
+            r = fine_grid(kind, 128)
+        else:
+            r = late_floor(kind, 32)
+            if case == "late_fast":
+                r.velocity *= 10
+                r.omega *= 10
+        r.resolve_contacts()
+        results[kind] = (r.origin.copy(), r.velocity.copy(), r.omega.copy(), r.contact_residuals())
+    for a, b in zip(results["native"][:3], results["oracle"][:3]):
+        np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-12)
+
+
+def test_wall_manifold_is_passive_and_order_independent():
+    """test_config_and_rigid.py:203-219."""
+    normal = np.array([0.0, 1.0, 0.0])
+    arms = np.array([[-5.5, -4.5, -5.0], [-5.5, -4.5, 5.0], [4.5, -4.5, -5.0]])
+    inertia = np.diag([1.8, 2.6, 2.9])
+    v0, w0 = np.array([0.02, -1e-4, -0.01]), np.array([1e-5, 2e-5, -3e-5])
+    e0 = (v0 @ v0 + w0 @ inertia @ w0) / 2
+    results = []
+    for points in (arms, arms[::-1]):
+        v, w = v0.copy(), w0.copy()
+        apply_wall_contact_manifold(v, w, 1.0, np.linalg.inv(inertia), points, normal)
+        assert ((v + np.cross(w, arms)) @ normal).min() >= -1e-12
+        assert (v @ v + w @ inertia @ w) / 2 <= e0
+        np.testing.assert_array_equal(v[[0, 2]], v0[[0, 2]])
+        results.append(np.r_[v, w])
+    np.testing.assert_allclose(*results, rtol=1e-12, atol=1e-14)
+
+
+def test_quaternion_three_axis_rotation():
+    """test_config_and_rigid.py:246-254."""
+    q = np.array([1.0, 0.0, 0.0, 0.0])
+    omega = np.array([0.03, -0.04, 0.02])
+    for _ in range(100):
+        q = advance_quaternion(q, omega)
+    assert abs(np.linalg.norm(q) - 1) < 1e-12
+    r = rotation_matrix(q)
+    np.testing.assert_allclose(r.T @ r, np.eye(3), atol=1e-13)
+    np.testing.assert_allclose(r @ omega, omega, atol=1e-13)
+
+
+def test_pair_collision_conserves_momentum_without_restitution():
+    """test_config_and_rigid.py:256-284."""
+    va, vb = np.array([1.0, 0.3, -0.1]), np.array([-1.0, 0.0, 0.2])
+    wa, wb = np.array([0.0, 0.1, 0.2]), np.array([0.1, 0.0, 0.0])
+    ra, rb, n = np.array([1.0, 0.2, 0.3]), np.array([-1.0, 0.2, 0.3]), np.array([-1.0, 0.0, 0.0])
+    before = 2 * va + 3 * vb
+    ang0 = np.cross([-1.0, 0, 0], 2 * va) + np.cross([1.0, 0, 0], 3 * vb) + wa + wb
+    k0 = (2 * va @ va + 3 * vb @ vb + wa @ wa + wb @ wb) / 2
+    apply_contact_impulse(va, wa, 2.0, np.eye(3), ra, n, vb, wb, 3.0, np.eye(3), rb)
+    np.testing.assert_allclose(2 * va + 3 * vb, before, atol=1e-14)
+    ang1 = np.cross([-1.0, 0, 0], 2 * va) + np.cross([1.0, 0, 0], 3 * vb) + wa + wb
+    np.testing.assert_allclose(ang1, ang0, atol=1e-14)
+    assert abs((va + np.cross(wa, ra) - vb - np.cross(wb, rb)) @ n) < 1e-12
+    assert (2 * va @ va + 3 * vb @ vb + wa @ wa + wb @ wb) / 2 < k0
+
+
+def test_wall_impulse_includes_rotation():
+    """test_config_and_rigid.py:286-291."""
+    v, w = np.array([0.2, -1.0, 0.3]), np.array([0.1, 0.0, 0.2])
+    arm, n = np.array([1.0, -1.0, 0.5]), np.array([0.0, 1.0, 0.0])
+    apply_contact_impulse(v, w, 1.0, np.eye(3), arm, n)
+    assert abs((v + np.cross(w, arm)) @ n) < 1e-12
+    np.testing.assert_allclose(v[[0, 2]], [0.2, 0.3])
+
+
+def test_sat_checks_depth_and_rotated_boxes():
+    """test_config_and_rigid.py:293-308."""
+    eye, half = np.eye(3), np.ones(3)
+    assert separating_axis(np.zeros(3), eye, half, np.array([0.0, 0.0, 3.0]), eye, half) is None
+    hit = separating_axis(np.zeros(3), eye, half, np.array([0.0, 0.0, 1.5]), eye, half)
+    np.testing.assert_allclose(hit[0], [0, 0, -1])
+    assert abs(hit[1] - 0.5) < 1e-12
+    r = rotation_matrix(advance_quaternion([1, 0, 0, 0], np.array([0.3, 0.2, 0.1])))
+    assert separating_axis(np.zeros(3), eye, half, np.zeros(3), r, half) is not None
+
+
+def test_native_integration_matches_oracle_euler_update():
+    """Integration and quaternion update of the native solver against the
+    restated NumPy rigid.py:178-209 on a seeded state."""
+    from oracle.rigid3d_oracle import qstep, rot
+
+    rng = np.random.default_rng(4242)
+    B = 3
+    r = RigidBodies.__new__(RigidBodies)
+    r.s = SimpleNamespace(shape=(64, 64, 64), body_count=B, steps=0, scales=SimpleNamespace(dt_s=1e-5, dx_m=1e-3),
+                          config=SimpleNamespace(gravity=(0, -9.8, 0), linear_damping=0.99, angular_damping=0.98,
+                                                 boundary_cells=1, body_contact_iterations=8))
+    r.origin = rng.uniform(20, 40, (B, 3))
+    r.q = rng.normal(size=(B, 4))
+    r.q /= np.linalg.norm(r.q, axis=1)[:, None]
+    r.velocity = rng.normal(0, 0.01, (B, 3))
+    r.omega = rng.normal(0, 0.002, (B, 3))
+    r.com = rng.normal(0, 0.3, (B, 3))
+    r.mass = rng.uniform(20, 60, B)
+    a = rng.normal(size=(B, 3, 3))
+    r.inertia = np.einsum("bij,bkj->bik", a, a) + 40 * np.eye(3)
+    r.lower, r.upper = -np.full((B, 3), 4.0), np.full((B, 3), 4.0)
+    loads = rng.normal(0, 0.05, (B, 6)).astype(np.float32)
+    expected = {k: getattr(r, k).copy() for k in ("origin", "q", "velocity", "omega")}
+    g = np.array([0, -9.8, 0]) * 1e-10 / 1e-3
+    for b in range(B):
+        R = rot(expected["q"][b])
+        inertia = R @ r.inertia[b] @ R.T
+        centre = expected["origin"][b] + R @ r.com[b]
+        expected["velocity"][b] += loads[b, :3].astype(float) / r.mass[b] + g
+        w = expected["omega"][b]
+        expected["omega"][b] = w + np.linalg.solve(inertia, loads[b, 3:].astype(float) - np.cross(w, inertia @ w))
+        expected["velocity"][b] *= 0.99
+        expected["omega"][b] *= 0.98
+        centre += expected["velocity"][b]
+        expected["q"][b] = qstep(expected["q"][b], expected["omega"][b])
+        expected["origin"][b] = centre - rot(expected["q"][b]) @ r.com[b]
+    r.integrate(loads)
+    for name, value in expected.items():
+        np.testing.assert_allclose(getattr(r, name), value, rtol=1e-12, atol=1e-14, err_msg=name)
