@@ -51,6 +51,7 @@ typedef struct icy3d_control {
   int32_t repair_count;
   int32_t refill_lower[3], refill_upper[3];
   int32_t geometry_lower[3], geometry_upper[3];
+  int32_t tank_count, ice_count; /* lengths of the boundary-cell lists */
   float maximum_speed, peak_speed, phase_clip_volume;
   float remap_energy, discarded_heat, max_outflow;
   float loads[ICY3D_MAX_BODIES][6];                     /* hydrodynamic force, torque */
@@ -82,6 +83,9 @@ typedef struct icy3d_grid {
   float *material_mass, *material_energy, *material_heat; /* [bodies][mx][my][mz] */
   float *water_volume, *water_energy, *target_energy, *sampled_ice_t; /* [N] */
   float *energy_flux, *volume_flux;                /* [3][N] */
+  int32_t *tank_cells, *ice_cells;                 /* [N] flag-1 / flag-2 cell lists */
+  float* partials;                                 /* [partials_capacity] block partial sums */
+  int32_t partials_capacity, reserved;
   icy3d_control* control;
 } icy3d_grid;
 
@@ -117,17 +121,38 @@ int icy3d_stream_home(const icy3d_grid* g, void* stream);               /* home.
 /* Fused device part of IceFlow3D.step up to the stability check
  * (simulator.py:826-838): [gradients] collide_phase, forces, collide,
  * zero loads, stream, recover+scan, gradients. */
-int icy3d_flow_step(const icy3d_grid* g, int parity, int refresh_gradients, void* stream);
-int icy3d_repair_phase_layout(const icy3d_grid* g, int parity, void* stream); /* phase.py:163-184 */
+/* classify != 0 recomputes the link classes and boundary lists first
+ * (needed after any external edit of the labels). */
+int icy3d_flow_step(const icy3d_grid* g, int parity, int refresh_gradients, int classify, void* stream);
+
+/* Kernel-group timing of the fused flow step with HIP events on the same
+ * stream (no reference counterpart; the reference measures with Taichi's
+ * kernel profiler, iceflow3d/benchmarks/profile_complete_step.py:63-69).
+ * Groups: 0 phase collide, 1 classify+forces, 2 HOME collide, 3 HOME
+ * stream, 4 phase recovery+stability, 5 gradients. */
+#define ICY3D_PROFILE_GROUPS 6
+typedef struct icy3d_profile icy3d_profile;
+int icy3d_profile_create(icy3d_profile** out);
+int icy3d_profile_destroy(icy3d_profile* profile);
+int icy3d_flow_step_profiled(const icy3d_grid* g, int parity, int refresh_gradients, int classify,
+                             icy3d_profile* profile, void* stream);
+int icy3d_profile_collect(icy3d_profile* profile); /* waits for the last step; accumulates */
+int icy3d_profile_read(const icy3d_profile* profile, double* total_ms, int64_t* count);
+int icy3d_repair_phase_layout(const icy3d_grid* g, int parity, const int32_t lo[3], const int32_t hi[3],
+                              void* stream); /* phase.py:163-184 */
 int icy3d_export_phase(const icy3d_grid* g, int parity, float* out, void* stream); /* phase.py:186-191 */
 
 /* ---- geometry and refill (simulator.py:288-323, 589-736) ---------------- */
-int icy3d_rasterize(const icy3d_grid* g, void* stream);
-int icy3d_find_refill_changes(const icy3d_grid* g, void* stream);
+/* Box arguments (lo inclusive, hi exclusive) restrict a geometry pass to the
+ * region where ice can have moved; NULL means the whole grid. */
+int icy3d_rasterize(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream);
+int icy3d_find_refill_changes(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream);
 int icy3d_seed_refill_region(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream);
 int icy3d_propagate_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream);
 int icy3d_accept_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream);
-int icy3d_commit_refill(const icy3d_grid* g, void* stream);
+int icy3d_commit_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream);
+/* Rebuild the tank (flag 1) and ice (flag 2) cell lists from the flags. */
+int icy3d_build_boundary_lists(const icy3d_grid* g, void* stream);
 int icy3d_project_refill(const icy3d_grid* g, int parity, const int32_t lo[3], const int32_t hi[3], void* stream);
 
 /* ---- thermal (thermal.py:98-440) ---------------------------------------- */

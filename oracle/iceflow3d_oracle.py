@@ -548,9 +548,10 @@ class Oracle3D:
         p = -g * (np.cumsum(rho[::-1])[::-1] - 0.5 * rho)
         excess = np.where(ids == -1, (phase - ref[None, :, None]) * (1 - self.rho_air), 0.0)
         dynamic = -g * (np.cumsum(excess[:, ::-1, :], axis=1)[:, ::-1, :] - 0.5 * excess)
-        self._arrays["ref_density"][:] = rho
-        self._arrays["ref_pressure"][:] = p
-        self._arrays["pressure"][...] = dynamic
+        # simulator.py:381-383: stored as FP32 fields.
+        self._arrays["ref_density"][:] = rho.astype(np.float32)
+        self._arrays["ref_pressure"][:] = p.astype(np.float32)
+        self._arrays["pressure"][...] = dynamic.astype(np.float32)
 
     def check_stability(self, scanned=False):
         """simulator.py:807-821."""

@@ -134,21 +134,24 @@ class OracleRigid:
         self.upload()
 
     def upload(self):
-        """rigid.py:136-151 (kept FP64 in the oracle)."""
+        """rigid.py:136-151: the device receives the FP64 host state rounded
+        once to FP32 (``state.astype(np.float32)``)."""
         s = self.s
+        f32 = lambda v: np.asarray(v, dtype=np.float32).astype(float)  # noqa: E731
         for b in range(len(self.q)):
             r = rot(self.q[b])
-            s.body_origin[b] = self.origin[b]
-            s.body_rotation[b] = r
-            s.body_center[b] = self.origin[b] + r @ self.com[b]
-            s.body_velocity[b] = self.velocity[b]
-            s.body_angular_velocity[b] = self.omega[b]
+            s.body_origin[b] = f32(self.origin[b])
+            s.body_rotation[b] = f32(r)
+            s.body_center[b] = f32(self.origin[b] + r @ self.com[b])
+            s.body_velocity[b] = f32(self.velocity[b])
+            s.body_angular_velocity[b] = f32(self.omega[b])
 
     def update_mass(self, *, upload=True):
         """rigid.py:153-176."""
         s = self.s
         s._reduce_mass()
-        red = s.reduced_state.to_numpy()
+        # rigid.py:156-158: downloaded through an FP32 buffer.
+        red = s.reduced_state.to_numpy().astype(np.float32).astype(float)
         for b in range(len(red)):
             m, first = red[b, 0], red[b, 1:4]
             second = red[b, 4:13].reshape(3, 3)
@@ -170,7 +173,9 @@ class OracleRigid:
     def advance(self):
         """rigid.py:178-211."""
         s = self.s
-        forces, torques = s.hydrodynamic_force.to_numpy(), s.hydrodynamic_torque.to_numpy()
+        # rigid.py:180-182: loads are downloaded through an FP32 buffer.
+        forces = s.hydrodynamic_force.to_numpy().astype(np.float32).astype(float)
+        torques = s.hydrodynamic_torque.to_numpy().astype(np.float32).astype(float)
         gravity = np.array(s.config.gravity) * s.scales.dt_s**2 / s.scales.dx_m
         for b in range(len(self.mass)):
             if self.mass[b] <= 1e-15:

@@ -14,6 +14,7 @@ LIB_PATH = HERE / "libicy3d.so"
 MAX_BODIES = 32
 STATE_STRIDE = 21
 REDUCED_STRIDE = 19
+LIST_BLOCKS = 1024  # grid of the boundary-list kernels (kListBlocks)
 
 _f = ctypes.c_float
 _i = ctypes.c_int32
@@ -35,12 +36,19 @@ GRID_POINTERS = (
     "ref_density", "ref_pressure", "body", "material_size", "body_t0",
     "material_mass", "material_energy", "material_heat",
     "water_volume", "water_energy", "target_energy", "sampled_ice_t",
-    "energy_flux", "volume_flux", "control",
+    "energy_flux", "volume_flux", "tank_cells", "ice_cells", "partials",
 )
+GRID_TAIL_INTS = ("partials_capacity", "reserved")
 
 
 class Grid(ctypes.Structure):
-    _fields_ = [(n, _i) for n in GRID_INTS] + [(n, _f) for n in GRID_FLOATS] + [(n, _p) for n in GRID_POINTERS]
+    _fields_ = (
+        [(n, _i) for n in GRID_INTS]
+        + [(n, _f) for n in GRID_FLOATS]
+        + [(n, _p) for n in GRID_POINTERS]
+        + [(n, _i) for n in GRID_TAIL_INTS]
+        + [("control", _p)]
+    )
 
 
 class Control(ctypes.Structure):
@@ -49,6 +57,7 @@ class Control(ctypes.Structure):
         ("remaining", _i), ("repair_count", _i),
         ("refill_lower", _i * 3), ("refill_upper", _i * 3),
         ("geometry_lower", _i * 3), ("geometry_upper", _i * 3),
+        ("tank_count", _i), ("ice_count", _i),
         ("maximum_speed", _f), ("peak_speed", _f), ("phase_clip_volume", _f),
         ("remap_energy", _f), ("discarded_heat", _f), ("max_outflow", _f),
         ("loads", (_f * 6) * MAX_BODIES),
@@ -78,9 +87,10 @@ _GRID_CALLS = {
     "classify": [_I3, _I3], "gradients": [_I3, _I3], "refill_gradients": [_I3, _I3],
     "collide_phase": [_i, _i], "recover_phase": [_i, _i], "recover_pressure": [],
     "scan_stability": [], "forces": [_i, _i], "collide_momentum": [], "stream_home": [],
-    "flow_step": [_i, _i], "repair_phase_layout": [_i], "export_phase": [_i, _p],
-    "rasterize": [], "find_refill_changes": [], "seed_refill_region": [_I3, _I3],
-    "propagate_refill": [_I3, _I3], "accept_refill": [_I3, _I3], "commit_refill": [],
+    "flow_step": [_i, _i, _i], "repair_phase_layout": [_i, _I3, _I3], "export_phase": [_i, _p],
+    "rasterize": [_I3, _I3], "find_refill_changes": [_I3, _I3], "seed_refill_region": [_I3, _I3],
+    "propagate_refill": [_I3, _I3], "accept_refill": [_I3, _I3], "commit_refill": [_I3, _I3],
+    "build_boundary_lists": [],
     "project_refill": [_i, _I3, _I3],
     "synchronize_water": [], "thermal_fluxes": [_i], "apply_thermal_fluxes": [_f, _i],
     "measure_outflow": [], "body_conduction": [_f], "exchange_heat": [_f, _I3, _I3],
@@ -117,6 +127,13 @@ def lib():
     handle.icy3d_probe_phase.argtypes = [_p, _i, _f, _f, _p, _p]
     handle.icy3d_probe_viscosity.argtypes = [_p, _i, _p, _i, _f, _f, _f, _p, _p]
     for name in ("probe_home", "probe_phase", "probe_viscosity"):
+        getattr(handle, "icy3d_" + name).restype = ctypes.c_int
+    handle.icy3d_profile_create.argtypes = [ctypes.POINTER(_p)]
+    handle.icy3d_profile_destroy.argtypes = [_p]
+    handle.icy3d_profile_collect.argtypes = [_p]
+    handle.icy3d_profile_read.argtypes = [_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
+    handle.icy3d_flow_step_profiled.argtypes = [_GRID, _i, _i, _i, _p, _p]
+    for name in ("profile_create", "profile_destroy", "profile_collect", "profile_read", "flow_step_profiled"):
         getattr(handle, "icy3d_" + name).restype = ctypes.c_int
     from .iceflow3d.rigid import _RigidArrays
 

@@ -272,9 +272,84 @@ __device__ __forceinline__ int wave_sum_int(int v) {
   return v;
 }
 __device__ __forceinline__ bool wave_leader() { return (threadIdx.x & 63) == 0; }
-// Non-negative float max through the ordered integer image.
+// Non-negative float max through the ordered integer image; the relaxed
+// pre-read skips the atomic once the running maximum is already larger, so
+// a full-grid reduction issues only a handful of atomics.
 __device__ __forceinline__ void atomic_max_nonneg(float* address, float value) {
-  atomicMax(reinterpret_cast<int*>(address), __float_as_int(value));
+  if (value > *reinterpret_cast<volatile float*>(address))
+    atomicMax(reinterpret_cast<int*>(address), __float_as_int(value));
+}
+
+// Block-wide sum (blockDim.x a multiple of 64, at most 1024); every thread
+// of the block must call it.  The result is valid in thread 0 only.
+__device__ __forceinline__ float block_sum(float v) {
+  __shared__ float wave_part[16];
+  v = wave_sum(v);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) wave_part[wave] = v;
+  __syncthreads();
+  float total = 0.0f;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) total += wave_part[w];
+  __syncthreads();
+  return total;
+}
+
+}  // namespace icy
+
+namespace {
+// Second stage of a deterministic sum: one block folds the per-block
+// partials in a fixed order and adds the total to *target.
+__global__ void __launch_bounds__(256) k_finish_sum(const float* __restrict__ partials, int count, float* target) {
+  float v = 0.0f;
+  for (int t = threadIdx.x; t < count; t += blockDim.x) v += partials[t];
+  const float total = icy::block_sum(v);
+  if (threadIdx.x == 0) *target += total;
+}
+}  // namespace
+
+namespace icy {
+
+// Grid-stride iteration over a device-resident cell list.
+#define ICY_FOR_LIST(list, count_expr, var)                                                       \
+  for (int _t = blockIdx.x * blockDim.x + threadIdx.x, _count = (count_expr); _t < _count;       \
+       _t += gridDim.x * blockDim.x)                                                              \
+    if (const int var = (list)[_t]; true)
+
+constexpr unsigned kListBlocks = 1024;
+// Boundary kernels give each listed cell a group of 32 lanes, one lane per
+// D3Q27 direction (lanes 27..31 idle), and reduce within the group.
+constexpr int kCellLanes = 32;
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int off = kCellLanes / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kCellLanes);
+  return v;
+}
+__device__ __forceinline__ int block_sum_int(int v) {
+  __shared__ int wave_part_int[16];
+  v = wave_sum_int(v);
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) wave_part_int[wave] = v;
+  __syncthreads();
+  int total = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) total += wave_part_int[w];
+  __syncthreads();
+  return total;
+}
+// Partial load slots written by each boundary-stream block.
+constexpr int kLoadSlots = ICY3D_MAX_BODIES * 6;
+
+// Wave-aggregated append of flagged cells to a list (one atomic per wave).
+__device__ __forceinline__ void wave_append(bool take, int value, int32_t* list, int32_t* count) {
+  const unsigned long long mask = __ballot(take);
+  if (mask == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (take) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = value;
 }
 
 }  // namespace icy

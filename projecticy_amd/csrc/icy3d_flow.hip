@@ -10,6 +10,14 @@
 // bulk cells, which keeps register pressure of the hot kernels low.
 #include "icy3d_device.h"
 
+// Event pairs around each kernel group of the fused flow step.
+struct icy3d_profile {
+  hipEvent_t events[ICY3D_PROFILE_GROUPS + 1];
+  double total_ms[ICY3D_PROFILE_GROUPS] = {};
+  int64_t count = 0;
+  int pending = 0;
+};
+
 namespace icy {
 
 // ---------------------------------------------------------------------------
@@ -80,6 +88,96 @@ __global__ void k_classify(icy3d_grid g, Box box) {
   Cell c;
   if (!box_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
   g.flag[c.n] = (int8_t)boundary_flag(g, c);
+}
+// Compact the tank (flag 1) and ice (flag 2) cells into lists so that the
+// boundary kernels launch over the boundary only.  Three passes (count per
+// block, scan of block counts, scatter) give lists in cell order without a
+// contended global counter.  Block counts live in the partials buffer.
+constexpr int kCompactCells = 4096;  // cells per compaction block (16 per thread)
+__global__ void __launch_bounds__(256) k_count_flags(icy3d_grid g) {
+  const int N = (int)cells(g);
+  int tank = 0, ice = 0;
+  for (int it = 0; it < kCompactCells / 256; ++it) {
+    const int n = blockIdx.x * kCompactCells + it * 256 + threadIdx.x;
+    const int flag = n < N ? g.flag[n] : -1;
+    tank += flag == 1;
+    ice += flag == 2;
+  }
+  const int tanks = block_sum_int(tank);
+  const int ices = block_sum_int(ice);
+  int* counts = reinterpret_cast<int*>(g.partials);
+  if (threadIdx.x == 0) {
+    counts[2 * blockIdx.x] = tanks;
+    counts[2 * blockIdx.x + 1] = ices;
+  }
+}
+// One block of 1024 threads turns block counts into exclusive offsets.
+__global__ void __launch_bounds__(1024) k_scan_flag_counts(icy3d_grid g, int blocks) {
+  __shared__ int tank_sum[1024], ice_sum[1024];
+  int* counts = reinterpret_cast<int*>(g.partials);
+  const int per = (blocks + blockDim.x - 1) / blockDim.x;
+  const int begin = threadIdx.x * per, end = min(blocks, begin + per);
+  int tanks = 0, ices = 0;
+  for (int b = begin; b < end; ++b) {
+    tanks += counts[2 * b];
+    ices += counts[2 * b + 1];
+  }
+  tank_sum[threadIdx.x] = tanks;
+  ice_sum[threadIdx.x] = ices;
+  __syncthreads();
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    int t = threadIdx.x >= (unsigned)off ? tank_sum[threadIdx.x - off] : 0;
+    int i = threadIdx.x >= (unsigned)off ? ice_sum[threadIdx.x - off] : 0;
+    __syncthreads();
+    tank_sum[threadIdx.x] += t;
+    ice_sum[threadIdx.x] += i;
+    __syncthreads();
+  }
+  int tank_base = tank_sum[threadIdx.x] - tanks, ice_base = ice_sum[threadIdx.x] - ices;
+  for (int b = begin; b < end; ++b) {
+    const int t = counts[2 * b], i = counts[2 * b + 1];
+    counts[2 * b] = tank_base;
+    counts[2 * b + 1] = ice_base;
+    tank_base += t;
+    ice_base += i;
+  }
+  if (threadIdx.x == blockDim.x - 1) {
+    g.control->tank_count = tank_sum[threadIdx.x];
+    g.control->ice_count = ice_sum[threadIdx.x];
+  }
+}
+__device__ __forceinline__ void block_scatter(bool take, int value, int base, int32_t* list) {
+  __shared__ int wave_base[16];
+  const unsigned long long mask = __ballot(take);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wave_base[wave] = __popcll(mask);
+  __syncthreads();
+  int offset = base;
+  for (int w = 0; w < wave; ++w) offset += wave_base[w];
+  if (take) list[offset + __popcll(mask & ((1ull << lane) - 1ull))] = value;
+  __syncthreads();
+}
+__global__ void __launch_bounds__(256) k_scatter_flags(icy3d_grid g) {
+  const int N = (int)cells(g);
+  const int* offsets = reinterpret_cast<const int*>(g.partials);
+  int tank_base = offsets[2 * blockIdx.x], ice_base = offsets[2 * blockIdx.x + 1];
+  __shared__ int totals[2];
+  for (int it = 0; it < kCompactCells / 256; ++it) {
+    const int n = blockIdx.x * kCompactCells + it * 256 + threadIdx.x;
+    const int flag = n < N ? g.flag[n] : -1;
+    block_scatter(flag == 1, n, tank_base, g.tank_cells);
+    block_scatter(flag == 2, n, ice_base, g.ice_cells);
+    const int tanks = block_sum_int(flag == 1 ? 1 : 0);
+    const int ices = block_sum_int(flag == 2 ? 1 : 0);
+    if (threadIdx.x == 0) {
+      totals[0] = tanks;
+      totals[1] = ices;
+    }
+    __syncthreads();
+    tank_base += totals[0];
+    ice_base += totals[1];
+    __syncthreads();
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -239,10 +337,34 @@ __global__ void k_gradients_regular(icy3d_grid g, Box box) {
   else if (flag < 0)
     store_gradient(g, c.n, 0.0f, 0.0f, 0.0f);
 }
-__global__ void k_gradients_general(icy3d_grid g, Box box) {
-  Cell c;
-  if (!box_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
-  if (g.flag[c.n] == 2) general_gradient(g, c);
+// gradients.py:116-145 with one lane per direction (wall links sample the
+// mirror), summed within the cell's 32-lane group.
+__global__ void __launch_bounds__(256) k_gradients_general_list(icy3d_grid g, Box box) {
+  const int lane = threadIdx.x & (kCellLanes - 1);
+  const int groups = blockDim.x / kCellLanes;
+  const int count = g.control->ice_count;
+  for (int t = blockIdx.x * groups + threadIdx.x / kCellLanes; t < count; t += gridDim.x * groups) {
+    const int n = g.ice_cells[t];
+    const Cell c = cell_of(g, n);
+    if (c.i < box.lo[0] || c.i >= box.hi[0] || c.j < box.lo[1] || c.j >= box.hi[1] || c.k < box.lo[2] ||
+        c.k >= box.hi[2])
+      continue;
+    const float centre = g.phi[n];
+    float rx = 0.0f, ry = 0.0f, rz = 0.0f;
+    if (lane < 27 && lane != 13) {
+      const int dx = cx(lane), dy = cy(lane), dz = cz(lane);
+      const int i = c.i + dx, j = c.j + dy, k = c.k + dz;
+      const float sample = fluid_at(g, i, j, k) ? g.phi[lin(g, i, j, k)] : wall_phase_sample(g, c.i, c.j, c.k, dx, dy, dz);
+      const float d = sample - centre, w = w27(lane);
+      rx = w * (float)dx * d;
+      ry = w * (float)dy * d;
+      rz = w * (float)dz * d;
+    }
+    rx = group_sum(rx);
+    ry = group_sum(ry);
+    rz = group_sum(rz);
+    if (lane == 0) store_gradient(g, n, 3.0f * rx, 3.0f * ry, 3.0f * rz);
+  }
 }
 __global__ void k_write_gradient_all(icy3d_grid g) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -373,8 +495,8 @@ __global__ void k_recover_phase(icy3d_grid g, int parity, int recover_flow) {
     }
   }
   if (!isfinite(clip)) clip = 0.0f;
-  clip = wave_sum(clip);
-  if (wave_leader() && clip != 0.0f) atomicAdd(&g.control->phase_clip_volume, clip);
+  const float block_clip = block_sum(clip);
+  if (threadIdx.x == 0) g.partials[blockIdx.x] = block_clip;
   if (recover_flow && g.check_stability) commit_stability(g, st);
 }
 
@@ -398,11 +520,11 @@ __global__ void k_fold_peak(icy3d_grid g) {
 }
 
 // phase.py:163-184
-__global__ void k_repair_layout(icy3d_grid g) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_repair_layout(icy3d_grid g, Box box) {
+  Cell c;
+  if (!box_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
   const size_t N = cells(g);
-  if (n >= (int)N) return;
-  const Cell c = cell_of(g, n);
+  const int n = c.n;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const int q = 2 * a + 1;
@@ -693,11 +815,8 @@ __device__ void acceleration_at(const icy3d_grid& g, const Cell& c, const float 
 }
 
 template <int MODE>
-__global__ void k_forces(icy3d_grid g, int with_flux, int moment_pressure) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void forces_cell(const icy3d_grid& g, int n, int with_flux, int moment_pressure) {
   const size_t N = cells(g);
-  const int want = MODE == kInterior ? 0 : (MODE == kTank ? 1 : 2);
-  if (n >= (int)N || g.flag[n] != want) return;
   const Cell c = cell_of(g, n);
   float f[3], a[3];
   force_at<MODE>(g, c, with_flux != 0, f);
@@ -706,6 +825,126 @@ __global__ void k_forces(icy3d_grid g, int with_flux, int moment_pressure) {
   for (int e = 0; e < 3; ++e) {
     g.force[e * N + n] = f[e];
     g.accel[e * N + n] = a[e];
+  }
+}
+__global__ void k_forces_interior(icy3d_grid g, int with_flux, int moment_pressure) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= (int)cells(g) || g.flag[n] != 0) return;
+  forces_cell<kInterior>(g, n, with_flux, moment_pressure);
+}
+__global__ void k_forces_tank(icy3d_grid g, int with_flux, int moment_pressure) {
+  ICY_FOR_LIST(g.tank_cells, g.control->tank_count, n) forces_cell<kTank>(g, n, with_flux, moment_pressure);
+}
+// simulator.py:418-519 and home.py:127-197 for ice-adjacent cells, one lane
+// per link: capillary (fluid link or mirrored stress), product-rule pressure
+// term, donor phase-flux term (lanes 0..5) and the six neighbour velocities
+// of the viscous derivative (lanes 0..5); lane 0 assembles the result.
+__global__ void __launch_bounds__(256) k_forces_general(icy3d_grid g, int with_flux, int moment_pressure) {
+  const int lane = threadIdx.x & (kCellLanes - 1);
+  const int groups = blockDim.x / kCellLanes;
+  const int count = g.control->ice_count;
+  const size_t N = cells(g);
+  for (int t = blockIdx.x * groups + threadIdx.x / kCellLanes; t < count; t += gridDim.x * groups) {
+    const int n = g.ice_cells[t];
+    const Cell c = cell_of(g, n);
+    const float phi = g.phi[n], rho = density(g, phi);
+    float f[3] = {0.0f, 0.0f, 0.0f}, acc[3] = {0.0f, 0.0f, 0.0f}, nb[3] = {0.0f, 0.0f, 0.0f};
+    if (lane < 27 && lane != 13) {
+      const int dx = cx(lane), dy = cy(lane), dz = cz(lane);
+      const int i = c.i + dx, j = c.j + dy, k = c.k + dz;
+      const bool link = fluid_at(g, i, j, k);
+      const float w = w27(lane);
+      if (g.sigma > 0.0f) {
+        if (link) {
+          float v[3];
+          load_grad(g, lin(g, i, j, k), v);
+          const float gg = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+          const float gc = dx * v[0] + dy * v[1] + dz * v[2];
+          const float coef = 4.5f * g.sigma * g.width * w;
+          f[0] = coef * (gg * dx - gc * v[0]);
+          f[1] = coef * (gg * dy - gc * v[1]);
+          f[2] = coef * (gg * dz - gc * v[2]);
+        } else {
+          float st[9];
+          capillary_neighbor(g, c.i, c.j, c.k, dx, dy, dz, st);
+#pragma unroll
+          for (int e = 0; e < 3; ++e) f[e] = 3.0f * w * (st[3 * e] * dx + st[3 * e + 1] * dy + st[3 * e + 2] * dz);
+        }
+      }
+      if (link) {
+        const int m = lin(g, i, j, k);
+        const float rj = density(g, g.phi[m]);
+        const float term = moment_pressure ? w * g.home[m] * ((rho - rj) / rho)
+                                           : 3.0f * w * g.pressure[m] * (1.0f / rj - 1.0f / rho);
+        acc[0] = dx * term;
+        acc[1] = dy * term;
+        acc[2] = dz * term;
+      }
+    }
+    if (lane < 6) {
+      const int axis = lane >> 1, d = 2 * (lane & 1) - 1;
+      const int ex = (axis == 0) * d, ey = (axis == 1) * d, ez = (axis == 2) * d;
+      if (with_flux && fluid_at(g, c.i + ex, c.j + ey, c.k + ez)) {
+        const int m = lin(g, c.i + ex, c.j + ey, c.k + ez);
+        const float mass_flux = 0.5f * (g.flux[axis * N + n] + g.flux[axis * N + m]);
+        const bool own = (float)d * mass_flux >= 0.0f;
+        float fl[3];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) fl[e] = mass_flux * ((own ? g.u[e * N + n] : g.u[e * N + m]) - g.u[e * N + n]);
+        if (g.flux_limit > 0.0f) {
+          const float budget = g.flux_limit / 6.0f * fminf(rho, density(g, g.phi[m]));
+          const float scale = fminf(1.0f, budget / (fabsf(fl[0]) + fabsf(fl[1]) + fabsf(fl[2]) + 1e-20f));
+#pragma unroll
+          for (int e = 0; e < 3; ++e) fl[e] *= scale;
+        }
+#pragma unroll
+        for (int e = 0; e < 3; ++e) f[e] -= (float)d * fl[e];
+      }
+      stress_neighbor_velocity(g, c.i, c.j, c.k, ex, ey, ez, nb);
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      f[e] = group_sum(f[e]);
+      acc[e] = group_sum(acc[e]);
+    }
+    // Neighbour velocities for the centred derivative: lane 2a+1 is +e_a.
+    float plus[3][3], minus[3][3];
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis)
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        minus[axis][e] = __shfl(nb[e], (threadIdx.x & ~(kCellLanes - 1)) % 64 + 2 * axis, 64);
+        plus[axis][e] = __shfl(nb[e], (threadIdx.x & ~(kCellLanes - 1)) % 64 + 2 * axis + 1, 64);
+      }
+    if (lane == 0) {
+      const float gy = g.gravity_y;
+      const float tw = water_temperature(g, n);
+      float force[3] = {f[0], f[1], f[2]};
+      force[1] += (rho - g.ref_density[c.j]) * gy - fmaxf(0.0f, 2.0f * phi - 1.0f) * g.expansion * (tw - g.water_t0) * gy;
+      float a[3] = {force[0] / rho + acc[0], force[1] / rho + acc[1], force[2] / rho + acc[2]};
+      float gr[3];
+      load_grad(g, n, gr);
+      const float dr = 1.0f - g.rho_air;
+      gr[0] *= dr;
+      gr[1] *= dr;
+      gr[2] *= dr;
+      if (gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2] > 0.0f) {
+        float G[3][3];
+#pragma unroll
+        for (int axis = 0; axis < 3; ++axis)
+#pragma unroll
+          for (int e = 0; e < 3; ++e) G[e][axis] = 0.5f * (plus[axis][e] - minus[axis][e]);
+        const float coef = lattice_nu(g, n) / rho;
+#pragma unroll
+        for (int e = 0; e < 3; ++e)
+          a[e] += coef * ((G[e][0] + G[0][e]) * gr[0] + (G[e][1] + G[1][e]) * gr[1] + (G[e][2] + G[2][e]) * gr[2]);
+      }
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        g.force[e * N + n] = force[e];
+        g.accel[e * N + n] = a[e];
+      }
+    }
   }
 }
 
@@ -894,123 +1133,245 @@ __global__ void __launch_bounds__(256) k_stream_bulk(icy3d_grid g) {
   store_streamed(g, n, s);
 }
 
-// home.py:213-254: moving-ice reflection; accumulates body force/torque.
-__device__ float home_reflection(const icy3d_grid& g, const Cell& c, int outgoing, const Moments& post_i,
-                                 const Moments& home_i, float load[][6], int* load_body, int& loads) {
+// ---------------------------------------------------------------------------
+// Tiled HOME pull stream (home.py:286-321 for flags 0/1 and solid cells).
+//
+// A block owns a (4 j) x (64 k) column of cells and sweeps a chunk of 16 x
+// slabs.  Each x slab of post-collision moments (with a one-cell j/k halo
+// and the labels) is staged once in LDS.  A thread reads the 9 in-plane
+// neighbours (j+dy, k+dz) of slab s once and feeds three cells of its
+// column: x = s+1 (c_x = +1), x = s (c_x = 0) and x = s-1 (c_x = -1), so
+// neighbour reads drop from 27 to 9 moment vectors per cell.  Cell s-1 is
+// complete after slab s and is written then.  A non-fluid neighbour gives
+// the halfway bounce-back R_q(p, -u, S) = R_{-q}(p, u, S) of the cell's own
+// moments; for c_x = +1 the cell's own slab arrives one step later, so those
+// links are remembered in a 9-bit mask.
+constexpr int kTileJ = 4, kTileK = 64, kTileX = 16;
+constexpr int kHaloJ = kTileJ + 2, kHaloK = kTileK + 2, kSlabCells = kHaloJ * kHaloK;
+
+template <int CX, int DY, int DZ>
+__device__ __forceinline__ void pull_slab_link(const Moments& m, bool fluid, const Moments& own_cur,
+                                               const Moments& own_prev, MomentSums& prev, MomentSums& cur,
+                                               MomentSums& next, unsigned& defer) {
+  // Direction pulled by the cell at x = s + CX from neighbour (s, j+DY, k+DZ).
+  constexpr int Q = 9 * (CX + 1) + 3 * (-DY + 1) + (-DZ + 1);
+  if (fluid) {
+    const float v = reconstruct<Q>(m);
+    if (CX == 1) accumulate<Q>(next, v);
+    if (CX == 0) accumulate<Q>(cur, v);
+    if (CX == -1) accumulate<Q>(prev, v);
+  } else {
+    if (CX == 1) defer |= 1u << ((DY + 1) * 3 + (DZ + 1));
+    if (CX == 0) accumulate<Q>(cur, reconstruct<26 - Q>(own_cur));
+    if (CX == -1) accumulate<Q>(prev, reconstruct<26 - Q>(own_prev));
+  }
+}
+template <int DY, int DZ>
+__device__ __forceinline__ void apply_deferred(unsigned defer, const Moments& own_cur, MomentSums& cur) {
+  constexpr int Q = 9 * 2 + 3 * (-DY + 1) + (-DZ + 1);  // c_x = +1 link pulled last step
+  if (defer & (1u << ((DY + 1) * 3 + (DZ + 1)))) accumulate<Q>(cur, reconstruct<26 - Q>(own_cur));
+}
+template <int DY, int DZ>
+__device__ __forceinline__ void pull_neighbour(const float (*sm)[kSlabCells], const int8_t* sid, int base,
+                                               const Moments& own_cur, const Moments& own_prev, MomentSums& prev,
+                                               MomentSums& cur, MomentSums& next, unsigned& defer) {
+  const int idx = base + DY * kHaloK + DZ;
+  const bool fluid = sid[idx] == -1;
+  Moments m;
+#pragma unroll
+  for (int t = 0; t < 10; ++t) m.m[t] = sm[t][idx];
+  pull_slab_link<1, DY, DZ>(m, fluid, own_cur, own_prev, prev, cur, next, defer);
+  pull_slab_link<0, DY, DZ>(m, fluid, own_cur, own_prev, prev, cur, next, defer);
+  pull_slab_link<-1, DY, DZ>(m, fluid, own_cur, own_prev, prev, cur, next, defer);
+}
+
+__device__ __forceinline__ void finish_streamed_cell(const icy3d_grid& g, int x, int j, int k, const MomentSums& s) {
+  if (x < 0 || x >= g.nx || j >= g.ny || k >= g.nz) return;
+  const int n = lin(g, x, j, k);
+  const int flag = g.flag[n];
+  if (flag == 0 || flag == 1) {
+    store_streamed(g, n, s);
+  } else if (flag < 0) {
+    const size_t N = cells(g);
+    float v[3] = {0.0f, 0.0f, 0.0f};
+    const int b = g.id[n];
+    if (b >= 0) body_velocity_at(g, x + 0.5f, j + 0.5f, k + 0.5f, b, v);
+    g.u[n] = v[0];
+    g.u[N + n] = v[1];
+    g.u[2 * N + n] = v[2];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_stream_tiled(icy3d_grid g) {
+  __shared__ float sm[10][kSlabCells];
+  __shared__ int8_t sid[kSlabCells];
+  const size_t N = cells(g);
+  const int tk = threadIdx.x & (kTileK - 1), tj = threadIdx.x / kTileK;
+  const int k0 = blockIdx.x * kTileK, j0 = blockIdx.y * kTileJ, x0 = blockIdx.z * kTileX;
+  const int j = j0 + tj, k = k0 + tk;
+  const int base = (tj + 1) * kHaloK + (tk + 1);
+  const MomentSums zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  MomentSums prev = zero, cur = zero, next = zero;
+  Moments own_prev;
+#pragma unroll
+  for (int t = 0; t < 10; ++t) own_prev.m[t] = 0.0f;
+  unsigned defer_prev = 0;
+  const int x_end = min(x0 + kTileX, g.nx);
+  for (int s = x0 - 1; s <= x_end; ++s) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < kSlabCells; t += blockDim.x) {
+      const int gj = j0 - 1 + t / kHaloK, gk = k0 - 1 + t % kHaloK;
+      const bool in = s >= 0 && s < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz;
+      const int n = in ? (s * g.ny + gj) * g.nz + gk : 0;
+      const int8_t label = in ? g.id[n] : (int8_t)-2;
+      sid[t] = label;
+#pragma unroll
+      for (int m = 0; m < 10; ++m) sm[m][t] = label == -1 ? g.home_post[m * N + n] : 0.0f;
+    }
+    __syncthreads();
+    Moments own_cur;
+#pragma unroll
+    for (int t = 0; t < 10; ++t) own_cur.m[t] = sm[t][base];
+    apply_deferred<-1, -1>(defer_prev, own_cur, cur);
+    apply_deferred<-1, 0>(defer_prev, own_cur, cur);
+    apply_deferred<-1, 1>(defer_prev, own_cur, cur);
+    apply_deferred<0, -1>(defer_prev, own_cur, cur);
+    apply_deferred<0, 0>(defer_prev, own_cur, cur);
+    apply_deferred<0, 1>(defer_prev, own_cur, cur);
+    apply_deferred<1, -1>(defer_prev, own_cur, cur);
+    apply_deferred<1, 0>(defer_prev, own_cur, cur);
+    apply_deferred<1, 1>(defer_prev, own_cur, cur);
+    unsigned defer = 0;
+    pull_neighbour<-1, -1>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
+    pull_neighbour<-1, 0>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
+    pull_neighbour<-1, 1>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
+    pull_neighbour<0, -1>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
+    pull_neighbour<0, 0>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
+    pull_neighbour<0, 1>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
+    pull_neighbour<1, -1>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
+    pull_neighbour<1, 0>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
+    pull_neighbour<1, 1>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
+    if (s - 1 >= x0) finish_streamed_cell(g, s - 1, j, k, prev);
+    prev = cur;
+    cur = next;
+    next = zero;
+    own_prev = own_cur;
+    defer_prev = defer;
+  }
+}
+
+// home.py:213-254: one reflected population at an ice or tank link.  For an
+// ice link it also returns the link's force and torque on body ``body``.
+__device__ float reflect_link(const icy3d_grid& g, const Cell& c, int outgoing, const Moments& post_i,
+                              int& body, float load[6]) {
   const size_t N = cells(g);
   const int dx = cx(outgoing), dy = cy(outgoing), dz = cz(outgoing);
   const int ji = c.i + dx, jj = c.j + dy, jk = c.k + dz;
   const float raw = reconstruct_rt(outgoing, post_i);
-  float reflected = raw;
-  if (inside(g, ji, jj, jk)) {
-    const int m = lin(g, ji, jj, jk);
-    const int b = g.id[m];
-    if (b >= 0) {
-      const float eta = link_fraction(g, c.n, m);
-      const float px = c.i + 0.5f + eta * dx, py = c.j + 0.5f + eta * dy, pz = c.k + 0.5f + eta * dz;
-      float ub[3];
-      body_velocity_at(g, px, py, pz, b, ub);
-      const float p = home_i.m[0];
-      const float neq = reconstruct_rt(outgoing, home_i) -
-                        reconstruct_rt(outgoing, equilibrium_moments(p, g.old_u[c.n], g.old_u[N + c.n], g.old_u[2 * N + c.n]));
-      const int incoming = 26 - outgoing;
-      reflected = (reconstruct_rt(incoming, equilibrium_moments(p, ub[0], ub[1], ub[2])) + neq +
-                   eta * reconstruct_rt(incoming, post_i)) /
-                  (1.0f + eta);
-      const float rho = density(g, g.phi[c.n]);
-      const float cf[3] = {(float)dx, (float)dy, (float)dz};
-      float impulse[3];
-      const float pref = (1.0f - eta) * g.ref_pressure[c.j] + eta * g.ref_pressure[jj];
-      const float w = w27(outgoing);
+  body = -1;
+  if (!inside(g, ji, jj, jk)) return raw;
+  const int m = lin(g, ji, jj, jk);
+  const int b = g.id[m];
+  if (b < 0) return raw;  // stationary tank or unresolved cavity: bounce-back
+  const Moments home_i = load_moments(g.home, N, c.n);
+  const float eta = link_fraction(g, c.n, m);
+  const float px = c.i + 0.5f + eta * dx, py = c.j + 0.5f + eta * dy, pz = c.k + 0.5f + eta * dz;
+  float ub[3];
+  body_velocity_at(g, px, py, pz, b, ub);
+  const float p = home_i.m[0];
+  const float neq = reconstruct_rt(outgoing, home_i) -
+                    reconstruct_rt(outgoing, equilibrium_moments(p, g.old_u[c.n], g.old_u[N + c.n], g.old_u[2 * N + c.n]));
+  const int incoming = 26 - outgoing;
+  const float reflected =
+      (reconstruct_rt(incoming, equilibrium_moments(p, ub[0], ub[1], ub[2])) + neq + eta * reconstruct_rt(incoming, post_i)) /
+      (1.0f + eta);
+  const float rho = density(g, g.phi[c.n]);
+  const float cf[3] = {(float)dx, (float)dy, (float)dz};
+  const float pref = (1.0f - eta) * g.ref_pressure[c.j] + eta * g.ref_pressure[jj];
+  const float w = w27(outgoing);
+  float impulse[3];
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        impulse[a] = rho * ((cf[a] - ub[a]) * raw - (-cf[a] - ub[a]) * reflected);
-        impulse[a] += 6.0f * w * pref * cf[a];
-      }
-      float own_s[9], nb_s[9], gi[3];
-      load_grad(g, c.n, gi);
-      stress_of(g, gi[0], gi[1], gi[2], own_s);
-      capillary_neighbor(g, c.i, c.j, c.k, dx, dy, dz, nb_s);
-      float cap[3];
-#pragma unroll
-      for (int a = 0; a < 3; ++a)
-        cap[a] = -3.0f * w *
-                 ((own_s[3 * a] + nb_s[3 * a]) * cf[0] + (own_s[3 * a + 1] + nb_s[3 * a + 1]) * cf[1] +
-                  (own_s[3 * a + 2] + nb_s[3 * a + 2]) * cf[2]);
-      const float* centre = body_row(g, b) + 12;
-      const float r1[3] = {px - centre[0], py - centre[1], pz - centre[2]};
-      const float r2[3] = {c.i + 0.5f + 0.5f * cf[0] - centre[0], c.j + 0.5f + 0.5f * cf[1] - centre[1],
-                           c.k + 0.5f + 0.5f * cf[2] - centre[2]};
-      float t[3] = {r1[1] * impulse[2] - r1[2] * impulse[1], r1[2] * impulse[0] - r1[0] * impulse[2],
-                    r1[0] * impulse[1] - r1[1] * impulse[0]};
-      t[0] += r2[1] * cap[2] - r2[2] * cap[1];
-      t[1] += r2[2] * cap[0] - r2[0] * cap[2];
-      t[2] += r2[0] * cap[1] - r2[1] * cap[0];
-      // Accumulate per body in registers; the kernel flushes with atomics.
-      int slot = -1;
-      for (int s = 0; s < loads; ++s)
-        if (load_body[s] == b) slot = s;
-      if (slot < 0 && loads < 4) {
-        slot = loads++;
-        load_body[slot] = b;
-        for (int e = 0; e < 6; ++e) load[slot][e] = 0.0f;
-      }
-      if (slot >= 0) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          load[slot][a] += impulse[a] + cap[a];
-          load[slot][3 + a] += t[a];
-        }
-      } else {
-        float* dst = g.control->loads[b];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          atomicAdd(dst + a, impulse[a] + cap[a]);
-          atomicAdd(dst + 3 + a, t[a]);
-        }
-      }
-    }
+  for (int a = 0; a < 3; ++a) {
+    impulse[a] = rho * ((cf[a] - ub[a]) * raw - (-cf[a] - ub[a]) * reflected);
+    impulse[a] += 6.0f * w * pref * cf[a];
   }
+  float own_s[9], nb_s[9], gi[3];
+  load_grad(g, c.n, gi);
+  stress_of(g, gi[0], gi[1], gi[2], own_s);
+  capillary_neighbor(g, c.i, c.j, c.k, dx, dy, dz, nb_s);
+  float cap[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    cap[a] = -3.0f * w *
+             ((own_s[3 * a] + nb_s[3 * a]) * cf[0] + (own_s[3 * a + 1] + nb_s[3 * a + 1]) * cf[1] +
+              (own_s[3 * a + 2] + nb_s[3 * a + 2]) * cf[2]);
+  const float* centre = body_row(g, b) + 12;
+  const float r1[3] = {px - centre[0], py - centre[1], pz - centre[2]};
+  const float r2[3] = {c.i + 0.5f + 0.5f * cf[0] - centre[0], c.j + 0.5f + 0.5f * cf[1] - centre[1],
+                       c.k + 0.5f + 0.5f * cf[2] - centre[2]};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) load[a] = impulse[a] + cap[a];
+  load[3] = r1[1] * impulse[2] - r1[2] * impulse[1] + (r2[1] * cap[2] - r2[2] * cap[1]);
+  load[4] = r1[2] * impulse[0] - r1[0] * impulse[2] + (r2[2] * cap[0] - r2[0] * cap[2]);
+  load[5] = r1[0] * impulse[1] - r1[1] * impulse[0] + (r2[0] * cap[1] - r2[1] * cap[0]);
+  body = b;
   return reflected;
 }
 
-// home.py:323-345: flag-2 cells (ice or unresolved cavity neighbours).
-__global__ void k_stream_boundary(icy3d_grid g) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+// home.py:323-345 for flag-2 cells: one lane per pulled direction; the
+// 10 moment sums are reduced within the 32-lane group and body loads are
+// accumulated in LDS, then written as per-block partials.
+__global__ void __launch_bounds__(256) k_stream_boundary(icy3d_grid g) {
+  __shared__ float block_loads[kLoadSlots];
+  for (int t = threadIdx.x; t < kLoadSlots; t += blockDim.x) block_loads[t] = 0.0f;
+  __syncthreads();
+  const int lane = threadIdx.x & (kCellLanes - 1);
+  const int groups = blockDim.x / kCellLanes;
+  const int count = g.control->ice_count;
   const size_t N = cells(g);
-  if (n >= (int)N || g.flag[n] != 2) return;
-  const Cell c = cell_of(g, n);
-  const Moments post_i = load_moments(g.home_post, N, n);
-  const Moments home_i = load_moments(g.home, N, n);
-  float load[4][6];
-  int load_body[4];
-  int loads = 0;
-  float P = 0, J[3] = {0, 0, 0}, S[6] = {0, 0, 0, 0, 0, 0};
-  for (int q = 0; q < 27; ++q) {
-    const int dx = cx(q), dy = cy(q), dz = cz(q);
-    const int i = c.i - dx, j = c.j - dy, k = c.k - dz;
-    float v;
-    if (fluid_at(g, i, j, k))
-      v = reconstruct_rt(q, q == 13 ? post_i : load_moments(g.home_post, N, lin(g, i, j, k)));
-    else
-      v = home_reflection(g, c, 26 - q, post_i, home_i, load, load_body, loads);
-    P += v;
-    J[0] += dx * v;
-    J[1] += dy * v;
-    J[2] += dz * v;
-    S[0] += (float)(dx * dx) * v;
-    S[1] += (float)(dy * dy) * v;
-    S[2] += (float)(dz * dz) * v;
-    S[3] += (float)(dx * dy) * v;
-    S[4] += (float)(dx * dz) * v;
-    S[5] += (float)(dy * dz) * v;
+  for (int t = blockIdx.x * groups + threadIdx.x / kCellLanes; t < count; t += gridDim.x * groups) {
+    const int n = g.ice_cells[t];
+    const Cell c = cell_of(g, n);
+    float v = 0.0f;
+    const int q = lane < 27 ? lane : 13;
+    if (lane < 27) {
+      const int i = c.i - cx(q), j = c.j - cy(q), k = c.k - cz(q);
+      if (fluid_at(g, i, j, k)) {
+        v = reconstruct_rt(q, load_moments(g.home_post, N, lin(g, i, j, k)));
+      } else {
+        int body;
+        float load[6];
+        v = reflect_link(g, c, 26 - q, load_moments(g.home_post, N, n), body, load);
+        if (body >= 0)
+#pragma unroll
+          for (int e = 0; e < 6; ++e) atomicAdd(&block_loads[6 * body + e], load[e]);
+      }
+    }
+    const float fx = (float)cx(q), fy = (float)cy(q), fz = (float)cz(q);
+    MomentSums s;
+    s.p = group_sum(v);
+    s.jx = group_sum(fx * v);
+    s.jy = group_sum(fy * v);
+    s.jz = group_sum(fz * v);
+    s.xx = group_sum(fx * fx * v);
+    s.yy = group_sum(fy * fy * v);
+    s.zz = group_sum(fz * fz * v);
+    s.xy = group_sum(fx * fy * v);
+    s.xz = group_sum(fx * fz * v);
+    s.yz = group_sum(fy * fz * v);
+    if (lane == 0) store_streamed(g, n, s);
   }
-  MomentSums s = {P, J[0], J[1], J[2], S[0], S[1], S[2], S[3], S[4], S[5]};
-  store_streamed(g, n, s);
-  for (int t = 0; t < loads; ++t) {
-    float* dst = g.control->loads[load_body[t]];
-    for (int e = 0; e < 6; ++e) atomicAdd(dst + e, load[t][e]);
-  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kLoadSlots; t += blockDim.x) g.partials[(size_t)blockIdx.x * kLoadSlots + t] = block_loads[t];
+}
+// Adds the per-block load partials of k_stream_boundary to the loads: one
+// block per load slot, fixed summation order.
+__global__ void __launch_bounds__(256) k_finish_loads(icy3d_grid g, int blocks) {
+  const int slot = blockIdx.x;
+  float sum = 0.0f;
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) sum += g.partials[(size_t)b * kLoadSlots + slot];
+  sum = block_sum(sum);
+  if (threadIdx.x == 0) (&g.control->loads[0][0])[slot] += sum;
 }
 
 __global__ void k_zero_loads(icy3d_grid g) {
@@ -1074,11 +1435,20 @@ Box full_box(const icy3d_grid& g) {
   Box b = {{0, 0, 0}, {g.nx, g.ny, g.nz}};
   return b;
 }
+// Boxes from the caller are clamped to the grid (lo inclusive, hi exclusive).
 Box make_box(const int32_t lo[3], const int32_t hi[3]) {
   Box b;
   for (int a = 0; a < 3; ++a) {
     b.lo[a] = lo[a];
     b.hi[a] = hi[a];
+  }
+  return b;
+}
+Box clamp_box(const icy3d_grid& g, Box b) {
+  const int shape[3] = {g.nx, g.ny, g.nz};
+  for (int a = 0; a < 3; ++a) {
+    b.lo[a] = b.lo[a] < 0 ? 0 : b.lo[a];
+    b.hi[a] = b.hi[a] > shape[a] ? shape[a] : b.hi[a];
   }
   return b;
 }
@@ -1099,34 +1469,55 @@ int launch_classify(const icy3d_grid& g, const Box& b, hipStream_t s) {
   ICY_LAUNCH_CHECK();
   return 0;
 }
-int launch_gradients(const icy3d_grid& g, const Box& b, hipStream_t s) {
+int launch_lists(const icy3d_grid& g, hipStream_t s) {
+  const unsigned blocks = (unsigned)((grid_cells(g) + kCompactCells - 1) / kCompactCells);
+  if ((int)(2 * blocks) > g.partials_capacity) return ICY3D_ERR_ARGUMENT;
+  k_count_flags<<<blocks, kBlock, 0, s>>>(g);
+  k_scan_flag_counts<<<1, 1024, 0, s>>>(g, (int)blocks);
+  k_scatter_flags<<<blocks, kBlock, 0, s>>>(g);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int launch_classify_all(const icy3d_grid& g, hipStream_t s) {
+  int rc = launch_classify(g, full_box(g), s);
+  return rc ? rc : launch_lists(g, s);
+}
+// Full-grid gradients use the ice list for the mirror path; box-limited
+// refreshes test the flag per cell.
+// Mirror gradients always run over the ice list (one code path, so a
+// box refresh reproduces a full pass bit for bit); the box only filters.
+int launch_gradients(const icy3d_grid& g, const Box& b, bool, hipStream_t s) {
   size_t n = box_cells(b);
   if (!n) return 0;
   k_gradients_regular<<<icy_blocks(n), kBlock, 0, s>>>(g, b);
-  k_gradients_general<<<icy_blocks(n), kBlock, 0, s>>>(g, b);
+  k_gradients_general_list<<<kListBlocks, kBlock, 0, s>>>(g, b);
   ICY_LAUNCH_CHECK();
   return 0;
 }
 int launch_forces(const icy3d_grid& g, int with_flux, int moment_pressure, hipStream_t s) {
-  const unsigned blocks = icy_blocks(grid_cells(g));
-  k_forces<kInterior><<<blocks, kBlock, 0, s>>>(g, with_flux, moment_pressure);
-  k_forces<kTank><<<blocks, kBlock, 0, s>>>(g, with_flux, moment_pressure);
-  k_forces<kGeneral><<<blocks, kBlock, 0, s>>>(g, with_flux, moment_pressure);
+  k_forces_interior<<<icy_blocks(grid_cells(g)), kBlock, 0, s>>>(g, with_flux, moment_pressure);
+  k_forces_tank<<<kListBlocks, kBlock, 0, s>>>(g, with_flux, moment_pressure);
+  k_forces_general<<<kListBlocks, kBlock, 0, s>>>(g, with_flux, moment_pressure);
   ICY_LAUNCH_CHECK();
   return 0;
 }
 int launch_recover(const icy3d_grid& g, int parity, int recover_flow, hipStream_t s) {
   bool scan = recover_flow && g.check_stability;
+  const unsigned blocks = icy_blocks(grid_cells(g));
+  if ((int)blocks > g.partials_capacity) return ICY3D_ERR_ARGUMENT;
   if (scan) k_reset_stability<<<1, 1, 0, s>>>(g);
-  k_recover_phase<<<icy_blocks(grid_cells(g)), kBlock, 0, s>>>(g, parity, recover_flow);
+  k_recover_phase<<<blocks, kBlock, 0, s>>>(g, parity, recover_flow);
+  k_finish_sum<<<1, 256, 0, s>>>(g.partials, (int)blocks, &g.control->phase_clip_volume);
   if (scan) k_fold_peak<<<1, 1, 0, s>>>(g);
   ICY_LAUNCH_CHECK();
   return 0;
 }
 int launch_stream(const icy3d_grid& g, hipStream_t s) {
-  const unsigned blocks = icy_blocks(grid_cells(g));
-  k_stream_bulk<<<blocks, kBlock, 0, s>>>(g);
-  k_stream_boundary<<<blocks, kBlock, 0, s>>>(g);
+  if ((int)(kListBlocks * kLoadSlots) > g.partials_capacity) return ICY3D_ERR_ARGUMENT;
+  const dim3 tiles((g.nz + kTileK - 1) / kTileK, (g.ny + kTileJ - 1) / kTileJ, (g.nx + kTileX - 1) / kTileX);
+  k_stream_tiled<<<tiles, kTileJ * kTileK, 0, s>>>(g);
+  k_stream_boundary<<<kListBlocks, kBlock, 0, s>>>(g);
+  if (g.bodies > 0) k_finish_loads<<<g.bodies * 6, 256, 0, s>>>(g, (int)kListBlocks);
   ICY_LAUNCH_CHECK();
   return 0;
 }
@@ -1142,17 +1533,25 @@ int icy3d_abi_sizes(int32_t* grid_bytes, int32_t* control_bytes) {
 
 int icy3d_classify(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;
-  return launch_classify(*g, lo ? make_box(lo, hi) : full_box(*g), (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  int rc = launch_classify(*g, lo ? clamp_box(*g, make_box(lo, hi)) : full_box(*g), s);
+  return rc ? rc : launch_lists(*g, s);
 }
 int icy3d_gradients(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;
-  return launch_gradients(*g, lo ? make_box(lo, hi) : full_box(*g), (hipStream_t)stream);
+  return launch_gradients(*g, lo ? clamp_box(*g, make_box(lo, hi)) : full_box(*g), lo == nullptr, (hipStream_t)stream);
 }
 int icy3d_refill_gradients(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;
-  Box b = make_box(lo, hi);
-  int rc = launch_classify(*g, b, (hipStream_t)stream);
-  return rc ? rc : launch_gradients(*g, b, (hipStream_t)stream);
+  hipStream_t s = (hipStream_t)stream;
+  Box b = clamp_box(*g, make_box(lo, hi));
+  int rc = launch_classify(*g, b, s);
+  if (!rc) rc = launch_lists(*g, s);
+  return rc ? rc : launch_gradients(*g, b, false, s);
+}
+int icy3d_build_boundary_lists(const icy3d_grid* g, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  return launch_lists(*g, (hipStream_t)stream);
 }
 int icy3d_collide_phase(const icy3d_grid* g, int parity, int moving_walls, void* stream) {
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;
@@ -1169,9 +1568,11 @@ int icy3d_phase_warmup(const icy3d_grid* g, int steps, int* parity, void* stream
   hipStream_t s = (hipStream_t)stream;
   Box b = full_box(*g);
   const unsigned blocks = icy_blocks(grid_cells(*g));
+  // Labels do not change during the warm-up: classify once.
+  int rc0 = launch_classify_all(*g, s);
+  if (rc0) return rc0;
   for (int t = 0; t < steps; ++t) {
-    int rc = launch_classify(*g, b, s);
-    if (!rc) rc = launch_gradients(*g, b, s);
+    int rc = launch_gradients(*g, b, true, s);
     if (rc) return rc;
     k_collide_phase<<<blocks, kBlock, 0, s>>>(*g, *parity, 0);
     *parity = 1 - *parity;
@@ -1198,7 +1599,7 @@ int icy3d_scan_stability(const icy3d_grid* g, void* stream) {
 int icy3d_forces(const icy3d_grid* g, int with_flux, int moment_pressure, void* stream) {
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;
   hipStream_t s = (hipStream_t)stream;
-  int rc = launch_classify(*g, full_box(*g), s);
+  int rc = launch_classify_all(*g, s);
   return rc ? rc : launch_forces(*g, with_flux, moment_pressure, s);
 }
 int icy3d_collide_momentum(const icy3d_grid* g, void* stream) {
@@ -1211,34 +1612,95 @@ int icy3d_stream_home(const icy3d_grid* g, void* stream) {
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;
   return launch_stream(*g, (hipStream_t)stream);
 }
-int icy3d_flow_step(const icy3d_grid* g, int parity, int refresh_gradients, void* stream) {
+int icy3d_flow_step_profiled(const icy3d_grid* g, int parity, int refresh_gradients, int classify,
+                             icy3d_profile* prof, void* stream) {
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;
   hipStream_t s = (hipStream_t)stream;
   const icy3d_grid& G = *g;
   const unsigned blocks = icy_blocks(grid_cells(G));
   Box b = full_box(G);
   int rc = 0;
-  if (refresh_gradients) {
-    rc = launch_classify(G, b, s);
-    if (!rc) rc = launch_gradients(G, b, s);
+  auto mark = [&](int event) {
+    if (prof) (void)hipEventRecord(prof->events[event], s);
+  };
+  if (classify || refresh_gradients) {
+    rc = launch_classify_all(G, s);
     if (rc) return rc;
   }
+  if (refresh_gradients) {
+    rc = launch_gradients(G, b, true, s);
+    if (rc) return rc;
+  }
+  mark(0);
   k_collide_phase<<<blocks, kBlock, 0, s>>>(G, parity, 1);
   parity = 1 - parity;
-  rc = launch_classify(G, b, s);
-  if (!rc) rc = launch_forces(G, 1, 1, s);
+  mark(1);
+  rc = launch_forces(G, 1, 1, s);
   if (rc) return rc;
+  mark(2);
   k_collide_momentum<<<blocks, kBlock, 0, s>>>(G);
   k_zero_loads<<<1, ICY3D_MAX_BODIES * 6, 0, s>>>(G);
+  mark(3);
   rc = launch_stream(G, s);
+  mark(4);
   if (!rc) rc = launch_recover(G, parity, 1, s);
-  if (!rc) rc = launch_gradients(G, b, s);
+  mark(5);
+  if (!rc) rc = launch_gradients(G, b, true, s);
+  mark(6);
+  if (prof) prof->pending = 1;
   return rc;
 }
-int icy3d_repair_phase_layout(const icy3d_grid* g, int parity, void* stream) {
+int icy3d_flow_step(const icy3d_grid* g, int parity, int refresh_gradients, int classify, void* stream) {
+  return icy3d_flow_step_profiled(g, parity, refresh_gradients, classify, nullptr, stream);
+}
+int icy3d_profile_create(icy3d_profile** out) {
+  if (!out) return ICY3D_ERR_ARGUMENT;
+  icy3d_profile* p = new icy3d_profile();
+  for (int e = 0; e <= ICY3D_PROFILE_GROUPS; ++e) {
+    hipError_t err = hipEventCreate(&p->events[e]);
+    if (err != hipSuccess) {
+      for (int k = 0; k < e; ++k) (void)hipEventDestroy(p->events[k]);
+      delete p;
+      return (int)err;
+    }
+  }
+  *out = p;
+  return 0;
+}
+int icy3d_profile_destroy(icy3d_profile* p) {
+  if (!p) return 0;
+  for (int e = 0; e <= ICY3D_PROFILE_GROUPS; ++e) (void)hipEventDestroy(p->events[e]);
+  delete p;
+  return 0;
+}
+int icy3d_profile_collect(icy3d_profile* p) {
+  if (!p) return ICY3D_ERR_ARGUMENT;
+  if (!p->pending) return 0;
+  hipError_t err = hipEventSynchronize(p->events[ICY3D_PROFILE_GROUPS]);
+  if (err != hipSuccess) return (int)err;
+  for (int k = 0; k < ICY3D_PROFILE_GROUPS; ++k) {
+    float ms = 0.0f;
+    err = hipEventElapsedTime(&ms, p->events[k], p->events[k + 1]);
+    if (err != hipSuccess) return (int)err;
+    p->total_ms[k] += ms;
+  }
+  p->count += 1;
+  p->pending = 0;
+  return 0;
+}
+int icy3d_profile_read(const icy3d_profile* p, double* total_ms, int64_t* count) {
+  if (!p || !total_ms || !count) return ICY3D_ERR_ARGUMENT;
+  for (int k = 0; k < ICY3D_PROFILE_GROUPS; ++k) total_ms[k] = p->total_ms[k];
+  *count = p->count;
+  return 0;
+}
+int icy3d_repair_phase_layout(const icy3d_grid* g, int parity, const int32_t lo[3], const int32_t hi[3],
+                              void* stream) {
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;
   if (!parity) return 0;
-  k_repair_layout<<<icy_blocks(grid_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g);
+  Box b = lo ? clamp_box(*g, make_box(lo, hi)) : full_box(*g);
+  size_t n = box_cells(b);
+  if (n) k_repair_layout<<<icy_blocks(n), kBlock, 0, (hipStream_t)stream>>>(*g, b);
   ICY_LAUNCH_CHECK();
   return 0;
 }
@@ -1250,7 +1712,7 @@ int icy3d_export_phase(const icy3d_grid* g, int parity, float* out, void* stream
 }
 int icy3d_project_refill(const icy3d_grid* g, int parity, const int32_t lo[3], const int32_t hi[3], void* stream) {
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;
-  Box b = make_box(lo, hi);
+  Box b = clamp_box(*g, make_box(lo, hi));
   size_t n = box_cells(b);
   if (!n) return 0;
   hipStream_t s = (hipStream_t)stream;

@@ -76,14 +76,14 @@ __global__ void k_reset_geometry_bounds(icy3d_grid g) {
   }
 }
 
-// simulator.py:288-323
-__global__ void k_rasterize(icy3d_grid g) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  bool active = n < (int)cells(g);
+// simulator.py:288-323 over a box that contains every cell whose label or
+// coverage can change (the union of the bodies' current and previous boxes).
+__global__ void k_rasterize(icy3d_grid g, GBox box) {
   Cell c = {0, 0, 0, 0};
+  bool active = gbox_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c);
   bool changed = false;
   if (active) {
-    c = cell_of(g, n);
+    const int n = c.n;
     const int b = g.boundary;
     const int old_id = g.id[n];
     g.prev_id[n] = (int8_t)old_id;
@@ -119,13 +119,12 @@ __global__ void k_reset_refill_counts(icy3d_grid g) {
 }
 
 // simulator.py:595-613
-__global__ void k_find_refill_changes(icy3d_grid g) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  bool active = n < (int)cells(g);
-  bool opening = false, covered = false;
+__global__ void k_find_refill_changes(icy3d_grid g, GBox box) {
   Cell c = {0, 0, 0, 0};
+  bool active = gbox_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c);
+  bool opening = false, covered = false;
   if (active) {
-    c = cell_of(g, n);
+    const int n = c.n;
     const bool is_fluid = g.id[n] == -1;
     const bool was_fluid = g.prev_id[n] == -1;
     opening = is_fluid && !was_fluid;
@@ -232,10 +231,11 @@ __global__ void k_accept_refill(icy3d_grid g, GBox box) {
 }
 
 // simulator.py:662-678
-__global__ void k_commit_refill(icy3d_grid g) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_commit_refill(icy3d_grid g, GBox box) {
+  Cell cb;
+  if (!gbox_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, cb)) return;
   const size_t N = cells(g);
-  if (n >= (int)N) return;
+  const int n = cb.n;
   int id = g.id[n];
   if (id == -1 && g.prev_id[n] != -1) {
     if (g.refill_valid[n]) {
@@ -251,10 +251,7 @@ __global__ void k_commit_refill(icy3d_grid g) {
   }
   if (id != -1) {
     float v[3] = {0.0f, 0.0f, 0.0f};
-    if (id >= 0) {
-      const Cell c = cell_of(g, n);
-      body_velocity_at(g, c.i + 0.5f, c.j + 0.5f, c.k + 0.5f, id, v);
-    }
+    if (id >= 0) body_velocity_at(g, cb.i + 0.5f, cb.j + 0.5f, cb.k + 0.5f, id, v);
     g.phi[n] = 0.0f;
     g.u[n] = v[0];
     g.u[N + n] = v[1];
@@ -284,6 +281,10 @@ size_t gbox_cells(const GBox& b) {
   long ex = b.hi[0] - b.lo[0], ey = b.hi[1] - b.lo[1], ez = b.hi[2] - b.lo[2];
   return (ex <= 0 || ey <= 0 || ez <= 0) ? 0 : (size_t)ex * ey * ez;
 }
+GBox full_gbox(const icy3d_grid& g) {
+  GBox b = {{0, 0, 0}, {g.nx, g.ny, g.nz}};
+  return b;
+}
 bool gbox_inside(const icy3d_grid& g, const GBox& b) {
   const int shape[3] = {g.nx, g.ny, g.nz};
   for (int a = 0; a < 3; ++a)
@@ -294,19 +295,25 @@ bool gbox_inside(const icy3d_grid& g, const GBox& b) {
 
 extern "C" {
 
-int icy3d_rasterize(const icy3d_grid* g, void* stream) {
+int icy3d_rasterize(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
   if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
+  GBox b = lo ? make_gbox(lo, hi) : full_gbox(*g);
+  if (!gbox_inside(*g, b)) return ICY3D_ERR_ARGUMENT;
   hipStream_t s = (hipStream_t)stream;
   k_reset_geometry_bounds<<<1, 1, 0, s>>>(*g);
-  k_rasterize<<<icy_blocks(all_cells(*g)), kBlock, 0, s>>>(*g);
+  size_t n = gbox_cells(b);
+  if (n) k_rasterize<<<icy_blocks(n), kBlock, 0, s>>>(*g, b);
   ICY_LAUNCH_CHECK();
   return 0;
 }
-int icy3d_find_refill_changes(const icy3d_grid* g, void* stream) {
+int icy3d_find_refill_changes(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
   if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
+  GBox b = lo ? make_gbox(lo, hi) : full_gbox(*g);
+  if (!gbox_inside(*g, b)) return ICY3D_ERR_ARGUMENT;
   hipStream_t s = (hipStream_t)stream;
   k_reset_refill_counts<<<1, 1, 0, s>>>(*g);
-  k_find_refill_changes<<<icy_blocks(all_cells(*g)), kBlock, 0, s>>>(*g);
+  size_t n = gbox_cells(b);
+  if (n) k_find_refill_changes<<<icy_blocks(n), kBlock, 0, s>>>(*g, b);
   ICY_LAUNCH_CHECK();
   return 0;
 }
@@ -339,9 +346,12 @@ int icy3d_accept_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t 
   ICY_LAUNCH_CHECK();
   return 0;
 }
-int icy3d_commit_refill(const icy3d_grid* g, void* stream) {
+int icy3d_commit_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
   if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
-  k_commit_refill<<<icy_blocks(all_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g);
+  GBox b = lo ? make_gbox(lo, hi) : full_gbox(*g);
+  if (!gbox_inside(*g, b)) return ICY3D_ERR_ARGUMENT;
+  size_t n = gbox_cells(b);
+  if (n) k_commit_refill<<<icy_blocks(n), kBlock, 0, (hipStream_t)stream>>>(*g, b);
   ICY_LAUNCH_CHECK();
   return 0;
 }

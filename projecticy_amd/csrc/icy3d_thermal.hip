@@ -41,7 +41,7 @@ __global__ void k_build_water_targets(icy3d_grid g) {
   }
   g.target_energy[n] = target * specific;
 }
-// thermal.py:120-127
+// thermal.py:120-127 (remap energy as a deterministic two-stage sum)
 __global__ void k_commit_water_targets(icy3d_grid g) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   float delta = 0.0f;
@@ -51,8 +51,8 @@ __global__ void k_commit_water_targets(icy3d_grid g) {
     g.water_volume[n] = water_target(g, n);
     g.water_energy[n] = e;
   }
-  delta = wave_sum(delta);
-  if (wave_leader() && delta != 0.0f) atomicAdd(&g.control->remap_energy, delta);
+  const float total = block_sum(delta);
+  if (threadIdx.x == 0) g.partials[blockIdx.x] = total;
 }
 // thermal.py:88-96
 __global__ void k_initialize_water_heat(icy3d_grid g) {
@@ -282,8 +282,8 @@ __global__ void k_apply_melting(icy3d_grid g) {
     g.material_mass[t] = mass;
     g.material_energy[t] = energy;
   }
-  discarded = wave_sum(discarded);
-  if (wave_leader() && discarded != 0.0f) atomicAdd(&g.control->discarded_heat, discarded);
+  const float block_total = block_sum(discarded);
+  if (threadIdx.x == 0) g.partials[blockIdx.x] = block_total;
 }
 
 // simulator.py:738-776: one thread per material column, FP32 sums.
@@ -426,8 +426,10 @@ int icy3d_synchronize_water(const icy3d_grid* g, void* stream) {
   if (!ok_grid(g)) return ICY3D_ERR_ARGUMENT;
   hipStream_t s = (hipStream_t)stream;
   const unsigned blocks = icy_blocks(n_cells(*g));
+  if ((int)blocks > g->partials_capacity) return ICY3D_ERR_ARGUMENT;
   k_build_water_targets<<<blocks, kBlock, 0, s>>>(*g);
   k_commit_water_targets<<<blocks, kBlock, 0, s>>>(*g);
+  k_finish_sum<<<1, 256, 0, s>>>(g->partials, (int)blocks, &g->control->remap_energy);
   ICY_LAUNCH_CHECK();
   return 0;
 }
@@ -486,7 +488,12 @@ int icy3d_exchange_heat(const icy3d_grid* g, float dt, const int32_t lo[3], cons
 int icy3d_apply_melting(const icy3d_grid* g, void* stream) {
   if (!ok_grid(g)) return ICY3D_ERR_ARGUMENT;
   size_t n = n_material(*g);
-  if (n) k_apply_melting<<<icy_blocks(n), kBlock, 0, (hipStream_t)stream>>>(*g);
+  if (!n) return 0;
+  const unsigned blocks = icy_blocks(n);
+  if ((int)blocks > g->partials_capacity) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  k_apply_melting<<<blocks, kBlock, 0, s>>>(*g);
+  k_finish_sum<<<1, 256, 0, s>>>(g->partials, (int)blocks, &g->control->discarded_heat);
   ICY_LAUNCH_CHECK();
   return 0;
 }

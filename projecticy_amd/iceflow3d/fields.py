@@ -21,11 +21,12 @@ class DeviceField:
     ``[*shape, c]`` (``matrix`` reshapes the trailing axis to 3x3).
     """
 
-    def __init__(self, tensor, *, components=0, logical_dtype=None, matrix=False):
+    def __init__(self, tensor, *, components=0, logical_dtype=None, matrix=False, on_write=None):
         self.t = tensor
         self.components = components
         self.logical_dtype = logical_dtype or _numpy_dtype(tensor.dtype)
         self.matrix = matrix
+        self.on_write = on_write
 
     @property
     def shape(self):
@@ -59,11 +60,15 @@ class DeviceField:
         if value.shape != self.shape:
             value = np.broadcast_to(value, self.shape)
         self.t.copy_(self._physical(value))
+        if self.on_write is not None:
+            self.on_write()
 
     def fill(self, value):
         value = np.asarray(value, dtype=float)
         if value.ndim == 0:
             self.t.fill_(float(value))
+            if self.on_write is not None:
+                self.on_write()
         else:
             self.from_numpy(np.broadcast_to(value, self.shape))
 

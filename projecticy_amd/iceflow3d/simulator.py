@@ -72,6 +72,9 @@ class IceFlow3D:
         self._call("initialize_material")
         self.rigid.update_mass()
         self._rasterize()
+        # The first rasterisation recorded labels of a zeroed grid as the
+        # previous labels; the first refresh must therefore cover the grid.
+        self._raster_full = True
         self._call("initialize_phase")
         if config.phase_warmup_steps:
             parity = ctypes.c_int32(self._parity)
@@ -116,6 +119,9 @@ class IceFlow3D:
             "material_mass": zeros(*mshape), "material_energy": zeros(*mshape), "material_heat": zeros(*mshape),
             "water_volume": zeros(*N), "water_energy": zeros(*N), "target_energy": zeros(*N),
             "sampled_ice_t": zeros(*N), "energy_flux": zeros(3, *N), "volume_flux": zeros(3, *N),
+            "tank_cells": zeros(math.prod(N), dtype=torch.int32),
+            "ice_cells": zeros(math.prod(N), dtype=torch.int32),
+            "partials": zeros(self._partials_capacity(mshape)),
             "control": torch.zeros(ctypes.sizeof(Control), dtype=torch.uint8, device=dev),
         }
         self._control_host = torch.zeros(ctypes.sizeof(Control), dtype=torch.uint8).pin_memory()
@@ -139,6 +145,13 @@ class IceFlow3D:
         g.velocity_m_s, g.initial_cell_mass, g.mass_kg = self.scales.velocity_m_s, self.initial_cell_mass, self.scales.mass_kg
         for name in _native.GRID_POINTERS:
             setattr(g, name, t[name].data_ptr())
+        g.partials_capacity = t["partials"].numel()
+        g.control = t["control"].data_ptr()
+        # Labels, link classes and boundary lists are rebuilt over the whole
+        # grid until the first geometry refresh establishes the ice region.
+        self._flags_valid = False
+        self._raster_full = True
+        self._raster_region = None
         self._grid = g
         self._gp = ctypes.pointer(g)
         self._body_state = np.zeros((B, _native.STATE_STRIDE), np.float32)
@@ -148,6 +161,46 @@ class IceFlow3D:
         control = self._read_control()
         control.invalid_index = math.prod(self.shape)
         self._write_control(control)
+
+    def _partials_capacity(self, mshape):
+        """Block partial sums: two ints per grid block (list compaction), one
+        float per material block, and the boundary-stream load slots."""
+        blocks = lambda count: (count + 255) // 256  # noqa: E731
+        return max(2 * blocks(math.prod(self.shape)), blocks(math.prod(mshape)),
+                   _native.LIST_BLOCKS * _native.MAX_BODIES * 6)
+
+    def _geometry_edited(self):
+        """External edits of labels, coverage, material or body state: the
+        next rasterisation covers the grid and link classes are rebuilt."""
+        self._raster_full = True
+        self._flags_valid = False
+
+    def _bodies_box(self):
+        """Cells whose coverage can be nonzero for the uploaded body state:
+        the rotated material box plus its trilinear support, with a margin."""
+        state = self._body_state
+        sizes = np.array([b.size_cells for b in self.config.bodies], dtype=np.float64)
+        shape = np.array(self.shape)
+        lo, hi = shape.copy(), np.zeros(3, dtype=np.int64)
+        for b in range(self.body_count):
+            origin = state[b, 0:3].astype(np.float64)
+            extent = np.abs(state[b, 3:12].astype(np.float64).reshape(3, 3)) @ (0.5 * sizes[b] + 1.0)
+            lo = np.minimum(lo, np.floor(origin - extent).astype(np.int64) - 1)
+            hi = np.maximum(hi, np.ceil(origin + extent).astype(np.int64) + 2)
+        return np.clip(lo, 0, shape), np.clip(hi, 0, shape)
+
+    def _raster_box(self):
+        """Union of the bodies' current and previous boxes: every cell whose
+        label, coverage or previous label can change in this refresh."""
+        current = self._bodies_box()
+        if self._raster_full or self._raster_region is None:
+            box = (np.zeros(3, dtype=np.int64), np.array(self.shape))
+            self._raster_full = False
+        else:
+            last = self._raster_region
+            box = (np.minimum(current[0], last[0]), np.maximum(current[1], last[1]))
+        self._raster_region = current
+        return box
 
     def _control_offsets(self):
         floats = self._t["control"].view(torch.float32)
@@ -167,7 +220,6 @@ class IceFlow3D:
         self.water_phase = DeviceField(t["phi"])
         self.pressure_lattice = DeviceField(t["pressure"])
         self.old_pressure = DeviceField(t["old_pressure"])
-        self.coverage = DeviceField(t["coverage"])
         self.velocity_lattice = vec("u")
         self.old_velocity = vec("old_u")
         self.force_density = vec("force")
@@ -175,9 +227,11 @@ class IceFlow3D:
         self.phase_gradient_lattice = vec("grad")
         self.phase_mass_flux = vec("flux")
         self.capillary_stress = DerivedField(self._capillary_stress, (*self.shape, 3, 3))
-        self.solid_body_id = DeviceField(t["id"], logical_dtype=np.int32)
-        self.previous_body_id = DeviceField(t["prev_id"], logical_dtype=np.int32)
-        self.coverage_body_id = DeviceField(t["cov_id"], logical_dtype=np.int32)
+        edited = self._geometry_edited
+        self.coverage = DeviceField(t["coverage"], on_write=edited)
+        self.solid_body_id = DeviceField(t["id"], logical_dtype=np.int32, on_write=edited)
+        self.previous_body_id = DeviceField(t["prev_id"], logical_dtype=np.int32, on_write=edited)
+        self.coverage_body_id = DeviceField(t["cov_id"], logical_dtype=np.int32, on_write=edited)
         self.refill_state = DeviceField(t["refill_state"], components=5)
         self.refill_next = DeviceField(t["refill_next"], components=5)
         self.refill_valid = DeviceField(t["refill_valid"])
@@ -185,8 +239,8 @@ class IceFlow3D:
         self.reference_density = DeviceField(t["ref_density"])
         self.reference_pressure = DeviceField(t["ref_pressure"])
         body = t["body"]
-        self.body_origin = DeviceField(body[:, 0:3])
-        self.body_rotation = _MatrixRows(body[:, 3:12])
+        self.body_origin = DeviceField(body[:, 0:3], on_write=edited)
+        self.body_rotation = _MatrixRows(body[:, 3:12], on_write=edited)
         self.body_center = DeviceField(body[:, 12:15])
         self.body_velocity = DeviceField(body[:, 15:18])
         self.body_angular_velocity = DeviceField(body[:, 18:21])
@@ -195,7 +249,7 @@ class IceFlow3D:
         self.reduced_state = DeviceField(self._reduced)
         self.material_size = DeviceField(t["material_size"])
         self.initial_body_temperature = DeviceField(t["body_t0"])
-        self.material_mass = DeviceField(t["material_mass"])
+        self.material_mass = DeviceField(t["material_mass"], on_write=edited)
         self.material_energy = DeviceField(t["material_energy"])
         self.material_heat = DeviceField(t["material_heat"])
         self.water_volume = DeviceField(t["water_volume"])
@@ -281,6 +335,7 @@ class IceFlow3D:
     # ------------------------------------------------- reference kernel methods
     def _classify_home(self):
         self._call("classify", None, None)
+        self._flags_valid = True
 
     def _gradients(self, classified=False):
         if not classified:
@@ -302,6 +357,7 @@ class IceFlow3D:
 
     def _forces(self, with_flux, moment_pressure=False):
         self._call("forces", int(with_flux), int(moment_pressure))
+        self._flags_valid = True
 
     def _collide_momentum(self):
         self._call("collide_momentum")
@@ -322,11 +378,16 @@ class IceFlow3D:
         use = pressure is not None
         self._call("set_equilibrium_all", self._parity, int(with_phase), int(use), float(pressure or 0.0))
 
-    def _rasterize(self):
-        self._call("rasterize")
+    def _rasterize(self, box=None):
+        if box is None:
+            box = self._raster_box()
+        self._call("rasterize", int3(box[0]), int3(box[1]))
+        self._flags_valid = False
 
-    def _repair_phase_layout(self):
-        self._call("repair_phase_layout", self._parity)
+    def _repair_phase_layout(self, box=None):
+        lo, hi = (np.zeros(3, dtype=int), np.array(self.shape)) if box is None else box
+        # A link (I, I+e) is swapped by the thread of I: extend one cell down.
+        self._call("repair_phase_layout", self._parity, int3(np.maximum(np.asarray(lo) - 1, 0)), int3(hi))
 
     def _synchronize_water(self):
         self._call("synchronize_water")
@@ -457,9 +518,10 @@ class IceFlow3D:
 
     def _refresh_geometry(self, phase_changed=True):
         """simulator.py:708-736; the refill loop reads one counter per sweep."""
-        self._rasterize()
-        self._repair_phase_layout()
-        self._call("find_refill_changes")
+        box = self._raster_box()
+        self._rasterize(box)
+        self._repair_phase_layout(box)
+        self._call("find_refill_changes", int3(box[0]), int3(box[1]))
         control = self._read_control()
         remaining, covered = int(control.openings), int(control.covered)
         shape = np.array(self.shape)
@@ -477,13 +539,15 @@ class IceFlow3D:
                 self.pending_cells = new
                 break
             remaining = new
-        self._call("commit_refill")
+        self._call("commit_refill", int3(box[0]), int3(box[1]))
         # gradients.py:388-400: changed box plus the mirror dependency radius.
         geo_lo, geo_hi = np.array(control.geometry_lower), np.array(control.geometry_upper)
         lo, hi = np.minimum(lower, geo_lo), np.maximum(upper, geo_hi)
         if np.all(hi >= lo):
             lo, hi = np.maximum(lo - 6, 0), np.minimum(hi + 7, shape)
             self._call("refill_gradients", int3(lo), int3(hi))
+        # Labels changed only inside the refreshed box, which was reclassified.
+        self._flags_valid = True
         if has_openings or covered:
             self._call("project_refill", self._parity, int3(lower), int3(upper + 1))
         if phase_changed or has_openings or covered:
@@ -531,16 +595,49 @@ class IceFlow3D:
             if int(self._read_control().thermal_invalid):
                 raise RuntimeError(f"invalid thermal state at step {self.steps + 1}")
 
+    # Kernel-group names of the fused flow step (include/icy3d.h).
+    PROFILE_GROUPS = ("phase_collide", "forces", "home_collide", "home_stream", "phase_recover", "gradients")
+
+    def attach_profiler(self):
+        """Time each kernel group of the flow step with HIP events on the
+        solver's stream until ``detach_profiler()``; returns the running
+        totals through ``profile_totals()``."""
+        if getattr(self, "_profile", None) is None:
+            handle = ctypes.c_void_p()
+            check(self._lib.icy3d_profile_create(ctypes.byref(handle)), "profile_create")
+            self._profile = handle
+
+    def profile_totals(self):
+        """{group: (total milliseconds, steps)} accumulated so far."""
+        totals = (ctypes.c_double * len(self.PROFILE_GROUPS))()
+        count = ctypes.c_int64()
+        check(self._lib.icy3d_profile_read(self._profile, totals, ctypes.byref(count)), "profile_read")
+        return {name: (totals[i], count.value) for i, name in enumerate(self.PROFILE_GROUPS)}
+
+    def detach_profiler(self):
+        if getattr(self, "_profile", None) is not None:
+            self._lib.icy3d_profile_destroy(self._profile)
+            self._profile = None
+
     def step(self, num_steps=1):
         """simulator.py:823-850: one fused device launch sequence for the
         flow part, then host rigid bodies, geometry refresh and heat."""
         num_steps = integer("num_steps", num_steps, 0)
+        profile = getattr(self, "_profile", None)
         for _ in range(num_steps):
             refresh = not self._gradient_cache_valid
-            self._call("flow_step", self._parity, int(refresh))
+            classify = int(not self._flags_valid)
+            if profile is None:
+                self._call("flow_step", self._parity, int(refresh), classify)
+            else:
+                check(self._lib.icy3d_flow_step_profiled(self._gp, self._parity, int(refresh), classify, profile,
+                                                         self._stream()), "flow_step")
             self._gradient_cache_valid = True
+            self._flags_valid = True
             self._parity = 1 - self._parity
             control = self._read_control()
+            if profile is not None:
+                check(self._lib.icy3d_profile_collect(profile), "profile_collect")
             speed_bound = None
             if self.config.check_stability:
                 if control.invalid_index < math.prod(self.shape):
@@ -643,5 +740,7 @@ class _MatrixRows(DeviceField):
         return self.t.detach().cpu().numpy().reshape(-1, 3, 3).copy()
 
     def from_numpy(self, value):
-        value = np.broadcast_to(np.asarray(value, dtype=np.float32), self.shape)
-        self.t.copy_(torch.from_numpy(np.ascontiguousarray(value.reshape(-1, 9))))
+        value = np.array(np.broadcast_to(np.asarray(value, dtype=np.float32), self.shape))
+        self.t.copy_(torch.from_numpy(value.reshape(-1, 9)))
+        if self.on_write is not None:
+            self.on_write()
