@@ -300,9 +300,18 @@ __device__ __forceinline__ float block_sum(float v) {
 namespace {
 // Second stage of a deterministic sum: one block folds the per-block
 // partials in a fixed order and adds the total to *target.
-__global__ void __launch_bounds__(256) k_finish_sum(const float* __restrict__ partials, int count, float* target) {
+// Four loads are in flight per thread per round.
+constexpr int kFinishThreads = 1024;
+__global__ void __launch_bounds__(kFinishThreads) k_finish_sum(const float* __restrict__ partials, int count,
+                                                              float* target) {
   float v = 0.0f;
-  for (int t = threadIdx.x; t < count; t += blockDim.x) v += partials[t];
+  int t = threadIdx.x;
+  for (; t + 3 * kFinishThreads < count; t += 4 * kFinishThreads) {
+    const float a = partials[t], b = partials[t + kFinishThreads];
+    const float c = partials[t + 2 * kFinishThreads], d = partials[t + 3 * kFinishThreads];
+    v += (a + b) + (c + d);
+  }
+  for (; t < count; t += kFinishThreads) v += partials[t];
   const float total = icy::block_sum(v);
   if (threadIdx.x == 0) *target += total;
 }

@@ -1049,7 +1049,7 @@ __global__ void k_collide_momentum(icy3d_grid g) {
 }
 
 // ---------------------------------------------------------------------------
-// HOME pull streaming (home.py:286-321): flags 0/1 and solid velocities.
+// HOME pull streaming (home.py:286-321): moment sums of the pulled populations.
 struct MomentSums {
   float p, jx, jy, jz, xx, yy, zz, xy, xz, yz;
 };
@@ -1066,29 +1066,6 @@ __device__ __forceinline__ void accumulate(MomentSums& s, float v) {
   if (X && Y) s.xy += (float)(X * Y) * v;
   if (X && Z) s.xz += (float)(X * Z) * v;
   if (Y && Z) s.yz += (float)(Y * Z) * v;
-}
-
-template <int Q>
-__device__ __forceinline__ void pull_bulk(const icy3d_grid& g, const Cell& c, const Moments& own, MomentSums& s) {
-  const size_t N = cells(g);
-  const int i = c.i - cx(Q), j = c.j - cy(Q), k = c.k - cz(Q);
-  float v;
-  if (Q != 13 && !fluid_at(g, i, j, k)) {
-    // Stationary halfway bounce-back: R_q(p,-u,S) = R_-q(p,u,S).
-    Moments m = own;
-    m.m[1] = -m.m[1];
-    m.m[2] = -m.m[2];
-    m.m[3] = -m.m[3];
-    v = reconstruct<Q>(m);
-  } else {
-    v = reconstruct<Q>(Q == 13 ? own : load_moments(g.home_post, N, lin(g, i, j, k)));
-  }
-  accumulate<Q>(s, v);
-}
-template <int Q>
-__device__ __forceinline__ void pull_all(const icy3d_grid& g, const Cell& c, const Moments& own, MomentSums& s) {
-  pull_bulk<Q>(g, c, own, s);
-  if constexpr (Q + 1 < 27) pull_all<Q + 1>(g, c, own, s);
 }
 
 __device__ __forceinline__ void store_streamed(const icy3d_grid& g, int n, const MomentSums& s) {
@@ -1111,88 +1088,132 @@ __device__ __forceinline__ void store_streamed(const icy3d_grid& g, int n, const
   g.u[2 * N + n] = s.jz + 0.5f * g.accel[2 * N + n];
 }
 
-__global__ void __launch_bounds__(256) k_stream_bulk(icy3d_grid g) {
-  int n = blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t N = cells(g);
-  if (n >= (int)N) return;
-  const int flag = g.flag[n];
-  if (flag == 2) return;
-  const Cell c = cell_of(g, n);
-  if (flag < 0) {
-    float v[3] = {0.0f, 0.0f, 0.0f};
-    int b = g.id[n];
-    if (b >= 0) body_velocity_at(g, c.i + 0.5f, c.j + 0.5f, c.k + 0.5f, b, v);
-    g.u[n] = v[0];
-    g.u[N + n] = v[1];
-    g.u[2 * N + n] = v[2];
-    return;
-  }
-  const Moments own = load_moments(g.home_post, N, n);
-  MomentSums s = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-  pull_all<0>(g, c, own, s);
-  store_streamed(g, n, s);
-}
-
 // ---------------------------------------------------------------------------
 // Tiled HOME pull stream (home.py:286-321 for flags 0/1 and solid cells).
 //
-// A block owns a (4 j) x (64 k) column of cells and sweeps a chunk of 16 x
-// slabs.  Each x slab of post-collision moments (with a one-cell j/k halo
-// and the labels) is staged once in LDS.  A thread reads the 9 in-plane
-// neighbours (j+dy, k+dz) of slab s once and feeds three cells of its
-// column: x = s+1 (c_x = +1), x = s (c_x = 0) and x = s-1 (c_x = -1), so
-// neighbour reads drop from 27 to 9 moment vectors per cell.  Cell s-1 is
-// complete after slab s and is written then.  A non-fluid neighbour gives
-// the halfway bounce-back R_q(p, -u, S) = R_{-q}(p, u, S) of the cell's own
-// moments; for c_x = +1 the cell's own slab arrives one step later, so those
-// links are remembered in a 9-bit mask.
+// A block owns a (kTileJ x kTileK) column of cells and sweeps kTileX x
+// slabs.  Each slab (with a one-cell j/k halo) is staged once in LDS as 17
+// per-cell terms of the reconstruction (home.py:30-53):
+//   P = p - 1.5 tr S,  U = 3u,  D_a = 4.5 S_aa,  O_ab = 9 S_ab,
+//   T_ab = 13.5 (S_aa u_b + 2 S_ab u_a - 2 u_a^2 u_b)  (a != b),
+//   X = 27 (S_xz u_y + S_yz u_x + S_xy u_z - 2 u_x u_y u_z).
+// For a fixed (c_y, c_z) the reconstruction is a quadratic in c_x,
+//   f(c) = w(c) (P0 + c_x P1 + c_x^2 P2),
+// so one read of neighbour (s, j+dy, k+dz) gives the three populations
+// pulled by cells x = s+1, s, s-1 (c_x = +1, 0, -1).  Cell s-1 is complete
+// after slab s.  A non-fluid neighbour sets the link's bit in the cell's
+// 27-bit mask; at completion the masked links take the halfway bounce-back
+// R_q(p, -u, S) = R_{-q}(p, u, S) of the cell's own post-collision moments.
+// The next slab's loads are issued before the current slab is consumed.
 constexpr int kTileJ = 4, kTileK = 64, kTileX = 16;
 constexpr int kHaloJ = kTileJ + 2, kHaloK = kTileK + 2, kSlabCells = kHaloJ * kHaloK;
+constexpr int kStaged = 17;
+constexpr int kStageRounds = (kSlabCells + kTileJ * kTileK - 1) / (kTileJ * kTileK);
 
-template <int CX, int DY, int DZ>
-__device__ __forceinline__ void pull_slab_link(const Moments& m, bool fluid, const Moments& own_cur,
-                                               const Moments& own_prev, MomentSums& prev, MomentSums& cur,
-                                               MomentSums& next, unsigned& defer) {
-  // Direction pulled by the cell at x = s + CX from neighbour (s, j+DY, k+DZ).
-  constexpr int Q = 9 * (CX + 1) + 3 * (-DY + 1) + (-DZ + 1);
-  if (fluid) {
-    const float v = reconstruct<Q>(m);
-    if (CX == 1) accumulate<Q>(next, v);
-    if (CX == 0) accumulate<Q>(cur, v);
-    if (CX == -1) accumulate<Q>(prev, v);
-  } else {
-    if (CX == 1) defer |= 1u << ((DY + 1) * 3 + (DZ + 1));
-    if (CX == 0) accumulate<Q>(cur, reconstruct<26 - Q>(own_cur));
-    if (CX == -1) accumulate<Q>(prev, reconstruct<26 - Q>(own_prev));
+struct SlabFetch {
+  float m[10];
+  int label;
+};
+
+__device__ __forceinline__ void fetch_slab_cell(const icy3d_grid& g, int s, int j0, int k0, int t, SlabFetch& f) {
+  const int gj = j0 - 1 + t / kHaloK, gk = k0 - 1 + t % kHaloK;
+  f.label = -2;
+#pragma unroll
+  for (int m = 0; m < 10; ++m) f.m[m] = 0.0f;
+  if (t < kSlabCells && s >= 0 && s < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz) {
+    const size_t N = cells(g);
+    const int n = (s * g.ny + gj) * g.nz + gk;
+    f.label = g.id[n];
+#pragma unroll
+    for (int m = 0; m < 10; ++m) f.m[m] = g.home_post[m * N + n];
   }
 }
-template <int DY, int DZ>
-__device__ __forceinline__ void apply_deferred(unsigned defer, const Moments& own_cur, MomentSums& cur) {
-  constexpr int Q = 9 * 2 + 3 * (-DY + 1) + (-DZ + 1);  // c_x = +1 link pulled last step
-  if (defer & (1u << ((DY + 1) * 3 + (DZ + 1)))) accumulate<Q>(cur, reconstruct<26 - Q>(own_cur));
-}
-template <int DY, int DZ>
-__device__ __forceinline__ void pull_neighbour(const float (*sm)[kSlabCells], const int8_t* sid, int base,
-                                               const Moments& own_cur, const Moments& own_prev, MomentSums& prev,
-                                               MomentSums& cur, MomentSums& next, unsigned& defer) {
-  const int idx = base + DY * kHaloK + DZ;
-  const bool fluid = sid[idx] == -1;
-  Moments m;
+
+__device__ __forceinline__ void stage_slab_cell(float (*sv)[kSlabCells], int8_t* sid, int t, const SlabFetch& f) {
+  if (t >= kSlabCells) return;
+  sid[t] = (int8_t)f.label;
+  const bool fluid = f.label == -1;
+  const float p = f.m[0], ux = f.m[1], uy = f.m[2], uz = f.m[3];
+  const float sxx = f.m[4], syy = f.m[5], szz = f.m[6], sxy = f.m[7], sxz = f.m[8], syz = f.m[9];
+  float v[kStaged];
+  v[0] = p - 1.5f * (sxx + syy + szz);
+  v[1] = 3.0f * ux;
+  v[2] = 3.0f * uy;
+  v[3] = 3.0f * uz;
+  v[4] = 4.5f * sxx;
+  v[5] = 4.5f * syy;
+  v[6] = 4.5f * szz;
+  v[7] = 9.0f * sxy;
+  v[8] = 9.0f * sxz;
+  v[9] = 9.0f * syz;
+  v[10] = 13.5f * (sxx * uy + 2.0f * sxy * ux - 2.0f * ux * ux * uy);  // T_xy
+  v[11] = 13.5f * (sxx * uz + 2.0f * sxz * ux - 2.0f * ux * ux * uz);  // T_xz
+  v[12] = 13.5f * (syy * ux + 2.0f * sxy * uy - 2.0f * uy * uy * ux);  // T_yx
+  v[13] = 13.5f * (syy * uz + 2.0f * syz * uy - 2.0f * uy * uy * uz);  // T_yz
+  v[14] = 13.5f * (szz * ux + 2.0f * sxz * uz - 2.0f * uz * uz * ux);  // T_zx
+  v[15] = 13.5f * (szz * uy + 2.0f * syz * uz - 2.0f * uz * uz * uy);  // T_zy
+  v[16] = 27.0f * (sxz * uy + syz * ux + sxy * uz - 2.0f * ux * uy * uz);
 #pragma unroll
-  for (int t = 0; t < 10; ++t) m.m[t] = sm[t][idx];
-  pull_slab_link<1, DY, DZ>(m, fluid, own_cur, own_prev, prev, cur, next, defer);
-  pull_slab_link<0, DY, DZ>(m, fluid, own_cur, own_prev, prev, cur, next, defer);
-  pull_slab_link<-1, DY, DZ>(m, fluid, own_cur, own_prev, prev, cur, next, defer);
+  for (int q = 0; q < kStaged; ++q) sv[q][t] = fluid ? v[q] : 0.0f;
 }
 
-__device__ __forceinline__ void finish_streamed_cell(const icy3d_grid& g, int x, int j, int k, const MomentSums& s) {
+__host__ __device__ constexpr float lattice_weight(int order) {
+  return order == 0 ? 8.0f / 27.0f : order == 1 ? 2.0f / 27.0f : order == 2 ? 1.0f / 54.0f : 1.0f / 216.0f;
+}
+
+template <int DY, int DZ>
+__device__ __forceinline__ void pull_neighbour(const float (*sv)[kSlabCells], const int8_t* sid, int base,
+                                               MomentSums& prev, MomentSums& cur, MomentSums& next,
+                                               unsigned& mask_prev, unsigned& mask_cur, unsigned& mask_next) {
+  constexpr int CY = -DY, CZ = -DZ;  // pulled direction (c_x, CY, CZ) from (s, j+DY, k+DZ)
+  constexpr int QM = 3 * (CY + 1) + (CZ + 1), Q0 = 9 + QM, QP = 18 + QM;
+  constexpr int O = (CY != 0) + (CZ != 0);
+  constexpr float W0 = lattice_weight(O), W1 = lattice_weight(O + 1);
+  constexpr float fy = CY, fz = CZ, ay = fy * fy - 1.0f / 3.0f, az = fz * fz - 1.0f / 3.0f;
+  const int idx = base + DY * kHaloK + DZ;
+  if (sid[idx] != -1) {
+    mask_next |= 1u << QP;
+    mask_cur |= 1u << Q0;
+    mask_prev |= 1u << QM;
+    return;
+  }
+  float P0 = sv[0][idx], P1 = sv[1][idx], P2 = sv[4][idx];
+  float bx = 0.0f;
+  if (CY != 0) bx += fy * sv[10][idx];
+  if (CZ != 0) bx += fz * sv[11][idx];
+  if (CY != 0) P0 += fy * sv[2][idx] + sv[5][idx];
+  if (CZ != 0) P0 += fz * sv[3][idx] + sv[6][idx];
+  if (CY != 0 && CZ != 0) P0 += fy * fz * sv[9][idx];
+  if (CZ != 0) P0 += ay * fz * sv[13][idx];
+  if (CY != 0) P0 += az * fy * sv[15][idx];
+  if (CY != 0 || CZ != 0) P0 -= bx * (1.0f / 3.0f);
+  if (CY != 0) P1 += fy * sv[7][idx];
+  if (CZ != 0) P1 += fz * sv[8][idx];
+  P1 += ay * sv[12][idx] + az * sv[14][idx];
+  if (CY != 0 && CZ != 0) P1 += fy * fz * sv[16][idx];
+  if (CY != 0 || CZ != 0) P2 += bx;
+  const float a = P0 + P2;
+  accumulate<QP>(next, W1 * (a + P1));
+  accumulate<Q0>(cur, W0 * P0);
+  accumulate<QM>(prev, W1 * (a - P1));
+}
+
+template <int Q>
+__device__ __forceinline__ void add_bounce(const Moments& own, unsigned mask, MomentSums& s) {
+  if (mask & (1u << Q)) accumulate<Q>(s, reconstruct<26 - Q>(own));
+  if constexpr (Q + 1 < 27) add_bounce<Q + 1>(own, mask, s);
+}
+
+__device__ __forceinline__ void finish_streamed_cell(const icy3d_grid& g, int x, int j, int k, MomentSums& s,
+                                                     unsigned mask) {
   if (x < 0 || x >= g.nx || j >= g.ny || k >= g.nz) return;
+  const size_t N = cells(g);
   const int n = lin(g, x, j, k);
   const int flag = g.flag[n];
   if (flag == 0 || flag == 1) {
+    if (mask) add_bounce<0>(load_moments(g.home_post, N, n), mask, s);
     store_streamed(g, n, s);
   } else if (flag < 0) {
-    const size_t N = cells(g);
     float v[3] = {0.0f, 0.0f, 0.0f};
     const int b = g.id[n];
     if (b >= 0) body_velocity_at(g, x + 0.5f, j + 0.5f, k + 0.5f, b, v);
@@ -1202,61 +1223,45 @@ __device__ __forceinline__ void finish_streamed_cell(const icy3d_grid& g, int x,
   }
 }
 
-__global__ void __launch_bounds__(256) k_stream_tiled(icy3d_grid g) {
-  __shared__ float sm[10][kSlabCells];
+__global__ void __launch_bounds__(kTileJ* kTileK) k_stream_tiled(icy3d_grid g) {
+  __shared__ float sv[kStaged][kSlabCells];
   __shared__ int8_t sid[kSlabCells];
-  const size_t N = cells(g);
-  const int tk = threadIdx.x & (kTileK - 1), tj = threadIdx.x / kTileK;
+  const int tk = threadIdx.x % kTileK, tj = threadIdx.x / kTileK;
   const int k0 = blockIdx.x * kTileK, j0 = blockIdx.y * kTileJ, x0 = blockIdx.z * kTileX;
   const int j = j0 + tj, k = k0 + tk;
   const int base = (tj + 1) * kHaloK + (tk + 1);
   const MomentSums zero = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   MomentSums prev = zero, cur = zero, next = zero;
-  Moments own_prev;
-#pragma unroll
-  for (int t = 0; t < 10; ++t) own_prev.m[t] = 0.0f;
-  unsigned defer_prev = 0;
+  unsigned mask_prev = 0, mask_cur = 0, mask_next = 0;
   const int x_end = min(x0 + kTileX, g.nx);
+  SlabFetch f[kStageRounds];
+#pragma unroll
+  for (int r = 0; r < kStageRounds; ++r) fetch_slab_cell(g, x0 - 1, j0, k0, threadIdx.x + r * blockDim.x, f[r]);
   for (int s = x0 - 1; s <= x_end; ++s) {
     __syncthreads();
-    for (int t = threadIdx.x; t < kSlabCells; t += blockDim.x) {
-      const int gj = j0 - 1 + t / kHaloK, gk = k0 - 1 + t % kHaloK;
-      const bool in = s >= 0 && s < g.nx && gj >= 0 && gj < g.ny && gk >= 0 && gk < g.nz;
-      const int n = in ? (s * g.ny + gj) * g.nz + gk : 0;
-      const int8_t label = in ? g.id[n] : (int8_t)-2;
-      sid[t] = label;
 #pragma unroll
-      for (int m = 0; m < 10; ++m) sm[m][t] = label == -1 ? g.home_post[m * N + n] : 0.0f;
+    for (int r = 0; r < kStageRounds; ++r) stage_slab_cell(sv, sid, threadIdx.x + r * blockDim.x, f[r]);
+    if (s < x_end) {
+#pragma unroll
+      for (int r = 0; r < kStageRounds; ++r) fetch_slab_cell(g, s + 1, j0, k0, threadIdx.x + r * blockDim.x, f[r]);
     }
     __syncthreads();
-    Moments own_cur;
-#pragma unroll
-    for (int t = 0; t < 10; ++t) own_cur.m[t] = sm[t][base];
-    apply_deferred<-1, -1>(defer_prev, own_cur, cur);
-    apply_deferred<-1, 0>(defer_prev, own_cur, cur);
-    apply_deferred<-1, 1>(defer_prev, own_cur, cur);
-    apply_deferred<0, -1>(defer_prev, own_cur, cur);
-    apply_deferred<0, 0>(defer_prev, own_cur, cur);
-    apply_deferred<0, 1>(defer_prev, own_cur, cur);
-    apply_deferred<1, -1>(defer_prev, own_cur, cur);
-    apply_deferred<1, 0>(defer_prev, own_cur, cur);
-    apply_deferred<1, 1>(defer_prev, own_cur, cur);
-    unsigned defer = 0;
-    pull_neighbour<-1, -1>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
-    pull_neighbour<-1, 0>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
-    pull_neighbour<-1, 1>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
-    pull_neighbour<0, -1>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
-    pull_neighbour<0, 0>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
-    pull_neighbour<0, 1>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
-    pull_neighbour<1, -1>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
-    pull_neighbour<1, 0>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
-    pull_neighbour<1, 1>(sm, sid, base, own_cur, own_prev, prev, cur, next, defer);
-    if (s - 1 >= x0) finish_streamed_cell(g, s - 1, j, k, prev);
+    pull_neighbour<-1, -1>(sv, sid, base, prev, cur, next, mask_prev, mask_cur, mask_next);
+    pull_neighbour<-1, 0>(sv, sid, base, prev, cur, next, mask_prev, mask_cur, mask_next);
+    pull_neighbour<-1, 1>(sv, sid, base, prev, cur, next, mask_prev, mask_cur, mask_next);
+    pull_neighbour<0, -1>(sv, sid, base, prev, cur, next, mask_prev, mask_cur, mask_next);
+    pull_neighbour<0, 0>(sv, sid, base, prev, cur, next, mask_prev, mask_cur, mask_next);
+    pull_neighbour<0, 1>(sv, sid, base, prev, cur, next, mask_prev, mask_cur, mask_next);
+    pull_neighbour<1, -1>(sv, sid, base, prev, cur, next, mask_prev, mask_cur, mask_next);
+    pull_neighbour<1, 0>(sv, sid, base, prev, cur, next, mask_prev, mask_cur, mask_next);
+    pull_neighbour<1, 1>(sv, sid, base, prev, cur, next, mask_prev, mask_cur, mask_next);
+    if (s - 1 >= x0) finish_streamed_cell(g, s - 1, j, k, prev, mask_prev);
     prev = cur;
     cur = next;
     next = zero;
-    own_prev = own_cur;
-    defer_prev = defer;
+    mask_prev = mask_cur;
+    mask_cur = mask_next;
+    mask_next = 0;
   }
 }
 
@@ -1507,7 +1512,7 @@ int launch_recover(const icy3d_grid& g, int parity, int recover_flow, hipStream_
   if ((int)blocks > g.partials_capacity) return ICY3D_ERR_ARGUMENT;
   if (scan) k_reset_stability<<<1, 1, 0, s>>>(g);
   k_recover_phase<<<blocks, kBlock, 0, s>>>(g, parity, recover_flow);
-  k_finish_sum<<<1, 256, 0, s>>>(g.partials, (int)blocks, &g.control->phase_clip_volume);
+  k_finish_sum<<<1, kFinishThreads, 0, s>>>(g.partials, (int)blocks, &g.control->phase_clip_volume);
   if (scan) k_fold_peak<<<1, 1, 0, s>>>(g);
   ICY_LAUNCH_CHECK();
   return 0;

@@ -429,7 +429,7 @@ int icy3d_synchronize_water(const icy3d_grid* g, void* stream) {
   if ((int)blocks > g->partials_capacity) return ICY3D_ERR_ARGUMENT;
   k_build_water_targets<<<blocks, kBlock, 0, s>>>(*g);
   k_commit_water_targets<<<blocks, kBlock, 0, s>>>(*g);
-  k_finish_sum<<<1, 256, 0, s>>>(g->partials, (int)blocks, &g->control->remap_energy);
+  k_finish_sum<<<1, kFinishThreads, 0, s>>>(g->partials, (int)blocks, &g->control->remap_energy);
   ICY_LAUNCH_CHECK();
   return 0;
 }
@@ -493,7 +493,7 @@ int icy3d_apply_melting(const icy3d_grid* g, void* stream) {
   if ((int)blocks > g->partials_capacity) return ICY3D_ERR_ARGUMENT;
   hipStream_t s = (hipStream_t)stream;
   k_apply_melting<<<blocks, kBlock, 0, s>>>(*g);
-  k_finish_sum<<<1, 256, 0, s>>>(g->partials, (int)blocks, &g->control->discarded_heat);
+  k_finish_sum<<<1, kFinishThreads, 0, s>>>(g->partials, (int)blocks, &g->control->discarded_heat);
   ICY_LAUNCH_CHECK();
   return 0;
 }
