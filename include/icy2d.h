@@ -1,0 +1,159 @@
+/* icy2d — C ABI of the MI355X (gfx950) 2D falling-and-melting ice step.
+ *
+ * Drop-in boundary for the reference's Python solver class
+ * ``iceflow2d.simulator.IceFlow2D`` (iceflow2d/simulator.py:1826-4939) and
+ * its thermal component ``MovingBodyThermal2D`` (:126-1823).  The reference
+ * has no native FFI: its operator API is the Python class whose methods
+ * each launch Taichi kernels.  Every entry point below replaces one such
+ * kernel (cited per function) and is enqueued on the caller's HIP stream.
+ * The Python host (``projecticy_amd.iceflow2d``) owns every device buffer;
+ * this library never allocates memory and keeps no global state.
+ *
+ * Conventions
+ *   - Cells n = i * ny + j (j fastest).  Populations are q-plane major:
+ *     population q of cell n is ``f[q * N + n]`` (D2Q9 order of
+ *     iceflow2d/lattice.py:8-62).  Vectors are planes: ``u[c * N + n]``.
+ *   - x-face fields have (nx + 1) * ny entries (face (i, j) at i * ny + j);
+ *     y-face fields nx * (ny + 1) (face (i, j) at i * (ny + 1) + j).
+ *   - Body fields are body_nx * body_ny, index bi * body_ny + bj.
+ *   - Return codes: 0 success, >0 hipError_t of a failed launch,
+ *     <0 ICY2D_ERR_*.  ``stream`` is a hipStream_t passed as void*.
+ *   - Scalars the reference keeps in 0-d fields live in the device control
+ *     block ``icy2d_control``; one device-to-host copy returns all of them.
+ */
+#ifndef ICY2D_H
+#define ICY2D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ICY2D_OK = 0, ICY2D_ERR_ARGUMENT = -1 };
+
+/* Device control block: rigid state, reduction results and diagnostics.
+ * Triples are (x, y, angular). */
+typedef struct icy2d_control {
+  float velocity[2], angle, omega, origin[2], support[4]; /* simulator.py:1950-2089 */
+  int32_t projected, melt_status, wet_target, wet_target_max;
+  float max_outflow, reserved;
+  double mass, inertia, local_com[2];
+  double impulse[3];          /* hydrodynamic impulse x, y and torque */
+  double melted[3];           /* cumulative melted body momentum */
+  double carrier[3], correction[3];
+  double before[3], provisional[3], after[3], body_before[3];
+  double wet[4];              /* weight, first moment x, y, second moment */
+  double water[3];            /* initial volume, current volume, projection derivative */
+  double aperture[6];         /* volume, energy, base and full capacity, specific min, max */
+  double targets[3];          /* reconstructed energy, cooling, heating capacity */
+  double transfer[2];         /* volume and energy after the transfer */
+  double correction_abs, boundary_power, boundary_heat_input;
+  double unassigned[2], fallback[2], interval[2];
+  double reduced[4];          /* mass, first moment x, y, origin inertia */
+  double totals[4];           /* solid mass, water mass, body and water sensible energy */
+} icy2d_control;
+
+typedef struct icy2d_grid {
+  int32_t nx, ny, boundary, body_nx, body_ny, advection;
+  int32_t left_kind, right_kind, bottom_kind, top_kind;
+  /* lattice-unit constants */
+  float rho_w, rho_a, nu_w, nu_a, gx, gy, sigma, width, mobility, smagorinsky, interface_tau;
+  float cutoff, threshold, dx, expansion, t_ref, linear_damping, angular_damping;
+  double initial_mass, initial_inertia;
+  /* SI thermal constants */
+  double t_melt, cp_w, cp_i, k_w, k_i, latent, rho_w_si, area, velocity_scale, cell_mass, t_air;
+  double left_value, right_value, bottom_value, top_value;
+  /* flow fields (FP32) */
+  float *f, *fbuf, *h, *hbuf;     /* [9][N] */
+  float *phi, *u, *a, *p, *moving; /* u, a: [2][N]; moving: sum of f_q, q != 0 */
+  float *p_ref, *rho_ref;         /* [ny] */
+  int8_t *solid, *prev_solid, *eligible;
+  float* coverage;                /* world body solid fraction [N] */
+  /* thermal fields (FP64) */
+  double *body_mass, *body_energy, *body_heat, *melt_mass, *melt_sensible;
+  int32_t* melt_target;
+  double *water_v, *water_e, *target_e;
+  double *flux_ex, *flux_vx, *flux_ey, *flux_vy;
+  double* partials;               /* reduction scratch: partials_capacity doubles */
+  int32_t partials_capacity, reserved;
+  icy2d_control* control;
+} icy2d_grid;
+
+/* Geometry and rigid body ------------------------------------------------ */
+int icy2d_body_fraction(const icy2d_grid* g, int only_if_projected, void* stream); /* simulator.py:2532-2537 */
+int icy2d_commit_raster(const icy2d_grid* g, void* stream);                         /* :2539-2544 */
+int icy2d_contact_support(const icy2d_grid* g, int reduce, int only_if_projected,
+                          void* stream);                                              /* :3914-3971 */
+int icy2d_integrate_rigid(const icy2d_grid* g, void* stream);                       /* :3973-4017 */
+int icy2d_project_inside(const icy2d_grid* g, void* stream);                        /* :4019-4080 */
+int icy2d_update_solid_mask(const icy2d_grid* g, int init, void* stream);           /* :2699-2727 */
+int icy2d_reduce_mass_properties(const icy2d_grid* g, void* stream);                /* :3014-3044 */
+int icy2d_apply_mass_properties(const icy2d_grid* g, int initialize, void* stream); /* :3046-3161 */
+
+/* Initialisation ---------------------------------------------------------- */
+int icy2d_initialize_fluid(const icy2d_grid* g, int water_width, int water_height, void* stream); /* :2595-2624 */
+int icy2d_hydrostatic_equilibrium(const icy2d_grid* g, void* stream);                            /* :2664-2697 */
+int icy2d_initialize_water_volume(const icy2d_grid* g, void* stream);                            /* :4125-4133 */
+int icy2d_recover_initial_moments(const icy2d_grid* g, void* stream);                            /* :3833-3858 */
+int icy2d_initialize_body_state(const icy2d_grid* g, double ice_temperature, void* stream);       /* :579-588 */
+int icy2d_initialize_water_state(const icy2d_grid* g, double water_temperature, void* stream);    /* :589-617 */
+
+/* Flow -------------------------------------------------------------------- */
+/* Momentum collision with forces and Tao cut-link preparation (:3241-3552,
+ * :4798-4866) fused with the phase collision (:3554-3697) of the same cell;
+ * with_momentum = 0 runs the phase collision alone (the warm-up). */
+int icy2d_collide(const icy2d_grid* g, int with_momentum, void* stream);
+/* Cut-link preparation alone, applied in place to fbuf (the reference
+ * tests' helper kernel around :4798-4866). */
+int icy2d_prepare_cut_links(const icy2d_grid* g, void* stream);
+/* Streaming (:3699-3738; :3740-3785 when phase_only) in pull form, fused
+ * with the macroscopic update (:3787-3813; :3815-3825 when phase_only). */
+int icy2d_stream(const icy2d_grid* g, int phase_only, void* stream);
+int icy2d_update_pressure(const icy2d_grid* g, void* stream);   /* :3860-3897 */
+int icy2d_refill(const icy2d_grid* g, void* stream);            /* :3163-3236 */
+int icy2d_measure_outflow(const icy2d_grid* g, void* stream);   /* :2303-2338 */
+
+/* Volume projection (:4150-4492) ------------------------------------------ */
+int icy2d_clip_phase(const icy2d_grid* g, double max_bulk_l1, void* stream);
+int icy2d_seed_projection(const icy2d_grid* g, void* stream);
+int icy2d_dilate_projection(const icy2d_grid* g, int primary_to_next, void* stream);
+int icy2d_evaluate_projection(const icy2d_grid* g, double lambda, void* stream);
+int icy2d_apply_projection(const icy2d_grid* g, double lambda, void* stream);
+int icy2d_projection_residual_weight(const icy2d_grid* g, double residual, void* stream);
+int icy2d_apply_projection_residual(const icy2d_grid* g, double residual, double weight, void* stream);
+
+/* Melt-momentum closure (:2735-2986) -------------------------------------- */
+int icy2d_snapshot_melt_momentum(const icy2d_grid* g, void* stream);
+int icy2d_reduce_provisional_momentum(const icy2d_grid* g, void* stream);
+int icy2d_melt_weight_geometry(const icy2d_grid* g, void* stream); /* status -> control->melt_status */
+int icy2d_apply_melt_correction(const icy2d_grid* g, void* stream);
+int icy2d_reduce_final_momentum(const icy2d_grid* g, void* stream);
+int icy2d_local_melt_correction(const icy2d_grid* g, void* stream);
+int icy2d_finish_melt_momentum(const icy2d_grid* g, void* stream);
+
+/* Thermal (MovingBodyThermal2D) ------------------------------------------- */
+int icy2d_measure_aperture(const icy2d_grid* g, void* stream);             /* :633-714, :1792-1822 */
+int icy2d_build_aperture_targets(const icy2d_grid* g, void* stream);       /* :716-827 */
+int icy2d_apply_aperture_transfer(const icy2d_grid* g, void* stream);      /* :829-859 */
+int icy2d_finish_aperture_transfer(const icy2d_grid* g, void* stream);     /* :861-877 */
+int icy2d_body_conduction(const icy2d_grid* g, double dt, void* stream);   /* :879-907 */
+int icy2d_conduction_flux(const icy2d_grid* g, int axis, void* stream);    /* :909-1010, :1063-1164 */
+int icy2d_advection_flux(const icy2d_grid* g, int axis, void* stream);     /* :1012-1061, :1166-1215 */
+int icy2d_update_water(const icy2d_grid* g, double dt, int volume, void* stream); /* :1217-1246 */
+int icy2d_accumulate_boundary_heat(const icy2d_grid* g, double dt, void* stream); /* :1248-1250 */
+int icy2d_exchange_heat(const icy2d_grid* g, double dt, void* stream);     /* :1252-1372 */
+int icy2d_phase_change(const icy2d_grid* g, void* stream);                 /* :1374-1406 */
+int icy2d_freeze_eligibility(const icy2d_grid* g, void* stream);           /* :1408-1422 */
+int icy2d_inject_melt(const icy2d_grid* g, void* stream);                  /* :1424-1475 */
+int icy2d_measure_fallback(const icy2d_grid* g, void* stream);             /* :1477-1502 */
+int icy2d_apply_fallback(const icy2d_grid* g, int use_free, void* stream); /* :1504-1540 */
+int icy2d_reduce_totals(const icy2d_grid* g, void* stream);                /* :1565-1583 */
+
+/* Binding check: sizeof(icy2d_grid), sizeof(icy2d_control). */
+void icy2d_abi_sizes(int32_t* grid, int32_t* control);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

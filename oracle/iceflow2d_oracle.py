@@ -1,0 +1,555 @@
+"""Oracle (test infrastructure only): FP64 CPU restatement of the reference
+2D coupled step ``IceFlow2D`` (``iceflow2d/simulator.py:126-4939``).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
+may import this module.  It is the checker, never the product: the product
+path (``projecticy_amd``) does not import it and fails when its HIP library
+is missing.
+
+Layout: every per-cell kernel lives in ``oracle2d.c`` (sequential FP64 C,
+compiled by ``build_oracle2d()`` with gcc); this module owns the NumPy
+arrays and restates the reference's host logic in its own order: unit
+conversion (simulator.py:1849-1891), the initialisation sequence
+(:2159-2192), ``step`` (:2194-2227), the FAST/SLOW thermal schedule
+(:2233-2290, :418-509), the aperture synchronisation and its checks
+(:511-548), the safeguarded Newton volume projection (:4494-4560), the
+melt-momentum closure (:2989-3012) and the hydrostatic reference
+(:2626-2662).
+
+Parity pinning: tests/test_oracle2d.py checks this restatement against the
+reference's own closed-form tests (Tao single-node reflection, rigid
+integration, contact support, the thermal closure and aperture transfer,
+volume projection invariants, calorimetry).  Full multi-step trajectories
+have no reference-held vectors: "parity unpinned" beyond those checks.
+"""
+
+import ctypes
+import math
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle2d.so"
+MAX_BULK_PHASE_L1 = 2.0  # simulator.py:45
+
+_F = ctypes.POINTER(ctypes.c_float)
+_D = ctypes.POINTER(ctypes.c_double)
+_I8 = ctypes.POINTER(ctypes.c_int8)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+
+
+class O2D(ctypes.Structure):
+    """Mirror of ``struct o2d`` in oracle2d.h (field order matters)."""
+
+    _fields_ = (
+        [(n, ctypes.c_int32) for n in ("nx", "ny", "boundary", "body_nx", "body_ny", "left_kind", "right_kind",
+                                       "bottom_kind", "top_kind", "advection")]
+        + [(n, ctypes.c_double) for n in (
+            "rho_w", "rho_a", "nu_w", "nu_a", "gx", "gy", "sigma", "width", "mobility", "smagorinsky",
+            "interface_tau", "cutoff", "threshold", "dx", "expansion", "t_ref", "linear_damping",
+            "angular_damping", "initial_mass", "initial_inertia",
+            "t_melt", "cp_w", "cp_i", "k_w", "k_i", "latent", "rho_w_si", "area", "velocity_scale",
+            "cell_mass", "t_air", "left_value", "right_value", "bottom_value", "top_value")]
+        + [(n, _F) for n in ("f", "fbuf", "h", "hbuf", "phi", "u", "a", "p", "p_ref", "rho_ref")]
+        + [("solid", _I8), ("prev_solid", _I8), ("coverage", _F)]
+        + [(n, _D) for n in ("body_mass", "body_energy", "body_heat", "melt_mass_step", "melt_sensible_step")]
+        + [("melt_target", _I32)]
+        + [(n, _D) for n in ("water_v", "water_e", "target_e", "flux_ex", "flux_vx", "flux_ey", "flux_vy")]
+        + [("eligible", _I8)]
+        + [("velocity", ctypes.c_float * 2), ("angle", ctypes.c_float), ("omega", ctypes.c_float),
+           ("origin", ctypes.c_float * 2), ("support", ctypes.c_float * 4), ("projected", ctypes.c_int32)]
+        + [("mass", ctypes.c_double), ("inertia", ctypes.c_double), ("local_com", ctypes.c_double * 2),
+           ("impulse", ctypes.c_double * 2), ("torque", ctypes.c_double)]
+        + [(n, ctypes.c_double) for n in ("initial_water_volume", "water_volume_current", "projection_derivative")]
+        + [("cumulative_melted_momentum", ctypes.c_double * 2), ("cumulative_melted_angular", ctypes.c_double)]
+        + [("carrier_momentum", ctypes.c_double * 2), ("correction_momentum", ctypes.c_double * 2),
+           ("carrier_angular", ctypes.c_double), ("correction_angular", ctypes.c_double)]
+        + [(n, ctypes.c_double * 2) for n in ("fluid_before", "fluid_provisional", "fluid_after", "body_before")]
+        + [(n, ctypes.c_double) for n in ("angular_before", "angular_provisional", "angular_after",
+                                          "body_angular_before", "wet_weight")]
+        + [("weight_first", ctypes.c_double * 2), ("weight_second", ctypes.c_double),
+           ("wet_target", ctypes.c_int32), ("wet_target_max", ctypes.c_int32)]
+        + [(n, ctypes.c_double) for n in (
+            "volume_before", "energy_before", "base_capacity", "full_capacity", "specific_min", "specific_max",
+            "reconstructed_energy", "cooling_capacity", "heating_capacity", "correction_abs",
+            "volume_after", "energy_after", "boundary_power", "boundary_heat_input",
+            "unassigned_mass", "unassigned_energy", "fallback_free", "fallback_wet",
+            "interval_body_melt", "interval_water_melt", "reduced_mass")]
+        + [("reduced_first", ctypes.c_double * 2), ("reduced_inertia", ctypes.c_double),
+           ("max_outflow", ctypes.c_float)]
+    )
+
+
+def build_oracle2d(force=False):
+    """Compile oracle2d.c with gcc into oracle/_build (no reference build)."""
+    source = HERE / "oracle2d.c"
+    LIB_PATH.parent.mkdir(exist_ok=True)
+    stale = not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < max(
+        source.stat().st_mtime, (HERE / "oracle2d.h").stat().st_mtime)
+    if force or stale:
+        tmp = LIB_PATH.with_suffix(f".so.tmp{os.getpid()}")
+        subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-std=c11", "-fno-fast-math", str(source), "-o", str(tmp),
+                        "-lm"], check=True)
+        os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+_LIB = None
+
+
+def oracle2d_lib():
+    global _LIB
+    if _LIB is None:
+        build_oracle2d()
+        lib = ctypes.CDLL(str(LIB_PATH))
+        ptr = ctypes.POINTER(O2D)
+        signatures = {
+            "body_fraction": [], "commit_raster": [], "contact_support": [ctypes.c_int],
+            "integrate_rigid": [], "project_inside": [], "contact_and_rasterize": [ctypes.c_int],
+            "update_solid_mask": [ctypes.c_int], "initialize_fluid": [ctypes.c_int, ctypes.c_int],
+            "hydrostatic_equilibrium": [], "initialize_water_volume": [], "collide_momentum": [],
+            "collide_phase": [], "stream": [ctypes.c_int], "update_macroscopic": [],
+            "update_streamed_phase": [], "recover_initial_moments": [], "update_pressure": [], "refill": [],
+            "measure_outflow": [], "seed_projection": [], "dilate_projection": [ctypes.c_int],
+            "evaluate_projection": [ctypes.c_double, _D], "reduce_mass_properties": [],
+            "apply_mass_properties": [ctypes.c_int], "snapshot_melt_momentum": [],
+            "reduce_provisional_momentum": [], "apply_melt_correction": [], "reduce_final_momentum": [],
+            "local_melt_correction": [], "finish_melt_momentum": [], "initialize_body_state": [ctypes.c_double],
+            "initialize_water_state": [ctypes.c_double], "measure_aperture": [_D],
+            "build_aperture_targets": [], "apply_aperture_transfer": [], "finish_aperture_transfer": [],
+            "body_conduction": [ctypes.c_double], "conduction_flux": [ctypes.c_int],
+            "advection_flux": [ctypes.c_int], "update_water": [ctypes.c_double, ctypes.c_int],
+            "exchange_heat": [ctypes.c_double], "phase_change": [], "freeze_eligibility": [], "inject_melt": [],
+            "measure_fallback": [], "apply_fallback": [ctypes.c_int], "reduce_totals": [_D],
+        }
+        for name, args in signatures.items():
+            fn = getattr(lib, "o2d_" + name)
+            fn.argtypes = [ptr, *args]
+            fn.restype = None
+        for name, args in {"clip_phase": [ctypes.c_double], "apply_projection": [ctypes.c_double],
+                           "projection_residual_weight": [ctypes.c_double],
+                           "apply_projection_residual": [ctypes.c_double, ctypes.c_double]}.items():
+            fn = getattr(lib, "o2d_" + name)
+            fn.argtypes = [ptr, *args]
+            fn.restype = ctypes.c_double
+        lib.o2d_melt_weight_geometry.argtypes = [ptr]
+        lib.o2d_melt_weight_geometry.restype = ctypes.c_int
+        _LIB = lib
+    return _LIB
+
+
+def _ptr(array, kind):
+    return array.ctypes.data_as(kind)
+
+
+class Oracle2D:
+    """Sequential FP64 restatement of ``IceFlow2D`` (reference field names)."""
+
+    def __init__(self, config):
+        from projecticy_amd.iceflow2d.config import DIRICHLET, BOUNDARY_CODE, IceFlowConfig, LatticeScales
+
+        if not isinstance(config, IceFlowConfig):
+            raise TypeError("config must be an IceFlowConfig")
+        self.lib = oracle2d_lib()
+        self.config = config
+        self.nx, self.ny = config.nx, config.ny
+        self.steps = 0
+        self.scales = LatticeScales.from_iceflow_config(config)
+        th = config.thermal
+        props = th.properties
+        s = O2D()
+        self.s = s
+        self.ps = ctypes.pointer(s)
+        nx, ny, N = self.nx, self.ny, self.nx * self.ny
+        bx, by = config.ice_width, config.ice_height
+        s.nx, s.ny, s.boundary, s.body_nx, s.body_ny = nx, ny, config.boundary_cells, bx, by
+        s.advection = int(th.advection_enabled)
+        for side in ("left", "right", "bottom", "top"):
+            boundary = getattr(th.boundaries, side)
+            setattr(s, side + "_kind", BOUNDARY_CODE[boundary.kind])
+            setattr(s, side + "_value", boundary.value)
+        # simulator.py:1849-1891
+        u_lat = self.scales.reference_lattice_velocity
+        u_ref = self.scales.reference_velocity_m_s
+        c_u = self.scales.velocity_scale_m_s
+        self.dt = self.scales.dt_s
+        s.rho_w, s.rho_a = 1.0, float(config.rho_air / config.rho_water)
+        s.nu_w = float(config.viscosity_water * u_lat / (config.dx * u_ref))
+        s.nu_a = float(config.viscosity_air * u_lat / (config.dx * u_ref))
+        s.gx = float(config.gravity[0] * config.dx / (c_u * c_u))
+        s.gy = float(config.gravity[1] * config.dx / (c_u * c_u))
+        s.sigma = float(config.sigma * u_lat * u_lat / (u_ref * u_ref) / config.dx / config.rho_water)
+        s.width, s.mobility, s.smagorinsky = config.interface_width, config.mobility, config.smagorinsky_constant
+        tau = config.air_interface_relaxation_time
+        s.interface_tau = 0.5 if tau is None else float(tau)
+        s.cutoff = config.volume_projection_interface_cutoff
+        s.threshold = th.solid_liquid_threshold
+        s.dx = config.dx
+        s.expansion, s.t_ref = th.thermal_expansion_water_1_k, th.buoyancy_reference_temperature_c
+        s.linear_damping, s.angular_damping = config.linear_damping, config.angular_damping
+        s.initial_mass, s.initial_inertia = config.ice_mass_lattice, config.ice_inertia_lattice
+        s.t_melt, s.cp_w, s.cp_i = props.melting_temperature_c, props.specific_heat_water_j_kg_k, props.specific_heat_ice_j_kg_k
+        s.k_w, s.k_i, s.latent = props.conductivity_water_w_m_k, props.conductivity_ice_w_m_k, props.latent_heat_j_kg
+        s.rho_w_si = config.rho_water
+        s.area = config.dx * config.dx
+        s.velocity_scale = c_u
+        s.cell_mass = config.rho_ice * s.area
+        s.t_air = th.initial_air_temperature_c
+        cutoff = config.volume_projection_interface_cutoff
+        radius = 0.25 * config.interface_width * math.log((1.0 - cutoff) / cutoff)
+        self.band_radius = max(2, int(math.ceil(radius)))
+        self.band_radius += self.band_radius % 2
+        alpha = max(s.k_w / (config.rho_water * s.cp_w), s.k_i / (config.rho_ice * s.cp_i))
+        self.max_diffusion_dt = th.max_fourier_number * s.area / alpha
+        self._dirichlet = DIRICHLET
+
+        self.arrays = {}
+
+        def alloc(name, shape, dtype, kind):
+            array = np.zeros(shape, dtype=dtype)
+            self.arrays[name] = array
+            setattr(s, name, _ptr(array, kind))
+            return array
+
+        for name in ("f", "fbuf", "h", "hbuf"):
+            alloc(name, 9 * N, np.float32, _F)
+        for name in ("phi", "p", "coverage"):
+            alloc(name, N, np.float32, _F)
+        for name in ("u", "a"):
+            alloc(name, 2 * N, np.float32, _F)
+        for name in ("p_ref", "rho_ref"):
+            alloc(name, ny, np.float32, _F)
+        for name in ("solid", "prev_solid", "eligible"):
+            alloc(name, N, np.int8, _I8)
+        for name in ("body_mass", "body_energy", "body_heat", "melt_mass_step", "melt_sensible_step"):
+            alloc(name, bx * by, np.float64, _D)
+        alloc("melt_target", bx * by, np.int32, _I32)
+        for name in ("water_v", "water_e", "target_e"):
+            alloc(name, N, np.float64, _D)
+        for name in ("flux_ex", "flux_vx"):
+            alloc(name, (nx + 1) * ny, np.float64, _D)
+        for name in ("flux_ey", "flux_vy"):
+            alloc(name, nx * (ny + 1), np.float64, _D)
+
+        self._thermal_pending = 0
+        self._melt_pending = False
+        self._initialize()
+
+    def _k(self, name, *args):
+        return getattr(self.lib, "o2d_" + name)(self.ps, *args)
+
+    # ------------------------------------------------------------ init
+    def _initialize(self):
+        """simulator.py:2159-2192."""
+        c, s = self.config, self.s
+        s.velocity[0], s.velocity[1] = c.ice_initial_velocity
+        s.angle, s.omega = c.ice_initial_angle, c.ice_initial_angular_velocity
+        s.mass, s.inertia = c.ice_mass_lattice, c.ice_inertia_lattice
+        s.origin[0], s.origin[1] = c.ice_initial_center
+        self._k("initialize_body_state", c.thermal.initial_ice_temperature_c)
+        self._k("contact_and_rasterize", 1)
+        self._k("update_solid_mask", 1)
+        self._k("initialize_fluid", c.water_width, c.water_height)
+        self._k("initialize_water_volume")
+        for _ in range(c.phase_warmup_steps):
+            self._k("collide_phase")
+            self._k("stream", 1)
+            self._k("update_streamed_phase")
+        self._k("recover_initial_moments")
+        self.correct_water_volume()
+        self._k("initialize_body_state", c.thermal.initial_ice_temperature_c)
+        self._k("initialize_water_state", c.thermal.initial_water_temperature_c)
+        self._update_mass_properties(initialize=True)
+        self._build_hydrostatic_reference()
+        self._k("hydrostatic_equilibrium")
+
+    def _build_hydrostatic_reference(self):
+        """simulator.py:2626-2662 (host NumPy FP64)."""
+        c, s = self.config, self.s
+        phase = np.clip(self.arrays["phi"].reshape(self.nx, self.ny).astype(np.float64), 0.0, 1.0)
+        active = (self.wall_mask() == 0) & (self.arrays["solid"].reshape(self.nx, self.ny) == 0)
+        count = np.count_nonzero(active, axis=0)
+        profile = (np.arange(self.ny) < c.water_height).astype(np.float64)
+        np.divide(np.where(active, phase, 0.0).sum(axis=0), count, out=profile, where=count > 0)
+        density = s.rho_a + (s.rho_w - s.rho_a) * profile
+        pressure = np.zeros(self.ny)
+        top = c.water_height
+        pressure[top - 1] = -0.5 * s.gy * density[top - 1]
+        for row in range(top - 2, -1, -1):
+            pressure[row] = pressure[row + 1] - 0.5 * s.gy * (density[row] + density[row + 1])
+        pressure[top] = 0.5 * s.gy * density[top]
+        for row in range(top + 1, self.ny):
+            pressure[row] = pressure[row - 1] + 0.5 * s.gy * (density[row - 1] + density[row])
+        if not (np.isfinite(density).all() and np.isfinite(pressure).all()):
+            raise RuntimeError("hydrostatic reference construction produced non-finite values")
+        self.arrays["rho_ref"][:] = density.astype(np.float32)
+        self.arrays["p_ref"][:] = pressure.astype(np.float32)
+
+    def wall_mask(self):
+        b = self.config.boundary_cells
+        x, y = np.ogrid[: self.nx, : self.ny]
+        return ((x < b) | (x >= self.nx - b) | (y < b) | (y >= self.ny - b)).astype(np.int8)
+
+    def _update_mass_properties(self, initialize=False):
+        """simulator.py:2729-2733."""
+        self._k("reduce_mass_properties")
+        self._k("apply_mass_properties", int(initialize))
+
+    # ------------------------------------------------------------ step
+    def step(self, num_steps=1):
+        """simulator.py:2194-2227."""
+        if isinstance(num_steps, bool) or not isinstance(num_steps, (int, np.integer)) or num_steps < 0:
+            raise ValueError("num_steps must be a non-negative integer")
+        for _ in range(int(num_steps)):
+            self._k("collide_momentum")
+            self._k("collide_phase")
+            self._k("stream", 0)
+            self._k("update_macroscopic")
+            self._k("update_pressure")
+            self._k("integrate_rigid")
+            self._k("contact_and_rasterize", 1)
+            self._k("update_solid_mask", 0)
+            self._k("refill")
+            self._advance_fast()
+            self._thermal_pending += 1
+            if self._thermal_pending >= self.config.thermal.update_interval_lbm_steps:
+                self._advance_slow(self._thermal_pending)
+                self._thermal_pending = 0
+            self.correct_water_volume()
+            self._finalize_melt_momentum()
+            self.synchronize_water_aperture()
+            self.steps += 1
+
+    @property
+    def physical_time_s(self):
+        return self.steps * self.dt
+
+    def _substeps(self, load, process, dt, outflow=None):
+        """thermal substep count and limit (simulator.py:342-401)."""
+        count = max(1, int(math.ceil(load - 1.0e-14)))
+        limit = self.config.thermal.max_substeps_per_update
+        if count > limit:
+            detail = f", maximum_outflow_rate_lattice={outflow!r}" if outflow is not None else ""
+            raise RuntimeError(
+                f"thermal stability requires {count} {process} substeps for one coupling update, exceeding "
+                f"max_substeps_per_update={limit}; time_step_s={dt:.9g}{detail}. Reduce the thermal update "
+                "interval or the lattice velocity, and inspect the LBM state before raising this safety limit.")
+        return count
+
+    def _advance_fast(self):
+        """simulator.py:2233-2247 and :418-453."""
+        self._k("measure_outflow")
+        speed = float(self.s.max_outflow)
+        th = self.config.thermal
+        load = 0.0
+        if speed < 0.0 or not math.isfinite(speed):
+            if not speed >= 0.0:
+                raise ValueError("maximum_outflow_rate_lattice must be non-negative")
+        if th.advection_enabled and speed != 0.0:
+            limit = th.max_courant_number * self.dt / speed
+            if math.isfinite(limit):
+                load = self.dt / limit
+        substeps = self._substeps(load, "advection", self.dt, speed)
+        self.synchronize_water_aperture()
+        sub = self.dt / substeps
+        for k in range(substeps):
+            self._k("advection_flux", 0)
+            self._k("advection_flux", 1)
+            self._k("update_water", sub, 1)
+            if k + 1 < substeps:
+                self.synchronize_water_aperture()
+
+    def _advance_slow(self, count):
+        """simulator.py:2249-2286 and :455-509."""
+        if self._melt_pending:
+            raise RuntimeError("a moving thermal momentum interval is still pending phase projection")
+        self.synchronize_water_aperture()
+        self._k("snapshot_melt_momentum")
+        dt = count * self.dt
+        substeps = self._substeps(dt / self.max_diffusion_dt, "diffusion", dt)
+        sub = dt / substeps
+        self.s.interval_body_melt = self.s.interval_water_melt = 0.0
+        for k in range(substeps):
+            self._k("body_conduction", sub)
+            self.s.boundary_power = 0.0
+            self._k("conduction_flux", 0)
+            self._k("conduction_flux", 1)
+            self._k("update_water", sub, 0)
+            self.s.boundary_heat_input += sub * self.s.boundary_power
+            self._k("exchange_heat", sub)
+            self._k("phase_change")
+            self._k("freeze_eligibility")
+            self._k("inject_melt")
+            self._distribute_unassigned_melt()
+            self.synchronize_water_aperture()
+            if k + 1 < substeps:
+                self._k("contact_and_rasterize", 0)
+        self._k("contact_and_rasterize", 0)
+        self._update_mass_properties()
+        self._k("update_solid_mask", 0)
+        self._k("refill")
+        self._melt_pending = True
+
+    def _distribute_unassigned_melt(self):
+        """simulator.py:1542-1563."""
+        if float(self.s.unassigned_mass) <= 0.0:
+            return
+        self._k("measure_fallback")
+        if self.s.fallback_free > 1.0e-30:
+            self._k("apply_fallback", 1)
+        elif self.s.fallback_wet > 1.0e-30:
+            self._k("apply_fallback", 0)
+        else:
+            raise RuntimeError("melt water has no legal LBM water cell for conservative mass and energy injection")
+
+    def synchronize_water_aperture(self):
+        """simulator.py:511-548."""
+        out = (ctypes.c_double * 2)()
+        self._k("measure_aperture", out)
+        total, full = out[0], out[1]
+        tolerance = 1.0e-12 * max(self.s.area, abs(total), abs(full))
+        if total < -tolerance:
+            raise RuntimeError(f"thermal water advection produced a negative total volume: {total:.17g} m^2")
+        if total > full + tolerance:
+            raise RuntimeError("thermal water volume exceeds the current fluid aperture capacity: "
+                               f"volume={total:.17g} m^2, capacity={full:.17g} m^2")
+        self._k("build_aperture_targets")
+        self._k("apply_aperture_transfer")
+        self._k("finish_aperture_transfer")
+
+    def _finalize_melt_momentum(self):
+        """simulator.py:2989-3012."""
+        if not self._melt_pending:
+            return
+        self._k("reduce_provisional_momentum")
+        status = self.lib.o2d_melt_weight_geometry(self.ps)
+        if status == -1:
+            raise RuntimeError("melt momentum has no legal wet LBM cell for conservative linear/angular-momentum closure")
+        if status == -2:
+            raise RuntimeError("melt momentum correction requires two distinct wet cells")
+        if status == 1:
+            self._k("apply_melt_correction")
+        self._k("reduce_final_momentum")
+        for _ in range(2):
+            self._k("local_melt_correction")
+            self._k("reduce_final_momentum")
+        self._k("finish_melt_momentum")
+        self._melt_pending = False
+
+    def water_volume_target(self):
+        """simulator.py:4922-4927."""
+        return self.s.initial_water_volume + self.config.ice_mass_lattice - self.s.mass
+
+    def correct_water_volume(self):
+        """simulator.py:4494-4560."""
+        c = self.config
+        current = self.lib.o2d_clip_phase(self.ps, MAX_BULK_PHASE_L1)
+        target = self.water_volume_target()
+        tolerance = c.volume_projection_tolerance * max(1.0, abs(target))
+        initial_error = target - current
+        if abs(initial_error) <= tolerance:
+            return
+        self._k("seed_projection")
+        for k in range(self.band_radius):
+            self._k("dilate_projection", 1 if k % 2 == 0 else 0)
+        limit = 4.0 * c.volume_projection_max_shift / c.interface_width
+        out = (ctypes.c_double * 2)()
+
+        def evaluate(lam):
+            self._k("evaluate_projection", lam, out)
+            return out[0], out[1]
+
+        lower_mass, upper_mass = evaluate(-limit)[0], evaluate(limit)[0]
+        if abs(upper_mass - lower_mass) <= tolerance:
+            raise RuntimeError("water-volume projection is infeasible: no adjustable phi=0.5-connected interface is available")
+        if target < lower_mass - tolerance or target > upper_mass + tolerance:
+            raise RuntimeError("water-volume projection exceeds the permitted interface shift: "
+                               f"target={target:.17g}, reachable=[{lower_mass:.17g}, {upper_mass:.17g}]")
+        lo, hi, lam, best_lam, best = -limit, limit, 0.0, 0.0, abs(initial_error)
+        for _ in range(c.volume_projection_max_iterations):
+            mass, derivative = evaluate(lam)
+            residual = mass - target
+            if abs(residual) < best:
+                best, best_lam = abs(residual), lam
+            if abs(residual) <= tolerance:
+                best_lam = lam
+                break
+            if residual < 0.0:
+                lo = lam
+            else:
+                hi = lam
+            proposal = math.nan
+            if derivative > 0.0 and math.isfinite(derivative):
+                proposal = lam - residual / derivative
+            if not math.isfinite(proposal) or not lo < proposal < hi:
+                proposal = 0.5 * (lo + hi)
+            lam = proposal
+        current = self.lib.o2d_apply_projection(self.ps, best_lam)
+        for _ in range(8):
+            residual = target - current
+            if abs(residual) <= tolerance:
+                break
+            weight = self.lib.o2d_projection_residual_weight(self.ps, residual)
+            if not math.isfinite(weight) or weight <= 0.0:
+                break
+            current = self.lib.o2d_apply_projection_residual(self.ps, residual, weight)
+        error = abs(target - current)
+        if error > tolerance:
+            raise RuntimeError("water-volume projection cannot close its endpoint active set: "
+                               f"residual={error:.6e}, tolerance={tolerance:.6e}")
+
+    # --------------------------------------------------------- views
+    def state(self):
+        """Logical views in the reference's layouts ([nx, ny(, c)])."""
+        nx, ny, a = self.nx, self.ny, self.arrays
+        s = self.s
+
+        def planes(name, count):
+            return np.moveaxis(a[name].reshape(count, nx, ny), 0, -1).copy()
+
+        center = self.body_center()
+        return {
+            "momentum_populations": planes("f", 9),
+            "phase_populations": planes("h", 9),
+            "water_phase": a["phi"].reshape(nx, ny).copy(),
+            "momentum_velocity_lattice": planes("u", 2),
+            "fluid_acceleration_lattice": planes("a", 2),
+            "pressure_lattice": a["p"].reshape(nx, ny).copy(),
+            "solid_mask": a["solid"].reshape(nx, ny).copy(),
+            "coverage": a["coverage"].reshape(nx, ny).copy(),
+            "water_volume_m2": a["water_v"].reshape(nx, ny).copy(),
+            "water_sensible_energy": a["water_e"].reshape(nx, ny).copy(),
+            "body_solid_mass": a["body_mass"].reshape(self.config.ice_width, self.config.ice_height).copy(),
+            "body_sensible_energy": a["body_energy"].reshape(self.config.ice_width, self.config.ice_height).copy(),
+            "body_center": center,
+            "body_velocity": np.array(s.velocity[:], dtype=np.float32),
+            "body_angle": np.float32(s.angle),
+            "body_angular_velocity": np.float32(s.omega),
+            "body_mass_lattice": float(s.mass),
+            "body_inertia_lattice": float(s.inertia),
+        }
+
+    def body_center(self):
+        """simulator.py:4846-4856."""
+        s = self.s
+        angle = np.float32(s.angle)
+        c, sn = np.cos(angle), np.sin(angle)
+        lx, ly = s.local_com[0], s.local_com[1]
+        off = np.array([c * lx - sn * ly, sn * lx + c * ly])
+        return (np.array(s.origin[:], dtype=np.float64) + off).astype(np.float32)
+
+    def mass_energy_totals(self):
+        """simulator.py:550-571: (solid mass, water mass, body sensible, water sensible, latent) per metre."""
+        out = (ctypes.c_double * 4)()
+        self._k("reduce_totals", out)
+        initial = self.s.cell_mass * self.config.ice_width * self.config.ice_height
+        melted = initial - out[0]
+        return {
+            "initial_body_mass_kg_m": initial, "solid_body_mass_kg_m": out[0], "melted_mass_kg_m": melted,
+            "water_mass_kg_m": out[1], "total_mass_kg_m": out[0] + out[1], "body_sensible_energy_j_m": out[2],
+            "water_sensible_energy_j_m": out[3], "latent_energy_j_m": self.s.latent * melted,
+            "total_energy_j_m": out[2] + out[3] + self.s.latent * melted,
+        }

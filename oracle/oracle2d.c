@@ -1,0 +1,1575 @@
+/* Oracle (test infrastructure only): FP64 restatement of the reference 2D
+ * coupled step, one function per reference kernel, each citing the
+ * iceflow2d file:line it follows.  Loops are sequential.  The reference
+ * kernels are race-free apart from their atomics (push streaming writes
+ * each destination slot once; every other kernel writes only its own cell
+ * or a reduction), so sequential order reproduces the parallel semantics
+ * and the atomics become exact-order FP64 sums.  Fields the reference
+ * keeps in FP32 are float here: every store rounds like the reference's. */
+#include "oracle2d.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define Q9 9
+#define ADIABATIC 0
+#define DIRICHLET 1
+
+/* lattice.py:8-62 */
+static const int CX[Q9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+static const int CY[Q9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+static const int OPP[Q9] = {0, 3, 4, 1, 2, 7, 8, 5, 6};
+static double W9(int q) { return q == 0 ? (double)(float)(4.0 / 9.0) : q <= 4 ? (double)(float)(1.0 / 9.0) : (double)(float)(1.0 / 36.0); }
+
+static long N_(const o2d* s) { return (long)s->nx * s->ny; }
+static long at(const o2d* s, int i, int j) { return (long)i * s->ny + j; }
+static int inside(const o2d* s, int i, int j) { return i >= 0 && i < s->nx && j >= 0 && j < s->ny; }
+static double clamp01(double v) { return fmin(1.0, fmax(0.0, v)); }
+
+/* simulator.py:4566-4574 */
+static int wall_at(const o2d* s, int i, int j) {
+  int b = s->boundary;
+  return i < b || i >= s->nx - b || j < b || j >= s->ny - b;
+}
+/* simulator.py:2300-2301 */
+static int fluid_at(const o2d* s, int i, int j) { return !wall_at(s, i, j) && s->solid[at(s, i, j)] == 0; }
+
+/* lattice.py:64-87 */
+static double momentum_eq(int q, double p, double rho, double ux, double uy) {
+  double cu = CX[q] * ux + CY[q] * uy, uu = ux * ux + uy * uy;
+  double value = 3.0 * W9(q) * p + W9(q) * rho * (3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
+  if (q == 0) value -= 3.0 * p;
+  return value;
+}
+/* lattice.py:90-104 */
+static double phase_eq(int q, double phi, double ux, double uy) {
+  double cu = CX[q] * ux + CY[q] * uy, uu = ux * ux + uy * uy;
+  return W9(q) * phi * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * uu);
+}
+/* lattice.py:112-115 */
+static double mixture(const o2d* s, double phi) {
+  double w = clamp01(phi);
+  return s->rho_w * w + s->rho_a * (1.0 - w);
+}
+/* lattice.py:118-147 */
+static double basis(int c, double u, int order) {
+  if (c == 0) return order == 0 ? 1.0 - u * u : order == 1 ? -2.0 * u : -1.0;
+  if (c == 1) return order == 0 ? 0.5 * u * (u + 1.0) : order == 1 ? u + 0.5 : 0.5;
+  return order == 0 ? 0.5 * u * (u - 1.0) : order == 1 ? u - 0.5 : 0.5;
+}
+/* lattice.py:150-177; m = {00,10,01,20,02,11,21,12,22} */
+static double reconstruct_cm(int cx, int cy, double ux, double uy, const double m[9]) {
+  double bx0 = basis(cx, ux, 0), bx1 = basis(cx, ux, 1), bx2 = basis(cx, ux, 2);
+  double by0 = basis(cy, uy, 0), by1 = basis(cy, uy, 1), by2 = basis(cy, uy, 2);
+  return bx0 * by0 * m[0] + bx1 * by0 * m[1] + bx0 * by1 * m[2] + bx2 * by0 * m[3] + bx0 * by2 * m[4] +
+         bx1 * by1 * m[5] + bx2 * by1 * m[6] + bx1 * by2 * m[7] + bx2 * by2 * m[8];
+}
+
+/* ------------------------------------------------------------------ body */
+static void rotate(double angle, double x, double y, double* ox, double* oy) {
+  double c = cos(angle), sn = sin(angle);
+  *ox = c * x - sn * y;
+  *oy = sn * x + c * y;
+}
+/* simulator.py:4835-4845 (_body_center_at): f64 sum, f32 result */
+static void body_center(const o2d* s, double c[2]) {
+  double ox, oy;
+  rotate((double)s->angle, s->local_com[0], s->local_com[1], &ox, &oy);
+  c[0] = (float)((double)s->origin[0] + ox);
+  c[1] = (float)((double)s->origin[1] + oy);
+}
+/* simulator.py:2449-2453 */
+static void body_velocity_at(const o2d* s, double px, double py, double v[2]) {
+  double c[2];
+  body_center(s, c);
+  double rx = px - c[0], ry = py - c[1];
+  v[0] = s->velocity[0] - s->omega * ry;
+  v[1] = s->velocity[1] + s->omega * rx;
+}
+static int body_active(const o2d* s) { return s->inertia > 0.0; }
+/* simulator.py:2482-2496 */
+static void local_coordinates(const o2d* s, double px, double py, double* lx, double* ly) {
+  double c = cos((double)s->angle), sn = sin((double)s->angle);
+  double rx = px - s->origin[0], ry = py - s->origin[1];
+  *lx = c * rx + sn * ry;
+  *ly = -sn * rx + c * ry;
+}
+/* simulator.py:1586-1594 */
+static double body_fraction(const o2d* s, int bi, int bj) {
+  double mass = fmin(s->cell_mass, fmax(0.0, s->body_mass[(long)bi * s->body_ny + bj]));
+  double fraction = mass / s->cell_mass;
+  if (mass > 0.0) fraction = fmax(fraction, 1.0e-30);
+  return (float)fraction;
+}
+/* simulator.py:1596-1603 */
+static double body_temperature(const o2d* s, int bi, int bj) {
+  long k = (long)bi * s->body_ny + bj;
+  double t = s->t_melt, mass = s->body_mass[k];
+  if (mass > 1.0e-30) t += s->body_energy[k] / (mass * s->cp_i);
+  return fmin(s->t_melt, t);
+}
+/* simulator.py:1605-1613 */
+static double water_temperature(const o2d* s, long n) {
+  double v = s->water_v[n];
+  if (v > 1.0e-30) return s->t_melt + s->water_e[n] / (s->rho_w_si * s->cp_w * v);
+  return s->t_air;
+}
+/* simulator.py:2508-2530 */
+static double sample_fraction(const o2d* s, double lx, double ly) {
+  double gx = lx + 0.5 * s->body_nx - 0.5, gy = ly + 0.5 * s->body_ny - 0.5;
+  int bi0 = (int)floor(gx), bj0 = (int)floor(gy);
+  double fraction = 0.0;
+  for (int di = 0; di < 2; ++di)
+    for (int dj = 0; dj < 2; ++dj) {
+      int bi = bi0 + di, bj = bj0 + dj;
+      if (bi >= 0 && bi < s->body_nx && bj >= 0 && bj < s->body_ny) {
+        double wx = 1.0 - fabs(gx - bi), wy = 1.0 - fabs(gy - bj);
+        fraction += fmax(0.0, wx) * fmax(0.0, wy) * body_fraction(s, bi, bj);
+      }
+    }
+  return clamp01(fraction);
+}
+/* simulator.py:4587-4603 (in lattice cells; the reference scales by dx,
+ * which does not change any sign or ratio it is used for) */
+static double signed_distance(const o2d* s, int i, int j) {
+  double lx, ly;
+  local_coordinates(s, i + 0.5, j + 0.5, &lx, &ly);
+  double hw = 0.5 * s->body_nx, hh = 0.5 * s->body_ny;
+  int bi = (int)floor(lx + hw), bj = (int)floor(ly + hh);
+  double dx = fabs(lx) - hw, dy = fabs(ly) - hh;
+  double ox = fmax(dx, 0.0), oy = fmax(dy, 0.0);
+  double d = sqrt(ox * ox + oy * oy) + fmin(fmax(dx, dy), 0.0);
+  if (bi >= 0 && bi < s->body_nx && bj >= 0 && bj < s->body_ny) d = s->threshold - s->coverage[at(s, i, j)];
+  return d;
+}
+
+/* simulator.py:2532-2537 */
+void o2d_body_fraction(o2d* s) {
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      double lx, ly;
+      local_coordinates(s, i + 0.5, j + 0.5, &lx, &ly);
+      s->coverage[at(s, i, j)] = (float)sample_fraction(s, lx, ly);
+    }
+}
+/* simulator.py:2539-2544 */
+void o2d_commit_raster(o2d* s) {
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      if (wall_at(s, i, j)) s->coverage[at(s, i, j)] = 0.0f;
+}
+/* simulator.py:3914-3971; reduce = 0 only resets the extrema */
+void o2d_contact_support(o2d* s, int reduce) {
+  double e[4] = {1.0e30, -1.0e30, 1.0e30, -1.0e30};
+  if (reduce) {
+    for (int i = 0; i < s->nx; ++i)
+      for (int j = 0; j < s->ny; ++j) {
+        if (s->coverage[at(s, i, j)] >= s->threshold && signed_distance(s, i, j) <= 0.0) {
+          e[0] = fmin(e[0], (float)(i - (double)s->origin[0]));
+          e[1] = fmax(e[1], (float)(i + 1 - (double)s->origin[0]));
+          e[2] = fmin(e[2], (float)(j - (double)s->origin[1]));
+          e[3] = fmax(e[3], (float)(j + 1 - (double)s->origin[1]));
+        }
+      }
+    if (!(e[2] < 1.0e20)) e[0] = e[1] = e[2] = e[3] = 0.0;
+  }
+  for (int k = 0; k < 4; ++k) s->support[k] = (float)e[k];
+}
+/* simulator.py:3973-4017 */
+void o2d_integrate_rigid(o2d* s) {
+  if (body_active(s)) {
+    double mass = (float)s->mass, inertia = (float)s->inertia;
+    double jx = (float)s->impulse[0] + mass * s->gx, jy = (float)s->impulse[1] + mass * s->gy;
+    double torque = (float)s->torque;
+    double vx = s->linear_damping * (s->velocity[0] + jx / mass);
+    double vy = s->linear_damping * (s->velocity[1] + jy / mass);
+    double omega = s->angular_damping * (s->omega + torque / inertia);
+    double lcx = (float)s->local_com[0], lcy = (float)s->local_com[1];
+    double angle = s->angle + omega, c[2];
+    body_center(s, c);
+    c[0] += vx;
+    c[1] += vy;
+    s->velocity[0] = (float)vx;
+    s->velocity[1] = (float)vy;
+    s->angle = (float)angle;
+    s->omega = (float)omega;
+    double ox, oy;
+    rotate((double)s->angle, lcx, lcy, &ox, &oy);
+    s->origin[0] = (float)(c[0] - ox);
+    s->origin[1] = (float)(c[1] - oy);
+  } else {
+    s->velocity[0] = s->velocity[1] = 0.0f;
+    s->omega = 0.0f;
+  }
+  s->impulse[0] = s->impulse[1] = 0.0;
+  s->torque = 0.0;
+}
+/* simulator.py:4019-4080 */
+void o2d_project_inside(o2d* s) {
+  s->projected = 0;
+  double c[2], v[2] = {s->velocity[0], s->velocity[1]};
+  body_center(s, c);
+  double rcx, rcy;
+  rotate((double)s->angle, (float)s->local_com[0], (float)s->local_com[1], &rcx, &rcy);
+  if (body_active(s) && s->support[1] > s->support[0]) {
+    double lower = s->boundary, upper_x = s->nx - s->boundary, upper_y = s->ny - s->boundary;
+    double lo_x = lower - (s->support[0] - rcx), hi_x = upper_x - (s->support[1] - rcx);
+    double lo_y = lower - (s->support[2] - rcy), hi_y = upper_y - (s->support[3] - rcy);
+    if (c[0] <= lo_x) {
+      if (c[0] < lo_x) s->projected = 1;
+      c[0] = lo_x;
+      v[0] = fmax(v[0], 0.0);
+    } else if (c[0] >= hi_x) {
+      if (c[0] > hi_x) s->projected = 1;
+      c[0] = hi_x;
+      v[0] = fmin(v[0], 0.0);
+    }
+    if (c[1] <= lo_y) {
+      if (c[1] < lo_y) s->projected = 1;
+      c[1] = lo_y;
+      v[1] = fmax(v[1], 0.0);
+    } else if (c[1] >= hi_y) {
+      if (c[1] > hi_y) s->projected = 1;
+      c[1] = hi_y;
+      v[1] = fmin(v[1], 0.0);
+    }
+  }
+  s->velocity[0] = (float)v[0];
+  s->velocity[1] = (float)v[1];
+  if (s->projected) {
+    s->origin[0] = (float)(c[0] - rcx);
+    s->origin[1] = (float)(c[1] - rcy);
+  }
+}
+/* simulator.py:4082-4123 */
+void o2d_contact_and_rasterize(o2d* s, int resolve_contact) {
+  o2d_body_fraction(s);
+  if (resolve_contact) {
+    o2d_contact_support(s, 1);
+    o2d_project_inside(s);
+    if (s->projected) {
+      o2d_body_fraction(s);
+      o2d_contact_support(s, 1);
+    }
+  } else {
+    s->projected = 0;
+    o2d_contact_support(s, 0);
+  }
+  o2d_commit_raster(s);
+}
+/* simulator.py:2699-2711 (init = 1) and :2713-2727 */
+void o2d_update_solid_mask(o2d* s, int init) {
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      int solid = body_active(s) && !wall_at(s, i, j) && signed_distance(s, i, j) <= 0.0;
+      if (!init) s->prev_solid[n] = s->solid[n];
+      s->solid[n] = (int8_t)solid;
+      if (init) s->prev_solid[n] = s->solid[n];
+    }
+}
+
+/* ------------------------------------------------------------- init */
+/* simulator.py:2595-2624 */
+void o2d_initialize_fluid(o2d* s, int water_width, int water_height) {
+  long N = N_(s);
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      double indicator = (i < water_width && j < water_height) ? 1.0 : 0.0;
+      int active = fluid_at(s, i, j);
+      double phi0 = active ? indicator : 0.0, v[2] = {0.0, 0.0};
+      if (s->solid[n] == 1) body_velocity_at(s, i + 0.5, j + 0.5, v);
+      s->phi[n] = (float)(s->solid[n] == 1 ? indicator : phi0);
+      s->u[n] = (float)v[0];
+      s->u[N + n] = (float)v[1];
+      s->a[n] = s->a[N + n] = 0.0f;
+      s->p[n] = 0.0f;
+      double rho = mixture(s, phi0);
+      for (int q = 0; q < Q9; ++q) {
+        s->f[q * N + n] = (float)momentum_eq(q, 0.0, rho, v[0], v[1]);
+        s->h[q * N + n] = active ? (float)phase_eq(q, phi0, v[0], v[1]) : 0.0f;
+      }
+    }
+}
+/* simulator.py:2664-2697 */
+void o2d_hydrostatic_equilibrium(o2d* s) {
+  long N = N_(s);
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      if (fluid_at(s, i, j)) {
+        double phi = s->phi[n], rho = mixture(s, phi);
+        s->u[n] = s->u[N + n] = 0.0f;
+        s->p[n] = s->p_ref[j];
+        for (int q = 0; q < Q9; ++q) {
+          s->f[q * N + n] = (float)momentum_eq(q, 0.0, rho, 0.0, 0.0);
+          s->h[q * N + n] = (float)phase_eq(q, phi, 0.0, 0.0);
+        }
+      } else {
+        double v[2] = {0.0, 0.0};
+        if (s->solid[n] == 1) body_velocity_at(s, i + 0.5, j + 0.5, v);
+        s->u[n] = (float)v[0];
+        s->u[N + n] = (float)v[1];
+        s->p[n] = 0.0f;
+      }
+    }
+}
+/* simulator.py:4125-4133 */
+void o2d_initialize_water_volume(o2d* s) {
+  double total = 0.0;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j)
+      if (fluid_at(s, i, j)) total += (double)s->phi[at(s, i, j)];
+  s->initial_water_volume = total;
+  s->water_volume_current = total;
+}
+
+/* --------------------------------------------------------- flow kernels */
+/* simulator.py:2459-2477 */
+static double phase_neighbor(const o2d* s, int i, int j, int sx, int sy, int distance) {
+  double value = s->phi[at(s, i, j)];
+  int i1 = i + sx, j1 = j + sy;
+  if (inside(s, i1, j1) && fluid_at(s, i1, j1)) {
+    value = s->phi[at(s, i1, j1)];
+    if (distance == 2) {
+      int i2 = i + 2 * sx, j2 = j + 2 * sy;
+      value = (inside(s, i2, j2) && fluid_at(s, i2, j2)) ? s->phi[at(s, i2, j2)] : s->phi[at(s, i, j)];
+    }
+  }
+  return value;
+}
+/* blended isotropic gradient used by simulator.py:3263-3290 and :3566-3589 */
+static void blended_gradient(const o2d* s, int i, int j, double g[2], double source[2], double* laplacian) {
+  double phi = s->phi[at(s, i, j)];
+  const double alpha = 1.0 / 3.0;
+  g[0] = g[1] = source[0] = source[1] = 0.0;
+  *laplacian = 0.0;
+  for (int q = 0; q < Q9; ++q) {
+    double p1 = phase_neighbor(s, i, j, CX[q], CY[q], 1), p2 = phase_neighbor(s, i, j, CX[q], CY[q], 2);
+    double w = W9(q);
+    for (int e = 0; e < 2; ++e) {
+      double c = e == 0 ? CX[q] : CY[q];
+      source[e] += 3.0 * w * c * (p1 - phi);
+      g[e] += (1.0 - alpha) * 3.0 * w * c * (p1 - phi);
+      g[e] += alpha * 1.5 * w * c * (4.0 * p1 - p2 - 3.0 * phi);
+    }
+    *laplacian += 6.0 * w * (p1 - phi);
+  }
+}
+/* simulator.py:3241-3368: acceleration (force / rho), Smagorinsky nu, source gradient */
+static void fluid_force(const o2d* s, int i, int j, double out[5]) {
+  long N = N_(s), n = at(s, i, j);
+  double phi = s->phi[n], bounded = clamp01(phi), rho = s->rho_w * bounded + s->rho_a * (1.0 - bounded);
+  double g[2], source[2], lap;
+  blended_gradient(s, i, j, g, source, &lap);
+  double beta = 12.0 * s->sigma / s->width, kappa = 1.5 * s->sigma * s->width;
+  double fx = (rho - s->rho_ref[j]) * s->gx, fy = (rho - s->rho_ref[j]) * s->gy;
+  double temperature = (float)water_temperature(s, n);
+  double anomaly = -s->expansion * (temperature - s->t_ref);
+  double ww = clamp01(2.0 * bounded - 1.0);
+  fx += ww * s->rho_w * anomaly * s->gx;
+  fy += ww * s->rho_w * anomaly * s->gy;
+  fx /= rho;
+  fy /= rho;
+  double chemical = 4.0 * beta * bounded * (bounded - 1.0) * (bounded - 0.5) - kappa * lap;
+  fx += chemical * g[0] / rho;
+  fy += chemical * g[1] / rho;
+  double ux = s->u[n], uy = s->u[N + n];
+  double gxx = 0, gxy = 0, gyx = 0, gyy = 0;
+  for (int q = 0; q < Q9; ++q) {
+    int ni = i + CX[q], nj = j + CY[q];
+    double v[2] = {0.0, 0.0};
+    if (inside(s, ni, nj) && fluid_at(s, ni, nj)) {
+      v[0] = s->u[at(s, ni, nj)];
+      v[1] = s->u[N + at(s, ni, nj)];
+    } else if (inside(s, ni, nj) && s->solid[at(s, ni, nj)] == 1) {
+      body_velocity_at(s, ni + 0.5, nj + 0.5, v);
+      v[0] = (float)v[0];
+      v[1] = (float)v[1];
+    }
+    double dxv = v[0] - ux, dyv = v[1] - uy, w3 = 3.0 * W9(q);
+    gxx += w3 * CX[q] * dxv;
+    gxy += w3 * CY[q] * dxv;
+    gyx += w3 * CX[q] * dyv;
+    gyy += w3 * CY[q] * dyv;
+  }
+  double cs = s->smagorinsky;
+  double nu_t = cs * cs * sqrt(2.0 * (gxx * gxx + gyy * gyy + 0.5 * (gxy + gyx) * (gxy + gyx)));
+  out[0] = (float)fx;
+  out[1] = (float)fy;
+  out[2] = (float)nu_t;
+  out[3] = (float)source[0];
+  out[4] = (float)source[1];
+}
+/* simulator.py:4782-4796 */
+static double cut_fraction(const o2d* s, int i, int j, int ni, int nj) {
+  double fluid = fmax(signed_distance(s, i, j) * s->dx, 1.0e-6);
+  return fmin(0.95, fmax(0.05, fluid / (fluid - signed_distance(s, ni, nj) * s->dx + 1.0e-12)));
+}
+/* simulator.py:4798-4866: Tao reflection into the cut link's outgoing slot */
+static void prepare_cut_links(o2d* s, int i, int j, const double raw[Q9], double prepared[Q9]) {
+  long N = N_(s), n = at(s, i, j);
+  for (int q = 0; q < Q9; ++q) prepared[q] = raw[q];
+  double rho = mixture(s, s->phi[n]);
+  double pdyn = (double)s->p[n] - s->p_ref[j];
+  double ux = s->u[n] + 0.5 * s->a[n], uy = s->u[N + n] + 0.5 * s->a[N + n];
+  double center[2];
+  body_center(s, center);
+  double jx = 0.0, jy = 0.0, torque = 0.0;
+  int cut = 0;
+  for (int q = 1; q < Q9; ++q) {
+    int ni = i + CX[q], nj = j + CY[q];
+    if (!(inside(s, ni, nj) && s->solid[at(s, ni, nj)] == 1)) continue;
+    double eta = cut_fraction(s, i, j, ni, nj);
+    double bx = i + 0.5 + eta * CX[q], by = j + 0.5 + eta * CY[q], ub[2];
+    body_velocity_at(s, bx, by, ub);
+    int o = OPP[q];
+    double noneq = s->f[q * N + n] - momentum_eq(q, pdyn, rho, ux, uy);
+    double wall = momentum_eq(o, pdyn, rho, ub[0], ub[1]);
+    double reflected = (float)((wall + noneq + eta * raw[o]) / (1.0 + eta));
+    prepared[q] = reflected;
+    double ix = (CX[q] - ub[0]) * raw[q] - (-CX[q] - ub[0]) * reflected;
+    double iy = (CY[q] - ub[1]) * raw[q] - (-CY[q] - ub[1]) * reflected;
+    double pref = (1.0 - eta) * s->p_ref[j] + eta * s->p_ref[nj];
+    ix += 6.0 * W9(q) * pref * CX[q];
+    iy += 6.0 * W9(q) * pref * CY[q];
+    ix = (float)ix;
+    iy = (float)iy;
+    double rx = (float)(bx - center[0]), ry = (float)(by - center[1]);
+    jx += ix;
+    jy += iy;
+    torque += rx * iy - ry * ix;
+    cut = 1;
+  }
+  if (cut) {
+    s->impulse[0] += jx;
+    s->impulse[1] += jy;
+    s->torque += torque;
+  }
+}
+/* simulator.py:3370-3552 */
+void o2d_collide_momentum(o2d* s) {
+  long N = N_(s);
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      if (!fluid_at(s, i, j)) continue;
+      long n = at(s, i, j);
+      double fd[5];
+      fluid_force(s, i, j, fd);
+      s->a[n] = (float)fd[0];
+      s->a[N + n] = (float)fd[1];
+      double phi = s->phi[n], ux = s->u[n] + 0.5 * fd[0], uy = s->u[N + n] + 0.5 * fd[1];
+      double fraction = clamp01(phi);
+      double nu = fraction * s->nu_w + (1.0 - fraction) * s->nu_a + fd[2];
+      double tau = 0.5 + 3.0 * nu;
+      double near_solid = 0.0;
+      for (int q = 1; q < Q9; ++q) {
+        int ni = i + CX[q], nj = j + CY[q];
+        if (inside(s, ni, nj) && s->solid[at(s, ni, nj)] == 1) near_solid = 1.0;
+      }
+      double indicator = fmax(1.0 - fraction, near_solid);
+      double tau_floor = 0.5 + (s->interface_tau - 0.5) * indicator;
+      double shear_tau = fmax(tau, tau_floor);
+      double rho = mixture(s, phi), pdyn = (double)s->p[n] - s->p_ref[j];
+      double Fx = rho * fd[0], Fy = rho * fd[1], drho = s->rho_w - s->rho_a;
+      double raw[6] = {0}, eq[9] = {0}, src[9] = {0};
+      for (int q = 0; q < Q9; ++q) {
+        double cx = CX[q], cy = CY[q], v = s->f[q * N + n];
+        double e = momentum_eq(q, pdyn, rho, ux, uy);
+        double b = W9(q) * 3.0 * (cx * Fx + cy * Fy + drho * (cx * ux + cy * uy) * (cx * fd[3] + cy * fd[4]));
+        double k[9] = {1, cx, cy, cx * cx, cy * cy, cx * cy, cx * cx * cy, cx * cy * cy, cx * cx * cy * cy};
+        for (int m = 0; m < 6; ++m) raw[m] += k[m] * v;
+        for (int m = 0; m < 9; ++m) {
+          eq[m] += k[m] * e;
+          src[m] += k[m] * b;
+        }
+      }
+      double w_phys = 1.0 / tau, w_shear = 1.0 / shear_tau;
+      double sp = 1.0 - 0.5 * w_phys, ss = 1.0 - 0.5 * w_shear;
+      double post[9];
+      for (int m = 0; m < 3; ++m) post[m] = raw[m] - w_phys * (raw[m] - eq[m]) + sp * src[m];
+      double trace = eq[3] + eq[4] + 0.5 * (src[3] + src[4]);
+      double diff = raw[3] - raw[4];
+      double post_diff = diff - w_shear * (diff - (eq[3] - eq[4])) + ss * (src[3] - src[4]);
+      post[3] = 0.5 * (trace + post_diff);
+      post[4] = 0.5 * (trace - post_diff);
+      post[5] = raw[5] - w_shear * (raw[5] - eq[5]) + ss * src[5];
+      for (int m = 6; m < 9; ++m) post[m] = eq[m] + 0.5 * src[m];
+      double rawpost[Q9], prepared[Q9];
+      for (int q = 0; q < Q9; ++q) rawpost[q] = (float)reconstruct_cm(CX[q], CY[q], 0.0, 0.0, post);
+      prepare_cut_links(s, i, j, rawpost, prepared);
+      for (int q = 0; q < Q9; ++q) s->fbuf[q * N + n] = (float)prepared[q];
+    }
+}
+/* simulator.py:3554-3697 */
+void o2d_collide_phase(o2d* s) {
+  long N = N_(s);
+  double rate = 1.0 / (3.0 * s->mobility + 0.5);
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      if (!fluid_at(s, i, j)) continue;
+      long n = at(s, i, j);
+      double phi = s->phi[n];
+      double ux = s->u[n] + 0.5 * s->a[n], uy = s->u[N + n] + 0.5 * s->a[N + n];
+      double g[2], source[2], lap;
+      blended_gradient(s, i, j, g, source, &lap);
+      double norm = sqrt(g[0] * g[0] + g[1] * g[1]) + 1.0e-14;
+      double nx_ = g[0] / norm, ny_ = g[1] / norm;
+      double compression = 4.0 * phi * (1.0 - phi) / s->width;
+      double r10 = 0, r01 = 0, eq[9] = {0}, src[9] = {0};
+      for (int q = 0; q < Q9; ++q) {
+        double rx = CX[q] - ux, ry = CY[q] - uy, hv = s->h[q * N + n];
+        double he = phase_eq(q, phi, ux, uy);
+        double force = W9(q) * compression * (CX[q] * nx_ + CY[q] * ny_);
+        double k[9] = {1, rx, ry, rx * rx, ry * ry, rx * ry, rx * rx * ry, rx * ry * ry, rx * rx * ry * ry};
+        r10 += hv * rx;
+        r01 += hv * ry;
+        for (int m = 0; m < 9; ++m) {
+          eq[m] += he * k[m];
+          src[m] += force * k[m];
+        }
+      }
+      double post[9];
+      for (int m = 0; m < 9; ++m) post[m] = eq[m] + 0.5 * src[m];
+      post[1] = r10 - rate * (r10 - eq[1]) + (1.0 - 0.5 * rate) * src[1];
+      post[2] = r01 - rate * (r01 - eq[2]) + (1.0 - 0.5 * rate) * src[2];
+      for (int q = 0; q < Q9; ++q) s->hbuf[q * N + n] = (float)reconstruct_cm(CX[q], CY[q], ux, uy, post);
+    }
+}
+/* simulator.py:3699-3738 (phase_only = 0) and :3740-3785 (phase_only = 1) */
+void o2d_stream(o2d* s, int phase_only) {
+  long N = N_(s);
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      if (!fluid_at(s, i, j)) continue;
+      long n = at(s, i, j);
+      for (int q = 0; q < Q9; ++q) {
+        int ni = i + CX[q], nj = j + CY[q];
+        double fo = s->fbuf[q * N + n], ho = s->hbuf[q * N + n];
+        if (inside(s, ni, nj) && fluid_at(s, ni, nj)) {
+          long m = at(s, ni, nj);
+          if (!phase_only) s->f[q * N + m] = (float)fo;
+          s->h[q * N + m] = (float)ho;
+        } else {
+          int o = OPP[q];
+          if (!phase_only) s->f[o * N + n] = (float)fo;
+          if (inside(s, ni, nj) && s->solid[at(s, ni, nj)] == 1) {
+            double eta = cut_fraction(s, i, j, ni, nj), ub[2];
+            body_velocity_at(s, i + 0.5 + eta * CX[q], j + 0.5 + eta * CY[q], ub);
+            ho -= 6.0 * W9(q) * s->phi[n] * (CX[q] * ub[0] + CY[q] * ub[1]);
+          }
+          s->h[o * N + n] = (float)ho;
+        }
+      }
+    }
+}
+/* simulator.py:3787-3813 */
+void o2d_update_macroscopic(o2d* s) {
+  long N = N_(s);
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      if (fluid_at(s, i, j)) {
+        double phi = 0.0, mx = 0.0, my = 0.0;
+        for (int q = 0; q < Q9; ++q) {
+          phi += s->h[q * N + n];
+          mx += CX[q] * (double)s->f[q * N + n];
+          my += CY[q] * (double)s->f[q * N + n];
+        }
+        s->phi[n] = (float)phi;
+        double rho = fmax(mixture(s, (float)phi), 1.0e-12);
+        s->u[n] = (float)(mx / rho);
+        s->u[N + n] = (float)(my / rho);
+      } else {
+        double v[2] = {0.0, 0.0};
+        if (s->solid[n] == 1) body_velocity_at(s, i + 0.5, j + 0.5, v);
+        if (wall_at(s, i, j)) s->phi[n] = 0.0f;
+        s->u[n] = (float)v[0];
+        s->u[N + n] = (float)v[1];
+        s->a[n] = s->a[N + n] = 0.0f;
+      }
+    }
+}
+/* simulator.py:3815-3825 */
+void o2d_update_streamed_phase(o2d* s) {
+  long N = N_(s);
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      if (fluid_at(s, i, j)) {
+        double phi = 0.0;
+        for (int q = 0; q < Q9; ++q) phi += s->h[q * N + n];
+        s->phi[n] = (float)phi;
+      } else if (wall_at(s, i, j)) {
+        s->phi[n] = 0.0f;
+      }
+    }
+}
+/* simulator.py:3833-3858 */
+void o2d_recover_initial_moments(o2d* s) {
+  long N = N_(s);
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      if (fluid_at(s, i, j)) {
+        double phi = 0.0, mx = 0.0, my = 0.0;
+        for (int q = 0; q < Q9; ++q) {
+          phi += s->h[q * N + n];
+          mx += CX[q] * (double)s->f[q * N + n];
+          my += CY[q] * (double)s->f[q * N + n];
+        }
+        s->phi[n] = (float)phi;
+        double rho = fmax(mixture(s, (float)phi), 1.0e-12);
+        s->u[n] = (float)(mx / rho);
+        s->u[N + n] = (float)(my / rho);
+      } else if (s->solid[n] == 1) {
+        double v[2];
+        body_velocity_at(s, i + 0.5, j + 0.5, v);
+        s->u[n] = (float)v[0];
+        s->u[N + n] = (float)v[1];
+      } else {
+        s->phi[n] = 0.0f;
+        s->u[n] = s->u[N + n] = 0.0f;
+      }
+    }
+}
+/* simulator.py:3860-3897 */
+void o2d_update_pressure(o2d* s) {
+  long N = N_(s);
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      if (fluid_at(s, i, j)) {
+        double moving = 0.0;
+        for (int q = 1; q < Q9; ++q) moving += s->f[q * N + n];
+        double phi = s->phi[n], rho = mixture(s, phi);
+        double ux = s->u[n] + 0.5 * s->a[n], uy = s->u[N + n] + 0.5 * s->a[N + n];
+        double gx = 0.0, gy = 0.0;
+        for (int q = 0; q < Q9; ++q) {
+          double nb = phase_neighbor(s, i, j, CX[q], CY[q], 1);
+          gx += 3.0 * W9(q) * CX[q] * (nb - phi);
+          gy += 3.0 * W9(q) * CY[q] * (nb - phi);
+        }
+        double drho = s->rho_w - s->rho_a;
+        double s0 = -1.5 * W9(0) * (ux * ux + uy * uy);
+        double dynamic = 0.6 * (moving + 0.5 * (ux * drho * gx + uy * drho * gy) + rho * s0);
+        s->p[n] = (float)(s->p_ref[j] + dynamic);
+      } else if (wall_at(s, i, j)) {
+        s->p[n] = 0.0f;
+      }
+    }
+}
+/* simulator.py:3163-3236 */
+void o2d_refill(o2d* s) {
+  long N = N_(s);
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      if (wall_at(s, i, j)) continue;
+      int became_solid = s->prev_solid[n] == 0 && s->solid[n] == 1;
+      int became_fluid = s->prev_solid[n] == 1 && s->solid[n] == 0;
+      if (became_solid) {
+        double v[2];
+        body_velocity_at(s, i + 0.5, j + 0.5, v);
+        s->phi[n] = (float)clamp01(s->phi[n]);
+        s->u[n] = (float)v[0];
+        s->u[N + n] = (float)v[1];
+        s->a[n] = s->a[N + n] = 0.0f;
+        s->p[n] = (float)((double)s->p[n] - s->p_ref[j]);
+      } else if (became_fluid) {
+        double phi_sum = 0, p_sum = 0, vx = 0, vy = 0, count = 0;
+        for (int di = -1; di <= 1; ++di)
+          for (int dj = -1; dj <= 1; ++dj) {
+            int ni = i + di, nj = j + dj;
+            if ((di || dj) && inside(s, ni, nj) && fluid_at(s, ni, nj) && s->prev_solid[at(s, ni, nj)] == 0) {
+              long m = at(s, ni, nj);
+              phi_sum += clamp01(s->phi[m]);
+              p_sum += (double)s->p[m] - s->p_ref[nj];
+              vx += s->u[m];
+              vy += s->u[N + m];
+              count += 1.0;
+            }
+          }
+        double phi0 = clamp01(s->phi[n]), p0 = s->p[n], v[2];
+        body_velocity_at(s, i + 0.5, j + 0.5, v);
+        if (count > 0.0) {
+          phi0 = phi_sum / count;
+          p0 = p_sum / count;
+          v[0] = vx / count;
+          v[1] = vy / count;
+        }
+        phi0 = (float)phi0;
+        v[0] = (float)v[0];
+        v[1] = (float)v[1];
+        p0 = (float)(p0 + s->p_ref[j]);
+        double rho = mixture(s, phi0);
+        s->phi[n] = (float)phi0;
+        s->u[n] = (float)v[0];
+        s->u[N + n] = (float)v[1];
+        s->a[n] = s->a[N + n] = 0.0f;
+        s->p[n] = (float)p0;
+        double pdyn = p0 - s->p_ref[j];
+        for (int q = 0; q < Q9; ++q) {
+          s->f[q * N + n] = (float)momentum_eq(q, pdyn, rho, v[0], v[1]);
+          s->h[q * N + n] = (float)phase_eq(q, phi0, v[0], v[1]);
+        }
+      }
+    }
+}
+/* simulator.py:2303-2338 */
+void o2d_measure_outflow(o2d* s) {
+  long N = N_(s);
+  double best = 0.0;
+  const int OFF[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      if (!(fluid_at(s, i, j) && s->phi[n] > s->cutoff)) continue;
+      double vx = (float)(s->u[n] + 0.5 * s->a[n]), vy = (float)(s->u[N + n] + 0.5 * s->a[N + n]);
+      double outflow = 0.0;
+      for (int k = 0; k < 4; ++k) {
+        int ni = i + OFF[k][0], nj = j + OFF[k][1];
+        if (!inside(s, ni, nj)) continue;
+        long m = at(s, ni, nj);
+        if (fluid_at(s, ni, nj) && s->phi[m] > s->cutoff) {
+          double wx = (float)(s->u[m] + 0.5 * s->a[m]), wy = (float)(s->u[N + m] + 0.5 * s->a[N + m]);
+          outflow += fmax(0.0, OFF[k][0] * 0.5 * (vx + wx) + OFF[k][1] * 0.5 * (vy + wy));
+        }
+      }
+      if (!isfinite(vx) || !isfinite(vy)) outflow = INFINITY;
+      best = fmax(best, outflow);
+    }
+  s->max_outflow = (float)best;
+}
+
+/* -------------------------------------------------- volume projection */
+/* simulator.py:4150-4229; returns the new water volume */
+double o2d_clip_phase(o2d* s, double max_bulk_l1) {
+  long N = N_(s);
+  double total = 0.0;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      if (!fluid_at(s, i, j)) continue;
+      long n = at(s, i, j);
+      double old_phase = s->phi[n], bounded = clamp01(old_phase);
+      if (bounded <= s->cutoff) bounded = 0.0;
+      else if (bounded >= 1.0 - s->cutoff) bounded = 1.0;
+      double ux = s->u[n], uy = s->u[N + n];
+      if (bounded != old_phase) {
+        double pdyn = (double)s->p[n] - s->p_ref[j];
+        double ro = mixture(s, old_phase), rn = mixture(s, bounded);
+        for (int q = 0; q < Q9; ++q)
+          s->f[q * N + n] = (float)(s->f[q * N + n] + (momentum_eq(q, pdyn, rn, ux, uy) - momentum_eq(q, pdyn, ro, ux, uy)));
+      }
+      double l1 = 0.0;
+      for (int q = 0; q < Q9; ++q) l1 += fabs((double)s->h[q * N + n]);
+      int regularize = (bounded == 0.0 || bounded == 1.0) && l1 > max_bulk_l1;
+      if (regularize) {
+        double px = s->u[n] + 0.5 * s->a[n], py = s->u[N + n] + 0.5 * s->a[N + n], sum = 0.0;
+        for (int q = 0; q < Q9; ++q) {
+          float r = (float)phase_eq(q, bounded, px, py);
+          s->h[q * N + n] = r;
+          sum += r;
+        }
+        s->h[n] = (float)(s->h[n] + (bounded - sum));
+      } else if (bounded != old_phase) {
+        double sum = 0.0;
+        for (int q = 0; q < Q9; ++q) {
+          float lifted = (float)(s->h[q * N + n] + phase_eq(q, bounded, ux, uy) - phase_eq(q, old_phase, ux, uy));
+          s->h[q * N + n] = lifted;
+          sum += lifted;
+        }
+        s->h[n] = (float)(s->h[n] + (bounded - sum));
+      }
+      s->phi[n] = (float)bounded;
+      total += (double)(float)bounded;
+    }
+  s->water_volume_current = total;
+  return total;
+}
+/* simulator.py:4231-4257; mask planes 0/1 live in hbuf */
+void o2d_seed_projection(o2d* s) {
+  long N = N_(s);
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      int seed = 0;
+      if (fluid_at(s, i, j)) {
+        double phase = s->phi[n];
+        if (s->cutoff < phase && phase < 1.0 - s->cutoff) {
+          if (phase == 0.5) seed = 1;
+          for (int di = -1; di <= 1; ++di)
+            for (int dj = -1; dj <= 1; ++dj) {
+              int ni = i + di, nj = j + dj;
+              if ((di || dj) && inside(s, ni, nj) && fluid_at(s, ni, nj)) {
+                double nb = s->phi[at(s, ni, nj)];
+                if ((phase < 0.5 && 0.5 <= nb) || (nb < 0.5 && 0.5 <= phase)) seed = 1;
+              }
+            }
+        }
+      }
+      s->hbuf[n] = (float)seed;
+      s->hbuf[N + n] = 0.0f;
+    }
+}
+/* simulator.py:4259-4289 */
+void o2d_dilate_projection(o2d* s, int primary_to_next) {
+  long N = N_(s);
+  const float* src = primary_to_next ? s->hbuf : s->hbuf + N;
+  float* dst = primary_to_next ? s->hbuf + N : s->hbuf;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      int value = 0;
+      double phase = s->phi[n];
+      if (fluid_at(s, i, j) && s->cutoff < phase && phase < 1.0 - s->cutoff) {
+        value = (int)src[n];
+        if (value == 0)
+          for (int di = -1; di <= 1; ++di)
+            for (int dj = -1; dj <= 1; ++dj) {
+              int ni = i + di, nj = j + dj;
+              if ((di || dj) && inside(s, ni, nj) && src[at(s, ni, nj)] == 1.0f) value = 1;
+            }
+      }
+      dst[n] = (float)value;
+    }
+}
+/* simulator.py:4291-4351: out = {volume, derivative} */
+void o2d_evaluate_projection(o2d* s, double lambda, double out[2]) {
+  double e = exp(lambda), volume = 0.0, derivative = 0.0;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      if (!fluid_at(s, i, j)) continue;
+      double phase = s->phi[n], mapped = phase;
+      int adjustable = s->hbuf[n] == 1.0f;
+      if (adjustable) mapped = phase * e / (1.0 - phase + phase * e);
+      float m32 = (float)mapped;
+      if (m32 <= s->cutoff) m32 = 0.0f;
+      else if (m32 >= 1.0 - s->cutoff) m32 = 1.0f;
+      volume += (double)m32;
+      if (adjustable && s->cutoff < m32 && m32 < 1.0 - s->cutoff) derivative += (double)m32 * (1.0 - (double)m32);
+    }
+  out[0] = volume;
+  out[1] = derivative;
+}
+static void lift_phase(o2d* s, int i, int j, double old_phase, double new_phase) {
+  long N = N_(s), n = at(s, i, j);
+  double ux = s->u[n], uy = s->u[N + n], pdyn = (double)s->p[n] - s->p_ref[j];
+  double ro = mixture(s, old_phase), rn = mixture(s, new_phase);
+  for (int q = 0; q < Q9; ++q)
+    s->f[q * N + n] = (float)(s->f[q * N + n] + (momentum_eq(q, pdyn, rn, ux, uy) - momentum_eq(q, pdyn, ro, ux, uy)));
+  double sum = 0.0;
+  for (int q = 0; q < Q9; ++q) {
+    float lifted = (float)(s->h[q * N + n] + phase_eq(q, new_phase, ux, uy) - phase_eq(q, old_phase, ux, uy));
+    s->h[q * N + n] = lifted;
+    sum += lifted;
+  }
+  s->h[n] = (float)(s->h[n] + (new_phase - sum));
+  s->phi[n] = (float)new_phase;
+}
+/* simulator.py:4353-4406 */
+double o2d_apply_projection(o2d* s, double lambda) {
+  double e = exp(lambda), total = 0.0;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      if (!fluid_at(s, i, j)) continue;
+      float old_phase = s->phi[n], new_phase = old_phase;
+      if (s->hbuf[n] == 1.0f) {
+        double p = old_phase;
+        new_phase = (float)(p * e / (1.0 - p + p * e));
+      }
+      if (new_phase <= s->cutoff) new_phase = 0.0f;
+      else if (new_phase >= 1.0 - s->cutoff) new_phase = 1.0f;
+      if (new_phase != old_phase) lift_phase(s, i, j, old_phase, new_phase);
+      total += (double)new_phase;
+    }
+  s->water_volume_current = total;
+  return total;
+}
+static int residual_free(const o2d* s, long n, double phase, double residual) {
+  const double margin = 4.0 * 1.1920928955078125e-07;
+  if (!(s->hbuf[n] == 1.0f && s->cutoff < phase && phase < 1.0 - s->cutoff)) return 0;
+  double magnitude = fabs(residual);
+  if (residual > 0.0) return 1.0 - s->cutoff - phase > magnitude + margin;
+  return phase - s->cutoff > magnitude + margin;
+}
+/* simulator.py:4408-4431 */
+double o2d_projection_residual_weight(o2d* s, double residual) {
+  double weight = 0.0;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      if (!fluid_at(s, i, j)) continue;
+      double phase = s->phi[n];
+      if (residual_free(s, n, phase, residual)) weight += phase * (1.0 - phase);
+    }
+  s->projection_derivative = weight;
+  return weight;
+}
+/* simulator.py:4433-4492 */
+double o2d_apply_projection_residual(o2d* s, double residual, double total_weight) {
+  double total = 0.0;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      if (!fluid_at(s, i, j)) continue;
+      float old_phase = s->phi[n], new_phase = old_phase;
+      double p = old_phase;
+      if (residual_free(s, n, p, residual) && total_weight > 0.0)
+        new_phase = (float)(p + residual * (p * (1.0 - p)) / total_weight);
+      if (new_phase != old_phase) lift_phase(s, i, j, old_phase, new_phase);
+      total += (double)new_phase;
+    }
+  s->water_volume_current = total;
+  return total;
+}
+
+/* ------------------------------------------------------ mass properties */
+/* simulator.py:3014-3044 */
+void o2d_reduce_mass_properties(o2d* s) {
+  double m = 0, fx = 0, fy = 0, I = 0;
+  for (int bi = 0; bi < s->body_nx; ++bi)
+    for (int bj = 0; bj < s->body_ny; ++bj) {
+      double mass = s->body_mass[(long)bi * s->body_ny + bj] * (1.0 / (s->rho_w_si * s->area));
+      double fraction = body_fraction(s, bi, bj);
+      double lx = bi + 0.5 - 0.5 * s->body_nx, ly = bj + 0.5 - 0.5 * s->body_ny;
+      m += mass;
+      fx += mass * lx;
+      fy += mass * ly;
+      I += mass * (lx * lx + ly * ly + fraction / 6.0);
+    }
+  s->reduced_mass = m;
+  s->reduced_first[0] = fx;
+  s->reduced_first[1] = fy;
+  s->reduced_inertia = I;
+}
+/* simulator.py:3046-3161 */
+void o2d_apply_mass_properties(o2d* s, int initialize) {
+  double reduced = s->reduced_mass, old_mass = s->mass, initial = s->initial_mass;
+  double tol = 1.0e-12 * fmax(initial, 1.0), new_mass = reduced;
+  int hold = 0;
+  if (!initialize && reduced > 0.0) {
+    if (fabs(reduced - old_mass) <= tol || reduced > old_mass) {
+      new_mass = old_mass;
+      hold = 1;
+    }
+  }
+  double inactive_mass = fmax(1.0e-12, 1.0e-14 * fmax(initial, 1.0));
+  double inactive_inertia = 1.0e-14 * fmax(s->initial_inertia, 1.0);
+  int resolved = new_mass > inactive_mass, was_active = body_active(s);
+  double old_com[2] = {s->local_com[0], s->local_com[1]}, new_com[2] = {old_com[0], old_com[1]};
+  double raw_inertia = 0.0;
+  if (resolved) {
+    if (hold) {
+      raw_inertia = s->inertia;
+    } else {
+      new_com[0] = s->reduced_first[0] / new_mass;
+      new_com[1] = s->reduced_first[1] / new_mass;
+      raw_inertia = s->reduced_inertia - new_mass * (new_com[0] * new_com[0] + new_com[1] * new_com[1]);
+    }
+    raw_inertia = fmax(raw_inertia, 0.0);
+  }
+  int mechanical = resolved && raw_inertia > inactive_inertia && (initialize || was_active);
+  double new_inertia = mechanical ? raw_inertia : 0.0;
+  double angle = s->angle, ow[2], nw[2];
+  rotate(angle, old_com[0], old_com[1], &ow[0], &ow[1]);
+  rotate(angle, new_com[0], new_com[1], &nw[0], &nw[1]);
+  double origin[2] = {s->origin[0], s->origin[1]};
+  int retiring = was_active && !mechanical;
+  if (!initialize && (new_mass < old_mass || retiring)) {
+    double v0[2] = {s->velocity[0], s->velocity[1]}, omega = s->omega;
+    double shift[2] = {nw[0] - ow[0], nw[1] - ow[1]}, sv[2] = {0.0, 0.0};
+    if (mechanical) {
+      sv[0] = v0[0] - omega * shift[1];
+      sv[1] = v0[1] + omega * shift[0];
+    }
+    float sv32[2] = {(float)sv[0], (float)sv[1]};
+    double svs[2] = {sv32[0], sv32[1]};
+    double nc[2] = {(float)(origin[0] + nw[0]), (float)(origin[1] + nw[1])};
+    double oc[2];
+    body_center(s, oc);
+    double lost[2] = {old_mass * v0[0] - new_mass * svs[0], old_mass * v0[1] - new_mass * svs[1]};
+    s->cumulative_melted_momentum[0] += lost[0];
+    s->cumulative_melted_momentum[1] += lost[1];
+    double ob[2] = {old_mass * v0[0], old_mass * v0[1]}, nb[2] = {new_mass * svs[0], new_mass * svs[1]};
+    double old_l = oc[0] * ob[1] - oc[1] * ob[0] + s->inertia * omega;
+    double new_l = nc[0] * nb[1] - nc[1] * nb[0] + new_inertia * omega;
+    s->cumulative_melted_angular += old_l - new_l;
+    s->velocity[0] = sv32[0];
+    s->velocity[1] = sv32[1];
+  }
+  s->local_com[0] = new_com[0];
+  s->local_com[1] = new_com[1];
+  s->mass = new_mass;
+  s->inertia = new_inertia;
+  if (!mechanical) {
+    s->velocity[0] = s->velocity[1] = 0.0f;
+    s->omega = 0.0f;
+    for (int k = 0; k < 4; ++k) s->support[k] = 0.0f;
+  }
+}
+
+/* ------------------------------------------------------ melt momentum */
+static void fluid_momentum(const o2d* s, double m[2], double* angular) {
+  long N = N_(s);
+  m[0] = m[1] = *angular = 0.0;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      if (!fluid_at(s, i, j)) continue;
+      long n = at(s, i, j);
+      double mx = 0.0, my = 0.0;
+      for (int q = 0; q < Q9; ++q) {
+        mx += CX[q] * (double)s->f[q * N + n];
+        my += CY[q] * (double)s->f[q * N + n];
+      }
+      m[0] += mx;
+      m[1] += my;
+      *angular += (i + 0.5) * my - (j + 0.5) * mx;
+    }
+}
+/* simulator.py:2735-2760 */
+void o2d_snapshot_melt_momentum(o2d* s) {
+  s->body_before[0] = s->cumulative_melted_momentum[0];
+  s->body_before[1] = s->cumulative_melted_momentum[1];
+  s->body_angular_before = s->cumulative_melted_angular;
+  fluid_momentum(s, s->fluid_before, &s->angular_before);
+}
+/* simulator.py:2762-2797 */
+void o2d_reduce_provisional_momentum(o2d* s) {
+  fluid_momentum(s, s->fluid_provisional, &s->angular_provisional);
+  s->wet_weight = 0.0;
+  s->wet_target = s->nx * s->ny;
+  s->wet_target_max = -1;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      if (!fluid_at(s, i, j)) continue;
+      double phase = clamp01(s->phi[at(s, i, j)]);
+      if (phase > s->cutoff) {
+        s->wet_weight += (double)(float)phase;
+        int code = i * s->ny + j;
+        if (code < s->wet_target) s->wet_target = code;
+        if (code > s->wet_target_max) s->wet_target_max = code;
+      }
+    }
+  s->fluid_after[0] = s->fluid_provisional[0];
+  s->fluid_after[1] = s->fluid_provisional[1];
+  s->angular_after = s->angular_provisional;
+}
+static void melt_correction(const o2d* s, double c[2], double* angular) {
+  c[0] = s->cumulative_melted_momentum[0] - s->body_before[0] - (s->fluid_after[0] - s->fluid_before[0]);
+  c[1] = s->cumulative_melted_momentum[1] - s->body_before[1] - (s->fluid_after[1] - s->fluid_before[1]);
+  *angular = s->cumulative_melted_angular - s->body_angular_before - (s->angular_after - s->angular_before);
+}
+static void wet_centroid(const o2d* s, double c[2]) {
+  c[0] = c[1] = 0.0;
+  if (s->wet_weight > 1.0e-30) {
+    c[0] = s->weight_first[0] / s->wet_weight;
+    c[1] = s->weight_first[1] / s->wet_weight;
+  }
+}
+static double wet_polar(const o2d* s) {
+  double c[2];
+  wet_centroid(s, c);
+  return fmax(0.0, s->weight_second - s->wet_weight * (c[0] * c[0] + c[1] * c[1]));
+}
+/* simulator.py:2799-2840 */
+int o2d_melt_weight_geometry(o2d* s) {
+  s->weight_first[0] = s->weight_first[1] = s->weight_second = 0.0;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      double phase = clamp01(s->phi[at(s, i, j)]);
+      if (fluid_at(s, i, j) && phase > s->cutoff) {
+        double w = (float)phase, px = i + 0.5, py = j + 0.5;
+        s->weight_first[0] += w * px;
+        s->weight_first[1] += w * py;
+        s->weight_second += w * (px * px + py * py);
+      }
+    }
+  double c[2], angular;
+  melt_correction(s, c, &angular);
+  double norm = sqrt(c[0] * c[0] + c[1] * c[1]);
+  if (s->wet_weight <= 1.0e-30) return (norm > 1.0e-12 || fabs(angular) > 1.0e-12) ? -1 : 0;
+  double cen[2];
+  wet_centroid(s, cen);
+  double couple = angular - (cen[0] * c[1] - cen[1] * c[0]);
+  if (wet_polar(s) <= 1.0e-30 && fabs(couple) > 1.0e-12) return -2;
+  if (norm > 1.0e-14 || fabs(angular) > 1.0e-14) return 1;
+  return 0;
+}
+static void add_momentum(o2d* s, int i, int j, double dmx, double dmy) {
+  long N = N_(s), n = at(s, i, j);
+  float mx = (float)dmx, my = (float)dmy;
+  double rho = fmax(mixture(s, clamp01(s->phi[n])), 1.0e-12);
+  s->u[n] = (float)(s->u[n] + mx / rho);
+  s->u[N + n] = (float)(s->u[N + n] + my / rho);
+  for (int q = 0; q < Q9; ++q) s->f[q * N + n] = (float)(s->f[q * N + n] + 3.0 * W9(q) * (CX[q] * (double)mx + CY[q] * (double)my));
+}
+/* simulator.py:2842-2884 */
+void o2d_apply_melt_correction(o2d* s) {
+  double c[2], angular, cen[2];
+  melt_correction(s, c, &angular);
+  wet_centroid(s, cen);
+  double polar = wet_polar(s), scale = 0.0, denominator = s->wet_weight;
+  if (polar > 1.0e-30) scale = (angular - (cen[0] * c[1] - cen[1] * c[0])) / polar;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      double phase = clamp01(s->phi[at(s, i, j)]);
+      if (!(fluid_at(s, i, j) && phase > s->cutoff) || !(denominator > 1.0e-30)) continue;
+      double w = (float)phase, rx = i + 0.5 - cen[0], ry = j + 0.5 - cen[1];
+      add_momentum(s, i, j, w * (c[0] / denominator - scale * ry), w * (c[1] / denominator + scale * rx));
+    }
+}
+/* simulator.py:2886-2903 */
+void o2d_reduce_final_momentum(o2d* s) { fluid_momentum(s, s->fluid_after, &s->angular_after); }
+/* simulator.py:2905-2958 */
+void o2d_local_melt_correction(o2d* s) {
+  int e1 = s->wet_target, e2 = s->wet_target_max, total = s->nx * s->ny;
+  if (!(e1 >= 0 && e1 < total && e2 >= 0 && e2 < total && e1 != e2)) return;
+  int i1 = e1 / s->ny, j1 = e1 - i1 * s->ny, i2 = e2 / s->ny, j2 = e2 - i2 * s->ny;
+  double p1[2] = {i1 + 0.5, j1 + 0.5}, p2[2] = {i2 + 0.5, j2 + 0.5};
+  double c[2], angular;
+  melt_correction(s, c, &angular);
+  double d[2] = {p1[0] - p2[0], p1[1] - p2[1]}, mid[2] = {0.5 * (p1[0] + p2[0]), 0.5 * (p1[1] + p2[1])};
+  double couple = angular - (mid[0] * c[1] - mid[1] * c[0]);
+  double dn = fmax(d[0] * d[0] + d[1] * d[1], 1.0e-30);
+  double qx = couple * -d[1] / dn, qy = couple * d[0] / dn;
+  add_momentum(s, i1, j1, 0.5 * c[0] + qx, 0.5 * c[1] + qy);
+  add_momentum(s, i2, j2, 0.5 * c[0] - qx, 0.5 * c[1] - qy);
+}
+/* simulator.py:2960-2986 */
+void o2d_finish_melt_momentum(o2d* s) {
+  for (int e = 0; e < 2; ++e) {
+    s->carrier_momentum[e] += s->fluid_provisional[e] - s->fluid_before[e];
+    s->correction_momentum[e] += s->fluid_after[e] - s->fluid_provisional[e];
+  }
+  s->carrier_angular += s->angular_provisional - s->angular_before;
+  s->correction_angular += s->angular_after - s->angular_provisional;
+}
+
+/* ------------------------------------------------------------ thermal */
+static int water_cell(const o2d* s, int i, int j) {
+  return !wall_at(s, i, j) && s->solid[at(s, i, j)] == 0 && s->phi[at(s, i, j)] > s->cutoff;
+}
+/* simulator.py:579-588 */
+void o2d_initialize_body_state(o2d* s, double t_ice) {
+  long M = (long)s->body_nx * s->body_ny;
+  for (long k = 0; k < M; ++k) {
+    s->body_mass[k] = s->cell_mass;
+    s->body_energy[k] = s->cell_mass * (s->cp_i * (t_ice - s->t_melt));
+  }
+}
+/* simulator.py:589-617 */
+void o2d_initialize_water_state(o2d* s, double t_water) {
+  double rho_cp = s->rho_w_si * s->cp_w;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      double phase = clamp01(s->phi[n]), v = 0.0;
+      if (!wall_at(s, i, j) && s->solid[n] == 0 && phase > s->cutoff) v = s->area * (double)(float)phase;
+      s->water_v[n] = v;
+      s->water_e[n] = rho_cp * v * (t_water - s->t_melt);
+    }
+  s->boundary_power = s->boundary_heat_input = s->correction_abs = 0.0;
+}
+/* simulator.py:633-714 with :1792-1822 (returns {volume, full capacity}) */
+void o2d_measure_aperture(o2d* s, double out[2]) {
+  double v = 0, e = 0, lo = 1.0e30, hi = -1.0e30, base = 0, full = 0;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      double vol = s->water_v[n];
+      v += vol;
+      e += s->water_e[n];
+      if (vol > 1.0e-30) {
+        double sp = s->water_e[n] / vol;
+        lo = fmin(lo, sp);
+        hi = fmax(hi, sp);
+      }
+      double phase = clamp01(s->phi[n]);
+      if (!wall_at(s, i, j) && s->solid[n] == 0 && phase > s->cutoff) {
+        base += s->area * (double)(float)phase;
+        full += s->area;
+      }
+    }
+  s->volume_before = v;
+  s->energy_before = e;
+  s->specific_min = lo;
+  s->specific_max = hi;
+  s->base_capacity = base;
+  s->full_capacity = full;
+  out[0] = v;
+  out[1] = full;
+}
+static void aperture_scales(const o2d* s, double* base_scale, double* spare_scale) {
+  double total = fmax(0.0, s->volume_before), base = s->base_capacity, full = s->full_capacity;
+  *base_scale = *spare_scale = 0.0;
+  if (total <= base && base > 1.0e-30) {
+    *base_scale = total / base;
+  } else if (total > base) {
+    *base_scale = 1.0;
+    if (full > base + 1.0e-30) *spare_scale = fmin(1.0, (total - base) / (full - base));
+  }
+}
+/* simulator.py:1629-1658 */
+static double aperture_target(const o2d* s, int i, int j) {
+  double bs, ss, target = 0.0, phase = clamp01(s->phi[at(s, i, j)]);
+  aperture_scales(s, &bs, &ss);
+  if (!wall_at(s, i, j) && s->solid[at(s, i, j)] == 0 && phase > s->cutoff) {
+    double p = (float)phase;
+    target = s->area * (bs * p + ss * (1.0 - p));
+  }
+  return fmin(s->area, fmax(0.0, target));
+}
+/* simulator.py:716-827 */
+void o2d_build_aperture_targets(o2d* s) {
+  double low = s->specific_min, high = s->specific_max, total = fmax(0.0, s->volume_before);
+  double rec = 0, cool = 0, heat = 0;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      double target = aperture_target(s, i, j), vol = s->water_v[n], energy = 0.0;
+      if (target > 0.0) {
+        double sp = 0.0;
+        if (vol > 1.0e-30) {
+          sp = s->water_e[n] / vol;
+        } else {
+          double weight = 0.0;
+          for (int r = 1; r <= 3; ++r) {
+            if (weight > 1.0e-30) break;
+            for (int di = -r; di <= r; ++di)
+              for (int dj = -r; dj <= r; ++dj) {
+                int ni = i + di, nj = j + dj;
+                int ring = (abs(di) > abs(dj) ? abs(di) : abs(dj)) == r;
+                if (ring && inside(s, ni, nj) && !wall_at(s, ni, nj)) {
+                  long m = at(s, ni, nj);
+                  double nv = s->water_v[m];
+                  if (nv > 1.0e-30) {
+                    double d2 = (double)(di * di + dj * dj);
+                    sp += s->water_e[m] / d2;
+                    weight += nv / d2;
+                  }
+                }
+              }
+          }
+          if (weight > 1.0e-30) sp /= weight;
+          else if (total > 1.0e-30) sp = s->energy_before / total;
+        }
+        sp = fmin(high, fmax(low, sp));
+        energy = target * sp;
+        cool += target * (sp - low);
+        heat += target * (high - sp);
+      }
+      s->target_e[n] = energy;
+      rec += energy;
+    }
+  s->reconstructed_energy = rec;
+  s->cooling_capacity = cool;
+  s->heating_capacity = heat;
+}
+/* simulator.py:829-859 */
+void o2d_apply_aperture_transfer(o2d* s) {
+  double correction = s->energy_before - s->reconstructed_energy;
+  s->correction_abs += fabs(correction);
+  double low = s->specific_min, high = s->specific_max, cool = s->cooling_capacity, heat = s->heating_capacity;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      double target = aperture_target(s, i, j), energy = s->target_e[n];
+      if (target > 0.0) {
+        if (correction < 0.0 && cool > 1.0e-30) energy += fmax(-1.0, correction / cool) * fmax(0.0, energy - low * target);
+        else if (correction > 0.0 && heat > 1.0e-30) energy += fmin(1.0, correction / heat) * fmax(0.0, high * target - energy);
+      }
+      s->water_v[n] = target;
+      s->water_e[n] = energy;
+    }
+}
+/* simulator.py:861-877 */
+void o2d_finish_aperture_transfer(o2d* s) {
+  double v = 0, e = 0;
+  long N = N_(s);
+  for (long n = 0; n < N; ++n) {
+    v += s->water_v[n];
+    e += s->water_e[n];
+  }
+  s->volume_after = v;
+  s->energy_after = e;
+}
+/* simulator.py:879-907 */
+void o2d_body_conduction(o2d* s, double dt) {
+  s->unassigned_mass = s->unassigned_energy = 0.0;
+  const int OFF[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+  for (int bi = 0; bi < s->body_nx; ++bi)
+    for (int bj = 0; bj < s->body_ny; ++bj) {
+      long k = (long)bi * s->body_ny + bj;
+      s->melt_target[k] = -1;
+      double rate = 0.0, fr = body_fraction(s, bi, bj), t = body_temperature(s, bi, bj);
+      for (int o = 0; o < 4; ++o) {
+        int ni = bi + OFF[o][0], nj = bj + OFF[o][1];
+        if (ni >= 0 && ni < s->body_nx && nj >= 0 && nj < s->body_ny) {
+          double face = (float)fmin(fr, body_fraction(s, ni, nj));
+          rate += s->k_i * face * (t - body_temperature(s, ni, nj));
+        }
+      }
+      s->body_heat[k] = -dt * rate;
+    }
+}
+static double cell_aperture(const o2d* s, long n) { return fmin(1.0, fmax(0.0, s->water_v[n] / s->area)); }
+/* simulator.py:909-1010 and :1063-1164 (axis 0 = x faces, 1 = y faces) */
+void o2d_conduction_flux(o2d* s, int axis) {
+  int fx = axis == 0 ? s->nx + 1 : s->nx, fy = axis == 0 ? s->ny : s->ny + 1, limit = axis == 0 ? s->nx : s->ny;
+  int lo_kind = axis == 0 ? s->left_kind : s->bottom_kind, hi_kind = axis == 0 ? s->right_kind : s->top_kind;
+  double lo_value = axis == 0 ? s->left_value : s->bottom_value, hi_value = axis == 0 ? s->right_value : s->top_value;
+  double* flux = axis == 0 ? s->flux_ex : s->flux_ey;
+  for (int a = 0; a < fx; ++a)
+    for (int b = 0; b < fy; ++b) {
+      int face = axis == 0 ? a : b;
+      double out = 0.0, inward = 0.0;
+      if (face > 0 && face < limit) {
+        int li = axis == 0 ? a - 1 : a, lj = axis == 0 ? b : b - 1;
+        long l = at(s, li, lj), r = at(s, a, b);
+        int la = water_cell(s, li, lj), ra = water_cell(s, a, b);
+        if (la && ra) {
+          if (s->water_v[l] > 1.0e-30 && s->water_v[r] > 1.0e-30) {
+            double ap = fmin(cell_aperture(s, l), cell_aperture(s, r));
+            out = -s->k_w * ap * (water_temperature(s, r) - water_temperature(s, l));
+          }
+        } else if (wall_at(s, li, lj) && ra && s->water_v[r] > 1.0e-30) {
+          if (lo_kind == DIRICHLET) out = 2.0 * s->k_w * cell_aperture(s, r) * (lo_value - water_temperature(s, r));
+          inward = out;
+        } else if (la && s->water_v[l] > 1.0e-30 && wall_at(s, a, b)) {
+          if (hi_kind == DIRICHLET) out = 2.0 * s->k_w * cell_aperture(s, l) * (water_temperature(s, l) - hi_value);
+          inward = -out;
+        }
+      }
+      flux[(long)a * fy + b] = out;
+      if (inward != 0.0) s->boundary_power += inward;
+    }
+}
+/* simulator.py:1012-1061 and :1166-1215 */
+void o2d_advection_flux(o2d* s, int axis) {
+  long N = N_(s);
+  int fx = axis == 0 ? s->nx + 1 : s->nx, fy = axis == 0 ? s->ny : s->ny + 1, limit = axis == 0 ? s->nx : s->ny;
+  double* ef = axis == 0 ? s->flux_ex : s->flux_ey;
+  double* vf = axis == 0 ? s->flux_vx : s->flux_vy;
+  for (int a = 0; a < fx; ++a)
+    for (int b = 0; b < fy; ++b) {
+      int face = axis == 0 ? a : b;
+      double e = 0.0, v = 0.0;
+      if (s->advection && face > 0 && face < limit) {
+        int li = axis == 0 ? a - 1 : a, lj = axis == 0 ? b : b - 1;
+        if (water_cell(s, li, lj) && water_cell(s, a, b)) {
+          long l = at(s, li, lj), r = at(s, a, b);
+          double ul = (float)(s->u[axis * N + l] + 0.5 * s->a[axis * N + l]);
+          double ur = (float)(s->u[axis * N + r] + 0.5 * s->a[axis * N + r]);
+          double speed = 0.5 * (double)(float)(ul + ur) * s->velocity_scale;
+          long up = speed >= 0.0 ? l : r;
+          double uv = s->water_v[up], frac = fmin(1.0, fmax(0.0, uv / s->area));
+          v = speed * frac * s->dx;
+          e = v * (s->water_e[up] / fmax(uv, 1.0e-30));
+        }
+      }
+      ef[(long)a * fy + b] = e;
+      vf[(long)a * fy + b] = v;
+    }
+}
+/* simulator.py:1217-1234 (volume = 1) and :1236-1246 (volume = 0) */
+void o2d_update_water(o2d* s, double dt, int volume) {
+  int ny = s->ny;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      long n = at(s, i, j);
+      double de = s->flux_ex[(long)(i + 1) * ny + j] - s->flux_ex[(long)i * ny + j] +
+                  s->flux_ey[(long)i * (ny + 1) + j + 1] - s->flux_ey[(long)i * (ny + 1) + j];
+      double dv = 0.0;
+      if (volume)
+        dv = s->flux_vx[(long)(i + 1) * ny + j] - s->flux_vx[(long)i * ny + j] +
+             s->flux_vy[(long)i * (ny + 1) + j + 1] - s->flux_vy[(long)i * (ny + 1) + j];
+      if (!wall_at(s, i, j)) {
+        s->water_e[n] -= dt * de;
+        if (volume) s->water_v[n] -= dt * dv;
+      }
+    }
+}
+/* simulator.py:1615-1627 */
+static void body_grid(const o2d* s, int i, int j, double* gx, double* gy) {
+  double c = cos((double)s->angle), sn = sin((double)s->angle);
+  double rx = i + 0.5 - s->origin[0], ry = j + 0.5 - s->origin[1];
+  *gx = c * rx + sn * ry + 0.5 * s->body_nx - 0.5;
+  *gy = -sn * rx + c * ry + 0.5 * s->body_ny - 0.5;
+}
+/* simulator.py:1252-1372 */
+void o2d_exchange_heat(o2d* s, double dt) {
+  double k = 2.0 * s->k_w * s->k_i / (s->k_w + s->k_i);
+  const int OFF[5][2] = {{0, 0}, {1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+  double* delta = s->body_heat;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      if (!(water_cell(s, i, j) && s->water_v[n] > 1.0e-30)) continue;
+      double req[20] = {0};
+      int tgt[20];
+      double requested = 0.0, ap = fmin(1.0, s->water_v[n] / s->area), tw = water_temperature(s, n);
+      for (int f = 0; f < 5; ++f) {
+        int wi = i + OFF[f][0], wj = j + OFF[f][1];
+        if (!inside(s, wi, wj)) continue;
+        double coverage = s->coverage[at(s, wi, wj)];
+        if (!(coverage > 0.0)) continue;
+        double contact = fmin(ap, fmin(1.0, coverage / s->threshold));
+        double gx, gy;
+        body_grid(s, wi, wj, &gx, &gy);
+        gx = (float)gx;
+        gy = (float)gy;
+        int bi0 = (int)floor(gx), bj0 = (int)floor(gy);
+        double fx = (float)(gx - bi0), fy = (float)(gy - bj0);
+        for (int oi = 0; oi < 2; ++oi)
+          for (int oj = 0; oj < 2; ++oj) {
+            int bi = bi0 + oi, bj = bj0 + oj;
+            if (!(bi >= 0 && bi < s->body_nx && bj >= 0 && bj < s->body_ny)) continue;
+            if (!(s->body_mass[(long)bi * s->body_ny + bj] > 0.0)) continue;
+            double wx = oi ? fx : 1.0 - fx, wy = oj ? fy : 1.0 - fy;
+            double share = wx * wy * body_fraction(s, bi, bj) / coverage;
+            double heat = dt * k * contact * share * fmax(0.0, tw - body_temperature(s, bi, bj));
+            int slot = f * 4 + oi * 2 + oj;
+            req[slot] = heat;
+            tgt[slot] = bi * s->body_ny + bj;
+            requested += heat;
+          }
+      }
+      double scale = 0.0;
+      if (requested > 0.0) scale = fmin(1.0, fmax(0.0, s->water_e[n]) / requested);
+      double transferred = 0.0;
+      for (int slot = 0; slot < 20; ++slot) {
+        double heat = scale * req[slot];
+        if (heat > 0.0) {
+          delta[tgt[slot]] += heat;
+          if ((int)n > s->melt_target[tgt[slot]]) s->melt_target[tgt[slot]] = (int)n;
+          transferred += heat;
+        }
+      }
+      s->water_e[n] -= transferred;
+    }
+}
+/* simulator.py:1374-1406 */
+void o2d_phase_change(o2d* s) {
+  long M = (long)s->body_nx * s->body_ny;
+  for (long k = 0; k < M; ++k) {
+    double heat = s->body_heat[k], solid = s->body_mass[k], sensible = s->body_energy[k];
+    double melt = 0.0, melt_sensible = 0.0;
+    if (heat <= 0.0) {
+      if (solid > 0.0) sensible += heat;
+    } else {
+      double gain = fmin(heat, fmax(-sensible, 0.0));
+      sensible += gain;
+      double remaining = heat - gain;
+      melt = fmin(solid, remaining / s->latent);
+      solid -= melt;
+      melt_sensible = fmax(0.0, remaining - melt * s->latent);
+    }
+    if (solid <= 0.0) {
+      solid = 0.0;
+      sensible = 0.0;
+    }
+    s->body_mass[k] = solid;
+    s->body_energy[k] = sensible;
+    s->melt_mass_step[k] = melt;
+    s->melt_sensible_step[k] = melt_sensible;
+    if (melt > 0.0) s->interval_body_melt += melt;
+  }
+}
+/* simulator.py:1408-1422 */
+void o2d_freeze_eligibility(o2d* s) {
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      s->eligible[n] = (int8_t)(water_cell(s, i, j) && s->water_v[n] > 1.0e-30);
+    }
+}
+/* simulator.py:1424-1475 */
+void o2d_inject_melt(o2d* s) {
+  double c = cos((double)s->angle), sn = sin((double)s->angle);
+  for (int bi = 0; bi < s->body_nx; ++bi)
+    for (int bj = 0; bj < s->body_ny; ++bj) {
+      long k = (long)bi * s->body_ny + bj;
+      double melt = s->melt_mass_step[k];
+      if (!(melt > 0.0)) continue;
+      double lx = bi + 0.5 - 0.5 * s->body_nx, ly = bj + 0.5 - 0.5 * s->body_ny;
+      double wx = s->origin[0] + c * lx - sn * ly, wy = s->origin[1] + sn * lx + c * ly;
+      int enc = s->melt_target[k], ti = -1, tj = -1;
+      if (enc >= 0) {
+        ti = enc / s->ny;
+        tj = enc - ti * s->ny;
+      } else {
+        int bi0 = (int)floor(wx), bj0 = (int)floor(wy);
+        double best = 1.0e30;
+        for (int di = -2; di <= 2; ++di)
+          for (int dj = -2; dj <= 2; ++dj) {
+            int xi = bi0 + di, xj = bj0 + dj;
+            if (inside(s, xi, xj) && s->eligible[at(s, xi, xj)]) {
+              double d = (double)(di * di + dj * dj);
+              if (d < best) {
+                best = d;
+                ti = xi;
+                tj = xj;
+              }
+            }
+          }
+      }
+      double volume = melt / s->rho_w_si, energy = s->melt_sensible_step[k];
+      if (ti >= 0 && tj >= 0) {
+        s->water_v[at(s, ti, tj)] += volume;
+        s->water_e[at(s, ti, tj)] += energy;
+        s->interval_water_melt += melt;
+      } else {
+        s->unassigned_mass += melt;
+        s->unassigned_energy += energy;
+      }
+    }
+}
+/* simulator.py:1477-1502 */
+void o2d_measure_fallback(o2d* s) {
+  double fr = 0.0, wet = 0.0;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      if (water_cell(s, i, j) && s->water_v[n] > 1.0e-30) {
+        fr += fmax(s->area - s->water_v[n], 0.0);
+        wet += s->water_v[n];
+      }
+    }
+  s->fallback_free = fr;
+  s->fallback_wet = wet;
+}
+/* simulator.py:1504-1540 */
+void o2d_apply_fallback(o2d* s, int use_free) {
+  double den = use_free ? s->fallback_free : s->fallback_wet;
+  double mass = s->unassigned_mass, volume = mass / s->rho_w_si, energy = s->unassigned_energy;
+  if (den > 1.0e-30 && mass > 0.0) s->interval_water_melt += mass;
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      long n = at(s, i, j);
+      double w = 0.0;
+      if (water_cell(s, i, j) && s->water_v[n] > 1.0e-30) w = use_free ? fmax(s->area - s->water_v[n], 0.0) : s->water_v[n];
+      if (w > 0.0 && den > 1.0e-30) {
+        double fr = w / den;
+        s->water_v[n] += fr * volume;
+        s->water_e[n] += fr * energy;
+      }
+    }
+}
+/* simulator.py:1565-1583: out = {solid mass, water mass, body sensible, water sensible} */
+void o2d_reduce_totals(const o2d* s, double out[4]) {
+  long M = (long)s->body_nx * s->body_ny, N = N_(s);
+  out[0] = out[1] = out[2] = out[3] = 0.0;
+  for (long k = 0; k < M; ++k) {
+    out[0] += s->body_mass[k];
+    out[2] += s->body_energy[k];
+  }
+  for (long n = 0; n < N; ++n) {
+    out[1] += s->rho_w_si * s->water_v[n];
+    out[3] += s->water_e[n];
+  }
+}
