@@ -138,9 +138,11 @@ int icy2d_build_aperture_targets(const icy2d_grid* g, void* stream);       /* :7
 int icy2d_apply_aperture_transfer(const icy2d_grid* g, void* stream);      /* :829-859 */
 int icy2d_finish_aperture_transfer(const icy2d_grid* g, void* stream);     /* :861-877 */
 int icy2d_body_conduction(const icy2d_grid* g, double dt, void* stream);   /* :879-907 */
+/* axis 0 resets the boundary power (:630-631) before adding its faces; axis 1 adds. */
 int icy2d_conduction_flux(const icy2d_grid* g, int axis, void* stream);    /* :909-1010, :1063-1164 */
 int icy2d_advection_flux(const icy2d_grid* g, int axis, void* stream);     /* :1012-1061, :1166-1215 */
 int icy2d_update_water(const icy2d_grid* g, double dt, int volume, void* stream); /* :1217-1246 */
+int icy2d_reset_interval_sources(const icy2d_grid* g, void* stream);         /* :624-627 */
 int icy2d_accumulate_boundary_heat(const icy2d_grid* g, double dt, void* stream); /* :1248-1250 */
 int icy2d_exchange_heat(const icy2d_grid* g, double dt, void* stream);     /* :1252-1372 */
 int icy2d_phase_change(const icy2d_grid* g, void* stream);                 /* :1374-1406 */

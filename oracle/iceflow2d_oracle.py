@@ -124,6 +124,7 @@ def oracle2d_lib():
             "advection_flux": [ctypes.c_int], "update_water": [ctypes.c_double, ctypes.c_int],
             "exchange_heat": [ctypes.c_double], "phase_change": [], "freeze_eligibility": [], "inject_melt": [],
             "measure_fallback": [], "apply_fallback": [ctypes.c_int], "reduce_totals": [_D],
+            "prepare_cut_links_all": [],
         }
         for name, args in signatures.items():
             fn = getattr(lib, "o2d_" + name)
@@ -143,6 +144,104 @@ def oracle2d_lib():
 
 def _ptr(array, kind):
     return array.ctypes.data_as(kind)
+
+
+class ArrayField:
+    """Reference-style field view over a flat oracle array (planes [c][nx][ny])."""
+
+    def __init__(self, array, shape, components=0):
+        self.array, self.grid, self.components = array, shape, components
+
+    def _logical(self):
+        if self.components:
+            return np.moveaxis(self.array.reshape(self.components, *self.grid), 0, -1)
+        return self.array.reshape(self.grid)
+
+    @property
+    def shape(self):
+        return self._logical().shape
+
+    def to_numpy(self):
+        return self._logical().copy()
+
+    def from_numpy(self, value):
+        value = np.broadcast_to(np.asarray(value), self.shape)
+        if self.components:
+            self.array[:] = np.moveaxis(value, -1, 0).reshape(-1)
+        else:
+            self.array[:] = value.reshape(-1)
+
+    def fill(self, value):
+        self.array[:] = value
+
+    def __getitem__(self, key):
+        return self._logical()[key]
+
+    def __setitem__(self, key, value):
+        view = self._logical()
+        view[key] = value
+
+
+class StructValue:
+    """0-d field over a member (or a slice of an array member) of the oracle struct."""
+
+    def __init__(self, obj, name, length=None, dtype=np.float64):
+        self.obj, self.name, self.length, self.dtype = obj, name, length, dtype
+
+    def __getitem__(self, key):
+        value = getattr(self.obj, self.name)
+        if self.length is not None:
+            return np.array(value[: self.length], dtype=self.dtype)
+        return self.dtype(value)
+
+    def __setitem__(self, key, value):
+        if self.length is not None:
+            target = getattr(self.obj, self.name)
+            for k, v in enumerate(np.asarray(value, dtype=np.float64).ravel()[: self.length]):
+                target[k] = float(v)
+        else:
+            setattr(self.obj, self.name, np.asarray(value).item())
+
+    def to_numpy(self):
+        return np.asarray(self[None])
+
+
+class Derived:
+    def __init__(self, reader, writer=None):
+        self.reader, self.writer = reader, writer
+
+    def __getitem__(self, key):
+        value = self.reader()
+        return value if key is None else np.asarray(value)[key]
+
+    def __setitem__(self, key, value):
+        self.writer(key, value)
+
+    def to_numpy(self):
+        return np.asarray(self.reader())
+
+
+class OracleThermal:
+    """MovingBodyThermal2D-shaped view of the oracle state."""
+
+    def __init__(self, oracle):
+        a, c = oracle.arrays, oracle.config
+        shape = (oracle.nx, oracle.ny)
+        self._oracle = oracle
+        self.body_solid_mass = ArrayField(a["body_mass"], (c.ice_width, c.ice_height))
+        self.body_sensible_energy = ArrayField(a["body_energy"], (c.ice_width, c.ice_height))
+        self.water_volume_m2 = ArrayField(a["water_v"], shape)
+        self.water_sensible_energy = ArrayField(a["water_e"], shape)
+        self.world_body_solid_fraction = ArrayField(a["coverage"], shape)
+        self._initial_body_cell_mass = oracle.s.cell_mass
+        self.boundary_heat_input = StructValue(oracle.s, "boundary_heat_input")
+        self.time_s = 0.0
+        self.steps = 0
+
+    def mass_energy_totals(self):
+        from types import SimpleNamespace
+
+        return SimpleNamespace(**self._oracle.mass_energy_totals())
 
 
 class Oracle2D:
@@ -236,7 +335,84 @@ class Oracle2D:
 
         self._thermal_pending = 0
         self._melt_pending = False
+        self._make_views()
         self._initialize()
+
+    def _make_views(self):
+        a, s, shape = self.arrays, self.s, (self.nx, self.ny)
+        self.momentum_populations = ArrayField(a["f"], shape, 9)
+        self.momentum_stream_buffer = ArrayField(a["fbuf"], shape, 9)
+        self.phase_populations = ArrayField(a["h"], shape, 9)
+        self.phase_stream_buffer = ArrayField(a["hbuf"], shape, 9)
+        self.water_phase = ArrayField(a["phi"], shape)
+        self.momentum_velocity_lattice = ArrayField(a["u"], shape, 2)
+        self.fluid_acceleration_lattice = ArrayField(a["a"], shape, 2)
+        self.pressure_lattice = ArrayField(a["p"], shape)
+        self.solid_mask = ArrayField(a["solid"], shape)
+        self.previous_solid_mask = ArrayField(a["prev_solid"], shape)
+        self.wall_mask = Derived(self.wall_mask_array)
+        self.hydrostatic_reference_pressure = Derived(lambda: np.broadcast_to(a["p_ref"], shape))
+        self.body_signed_distance_m = Derived(self._signed_distance)
+        self.body_velocity = StructValue(s, "velocity", 2, np.float32)
+        self.body_angle = StructValue(s, "angle", dtype=np.float32)
+        self.body_angular_velocity = StructValue(s, "omega", dtype=np.float32)
+        self.body_reference_origin = StructValue(s, "origin", 2, np.float32)
+        self.body_mass_lattice = StructValue(s, "mass")
+        self.body_inertia_lattice = StructValue(s, "inertia")
+        self.hydrodynamic_impulse = StructValue(s, "impulse", 2)
+        self.hydrodynamic_torque = StructValue(s, "torque")
+        self._body_contact_support_extrema = StructValue(s, "support", 4, np.float32)
+        self._body_contact_geometry_active = Derived(lambda: int(s.support[1] > s.support[0]))
+        self.body_center = Derived(self._center, self._set_center)
+        self.water_volume_current = StructValue(s, "water_volume_current")
+        self.water_volume_target = Derived(self._target_volume)
+        self._water_density_lattice, self._air_density_lattice = s.rho_w, s.rho_a
+        self._body_half_width = 0.5 * self.config.ice_width
+        self._body_half_height = 0.5 * self.config.ice_height
+        self.thermal = OracleThermal(self)
+
+    def wall_mask_array(self):
+        b = self.config.boundary_cells
+        x, y = np.ogrid[: self.nx, : self.ny]
+        return ((x < b) | (x >= self.nx - b) | (y < b) | (y >= self.ny - b)).astype(np.int8)
+
+    def _signed_distance(self):
+        """simulator.py:4605-4633 in FP64."""
+        s = self.s
+        c, sn = math.cos(s.angle), math.sin(s.angle)
+        x, y = np.meshgrid(np.arange(self.nx) + 0.5 - s.origin[0], np.arange(self.ny) + 0.5 - s.origin[1],
+                           indexing="ij")
+        lx, ly = c * x + sn * y, -sn * x + c * y
+        hw, hh = self._body_half_width, self._body_half_height
+        dx, dy = np.abs(lx) - hw, np.abs(ly) - hh
+        d = np.hypot(np.maximum(dx, 0), np.maximum(dy, 0)) + np.minimum(np.maximum(dx, dy), 0)
+        inside = (lx >= -hw) & (lx < hw) & (ly >= -hh) & (ly < hh)
+        d[inside] = self.config.thermal.solid_liquid_threshold - self.arrays["coverage"].reshape(self.nx, self.ny)[inside]
+        return d * self.config.dx
+
+    def _set_center(self, key, value):
+        s = self.s
+        c, sn = math.cos(s.angle), math.sin(s.angle)
+        lx, ly = s.local_com[0], s.local_com[1]
+        s.origin[0] = float(value[0]) - (c * lx - sn * ly)
+        s.origin[1] = float(value[1]) - (sn * lx + c * ly)
+
+    # The reference's kernel methods used by its tests.
+    def _prepare_test_cut_links(self):
+        self._k("prepare_cut_links_all")
+
+    def _stream(self):
+        self._k("stream", 0)
+
+    def _integrate_rigid_ice(self):
+        self._k("integrate_rigid")
+
+    def _solve_contact_and_rasterize(self, resolve_contact=True):
+        self._k("contact_and_rasterize", int(resolve_contact))
+
+    def _prepare_moving_body_contact_support(self):
+        self._k("body_fraction")
+        self._k("contact_support", 1)
 
     def _k(self, name, *args):
         return getattr(self.lib, "o2d_" + name)(self.ps, *args)
@@ -270,7 +446,7 @@ class Oracle2D:
         """simulator.py:2626-2662 (host NumPy FP64)."""
         c, s = self.config, self.s
         phase = np.clip(self.arrays["phi"].reshape(self.nx, self.ny).astype(np.float64), 0.0, 1.0)
-        active = (self.wall_mask() == 0) & (self.arrays["solid"].reshape(self.nx, self.ny) == 0)
+        active = (self.wall_mask_array() == 0) & (self.arrays["solid"].reshape(self.nx, self.ny) == 0)
         count = np.count_nonzero(active, axis=0)
         profile = (np.arange(self.ny) < c.water_height).astype(np.float64)
         np.divide(np.where(active, phase, 0.0).sum(axis=0), count, out=profile, where=count > 0)
@@ -287,11 +463,6 @@ class Oracle2D:
             raise RuntimeError("hydrostatic reference construction produced non-finite values")
         self.arrays["rho_ref"][:] = density.astype(np.float32)
         self.arrays["p_ref"][:] = pressure.astype(np.float32)
-
-    def wall_mask(self):
-        b = self.config.boundary_cells
-        x, y = np.ogrid[: self.nx, : self.ny]
-        return ((x < b) | (x >= self.nx - b) | (y < b) | (y >= self.ny - b)).astype(np.int8)
 
     def _update_mass_properties(self, initialize=False):
         """simulator.py:2729-2733."""
@@ -439,7 +610,7 @@ class Oracle2D:
         self._k("finish_melt_momentum")
         self._melt_pending = False
 
-    def water_volume_target(self):
+    def _target_volume(self):
         """simulator.py:4922-4927."""
         return self.s.initial_water_volume + self.config.ice_mass_lattice - self.s.mass
 
@@ -447,7 +618,7 @@ class Oracle2D:
         """simulator.py:4494-4560."""
         c = self.config
         current = self.lib.o2d_clip_phase(self.ps, MAX_BULK_PHASE_L1)
-        target = self.water_volume_target()
+        target = self._target_volume()
         tolerance = c.volume_projection_tolerance * max(1.0, abs(target))
         initial_error = target - current
         if abs(initial_error) <= tolerance:
@@ -510,7 +681,7 @@ class Oracle2D:
         def planes(name, count):
             return np.moveaxis(a[name].reshape(count, nx, ny), 0, -1).copy()
 
-        center = self.body_center()
+        center = self._center()
         return {
             "momentum_populations": planes("f", 9),
             "phase_populations": planes("h", 9),
@@ -532,7 +703,7 @@ class Oracle2D:
             "body_inertia_lattice": float(s.inertia),
         }
 
-    def body_center(self):
+    def _center(self):
         """simulator.py:4846-4856."""
         s = self.s
         angle = np.float32(s.angle)

@@ -1573,3 +1573,18 @@ void o2d_reduce_totals(const o2d* s, double out[4]) {
     out[3] += s->water_e[n];
   }
 }
+
+/* The reference tests' helper kernel (iceflow2d/test_iceflow.py:14-23):
+ * cut-link preparation applied in place to the stream buffer. */
+void o2d_prepare_cut_links_all(o2d* s) {
+  long N = N_(s);
+  for (int i = 0; i < s->nx; ++i)
+    for (int j = 0; j < s->ny; ++j) {
+      if (!fluid_at(s, i, j)) continue;
+      long n = at(s, i, j);
+      double raw[Q9], prepared[Q9];
+      for (int q = 0; q < Q9; ++q) raw[q] = s->fbuf[q * N + n];
+      prepare_cut_links(s, i, j, raw, prepared);
+      for (int q = 0; q < Q9; ++q) s->fbuf[q * N + n] = (float)prepared[q];
+    }
+}

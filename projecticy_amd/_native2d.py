@@ -1,0 +1,101 @@
+"""ctypes binding of ``libicy2d.so`` (the C ABI declared in include/icy2d.h).
+
+Loaded after ``import torch`` so the library shares PyTorch's HIP runtime.
+A missing library is an error: there is no fallback path.
+"""
+
+import ctypes
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libicy2d.so"
+
+_f, _i, _d, _p = ctypes.c_float, ctypes.c_int32, ctypes.c_double, ctypes.c_void_p
+
+GRID_INTS = ("nx", "ny", "boundary", "body_nx", "body_ny", "advection", "left_kind", "right_kind", "bottom_kind",
+             "top_kind")
+GRID_FLOATS = ("rho_w", "rho_a", "nu_w", "nu_a", "gx", "gy", "sigma", "width", "mobility", "smagorinsky",
+               "interface_tau", "cutoff", "threshold", "dx", "expansion", "t_ref", "linear_damping", "angular_damping")
+GRID_DOUBLES = ("initial_mass", "initial_inertia", "t_melt", "cp_w", "cp_i", "k_w", "k_i", "latent", "rho_w_si",
+                "area", "velocity_scale", "cell_mass", "t_air", "left_value", "right_value", "bottom_value",
+                "top_value")
+GRID_POINTERS = ("f", "fbuf", "h", "hbuf", "phi", "u", "a", "p", "moving", "p_ref", "rho_ref", "solid",
+                 "prev_solid", "eligible", "coverage", "body_mass", "body_energy", "body_heat", "melt_mass",
+                 "melt_sensible", "melt_target", "water_v", "water_e", "target_e", "flux_ex", "flux_vx",
+                 "flux_ey", "flux_vy", "partials")
+
+
+class Control2D(ctypes.Structure):
+    _fields_ = [
+        ("velocity", _f * 2), ("angle", _f), ("omega", _f), ("origin", _f * 2), ("support", _f * 4),
+        ("projected", _i), ("melt_status", _i), ("wet_target", _i), ("wet_target_max", _i),
+        ("max_outflow", _f), ("reserved", _f),
+        ("mass", _d), ("inertia", _d), ("local_com", _d * 2),
+        ("impulse", _d * 3), ("melted", _d * 3), ("carrier", _d * 3), ("correction", _d * 3),
+        ("before", _d * 3), ("provisional", _d * 3), ("after", _d * 3), ("body_before", _d * 3),
+        ("wet", _d * 4), ("water", _d * 3), ("aperture", _d * 6), ("targets", _d * 3), ("transfer", _d * 2),
+        ("correction_abs", _d), ("boundary_power", _d), ("boundary_heat_input", _d),
+        ("unassigned", _d * 2), ("fallback", _d * 2), ("interval", _d * 2),
+        ("reduced", _d * 4), ("totals", _d * 4),
+    ]
+
+
+class Grid2D(ctypes.Structure):
+    _fields_ = (
+        [(n, _i) for n in GRID_INTS]
+        + [(n, _f) for n in GRID_FLOATS]
+        + [(n, _d) for n in GRID_DOUBLES]
+        + [(n, _p) for n in GRID_POINTERS]
+        + [("partials_capacity", _i), ("reserved", _i), ("control", _p)]
+    )
+
+
+class IcyError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what} failed with icy2d code {code}")
+        self.code = code
+
+
+_GRID = ctypes.POINTER(Grid2D)
+# name -> argument types between the grid pointer and the stream.
+CALLS = {
+    "body_fraction": [_i], "commit_raster": [], "contact_support": [_i, _i], "integrate_rigid": [],
+    "project_inside": [], "update_solid_mask": [_i], "reduce_mass_properties": [], "apply_mass_properties": [_i],
+    "initialize_fluid": [_i, _i], "hydrostatic_equilibrium": [], "initialize_water_volume": [],
+    "recover_initial_moments": [], "initialize_body_state": [_d], "initialize_water_state": [_d],
+    "collide": [_i], "prepare_cut_links": [], "stream": [_i], "update_pressure": [], "refill": [],
+    "measure_outflow": [], "clip_phase": [_d], "seed_projection": [], "dilate_projection": [_i],
+    "evaluate_projection": [_d], "apply_projection": [_d], "projection_residual_weight": [_d],
+    "apply_projection_residual": [_d, _d], "snapshot_melt_momentum": [], "reduce_provisional_momentum": [],
+    "melt_weight_geometry": [], "apply_melt_correction": [], "reduce_final_momentum": [],
+    "local_melt_correction": [], "finish_melt_momentum": [], "measure_aperture": [], "build_aperture_targets": [],
+    "apply_aperture_transfer": [], "finish_aperture_transfer": [], "body_conduction": [_d],
+    "conduction_flux": [_i], "advection_flux": [_i], "update_water": [_d, _i], "reset_interval_sources": [], "accumulate_boundary_heat": [_d],
+    "exchange_heat": [_d], "phase_change": [], "freeze_eligibility": [], "inject_melt": [], "measure_fallback": [],
+    "apply_fallback": [_i], "reduce_totals": [],
+}
+
+_LIB = None
+
+
+def lib():
+    """Load and type the 2D library (import torch first)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    import torch  # noqa: F401  -- share PyTorch's HIP runtime
+
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH.name} is not built; run `python -m projecticy_amd.build` first")
+    handle = ctypes.CDLL(str(LIB_PATH))
+    for name, args in CALLS.items():
+        fn = getattr(handle, "icy2d_" + name)
+        fn.argtypes = [_GRID, *args, _p]
+        fn.restype = ctypes.c_int
+    sizes = (_i(), _i())
+    handle.icy2d_abi_sizes(ctypes.byref(sizes[0]), ctypes.byref(sizes[1]))
+    if (sizes[0].value, sizes[1].value) != (ctypes.sizeof(Grid2D), ctypes.sizeof(Control2D)):
+        raise RuntimeError(f"icy2d ABI mismatch: library grid/control {sizes[0].value}/{sizes[1].value} bytes, "
+                           f"binding {ctypes.sizeof(Grid2D)}/{ctypes.sizeof(Control2D)}")
+    _LIB = handle
+    return _LIB

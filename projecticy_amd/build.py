@@ -1,8 +1,9 @@
-"""Build the gfx950 shared library ``libicy3d.so`` in-tree with hipcc.
+"""Build the gfx950 shared libraries in-tree with hipcc.
 
-One fat library holds every HIP kernel, the C ABI of include/icy3d.h and
-the host FP64 rigid-body solver.  Called by ``__graft_entry__.build()``; the
-Python package only loads the result and never builds at import time.
+``libicy3d.so`` holds the 3D kernels, the C ABI of include/icy3d.h and the
+host FP64 rigid-body solver; ``libicy2d.so`` holds the 2D kernels and the C
+ABI of include/icy2d.h.  Called by ``__graft_entry__.build()``; the Python
+package only loads the results and never builds at import time.
 """
 
 import os
@@ -12,8 +13,12 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
+INCLUDE = HERE.parent / "include"
+LIBRARIES = {
+    "libicy3d.so": ["icy3d_flow.hip", "icy3d_geometry.hip", "icy3d_thermal.hip", "rigid3d.cpp"],
+    "libicy2d.so": ["icy2d.hip"],
+}
 LIB = HERE / "libicy3d.so"
-SOURCES = ["icy3d_flow.hip", "icy3d_geometry.hip", "icy3d_thermal.hip", "rigid3d.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # Packed FP32 VALU forms are disabled: ordinary float code only.
@@ -25,37 +30,48 @@ HIP_FLAGS = [
     "-ffp-contract=fast",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
+    "-I", str(INCLUDE),
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
 
-def build(verbose=False, jobs=4):
-    build_dir = HERE / "_build"
-    build_dir.mkdir(exist_ok=True)
+def _compile(sources, build_dir, verbose, jobs):
     objects, procs = [], []
-    for name in SOURCES:
+    for name in sources:
         src = CSRC / name
         obj = build_dir / (name + ".o")
         objects.append(obj)
         cmd = [HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)]
         if name.endswith(".cpp"):
-            cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-Wall", "-c", str(src), "-o", str(obj)]
+            cmd = [HIPCC, "-O3", "-std=c++17", "-fPIC", "-Wall", "-I", str(INCLUDE), "-c", str(src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
-        procs.append(subprocess.Popen(cmd))
+        procs.append((name, subprocess.Popen(cmd)))
         if len(procs) >= jobs:
-            if procs.pop(0).wait() != 0:
-                raise RuntimeError(f"hipcc failed for {name}")
-    for p in procs:
+            name0, p = procs.pop(0)
+            if p.wait() != 0:
+                raise RuntimeError(f"hipcc failed for {name0}")
+    for name0, p in procs:
         if p.wait() != 0:
-            raise RuntimeError("hipcc failed")
-    tmp = LIB.with_suffix(f".so.tmp{os.getpid()}")
-    link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objects), "-o", str(tmp)]
-    if verbose:
-        print(" ".join(link), flush=True)
-    subprocess.run(link, check=True)
-    os.replace(tmp, LIB)
-    return LIB
+            raise RuntimeError(f"hipcc failed for {name0}")
+    return objects
+
+
+def build(verbose=False, jobs=4):
+    build_dir = HERE / "_build"
+    build_dir.mkdir(exist_ok=True)
+    built = []
+    for lib_name, sources in LIBRARIES.items():
+        objects = _compile(sources, build_dir, verbose, jobs)
+        lib = HERE / lib_name
+        tmp = lib.with_suffix(f".so.tmp{os.getpid()}")
+        link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objects), "-o", str(tmp)]
+        if verbose:
+            print(" ".join(link), flush=True)
+        subprocess.run(link, check=True)
+        os.replace(tmp, lib)
+        built.append(lib)
+    return built
 
 
 if __name__ == "__main__":

@@ -1,0 +1,2173 @@
+// MI355X (gfx950) kernels of the 2D falling-and-melting ice step.
+//
+// One thread per cell (n = i * ny + j, 256-thread blocks over the flat
+// index, so a wavefront reads 64 consecutive j of one column: coalesced
+// q-plane loads).  Flow arithmetic is FP32 like the reference; the thermal
+// state, the projection and melt-momentum sums and the rigid mass
+// properties are FP64 like the reference.  Every FP64 sum is a two-stage
+// reduction (per-block partials, then one block folding them in a fixed
+// order), so results do not depend on scheduling.  Rigid-body updates run
+// as single-thread kernels on the control block so the pose never leaves
+// the device between the kernels that use it.
+//
+// Differences in structure from the reference, with identical results:
+//   - momentum and phase collision of a cell run in one pass (the phase
+//     collision reads the acceleration the momentum collision just wrote
+//     for the same cell, as the reference's second kernel does);
+//   - streaming is written as a pull: cell x takes population q from
+//     x - c_q when that cell is fluid, else the bounce-back of its own
+//     outgoing population opp(q) (with the moving-ice phase correction), which is
+//     exactly what the reference's push writes into x; the same thread then
+//     forms phi, u and the pressure's population sum from registers.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "icy2d.h"
+
+namespace icy2 {
+
+constexpr int kBlock = 256;
+
+__host__ __device__ constexpr int cxq(int q) { return (q == 1 || q == 5 || q == 8) ? 1 : (q == 3 || q == 6 || q == 7) ? -1 : 0; }
+__host__ __device__ constexpr int cyq(int q) { return (q == 2 || q == 5 || q == 6) ? 1 : (q == 4 || q == 7 || q == 8) ? -1 : 0; }
+__host__ __device__ constexpr int oppq(int q) { return q == 0 ? 0 : q <= 4 ? ((q + 1) % 4) + 1 : ((q - 3) % 4) + 5; }
+__host__ __device__ constexpr float wq(int q) { return q == 0 ? 4.0f / 9.0f : q <= 4 ? 1.0f / 9.0f : 1.0f / 36.0f; }
+static_assert(oppq(1) == 3 && oppq(2) == 4 && oppq(3) == 1 && oppq(4) == 2, "axis opposites");
+static_assert(oppq(5) == 7 && oppq(6) == 8 && oppq(7) == 5 && oppq(8) == 6, "diagonal opposites");
+
+__device__ __forceinline__ int cells(const icy2d_grid& g) { return g.nx * g.ny; }
+__device__ __forceinline__ int at(const icy2d_grid& g, int i, int j) { return i * g.ny + j; }
+__device__ __forceinline__ bool inside(const icy2d_grid& g, int i, int j) { return i >= 0 && i < g.nx && j >= 0 && j < g.ny; }
+// simulator.py:4566-4574
+__device__ __forceinline__ bool wall_at(const icy2d_grid& g, int i, int j) {
+  const int b = g.boundary;
+  return i < b || i >= g.nx - b || j < b || j >= g.ny - b;
+}
+// simulator.py:2300-2301
+__device__ __forceinline__ bool fluid_at(const icy2d_grid& g, int i, int j) { return !wall_at(g, i, j) && g.solid[at(g, i, j)] == 0; }
+__device__ __forceinline__ float clamp01(float v) { return fminf(1.0f, fmaxf(0.0f, v)); }
+__device__ __forceinline__ double clamp01d(double v) { return fmin(1.0, fmax(0.0, v)); }
+
+// lattice.py:64-104
+template <int Q>
+__device__ __forceinline__ float momentum_eq(float p, float rho, float ux, float uy) {
+  const float cu = (float)cxq(Q) * ux + (float)cyq(Q) * uy, uu = ux * ux + uy * uy;
+  float value = 3.0f * wq(Q) * p + wq(Q) * rho * (3.0f * cu + 4.5f * cu * cu - 1.5f * uu);
+  if (Q == 0) value -= 3.0f * p;
+  return value;
+}
+__device__ __forceinline__ float momentum_eq_rt(int q, float p, float rho, float ux, float uy) {
+  const float cu = (float)cxq(q) * ux + (float)cyq(q) * uy, uu = ux * ux + uy * uy;
+  float value = 3.0f * wq(q) * p + wq(q) * rho * (3.0f * cu + 4.5f * cu * cu - 1.5f * uu);
+  if (q == 0) value -= 3.0f * p;
+  return value;
+}
+template <int Q>
+__device__ __forceinline__ float phase_eq(float phi, float ux, float uy) {
+  const float cu = (float)cxq(Q) * ux + (float)cyq(Q) * uy, uu = ux * ux + uy * uy;
+  return wq(Q) * phi * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * uu);
+}
+__device__ __forceinline__ float phase_eq_rt(int q, float phi, float ux, float uy) {
+  const float cu = (float)cxq(q) * ux + (float)cyq(q) * uy, uu = ux * ux + uy * uy;
+  return wq(q) * phi * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * uu);
+}
+// lattice.py:112-115
+__device__ __forceinline__ float mixture(const icy2d_grid& g, float phi) {
+  const float w = clamp01(phi);
+  return g.rho_w * w + g.rho_a * (1.0f - w);
+}
+// lattice.py:118-177: separable inverse central-moment basis, m = {00,10,01,20,02,11,21,12,22}
+template <int C>
+__device__ __forceinline__ void basis(float u, float b[3]) {
+  if (C == 0) {
+    b[0] = 1.0f - u * u;
+    b[1] = -2.0f * u;
+    b[2] = -1.0f;
+  } else if (C == 1) {
+    b[0] = 0.5f * u * (u + 1.0f);
+    b[1] = u + 0.5f;
+    b[2] = 0.5f;
+  } else {
+    b[0] = 0.5f * u * (u - 1.0f);
+    b[1] = u - 0.5f;
+    b[2] = 0.5f;
+  }
+}
+template <int Q>
+__device__ __forceinline__ float reconstruct_cm(float ux, float uy, const float m[9]) {
+  float bx[3], by[3];
+  basis<cxq(Q)>(ux, bx);
+  basis<cyq(Q)>(uy, by);
+  return bx[0] * by[0] * m[0] + bx[1] * by[0] * m[1] + bx[0] * by[1] * m[2] + bx[2] * by[0] * m[3] +
+         bx[0] * by[2] * m[4] + bx[1] * by[1] * m[5] + bx[2] * by[1] * m[6] + bx[1] * by[2] * m[7] +
+         bx[2] * by[2] * m[8];
+}
+
+// --------------------------------------------------------------- body
+// The pose transform, the bilinear coverage sample and the signed distance
+// are evaluated in FP64: the cut-link fraction sdf_f / (sdf_f - sdf_s)
+// with sdf = threshold - coverage amplifies coverage rounding, and the
+// stored FP32 coverage is then the correctly rounded value.
+struct Pose {
+  double ox, oy, c, s;  // origin and rotation (cos, sin) of the material frame
+  float cx, cy, vx, vy, omega;
+};
+// simulator.py:4835-4845 (centre rounded to FP32 like the reference's) and :2449-2453
+__device__ __forceinline__ Pose load_pose(const icy2d_grid& g) {
+  const icy2d_control* ctl = g.control;
+  Pose p;
+  p.ox = ctl->origin[0];
+  p.oy = ctl->origin[1];
+  p.c = cos((double)ctl->angle);
+  p.s = sin((double)ctl->angle);
+  const double lx = ctl->local_com[0], ly = ctl->local_com[1];
+  p.cx = (float)(p.ox + (p.c * lx - p.s * ly));
+  p.cy = (float)(p.oy + (p.s * lx + p.c * ly));
+  p.vx = ctl->velocity[0];
+  p.vy = ctl->velocity[1];
+  p.omega = ctl->omega;
+  return p;
+}
+__device__ __forceinline__ void body_velocity(const Pose& p, float px, float py, float v[2]) {
+  const float rx = px - p.cx, ry = py - p.cy;
+  v[0] = p.vx + p.omega * -ry;
+  v[1] = p.vy + p.omega * rx;
+}
+// simulator.py:2482-2496
+__device__ __forceinline__ void local_coords(const Pose& p, double px, double py, double& lx, double& ly) {
+  const double rx = px - p.ox, ry = py - p.oy;
+  lx = p.c * rx + p.s * ry;
+  ly = -p.s * rx + p.c * ry;
+}
+// simulator.py:1586-1594
+__device__ __forceinline__ float body_fraction(const icy2d_grid& g, int bi, int bj) {
+  const double mass = fmin(g.cell_mass, fmax(0.0, g.body_mass[bi * g.body_ny + bj]));
+  double fraction = mass / g.cell_mass;
+  if (mass > 0.0) fraction = fmax(fraction, 1.0e-30);
+  return (float)fraction;
+}
+// simulator.py:1596-1603
+__device__ __forceinline__ double body_temperature(const icy2d_grid& g, int k) {
+  double t = g.t_melt;
+  const double mass = g.body_mass[k];
+  if (mass > 1.0e-30) t += g.body_energy[k] / (mass * g.cp_i);
+  return fmin(g.t_melt, t);
+}
+// simulator.py:1605-1613
+__device__ __forceinline__ double water_temperature(const icy2d_grid& g, int n) {
+  const double v = g.water_v[n];
+  if (v > 1.0e-30) return g.t_melt + g.water_e[n] / (g.rho_w_si * g.cp_w * v);
+  return g.t_air;
+}
+// simulator.py:2508-2530
+__device__ __forceinline__ float sample_fraction(const icy2d_grid& g, double lx, double ly) {
+  const double gx = lx + (0.5 * g.body_nx - 0.5), gy = ly + (0.5 * g.body_ny - 0.5);
+  const int bi0 = (int)floor(gx), bj0 = (int)floor(gy);
+  double fraction = 0.0;
+#pragma unroll
+  for (int di = 0; di < 2; ++di)
+#pragma unroll
+    for (int dj = 0; dj < 2; ++dj) {
+      const int bi = bi0 + di, bj = bj0 + dj;
+      if (bi >= 0 && bi < g.body_nx && bj >= 0 && bj < g.body_ny) {
+        const double wx = 1.0 - fabs(gx - bi), wy = 1.0 - fabs(gy - bj);
+        fraction += fmax(0.0, wx) * fmax(0.0, wy) * (double)body_fraction(g, bi, bj);
+      }
+    }
+  return (float)clamp01d(fraction);
+}
+// simulator.py:4587-4603 (lattice cells; scaled by dx where the reference uses metres)
+__device__ __forceinline__ float signed_distance(const icy2d_grid& g, const Pose& p, int i, int j) {
+  double lx, ly;
+  local_coords(p, i + 0.5, j + 0.5, lx, ly);
+  const double hw = 0.5 * g.body_nx, hh = 0.5 * g.body_ny;
+  const int bi = (int)floor(lx + hw), bj = (int)floor(ly + hh);
+  if (bi >= 0 && bi < g.body_nx && bj >= 0 && bj < g.body_ny) return g.threshold - g.coverage[at(g, i, j)];
+  const double dx = fabs(lx) - hw, dy = fabs(ly) - hh;
+  const double ox = fmax(dx, 0.0), oy = fmax(dy, 0.0);
+  return (float)(sqrt(ox * ox + oy * oy) + fmin(fmax(dx, dy), 0.0));
+}
+// simulator.py:4782-4796
+__device__ __forceinline__ float cut_fraction(const icy2d_grid& g, const Pose& p, int i, int j, int ni, int nj) {
+  const float fluid = fmaxf(signed_distance(g, p, i, j) * g.dx, 1.0e-6f);
+  return fminf(0.95f, fmaxf(0.05f, fluid / (fluid - signed_distance(g, p, ni, nj) * g.dx + 1.0e-12f)));
+}
+
+// ------------------------------------------------------- reductions
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+// Writes the block's sums of v[0..K) to partials[blockIdx.x * K + k].
+template <int K>
+__device__ __forceinline__ void block_partials(double (&v)[K], double* partials) {
+  __shared__ double part[K][kBlock / 64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) part[k][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double total = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) total += part[threadIdx.x][w];
+    partials[blockIdx.x * K + threadIdx.x] = total;
+  }
+}
+// Folds partials[b * K + k] over b in a fixed order.  Bit k of min_mask /
+// max_mask selects a minimum / maximum instead of a sum; bit k of add_mask
+// adds the result to out[k] instead of replacing it.
+__global__ void __launch_bounds__(kBlock) k2_finish(const double* __restrict__ partials, int blocks, int K, double* out,
+                                                    unsigned min_mask, unsigned max_mask, unsigned add_mask) {
+  __shared__ double part[kBlock];
+  for (int k = 0; k < K; ++k) {
+    const bool is_min = (min_mask >> k) & 1u, is_max = (max_mask >> k) & 1u;
+    double v = is_min ? 1.0e300 : is_max ? -1.0e300 : 0.0;
+    for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+      const double x = partials[b * K + k];
+      v = is_min ? fmin(v, x) : is_max ? fmax(v, x) : v + x;
+    }
+    part[threadIdx.x] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double total = part[0];
+      for (int t = 1; t < (int)blockDim.x; ++t)
+        total = is_min ? fmin(total, part[t]) : is_max ? fmax(total, part[t]) : total + part[t];
+      out[k] = ((add_mask >> k) & 1u) ? out[k] + total : total;
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int cell_index() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// ---------------------------------------------------- geometry kernels
+// simulator.py:2532-2537; only_if_projected repeats the pass after a
+// contact projection (simulator.py:4109-4111) without a host round trip.
+__global__ void k2_body_fraction(icy2d_grid g, int only_if_projected) {
+  if (only_if_projected && g.control->projected == 0) return;
+  const int n = cell_index();
+  if (n >= cells(g)) return;
+  const Pose p = load_pose(g);
+  const int i = n / g.ny, j = n - i * g.ny;
+  double lx, ly;
+  local_coords(p, i + 0.5, j + 0.5, lx, ly);
+  g.coverage[n] = sample_fraction(g, lx, ly);
+}
+// simulator.py:2539-2544
+__global__ void k2_commit_raster(icy2d_grid g) {
+  const int n = cell_index();
+  if (n >= cells(g)) return;
+  const int i = n / g.ny, j = n - i * g.ny;
+  if (wall_at(g, i, j)) g.coverage[n] = 0.0f;
+}
+// simulator.py:3940-3965: per-block extrema (min x, max x, min y, max y) as partials
+__global__ void k2_support_partials(icy2d_grid g, int only_if_projected) {
+  if (only_if_projected && g.control->projected == 0) return;
+  const int n = cell_index();
+  const Pose p = load_pose(g);
+  double v[4] = {1.0e30, -1.0e30, 1.0e30, -1.0e30};
+  if (n < cells(g)) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    if (g.coverage[n] >= g.threshold && signed_distance(g, p, i, j) <= 0.0f) {
+      v[0] = (float)(i - p.ox);
+      v[1] = (float)(i + 1 - p.ox);
+      v[2] = (float)(j - p.oy);
+      v[3] = (float)(j + 1 - p.oy);
+    }
+  }
+  __shared__ double part[4][kBlock];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) part[k][threadIdx.x] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = threadIdx.x;
+    double r = part[k][0];
+    for (int t = 1; t < (int)blockDim.x; ++t) r = (k & 1) ? fmax(r, part[k][t]) : fmin(r, part[k][t]);
+    g.partials[blockIdx.x * 4 + k] = r;
+  }
+}
+// simulator.py:3932-3938 and :3966-3971: store the extrema
+__global__ void k2_support_store(icy2d_grid g, int reduce, int only_if_projected) {
+  icy2d_control* ctl = g.control;
+  if (only_if_projected && ctl->projected == 0) return;
+  double* e = g.partials + g.partials_capacity - 4;
+  if (!reduce) {
+    // simulator.py:4115-4119: a fixed-pose refresh resets the flag and the extrema
+    ctl->projected = 0;
+    e[0] = e[2] = 1.0e30;
+    e[1] = e[3] = -1.0e30;
+  } else if (!(e[2] < 1.0e20)) {
+    e[0] = e[1] = e[2] = e[3] = 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) ctl->support[k] = (float)e[k];
+}
+// simulator.py:3973-4017
+__global__ void k2_integrate_rigid(icy2d_grid g) {
+  icy2d_control* ctl = g.control;
+  if (ctl->inertia > 0.0) {
+    const float mass = (float)ctl->mass, inertia = (float)ctl->inertia;
+    const float jx = (float)ctl->impulse[0] + mass * g.gx, jy = (float)ctl->impulse[1] + mass * g.gy;
+    const float torque = (float)ctl->impulse[2];
+    const float vx = g.linear_damping * (ctl->velocity[0] + jx / mass);
+    const float vy = g.linear_damping * (ctl->velocity[1] + jy / mass);
+    const float omega = g.angular_damping * (ctl->omega + torque / inertia);
+    const float lcx = (float)ctl->local_com[0], lcy = (float)ctl->local_com[1];
+    const Pose p = load_pose(g);
+    const float angle = ctl->angle + omega;
+    const float cx = p.cx + vx, cy = p.cy + vy;
+    const float c = cosf(angle), s = sinf(angle);
+    ctl->velocity[0] = vx;
+    ctl->velocity[1] = vy;
+    ctl->angle = angle;
+    ctl->omega = omega;
+    ctl->origin[0] = cx - (c * lcx - s * lcy);
+    ctl->origin[1] = cy - (s * lcx + c * lcy);
+  } else {
+    ctl->velocity[0] = ctl->velocity[1] = 0.0f;
+    ctl->omega = 0.0f;
+  }
+  ctl->impulse[0] = ctl->impulse[1] = ctl->impulse[2] = 0.0;
+}
+// simulator.py:4019-4080
+__global__ void k2_project_inside(icy2d_grid g) {
+  icy2d_control* ctl = g.control;
+  int projected = 0;
+  const Pose p = load_pose(g);
+  float cx = p.cx, cy = p.cy, vx = ctl->velocity[0], vy = ctl->velocity[1];
+  const float lcx = (float)ctl->local_com[0], lcy = (float)ctl->local_com[1];
+  const float rcx = p.c * lcx - p.s * lcy, rcy = p.s * lcx + p.c * lcy;
+  if (ctl->inertia > 0.0 && ctl->support[1] > ctl->support[0]) {
+    const float lower = (float)g.boundary, upper_x = (float)(g.nx - g.boundary), upper_y = (float)(g.ny - g.boundary);
+    const float lo_x = lower - (ctl->support[0] - rcx), hi_x = upper_x - (ctl->support[1] - rcx);
+    const float lo_y = lower - (ctl->support[2] - rcy), hi_y = upper_y - (ctl->support[3] - rcy);
+    if (cx <= lo_x) {
+      if (cx < lo_x) projected = 1;
+      cx = lo_x;
+      vx = fmaxf(vx, 0.0f);
+    } else if (cx >= hi_x) {
+      if (cx > hi_x) projected = 1;
+      cx = hi_x;
+      vx = fminf(vx, 0.0f);
+    }
+    if (cy <= lo_y) {
+      if (cy < lo_y) projected = 1;
+      cy = lo_y;
+      vy = fmaxf(vy, 0.0f);
+    } else if (cy >= hi_y) {
+      if (cy > hi_y) projected = 1;
+      cy = hi_y;
+      vy = fminf(vy, 0.0f);
+    }
+  }
+  ctl->velocity[0] = vx;
+  ctl->velocity[1] = vy;
+  ctl->projected = projected;
+  if (projected) {
+    ctl->origin[0] = cx - rcx;
+    ctl->origin[1] = cy - rcy;
+  }
+}
+// simulator.py:2699-2711 (init) and :2713-2727
+__global__ void k2_update_solid_mask(icy2d_grid g, int init) {
+  const int n = cell_index();
+  if (n >= cells(g)) return;
+  const Pose p = load_pose(g);
+  const int i = n / g.ny, j = n - i * g.ny;
+  const bool solid = g.control->inertia > 0.0 && !wall_at(g, i, j) && signed_distance(g, p, i, j) <= 0.0f;
+  if (!init) g.prev_solid[n] = g.solid[n];
+  g.solid[n] = (int8_t)solid;
+  if (init) g.prev_solid[n] = (int8_t)solid;
+}
+
+// -------------------------------------------------------- initialisation
+// simulator.py:2595-2624
+__global__ void k2_initialize_fluid(icy2d_grid g, int water_width, int water_height) {
+  const int n = cell_index(), N = cells(g);
+  if (n >= N) return;
+  const Pose p = load_pose(g);
+  const int i = n / g.ny, j = n - i * g.ny;
+  const float indicator = (i < water_width && j < water_height) ? 1.0f : 0.0f;
+  const bool active = fluid_at(g, i, j);
+  const float phi0 = active ? indicator : 0.0f;
+  float v[2] = {0.0f, 0.0f};
+  if (g.solid[n] == 1) body_velocity(p, i + 0.5f, j + 0.5f, v);
+  g.phi[n] = g.solid[n] == 1 ? indicator : phi0;
+  g.u[n] = v[0];
+  g.u[N + n] = v[1];
+  g.a[n] = g.a[N + n] = 0.0f;
+  g.p[n] = 0.0f;
+  const float rho = mixture(g, phi0);
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    g.f[q * N + n] = momentum_eq_rt(q, 0.0f, rho, v[0], v[1]);
+    g.h[q * N + n] = active ? phase_eq_rt(q, phi0, v[0], v[1]) : 0.0f;
+  }
+}
+// simulator.py:2664-2697
+__global__ void k2_hydrostatic_equilibrium(icy2d_grid g) {
+  const int n = cell_index(), N = cells(g);
+  if (n >= N) return;
+  const int i = n / g.ny, j = n - i * g.ny;
+  if (fluid_at(g, i, j)) {
+    const float phi = g.phi[n], rho = mixture(g, phi);
+    g.u[n] = g.u[N + n] = 0.0f;
+    g.p[n] = g.p_ref[j];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      g.f[q * N + n] = momentum_eq_rt(q, 0.0f, rho, 0.0f, 0.0f);
+      g.h[q * N + n] = phase_eq_rt(q, phi, 0.0f, 0.0f);
+    }
+  } else {
+    float v[2] = {0.0f, 0.0f};
+    if (g.solid[n] == 1) body_velocity(load_pose(g), i + 0.5f, j + 0.5f, v);
+    g.u[n] = v[0];
+    g.u[N + n] = v[1];
+    g.p[n] = 0.0f;
+  }
+}
+// simulator.py:4125-4133 (partials of the fluid phase sum)
+__global__ void k2_water_volume_partials(icy2d_grid g) {
+  const int n = cell_index();
+  double v[1] = {0.0};
+  if (n < cells(g)) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    if (fluid_at(g, i, j)) v[0] = (double)g.phi[n];
+  }
+  block_partials<1>(v, g.partials);
+}
+__global__ void k2_copy_initial_volume(icy2d_grid g) { g.control->water[1] = g.control->water[0]; }
+// simulator.py:3833-3858
+__global__ void k2_recover_initial_moments(icy2d_grid g) {
+  const int n = cell_index(), N = cells(g);
+  if (n >= N) return;
+  const int i = n / g.ny, j = n - i * g.ny;
+  if (fluid_at(g, i, j)) {
+    float phi = 0.0f, mx = 0.0f, my = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      phi += g.h[q * N + n];
+      const float fv = g.f[q * N + n];
+      if (cxq(q)) mx += (float)cxq(q) * fv;
+      if (cyq(q)) my += (float)cyq(q) * fv;
+    }
+    g.phi[n] = phi;
+    const float rho = fmaxf(mixture(g, phi), 1.0e-12f);
+    g.u[n] = mx / rho;
+    g.u[N + n] = my / rho;
+  } else if (g.solid[n] == 1) {
+    float v[2];
+    body_velocity(load_pose(g), i + 0.5f, j + 0.5f, v);
+    g.u[n] = v[0];
+    g.u[N + n] = v[1];
+  } else {
+    g.phi[n] = 0.0f;
+    g.u[n] = g.u[N + n] = 0.0f;
+  }
+}
+
+// --------------------------------------------------------------- flow
+// simulator.py:2459-2477
+__device__ __forceinline__ float phase_neighbor(const icy2d_grid& g, int i, int j, int sx, int sy, int distance,
+                                                float centre) {
+  float value = centre;
+  const int i1 = i + sx, j1 = j + sy;
+  if (inside(g, i1, j1) && fluid_at(g, i1, j1)) {
+    value = g.phi[at(g, i1, j1)];
+    if (distance == 2) {
+      const int i2 = i + 2 * sx, j2 = j + 2 * sy;
+      value = (inside(g, i2, j2) && fluid_at(g, i2, j2)) ? g.phi[at(g, i2, j2)] : centre;
+    }
+  }
+  return value;
+}
+// Blended isotropic gradient of simulator.py:3263-3290 / :3566-3589.
+struct PhaseStencil {
+  float g[2], source[2], laplacian;
+};
+__device__ __forceinline__ PhaseStencil phase_stencil(const icy2d_grid& g, int i, int j, float phi) {
+  PhaseStencil s = {{0.0f, 0.0f}, {0.0f, 0.0f}, 0.0f};
+  const float alpha = 1.0f / 3.0f;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    const float p1 = phase_neighbor(g, i, j, cxq(q), cyq(q), 1, phi);
+    const float p2 = phase_neighbor(g, i, j, cxq(q), cyq(q), 2, phi);
+    const float c[2] = {(float)cxq(q), (float)cyq(q)};
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      s.source[e] += 3.0f * wq(q) * c[e] * (p1 - phi);
+      s.g[e] += (1.0f - alpha) * 3.0f * wq(q) * c[e] * (p1 - phi);
+      s.g[e] += alpha * 1.5f * wq(q) * c[e] * (4.0f * p1 - p2 - 3.0f * phi);
+    }
+    s.laplacian += 6.0f * wq(q) * (p1 - phi);
+  }
+  return s;
+}
+
+// Tao single-node reflection into the cut link's outgoing slot and its
+// momentum exchange (simulator.py:4798-4866).  Returns true when a link was cut.
+__device__ __forceinline__ bool prepare_cut_links(const icy2d_grid& g, const Pose& p, int i, int j, float raw[9],
+                                                  double load[3]) {
+  const int N = cells(g), n = at(g, i, j);
+  const float rho = mixture(g, g.phi[n]);
+  const float pdyn = g.p[n] - g.p_ref[j];
+  const float ux = g.u[n] + 0.5f * g.a[n], uy = g.u[N + n] + 0.5f * g.a[N + n];
+  float prepared[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) prepared[q] = raw[q];
+  bool cut = false;
+#pragma unroll
+  for (int q = 1; q < 9; ++q) {
+    const int ni = i + cxq(q), nj = j + cyq(q);
+    if (!(inside(g, ni, nj) && g.solid[at(g, ni, nj)] == 1)) continue;
+    const float eta = cut_fraction(g, p, i, j, ni, nj);
+    const float bx = (i + 0.5f) + eta * (float)cxq(q), by = (j + 0.5f) + eta * (float)cyq(q);
+    float ub[2];
+    body_velocity(p, bx, by, ub);
+    const int o = oppq(q);
+    const float noneq = g.f[q * N + n] - momentum_eq_rt(q, pdyn, rho, ux, uy);
+    const float wall = momentum_eq_rt(o, pdyn, rho, ub[0], ub[1]);
+    const float reflected = (wall + noneq + eta * raw[o]) / (1.0f + eta);
+    prepared[q] = reflected;
+    float ix = ((float)cxq(q) - ub[0]) * raw[q] - ((float)-cxq(q) - ub[0]) * reflected;
+    float iy = ((float)cyq(q) - ub[1]) * raw[q] - ((float)-cyq(q) - ub[1]) * reflected;
+    const float pref = (1.0f - eta) * g.p_ref[j] + eta * g.p_ref[nj];
+    ix += 6.0f * wq(q) * pref * (float)cxq(q);
+    iy += 6.0f * wq(q) * pref * (float)cyq(q);
+    const double rx = (double)(bx - p.cx), ry = (double)(by - p.cy);
+    load[0] += (double)ix;
+    load[1] += (double)iy;
+    load[2] += rx * (double)iy - ry * (double)ix;
+    cut = true;
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) raw[q] = prepared[q];
+  return cut;
+}
+
+// simulator.py:3241-3552 (momentum) then :3554-3697 (phase) for one cell.
+__global__ void __launch_bounds__(kBlock) k2_collide(icy2d_grid g, int with_momentum) {
+  const int n = cell_index(), N = cells(g);
+  double load[3] = {0.0, 0.0, 0.0};
+  if (n < N) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    if (fluid_at(g, i, j)) {
+      const Pose pose = load_pose(g);
+      const float phi = g.phi[n];
+      const PhaseStencil st = phase_stencil(g, i, j, phi);
+      float ax, ay;
+      if (with_momentum) {
+        // _fluid_force_viscosity_and_gradient (simulator.py:3241-3368)
+        const float bounded = clamp01(phi), rho_b = g.rho_w * bounded + g.rho_a * (1.0f - bounded);
+        const float beta = 12.0f * g.sigma / g.width, kappa = 1.5f * g.sigma * g.width;
+        const float anomaly_rho = rho_b - g.rho_ref[j];
+        float fx = anomaly_rho * g.gx, fy = anomaly_rho * g.gy;
+        const float temperature = (float)water_temperature(g, n);
+        const float ratio = -g.expansion * (temperature - g.t_ref);
+        const float ww = clamp01(2.0f * bounded - 1.0f);
+        fx += ww * g.rho_w * ratio * g.gx;
+        fy += ww * g.rho_w * ratio * g.gy;
+        fx /= rho_b;
+        fy /= rho_b;
+        const float chemical = 4.0f * beta * bounded * (bounded - 1.0f) * (bounded - 0.5f) - kappa * st.laplacian;
+        fx += chemical * st.g[0] / rho_b;
+        fy += chemical * st.g[1] / rho_b;
+        const float ux0 = g.u[n], uy0 = g.u[N + n];
+        float gxx = 0.0f, gxy = 0.0f, gyx = 0.0f, gyy = 0.0f;
+#pragma unroll
+        for (int q = 1; q < 9; ++q) {
+          const int ni = i + cxq(q), nj = j + cyq(q);
+          float v[2] = {0.0f, 0.0f};
+          if (inside(g, ni, nj)) {
+            const int m = at(g, ni, nj);
+            if (fluid_at(g, ni, nj)) {
+              v[0] = g.u[m];
+              v[1] = g.u[N + m];
+            } else if (g.solid[m] == 1) {
+              body_velocity(pose, ni + 0.5f, nj + 0.5f, v);
+            }
+          }
+          const float dxv = v[0] - ux0, dyv = v[1] - uy0, w3 = 3.0f * wq(q);
+          if (cxq(q)) gxx += w3 * (float)cxq(q) * dxv;
+          if (cyq(q)) gxy += w3 * (float)cyq(q) * dxv;
+          if (cxq(q)) gyx += w3 * (float)cxq(q) * dyv;
+          if (cyq(q)) gyy += w3 * (float)cyq(q) * dyv;
+        }
+        const float cs = g.smagorinsky;
+        const float nu_t = cs * cs * sqrtf(2.0f * (gxx * gxx + gyy * gyy + 0.5f * (gxy + gyx) * (gxy + gyx)));
+        ax = fx;
+        ay = fy;
+        g.a[n] = ax;
+        g.a[N + n] = ay;
+        // _collide_momentum (simulator.py:3370-3552)
+        const float ux = ux0 + 0.5f * ax, uy = uy0 + 0.5f * ay;
+        const float fraction = clamp01(phi);
+        const float nu = fraction * g.nu_w + (1.0f - fraction) * g.nu_a + nu_t;
+        const float tau = 0.5f + 3.0f * nu;
+        float near_solid = 0.0f;
+#pragma unroll
+        for (int q = 1; q < 9; ++q) {
+          const int ni = i + cxq(q), nj = j + cyq(q);
+          if (inside(g, ni, nj) && g.solid[at(g, ni, nj)] == 1) near_solid = 1.0f;
+        }
+        const float indicator = fmaxf(1.0f - fraction, near_solid);
+        const float shear_tau = fmaxf(tau, 0.5f + (g.interface_tau - 0.5f) * indicator);
+        const float rho = mixture(g, phi), pdyn = g.p[n] - g.p_ref[j];
+        const float Fx = rho * ax, Fy = rho * ay, drho = g.rho_w - g.rho_a;
+        const float gsx = st.source[0], gsy = st.source[1];
+        float raw[6] = {0, 0, 0, 0, 0, 0}, eq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, src[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+          const float cx = (float)cxq(q), cy = (float)cyq(q), v = g.f[q * N + n];
+          const float e = momentum_eq_rt(q, pdyn, rho, ux, uy);
+          const float b = wq(q) * 3.0f * (cx * Fx + cy * Fy + drho * (cx * ux + cy * uy) * (cx * gsx + cy * gsy));
+          const float k[9] = {1.0f, cx, cy, cx * cx, cy * cy, cx * cy, cx * cx * cy, cx * cy * cy, cx * cx * cy * cy};
+#pragma unroll
+          for (int m = 0; m < 6; ++m) raw[m] += k[m] * v;
+#pragma unroll
+          for (int m = 0; m < 9; ++m) {
+            eq[m] += k[m] * e;
+            src[m] += k[m] * b;
+          }
+        }
+        const float w_phys = 1.0f / tau, w_shear = 1.0f / shear_tau;
+        const float sp = 1.0f - 0.5f * w_phys, ss = 1.0f - 0.5f * w_shear;
+        float post[9];
+#pragma unroll
+        for (int m = 0; m < 3; ++m) post[m] = raw[m] - w_phys * (raw[m] - eq[m]) + sp * src[m];
+        const float trace = eq[3] + eq[4] + 0.5f * (src[3] + src[4]);
+        const float diff = raw[3] - raw[4];
+        const float post_diff = diff - w_shear * (diff - (eq[3] - eq[4])) + ss * (src[3] - src[4]);
+        post[3] = 0.5f * (trace + post_diff);
+        post[4] = 0.5f * (trace - post_diff);
+        post[5] = raw[5] - w_shear * (raw[5] - eq[5]) + ss * src[5];
+#pragma unroll
+        for (int m = 6; m < 9; ++m) post[m] = eq[m] + 0.5f * src[m];
+        float out[9];
+        out[0] = reconstruct_cm<0>(0.0f, 0.0f, post);
+        out[1] = reconstruct_cm<1>(0.0f, 0.0f, post);
+        out[2] = reconstruct_cm<2>(0.0f, 0.0f, post);
+        out[3] = reconstruct_cm<3>(0.0f, 0.0f, post);
+        out[4] = reconstruct_cm<4>(0.0f, 0.0f, post);
+        out[5] = reconstruct_cm<5>(0.0f, 0.0f, post);
+        out[6] = reconstruct_cm<6>(0.0f, 0.0f, post);
+        out[7] = reconstruct_cm<7>(0.0f, 0.0f, post);
+        out[8] = reconstruct_cm<8>(0.0f, 0.0f, post);
+        prepare_cut_links(g, pose, i, j, out, load);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) g.fbuf[q * N + n] = out[q];
+      } else {
+        ax = g.a[n];
+        ay = g.a[N + n];
+      }
+      // _collide_phase (simulator.py:3554-3697)
+      const float ux = g.u[n] + 0.5f * ax, uy = g.u[N + n] + 0.5f * ay;
+      const float rate = 1.0f / (3.0f * g.mobility + 0.5f);
+      const float norm = sqrtf(st.g[0] * st.g[0] + st.g[1] * st.g[1]) + 1.0e-14f;
+      const float nx_ = st.g[0] / norm, ny_ = st.g[1] / norm;
+      const float compression = 4.0f * phi * (1.0f - phi) / g.width;
+      float r10 = 0.0f, r01 = 0.0f, eq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, src[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        const float rx = (float)cxq(q) - ux, ry = (float)cyq(q) - uy, hv = g.h[q * N + n];
+        const float he = phase_eq_rt(q, phi, ux, uy);
+        const float force = wq(q) * compression * ((float)cxq(q) * nx_ + (float)cyq(q) * ny_);
+        const float rx2 = rx * rx, ry2 = ry * ry;
+        const float k[9] = {1.0f, rx, ry, rx2, ry2, rx * ry, rx2 * ry, rx * ry2, rx2 * ry2};
+        r10 += hv * rx;
+        r01 += hv * ry;
+#pragma unroll
+        for (int m = 0; m < 9; ++m) {
+          eq[m] += he * k[m];
+          src[m] += force * k[m];
+        }
+      }
+      float post[9];
+#pragma unroll
+      for (int m = 0; m < 9; ++m) post[m] = eq[m] + 0.5f * src[m];
+      post[1] = r10 - rate * (r10 - eq[1]) + (1.0f - 0.5f * rate) * src[1];
+      post[2] = r01 - rate * (r01 - eq[2]) + (1.0f - 0.5f * rate) * src[2];
+      g.hbuf[0 * N + n] = reconstruct_cm<0>(ux, uy, post);
+      g.hbuf[1 * N + n] = reconstruct_cm<1>(ux, uy, post);
+      g.hbuf[2 * N + n] = reconstruct_cm<2>(ux, uy, post);
+      g.hbuf[3 * N + n] = reconstruct_cm<3>(ux, uy, post);
+      g.hbuf[4 * N + n] = reconstruct_cm<4>(ux, uy, post);
+      g.hbuf[5 * N + n] = reconstruct_cm<5>(ux, uy, post);
+      g.hbuf[6 * N + n] = reconstruct_cm<6>(ux, uy, post);
+      g.hbuf[7 * N + n] = reconstruct_cm<7>(ux, uy, post);
+      g.hbuf[8 * N + n] = reconstruct_cm<8>(ux, uy, post);
+    }
+  }
+  if (with_momentum) block_partials<3>(load, g.partials);
+}
+// The cut-link preparation alone, in place on fbuf (test helper of the
+// reference's own tests around simulator.py:4798-4866).
+__global__ void k2_prepare_cut_links(icy2d_grid g) {
+  const int n = cell_index(), N = cells(g);
+  double load[3] = {0.0, 0.0, 0.0};
+  if (n < N) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    if (fluid_at(g, i, j)) {
+      float raw[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) raw[q] = g.fbuf[q * N + n];
+      prepare_cut_links(g, load_pose(g), i, j, raw, load);
+#pragma unroll
+      for (int q = 0; q < 9; ++q) g.fbuf[q * N + n] = raw[q];
+    }
+  }
+  block_partials<3>(load, g.partials);
+}
+
+// Pull streaming + macroscopic update (simulator.py:3699-3825).
+template <int Q>
+__device__ __forceinline__ void pull(const icy2d_grid& g, const Pose& pose, int i, int j, int n, float phi_old,
+                                     bool phase_only, float& fq, float& hq) {
+  const int N = cells(g);
+  const int si = i - cxq(Q), sj = j - cyq(Q);  // source of population Q
+  if (Q == 0 || (inside(g, si, sj) && fluid_at(g, si, sj))) {
+    const int m = Q == 0 ? n : at(g, si, sj);
+    if (!phase_only) fq = g.fbuf[Q * N + m];
+    hq = g.hbuf[Q * N + m];
+  } else {
+    constexpr int out = oppq(Q);  // own population sent towards (si, sj)
+    if (!phase_only) fq = g.fbuf[out * N + n];
+    float ho = g.hbuf[out * N + n];
+    if (inside(g, si, sj) && g.solid[at(g, si, sj)] == 1) {
+      const float eta = cut_fraction(g, pose, i, j, si, sj);
+      float ub[2];
+      body_velocity(pose, (i + 0.5f) + eta * (float)cxq(out), (j + 0.5f) + eta * (float)cyq(out), ub);
+      ho -= 6.0f * wq(out) * phi_old * ((float)cxq(out) * ub[0] + (float)cyq(out) * ub[1]);
+    }
+    hq = ho;
+  }
+}
+__global__ void __launch_bounds__(kBlock) k2_stream(icy2d_grid g, int phase_only) {
+  const int n = cell_index(), N = cells(g);
+  if (n >= N) return;
+  const int i = n / g.ny, j = n - i * g.ny;
+  const Pose pose = load_pose(g);
+  if (fluid_at(g, i, j)) {
+    const float phi_old = g.phi[n];
+    float f[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, h[9];
+    const bool po = phase_only != 0;
+    pull<0>(g, pose, i, j, n, phi_old, po, f[0], h[0]);
+    pull<1>(g, pose, i, j, n, phi_old, po, f[1], h[1]);
+    pull<2>(g, pose, i, j, n, phi_old, po, f[2], h[2]);
+    pull<3>(g, pose, i, j, n, phi_old, po, f[3], h[3]);
+    pull<4>(g, pose, i, j, n, phi_old, po, f[4], h[4]);
+    pull<5>(g, pose, i, j, n, phi_old, po, f[5], h[5]);
+    pull<6>(g, pose, i, j, n, phi_old, po, f[6], h[6]);
+    pull<7>(g, pose, i, j, n, phi_old, po, f[7], h[7]);
+    pull<8>(g, pose, i, j, n, phi_old, po, f[8], h[8]);
+    float phi = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      g.h[q * N + n] = h[q];
+      phi += h[q];
+    }
+    g.phi[n] = phi;
+    if (!po) {
+      float mx = 0.0f, my = 0.0f, moving = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        g.f[q * N + n] = f[q];
+        if (cxq(q)) mx += (float)cxq(q) * f[q];
+        if (cyq(q)) my += (float)cyq(q) * f[q];
+        if (q) moving += f[q];
+      }
+      const float rho = fmaxf(mixture(g, phi), 1.0e-12f);
+      g.u[n] = mx / rho;
+      g.u[N + n] = my / rho;
+      g.moving[n] = moving;
+    }
+  } else {
+    if (wall_at(g, i, j)) g.phi[n] = 0.0f;
+    if (!phase_only) {
+      float v[2] = {0.0f, 0.0f};
+      if (g.solid[n] == 1) body_velocity(pose, i + 0.5f, j + 0.5f, v);
+      g.u[n] = v[0];
+      g.u[N + n] = v[1];
+      g.a[n] = g.a[N + n] = 0.0f;
+    }
+  }
+}
+// simulator.py:3860-3897
+__global__ void k2_update_pressure(icy2d_grid g) {
+  const int n = cell_index(), N = cells(g);
+  if (n >= N) return;
+  const int i = n / g.ny, j = n - i * g.ny;
+  if (fluid_at(g, i, j)) {
+    const float phi = g.phi[n], rho = mixture(g, phi);
+    const float ux = g.u[n] + 0.5f * g.a[n], uy = g.u[N + n] + 0.5f * g.a[N + n];
+    float gx = 0.0f, gy = 0.0f;
+#pragma unroll
+    for (int q = 1; q < 9; ++q) {
+      const float nb = phase_neighbor(g, i, j, cxq(q), cyq(q), 1, phi);
+      if (cxq(q)) gx += 3.0f * wq(q) * (float)cxq(q) * (nb - phi);
+      if (cyq(q)) gy += 3.0f * wq(q) * (float)cyq(q) * (nb - phi);
+    }
+    const float drho = g.rho_w - g.rho_a;
+    const float s0 = -1.5f * wq(0) * (ux * ux + uy * uy);
+    const float dynamic = 0.6f * (g.moving[n] + 0.5f * (ux * drho * gx + uy * drho * gy) + rho * s0);
+    g.p[n] = g.p_ref[j] + dynamic;
+  } else if (wall_at(g, i, j)) {
+    g.p[n] = 0.0f;
+  }
+}
+// simulator.py:3163-3236
+__global__ void k2_refill(icy2d_grid g) {
+  const int n = cell_index(), N = cells(g);
+  if (n >= N) return;
+  const int i = n / g.ny, j = n - i * g.ny;
+  if (wall_at(g, i, j)) return;
+  const bool became_solid = g.prev_solid[n] == 0 && g.solid[n] == 1;
+  const bool became_fluid = g.prev_solid[n] == 1 && g.solid[n] == 0;
+  if (!became_solid && !became_fluid) return;
+  const Pose pose = load_pose(g);
+  if (became_solid) {
+    float v[2];
+    body_velocity(pose, i + 0.5f, j + 0.5f, v);
+    g.phi[n] = clamp01(g.phi[n]);
+    g.u[n] = v[0];
+    g.u[N + n] = v[1];
+    g.a[n] = g.a[N + n] = 0.0f;
+    g.p[n] = g.p[n] - g.p_ref[j];
+    return;
+  }
+  float phi_sum = 0.0f, p_sum = 0.0f, vx = 0.0f, vy = 0.0f, count = 0.0f;
+  for (int di = -1; di <= 1; ++di)
+    for (int dj = -1; dj <= 1; ++dj) {
+      const int ni = i + di, nj = j + dj;
+      if ((di || dj) && inside(g, ni, nj) && fluid_at(g, ni, nj) && g.prev_solid[at(g, ni, nj)] == 0) {
+        const int m = at(g, ni, nj);
+        phi_sum += clamp01(g.phi[m]);
+        p_sum += g.p[m] - g.p_ref[nj];
+        vx += g.u[m];
+        vy += g.u[N + m];
+        count += 1.0f;
+      }
+    }
+  float phi0 = clamp01(g.phi[n]), p0 = g.p[n], v[2];
+  body_velocity(pose, i + 0.5f, j + 0.5f, v);
+  if (count > 0.0f) {
+    phi0 = phi_sum / count;
+    p0 = p_sum / count;
+    v[0] = vx / count;
+    v[1] = vy / count;
+  }
+  p0 += g.p_ref[j];
+  const float rho = mixture(g, phi0);
+  g.phi[n] = phi0;
+  g.u[n] = v[0];
+  g.u[N + n] = v[1];
+  g.a[n] = g.a[N + n] = 0.0f;
+  g.p[n] = p0;
+  const float pdyn = p0 - g.p_ref[j];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    g.f[q * N + n] = momentum_eq_rt(q, pdyn, rho, v[0], v[1]);
+    g.h[q * N + n] = phase_eq_rt(q, phi0, v[0], v[1]);
+  }
+}
+// simulator.py:2303-2338 (per-block maxima as partials)
+__global__ void k2_outflow_partials(icy2d_grid g) {
+  const int n = cell_index(), N = cells(g);
+  double best = 0.0;
+  if (n < N) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    if (fluid_at(g, i, j) && g.phi[n] > g.cutoff) {
+      const float vx = g.u[n] + 0.5f * g.a[n], vy = g.u[N + n] + 0.5f * g.a[N + n];
+      float outflow = 0.0f;
+      const int off[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int ni = i + off[k][0], nj = j + off[k][1];
+        if (!inside(g, ni, nj)) continue;
+        const int m = at(g, ni, nj);
+        if (fluid_at(g, ni, nj) && g.phi[m] > g.cutoff) {
+          const float wx = g.u[m] + 0.5f * g.a[m], wy = g.u[N + m] + 0.5f * g.a[N + m];
+          outflow += fmaxf(0.0f, (float)off[k][0] * (0.5f * (vx + wx)) + (float)off[k][1] * (0.5f * (vy + wy)));
+        }
+      }
+      if (!isfinite(vx) || !isfinite(vy)) outflow = INFINITY;
+      best = outflow;
+    }
+  }
+  __shared__ double part[kBlock];
+  part[threadIdx.x] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = part[0];
+    for (int t = 1; t < (int)blockDim.x; ++t) r = fmax(r, part[t]);
+    g.partials[blockIdx.x] = r;
+  }
+}
+__global__ void k2_store_outflow(icy2d_grid g) {
+  g.control->max_outflow = (float)g.partials[g.partials_capacity - 1];
+}
+
+// ----------------------------------------------------- volume projection
+__device__ __forceinline__ void lift_phase(const icy2d_grid& g, int n, int j, float old_phase, float new_phase) {
+  const int N = cells(g);
+  const float ux = g.u[n], uy = g.u[N + n], pdyn = g.p[n] - g.p_ref[j];
+  const float ro = mixture(g, old_phase), rn = mixture(g, new_phase);
+#pragma unroll
+  for (int q = 0; q < 9; ++q)
+    g.f[q * N + n] += momentum_eq_rt(q, pdyn, rn, ux, uy) - momentum_eq_rt(q, pdyn, ro, ux, uy);
+  float sum = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    const float lifted = g.h[q * N + n] + phase_eq_rt(q, new_phase, ux, uy) - phase_eq_rt(q, old_phase, ux, uy);
+    g.h[q * N + n] = lifted;
+    sum += lifted;
+  }
+  g.h[n] += new_phase - sum;
+  g.phi[n] = new_phase;
+}
+// simulator.py:4150-4229
+__global__ void k2_clip_phase(icy2d_grid g, double max_bulk_l1) {
+  const int n = cell_index(), N = cells(g);
+  double v[1] = {0.0};
+  if (n < N) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    if (fluid_at(g, i, j)) {
+      const float old_phase = g.phi[n];
+      float bounded = clamp01(old_phase);
+      if (bounded <= g.cutoff) bounded = 0.0f;
+      else if (bounded >= 1.0f - g.cutoff) bounded = 1.0f;
+      const float ux = g.u[n], uy = g.u[N + n];
+      if (bounded != old_phase) {
+        const float pdyn = g.p[n] - g.p_ref[j];
+        const float ro = mixture(g, old_phase), rn = mixture(g, bounded);
+#pragma unroll
+        for (int q = 0; q < 9; ++q)
+          g.f[q * N + n] += momentum_eq_rt(q, pdyn, rn, ux, uy) - momentum_eq_rt(q, pdyn, ro, ux, uy);
+      }
+      float l1 = 0.0f;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) l1 += fabsf(g.h[q * N + n]);
+      const bool regularize = (bounded == 0.0f || bounded == 1.0f) && (double)l1 > max_bulk_l1;
+      if (regularize) {
+        const float px = g.u[n] + 0.5f * g.a[n], py = g.u[N + n] + 0.5f * g.a[N + n];
+        float sum = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+          const float r = phase_eq_rt(q, bounded, px, py);
+          g.h[q * N + n] = r;
+          sum += r;
+        }
+        g.h[n] += bounded - sum;
+      } else if (bounded != old_phase) {
+        float sum = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+          const float lifted = g.h[q * N + n] + phase_eq_rt(q, bounded, ux, uy) - phase_eq_rt(q, old_phase, ux, uy);
+          g.h[q * N + n] = lifted;
+          sum += lifted;
+        }
+        g.h[n] += bounded - sum;
+      }
+      g.phi[n] = bounded;
+      v[0] = (double)bounded;
+    }
+  }
+  block_partials<1>(v, g.partials);
+}
+// simulator.py:4231-4257; the two masks are hbuf planes 0 and 1
+__global__ void k2_seed_projection(icy2d_grid g) {
+  const int n = cell_index(), N = cells(g);
+  if (n >= N) return;
+  const int i = n / g.ny, j = n - i * g.ny;
+  int seed = 0;
+  if (fluid_at(g, i, j)) {
+    const float phase = g.phi[n];
+    if (g.cutoff < phase && phase < 1.0f - g.cutoff) {
+      if (phase == 0.5f) seed = 1;
+      for (int di = -1; di <= 1; ++di)
+        for (int dj = -1; dj <= 1; ++dj) {
+          const int ni = i + di, nj = j + dj;
+          if ((di || dj) && inside(g, ni, nj) && fluid_at(g, ni, nj)) {
+            const float nb = g.phi[at(g, ni, nj)];
+            if ((phase < 0.5f && 0.5f <= nb) || (nb < 0.5f && 0.5f <= phase)) seed = 1;
+          }
+        }
+    }
+  }
+  g.hbuf[n] = (float)seed;
+  g.hbuf[N + n] = 0.0f;
+}
+// simulator.py:4259-4289
+__global__ void k2_dilate_projection(icy2d_grid g, int primary_to_next) {
+  const int n = cell_index(), N = cells(g);
+  if (n >= N) return;
+  const float* src = primary_to_next ? g.hbuf : g.hbuf + N;
+  float* dst = primary_to_next ? g.hbuf + N : g.hbuf;
+  const int i = n / g.ny, j = n - i * g.ny;
+  int value = 0;
+  const float phase = g.phi[n];
+  if (fluid_at(g, i, j) && g.cutoff < phase && phase < 1.0f - g.cutoff) {
+    value = (int)src[n];
+    if (value == 0)
+      for (int di = -1; di <= 1; ++di)
+        for (int dj = -1; dj <= 1; ++dj) {
+          const int ni = i + di, nj = j + dj;
+          if ((di || dj) && inside(g, ni, nj) && src[at(g, ni, nj)] == 1.0f) value = 1;
+        }
+  }
+  dst[n] = (float)value;
+}
+// simulator.py:4291-4351
+__global__ void k2_evaluate_projection(icy2d_grid g, double e) {
+  const int n = cell_index();
+  double v[2] = {0.0, 0.0};
+  if (n < cells(g)) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    if (fluid_at(g, i, j)) {
+      const double phase = g.phi[n];
+      double mapped = phase;
+      const bool adjustable = g.hbuf[n] == 1.0f;
+      if (adjustable) mapped = phase * e / (1.0 - phase + phase * e);
+      float m32 = (float)mapped;
+      if (m32 <= g.cutoff) m32 = 0.0f;
+      else if (m32 >= 1.0f - g.cutoff) m32 = 1.0f;
+      v[0] = (double)m32;
+      if (adjustable && g.cutoff < m32 && m32 < 1.0f - g.cutoff) v[1] = (double)m32 * (1.0 - (double)m32);
+    }
+  }
+  block_partials<2>(v, g.partials);
+}
+// simulator.py:4353-4406
+__global__ void k2_apply_projection(icy2d_grid g, double e) {
+  const int n = cell_index();
+  double v[1] = {0.0};
+  if (n < cells(g)) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    if (fluid_at(g, i, j)) {
+      const float old_phase = g.phi[n];
+      float new_phase = old_phase;
+      if (g.hbuf[n] == 1.0f) {
+        const double p = old_phase;
+        new_phase = (float)(p * e / (1.0 - p + p * e));
+      }
+      if (new_phase <= g.cutoff) new_phase = 0.0f;
+      else if (new_phase >= 1.0f - g.cutoff) new_phase = 1.0f;
+      if (new_phase != old_phase) lift_phase(g, n, j, old_phase, new_phase);
+      v[0] = (double)new_phase;
+    }
+  }
+  block_partials<1>(v, g.partials);
+}
+__device__ __forceinline__ bool residual_free(const icy2d_grid& g, int n, double phase, double residual) {
+  const double margin = 4.0 * 1.1920928955078125e-07;
+  const double cutoff = g.cutoff;
+  if (!(g.hbuf[n] == 1.0f && cutoff < phase && phase < 1.0 - cutoff)) return false;
+  const double magnitude = fabs(residual);
+  if (residual > 0.0) return 1.0 - cutoff - phase > magnitude + margin;
+  return phase - cutoff > magnitude + margin;
+}
+// simulator.py:4408-4431
+__global__ void k2_residual_weight(icy2d_grid g, double residual) {
+  const int n = cell_index();
+  double v[1] = {0.0};
+  if (n < cells(g)) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    const double phase = g.phi[n];
+    if (fluid_at(g, i, j) && residual_free(g, n, phase, residual)) v[0] = phase * (1.0 - phase);
+  }
+  block_partials<1>(v, g.partials);
+}
+// simulator.py:4433-4492
+__global__ void k2_apply_residual(icy2d_grid g, double residual, double total_weight) {
+  const int n = cell_index();
+  double v[1] = {0.0};
+  if (n < cells(g)) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    if (fluid_at(g, i, j)) {
+      const float old_phase = g.phi[n];
+      float new_phase = old_phase;
+      const double p = old_phase;
+      if (residual_free(g, n, p, residual) && total_weight > 0.0)
+        new_phase = (float)(p + residual * (p * (1.0 - p)) / total_weight);
+      if (new_phase != old_phase) lift_phase(g, n, j, old_phase, new_phase);
+      v[0] = (double)new_phase;
+    }
+  }
+  block_partials<1>(v, g.partials);
+}
+
+// ------------------------------------------------------- mass properties
+// simulator.py:3014-3044 over the material grid
+__global__ void k2_mass_partials(icy2d_grid g) {
+  const int k = cell_index();
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (k < g.body_nx * g.body_ny) {
+    const int bi = k / g.body_ny, bj = k - bi * g.body_ny;
+    const double mass = g.body_mass[k] * (1.0 / (g.rho_w_si * g.area));
+    const double fraction = (double)body_fraction(g, bi, bj);
+    const double lx = bi + 0.5 - 0.5 * g.body_nx, ly = bj + 0.5 - 0.5 * g.body_ny;
+    v[0] = mass;
+    v[1] = mass * lx;
+    v[2] = mass * ly;
+    v[3] = mass * (lx * lx + ly * ly + fraction / 6.0);
+  }
+  block_partials<4>(v, g.partials);
+}
+// simulator.py:3046-3161
+__global__ void k2_apply_mass_properties(icy2d_grid g, int initialize) {
+  icy2d_control* ctl = g.control;
+  const double reduced = ctl->reduced[0], old_mass = ctl->mass, initial = g.initial_mass;
+  const double tol = 1.0e-12 * fmax(initial, 1.0);
+  double new_mass = reduced;
+  bool hold = false;
+  if (!initialize && reduced > 0.0 && (fabs(reduced - old_mass) <= tol || reduced > old_mass)) {
+    new_mass = old_mass;
+    hold = true;
+  }
+  const double inactive_mass = fmax(1.0e-12, 1.0e-14 * fmax(initial, 1.0));
+  const double inactive_inertia = 1.0e-14 * fmax(g.initial_inertia, 1.0);
+  const bool resolved = new_mass > inactive_mass, was_active = ctl->inertia > 0.0;
+  const double old_com[2] = {ctl->local_com[0], ctl->local_com[1]};
+  double new_com[2] = {old_com[0], old_com[1]}, raw_inertia = 0.0;
+  if (resolved) {
+    if (hold) {
+      raw_inertia = ctl->inertia;
+    } else {
+      new_com[0] = ctl->reduced[1] / new_mass;
+      new_com[1] = ctl->reduced[2] / new_mass;
+      raw_inertia = ctl->reduced[3] - new_mass * (new_com[0] * new_com[0] + new_com[1] * new_com[1]);
+    }
+    raw_inertia = fmax(raw_inertia, 0.0);
+  }
+  const bool mechanical = resolved && raw_inertia > inactive_inertia && (initialize || was_active);
+  const double new_inertia = mechanical ? raw_inertia : 0.0;
+  const Pose p = load_pose(g);
+  const double c = p.c, s = p.s;
+  const double ow[2] = {c * old_com[0] - s * old_com[1], s * old_com[0] + c * old_com[1]};
+  const double nw[2] = {c * new_com[0] - s * new_com[1], s * new_com[0] + c * new_com[1]};
+  const bool retiring = was_active && !mechanical;
+  if (!initialize && (new_mass < old_mass || retiring)) {
+    const double v0[2] = {ctl->velocity[0], ctl->velocity[1]}, omega = ctl->omega;
+    double sv[2] = {0.0, 0.0};
+    if (mechanical) {
+      sv[0] = v0[0] - omega * (nw[1] - ow[1]);
+      sv[1] = v0[1] + omega * (nw[0] - ow[0]);
+    }
+    const float sv32[2] = {(float)sv[0], (float)sv[1]};
+    const double svs[2] = {sv32[0], sv32[1]};
+    const double nc[2] = {(double)(float)((double)p.ox + nw[0]), (double)(float)((double)p.oy + nw[1])};
+    const double oc[2] = {p.cx, p.cy};
+    ctl->melted[0] += old_mass * v0[0] - new_mass * svs[0];
+    ctl->melted[1] += old_mass * v0[1] - new_mass * svs[1];
+    const double ob[2] = {old_mass * v0[0], old_mass * v0[1]}, nb[2] = {new_mass * svs[0], new_mass * svs[1]};
+    const double old_l = oc[0] * ob[1] - oc[1] * ob[0] + ctl->inertia * omega;
+    const double new_l = nc[0] * nb[1] - nc[1] * nb[0] + new_inertia * omega;
+    ctl->melted[2] += old_l - new_l;
+    ctl->velocity[0] = sv32[0];
+    ctl->velocity[1] = sv32[1];
+  }
+  ctl->local_com[0] = new_com[0];
+  ctl->local_com[1] = new_com[1];
+  ctl->mass = new_mass;
+  ctl->inertia = new_inertia;
+  if (!mechanical) {
+    ctl->velocity[0] = ctl->velocity[1] = 0.0f;
+    ctl->omega = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ctl->support[k] = 0.0f;
+  }
+}
+
+// -------------------------------------------------------- melt momentum
+// Fluid momentum (x, y, angular about the origin) of simulator.py:2735-2760.
+__global__ void k2_momentum_partials(icy2d_grid g) {
+  const int n = cell_index(), N = cells(g);
+  double v[3] = {0.0, 0.0, 0.0};
+  if (n < N) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    if (fluid_at(g, i, j)) {
+      double mx = 0.0, my = 0.0;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        const double fv = g.f[q * N + n];
+        if (cxq(q)) mx += (double)cxq(q) * fv;
+        if (cyq(q)) my += (double)cyq(q) * fv;
+      }
+      v[0] = mx;
+      v[1] = my;
+      v[2] = (i + 0.5) * my - (j + 0.5) * mx;
+    }
+  }
+  block_partials<3>(v, g.partials);
+}
+__global__ void k2_snapshot_body(icy2d_grid g) {
+  icy2d_control* ctl = g.control;
+  for (int k = 0; k < 3; ++k) ctl->body_before[k] = ctl->melted[k];
+}
+// simulator.py:2762-2797: wet weight and the min/max wet cell codes
+__global__ void k2_wet_partials(icy2d_grid g) {
+  const int n = cell_index();
+  double v[1] = {0.0};
+  if (n < cells(g)) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    const float phase = clamp01(g.phi[n]);
+    if (fluid_at(g, i, j) && phase > g.cutoff) {
+      v[0] = (double)phase;
+      atomicMin(&g.control->wet_target, n);
+      atomicMax(&g.control->wet_target_max, n);
+    }
+  }
+  block_partials<1>(v, g.partials);
+}
+__global__ void k2_reset_wet(icy2d_grid g) {
+  g.control->wet_target = g.nx * g.ny;
+  g.control->wet_target_max = -1;
+}
+__global__ void k2_copy_provisional(icy2d_grid g) {
+  icy2d_control* ctl = g.control;
+  for (int k = 0; k < 3; ++k) ctl->after[k] = ctl->provisional[k];
+}
+__device__ __forceinline__ void melt_correction(const icy2d_control* ctl, double c[3]) {
+  for (int k = 0; k < 3; ++k) c[k] = ctl->melted[k] - ctl->body_before[k] - (ctl->after[k] - ctl->before[k]);
+}
+// simulator.py:2799-2840: first and second moments of the wet weights
+__global__ void k2_weight_partials(icy2d_grid g) {
+  const int n = cell_index();
+  double v[3] = {0.0, 0.0, 0.0};
+  if (n < cells(g)) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    const float phase = clamp01(g.phi[n]);
+    if (fluid_at(g, i, j) && phase > g.cutoff) {
+      const double w = phase, px = i + 0.5, py = j + 0.5;
+      v[0] = w * px;
+      v[1] = w * py;
+      v[2] = w * (px * px + py * py);
+    }
+  }
+  block_partials<3>(v, g.partials);
+}
+__device__ __forceinline__ void wet_geometry(const icy2d_control* ctl, double cen[2], double& polar) {
+  cen[0] = cen[1] = 0.0;
+  if (ctl->wet[0] > 1.0e-30) {
+    cen[0] = ctl->wet[1] / ctl->wet[0];
+    cen[1] = ctl->wet[2] / ctl->wet[0];
+  }
+  polar = fmax(0.0, ctl->wet[3] - ctl->wet[0] * (cen[0] * cen[0] + cen[1] * cen[1]));
+}
+__global__ void k2_melt_status(icy2d_grid g) {
+  icy2d_control* ctl = g.control;
+  double c[3], cen[2], polar;
+  melt_correction(ctl, c);
+  wet_geometry(ctl, cen, polar);
+  const double norm = sqrt(c[0] * c[0] + c[1] * c[1]);
+  int status = 0;
+  if (ctl->wet[0] <= 1.0e-30) {
+    if (norm > 1.0e-12 || fabs(c[2]) > 1.0e-12) status = -1;
+  } else {
+    const double couple = c[2] - (cen[0] * c[1] - cen[1] * c[0]);
+    if (polar <= 1.0e-30 && fabs(couple) > 1.0e-12) status = -2;
+    else if (norm > 1.0e-14 || fabs(c[2]) > 1.0e-14) status = 1;
+  }
+  ctl->melt_status = status;
+}
+__device__ __forceinline__ void add_momentum(const icy2d_grid& g, int n, double dmx, double dmy) {
+  const int N = cells(g);
+  const float mx = (float)dmx, my = (float)dmy;
+  const float rho = fmaxf(mixture(g, clamp01(g.phi[n])), 1.0e-12f);
+  g.u[n] += mx / rho;
+  g.u[N + n] += my / rho;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) g.f[q * N + n] += 3.0f * wq(q) * ((float)cxq(q) * mx + (float)cyq(q) * my);
+}
+// simulator.py:2842-2884
+__global__ void k2_apply_melt_correction(icy2d_grid g) {
+  const int n = cell_index();
+  if (n >= cells(g)) return;
+  const icy2d_control* ctl = g.control;
+  const int i = n / g.ny, j = n - i * g.ny;
+  const float phase = clamp01(g.phi[n]);
+  const double den = ctl->wet[0];
+  if (!(fluid_at(g, i, j) && phase > g.cutoff) || !(den > 1.0e-30)) return;
+  double c[3], cen[2], polar, scale = 0.0;
+  melt_correction(ctl, c);
+  wet_geometry(ctl, cen, polar);
+  if (polar > 1.0e-30) scale = (c[2] - (cen[0] * c[1] - cen[1] * c[0])) / polar;
+  const double w = phase, rx = i + 0.5 - cen[0], ry = j + 0.5 - cen[1];
+  add_momentum(g, n, w * (c[0] / den - scale * ry), w * (c[1] / den + scale * rx));
+}
+// simulator.py:2905-2958
+__global__ void k2_local_melt_correction(icy2d_grid g) {
+  const icy2d_control* ctl = g.control;
+  const int e1 = ctl->wet_target, e2 = ctl->wet_target_max, total = g.nx * g.ny;
+  if (!(e1 >= 0 && e1 < total && e2 >= 0 && e2 < total && e1 != e2)) return;
+  const int i1 = e1 / g.ny, j1 = e1 - i1 * g.ny, i2 = e2 / g.ny, j2 = e2 - i2 * g.ny;
+  const double p1[2] = {i1 + 0.5, j1 + 0.5}, p2[2] = {i2 + 0.5, j2 + 0.5};
+  double c[3];
+  melt_correction(ctl, c);
+  const double d[2] = {p1[0] - p2[0], p1[1] - p2[1]}, mid[2] = {0.5 * (p1[0] + p2[0]), 0.5 * (p1[1] + p2[1])};
+  const double couple = c[2] - (mid[0] * c[1] - mid[1] * c[0]);
+  const double dn = fmax(d[0] * d[0] + d[1] * d[1], 1.0e-30);
+  const double qx = couple * -d[1] / dn, qy = couple * d[0] / dn;
+  add_momentum(g, e1, 0.5 * c[0] + qx, 0.5 * c[1] + qy);
+  add_momentum(g, e2, 0.5 * c[0] - qx, 0.5 * c[1] - qy);
+}
+// simulator.py:2960-2986
+__global__ void k2_finish_melt_momentum(icy2d_grid g) {
+  icy2d_control* ctl = g.control;
+  for (int k = 0; k < 3; ++k) {
+    ctl->carrier[k] += ctl->provisional[k] - ctl->before[k];
+    ctl->correction[k] += ctl->after[k] - ctl->provisional[k];
+  }
+}
+
+// ------------------------------------------------------------- thermal
+__device__ __forceinline__ bool water_cell(const icy2d_grid& g, int i, int j) {
+  const int n = at(g, i, j);
+  return !wall_at(g, i, j) && g.solid[n] == 0 && g.phi[n] > g.cutoff;
+}
+// simulator.py:579-588
+__global__ void k2_initialize_body_state(icy2d_grid g, double ice_temperature) {
+  const int k = cell_index();
+  if (k >= g.body_nx * g.body_ny) return;
+  g.body_mass[k] = g.cell_mass;
+  g.body_energy[k] = g.cell_mass * (g.cp_i * (ice_temperature - g.t_melt));
+}
+// simulator.py:589-617
+__global__ void k2_initialize_water_state(icy2d_grid g, double water_temperature_c) {
+  const int n = cell_index();
+  if (n >= cells(g)) return;
+  const int i = n / g.ny, j = n - i * g.ny;
+  const float phase = clamp01(g.phi[n]);
+  double v = 0.0;
+  if (!wall_at(g, i, j) && g.solid[n] == 0 && phase > g.cutoff) v = g.area * (double)phase;
+  g.water_v[n] = v;
+  g.water_e[n] = (g.rho_w_si * g.cp_w) * v * (water_temperature_c - g.t_melt);
+}
+__global__ void k2_reset_thermal_diagnostics(icy2d_grid g) {
+  icy2d_control* ctl = g.control;
+  ctl->boundary_power = ctl->boundary_heat_input = ctl->correction_abs = 0.0;
+}
+// simulator.py:633-714 in control order: volume, energy, base and full
+// capacity (sums), specific energy min, max.
+__global__ void k2_aperture_partials(icy2d_grid g) {
+  const int n = cell_index();
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 1.0e30, -1.0e30};
+  if (n < cells(g)) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    const double vol = g.water_v[n];
+    v[0] = vol;
+    v[1] = g.water_e[n];
+    if (vol > 1.0e-30) v[4] = v[5] = g.water_e[n] / vol;
+    const float phase = clamp01(g.phi[n]);
+    if (!wall_at(g, i, j) && g.solid[n] == 0 && phase > g.cutoff) {
+      v[2] = g.area * (double)phase;
+      v[3] = g.area;
+    }
+  }
+  __shared__ double part[6][kBlock / 64];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const double y = __shfl_xor(x, off);
+      x = k == 4 ? fmin(x, y) : k == 5 ? fmax(x, y) : x + y;
+    }
+    if (lane == 0) part[k][wave] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    double r = part[k][0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+      r = k == 4 ? fmin(r, part[k][w]) : k == 5 ? fmax(r, part[k][w]) : r + part[k][w];
+    g.partials[blockIdx.x * 6 + k] = r;
+  }
+}
+__device__ __forceinline__ void aperture_scales(const icy2d_control* ctl, double& base_scale, double& spare_scale) {
+  const double total = fmax(0.0, ctl->aperture[0]), base = ctl->aperture[2], full = ctl->aperture[3];
+  base_scale = spare_scale = 0.0;
+  if (total <= base && base > 1.0e-30) {
+    base_scale = total / base;
+  } else if (total > base) {
+    base_scale = 1.0;
+    if (full > base + 1.0e-30) spare_scale = fmin(1.0, (total - base) / (full - base));
+  }
+}
+// simulator.py:1629-1658
+__device__ __forceinline__ double aperture_target(const icy2d_grid& g, int i, int j, double bs, double ss) {
+  const int n = at(g, i, j);
+  const float phase = clamp01(g.phi[n]);
+  double target = 0.0;
+  if (!wall_at(g, i, j) && g.solid[n] == 0 && phase > g.cutoff) {
+    const double p = phase;
+    target = g.area * (bs * p + ss * (1.0 - p));
+  }
+  return fmin(g.area, fmax(0.0, target));
+}
+// simulator.py:716-827 (control aperture = {volume, energy, base, full, min, max})
+__global__ void k2_build_aperture_targets(icy2d_grid g) {
+  const int n = cell_index();
+  const icy2d_control* ctl = g.control;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (n < cells(g)) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    double bs, ss;
+    aperture_scales(ctl, bs, ss);
+    const double low = ctl->aperture[4], high = ctl->aperture[5], total = fmax(0.0, ctl->aperture[0]);
+    const double target = aperture_target(g, i, j, bs, ss), vol = g.water_v[n];
+    double energy = 0.0;
+    if (target > 0.0) {
+      double sp = 0.0;
+      if (vol > 1.0e-30) {
+        sp = g.water_e[n] / vol;
+      } else {
+        double weight = 0.0;
+        for (int r = 1; r <= 3 && weight <= 1.0e-30; ++r)
+          for (int di = -r; di <= r; ++di)
+            for (int dj = -r; dj <= r; ++dj) {
+              const int ni = i + di, nj = j + dj;
+              if ((abs(di) > abs(dj) ? abs(di) : abs(dj)) != r || !inside(g, ni, nj) || wall_at(g, ni, nj)) continue;
+              const int m = at(g, ni, nj);
+              const double nv = g.water_v[m];
+              if (nv > 1.0e-30) {
+                const double d2 = (double)(di * di + dj * dj);
+                sp += g.water_e[m] / d2;
+                weight += nv / d2;
+              }
+            }
+        if (weight > 1.0e-30) sp /= weight;
+        else if (total > 1.0e-30) sp = ctl->aperture[1] / total;
+      }
+      sp = fmin(high, fmax(low, sp));
+      energy = target * sp;
+      v[1] = target * (sp - low);
+      v[2] = target * (high - sp);
+    }
+    g.target_e[n] = energy;
+    v[0] = energy;
+  }
+  block_partials<3>(v, g.partials);
+}
+// simulator.py:829-859
+__global__ void k2_apply_aperture_transfer(icy2d_grid g) {
+  const int n = cell_index();
+  if (n >= cells(g)) return;
+  const icy2d_control* ctl = g.control;
+  const int i = n / g.ny, j = n - i * g.ny;
+  const double correction = ctl->aperture[1] - ctl->targets[0];
+  const double low = ctl->aperture[4], high = ctl->aperture[5], cool = ctl->targets[1], heat = ctl->targets[2];
+  double bs, ss;
+  aperture_scales(ctl, bs, ss);
+  const double target = aperture_target(g, i, j, bs, ss);
+  double energy = g.target_e[n];
+  if (target > 0.0) {
+    if (correction < 0.0 && cool > 1.0e-30) energy += fmax(-1.0, correction / cool) * fmax(0.0, energy - low * target);
+    else if (correction > 0.0 && heat > 1.0e-30) energy += fmin(1.0, correction / heat) * fmax(0.0, high * target - energy);
+  }
+  g.water_v[n] = target;
+  g.water_e[n] = energy;
+}
+__global__ void k2_add_correction_abs(icy2d_grid g) {
+  icy2d_control* ctl = g.control;
+  ctl->correction_abs += fabs(ctl->aperture[1] - ctl->targets[0]);
+}
+// simulator.py:861-877
+__global__ void k2_water_totals_partials(icy2d_grid g) {
+  const int n = cell_index();
+  double v[2] = {0.0, 0.0};
+  if (n < cells(g)) {
+    v[0] = g.water_v[n];
+    v[1] = g.water_e[n];
+  }
+  block_partials<2>(v, g.partials);
+}
+// simulator.py:879-907
+__global__ void k2_body_conduction(icy2d_grid g, double dt) {
+  const int k = cell_index();
+  if (k == 0) g.control->unassigned[0] = g.control->unassigned[1] = 0.0;
+  if (k >= g.body_nx * g.body_ny) return;
+  const int bi = k / g.body_ny, bj = k - bi * g.body_ny;
+  g.melt_target[k] = -1;
+  const float fr = body_fraction(g, bi, bj);
+  const double t = body_temperature(g, k);
+  double rate = 0.0;
+  const int off[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+#pragma unroll
+  for (int o = 0; o < 4; ++o) {
+    const int ni = bi + off[o][0], nj = bj + off[o][1];
+    if (ni >= 0 && ni < g.body_nx && nj >= 0 && nj < g.body_ny) {
+      const double face = (double)fminf(fr, body_fraction(g, ni, nj));
+      rate += g.k_i * face * (t - body_temperature(g, ni * g.body_ny + nj));
+    }
+  }
+  g.body_heat[k] = -dt * rate;
+}
+__device__ __forceinline__ double cell_aperture(const icy2d_grid& g, int n) { return fmin(1.0, fmax(0.0, g.water_v[n] / g.area)); }
+// simulator.py:909-1010 (axis 0) and :1063-1164 (axis 1); boundary power as partials
+__global__ void k2_conduction_flux(icy2d_grid g, int axis) {
+  const int t = cell_index();
+  const int fx = axis == 0 ? g.nx + 1 : g.nx, fy = axis == 0 ? g.ny : g.ny + 1, limit = axis == 0 ? g.nx : g.ny;
+  double inward[1] = {0.0};
+  if (t < fx * fy) {
+    const int a = t / fy, b = t - a * fy, face = axis == 0 ? a : b;
+    const int lo_kind = axis == 0 ? g.left_kind : g.bottom_kind, hi_kind = axis == 0 ? g.right_kind : g.top_kind;
+    const double lo_value = axis == 0 ? g.left_value : g.bottom_value, hi_value = axis == 0 ? g.right_value : g.top_value;
+    double out = 0.0;
+    if (face > 0 && face < limit) {
+      const int li = axis == 0 ? a - 1 : a, lj = axis == 0 ? b : b - 1;
+      const int l = at(g, li, lj), r = at(g, a, b);
+      const bool la = water_cell(g, li, lj), ra = water_cell(g, a, b);
+      if (la && ra) {
+        if (g.water_v[l] > 1.0e-30 && g.water_v[r] > 1.0e-30)
+          out = -g.k_w * fmin(cell_aperture(g, l), cell_aperture(g, r)) * (water_temperature(g, r) - water_temperature(g, l));
+      } else if (wall_at(g, li, lj) && ra && g.water_v[r] > 1.0e-30) {
+        if (lo_kind == 1) out = 2.0 * g.k_w * cell_aperture(g, r) * (lo_value - water_temperature(g, r));
+        inward[0] = out;
+      } else if (la && g.water_v[l] > 1.0e-30 && wall_at(g, a, b)) {
+        if (hi_kind == 1) out = 2.0 * g.k_w * cell_aperture(g, l) * (water_temperature(g, l) - hi_value);
+        inward[0] = -out;
+      }
+    }
+    (axis == 0 ? g.flux_ex : g.flux_ey)[t] = out;
+  }
+  block_partials<1>(inward, g.partials);
+}
+// simulator.py:1012-1061 (axis 0) and :1166-1215 (axis 1)
+__global__ void k2_advection_flux(icy2d_grid g, int axis) {
+  const int t = cell_index(), N = cells(g);
+  const int fx = axis == 0 ? g.nx + 1 : g.nx, fy = axis == 0 ? g.ny : g.ny + 1, limit = axis == 0 ? g.nx : g.ny;
+  if (t >= fx * fy) return;
+  const int a = t / fy, b = t - a * fy, face = axis == 0 ? a : b;
+  double e = 0.0, v = 0.0;
+  if (g.advection && face > 0 && face < limit) {
+    const int li = axis == 0 ? a - 1 : a, lj = axis == 0 ? b : b - 1;
+    if (water_cell(g, li, lj) && water_cell(g, a, b)) {
+      const int l = at(g, li, lj), r = at(g, a, b);
+      const float ul = g.u[axis * N + l] + 0.5f * g.a[axis * N + l];
+      const float ur = g.u[axis * N + r] + 0.5f * g.a[axis * N + r];
+      const double speed = 0.5 * (double)(ul + ur) * g.velocity_scale;
+      const int up = speed >= 0.0 ? l : r;
+      const double uv = g.water_v[up], frac = fmin(1.0, fmax(0.0, uv / g.area));
+      v = speed * frac * (double)g.dx;
+      e = v * (g.water_e[up] / fmax(uv, 1.0e-30));
+    }
+  }
+  (axis == 0 ? g.flux_ex : g.flux_ey)[t] = e;
+  (axis == 0 ? g.flux_vx : g.flux_vy)[t] = v;
+}
+// simulator.py:1217-1246
+__global__ void k2_update_water(icy2d_grid g, double dt, int volume) {
+  const int n = cell_index();
+  if (n >= cells(g)) return;
+  const int i = n / g.ny, j = n - i * g.ny, ny = g.ny;
+  if (wall_at(g, i, j)) return;
+  const double de = g.flux_ex[(i + 1) * ny + j] - g.flux_ex[i * ny + j] + g.flux_ey[i * (ny + 1) + j + 1] -
+                    g.flux_ey[i * (ny + 1) + j];
+  g.water_e[n] -= dt * de;
+  if (volume) {
+    const double dv = g.flux_vx[(i + 1) * ny + j] - g.flux_vx[i * ny + j] + g.flux_vy[i * (ny + 1) + j + 1] -
+                      g.flux_vy[i * (ny + 1) + j];
+    g.water_v[n] -= dt * dv;
+  }
+}
+// simulator.py:624-627
+__global__ void k2_reset_interval_sources(icy2d_grid g) { g.control->interval[0] = g.control->interval[1] = 0.0; }
+__global__ void k2_accumulate_boundary_heat(icy2d_grid g, double dt) {
+  g.control->boundary_heat_input += dt * g.control->boundary_power;
+}
+// simulator.py:1252-1372
+__global__ void __launch_bounds__(kBlock) k2_exchange_heat(icy2d_grid g, double dt) {
+  const int n = cell_index();
+  if (n >= cells(g)) return;
+  const int i = n / g.ny, j = n - i * g.ny;
+  if (!(water_cell(g, i, j) && g.water_v[n] > 1.0e-30)) return;
+  const double k = 2.0 * g.k_w * g.k_i / (g.k_w + g.k_i);
+  const Pose pose = load_pose(g);
+  double req[20];
+  int tgt[20];
+#pragma unroll
+  for (int s = 0; s < 20; ++s) {
+    req[s] = 0.0;
+    tgt[s] = 0;
+  }
+  double requested = 0.0;
+  const double ap = fmin(1.0, g.water_v[n] / g.area), tw = water_temperature(g, n);
+  const int off[5][2] = {{0, 0}, {1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+#pragma unroll
+  for (int f = 0; f < 5; ++f) {
+    const int wi = i + off[f][0], wj = j + off[f][1];
+    if (!inside(g, wi, wj)) continue;
+    const double coverage = g.coverage[at(g, wi, wj)];
+    if (!(coverage > 0.0)) continue;
+    const double contact = fmin(ap, fmin(1.0, coverage / (double)g.threshold));
+    // simulator.py:1615-1627, stored as FP32 grid coordinates
+    const double rx = (wi + 0.5) - pose.ox, ry = (wj + 0.5) - pose.oy;
+    const float gx = (float)(pose.c * rx + pose.s * ry + 0.5 * g.body_nx - 0.5);
+    const float gy = (float)(-pose.s * rx + pose.c * ry + 0.5 * g.body_ny - 0.5);
+    const int bi0 = (int)floorf(gx), bj0 = (int)floorf(gy);
+    const double fx = (double)(gx - (float)bi0), fy = (double)(gy - (float)bj0);
+#pragma unroll
+    for (int oi = 0; oi < 2; ++oi)
+#pragma unroll
+      for (int oj = 0; oj < 2; ++oj) {
+        const int bi = bi0 + oi, bj = bj0 + oj;
+        if (!(bi >= 0 && bi < g.body_nx && bj >= 0 && bj < g.body_ny)) continue;
+        const int kb = bi * g.body_ny + bj;
+        if (!(g.body_mass[kb] > 0.0)) continue;
+        const double wx = oi ? fx : 1.0 - fx, wy = oj ? fy : 1.0 - fy;
+        const double share = wx * wy * (double)body_fraction(g, bi, bj) / coverage;
+        const double heat = dt * k * contact * share * fmax(0.0, tw - body_temperature(g, kb));
+        const int slot = f * 4 + oi * 2 + oj;
+        req[slot] = heat;
+        tgt[slot] = kb;
+        requested += heat;
+      }
+  }
+  double scale = 0.0;
+  if (requested > 0.0) scale = fmin(1.0, fmax(0.0, g.water_e[n]) / requested);
+  double transferred = 0.0;
+#pragma unroll
+  for (int s = 0; s < 20; ++s) {
+    const double heat = scale * req[s];
+    if (heat > 0.0) {
+      atomicAdd(&g.body_heat[tgt[s]], heat);
+      atomicMax(&g.melt_target[tgt[s]], n);
+      transferred += heat;
+    }
+  }
+  g.water_e[n] -= transferred;
+}
+// simulator.py:1374-1406 (interval melt as partials)
+__global__ void k2_phase_change(icy2d_grid g) {
+  const int k = cell_index();
+  double v[1] = {0.0};
+  if (k < g.body_nx * g.body_ny) {
+    const double heat = g.body_heat[k];
+    double solid = g.body_mass[k], sensible = g.body_energy[k], melt = 0.0, melt_sensible = 0.0;
+    if (heat <= 0.0) {
+      if (solid > 0.0) sensible += heat;
+    } else {
+      const double gain = fmin(heat, fmax(-sensible, 0.0));
+      sensible += gain;
+      const double remaining = heat - gain;
+      melt = fmin(solid, remaining / g.latent);
+      solid -= melt;
+      melt_sensible = fmax(0.0, remaining - melt * g.latent);
+    }
+    if (solid <= 0.0) {
+      solid = 0.0;
+      sensible = 0.0;
+    }
+    g.body_mass[k] = solid;
+    g.body_energy[k] = sensible;
+    g.melt_mass[k] = melt;
+    g.melt_sensible[k] = melt_sensible;
+    if (melt > 0.0) v[0] = melt;
+  }
+  block_partials<1>(v, g.partials);
+}
+// simulator.py:1408-1422
+__global__ void k2_freeze_eligibility(icy2d_grid g) {
+  const int n = cell_index();
+  if (n >= cells(g)) return;
+  const int i = n / g.ny, j = n - i * g.ny;
+  g.eligible[n] = (int8_t)(water_cell(g, i, j) && g.water_v[n] > 1.0e-30);
+}
+// simulator.py:1424-1475 (assigned melt mass and the unassigned pool as partials)
+__global__ void k2_inject_melt(icy2d_grid g) {
+  const int k = cell_index();
+  double v[3] = {0.0, 0.0, 0.0};
+  if (k < g.body_nx * g.body_ny && g.melt_mass[k] > 0.0) {
+    const icy2d_control* ctl = g.control;
+    const float c = cosf(ctl->angle), s = sinf(ctl->angle);
+    const int bi = k / g.body_ny, bj = k - bi * g.body_ny;
+    const double melt = g.melt_mass[k];
+    const double lx = bi + 0.5 - 0.5 * g.body_nx, ly = bj + 0.5 - 0.5 * g.body_ny;
+    const double wx = ctl->origin[0] + (double)c * lx - (double)s * ly;
+    const double wy = ctl->origin[1] + (double)s * lx + (double)c * ly;
+    const int enc = g.melt_target[k];
+    int ti = -1, tj = -1;
+    if (enc >= 0) {
+      ti = enc / g.ny;
+      tj = enc - ti * g.ny;
+    } else {
+      const int bi0 = (int)floor(wx), bj0 = (int)floor(wy);
+      float best = 1.0e30f;
+      for (int di = -2; di <= 2; ++di)
+        for (int dj = -2; dj <= 2; ++dj) {
+          const int xi = bi0 + di, xj = bj0 + dj;
+          if (inside(g, xi, xj) && g.eligible[at(g, xi, xj)]) {
+            const float d = (float)(di * di + dj * dj);
+            if (d < best) {
+              best = d;
+              ti = xi;
+              tj = xj;
+            }
+          }
+        }
+    }
+    const double volume = melt / g.rho_w_si, energy = g.melt_sensible[k];
+    if (ti >= 0 && tj >= 0) {
+      atomicAdd(&g.water_v[at(g, ti, tj)], volume);
+      atomicAdd(&g.water_e[at(g, ti, tj)], energy);
+      v[0] = melt;
+    } else {
+      v[1] = melt;
+      v[2] = energy;
+    }
+  }
+  block_partials<3>(v, g.partials);
+}
+__global__ void k2_store_injection(icy2d_grid g) {
+  const double* t = g.partials + g.partials_capacity - 3;
+  icy2d_control* ctl = g.control;
+  ctl->interval[1] += t[0];
+  ctl->unassigned[0] += t[1];
+  ctl->unassigned[1] += t[2];
+}
+// simulator.py:1477-1502
+__global__ void k2_fallback_partials(icy2d_grid g) {
+  const int n = cell_index();
+  double v[2] = {0.0, 0.0};
+  if (n < cells(g)) {
+    const int i = n / g.ny, j = n - i * g.ny;
+    if (water_cell(g, i, j) && g.water_v[n] > 1.0e-30) {
+      v[0] = fmax(g.area - g.water_v[n], 0.0);
+      v[1] = g.water_v[n];
+    }
+  }
+  block_partials<2>(v, g.partials);
+}
+// simulator.py:1504-1540
+__global__ void k2_apply_fallback(icy2d_grid g, int use_free) {
+  const int n = cell_index();
+  if (n >= cells(g)) return;
+  const icy2d_control* ctl = g.control;
+  const double den = use_free ? ctl->fallback[0] : ctl->fallback[1];
+  const double mass = ctl->unassigned[0], volume = mass / g.rho_w_si, energy = ctl->unassigned[1];
+  const int i = n / g.ny, j = n - i * g.ny;
+  double w = 0.0;
+  if (water_cell(g, i, j) && g.water_v[n] > 1.0e-30) w = use_free ? fmax(g.area - g.water_v[n], 0.0) : g.water_v[n];
+  if (w > 0.0 && den > 1.0e-30) {
+    const double fr = w / den;
+    g.water_v[n] += fr * volume;
+    g.water_e[n] += fr * energy;
+  }
+}
+__global__ void k2_count_fallback(icy2d_grid g, int use_free) {
+  icy2d_control* ctl = g.control;
+  const double den = use_free ? ctl->fallback[0] : ctl->fallback[1];
+  if (den > 1.0e-30 && ctl->unassigned[0] > 0.0) ctl->interval[1] += ctl->unassigned[0];
+}
+// simulator.py:1565-1583
+__global__ void k2_totals_partials(icy2d_grid g) {
+  const int n = cell_index();
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (n < g.body_nx * g.body_ny) {
+    v[0] = g.body_mass[n];
+    v[2] = g.body_energy[n];
+  }
+  if (n < cells(g)) {
+    v[1] = g.rho_w_si * g.water_v[n];
+    v[3] = g.water_e[n];
+  }
+  block_partials<4>(v, g.partials);
+}
+
+// ---------------------------------------------------------------- host
+unsigned blocks_for(long count) { return (unsigned)((count + kBlock - 1) / kBlock); }
+
+bool valid(const icy2d_grid* g) {
+  return g && g->nx > 0 && g->ny > 0 && g->body_nx > 0 && g->body_ny > 0 && g->control && g->partials &&
+         g->partials_capacity >= 8;
+}
+int launch_status() {
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (int)e;
+}
+// Launch a partials-producing kernel's finish into out[0..K).
+int finish(const icy2d_grid& g, unsigned blocks, int K, double* out, unsigned min_mask, unsigned max_mask,
+           unsigned add_mask, hipStream_t s) {
+  k2_finish<<<1, kBlock, 0, s>>>(g.partials, (int)blocks, K, out, min_mask, max_mask, add_mask);
+  return launch_status();
+}
+bool fits(const icy2d_grid& g, unsigned blocks, int K) { return (long)blocks * K + 8 <= (long)g.partials_capacity; }
+
+}  // namespace icy2
+
+using namespace icy2;
+
+#define ICY2D_GRID_CHECK(g)                     \
+  do {                                          \
+    if (!valid(g)) return ICY2D_ERR_ARGUMENT;   \
+  } while (0)
+#define S(stream) ((hipStream_t)(stream))
+
+extern "C" {
+
+int icy2d_body_fraction(const icy2d_grid* g, int only_if_projected, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_body_fraction<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, only_if_projected);
+  return launch_status();
+}
+int icy2d_commit_raster(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_commit_raster<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_contact_support(const icy2d_grid* g, int reduce, int only_if_projected, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 4)) return ICY2D_ERR_ARGUMENT;
+  if (reduce) {
+    k2_support_partials<<<blocks, kBlock, 0, S(stream)>>>(*g, only_if_projected);
+    if (int rc = launch_status()) return rc;
+    // min x, max x, min y, max y into the last four partials slots
+    if (int rc = finish(*g, blocks, 4, g->partials + g->partials_capacity - 4, 0x5u, 0xAu, 0u, S(stream))) return rc;
+  }
+  k2_support_store<<<1, 1, 0, S(stream)>>>(*g, reduce, only_if_projected);
+  return launch_status();
+}
+int icy2d_integrate_rigid(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_integrate_rigid<<<1, 1, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_project_inside(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_project_inside<<<1, 1, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_update_solid_mask(const icy2d_grid* g, int init, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_update_solid_mask<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, init);
+  return launch_status();
+}
+int icy2d_reduce_mass_properties(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->body_nx * g->body_ny);
+  if (!fits(*g, blocks, 4)) return ICY2D_ERR_ARGUMENT;
+  k2_mass_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
+  if (int rc = launch_status()) return rc;
+  return finish(*g, blocks, 4, g->control->reduced, 0u, 0u, 0u, S(stream));
+}
+int icy2d_apply_mass_properties(const icy2d_grid* g, int initialize, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_apply_mass_properties<<<1, 1, 0, S(stream)>>>(*g, initialize);
+  return launch_status();
+}
+int icy2d_initialize_fluid(const icy2d_grid* g, int water_width, int water_height, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_initialize_fluid<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, water_width, water_height);
+  return launch_status();
+}
+int icy2d_hydrostatic_equilibrium(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_hydrostatic_equilibrium<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_initialize_water_volume(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
+  k2_water_volume_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
+  if (int rc = launch_status()) return rc;
+  if (int rc = finish(*g, blocks, 1, g->control->water, 0u, 0u, 0u, S(stream))) return rc;
+  k2_copy_initial_volume<<<1, 1, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_recover_initial_moments(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_recover_initial_moments<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_initialize_body_state(const icy2d_grid* g, double ice_temperature, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_initialize_body_state<<<blocks_for((long)g->body_nx * g->body_ny), kBlock, 0, S(stream)>>>(*g, ice_temperature);
+  return launch_status();
+}
+int icy2d_initialize_water_state(const icy2d_grid* g, double water_temperature_c, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_initialize_water_state<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, water_temperature_c);
+  if (int rc = launch_status()) return rc;
+  k2_reset_thermal_diagnostics<<<1, 1, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_collide(const icy2d_grid* g, int with_momentum, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 3)) return ICY2D_ERR_ARGUMENT;
+  k2_collide<<<blocks, kBlock, 0, S(stream)>>>(*g, with_momentum);
+  if (int rc = launch_status()) return rc;
+  if (!with_momentum) return 0;
+  return finish(*g, blocks, 3, g->control->impulse, 0u, 0u, 0x7u, S(stream));
+}
+int icy2d_prepare_cut_links(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 3)) return ICY2D_ERR_ARGUMENT;
+  k2_prepare_cut_links<<<blocks, kBlock, 0, S(stream)>>>(*g);
+  if (int rc = launch_status()) return rc;
+  return finish(*g, blocks, 3, g->control->impulse, 0u, 0u, 0x7u, S(stream));
+}
+int icy2d_stream(const icy2d_grid* g, int phase_only, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_stream<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, phase_only);
+  return launch_status();
+}
+int icy2d_update_pressure(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_update_pressure<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_refill(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_refill<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_measure_outflow(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
+  k2_outflow_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
+  if (int rc = launch_status()) return rc;
+  if (int rc = finish(*g, blocks, 1, g->partials + g->partials_capacity - 1, 0u, 0x1u, 0u, S(stream))) return rc;
+  k2_store_outflow<<<1, 1, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_clip_phase(const icy2d_grid* g, double max_bulk_l1, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
+  k2_clip_phase<<<blocks, kBlock, 0, S(stream)>>>(*g, max_bulk_l1);
+  if (int rc = launch_status()) return rc;
+  return finish(*g, blocks, 1, g->control->water + 1, 0u, 0u, 0u, S(stream));
+}
+int icy2d_seed_projection(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_seed_projection<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_dilate_projection(const icy2d_grid* g, int primary_to_next, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_dilate_projection<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, primary_to_next);
+  return launch_status();
+}
+int icy2d_evaluate_projection(const icy2d_grid* g, double lambda, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 2)) return ICY2D_ERR_ARGUMENT;
+  k2_evaluate_projection<<<blocks, kBlock, 0, S(stream)>>>(*g, std::exp(lambda));
+  if (int rc = launch_status()) return rc;
+  return finish(*g, blocks, 2, g->control->water + 1, 0u, 0u, 0u, S(stream));
+}
+int icy2d_apply_projection(const icy2d_grid* g, double lambda, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
+  k2_apply_projection<<<blocks, kBlock, 0, S(stream)>>>(*g, std::exp(lambda));
+  if (int rc = launch_status()) return rc;
+  return finish(*g, blocks, 1, g->control->water + 1, 0u, 0u, 0u, S(stream));
+}
+int icy2d_projection_residual_weight(const icy2d_grid* g, double residual, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
+  k2_residual_weight<<<blocks, kBlock, 0, S(stream)>>>(*g, residual);
+  if (int rc = launch_status()) return rc;
+  return finish(*g, blocks, 1, g->control->water + 2, 0u, 0u, 0u, S(stream));
+}
+int icy2d_apply_projection_residual(const icy2d_grid* g, double residual, double weight, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
+  k2_apply_residual<<<blocks, kBlock, 0, S(stream)>>>(*g, residual, weight);
+  if (int rc = launch_status()) return rc;
+  return finish(*g, blocks, 1, g->control->water + 1, 0u, 0u, 0u, S(stream));
+}
+static int fluid_momentum(const icy2d_grid* g, double* out, hipStream_t s) {
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 3)) return ICY2D_ERR_ARGUMENT;
+  k2_momentum_partials<<<blocks, kBlock, 0, s>>>(*g);
+  if (int rc = launch_status()) return rc;
+  return finish(*g, blocks, 3, out, 0u, 0u, 0u, s);
+}
+int icy2d_snapshot_melt_momentum(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_snapshot_body<<<1, 1, 0, S(stream)>>>(*g);
+  if (int rc = launch_status()) return rc;
+  return fluid_momentum(g, g->control->before, S(stream));
+}
+int icy2d_reduce_provisional_momentum(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  if (int rc = fluid_momentum(g, g->control->provisional, S(stream))) return rc;
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  k2_reset_wet<<<1, 1, 0, S(stream)>>>(*g);
+  k2_wet_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
+  if (int rc = launch_status()) return rc;
+  if (int rc = finish(*g, blocks, 1, g->control->wet, 0u, 0u, 0u, S(stream))) return rc;
+  k2_copy_provisional<<<1, 1, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_melt_weight_geometry(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 3)) return ICY2D_ERR_ARGUMENT;
+  k2_weight_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
+  if (int rc = launch_status()) return rc;
+  if (int rc = finish(*g, blocks, 3, g->control->wet + 1, 0u, 0u, 0u, S(stream))) return rc;
+  k2_melt_status<<<1, 1, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_apply_melt_correction(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_apply_melt_correction<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_reduce_final_momentum(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  return fluid_momentum(g, g->control->after, S(stream));
+}
+int icy2d_local_melt_correction(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_local_melt_correction<<<1, 1, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_finish_melt_momentum(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_finish_melt_momentum<<<1, 1, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_measure_aperture(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 6)) return ICY2D_ERR_ARGUMENT;
+  k2_aperture_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
+  if (int rc = launch_status()) return rc;
+  return finish(*g, blocks, 6, g->control->aperture, 0x10u, 0x20u, 0u, S(stream));
+}
+int icy2d_build_aperture_targets(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 3)) return ICY2D_ERR_ARGUMENT;
+  k2_build_aperture_targets<<<blocks, kBlock, 0, S(stream)>>>(*g);
+  if (int rc = launch_status()) return rc;
+  return finish(*g, blocks, 3, g->control->targets, 0u, 0u, 0u, S(stream));
+}
+int icy2d_apply_aperture_transfer(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_add_correction_abs<<<1, 1, 0, S(stream)>>>(*g);
+  k2_apply_aperture_transfer<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_finish_aperture_transfer(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 2)) return ICY2D_ERR_ARGUMENT;
+  k2_water_totals_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
+  if (int rc = launch_status()) return rc;
+  return finish(*g, blocks, 2, g->control->transfer, 0u, 0u, 0u, S(stream));
+}
+int icy2d_body_conduction(const icy2d_grid* g, double dt, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_body_conduction<<<blocks_for((long)g->body_nx * g->body_ny), kBlock, 0, S(stream)>>>(*g, dt);
+  return launch_status();
+}
+int icy2d_conduction_flux(const icy2d_grid* g, int axis, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  if (axis != 0 && axis != 1) return ICY2D_ERR_ARGUMENT;
+  const long faces = axis == 0 ? (long)(g->nx + 1) * g->ny : (long)g->nx * (g->ny + 1);
+  const unsigned blocks = blocks_for(faces);
+  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
+  k2_conduction_flux<<<blocks, kBlock, 0, S(stream)>>>(*g, axis);
+  if (int rc = launch_status()) return rc;
+  // simulator.py:630-631: the power is reset before the x faces and summed over both passes
+  return finish(*g, blocks, 1, &g->control->boundary_power, 0u, 0u, axis == 0 ? 0u : 0x1u, S(stream));
+}
+int icy2d_advection_flux(const icy2d_grid* g, int axis, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  if (axis != 0 && axis != 1) return ICY2D_ERR_ARGUMENT;
+  const long faces = axis == 0 ? (long)(g->nx + 1) * g->ny : (long)g->nx * (g->ny + 1);
+  k2_advection_flux<<<blocks_for(faces), kBlock, 0, S(stream)>>>(*g, axis);
+  return launch_status();
+}
+int icy2d_update_water(const icy2d_grid* g, double dt, int volume, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_update_water<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, dt, volume);
+  return launch_status();
+}
+int icy2d_reset_interval_sources(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_reset_interval_sources<<<1, 1, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_accumulate_boundary_heat(const icy2d_grid* g, double dt, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_accumulate_boundary_heat<<<1, 1, 0, S(stream)>>>(*g, dt);
+  return launch_status();
+}
+int icy2d_exchange_heat(const icy2d_grid* g, double dt, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_exchange_heat<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, dt);
+  return launch_status();
+}
+int icy2d_phase_change(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->body_nx * g->body_ny);
+  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
+  k2_phase_change<<<blocks, kBlock, 0, S(stream)>>>(*g);
+  if (int rc = launch_status()) return rc;
+  return finish(*g, blocks, 1, g->control->interval, 0u, 0u, 0x1u, S(stream));
+}
+int icy2d_freeze_eligibility(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_freeze_eligibility<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_inject_melt(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->body_nx * g->body_ny);
+  if (!fits(*g, blocks, 3)) return ICY2D_ERR_ARGUMENT;
+  k2_inject_melt<<<blocks, kBlock, 0, S(stream)>>>(*g);
+  if (int rc = launch_status()) return rc;
+  // partial columns: assigned melt, unassigned mass, unassigned energy
+  if (int rc = finish(*g, blocks, 3, g->partials + g->partials_capacity - 3, 0u, 0u, 0u, S(stream))) return rc;
+  k2_store_injection<<<1, 1, 0, S(stream)>>>(*g);
+  return launch_status();
+}
+int icy2d_measure_fallback(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const unsigned blocks = blocks_for((long)g->nx * g->ny);
+  if (!fits(*g, blocks, 2)) return ICY2D_ERR_ARGUMENT;
+  k2_fallback_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
+  if (int rc = launch_status()) return rc;
+  return finish(*g, blocks, 2, g->control->fallback, 0u, 0u, 0u, S(stream));
+}
+int icy2d_apply_fallback(const icy2d_grid* g, int use_free, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  k2_count_fallback<<<1, 1, 0, S(stream)>>>(*g, use_free);
+  k2_apply_fallback<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, use_free);
+  return launch_status();
+}
+int icy2d_reduce_totals(const icy2d_grid* g, void* stream) {
+  ICY2D_GRID_CHECK(g);
+  const long count = (long)g->nx * g->ny > (long)g->body_nx * g->body_ny ? (long)g->nx * g->ny
+                                                                           : (long)g->body_nx * g->body_ny;
+  const unsigned blocks = blocks_for(count);
+  if (!fits(*g, blocks, 4)) return ICY2D_ERR_ARGUMENT;
+  k2_totals_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
+  if (int rc = launch_status()) return rc;
+  return finish(*g, blocks, 4, g->control->totals, 0u, 0u, 0u, S(stream));
+}
+void icy2d_abi_sizes(int32_t* grid, int32_t* control) {
+  if (grid) *grid = (int32_t)sizeof(icy2d_grid);
+  if (control) *control = (int32_t)sizeof(icy2d_control);
+}
+
+}  // extern "C"

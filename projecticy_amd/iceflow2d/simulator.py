@@ -1,0 +1,761 @@
+"""``IceFlow2D`` on MI355X: the reference's API over the icy2d C ABI.
+
+Same constructor checks, ``step(num_steps)`` contract, host schedule and
+public fields as ``iceflow2d/simulator.py:1826-4560``.  State lives in
+PyTorch-ROCm tensors (q-plane-major populations, j fastest, like the
+reference's own layout); every kernel is a hand-written gfx950 HIP kernel in
+``libicy2d.so``.  Scalars the reference keeps in 0-d fields live in one
+device control block that the host reads with a single copy wherever the
+reference reads a device scalar.
+"""
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from .. import _native2d as native
+from ..iceflow3d.fields import DeviceField
+from .config import BOUNDARY_CODE, IceFlowConfig, LatticeScales, MovingBodyThermalTotals
+
+MAX_BULK_PHASE_L1 = 2.0          # simulator.py:45
+MAX_LATTICE_VELOCITY_L1 = 1.0    # simulator.py:44
+
+
+def _require_device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("IceFlow2D requires a ROCm GPU; no CPU fallback is provided")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+class ControlValue:
+    """A 0-d (or short-vector) field stored in the device control block."""
+
+    def __init__(self, sim, name, index=None, dtype=np.float64, length=None):
+        self.sim, self.name, self.index, self.dtype, self.length = sim, name, index, dtype, length
+
+    def _get(self):
+        value = getattr(self.sim._read_control(), self.name)
+        if self.length is not None:
+            start = self.index or 0
+            return np.array([value[start + k] for k in range(self.length)], dtype=self.dtype)
+        if self.index is not None:
+            value = value[self.index]
+        return self.dtype(value)
+
+    def __getitem__(self, key):
+        return self._get()
+
+    def __setitem__(self, key, value):
+        control = self.sim._read_control()
+        target = getattr(control, self.name)
+        if self.length is not None:
+            start = self.index or 0
+            for k, v in enumerate(np.asarray(value, dtype=np.float64).ravel()[: self.length]):
+                target[start + k] = float(v)
+        elif self.index is not None:
+            target[self.index] = float(np.asarray(value).item())
+        else:
+            setattr(control, self.name, np.asarray(value, dtype=self.dtype).item())
+        self.sim._write_control(control)
+
+    def to_numpy(self):
+        return np.asarray(self._get())
+
+    def from_numpy(self, value):
+        self[None] = value
+
+    def fill(self, value):
+        self[None] = value
+
+
+class DerivedValue:
+    """A read-only view computed from other state (``_DerivedField``)."""
+
+    def __init__(self, reader, writer=None):
+        self._reader, self._writer = reader, writer
+
+    def __getitem__(self, key):
+        value = self._reader()
+        return value if key is None else np.asarray(value)[key]
+
+    def __setitem__(self, key, value):
+        if self._writer is None:
+            raise TypeError("this field is derived and read-only")
+        self._writer(key, value)
+
+    def to_numpy(self):
+        return np.asarray(self._reader())
+
+
+class Thermal2D:
+    """The ``MovingBodyThermal2D`` view of the shared device state."""
+
+    def __init__(self, sim):
+        self._sim = sim
+        self.steps = 0
+        self.time_s = 0.0
+        d = sim._t
+        bx, by = sim.config.ice_width, sim.config.ice_height
+        self.body_nx, self.body_ny = bx, by
+        self.body_solid_mass = DeviceField(d["body_mass"].view(bx, by))
+        self.body_sensible_energy = DeviceField(d["body_energy"].view(bx, by))
+        self.water_volume_m2 = DeviceField(d["water_v"].view(sim.nx, sim.ny))
+        self.water_sensible_energy = DeviceField(d["water_e"].view(sim.nx, sim.ny))
+        self.world_body_solid_fraction = DeviceField(d["coverage"].view(sim.nx, sim.ny))
+        self._initial_body_cell_mass = sim._g.cell_mass
+        self.boundary_heat_input = ControlValue(sim, "boundary_heat_input")
+        self.boundary_heat_input_j_m = self.boundary_heat_input
+        self.aperture_energy_correction_abs_j_m = ControlValue(sim, "correction_abs")
+        self.phase_aperture_volume_residual_m2 = DerivedValue(
+            lambda: sim._read_control().transfer[0] - sim._read_control().aperture[0])
+        self.phase_aperture_energy_residual_j_m = DerivedValue(
+            lambda: sim._read_control().transfer[1] - sim._read_control().aperture[1])
+        self.melt_injection_mass_residual_kg_m = DerivedValue(
+            lambda: sim._read_control().interval[1] - sim._read_control().interval[0])
+
+    def synchronize_water_aperture(self, *args):
+        self._sim._synchronize_water_aperture()
+
+    def mass_energy_totals(self):
+        """simulator.py:550-571."""
+        sim = self._sim
+        sim._call("reduce_totals")
+        t = sim._read_control().totals
+        initial = sim._g.cell_mass * self.body_nx * self.body_ny
+        melted = initial - t[0]
+        latent = sim._g.latent * melted
+        return MovingBodyThermalTotals(
+            initial_body_mass_kg_m=initial, solid_body_mass_kg_m=t[0], melted_mass_kg_m=melted,
+            water_mass_kg_m=t[1], total_mass_kg_m=t[0] + t[1], body_sensible_energy_j_m=t[2],
+            water_sensible_energy_j_m=t[3], latent_energy_j_m=latent, total_energy_j_m=t[2] + t[3] + latent)
+
+    def total_enthalpy_j_m(self, wall_mask=None):
+        return self.mass_energy_totals().total_energy_j_m
+
+
+class IceFlow2D:
+    """MI355X solver for the coupled falling-ice melting scenario."""
+
+    def __init__(self, config):
+        # simulator.py:1829-1843
+        if not isinstance(config, IceFlowConfig):
+            raise TypeError("config must be an IceFlowConfig")
+        if config.thermal is None:
+            raise ValueError("IceFlow2D requires thermal body-ALE coupling")
+        if config.ice_fixed or config.thermal.moving_body_scheme != "body_ale":
+            raise ValueError("IceFlow2D supports only freely moving body-ALE ice")
+        if config.rigid_boundary_scheme != "unified":
+            raise ValueError("IceFlow2D supports only unified cut-link boundaries")
+        if not config.well_balanced_hydrostatics:
+            raise ValueError("IceFlow2D requires well-balanced hydrostatics")
+        if config.thermal.water_buoyancy_model != "linear":
+            raise ValueError("IceFlow2D supports only the linear buoyancy model")
+        self.device = _require_device()
+        self._lib = native.lib()
+        self.config = config
+        self.nx, self.ny = int(config.nx), int(config.ny)
+        self.steps = 0
+        self.scales = LatticeScales.from_iceflow_config(config)
+        self._time_step_s = self.scales.dt_s
+        self._stream_handle = torch.cuda.current_stream(self.device)
+        self._setup_constants()
+        self._allocate()
+        self._make_fields()
+        self._thermal_lbm_steps_pending = 0
+        self._moving_melt_momentum_pending = False
+        self._initialize()
+
+    # ---------------------------------------------------------- setup
+    def _setup_constants(self):
+        """simulator.py:1849-1891 and the thermal constants of :143-330."""
+        c, th = self.config, self.config.thermal
+        props = th.properties
+        g = native.Grid2D()
+        self._g = g
+        g.nx, g.ny, g.boundary = self.nx, self.ny, c.boundary_cells
+        g.body_nx, g.body_ny = c.ice_width, c.ice_height
+        g.advection = int(th.advection_enabled)
+        for side in ("left", "right", "bottom", "top"):
+            boundary = getattr(th.boundaries, side)
+            setattr(g, side + "_kind", BOUNDARY_CODE[boundary.kind])
+            setattr(g, side + "_value", boundary.value)
+        u_lat = self.scales.reference_lattice_velocity
+        u_ref = self.scales.reference_velocity_m_s
+        c_u = self.scales.velocity_scale_m_s
+        self._water_density_lattice = 1.0
+        self._air_density_lattice = float(c.rho_air / c.rho_water)
+        self._gravity_lattice = (float(c.gravity[0] * c.dx / (c_u * c_u)), float(c.gravity[1] * c.dx / (c_u * c_u)))
+        g.rho_w, g.rho_a = 1.0, self._air_density_lattice
+        g.nu_w = float(c.viscosity_water * u_lat / (c.dx * u_ref))
+        g.nu_a = float(c.viscosity_air * u_lat / (c.dx * u_ref))
+        g.gx, g.gy = self._gravity_lattice
+        g.sigma = float(c.sigma * u_lat * u_lat / (u_ref * u_ref) / c.dx / c.rho_water)
+        g.width, g.mobility, g.smagorinsky = c.interface_width, c.mobility, c.smagorinsky_constant
+        tau = c.air_interface_relaxation_time
+        g.interface_tau = 0.5 if tau is None else float(tau)
+        g.cutoff = c.volume_projection_interface_cutoff
+        g.threshold = th.solid_liquid_threshold
+        g.dx = c.dx
+        g.expansion, g.t_ref = th.thermal_expansion_water_1_k, th.buoyancy_reference_temperature_c
+        g.linear_damping, g.angular_damping = c.linear_damping, c.angular_damping
+        g.initial_mass, g.initial_inertia = c.ice_mass_lattice, c.ice_inertia_lattice
+        g.t_melt = props.melting_temperature_c
+        g.cp_w, g.cp_i = props.specific_heat_water_j_kg_k, props.specific_heat_ice_j_kg_k
+        g.k_w, g.k_i, g.latent = props.conductivity_water_w_m_k, props.conductivity_ice_w_m_k, props.latent_heat_j_kg
+        g.rho_w_si = c.rho_water
+        g.area = c.dx * c.dx
+        g.velocity_scale = c_u
+        g.cell_mass = c.rho_ice * g.area
+        g.t_air = th.initial_air_temperature_c
+        self._body_half_width = 0.5 * float(c.ice_width)
+        self._body_half_height = 0.5 * float(c.ice_height)
+        self._initial_body_mass_lattice = float(c.ice_mass_lattice)
+        self._initial_body_inertia_lattice = float(c.ice_inertia_lattice)
+        cutoff = c.volume_projection_interface_cutoff
+        radius = 0.25 * float(c.interface_width) * math.log((1.0 - cutoff) / cutoff)
+        self._volume_projection_band_radius = max(2, int(math.ceil(radius)))
+        self._volume_projection_band_radius += self._volume_projection_band_radius % 2
+        alpha = max(g.k_w / (c.rho_water * g.cp_w), g.k_i / (c.rho_ice * g.cp_i))
+        self._maximum_diffusion_time_step_s = th.max_fourier_number * g.area / alpha
+
+    def _allocate(self):
+        nx, ny, N = self.nx, self.ny, self.nx * self.ny
+        M = self.config.ice_width * self.config.ice_height
+        dev = self.device
+        f32, f64, i8, i32 = torch.float32, torch.float64, torch.int8, torch.int32
+        shapes = {
+            "f": (9 * N, f32), "fbuf": (9 * N, f32), "h": (9 * N, f32), "hbuf": (9 * N, f32),
+            "phi": (N, f32), "u": (2 * N, f32), "a": (2 * N, f32), "p": (N, f32), "moving": (N, f32),
+            "p_ref": (ny, f32), "rho_ref": (ny, f32), "solid": (N, i8), "prev_solid": (N, i8), "eligible": (N, i8),
+            "coverage": (N, f32), "body_mass": (M, f64), "body_energy": (M, f64), "body_heat": (M, f64),
+            "melt_mass": (M, f64), "melt_sensible": (M, f64), "melt_target": (M, i32),
+            "water_v": (N, f64), "water_e": (N, f64), "target_e": (N, f64),
+            "flux_ex": ((nx + 1) * ny, f64), "flux_vx": ((nx + 1) * ny, f64),
+            "flux_ey": (nx * (ny + 1), f64), "flux_vy": (nx * (ny + 1), f64),
+        }
+        self._t = {name: torch.zeros(size, dtype=dtype, device=dev) for name, (size, dtype) in shapes.items()}
+        blocks = (max(N, M, (nx + 1) * (ny + 1)) + 255) // 256
+        capacity = blocks * 6 + 16
+        self._t["partials"] = torch.zeros(capacity, dtype=f64, device=dev)
+        self._control_t = torch.zeros(ctypes.sizeof(native.Control2D), dtype=torch.uint8, device=dev)
+        self._control_host = torch.zeros(ctypes.sizeof(native.Control2D), dtype=torch.uint8, pin_memory=True)
+        g = self._g
+        for name in native.GRID_POINTERS:
+            setattr(g, name, self._t[name].data_ptr())
+        g.partials_capacity = capacity
+        g.control = self._control_t.data_ptr()
+        self._gp = ctypes.pointer(g)
+
+    def _call(self, name, *args):
+        rc = getattr(self._lib, "icy2d_" + name)(self._gp, *args, ctypes.c_void_p(self._stream_handle.cuda_stream))
+        if rc != 0:
+            raise native.IcyError(rc, "icy2d_" + name)
+
+    def _read_control(self):
+        self._control_host.copy_(self._control_t, non_blocking=True)
+        self._stream_handle.synchronize()
+        return native.Control2D.from_buffer_copy(self._control_host.numpy().tobytes())
+
+    def _write_control(self, control):
+        raw = torch.frombuffer(bytearray(bytes(control)), dtype=torch.uint8)
+        self._control_t.copy_(raw)
+
+    def _make_fields(self):
+        nx, ny, t = self.nx, self.ny, self._t
+        self.momentum_populations = DeviceField(t["f"].view(9, nx, ny), components=9)
+        self.momentum_stream_buffer = DeviceField(t["fbuf"].view(9, nx, ny), components=9)
+        self.phase_populations = DeviceField(t["h"].view(9, nx, ny), components=9)
+        self.phase_stream_buffer = DeviceField(t["hbuf"].view(9, nx, ny), components=9)
+        self.water_phase = DeviceField(t["phi"].view(nx, ny))
+        self.momentum_velocity_lattice = DeviceField(t["u"].view(2, nx, ny), components=2)
+        self.fluid_acceleration_lattice = DeviceField(t["a"].view(2, nx, ny), components=2)
+        self.pressure_lattice = DeviceField(t["p"].view(nx, ny))
+        self.reference_pressure_by_row = DeviceField(t["p_ref"])
+        self.reference_density_by_row = DeviceField(t["rho_ref"])
+        self.hydrostatic_reference_pressure = DerivedValue(
+            lambda: np.broadcast_to(self.reference_pressure_by_row.to_numpy(), (nx, ny)))
+        self.hydrostatic_reference_density = DerivedValue(
+            lambda: np.broadcast_to(self.reference_density_by_row.to_numpy(), (nx, ny)))
+        self.solid_mask = DeviceField(t["solid"].view(nx, ny))
+        self.previous_solid_mask = DeviceField(t["prev_solid"].view(nx, ny))
+        self.wall_mask = DerivedValue(self._wall_numpy)
+        self.body_signed_distance_m = DerivedValue(self._signed_distance_numpy)
+        self.body_velocity = ControlValue(self, "velocity", 0, np.float32, 2)
+        self.body_angle = ControlValue(self, "angle", None, np.float32)
+        self.body_angular_velocity = ControlValue(self, "omega", None, np.float32)
+        self.body_reference_origin = ControlValue(self, "origin", 0, np.float32, 2)
+        self.body_mass_lattice = ControlValue(self, "mass")
+        self.body_inertia_lattice = ControlValue(self, "inertia")
+        self.body_local_center_of_mass = ControlValue(self, "local_com", 0, np.float64, 2)
+        self.body_center = DerivedValue(self._body_center_numpy, self._set_body_center_host)
+        self.body_initial_mass_lattice = DerivedValue(lambda: self._initial_body_mass_lattice)
+        self.body_active = DerivedValue(lambda: int(self._read_control().inertia > 0.0))
+        self.hydrodynamic_impulse = ControlValue(self, "impulse", 0, np.float64, 2)
+        self.hydrodynamic_torque = ControlValue(self, "impulse", 2)
+        self._body_contact_support_extrema = ControlValue(self, "support", 0, np.float32, 4)
+        self._body_contact_geometry_active = DerivedValue(
+            lambda: int(self._read_control().support[1] > self._read_control().support[0]))
+        self._body_contact_projection_changed = ControlValue(self, "projected", None, np.int32)
+        self.water_volume_current = ControlValue(self, "water", 1)
+        self.water_projection_derivative = ControlValue(self, "water", 2)
+        self.phase_change_initial_water_volume = ControlValue(self, "water", 0)
+        self.water_volume_target = DerivedValue(self._water_volume_target, self._set_water_volume_target_host)
+        self.maximum_water_outflow_rate_lattice = ControlValue(self, "max_outflow", None, np.float32)
+        self.cumulative_melted_momentum_lattice = ControlValue(self, "melted", 0, np.float64, 2)
+        self.cumulative_melted_angular_momentum_lattice = ControlValue(self, "melted", 2)
+        self.cumulative_melted_mass_lattice = DerivedValue(
+            lambda: self._initial_body_mass_lattice - self._read_control().mass)
+        self.cumulative_fluid_melt_carrier_momentum_lattice = ControlValue(self, "carrier", 0, np.float64, 2)
+        self.cumulative_fluid_melt_correction_momentum_lattice = ControlValue(self, "correction", 0, np.float64, 2)
+        self.cumulative_fluid_melt_momentum_lattice = DerivedValue(
+            lambda: np.array(self._read_control().carrier[:2]) + np.array(self._read_control().correction[:2]))
+        self.melt_momentum_residual_lattice = DerivedValue(
+            lambda: self.cumulative_fluid_melt_momentum_lattice[None] - np.array(self._read_control().melted[:2]))
+        self.cumulative_fluid_melt_angular_momentum_lattice = DerivedValue(
+            lambda: self._read_control().carrier[2] + self._read_control().correction[2])
+        self.melt_angular_momentum_residual_lattice = DerivedValue(
+            lambda: self._read_control().carrier[2] + self._read_control().correction[2]
+            - self._read_control().melted[2])
+        self.thermal = Thermal2D(self)
+        self._moving_body_fraction = self.thermal.world_body_solid_fraction
+        self.physical_velocity_lattice = DerivedValue(self._physical_velocity_numpy)
+
+    # ------------------------------------------------------ views
+    def _wall_numpy(self):
+        b = self.config.boundary_cells
+        x, y = np.ogrid[: self.nx, : self.ny]
+        return ((x < b) | (x >= self.nx - b) | (y < b) | (y >= self.ny - b)).astype(np.int8)
+
+    def _body_center_numpy(self):
+        """simulator.py:4846-4856."""
+        control = self._read_control()
+        angle = np.float32(control.angle)
+        c, s = np.cos(angle), np.sin(angle)
+        lx, ly = control.local_com[0], control.local_com[1]
+        offset = np.array([c * lx - s * ly, s * lx + c * ly])
+        return (np.array(control.origin[:], dtype=np.float64) + offset).astype(np.float32)
+
+    def _set_body_center_host(self, index, value):
+        """simulator.py:4858-4869."""
+        if index is not None:
+            raise IndexError("body_center is a scalar vector; use [None]")
+        control = self._read_control()
+        angle = np.float32(control.angle)
+        c, s = np.cos(angle), np.sin(angle)
+        lx, ly = control.local_com[0], control.local_com[1]
+        origin = np.asarray(value, dtype=np.float64) - np.array([c * lx - s * ly, s * lx + c * ly])
+        control.origin[0], control.origin[1] = float(origin[0]), float(origin[1])
+        self._write_control(control)
+
+    def _signed_distance_numpy(self):
+        """simulator.py:4605-4633."""
+        control = self._read_control()
+        origin = np.array(control.origin[:], dtype=np.float32)
+        angle = np.float32(control.angle)
+        c, s = np.cos(angle), np.sin(angle)
+        x, y = np.meshgrid(np.arange(self.nx, dtype=np.float32) + 0.5 - origin[0],
+                           np.arange(self.ny, dtype=np.float32) + 0.5 - origin[1], indexing="ij")
+        lx, ly = c * x + s * y, -s * x + c * y
+        dx, dy = np.abs(lx) - self._body_half_width, np.abs(ly) - self._body_half_height
+        distance = np.hypot(np.maximum(dx, 0.0), np.maximum(dy, 0.0)) + np.minimum(np.maximum(dx, dy), 0.0)
+        inside = ((lx >= -self._body_half_width) & (lx < self._body_half_width)
+                  & (ly >= -self._body_half_height) & (ly < self._body_half_height))
+        distance[inside] = self.config.thermal.solid_liquid_threshold - self._moving_body_fraction.to_numpy()[inside]
+        return (distance * self.config.dx).astype(np.float32)
+
+    def _physical_velocity_numpy(self):
+        v = self.momentum_velocity_lattice.to_numpy() + np.float32(0.5) * self.fluid_acceleration_lattice.to_numpy()
+        v[(self._wall_numpy() != 0) | (self.solid_mask.to_numpy() != 0)] = 0.0
+        return v
+
+    def _water_volume_target(self):
+        """simulator.py:4922-4927."""
+        control = self._read_control()
+        return control.water[0] + self._initial_body_mass_lattice - control.mass
+
+    def _set_water_volume_target_host(self, index, value):
+        if index is not None:
+            raise IndexError("water_volume_target is scalar; use [None]")
+        control = self._read_control()
+        control.water[0] = float(value) - self._initial_body_mass_lattice + control.mass
+        self._write_control(control)
+
+    @property
+    def physical_time_s(self):
+        return self.steps * self._time_step_s
+
+    def sample_thermal_fields(self):
+        """Host snapshot of temperature, enthalpy and liquid fraction (simulator.py:1694-1765)."""
+        g, th = self._g, self.config.thermal
+        volume = self.thermal.water_volume_m2.to_numpy()
+        energy = self.thermal.water_sensible_energy.to_numpy()
+        mass = self.thermal.body_solid_mass.to_numpy()
+        body_energy = self.thermal.body_sensible_energy.to_numpy()
+        wet = volume > 1.0e-30
+        temperature = np.full(volume.shape, th.initial_air_temperature_c)
+        temperature[wet] = g.t_melt + energy[wet] / (g.rho_w_si * g.cp_w * volume[wet])
+        enthalpy = np.where(wet, (g.rho_w_si * g.latent * volume + energy) / g.area, 0.0)
+        liquid = np.ones(volume.shape, dtype=np.float32)
+        material = wet.copy()
+        control = self._read_control()
+        origin = np.array(control.origin[:], dtype=np.float32)
+        angle = np.float32(control.angle)
+        c, s = np.cos(angle), np.sin(angle)
+        x, y = np.meshgrid(np.arange(self.nx, dtype=np.float32) + 0.5 - origin[0],
+                           np.arange(self.ny, dtype=np.float32) + 0.5 - origin[1], indexing="ij")
+        bi = np.floor(c * x + s * y + 0.5 * g.body_nx).astype(np.int32)
+        bj = np.floor(-s * x + c * y + 0.5 * g.body_ny).astype(np.int32)
+        ice = ((self.thermal.world_body_solid_fraction.to_numpy() >= th.solid_liquid_threshold)
+               & (bi >= 0) & (bi < g.body_nx) & (bj >= 0) & (bj < g.body_ny))
+        sel_mass, sel_energy = mass[bi[ice], bj[ice]], body_energy[bi[ice], bj[ice]]
+        body_t = np.full(sel_mass.shape, g.t_melt)
+        live = sel_mass > 1.0e-30
+        body_t[live] += sel_energy[live] / (sel_mass[live] * g.cp_i)
+        temperature[ice] = np.minimum(g.t_melt, body_t)
+        enthalpy[ice] = (sel_energy + np.maximum(0.0, g.cell_mass - sel_mass) * g.latent) / g.area
+        fraction = np.where(mass > 0.0, np.maximum(np.clip(mass, 0.0, g.cell_mass) / g.cell_mass, 1.0e-30), 0.0)
+        liquid[ice] = 1.0 - fraction.astype(np.float32)[bi[ice], bj[ice]]
+        material[ice] = True
+        wall = self._wall_numpy() != 0
+        temperature[wall], enthalpy[wall], liquid[wall], material[wall] = g.t_melt, 0.0, 0.0, False
+        return {"temperature_c": temperature, "enthalpy_j_m3": enthalpy, "liquid_fraction": liquid,
+                "phase_change_material": material.astype(np.int8)}
+
+    @property
+    def temperature(self):
+        return DerivedValue(lambda: self.sample_thermal_fields()["temperature_c"])
+
+    @property
+    def liquid_fraction(self):
+        return DerivedValue(lambda: self.sample_thermal_fields()["liquid_fraction"])
+
+    @property
+    def thermal_enthalpy(self):
+        return DerivedValue(lambda: self.sample_thermal_fields()["enthalpy_j_m3"])
+
+    # ---------------------------------------------------------- init
+    def _initialize(self):
+        """simulator.py:2159-2192."""
+        c = self.config
+        control = native.Control2D()
+        control.velocity[0], control.velocity[1] = c.ice_initial_velocity
+        control.angle, control.omega = c.ice_initial_angle, c.ice_initial_angular_velocity
+        control.mass, control.inertia = c.ice_mass_lattice, c.ice_inertia_lattice
+        control.origin[0], control.origin[1] = c.ice_initial_center
+        self._write_control(control)
+        self._call("initialize_body_state", c.thermal.initial_ice_temperature_c)
+        self._solve_contact_and_rasterize()
+        self._call("update_solid_mask", 1)
+        self._call("initialize_fluid", c.water_width, c.water_height)
+        self._call("initialize_water_volume")
+        self._warm_start_phase(c.phase_warmup_steps)
+        self._call("recover_initial_moments")
+        self._correct_water_volume()
+        self._call("initialize_body_state", c.thermal.initial_ice_temperature_c)
+        self._call("initialize_water_state", c.thermal.initial_water_temperature_c)
+        self.thermal.steps, self.thermal.time_s = 0, 0.0
+        self._update_moving_body_mass_properties(initialize=True)
+        self._build_hydrostatic_reference()
+        self._call("hydrostatic_equilibrium")
+
+    def _warm_start_phase(self, steps):
+        """simulator.py:3827-3831."""
+        for _ in range(int(steps)):
+            self._call("collide", 0)
+            self._call("stream", 1)
+
+    def _build_hydrostatic_reference(self):
+        """simulator.py:2626-2662 (host FP64, from the device phase and masks)."""
+        c, g = self.config, self._g
+        phase = np.clip(self.water_phase.to_numpy().astype(np.float64), 0.0, 1.0)
+        active = (self._wall_numpy() == 0) & (self.solid_mask.to_numpy() == 0)
+        count = np.count_nonzero(active, axis=0)
+        profile = (np.arange(self.ny) < c.water_height).astype(np.float64)
+        np.divide(np.where(active, phase, 0.0).sum(axis=0), count, out=profile, where=count > 0)
+        density = g.rho_a + (g.rho_w - g.rho_a) * profile
+        pressure = np.zeros(self.ny)
+        top, gy = c.water_height, self._gravity_lattice[1]
+        pressure[top - 1] = -0.5 * gy * density[top - 1]
+        for row in range(top - 2, -1, -1):
+            pressure[row] = pressure[row + 1] - 0.5 * gy * (density[row] + density[row + 1])
+        pressure[top] = 0.5 * gy * density[top]
+        for row in range(top + 1, self.ny):
+            pressure[row] = pressure[row - 1] + 0.5 * gy * (density[row - 1] + density[row])
+        if not (np.isfinite(density).all() and np.isfinite(pressure).all()):
+            raise RuntimeError("hydrostatic reference construction produced non-finite values")
+        self.reference_density_by_row.from_numpy(density.astype(np.float32))
+        self.reference_pressure_by_row.from_numpy(pressure.astype(np.float32))
+
+    def _update_moving_body_mass_properties(self, *, initialize=False):
+        """simulator.py:2729-2733."""
+        self._call("reduce_mass_properties")
+        self._call("apply_mass_properties", 1 if initialize else 0)
+
+    def _solve_contact_and_rasterize(self, *, resolve_contact=True):
+        """simulator.py:4082-4123; the projected re-pass is decided on the device."""
+        self._call("body_fraction", 0)
+        if resolve_contact:
+            self._call("contact_support", 1, 0)
+            self._call("project_inside")
+            self._call("body_fraction", 1)
+            self._call("contact_support", 1, 1)
+        else:
+            self._call("contact_support", 0, 0)
+        self._call("commit_raster")
+
+    def _prepare_moving_body_contact_support(self):
+        """simulator.py:3901-3912."""
+        self._call("body_fraction", 0)
+        self._call("contact_support", 1, 0)
+
+    # ---------------------------------------------------- reference kernels
+    def _collide_momentum(self):
+        self._call("collide", 1)
+
+    def _prepare_test_cut_links(self):
+        self._call("prepare_cut_links")
+
+    def _stream(self):
+        self._call("stream", 0)
+
+    def _update_pressure(self):
+        self._call("update_pressure")
+
+    def _integrate_rigid_ice(self):
+        self._call("integrate_rigid")
+
+    def _update_solid_mask(self):
+        self._call("update_solid_mask", 0)
+
+    def _refill_changed_nodes(self):
+        self._call("refill")
+
+    # ---------------------------------------------------------- step
+    def step(self, num_steps=1):
+        """simulator.py:2194-2227."""
+        if isinstance(num_steps, bool) or not isinstance(num_steps, (int, np.integer)) or num_steps < 0:
+            raise ValueError("num_steps must be a non-negative integer")
+        for _ in range(int(num_steps)):
+            self._call("collide", 1)
+            self._call("stream", 0)
+            self._call("update_pressure")
+            self._call("integrate_rigid")
+            self._solve_contact_and_rasterize()
+            self._call("update_solid_mask", 0)
+            self._call("refill")
+            self._advance_moving_thermal_fast(target_step=self.steps + 1)
+            self._thermal_lbm_steps_pending += 1
+            if self._thermal_lbm_steps_pending >= self.config.thermal.update_interval_lbm_steps:
+                self._advance_moving_thermal_slow(self._thermal_lbm_steps_pending)
+                self._thermal_lbm_steps_pending = 0
+            self._correct_water_volume()
+            self._finalize_melt_momentum_coupling()
+            self._synchronize_water_aperture()
+            self.steps += 1
+
+    def _substeps(self, load, process, dt, outflow=None):
+        """simulator.py:342-401."""
+        count = max(1, int(math.ceil(load - 1.0e-14)))
+        limit = self.config.thermal.max_substeps_per_update
+        if count > limit:
+            detail = f", maximum_outflow_rate_lattice={outflow!r}" if outflow is not None else ""
+            raise RuntimeError(
+                f"thermal stability requires {count} {process} substeps for one coupling update, exceeding "
+                f"max_substeps_per_update={limit}; time_step_s={dt:.9g}{detail}. Reduce the thermal update "
+                "interval or the lattice velocity, and inspect the LBM state before raising this safety limit.")
+        return count
+
+    def _advance_moving_thermal_fast(self, *, target_step):
+        """simulator.py:2233-2247 and :418-453."""
+        self._call("measure_outflow")
+        speed = float(self._read_control().max_outflow)
+        th = self.config.thermal
+        if not speed >= 0.0:
+            raise ValueError("maximum_outflow_rate_lattice must be finite")
+        dt = self._time_step_s
+        load = 0.0
+        if th.advection_enabled and speed != 0.0:
+            limit = th.max_courant_number * dt / speed
+            if math.isfinite(limit):
+                load = dt / limit
+        substeps = self._substeps(load, "advection", dt, speed)
+        self._synchronize_water_aperture()
+        sub = dt / substeps
+        for k in range(substeps):
+            self._call("advection_flux", 0)
+            self._call("advection_flux", 1)
+            self._call("update_water", sub, 1)
+            if k + 1 < substeps:
+                self._synchronize_water_aperture()
+
+    def _advance_moving_thermal_slow(self, lbm_steps):
+        """simulator.py:2249-2286 and :455-509."""
+        count = int(lbm_steps)
+        if count <= 0:
+            raise ValueError("lbm_steps must be positive")
+        if self._moving_melt_momentum_pending:
+            raise RuntimeError("a moving thermal momentum interval is still pending phase projection")
+        self._synchronize_water_aperture()
+        self._call("snapshot_melt_momentum")
+        dt = count * self._time_step_s
+        substeps = self._substeps(dt / self._maximum_diffusion_time_step_s, "diffusion", dt)
+        sub = dt / substeps
+        self._call("reset_interval_sources")
+        for k in range(substeps):
+            self._call("body_conduction", sub)
+            self._call("conduction_flux", 0)
+            self._call("conduction_flux", 1)
+            self._call("update_water", sub, 0)
+            self._call("accumulate_boundary_heat", sub)
+            self._call("exchange_heat", sub)
+            self._call("phase_change")
+            self._call("freeze_eligibility")
+            self._call("inject_melt")
+            self._distribute_unassigned_melt()
+            self._synchronize_water_aperture()
+            if k + 1 < substeps:
+                self._solve_contact_and_rasterize(resolve_contact=False)
+        self._solve_contact_and_rasterize(resolve_contact=False)
+        self.thermal.steps += substeps
+        self.thermal.time_s += dt
+        self._update_moving_body_mass_properties()
+        self._call("update_solid_mask", 0)
+        self._call("refill")
+        self._moving_melt_momentum_pending = True
+
+    def _distribute_unassigned_melt(self):
+        """simulator.py:1542-1563."""
+        if float(self._read_control().unassigned[0]) <= 0.0:
+            return
+        self._call("measure_fallback")
+        control = self._read_control()
+        if control.fallback[0] > 1.0e-30:
+            self._call("apply_fallback", 1)
+        elif control.fallback[1] > 1.0e-30:
+            self._call("apply_fallback", 0)
+        else:
+            raise RuntimeError("melt water has no legal LBM water cell for conservative mass and energy injection")
+
+    def _synchronize_water_aperture(self):
+        """simulator.py:511-548."""
+        self._call("measure_aperture")
+        control = self._read_control()
+        total, full = control.aperture[0], control.aperture[3]
+        tolerance = 1.0e-12 * max(self._g.area, abs(total), abs(full))
+        if total < -tolerance:
+            raise RuntimeError(f"thermal water advection produced a negative total volume: {total:.17g} m^2")
+        if total > full + tolerance:
+            raise RuntimeError("thermal water volume exceeds the current fluid aperture capacity: "
+                               f"volume={total:.17g} m^2, capacity={full:.17g} m^2")
+        self._call("build_aperture_targets")
+        self._call("apply_aperture_transfer")
+        self._call("finish_aperture_transfer")
+
+    def _finalize_melt_momentum_coupling(self):
+        """simulator.py:2989-3012."""
+        if not self._moving_melt_momentum_pending:
+            return
+        self._call("reduce_provisional_momentum")
+        self._call("melt_weight_geometry")
+        status = int(self._read_control().melt_status)
+        if status == -1:
+            raise RuntimeError("melt momentum has no legal wet LBM cell for conservative linear/angular-momentum closure")
+        if status == -2:
+            raise RuntimeError("melt momentum correction requires two distinct wet cells")
+        if status == 1:
+            self._call("apply_melt_correction")
+        self._call("reduce_final_momentum")
+        for _ in range(2):
+            self._call("local_melt_correction")
+            self._call("reduce_final_momentum")
+        self._call("finish_melt_momentum")
+        self._moving_melt_momentum_pending = False
+
+    def _correct_water_volume(self):
+        """simulator.py:4494-4560."""
+        c = self.config
+        self._call("clip_phase", MAX_BULK_PHASE_L1)
+        control = self._read_control()
+        current = control.water[1]
+        target = control.water[0] + self._initial_body_mass_lattice - control.mass
+        tolerance = c.volume_projection_tolerance * max(1.0, abs(target))
+        initial_error = target - current
+        if abs(initial_error) <= tolerance:
+            return
+        self._call("seed_projection")
+        for k in range(self._volume_projection_band_radius):
+            self._call("dilate_projection", 1 if k % 2 == 0 else 0)
+        limit = 4.0 * c.volume_projection_max_shift / c.interface_width
+
+        def evaluate(lam):
+            self._call("evaluate_projection", lam)
+            water = self._read_control().water
+            return water[1], water[2]
+
+        lower_mass, upper_mass = evaluate(-limit)[0], evaluate(limit)[0]
+        if abs(upper_mass - lower_mass) <= tolerance:
+            raise RuntimeError("water-volume projection is infeasible: no adjustable phi=0.5-connected interface is available")
+        if target < lower_mass - tolerance or target > upper_mass + tolerance:
+            raise RuntimeError("water-volume projection exceeds the permitted interface shift: "
+                               f"target={target:.17g}, reachable=[{lower_mass:.17g}, {upper_mass:.17g}]")
+        lo, hi, lam, best_lam, best = -limit, limit, 0.0, 0.0, abs(initial_error)
+        for _ in range(c.volume_projection_max_iterations):
+            mass, derivative = evaluate(lam)
+            residual = mass - target
+            if abs(residual) < best:
+                best, best_lam = abs(residual), lam
+            if abs(residual) <= tolerance:
+                best_lam = lam
+                break
+            if residual < 0.0:
+                lo = lam
+            else:
+                hi = lam
+            proposal = math.nan
+            if derivative > 0.0 and math.isfinite(derivative):
+                proposal = lam - residual / derivative
+            if not math.isfinite(proposal) or not lo < proposal < hi:
+                proposal = 0.5 * (lo + hi)
+            lam = proposal
+        self._call("apply_projection", best_lam)
+        current = self._read_control().water[1]
+        for _ in range(8):
+            residual = target - current
+            if abs(residual) <= tolerance:
+                break
+            self._call("projection_residual_weight", residual)
+            weight = self._read_control().water[2]
+            if not math.isfinite(weight) or weight <= 0.0:
+                break
+            self._call("apply_projection_residual", residual, weight)
+            current = self._read_control().water[1]
+        error = abs(target - current)
+        if error > tolerance:
+            raise RuntimeError("water-volume projection cannot close its endpoint active set: "
+                               f"residual={error:.6e}, tolerance={tolerance:.6e}")
+
+    def state(self):
+        """Logical arrays in the reference's layouts (for tests and snapshots)."""
+        control = self._read_control()
+        return {
+            "momentum_populations": self.momentum_populations.to_numpy(),
+            "phase_populations": self.phase_populations.to_numpy(),
+            "water_phase": self.water_phase.to_numpy(),
+            "momentum_velocity_lattice": self.momentum_velocity_lattice.to_numpy(),
+            "fluid_acceleration_lattice": self.fluid_acceleration_lattice.to_numpy(),
+            "pressure_lattice": self.pressure_lattice.to_numpy(),
+            "solid_mask": self.solid_mask.to_numpy(),
+            "coverage": self.thermal.world_body_solid_fraction.to_numpy(),
+            "water_volume_m2": self.thermal.water_volume_m2.to_numpy(),
+            "water_sensible_energy": self.thermal.water_sensible_energy.to_numpy(),
+            "body_solid_mass": self.thermal.body_solid_mass.to_numpy(),
+            "body_sensible_energy": self.thermal.body_sensible_energy.to_numpy(),
+            "body_center": self._body_center_numpy(),
+            "body_velocity": np.array(control.velocity[:], dtype=np.float32),
+            "body_angle": np.float32(control.angle),
+            "body_angular_velocity": np.float32(control.omega),
+            "body_mass_lattice": float(control.mass),
+            "body_inertia_lattice": float(control.inertia),
+        }
