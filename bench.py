@@ -13,7 +13,10 @@ The JSON line adds:
                 solver stream, a profiled pass after the timed region),
                 against the 8 TB/s MI355X HBM peak;
   cpu_baseline  the FP64 CPU oracle (sequential C, one core) timed on one
-                step of the same workload after its initialisation.
+                step of the same workload after its initialisation;
+  d2q9          the coupled D2Q9 step on the 1024x2048 falling-ice case
+                (FP32 populations, FP64 thermal), timed after the headline,
+                with its own CPU baseline (the 2D oracle on the 100x200 case).
 """
 
 import argparse
@@ -50,7 +53,61 @@ def parse():
     p.add_argument("--seed", type=int, default=2026)
     p.add_argument("--size", type=int, nargs=3, default=(256, 128, 128))
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--d2q9-steps", type=int, default=200,
+                   help="steps of the 1024x2048 D2Q9 case timed after the headline (0 skips it)")
     return p.parse_args()
+
+
+D2Q9_STEP_BYTES = 185  # two FP32 D2Q9 distributions, phi, FP64 water V and E, mask per cell-update
+
+
+def d2q9_measurement(args):
+    """The coupled D2Q9 step on the 1024x2048 falling-ice case (one GPU)."""
+    import torch
+
+    from projecticy_amd.iceflow2d import IceFlow2D
+    from projecticy_amd.iceflow2d.workloads import scaled_case
+
+    config = scaled_case(seed=args.seed)
+    cells = config.nx * config.ny
+    t0 = time.perf_counter()
+    sim = IceFlow2D(config)
+    torch.cuda.synchronize()
+    init_s = time.perf_counter() - t0
+    sim.step(10)
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    sim.step(args.d2q9_steps)
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - start
+    mlups = cells * args.d2q9_steps / seconds / 1e6
+    print(f"[bench] D2Q9 {config.nx}x{config.ny}: init {init_s:.1f} s, {args.d2q9_steps} steps in {seconds:.3f} s "
+          f"= {mlups:.1f} MLUPS", file=sys.stderr, flush=True)
+    result = {
+        "workload": f"D2Q9 falling ice {config.nx}x{config.ny}, {config.ice_width}-cell ice at "
+                    f"{math.degrees(config.ice_initial_angle):.3f} deg, 90 degC bath, full coupled step",
+        "value": round(mlups, 2), "unit": "MLUPS", "steps": args.d2q9_steps,
+        "ms_per_step": round(seconds / args.d2q9_steps * 1e3, 4), "dtype": "fp32 populations, fp64 thermal",
+        "step_hbm_roofline_frac": round(mlups * 1e6 * D2Q9_STEP_BYTES / (HBM_PEAK_GBS * 1e9), 4),
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = d2q9_cpu_baseline()
+    return result
+
+
+def d2q9_cpu_baseline():
+    """FP64 oracle (sequential C) on 20 steps of the 100x200 default case."""
+    from oracle.iceflow2d_oracle import Oracle2D
+    from projecticy_amd.iceflow2d.workloads import default_case
+
+    config = default_case(phase_warmup_steps=0)
+    ref = Oracle2D(config)
+    start = time.perf_counter()
+    ref.step(20)
+    seconds = time.perf_counter() - start
+    return {"value": config.nx * config.ny * 20 / seconds / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+            "sample": f"20 coupled steps of the 100x200 default case (no phase warm-up), {seconds:.2f} s, "
+                      "FP64 sequential C oracle"}
 
 
 def cpu_baseline(config):
@@ -152,6 +209,10 @@ def main():
         "roofline": roofline,
         "cpu_baseline": None if args.no_cpu_baseline else cpu_baseline(config),
     }
+    del sim
+    torch.cuda.empty_cache()
+    if args.d2q9_steps > 0:
+        result["d2q9"] = d2q9_measurement(args)
     print(json.dumps(result), flush=True)
 
 

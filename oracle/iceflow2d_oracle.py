@@ -235,6 +235,7 @@ class OracleThermal:
         self.world_body_solid_fraction = ArrayField(a["coverage"], shape)
         self._initial_body_cell_mass = oracle.s.cell_mass
         self.boundary_heat_input = StructValue(oracle.s, "boundary_heat_input")
+        self.boundary_heat_input_j_m = self.boundary_heat_input
         self.time_s = 0.0
         self.steps = 0
 
@@ -242,6 +243,27 @@ class OracleThermal:
         from types import SimpleNamespace
 
         return SimpleNamespace(**self._oracle.mass_energy_totals())
+
+    def total_enthalpy_j_m(self, wall_mask=None):
+        return self._oracle.mass_energy_totals()["total_energy_j_m"]
+
+    def synchronize_water_aperture(self, *args):
+        self._oracle.synchronize_water_aperture()
+
+    @property
+    def melt_injection_mass_residual_kg_m(self):
+        s = self._oracle.s
+        return Derived(lambda: s.interval_water_melt - s.interval_body_melt)
+
+    @property
+    def phase_aperture_volume_residual_m2(self):
+        s = self._oracle.s
+        return Derived(lambda: s.volume_after - s.volume_before)
+
+    @property
+    def phase_aperture_energy_residual_j_m(self):
+        s = self._oracle.s
+        return Derived(lambda: s.energy_after - s.energy_before)
 
 
 class Oracle2D:
@@ -369,6 +391,18 @@ class Oracle2D:
         self._water_density_lattice, self._air_density_lattice = s.rho_w, s.rho_a
         self._body_half_width = 0.5 * self.config.ice_width
         self._body_half_height = 0.5 * self.config.ice_height
+        self.phase_change_initial_water_volume = StructValue(s, "initial_water_volume")
+        self.cumulative_melted_momentum_lattice = StructValue(s, "cumulative_melted_momentum", 2)
+        self.cumulative_melted_angular_momentum_lattice = StructValue(s, "cumulative_melted_angular")
+        self.cumulative_fluid_melt_momentum_lattice = Derived(
+            lambda: np.array(s.carrier_momentum[:]) + np.array(s.correction_momentum[:]))
+        self.melt_momentum_residual_lattice = Derived(
+            lambda: np.array(s.carrier_momentum[:]) + np.array(s.correction_momentum[:])
+            - np.array(s.cumulative_melted_momentum[:]))
+        self.cumulative_fluid_melt_angular_momentum_lattice = Derived(lambda: s.carrier_angular + s.correction_angular)
+        self.melt_angular_momentum_residual_lattice = Derived(
+            lambda: s.carrier_angular + s.correction_angular - s.cumulative_melted_angular)
+        self.temperature = Derived(lambda: self.sample_thermal_fields()["temperature_c"])
         self.thermal = OracleThermal(self)
 
     def wall_mask_array(self):
@@ -559,6 +593,8 @@ class Oracle2D:
             if k + 1 < substeps:
                 self._k("contact_and_rasterize", 0)
         self._k("contact_and_rasterize", 0)
+        self.thermal.steps += substeps
+        self.thermal.time_s += dt
         self._update_mass_properties()
         self._k("update_solid_mask", 0)
         self._k("refill")
@@ -711,6 +747,31 @@ class Oracle2D:
         lx, ly = s.local_com[0], s.local_com[1]
         off = np.array([c * lx - sn * ly, sn * lx + c * ly])
         return (np.array(s.origin[:], dtype=np.float64) + off).astype(np.float32)
+
+    def sample_thermal_fields(self):
+        """simulator.py:1694-1765 (temperature only: water, ice and walls)."""
+        s, c = self.s, self.config
+        volume = self.arrays["water_v"].reshape(self.nx, self.ny)
+        energy = self.arrays["water_e"].reshape(self.nx, self.ny)
+        wet = volume > 1.0e-30
+        temperature = np.full(volume.shape, c.thermal.initial_air_temperature_c)
+        temperature[wet] = s.t_melt + energy[wet] / (s.rho_w_si * s.cp_w * volume[wet])
+        angle = np.float32(s.angle)
+        cs, sn = np.cos(angle), np.sin(angle)
+        x, y = np.meshgrid(np.arange(self.nx, dtype=np.float32) + 0.5 - np.float32(s.origin[0]),
+                           np.arange(self.ny, dtype=np.float32) + 0.5 - np.float32(s.origin[1]), indexing="ij")
+        bi = np.floor(cs * x + sn * y + 0.5 * c.ice_width).astype(np.int32)
+        bj = np.floor(-sn * x + cs * y + 0.5 * c.ice_height).astype(np.int32)
+        ice = ((self.arrays["coverage"].reshape(self.nx, self.ny) >= c.thermal.solid_liquid_threshold)
+               & (bi >= 0) & (bi < c.ice_width) & (bj >= 0) & (bj < c.ice_height))
+        mass = self.arrays["body_mass"].reshape(c.ice_width, c.ice_height)[bi[ice], bj[ice]]
+        body_e = self.arrays["body_energy"].reshape(c.ice_width, c.ice_height)[bi[ice], bj[ice]]
+        body_t = np.full(mass.shape, s.t_melt)
+        live = mass > 1.0e-30
+        body_t[live] += body_e[live] / (mass[live] * s.cp_i)
+        temperature[ice] = np.minimum(s.t_melt, body_t)
+        temperature[self.wall_mask_array() != 0] = s.t_melt
+        return {"temperature_c": temperature}
 
     def mass_energy_totals(self):
         """simulator.py:550-571: (solid mass, water mass, body sensible, water sensible, latent) per metre."""

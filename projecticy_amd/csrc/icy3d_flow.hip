@@ -271,25 +271,6 @@ __device__ __forceinline__ void store_gradient(const icy3d_grid& g, int n, float
   g.grad[2 * N + n] = gz;
 }
 
-// gradients.py:359-373 for flag 0 (all links fluid) and flag 1 (tank mirror).
-template <bool TANK>
-__device__ __forceinline__ void regular_gradient(const icy3d_grid& g, const Cell& c) {
-  const float centre = g.phi[c.n];
-  float rx = 0.0f, ry = 0.0f, rz = 0.0f;
-#pragma unroll
-  for (int q = 0; q < 27; ++q) {
-    if (q == 13) continue;
-    int i = c.i + cx(q), j = c.j + cy(q), k = c.k + cz(q);
-    if (TANK) tank_mirror(g, i, j, k);
-    float d = g.phi[lin(g, i, j, k)] - centre;
-    const float w = w27(q);
-    if (cx(q)) rx += w * (float)cx(q) * d;
-    if (cy(q)) ry += w * (float)cy(q) * d;
-    if (cz(q)) rz += w * (float)cz(q) * d;
-  }
-  store_gradient(g, c.n, 3.0f * rx, 3.0f * ry, 3.0f * rz);
-}
-
 // gradients.py:116-145: general gradient with wall and ice mirror samples.
 __device__ void general_gradient(const icy3d_grid& g, const Cell& c) {
   float rx = 0.0f, ry = 0.0f, rz = 0.0f;
@@ -326,17 +307,89 @@ __device__ void general_gradient(const icy3d_grid& g, const Cell& c) {
   store_gradient(g, c.n, 3.0f * rx, 3.0f * ry, 3.0f * rz);
 }
 
-__global__ void k_gradients_regular(icy3d_grid g, Box box) {
-  Cell c;
-  if (!box_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
-  int flag = g.flag[c.n];
-  if (flag == 0)
-    regular_gradient<false>(g, c);
-  else if (flag == 1)
-    regular_gradient<true>(g, c);
-  else if (flag < 0)
-    store_gradient(g, c.n, 0.0f, 0.0f, 0.0f);
+// Tiled regular gradient (gradients.py:359-373, flags 0 and 1).  The D3Q27
+// weights factor as w(c) = w1(cx) w1(cy) w1(cz) with w1(0) = 2/3 and
+// w1(+-1) = 1/6, so the isotropic gradient 3 sum_q w_q c_q phi(x + c_q) is
+// separable: d/dx = 3 D_x S_y S_z phi with D u = (u+ - u-) / 6 and
+// S u = (u- + 4 u0 + u+) / 6 (the centre value cancels since sum w c = 0).
+// A block sweeps x over a (kGradJ x kGradK) column: each phi slab (one-cell
+// halo, wall positions read through the tank mirror so that flag-1 cells see
+// their mirrored neighbours) is staged once in LDS, every thread forms
+// P = S_y S_z, Qy = D_y S_z and Qz = S_y D_z for its column, and the x
+// combination runs in registers over three consecutive slabs.
+constexpr int kGradJ = 4, kGradK = 64, kGradX = 16;
+constexpr int kGradHJ = kGradJ + 2, kGradHK = kGradK + 2, kGradSlab = kGradHJ * kGradHK;
+constexpr int kGradRounds = (kGradSlab + kGradJ * kGradK - 1) / (kGradJ * kGradK);
+
+__device__ __forceinline__ float staged_phi(const icy3d_grid& g, int i, int j, int k) {
+  tank_mirror(g, i, j, k);
+  i = min(max(i, 0), g.nx - 1);
+  j = min(max(j, 0), g.ny - 1);
+  k = min(max(k, 0), g.nz - 1);
+  return g.phi[lin(g, i, j, k)];
 }
+
+__global__ void __launch_bounds__(kGradJ* kGradK) k_gradients_tiled(icy3d_grid g, Box box) {
+  __shared__ float sp[kGradSlab];
+  const int tk = threadIdx.x % kGradK, tj = threadIdx.x / kGradK;
+  const int k0 = box.lo[2] + blockIdx.x * kGradK, j0 = box.lo[1] + blockIdx.y * kGradJ;
+  const int x0 = box.lo[0] + blockIdx.z * kGradX, x_end = min(x0 + kGradX, box.hi[0]);
+  const int j = j0 + tj, k = k0 + tk;
+  const bool mine = j < box.hi[1] && k < box.hi[2];
+  float fetched[kGradRounds];
+#pragma unroll
+  for (int r = 0; r < kGradRounds; ++r) {
+    const int t = threadIdx.x + r * blockDim.x;
+    fetched[r] = t < kGradSlab ? staged_phi(g, x0 - 1, j0 - 1 + t / kGradHK, k0 - 1 + t % kGradHK) : 0.0f;
+  }
+  float p_m2 = 0.0f, p_m1 = 0.0f, qy_m2 = 0.0f, qy_m1 = 0.0f, qz_m2 = 0.0f, qz_m1 = 0.0f;
+  const float sixth = 1.0f / 6.0f;
+  for (int s = x0 - 1; s <= x_end; ++s) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kGradRounds; ++r) {
+      const int t = threadIdx.x + r * blockDim.x;
+      if (t < kGradSlab) sp[t] = fetched[r];
+    }
+    if (s < x_end) {
+#pragma unroll
+      for (int r = 0; r < kGradRounds; ++r) {
+        const int t = threadIdx.x + r * blockDim.x;
+        fetched[r] = t < kGradSlab ? staged_phi(g, s + 1, j0 - 1 + t / kGradHK, k0 - 1 + t % kGradHK) : 0.0f;
+      }
+    }
+    __syncthreads();
+    float sz[3], dz[3];
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy) {
+      const int row = (tj + dy) * kGradHK + tk;
+      const float a = sp[row], b = sp[row + 1], c = sp[row + 2];
+      sz[dy] = (a + 4.0f * b + c) * sixth;
+      dz[dy] = (c - a) * sixth;
+    }
+    const float p = (sz[0] + 4.0f * sz[1] + sz[2]) * sixth;
+    const float qy = (sz[2] - sz[0]) * sixth;
+    const float qz = (dz[0] + 4.0f * dz[1] + dz[2]) * sixth;
+    const int x = s - 1;
+    if (x >= x0 && mine) {
+      const int n = lin(g, x, j, k);
+      const int flag = g.flag[n];
+      if (flag == 0 || flag == 1) {
+        store_gradient(g, n, 3.0f * (p - p_m2) * sixth, 3.0f * (qy_m2 + 4.0f * qy_m1 + qy) * sixth,
+                       3.0f * (qz_m2 + 4.0f * qz_m1 + qz) * sixth);
+      } else if (flag < 0) {
+        store_gradient(g, n, 0.0f, 0.0f, 0.0f);
+      }
+    }
+    p_m2 = p_m1;
+    p_m1 = p;
+    qy_m2 = qy_m1;
+    qy_m1 = qy;
+    qz_m2 = qz_m1;
+    qz_m1 = qz;
+  }
+}
+
 // gradients.py:116-145 with one lane per direction (wall links sample the
 // mirror), summed within the cell's 32-lane group.
 __global__ void __launch_bounds__(256) k_gradients_general_list(icy3d_grid g, Box box) {
@@ -827,6 +880,255 @@ __device__ __forceinline__ void forces_cell(const icy3d_grid& g, int n, int with
     g.accel[e * N + n] = a[e];
   }
 }
+// Tiled interior forces (simulator.py:418-519 and home.py:127-197 for
+// flag-0 cells, every link fluid).  The two 26-neighbour sums factor with
+// the D3Q27 weights w(c) = w1(cx) w1(cy) w1(cz):
+//   capillary  f_a = kcap sum_q w_q (c_a |g|^2 - (c.g) g_a)
+//                  = kcap [D_a(|g|^2 - g_a^2) - sum_{b != a} D_b(g_a g_b)],
+//   pressure   a  += sum_q w_q c p*_m (rho - rho_m) / rho = D(p*) - D(p* rho_m) / rho
+//              (3 p_m (1/rho_m - 1/rho) = D(3 p/rho_m) - D(3 p) / rho without moments),
+// where D_a applies (u+ - u-)/6 along a and (u- + 4 u0 + u+)/6 along the
+// other two axes.  Per slab the eight staged quantities (|g|^2 - g_a^2,
+// g_a g_b, and the two pressure fields) sit in LDS; each thread forms the
+// y/z parts for its column and the x combination runs in registers over
+// three slabs.  The six face-neighbour terms (donor phase flux, viscous
+// velocity gradient) read u, flux and phi directly, the x neighbours from the
+// same register pipeline.
+constexpr int kForceJ = 4, kForceK = 64, kForceX = 16;
+constexpr int kForceHJ = kForceJ + 2, kForceHK = kForceK + 2, kForceSlab = kForceHJ * kForceHK;
+constexpr int kForceRounds = (kForceSlab + kForceJ * kForceK - 1) / (kForceJ * kForceK);
+constexpr int kForceQ = 8;  // Ax, Ay, Az, Bxy, Bxz, Byz, X1, X2
+
+struct ForceFetch {
+  float q[kForceQ];
+};
+__device__ __forceinline__ void fetch_force_cell(const icy3d_grid& g, int i, int j, int k, int moment_pressure,
+                                                 ForceFetch& f) {
+  i = min(max(i, 0), g.nx - 1);
+  j = min(max(j, 0), g.ny - 1);
+  k = min(max(k, 0), g.nz - 1);
+  const size_t N = cells(g);
+  const int n = lin(g, i, j, k);
+  const float gx = g.grad[n], gy = g.grad[N + n], gz = g.grad[2 * N + n];
+  f.q[0] = gy * gy + gz * gz;
+  f.q[1] = gx * gx + gz * gz;
+  f.q[2] = gx * gx + gy * gy;
+  f.q[3] = gx * gy;
+  f.q[4] = gx * gz;
+  f.q[5] = gy * gz;
+  const float rho = density(g, g.phi[n]);
+  if (moment_pressure) {
+    const float p = g.home[n];
+    f.q[6] = p;
+    f.q[7] = p * rho;
+  } else {
+    const float p = g.pressure[n];
+    f.q[6] = 3.0f * p / rho;
+    f.q[7] = 3.0f * p;
+  }
+}
+struct SlabParts {
+  float px[5], qy[5], qz[5];  // D_x parts of Ax Bxy Bxz X1 X2; D_y of Ay Bxy Byz X1 X2; D_z of Az Bxz Byz X1 X2
+};
+__device__ __forceinline__ void yz_parts(const float* q, int base, float& P, float& Qy, float& Qz) {
+  const float sixth = 1.0f / 6.0f;
+  float sz[3], dz[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const float a = q[base + r * kForceHK], b = q[base + r * kForceHK + 1], c = q[base + r * kForceHK + 2];
+    sz[r] = (a + 4.0f * b + c) * sixth;
+    dz[r] = (c - a) * sixth;
+  }
+  P = (sz[0] + 4.0f * sz[1] + sz[2]) * sixth;
+  Qy = (sz[2] - sz[0]) * sixth;
+  Qz = (dz[0] + 4.0f * dz[1] + dz[2]) * sixth;
+}
+__device__ __forceinline__ SlabParts slab_parts(const float (*sq)[kForceSlab], int base) {
+  SlabParts r;
+  float P, Qy, Qz;
+  yz_parts(sq[0], base, P, Qy, Qz);  // Ax: D_x
+  r.px[0] = P;
+  yz_parts(sq[1], base, P, Qy, Qz);  // Ay: D_y
+  r.qy[0] = Qy;
+  yz_parts(sq[2], base, P, Qy, Qz);  // Az: D_z
+  r.qz[0] = Qz;
+  yz_parts(sq[3], base, P, Qy, Qz);  // Bxy: D_x, D_y
+  r.px[1] = P;
+  r.qy[1] = Qy;
+  yz_parts(sq[4], base, P, Qy, Qz);  // Bxz: D_x, D_z
+  r.px[2] = P;
+  r.qz[1] = Qz;
+  yz_parts(sq[5], base, P, Qy, Qz);  // Byz: D_y, D_z
+  r.qy[2] = Qy;
+  r.qz[2] = Qz;
+  yz_parts(sq[6], base, P, Qy, Qz);  // X1
+  r.px[3] = P;
+  r.qy[3] = Qy;
+  r.qz[3] = Qz;
+  yz_parts(sq[7], base, P, Qy, Qz);  // X2
+  r.px[4] = P;
+  r.qy[4] = Qy;
+  r.qz[4] = Qz;
+  return r;
+}
+struct ColumnCell {
+  float u[3], flux_x, phi;
+};
+__device__ __forceinline__ ColumnCell load_column(const icy3d_grid& g, int x, int j, int k) {
+  ColumnCell c = {{0.0f, 0.0f, 0.0f}, 0.0f, 0.0f};
+  if (x < 0 || x >= g.nx || j >= g.ny || k >= g.nz) return c;
+  const size_t N = cells(g);
+  const int n = lin(g, x, j, k);
+  c.u[0] = g.u[n];
+  c.u[1] = g.u[N + n];
+  c.u[2] = g.u[2 * N + n];
+  c.flux_x = g.flux[n];
+  c.phi = g.phi[n];
+  return c;
+}
+// Donor phase-flux momentum term of one face (simulator.py:499-519).
+__device__ __forceinline__ void face_flux(const icy3d_grid& g, float d, float flux_own, float flux_nb, float rho,
+                                          float phi_nb, const float u_own[3], const float u_nb[3], float f[3]) {
+  const float mass_flux = 0.5f * (flux_own + flux_nb);
+  const bool own = d * mass_flux >= 0.0f;
+  float fl[3];
+#pragma unroll
+  for (int e = 0; e < 3; ++e) fl[e] = mass_flux * ((own ? u_own[e] : u_nb[e]) - u_own[e]);
+  if (g.flux_limit > 0.0f) {
+    const float budget = g.flux_limit / 6.0f * fminf(rho, density(g, phi_nb));
+    const float scale = fminf(1.0f, budget / (fabsf(fl[0]) + fabsf(fl[1]) + fabsf(fl[2]) + 1e-20f));
+    fl[0] *= scale;
+    fl[1] *= scale;
+    fl[2] *= scale;
+  }
+  f[0] -= d * fl[0];
+  f[1] -= d * fl[1];
+  f[2] -= d * fl[2];
+}
+
+__global__ void __launch_bounds__(kForceJ* kForceK) k_forces_tiled(icy3d_grid g, int with_flux, int moment_pressure) {
+  __shared__ float sq[kForceQ][kForceSlab];
+  const size_t N = cells(g);
+  const int tk = threadIdx.x % kForceK, tj = threadIdx.x / kForceK;
+  const int k0 = blockIdx.x * kForceK, j0 = blockIdx.y * kForceJ, x0 = blockIdx.z * kForceX;
+  const int x_end = min(x0 + kForceX, g.nx);
+  const int j = j0 + tj, k = k0 + tk;
+  const bool mine = j < g.ny && k < g.nz;
+  const int base = tj * kForceHK + tk;  // top-left of the thread's 3x3 (j, k) window
+  ForceFetch fetched[kForceRounds];
+#pragma unroll
+  for (int r = 0; r < kForceRounds; ++r) {
+    const int t = threadIdx.x + r * blockDim.x;
+    if (t < kForceSlab) fetch_force_cell(g, x0 - 1, j0 - 1 + t / kForceHK, k0 - 1 + t % kForceHK, moment_pressure, fetched[r]);
+  }
+  SlabParts prev2 = {}, prev1 = {};
+  ColumnCell col_m1 = load_column(g, x0 - 2, j, k), col_0 = load_column(g, x0 - 1, j, k);
+  const float kcap = 4.5f * g.sigma * g.width;
+  for (int s = x0 - 1; s <= x_end; ++s) {
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kForceRounds; ++r) {
+      const int t = threadIdx.x + r * blockDim.x;
+      if (t < kForceSlab) {
+#pragma unroll
+        for (int m = 0; m < kForceQ; ++m) sq[m][t] = fetched[r].q[m];
+      }
+    }
+    if (s < x_end) {
+#pragma unroll
+      for (int r = 0; r < kForceRounds; ++r) {
+        const int t = threadIdx.x + r * blockDim.x;
+        if (t < kForceSlab)
+          fetch_force_cell(g, s + 1, j0 - 1 + t / kForceHK, k0 - 1 + t % kForceHK, moment_pressure, fetched[r]);
+      }
+    }
+    __syncthreads();
+    const SlabParts cur = slab_parts(sq, base);
+    const ColumnCell col_p1 = load_column(g, s, j, k);  // the x+1 neighbour of cell s - 1
+    const int x = s - 1;
+    if (x >= x0 && mine) {
+      const int n = lin(g, x, j, k);
+      if (g.flag[n] == 0) {
+        const float sixth = 1.0f / 6.0f;
+        const float phi = col_0.phi, rho = density(g, phi);
+        float f[3] = {0.0f, (rho - g.ref_density[j]) * g.gravity_y, 0.0f};
+        f[1] -= fmaxf(0.0f, 2.0f * phi - 1.0f) * g.expansion * (water_temperature(g, n) - g.water_t0) * g.gravity_y;
+        // x parts: D_x = (P(x+1) - P(x-1)) / 6; y, z parts: S_x = (Q(x-1) + 4 Q(x) + Q(x+1)) / 6
+        float dx[5], dy[5], dz[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+          dx[m] = (cur.px[m] - prev2.px[m]) * sixth;
+          dy[m] = (prev2.qy[m] + 4.0f * prev1.qy[m] + cur.qy[m]) * sixth;
+          dz[m] = (prev2.qz[m] + 4.0f * prev1.qz[m] + cur.qz[m]) * sixth;
+        }
+        if (g.sigma > 0.0f) {
+          f[0] += kcap * (dx[0] - dy[1] - dz[1]);
+          f[1] += kcap * (dy[0] - dx[1] - dz[2]);
+          f[2] += kcap * (dz[0] - dx[2] - dy[2]);
+        }
+        const float u_own[3] = {col_0.u[0], col_0.u[1], col_0.u[2]};
+        // face neighbours: -x, +x from the pipeline, -y, +y, -z, +z from memory
+        float un[6][3];
+        float phin[6], fluxn[6];
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          un[0][e] = col_m1.u[e];
+          un[1][e] = col_p1.u[e];
+        }
+        phin[0] = col_m1.phi;
+        phin[1] = col_p1.phi;
+        fluxn[0] = col_m1.flux_x;
+        fluxn[1] = col_p1.flux_x;
+        const int nb[4] = {n - g.nz, n + g.nz, n - 1, n + 1};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int m = nb[t];
+#pragma unroll
+          for (int e = 0; e < 3; ++e) un[2 + t][e] = g.u[e * N + m];
+          phin[2 + t] = g.phi[m];
+          fluxn[2 + t] = g.flux[(1 + t / 2) * N + m];
+        }
+        if (with_flux) {
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            const float flux_own = g.flux[a * N + n];
+            face_flux(g, -1.0f, flux_own, fluxn[2 * a], rho, phin[2 * a], u_own, un[2 * a], f);
+            face_flux(g, 1.0f, flux_own, fluxn[2 * a + 1], rho, phin[2 * a + 1], u_own, un[2 * a + 1], f);
+          }
+        }
+        // home.py:127-197
+        const float inv_rho = 1.0f / rho;
+        float acc[3] = {f[0] / rho, f[1] / rho, f[2] / rho};
+        acc[0] += dx[3] - dx[4] * inv_rho;
+        acc[1] += dy[3] - dy[4] * inv_rho;
+        acc[2] += dz[3] - dz[4] * inv_rho;
+        const float dr = 1.0f - g.rho_air;
+        const float gr[3] = {g.grad[n] * dr, g.grad[N + n] * dr, g.grad[2 * N + n] * dr};
+        if (gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2] > 0.0f) {
+          float G[3][3];
+#pragma unroll
+          for (int axis = 0; axis < 3; ++axis)
+#pragma unroll
+            for (int e = 0; e < 3; ++e) G[e][axis] = 0.5f * (un[2 * axis + 1][e] - un[2 * axis][e]);
+          const float coef = lattice_nu(g, n) / rho;
+#pragma unroll
+          for (int e = 0; e < 3; ++e)
+            acc[e] += coef * ((G[e][0] + G[0][e]) * gr[0] + (G[e][1] + G[1][e]) * gr[1] + (G[e][2] + G[2][e]) * gr[2]);
+        }
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          g.force[e * N + n] = f[e];
+          g.accel[e * N + n] = acc[e];
+        }
+      }
+    }
+    prev2 = prev1;
+    prev1 = cur;
+    col_m1 = col_0;
+    col_0 = col_p1;
+  }
+}
+
 __global__ void k_forces_interior(icy3d_grid g, int with_flux, int moment_pressure) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= (int)cells(g) || g.flag[n] != 0) return;
@@ -1494,13 +1796,16 @@ int launch_classify_all(const icy3d_grid& g, hipStream_t s) {
 int launch_gradients(const icy3d_grid& g, const Box& b, bool, hipStream_t s) {
   size_t n = box_cells(b);
   if (!n) return 0;
-  k_gradients_regular<<<icy_blocks(n), kBlock, 0, s>>>(g, b);
+  const dim3 tiles((b.hi[2] - b.lo[2] + kGradK - 1) / kGradK, (b.hi[1] - b.lo[1] + kGradJ - 1) / kGradJ,
+                   (b.hi[0] - b.lo[0] + kGradX - 1) / kGradX);
+  k_gradients_tiled<<<tiles, kGradJ * kGradK, 0, s>>>(g, b);
   k_gradients_general_list<<<kListBlocks, kBlock, 0, s>>>(g, b);
   ICY_LAUNCH_CHECK();
   return 0;
 }
 int launch_forces(const icy3d_grid& g, int with_flux, int moment_pressure, hipStream_t s) {
-  k_forces_interior<<<icy_blocks(grid_cells(g)), kBlock, 0, s>>>(g, with_flux, moment_pressure);
+  const dim3 tiles((g.nz + kForceK - 1) / kForceK, (g.ny + kForceJ - 1) / kForceJ, (g.nx + kForceX - 1) / kForceX);
+  k_forces_tiled<<<tiles, kForceJ * kForceK, 0, s>>>(g, with_flux, moment_pressure);
   k_forces_tank<<<kListBlocks, kBlock, 0, s>>>(g, with_flux, moment_pressure);
   k_forces_general<<<kListBlocks, kBlock, 0, s>>>(g, with_flux, moment_pressure);
   ICY_LAUNCH_CHECK();
