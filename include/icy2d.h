@@ -12,7 +12,8 @@
  * Conventions
  *   - Cells n = i * ny + j (j fastest).  Populations are q-plane major:
  *     population q of cell n is ``f[q * N + n]`` (D2Q9 order of
- *     iceflow2d/lattice.py:8-62).  Vectors are planes: ``u[c * N + n]``.
+ *     iceflow2d/lattice.py:8-62), FP32 or scaled FP16 elements (see
+ *     half_populations).  Vectors are planes: ``u[c * N + n]``.
  *   - x-face fields have (nx + 1) * ny entries (face (i, j) at i * ny + j);
  *     y-face fields nx * (ny + 1) (face (i, j) at i * (ny + 1) + j).
  *   - Body fields are body_nx * body_ny, index bi * body_ny + bj.
@@ -37,7 +38,8 @@ enum { ICY2D_OK = 0, ICY2D_ERR_ARGUMENT = -1 };
 typedef struct icy2d_control {
   float velocity[2], angle, omega, origin[2], support[4]; /* simulator.py:1950-2089 */
   int32_t projected, melt_status, wet_target, wet_target_max;
-  float max_outflow, reserved;
+  float max_outflow;
+  int32_t error_flags;        /* bit 0: negative water volume, bit 1: volume above the aperture capacity */
   double mass, inertia, local_com[2];
   double impulse[3];          /* hydrodynamic impulse x, y and torque */
   double melted[3];           /* cumulative melted body momentum */
@@ -52,6 +54,7 @@ typedef struct icy2d_control {
   double unassigned[2], fallback[2], interval[2];
   double reduced[4];          /* mass, first moment x, y, origin inertia */
   double totals[4];           /* solid mass, water mass, body and water sensible energy */
+  double error_values[2];     /* volume and capacity when error_flags was first set */
 } icy2d_control;
 
 typedef struct icy2d_grid {
@@ -76,7 +79,11 @@ typedef struct icy2d_grid {
   double *water_v, *water_e, *target_e;
   double *flux_ex, *flux_vx, *flux_ey, *flux_vy;
   double* partials;               /* reduction scratch: partials_capacity doubles */
-  int32_t partials_capacity, reserved;
+  int32_t partials_capacity;
+  /* Population storage: 0 = FP32; 1 = FP16 holding population * population_scale
+   * (a power of two; FP32 arithmetic in registers).  No reference counterpart. */
+  int32_t half_populations;
+  float population_scale, population_inverse_scale;
   icy2d_control* control;
 } icy2d_grid;
 
@@ -107,10 +114,12 @@ int icy2d_collide(const icy2d_grid* g, int with_momentum, void* stream);
 /* Cut-link preparation alone, applied in place to fbuf (the reference
  * tests' helper kernel around :4798-4866). */
 int icy2d_prepare_cut_links(const icy2d_grid* g, void* stream);
-/* Streaming (:3699-3738; :3740-3785 when phase_only) in pull form, fused
- * with the macroscopic update (:3787-3813; :3815-3825 when phase_only). */
-int icy2d_stream(const icy2d_grid* g, int phase_only, void* stream);
-int icy2d_update_pressure(const icy2d_grid* g, void* stream);   /* :3860-3897 */
+/* mode 1: streaming (:3699-3738; :3740-3785 when phase_only) in pull form;
+ * mode 2: the macroscopic update (:3787-3813; :3815-3825 when phase_only);
+ * mode 3: both in one pass (the step). */
+int icy2d_stream(const icy2d_grid* g, int phase_only, int mode, void* stream);
+/* :3860-3897; the population sum is the one recorded by the last mode-2/3 update. */
+int icy2d_update_pressure(const icy2d_grid* g, void* stream);
 int icy2d_refill(const icy2d_grid* g, void* stream);            /* :3163-3236 */
 int icy2d_measure_outflow(const icy2d_grid* g, void* stream);   /* :2303-2338 */
 
@@ -133,6 +142,7 @@ int icy2d_local_melt_correction(const icy2d_grid* g, void* stream);
 int icy2d_finish_melt_momentum(const icy2d_grid* g, void* stream);
 
 /* Thermal (MovingBodyThermal2D) ------------------------------------------- */
+/* Also checks the volume against the capacity (:522-540) into error_flags. */
 int icy2d_measure_aperture(const icy2d_grid* g, void* stream);             /* :633-714, :1792-1822 */
 int icy2d_build_aperture_targets(const icy2d_grid* g, void* stream);       /* :716-827 */
 int icy2d_apply_aperture_transfer(const icy2d_grid* g, void* stream);      /* :829-859 */

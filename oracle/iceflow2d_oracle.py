@@ -438,6 +438,9 @@ class Oracle2D:
     def _stream(self):
         self._k("stream", 0)
 
+    def _update_streamed_macroscopic_fields(self):
+        self._k("update_macroscopic")
+
     def _integrate_rigid_ice(self):
         self._k("integrate_rigid")
 

@@ -29,14 +29,14 @@ class Control2D(ctypes.Structure):
     _fields_ = [
         ("velocity", _f * 2), ("angle", _f), ("omega", _f), ("origin", _f * 2), ("support", _f * 4),
         ("projected", _i), ("melt_status", _i), ("wet_target", _i), ("wet_target_max", _i),
-        ("max_outflow", _f), ("reserved", _f),
+        ("max_outflow", _f), ("error_flags", _i),
         ("mass", _d), ("inertia", _d), ("local_com", _d * 2),
         ("impulse", _d * 3), ("melted", _d * 3), ("carrier", _d * 3), ("correction", _d * 3),
         ("before", _d * 3), ("provisional", _d * 3), ("after", _d * 3), ("body_before", _d * 3),
         ("wet", _d * 4), ("water", _d * 3), ("aperture", _d * 6), ("targets", _d * 3), ("transfer", _d * 2),
         ("correction_abs", _d), ("boundary_power", _d), ("boundary_heat_input", _d),
         ("unassigned", _d * 2), ("fallback", _d * 2), ("interval", _d * 2),
-        ("reduced", _d * 4), ("totals", _d * 4),
+        ("reduced", _d * 4), ("totals", _d * 4), ("error_values", _d * 2),
     ]
 
 
@@ -46,7 +46,8 @@ class Grid2D(ctypes.Structure):
         + [(n, _f) for n in GRID_FLOATS]
         + [(n, _d) for n in GRID_DOUBLES]
         + [(n, _p) for n in GRID_POINTERS]
-        + [("partials_capacity", _i), ("reserved", _i), ("control", _p)]
+        + [("partials_capacity", _i), ("half_populations", _i), ("population_scale", _f),
+           ("population_inverse_scale", _f), ("control", _p)]
     )
 
 
@@ -63,7 +64,7 @@ CALLS = {
     "project_inside": [], "update_solid_mask": [_i], "reduce_mass_properties": [], "apply_mass_properties": [_i],
     "initialize_fluid": [_i, _i], "hydrostatic_equilibrium": [], "initialize_water_volume": [],
     "recover_initial_moments": [], "initialize_body_state": [_d], "initialize_water_state": [_d],
-    "collide": [_i], "prepare_cut_links": [], "stream": [_i], "update_pressure": [], "refill": [],
+    "collide": [_i], "prepare_cut_links": [], "stream": [_i, _i], "update_pressure": [], "refill": [],
     "measure_outflow": [], "clip_phase": [_d], "seed_projection": [], "dilate_projection": [_i],
     "evaluate_projection": [_d], "apply_projection": [_d], "projection_residual_weight": [_d],
     "apply_projection_residual": [_d, _d], "snapshot_melt_momentum": [], "reduce_provisional_momentum": [],

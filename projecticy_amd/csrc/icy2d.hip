@@ -19,6 +19,7 @@
 //     outgoing population opp(q) (with the moving-ice phase correction), which is
 //     exactly what the reference's push writes into x; the same thread then
 //     forms phi, u and the pressure's population sum from registers.
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -48,6 +49,25 @@ __device__ __forceinline__ bool wall_at(const icy2d_grid& g, int i, int j) {
 __device__ __forceinline__ bool fluid_at(const icy2d_grid& g, int i, int j) { return !wall_at(g, i, j) && g.solid[at(g, i, j)] == 0; }
 __device__ __forceinline__ float clamp01(float v) { return fminf(1.0f, fmaxf(0.0f, v)); }
 __device__ __forceinline__ double clamp01d(double v) { return fmin(1.0, fmax(0.0, v)); }
+
+// Population storage (f, fbuf, h, hbuf): FP32, or FP16 holding the
+// population times population_scale (a power of two, so the scaling is exact)
+// with FP32 arithmetic in registers.  The Liang populations are small
+// (O(u) and O(p)); the scale keeps them in the FP16 normal range.
+__device__ __forceinline__ float pop_get(const float* base, const icy2d_grid& g, int idx) {
+  if (g.half_populations) return __half2float(reinterpret_cast<const __half*>(base)[idx]) * g.population_inverse_scale;
+  return base[idx];
+}
+__device__ __forceinline__ void pop_set(float* base, const icy2d_grid& g, int idx, float v) {
+  if (g.half_populations) {
+    reinterpret_cast<__half*>(base)[idx] = __float2half(v * g.population_scale);
+  } else {
+    base[idx] = v;
+  }
+}
+__device__ __forceinline__ void pop_add(float* base, const icy2d_grid& g, int idx, float v) {
+  pop_set(base, g, idx, pop_get(base, g, idx) + v);
+}
 
 // lattice.py:64-104
 template <int Q>
@@ -217,29 +237,35 @@ __device__ __forceinline__ void block_partials(double (&v)[K], double* partials)
     partials[blockIdx.x * K + threadIdx.x] = total;
   }
 }
-// Folds partials[b * K + k] over b in a fixed order.  Bit k of min_mask /
+// Folds partials[b * K + k] over b in a fixed order: one block per k,
+// each thread summing a strided subset with four loads in flight, then a
+// fixed-order combination of the thread results.  Bit k of min_mask /
 // max_mask selects a minimum / maximum instead of a sum; bit k of add_mask
 // adds the result to out[k] instead of replacing it.
-__global__ void __launch_bounds__(kBlock) k2_finish(const double* __restrict__ partials, int blocks, int K, double* out,
-                                                    unsigned min_mask, unsigned max_mask, unsigned add_mask) {
-  __shared__ double part[kBlock];
-  for (int k = 0; k < K; ++k) {
-    const bool is_min = (min_mask >> k) & 1u, is_max = (max_mask >> k) & 1u;
-    double v = is_min ? 1.0e300 : is_max ? -1.0e300 : 0.0;
-    for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
-      const double x = partials[b * K + k];
-      v = is_min ? fmin(v, x) : is_max ? fmax(v, x) : v + x;
-    }
-    part[threadIdx.x] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double total = part[0];
-      for (int t = 1; t < (int)blockDim.x; ++t)
-        total = is_min ? fmin(total, part[t]) : is_max ? fmax(total, part[t]) : total + part[t];
-      out[k] = ((add_mask >> k) & 1u) ? out[k] + total : total;
-    }
+constexpr int kFinishThreads = 1024;
+__global__ void __launch_bounds__(kFinishThreads) k2_finish(const double* __restrict__ partials, int blocks, int K,
+                                                            double* out, unsigned min_mask, unsigned max_mask,
+                                                            unsigned add_mask) {
+  __shared__ double part[kFinishThreads];
+  const int k = blockIdx.x;
+  const bool is_min = (min_mask >> k) & 1u, is_max = (max_mask >> k) & 1u;
+  const double identity = is_min ? 1.0e300 : is_max ? -1.0e300 : 0.0;
+  auto combine = [&](double a, double b) { return is_min ? fmin(a, b) : is_max ? fmax(a, b) : a + b; };
+  double v = identity;
+  int b = threadIdx.x;
+  for (; b + 3 * kFinishThreads < blocks; b += 4 * kFinishThreads) {
+    const double x0 = partials[b * K + k], x1 = partials[(b + kFinishThreads) * K + k];
+    const double x2 = partials[(b + 2 * kFinishThreads) * K + k], x3 = partials[(b + 3 * kFinishThreads) * K + k];
+    v = combine(v, combine(combine(x0, x1), combine(x2, x3)));
+  }
+  for (; b < blocks; b += kFinishThreads) v = combine(v, partials[b * K + k]);
+  part[threadIdx.x] = v;
+  __syncthreads();
+  for (int width = kFinishThreads / 2; width > 0; width >>= 1) {
+    if ((int)threadIdx.x < width) part[threadIdx.x] = combine(part[threadIdx.x], part[threadIdx.x + width]);
     __syncthreads();
   }
+  if (threadIdx.x == 0) out[k] = ((add_mask >> k) & 1u) ? out[k] + part[0] : part[0];
 }
 
 __device__ __forceinline__ int cell_index() { return blockIdx.x * blockDim.x + threadIdx.x; }
@@ -404,8 +430,8 @@ __global__ void k2_initialize_fluid(icy2d_grid g, int water_width, int water_hei
   const float rho = mixture(g, phi0);
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
-    g.f[q * N + n] = momentum_eq_rt(q, 0.0f, rho, v[0], v[1]);
-    g.h[q * N + n] = active ? phase_eq_rt(q, phi0, v[0], v[1]) : 0.0f;
+    pop_set(g.f, g, q * N + n, momentum_eq_rt(q, 0.0f, rho, v[0], v[1]));
+    pop_set(g.h, g, q * N + n, active ? phase_eq_rt(q, phi0, v[0], v[1]) : 0.0f);
   }
 }
 // simulator.py:2664-2697
@@ -419,8 +445,8 @@ __global__ void k2_hydrostatic_equilibrium(icy2d_grid g) {
     g.p[n] = g.p_ref[j];
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-      g.f[q * N + n] = momentum_eq_rt(q, 0.0f, rho, 0.0f, 0.0f);
-      g.h[q * N + n] = phase_eq_rt(q, phi, 0.0f, 0.0f);
+      pop_set(g.f, g, q * N + n, momentum_eq_rt(q, 0.0f, rho, 0.0f, 0.0f));
+      pop_set(g.h, g, q * N + n, phase_eq_rt(q, phi, 0.0f, 0.0f));
     }
   } else {
     float v[2] = {0.0f, 0.0f};
@@ -450,8 +476,8 @@ __global__ void k2_recover_initial_moments(icy2d_grid g) {
     float phi = 0.0f, mx = 0.0f, my = 0.0f;
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-      phi += g.h[q * N + n];
-      const float fv = g.f[q * N + n];
+      phi += pop_get(g.h, g, q * N + n);
+      const float fv = pop_get(g.f, g, q * N + n);
       if (cxq(q)) mx += (float)cxq(q) * fv;
       if (cyq(q)) my += (float)cyq(q) * fv;
     }
@@ -529,7 +555,7 @@ __device__ __forceinline__ bool prepare_cut_links(const icy2d_grid& g, const Pos
     float ub[2];
     body_velocity(p, bx, by, ub);
     const int o = oppq(q);
-    const float noneq = g.f[q * N + n] - momentum_eq_rt(q, pdyn, rho, ux, uy);
+    const float noneq = pop_get(g.f, g, q * N + n) - momentum_eq_rt(q, pdyn, rho, ux, uy);
     const float wall = momentum_eq_rt(o, pdyn, rho, ub[0], ub[1]);
     const float reflected = (wall + noneq + eta * raw[o]) / (1.0f + eta);
     prepared[q] = reflected;
@@ -622,7 +648,7 @@ __global__ void __launch_bounds__(kBlock) k2_collide(icy2d_grid g, int with_mome
         float raw[6] = {0, 0, 0, 0, 0, 0}, eq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, src[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
-          const float cx = (float)cxq(q), cy = (float)cyq(q), v = g.f[q * N + n];
+          const float cx = (float)cxq(q), cy = (float)cyq(q), v = pop_get(g.f, g, q * N + n);
           const float e = momentum_eq_rt(q, pdyn, rho, ux, uy);
           const float b = wq(q) * 3.0f * (cx * Fx + cy * Fy + drho * (cx * ux + cy * uy) * (cx * gsx + cy * gsy));
           const float k[9] = {1.0f, cx, cy, cx * cx, cy * cy, cx * cy, cx * cx * cy, cx * cy * cy, cx * cx * cy * cy};
@@ -659,7 +685,7 @@ __global__ void __launch_bounds__(kBlock) k2_collide(icy2d_grid g, int with_mome
         out[8] = reconstruct_cm<8>(0.0f, 0.0f, post);
         prepare_cut_links(g, pose, i, j, out, load);
 #pragma unroll
-        for (int q = 0; q < 9; ++q) g.fbuf[q * N + n] = out[q];
+        for (int q = 0; q < 9; ++q) pop_set(g.fbuf, g, q * N + n, out[q]);
       } else {
         ax = g.a[n];
         ay = g.a[N + n];
@@ -673,7 +699,7 @@ __global__ void __launch_bounds__(kBlock) k2_collide(icy2d_grid g, int with_mome
       float r10 = 0.0f, r01 = 0.0f, eq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, src[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
       for (int q = 0; q < 9; ++q) {
-        const float rx = (float)cxq(q) - ux, ry = (float)cyq(q) - uy, hv = g.h[q * N + n];
+        const float rx = (float)cxq(q) - ux, ry = (float)cyq(q) - uy, hv = pop_get(g.h, g, q * N + n);
         const float he = phase_eq_rt(q, phi, ux, uy);
         const float force = wq(q) * compression * ((float)cxq(q) * nx_ + (float)cyq(q) * ny_);
         const float rx2 = rx * rx, ry2 = ry * ry;
@@ -691,15 +717,15 @@ __global__ void __launch_bounds__(kBlock) k2_collide(icy2d_grid g, int with_mome
       for (int m = 0; m < 9; ++m) post[m] = eq[m] + 0.5f * src[m];
       post[1] = r10 - rate * (r10 - eq[1]) + (1.0f - 0.5f * rate) * src[1];
       post[2] = r01 - rate * (r01 - eq[2]) + (1.0f - 0.5f * rate) * src[2];
-      g.hbuf[0 * N + n] = reconstruct_cm<0>(ux, uy, post);
-      g.hbuf[1 * N + n] = reconstruct_cm<1>(ux, uy, post);
-      g.hbuf[2 * N + n] = reconstruct_cm<2>(ux, uy, post);
-      g.hbuf[3 * N + n] = reconstruct_cm<3>(ux, uy, post);
-      g.hbuf[4 * N + n] = reconstruct_cm<4>(ux, uy, post);
-      g.hbuf[5 * N + n] = reconstruct_cm<5>(ux, uy, post);
-      g.hbuf[6 * N + n] = reconstruct_cm<6>(ux, uy, post);
-      g.hbuf[7 * N + n] = reconstruct_cm<7>(ux, uy, post);
-      g.hbuf[8 * N + n] = reconstruct_cm<8>(ux, uy, post);
+      pop_set(g.hbuf, g, 0 * N + n, reconstruct_cm<0>(ux, uy, post));
+      pop_set(g.hbuf, g, 1 * N + n, reconstruct_cm<1>(ux, uy, post));
+      pop_set(g.hbuf, g, 2 * N + n, reconstruct_cm<2>(ux, uy, post));
+      pop_set(g.hbuf, g, 3 * N + n, reconstruct_cm<3>(ux, uy, post));
+      pop_set(g.hbuf, g, 4 * N + n, reconstruct_cm<4>(ux, uy, post));
+      pop_set(g.hbuf, g, 5 * N + n, reconstruct_cm<5>(ux, uy, post));
+      pop_set(g.hbuf, g, 6 * N + n, reconstruct_cm<6>(ux, uy, post));
+      pop_set(g.hbuf, g, 7 * N + n, reconstruct_cm<7>(ux, uy, post));
+      pop_set(g.hbuf, g, 8 * N + n, reconstruct_cm<8>(ux, uy, post));
     }
   }
   if (with_momentum) block_partials<3>(load, g.partials);
@@ -714,10 +740,10 @@ __global__ void k2_prepare_cut_links(icy2d_grid g) {
     if (fluid_at(g, i, j)) {
       float raw[9];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) raw[q] = g.fbuf[q * N + n];
+      for (int q = 0; q < 9; ++q) raw[q] = pop_get(g.fbuf, g, q * N + n);
       prepare_cut_links(g, load_pose(g), i, j, raw, load);
 #pragma unroll
-      for (int q = 0; q < 9; ++q) g.fbuf[q * N + n] = raw[q];
+      for (int q = 0; q < 9; ++q) pop_set(g.fbuf, g, q * N + n, raw[q]);
     }
   }
   block_partials<3>(load, g.partials);
@@ -731,12 +757,12 @@ __device__ __forceinline__ void pull(const icy2d_grid& g, const Pose& pose, int 
   const int si = i - cxq(Q), sj = j - cyq(Q);  // source of population Q
   if (Q == 0 || (inside(g, si, sj) && fluid_at(g, si, sj))) {
     const int m = Q == 0 ? n : at(g, si, sj);
-    if (!phase_only) fq = g.fbuf[Q * N + m];
-    hq = g.hbuf[Q * N + m];
+    if (!phase_only) fq = pop_get(g.fbuf, g, Q * N + m);
+    hq = pop_get(g.hbuf, g, Q * N + m);
   } else {
     constexpr int out = oppq(Q);  // own population sent towards (si, sj)
-    if (!phase_only) fq = g.fbuf[out * N + n];
-    float ho = g.hbuf[out * N + n];
+    if (!phase_only) fq = pop_get(g.fbuf, g, out * N + n);
+    float ho = pop_get(g.hbuf, g, out * N + n);
     if (inside(g, si, sj) && g.solid[at(g, si, sj)] == 1) {
       const float eta = cut_fraction(g, pose, i, j, si, sj);
       float ub[2];
@@ -746,36 +772,50 @@ __device__ __forceinline__ void pull(const icy2d_grid& g, const Pose& pose, int 
     hq = ho;
   }
 }
-__global__ void __launch_bounds__(kBlock) k2_stream(icy2d_grid g, int phase_only) {
+// mode bit 0: stream (pull); bit 1: update phi, u (and the pressure's
+// population sum) from the cell's populations, as the reference's separate
+// macroscopic kernel does after streaming.  The step runs both in one pass.
+constexpr int kStreamPull = 1, kStreamUpdate = 2;
+__global__ void __launch_bounds__(kBlock) k2_stream(icy2d_grid g, int phase_only, int mode) {
   const int n = cell_index(), N = cells(g);
   if (n >= N) return;
   const int i = n / g.ny, j = n - i * g.ny;
   const Pose pose = load_pose(g);
+  const bool po = phase_only != 0, pulling = mode & kStreamPull, update = mode & kStreamUpdate;
   if (fluid_at(g, i, j)) {
     const float phi_old = g.phi[n];
     float f[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, h[9];
-    const bool po = phase_only != 0;
-    pull<0>(g, pose, i, j, n, phi_old, po, f[0], h[0]);
-    pull<1>(g, pose, i, j, n, phi_old, po, f[1], h[1]);
-    pull<2>(g, pose, i, j, n, phi_old, po, f[2], h[2]);
-    pull<3>(g, pose, i, j, n, phi_old, po, f[3], h[3]);
-    pull<4>(g, pose, i, j, n, phi_old, po, f[4], h[4]);
-    pull<5>(g, pose, i, j, n, phi_old, po, f[5], h[5]);
-    pull<6>(g, pose, i, j, n, phi_old, po, f[6], h[6]);
-    pull<7>(g, pose, i, j, n, phi_old, po, f[7], h[7]);
-    pull<8>(g, pose, i, j, n, phi_old, po, f[8], h[8]);
+    if (pulling) {
+      pull<0>(g, pose, i, j, n, phi_old, po, f[0], h[0]);
+      pull<1>(g, pose, i, j, n, phi_old, po, f[1], h[1]);
+      pull<2>(g, pose, i, j, n, phi_old, po, f[2], h[2]);
+      pull<3>(g, pose, i, j, n, phi_old, po, f[3], h[3]);
+      pull<4>(g, pose, i, j, n, phi_old, po, f[4], h[4]);
+      pull<5>(g, pose, i, j, n, phi_old, po, f[5], h[5]);
+      pull<6>(g, pose, i, j, n, phi_old, po, f[6], h[6]);
+      pull<7>(g, pose, i, j, n, phi_old, po, f[7], h[7]);
+      pull<8>(g, pose, i, j, n, phi_old, po, f[8], h[8]);
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        pop_set(g.h, g, q * N + n, h[q]);
+        if (!po) pop_set(g.f, g, q * N + n, f[q]);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        h[q] = pop_get(g.h, g, q * N + n);
+        if (!po) f[q] = pop_get(g.f, g, q * N + n);
+      }
+    }
+    if (!update) return;
     float phi = 0.0f;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) {
-      g.h[q * N + n] = h[q];
-      phi += h[q];
-    }
+    for (int q = 0; q < 9; ++q) phi += h[q];
     g.phi[n] = phi;
     if (!po) {
       float mx = 0.0f, my = 0.0f, moving = 0.0f;
 #pragma unroll
       for (int q = 0; q < 9; ++q) {
-        g.f[q * N + n] = f[q];
         if (cxq(q)) mx += (float)cxq(q) * f[q];
         if (cyq(q)) my += (float)cyq(q) * f[q];
         if (q) moving += f[q];
@@ -785,9 +825,9 @@ __global__ void __launch_bounds__(kBlock) k2_stream(icy2d_grid g, int phase_only
       g.u[N + n] = my / rho;
       g.moving[n] = moving;
     }
-  } else {
+  } else if (update) {
     if (wall_at(g, i, j)) g.phi[n] = 0.0f;
-    if (!phase_only) {
+    if (!po) {
       float v[2] = {0.0f, 0.0f};
       if (g.solid[n] == 1) body_velocity(pose, i + 0.5f, j + 0.5f, v);
       g.u[n] = v[0];
@@ -870,8 +910,8 @@ __global__ void k2_refill(icy2d_grid g) {
   const float pdyn = p0 - g.p_ref[j];
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
-    g.f[q * N + n] = momentum_eq_rt(q, pdyn, rho, v[0], v[1]);
-    g.h[q * N + n] = phase_eq_rt(q, phi0, v[0], v[1]);
+    pop_set(g.f, g, q * N + n, momentum_eq_rt(q, pdyn, rho, v[0], v[1]));
+    pop_set(g.h, g, q * N + n, phase_eq_rt(q, phi0, v[0], v[1]));
   }
 }
 // simulator.py:2303-2338 (per-block maxima as partials)
@@ -918,15 +958,15 @@ __device__ __forceinline__ void lift_phase(const icy2d_grid& g, int n, int j, fl
   const float ro = mixture(g, old_phase), rn = mixture(g, new_phase);
 #pragma unroll
   for (int q = 0; q < 9; ++q)
-    g.f[q * N + n] += momentum_eq_rt(q, pdyn, rn, ux, uy) - momentum_eq_rt(q, pdyn, ro, ux, uy);
+    pop_add(g.f, g, q * N + n, momentum_eq_rt(q, pdyn, rn, ux, uy) - momentum_eq_rt(q, pdyn, ro, ux, uy));
   float sum = 0.0f;
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
-    const float lifted = g.h[q * N + n] + phase_eq_rt(q, new_phase, ux, uy) - phase_eq_rt(q, old_phase, ux, uy);
-    g.h[q * N + n] = lifted;
+    const float lifted = pop_get(g.h, g, q * N + n) + phase_eq_rt(q, new_phase, ux, uy) - phase_eq_rt(q, old_phase, ux, uy);
+    pop_set(g.h, g, q * N + n, lifted);
     sum += lifted;
   }
-  g.h[n] += new_phase - sum;
+  pop_add(g.h, g, n, new_phase - sum);
   g.phi[n] = new_phase;
 }
 // simulator.py:4150-4229
@@ -946,11 +986,11 @@ __global__ void k2_clip_phase(icy2d_grid g, double max_bulk_l1) {
         const float ro = mixture(g, old_phase), rn = mixture(g, bounded);
 #pragma unroll
         for (int q = 0; q < 9; ++q)
-          g.f[q * N + n] += momentum_eq_rt(q, pdyn, rn, ux, uy) - momentum_eq_rt(q, pdyn, ro, ux, uy);
+          pop_add(g.f, g, q * N + n, momentum_eq_rt(q, pdyn, rn, ux, uy) - momentum_eq_rt(q, pdyn, ro, ux, uy));
       }
       float l1 = 0.0f;
 #pragma unroll
-      for (int q = 0; q < 9; ++q) l1 += fabsf(g.h[q * N + n]);
+      for (int q = 0; q < 9; ++q) l1 += fabsf(pop_get(g.h, g, q * N + n));
       const bool regularize = (bounded == 0.0f || bounded == 1.0f) && (double)l1 > max_bulk_l1;
       if (regularize) {
         const float px = g.u[n] + 0.5f * g.a[n], py = g.u[N + n] + 0.5f * g.a[N + n];
@@ -958,19 +998,19 @@ __global__ void k2_clip_phase(icy2d_grid g, double max_bulk_l1) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
           const float r = phase_eq_rt(q, bounded, px, py);
-          g.h[q * N + n] = r;
+          pop_set(g.h, g, q * N + n, r);
           sum += r;
         }
-        g.h[n] += bounded - sum;
+        pop_add(g.h, g, n, bounded - sum);
       } else if (bounded != old_phase) {
         float sum = 0.0f;
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
-          const float lifted = g.h[q * N + n] + phase_eq_rt(q, bounded, ux, uy) - phase_eq_rt(q, old_phase, ux, uy);
-          g.h[q * N + n] = lifted;
+          const float lifted = pop_get(g.h, g, q * N + n) + phase_eq_rt(q, bounded, ux, uy) - phase_eq_rt(q, old_phase, ux, uy);
+          pop_set(g.h, g, q * N + n, lifted);
           sum += lifted;
         }
-        g.h[n] += bounded - sum;
+        pop_add(g.h, g, n, bounded - sum);
       }
       g.phi[n] = bounded;
       v[0] = (double)bounded;
@@ -998,28 +1038,27 @@ __global__ void k2_seed_projection(icy2d_grid g) {
         }
     }
   }
-  g.hbuf[n] = (float)seed;
-  g.hbuf[N + n] = 0.0f;
+  pop_set(g.hbuf, g, n, (float)seed);
+  pop_set(g.hbuf, g, N + n, 0.0f);
 }
 // simulator.py:4259-4289
 __global__ void k2_dilate_projection(icy2d_grid g, int primary_to_next) {
   const int n = cell_index(), N = cells(g);
   if (n >= N) return;
-  const float* src = primary_to_next ? g.hbuf : g.hbuf + N;
-  float* dst = primary_to_next ? g.hbuf + N : g.hbuf;
+  const int src = primary_to_next ? 0 : N, dst = primary_to_next ? N : 0;
   const int i = n / g.ny, j = n - i * g.ny;
   int value = 0;
   const float phase = g.phi[n];
   if (fluid_at(g, i, j) && g.cutoff < phase && phase < 1.0f - g.cutoff) {
-    value = (int)src[n];
+    value = (int)pop_get(g.hbuf, g, src + n);
     if (value == 0)
       for (int di = -1; di <= 1; ++di)
         for (int dj = -1; dj <= 1; ++dj) {
           const int ni = i + di, nj = j + dj;
-          if ((di || dj) && inside(g, ni, nj) && src[at(g, ni, nj)] == 1.0f) value = 1;
+          if ((di || dj) && inside(g, ni, nj) && pop_get(g.hbuf, g, src + at(g, ni, nj)) == 1.0f) value = 1;
         }
   }
-  dst[n] = (float)value;
+  pop_set(g.hbuf, g, dst + n, (float)value);
 }
 // simulator.py:4291-4351
 __global__ void k2_evaluate_projection(icy2d_grid g, double e) {
@@ -1030,7 +1069,7 @@ __global__ void k2_evaluate_projection(icy2d_grid g, double e) {
     if (fluid_at(g, i, j)) {
       const double phase = g.phi[n];
       double mapped = phase;
-      const bool adjustable = g.hbuf[n] == 1.0f;
+      const bool adjustable = pop_get(g.hbuf, g, n) == 1.0f;
       if (adjustable) mapped = phase * e / (1.0 - phase + phase * e);
       float m32 = (float)mapped;
       if (m32 <= g.cutoff) m32 = 0.0f;
@@ -1050,7 +1089,7 @@ __global__ void k2_apply_projection(icy2d_grid g, double e) {
     if (fluid_at(g, i, j)) {
       const float old_phase = g.phi[n];
       float new_phase = old_phase;
-      if (g.hbuf[n] == 1.0f) {
+      if (pop_get(g.hbuf, g, n) == 1.0f) {
         const double p = old_phase;
         new_phase = (float)(p * e / (1.0 - p + p * e));
       }
@@ -1065,7 +1104,7 @@ __global__ void k2_apply_projection(icy2d_grid g, double e) {
 __device__ __forceinline__ bool residual_free(const icy2d_grid& g, int n, double phase, double residual) {
   const double margin = 4.0 * 1.1920928955078125e-07;
   const double cutoff = g.cutoff;
-  if (!(g.hbuf[n] == 1.0f && cutoff < phase && phase < 1.0 - cutoff)) return false;
+  if (!(pop_get(g.hbuf, g, n) == 1.0f && cutoff < phase && phase < 1.0 - cutoff)) return false;
   const double magnitude = fabs(residual);
   if (residual > 0.0) return 1.0 - cutoff - phase > magnitude + margin;
   return phase - cutoff > magnitude + margin;
@@ -1193,7 +1232,7 @@ __global__ void k2_momentum_partials(icy2d_grid g) {
       double mx = 0.0, my = 0.0;
 #pragma unroll
       for (int q = 0; q < 9; ++q) {
-        const double fv = g.f[q * N + n];
+        const double fv = pop_get(g.f, g, q * N + n);
         if (cxq(q)) mx += (double)cxq(q) * fv;
         if (cyq(q)) my += (double)cyq(q) * fv;
       }
@@ -1212,16 +1251,38 @@ __global__ void k2_snapshot_body(icy2d_grid g) {
 __global__ void k2_wet_partials(icy2d_grid g) {
   const int n = cell_index();
   double v[1] = {0.0};
+  int lo = INT32_MAX, hi = -1;
   if (n < cells(g)) {
     const int i = n / g.ny, j = n - i * g.ny;
     const float phase = clamp01(g.phi[n]);
     if (fluid_at(g, i, j) && phase > g.cutoff) {
       v[0] = (double)phase;
-      atomicMin(&g.control->wet_target, n);
-      atomicMax(&g.control->wet_target_max, n);
+      lo = hi = n;
     }
   }
   block_partials<1>(v, g.partials);
+  // One atomic per block for the first and last wet cell (order-independent).
+  __shared__ int wave_lo[kBlock / 64], wave_hi[kBlock / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lo = min(lo, __shfl_xor(lo, off));
+    hi = max(hi, __shfl_xor(hi, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    wave_lo[threadIdx.x >> 6] = lo;
+    wave_hi[threadIdx.x >> 6] = hi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+      lo = min(lo, wave_lo[w]);
+      hi = max(hi, wave_hi[w]);
+    }
+    if (hi >= 0) {
+      atomicMin(&g.control->wet_target, lo);
+      atomicMax(&g.control->wet_target_max, hi);
+    }
+  }
 }
 __global__ void k2_reset_wet(icy2d_grid g) {
   g.control->wet_target = g.nx * g.ny;
@@ -1281,7 +1342,7 @@ __device__ __forceinline__ void add_momentum(const icy2d_grid& g, int n, double 
   g.u[n] += mx / rho;
   g.u[N + n] += my / rho;
 #pragma unroll
-  for (int q = 0; q < 9; ++q) g.f[q * N + n] += 3.0f * wq(q) * ((float)cxq(q) * mx + (float)cyq(q) * my);
+  for (int q = 0; q < 9; ++q) pop_add(g.f, g, q * N + n, 3.0f * wq(q) * ((float)cxq(q) * mx + (float)cyq(q) * my));
 }
 // simulator.py:2842-2884
 __global__ void k2_apply_melt_correction(icy2d_grid g) {
@@ -1388,6 +1449,20 @@ __global__ void k2_aperture_partials(icy2d_grid g) {
       r = k == 4 ? fmin(r, part[k][w]) : k == 5 ? fmax(r, part[k][w]) : r + part[k][w];
     g.partials[blockIdx.x * 6 + k] = r;
   }
+}
+// simulator.py:522-540: the thermal volume must stay within [0, capacity].
+__global__ void k2_check_aperture(icy2d_grid g) {
+  icy2d_control* ctl = g.control;
+  const double total = ctl->aperture[0], full = ctl->aperture[3];
+  const double tol = 1.0e-12 * fmax(g.area, fmax(fabs(total), fabs(full)));
+  int flags = 0;
+  if (total < -tol) flags |= 1;
+  if (total > full + tol) flags |= 2;
+  if (flags && !ctl->error_flags) {
+    ctl->error_values[0] = total;
+    ctl->error_values[1] = full;
+  }
+  ctl->error_flags |= flags;
 }
 __device__ __forceinline__ void aperture_scales(const icy2d_control* ctl, double& base_scale, double& spare_scale) {
   const double total = fmax(0.0, ctl->aperture[0]), base = ctl->aperture[2], full = ctl->aperture[3];
@@ -1794,7 +1869,7 @@ int launch_status() {
 // Launch a partials-producing kernel's finish into out[0..K).
 int finish(const icy2d_grid& g, unsigned blocks, int K, double* out, unsigned min_mask, unsigned max_mask,
            unsigned add_mask, hipStream_t s) {
-  k2_finish<<<1, kBlock, 0, s>>>(g.partials, (int)blocks, K, out, min_mask, max_mask, add_mask);
+  k2_finish<<<K, kFinishThreads, 0, s>>>(g.partials, (int)blocks, K, out, min_mask, max_mask, add_mask);
   return launch_status();
 }
 bool fits(const icy2d_grid& g, unsigned blocks, int K) { return (long)blocks * K + 8 <= (long)g.partials_capacity; }
@@ -1916,9 +1991,10 @@ int icy2d_prepare_cut_links(const icy2d_grid* g, void* stream) {
   if (int rc = launch_status()) return rc;
   return finish(*g, blocks, 3, g->control->impulse, 0u, 0u, 0x7u, S(stream));
 }
-int icy2d_stream(const icy2d_grid* g, int phase_only, void* stream) {
+int icy2d_stream(const icy2d_grid* g, int phase_only, int mode, void* stream) {
   ICY2D_GRID_CHECK(g);
-  k2_stream<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, phase_only);
+  if (mode < 1 || mode > 3) return ICY2D_ERR_ARGUMENT;
+  k2_stream<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, phase_only, mode);
   return launch_status();
 }
 int icy2d_update_pressure(const icy2d_grid* g, void* stream) {
@@ -2050,7 +2126,9 @@ int icy2d_measure_aperture(const icy2d_grid* g, void* stream) {
   if (!fits(*g, blocks, 6)) return ICY2D_ERR_ARGUMENT;
   k2_aperture_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
   if (int rc = launch_status()) return rc;
-  return finish(*g, blocks, 6, g->control->aperture, 0x10u, 0x20u, 0u, S(stream));
+  if (int rc = finish(*g, blocks, 6, g->control->aperture, 0x10u, 0x20u, 0u, S(stream))) return rc;
+  k2_check_aperture<<<1, 1, 0, S(stream)>>>(*g);
+  return launch_status();
 }
 int icy2d_build_aperture_targets(const icy2d_grid* g, void* stream) {
   ICY2D_GRID_CHECK(g);

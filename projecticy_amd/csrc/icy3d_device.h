@@ -282,6 +282,29 @@ __device__ __forceinline__ void atomic_max_nonneg(float* address, float value) {
 
 // Block-wide sum (blockDim.x a multiple of 64, at most 1024); every thread
 // of the block must call it.  The result is valid in thread 0 only.
+// Block-wide maximum / minimum; the result is valid in thread 0.
+__device__ __forceinline__ float block_max(float v) {
+  __shared__ float wave_part[16];
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0) wave_part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  float r = v;
+  if (threadIdx.x == 0)
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = fmaxf(r, wave_part[w]);
+  __syncthreads();
+  return r;
+}
+__device__ __forceinline__ int block_min_int(int v) {
+  __shared__ int wave_part[16];
+  v = wave_min_int(v);
+  if ((threadIdx.x & 63) == 0) wave_part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int r = v;
+  if (threadIdx.x == 0)
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = min(r, wave_part[w]);
+  __syncthreads();
+  return r;
+}
 __device__ __forceinline__ float block_sum(float v) {
   __shared__ float wave_part[16];
   v = wave_sum(v);

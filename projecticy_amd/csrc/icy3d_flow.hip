@@ -550,7 +550,33 @@ __global__ void k_recover_phase(icy3d_grid g, int parity, int recover_flow) {
   if (!isfinite(clip)) clip = 0.0f;
   const float block_clip = block_sum(clip);
   if (threadIdx.x == 0) g.partials[blockIdx.x] = block_clip;
-  if (recover_flow && g.check_stability) commit_stability(g, st);
+  if (recover_flow && g.check_stability) {
+    // Per-block maximum speed and first invalid index (exact in FP32 below
+    // 2^24 cells), folded by k_fold_stability.
+    const float speed = block_max(st.speed);
+    const int invalid = block_min_int(st.invalid);
+    if (threadIdx.x == 0) {
+      g.partials[gridDim.x + blockIdx.x] = speed;
+      g.partials[2 * gridDim.x + blockIdx.x] = invalid == INT32_MAX ? -1.0f : (float)invalid;
+    }
+  }
+}
+// Folds the per-block stability partials of k_recover_phase into the control block.
+__global__ void __launch_bounds__(1024) k_fold_stability(icy3d_grid g, int blocks) {
+  float speed = 0.0f;
+  int invalid = INT32_MAX;
+  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+    speed = fmaxf(speed, g.partials[blocks + b]);
+    const float v = g.partials[2 * blocks + b];
+    if (v >= 0.0f) invalid = min(invalid, (int)v);
+  }
+  speed = block_max(speed);
+  invalid = block_min_int(invalid);
+  if (threadIdx.x == 0) {
+    g.control->maximum_speed = speed;
+    if (invalid != INT32_MAX) g.control->invalid_index = min(g.control->invalid_index, invalid);
+    g.control->peak_speed = fmaxf(g.control->peak_speed, speed);
+  }
 }
 
 __global__ void k_recover_pressure(icy3d_grid g) {
@@ -897,10 +923,14 @@ __device__ __forceinline__ void forces_cell(const icy3d_grid& g, int n, int with
 constexpr int kForceJ = 4, kForceK = 64, kForceX = 16;
 constexpr int kForceHJ = kForceJ + 2, kForceHK = kForceK + 2, kForceSlab = kForceHJ * kForceHK;
 constexpr int kForceRounds = (kForceSlab + kForceJ * kForceK - 1) / (kForceJ * kForceK);
-constexpr int kForceQ = 8;  // Ax, Ay, Az, Bxy, Bxz, Byz, X1, X2
+// Staged per cell: Ax, Ay, Az, Bxy, Bxz, Byz, X1, X2 (stencil sums), then
+// u (3), phi and the mass flux (3) for the face-neighbour terms.
+constexpr int kForceQ = 15;
+constexpr int kStencilQ = 8, kU = 8, kPhi = 11, kFlux = 12;
 
+// Raw values prefetched for one staged cell: grad (3), phi, pressure field, u (3), flux (3).
 struct ForceFetch {
-  float q[kForceQ];
+  float grad[3], phi, p, u[3], flux[3];
 };
 __device__ __forceinline__ void fetch_force_cell(const icy3d_grid& g, int i, int j, int k, int moment_pressure,
                                                  ForceFetch& f) {
@@ -909,23 +939,38 @@ __device__ __forceinline__ void fetch_force_cell(const icy3d_grid& g, int i, int
   k = min(max(k, 0), g.nz - 1);
   const size_t N = cells(g);
   const int n = lin(g, i, j, k);
-  const float gx = g.grad[n], gy = g.grad[N + n], gz = g.grad[2 * N + n];
-  f.q[0] = gy * gy + gz * gz;
-  f.q[1] = gx * gx + gz * gz;
-  f.q[2] = gx * gx + gy * gy;
-  f.q[3] = gx * gy;
-  f.q[4] = gx * gz;
-  f.q[5] = gy * gz;
-  const float rho = density(g, g.phi[n]);
-  if (moment_pressure) {
-    const float p = g.home[n];
-    f.q[6] = p;
-    f.q[7] = p * rho;
-  } else {
-    const float p = g.pressure[n];
-    f.q[6] = 3.0f * p / rho;
-    f.q[7] = 3.0f * p;
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    f.grad[e] = g.grad[e * N + n];
+    f.u[e] = g.u[e * N + n];
+    f.flux[e] = g.flux[e * N + n];
   }
+  f.phi = g.phi[n];
+  f.p = moment_pressure ? g.home[n] : g.pressure[n];
+}
+__device__ __forceinline__ void stage_force_cell(const icy3d_grid& g, float (*sq)[kForceSlab], int t, int moment_pressure,
+                                                 const ForceFetch& f) {
+  const float gx = f.grad[0], gy = f.grad[1], gz = f.grad[2];
+  sq[0][t] = gy * gy + gz * gz;
+  sq[1][t] = gx * gx + gz * gz;
+  sq[2][t] = gx * gx + gy * gy;
+  sq[3][t] = gx * gy;
+  sq[4][t] = gx * gz;
+  sq[5][t] = gy * gz;
+  const float rho = density(g, f.phi);
+  if (moment_pressure) {
+    sq[6][t] = f.p;
+    sq[7][t] = f.p * rho;
+  } else {
+    sq[6][t] = 3.0f * f.p / rho;
+    sq[7][t] = 3.0f * f.p;
+  }
+#pragma unroll
+  for (int e = 0; e < 3; ++e) {
+    sq[kU + e][t] = f.u[e];
+    sq[kFlux + e][t] = f.flux[e];
+  }
+  sq[kPhi][t] = f.phi;
 }
 struct SlabParts {
   float px[5], qy[5], qz[5];  // D_x parts of Ax Bxy Bxz X1 X2; D_y of Ay Bxy Byz X1 X2; D_z of Az Bxz Byz X1 X2
@@ -971,21 +1016,15 @@ __device__ __forceinline__ SlabParts slab_parts(const float (*sq)[kForceSlab], i
   r.qz[4] = Qz;
   return r;
 }
+// Own-column values carried between slabs.
 struct ColumnCell {
   float u[3], flux_x, phi;
 };
-__device__ __forceinline__ ColumnCell load_column(const icy3d_grid& g, int x, int j, int k) {
-  ColumnCell c = {{0.0f, 0.0f, 0.0f}, 0.0f, 0.0f};
-  if (x < 0 || x >= g.nx || j >= g.ny || k >= g.nz) return c;
-  const size_t N = cells(g);
-  const int n = lin(g, x, j, k);
-  c.u[0] = g.u[n];
-  c.u[1] = g.u[N + n];
-  c.u[2] = g.u[2 * N + n];
-  c.flux_x = g.flux[n];
-  c.phi = g.phi[n];
-  return c;
-}
+// Terms of the y and z faces, formed while the cell's own slab is in LDS:
+// the donor-flux force of those four faces and the y/z derivatives of u.
+struct FaceTerms {
+  float f[3], dudy[3], dudz[3];
+};
 // Donor phase-flux momentum term of one face (simulator.py:499-519).
 __device__ __forceinline__ void face_flux(const icy3d_grid& g, float d, float flux_own, float flux_nb, float rho,
                                           float phi_nb, const float u_own[3], const float u_nb[3], float f[3]) {
@@ -1005,6 +1044,35 @@ __device__ __forceinline__ void face_flux(const icy3d_grid& g, float d, float fl
   f[1] -= d * fl[1];
   f[2] -= d * fl[2];
 }
+__device__ __forceinline__ FaceTerms face_terms(const icy3d_grid& g, const float (*sq)[kForceSlab], int centre,
+                                                int with_flux) {
+  FaceTerms t = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+  const float u_own[3] = {sq[kU][centre], sq[kU + 1][centre], sq[kU + 2][centre]};
+  const float rho = density(g, sq[kPhi][centre]);
+  // faces -y, +y, -z, +z
+  const int off[4] = {-kForceHK, kForceHK, -1, 1};
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int comp = 1 + a;  // flux component along y or z
+    const float flux_own = sq[kFlux + comp][centre];
+    float un[2][3];
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int m = centre + off[2 * a + side];
+#pragma unroll
+      for (int e = 0; e < 3; ++e) un[side][e] = sq[kU + e][m];
+      if (with_flux)
+        face_flux(g, side == 0 ? -1.0f : 1.0f, flux_own, sq[kFlux + comp][m], rho, sq[kPhi][m], u_own, un[side], t.f);
+    }
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+      const float d = 0.5f * (un[1][e] - un[0][e]);
+      if (a == 0) t.dudy[e] = d;
+      else t.dudz[e] = d;
+    }
+  }
+  return t;
+}
 
 __global__ void __launch_bounds__(kForceJ* kForceK) k_forces_tiled(icy3d_grid g, int with_flux, int moment_pressure) {
   __shared__ float sq[kForceQ][kForceSlab];
@@ -1014,7 +1082,8 @@ __global__ void __launch_bounds__(kForceJ* kForceK) k_forces_tiled(icy3d_grid g,
   const int x_end = min(x0 + kForceX, g.nx);
   const int j = j0 + tj, k = k0 + tk;
   const bool mine = j < g.ny && k < g.nz;
-  const int base = tj * kForceHK + tk;  // top-left of the thread's 3x3 (j, k) window
+  const int base = tj * kForceHK + tk;            // top-left of the thread's 3x3 (j, k) window
+  const int centre = base + kForceHK + 1;
   ForceFetch fetched[kForceRounds];
 #pragma unroll
   for (int r = 0; r < kForceRounds; ++r) {
@@ -1022,17 +1091,15 @@ __global__ void __launch_bounds__(kForceJ* kForceK) k_forces_tiled(icy3d_grid g,
     if (t < kForceSlab) fetch_force_cell(g, x0 - 1, j0 - 1 + t / kForceHK, k0 - 1 + t % kForceHK, moment_pressure, fetched[r]);
   }
   SlabParts prev2 = {}, prev1 = {};
-  ColumnCell col_m1 = load_column(g, x0 - 2, j, k), col_0 = load_column(g, x0 - 1, j, k);
+  ColumnCell col_m1 = {}, col_0 = {};
+  FaceTerms faces_0 = {};
   const float kcap = 4.5f * g.sigma * g.width;
   for (int s = x0 - 1; s <= x_end; ++s) {
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kForceRounds; ++r) {
       const int t = threadIdx.x + r * blockDim.x;
-      if (t < kForceSlab) {
-#pragma unroll
-        for (int m = 0; m < kForceQ; ++m) sq[m][t] = fetched[r].q[m];
-      }
+      if (t < kForceSlab) stage_force_cell(g, sq, t, moment_pressure, fetched[r]);
     }
     if (s < x_end) {
 #pragma unroll
@@ -1044,14 +1111,19 @@ __global__ void __launch_bounds__(kForceJ* kForceK) k_forces_tiled(icy3d_grid g,
     }
     __syncthreads();
     const SlabParts cur = slab_parts(sq, base);
-    const ColumnCell col_p1 = load_column(g, s, j, k);  // the x+1 neighbour of cell s - 1
+    ColumnCell col_p1;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) col_p1.u[e] = sq[kU + e][centre];
+    col_p1.flux_x = sq[kFlux][centre];
+    col_p1.phi = sq[kPhi][centre];
+    const FaceTerms faces_p1 = face_terms(g, sq, centre, with_flux);
     const int x = s - 1;
     if (x >= x0 && mine) {
       const int n = lin(g, x, j, k);
       if (g.flag[n] == 0) {
         const float sixth = 1.0f / 6.0f;
         const float phi = col_0.phi, rho = density(g, phi);
-        float f[3] = {0.0f, (rho - g.ref_density[j]) * g.gravity_y, 0.0f};
+        float f[3] = {faces_0.f[0], (rho - g.ref_density[j]) * g.gravity_y + faces_0.f[1], faces_0.f[2]};
         f[1] -= fmaxf(0.0f, 2.0f * phi - 1.0f) * g.expansion * (water_temperature(g, n) - g.water_t0) * g.gravity_y;
         // x parts: D_x = (P(x+1) - P(x-1)) / 6; y, z parts: S_x = (Q(x-1) + 4 Q(x) + Q(x+1)) / 6
         float dx[5], dy[5], dz[5];
@@ -1066,35 +1138,9 @@ __global__ void __launch_bounds__(kForceJ* kForceK) k_forces_tiled(icy3d_grid g,
           f[1] += kcap * (dy[0] - dx[1] - dz[2]);
           f[2] += kcap * (dz[0] - dx[2] - dy[2]);
         }
-        const float u_own[3] = {col_0.u[0], col_0.u[1], col_0.u[2]};
-        // face neighbours: -x, +x from the pipeline, -y, +y, -z, +z from memory
-        float un[6][3];
-        float phin[6], fluxn[6];
-#pragma unroll
-        for (int e = 0; e < 3; ++e) {
-          un[0][e] = col_m1.u[e];
-          un[1][e] = col_p1.u[e];
-        }
-        phin[0] = col_m1.phi;
-        phin[1] = col_p1.phi;
-        fluxn[0] = col_m1.flux_x;
-        fluxn[1] = col_p1.flux_x;
-        const int nb[4] = {n - g.nz, n + g.nz, n - 1, n + 1};
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          const int m = nb[t];
-#pragma unroll
-          for (int e = 0; e < 3; ++e) un[2 + t][e] = g.u[e * N + m];
-          phin[2 + t] = g.phi[m];
-          fluxn[2 + t] = g.flux[(1 + t / 2) * N + m];
-        }
         if (with_flux) {
-#pragma unroll
-          for (int a = 0; a < 3; ++a) {
-            const float flux_own = g.flux[a * N + n];
-            face_flux(g, -1.0f, flux_own, fluxn[2 * a], rho, phin[2 * a], u_own, un[2 * a], f);
-            face_flux(g, 1.0f, flux_own, fluxn[2 * a + 1], rho, phin[2 * a + 1], u_own, un[2 * a + 1], f);
-          }
+          face_flux(g, -1.0f, col_0.flux_x, col_m1.flux_x, rho, col_m1.phi, col_0.u, col_m1.u, f);
+          face_flux(g, 1.0f, col_0.flux_x, col_p1.flux_x, rho, col_p1.phi, col_0.u, col_p1.u, f);
         }
         // home.py:127-197
         const float inv_rho = 1.0f / rho;
@@ -1107,9 +1153,11 @@ __global__ void __launch_bounds__(kForceJ* kForceK) k_forces_tiled(icy3d_grid g,
         if (gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2] > 0.0f) {
           float G[3][3];
 #pragma unroll
-          for (int axis = 0; axis < 3; ++axis)
-#pragma unroll
-            for (int e = 0; e < 3; ++e) G[e][axis] = 0.5f * (un[2 * axis + 1][e] - un[2 * axis][e]);
+          for (int e = 0; e < 3; ++e) {
+            G[e][0] = 0.5f * (col_p1.u[e] - col_m1.u[e]);
+            G[e][1] = faces_0.dudy[e];
+            G[e][2] = faces_0.dudz[e];
+          }
           const float coef = lattice_nu(g, n) / rho;
 #pragma unroll
           for (int e = 0; e < 3; ++e)
@@ -1126,6 +1174,7 @@ __global__ void __launch_bounds__(kForceJ* kForceK) k_forces_tiled(icy3d_grid g,
     prev1 = cur;
     col_m1 = col_0;
     col_0 = col_p1;
+    faces_0 = faces_p1;
   }
 }
 
@@ -1815,10 +1864,11 @@ int launch_recover(const icy3d_grid& g, int parity, int recover_flow, hipStream_
   bool scan = recover_flow && g.check_stability;
   const unsigned blocks = icy_blocks(grid_cells(g));
   if ((int)blocks > g.partials_capacity) return ICY3D_ERR_ARGUMENT;
+  if (scan && 3 * (int)blocks > g.partials_capacity) return ICY3D_ERR_ARGUMENT;
   if (scan) k_reset_stability<<<1, 1, 0, s>>>(g);
   k_recover_phase<<<blocks, kBlock, 0, s>>>(g, parity, recover_flow);
   k_finish_sum<<<1, kFinishThreads, 0, s>>>(g.partials, (int)blocks, &g.control->phase_clip_volume);
-  if (scan) k_fold_peak<<<1, 1, 0, s>>>(g);
+  if (scan) k_fold_stability<<<1, 1024, 0, s>>>(g, (int)blocks);
   ICY_LAUNCH_CHECK();
   return 0;
 }

@@ -20,6 +20,7 @@ from ..iceflow3d.fields import DeviceField
 from .config import BOUNDARY_CODE, IceFlowConfig, LatticeScales, MovingBodyThermalTotals
 
 MAX_BULK_PHASE_L1 = 2.0          # simulator.py:45
+POPULATION_SCALE = 1024.0         # FP16 population storage holds population * 2^10
 MAX_LATTICE_VELOCITY_L1 = 1.0    # simulator.py:44
 
 
@@ -68,6 +69,27 @@ class ControlValue:
 
     def fill(self, value):
         self[None] = value
+
+
+class ScaledPopulationField(DeviceField):
+    """View of FP16 population planes that hold population * scale."""
+
+    def __init__(self, tensor, scale, **kwargs):
+        super().__init__(tensor, logical_dtype=np.float32, **kwargs)
+        self.scale = scale
+
+    def to_numpy(self):
+        array = self.t.detach().float().cpu().numpy() / np.float32(self.scale)
+        array = np.moveaxis(array, 0, -1)
+        return np.ascontiguousarray(array.astype(np.float32))
+
+    def from_numpy(self, value):
+        value = np.broadcast_to(np.asarray(value, dtype=np.float32), self.shape)
+        physical = np.moveaxis(value, -1, 0) * np.float32(self.scale)
+        self.t.copy_(torch.from_numpy(np.ascontiguousarray(physical)).to(torch.float16))
+
+    def fill(self, value):
+        self.from_numpy(np.full(self.shape, float(value), dtype=np.float32))
 
 
 class DerivedValue:
@@ -138,7 +160,10 @@ class Thermal2D:
 class IceFlow2D:
     """MI355X solver for the coupled falling-ice melting scenario."""
 
-    def __init__(self, config):
+    def __init__(self, config, *, population_storage="fp32"):
+        """``population_storage="fp16"`` keeps the four population buffers in
+        FP16 (scaled by 2^10) with FP32 arithmetic; it has no reference
+        counterpart.  Every other argument and check is the reference's."""
         # simulator.py:1829-1843
         if not isinstance(config, IceFlowConfig):
             raise TypeError("config must be an IceFlowConfig")
@@ -152,6 +177,9 @@ class IceFlow2D:
             raise ValueError("IceFlow2D requires well-balanced hydrostatics")
         if config.thermal.water_buoyancy_model != "linear":
             raise ValueError("IceFlow2D supports only the linear buoyancy model")
+        if population_storage not in ("fp32", "fp16"):
+            raise ValueError("population_storage must be 'fp32' or 'fp16'")
+        self.population_storage = population_storage
         self.device = _require_device()
         self._lib = native.lib()
         self.config = config
@@ -225,8 +253,9 @@ class IceFlow2D:
         M = self.config.ice_width * self.config.ice_height
         dev = self.device
         f32, f64, i8, i32 = torch.float32, torch.float64, torch.int8, torch.int32
+        pop = torch.float16 if self.population_storage == "fp16" else f32
         shapes = {
-            "f": (9 * N, f32), "fbuf": (9 * N, f32), "h": (9 * N, f32), "hbuf": (9 * N, f32),
+            "f": (9 * N, pop), "fbuf": (9 * N, pop), "h": (9 * N, pop), "hbuf": (9 * N, pop),
             "phi": (N, f32), "u": (2 * N, f32), "a": (2 * N, f32), "p": (N, f32), "moving": (N, f32),
             "p_ref": (ny, f32), "rho_ref": (ny, f32), "solid": (N, i8), "prev_solid": (N, i8), "eligible": (N, i8),
             "coverage": (N, f32), "body_mass": (M, f64), "body_energy": (M, f64), "body_heat": (M, f64),
@@ -245,6 +274,9 @@ class IceFlow2D:
         for name in native.GRID_POINTERS:
             setattr(g, name, self._t[name].data_ptr())
         g.partials_capacity = capacity
+        g.half_populations = int(self.population_storage == "fp16")
+        g.population_scale = POPULATION_SCALE
+        g.population_inverse_scale = 1.0 / POPULATION_SCALE
         g.control = self._control_t.data_ptr()
         self._gp = ctypes.pointer(g)
 
@@ -256,7 +288,20 @@ class IceFlow2D:
     def _read_control(self):
         self._control_host.copy_(self._control_t, non_blocking=True)
         self._stream_handle.synchronize()
-        return native.Control2D.from_buffer_copy(self._control_host.numpy().tobytes())
+        control = native.Control2D.from_buffer_copy(self._control_host.numpy().tobytes())
+        if control.error_flags:
+            self._raise_aperture_error(control)
+        return control
+
+    def _raise_aperture_error(self, control):
+        """simulator.py:529-540, checked on the device and raised at the next host read."""
+        total, full = control.error_values[0], control.error_values[1]
+        control.error_flags = 0
+        self._write_control(control)
+        if total < 0.0:
+            raise RuntimeError(f"thermal water advection produced a negative total volume: {total:.17g} m^2")
+        raise RuntimeError("thermal water volume exceeds the current fluid aperture capacity: "
+                           f"volume={total:.17g} m^2, capacity={full:.17g} m^2")
 
     def _write_control(self, control):
         raw = torch.frombuffer(bytearray(bytes(control)), dtype=torch.uint8)
@@ -264,10 +309,16 @@ class IceFlow2D:
 
     def _make_fields(self):
         nx, ny, t = self.nx, self.ny, self._t
-        self.momentum_populations = DeviceField(t["f"].view(9, nx, ny), components=9)
-        self.momentum_stream_buffer = DeviceField(t["fbuf"].view(9, nx, ny), components=9)
-        self.phase_populations = DeviceField(t["h"].view(9, nx, ny), components=9)
-        self.phase_stream_buffer = DeviceField(t["hbuf"].view(9, nx, ny), components=9)
+        if self.population_storage == "fp16":
+            def pop_field(name):
+                return ScaledPopulationField(t[name].view(9, nx, ny), POPULATION_SCALE, components=9)
+        else:
+            def pop_field(name):
+                return DeviceField(t[name].view(9, nx, ny), components=9)
+        self.momentum_populations = pop_field("f")
+        self.momentum_stream_buffer = pop_field("fbuf")
+        self.phase_populations = pop_field("h")
+        self.phase_stream_buffer = pop_field("hbuf")
         self.water_phase = DeviceField(t["phi"].view(nx, ny))
         self.momentum_velocity_lattice = DeviceField(t["u"].view(2, nx, ny), components=2)
         self.fluid_acceleration_lattice = DeviceField(t["a"].view(2, nx, ny), components=2)
@@ -464,7 +515,7 @@ class IceFlow2D:
         """simulator.py:3827-3831."""
         for _ in range(int(steps)):
             self._call("collide", 0)
-            self._call("stream", 1)
+            self._call("stream", 1, 3)
 
     def _build_hydrostatic_reference(self):
         """simulator.py:2626-2662 (host FP64, from the device phase and masks)."""
@@ -518,7 +569,10 @@ class IceFlow2D:
         self._call("prepare_cut_links")
 
     def _stream(self):
-        self._call("stream", 0)
+        self._call("stream", 0, 1)
+
+    def _update_streamed_macroscopic_fields(self):
+        self._call("stream", 0, 2)
 
     def _update_pressure(self):
         self._call("update_pressure")
@@ -539,7 +593,7 @@ class IceFlow2D:
             raise ValueError("num_steps must be a non-negative integer")
         for _ in range(int(num_steps)):
             self._call("collide", 1)
-            self._call("stream", 0)
+            self._call("stream", 0, 3)
             self._call("update_pressure")
             self._call("integrate_rigid")
             self._solve_contact_and_rasterize()
@@ -554,6 +608,8 @@ class IceFlow2D:
             self._finalize_melt_momentum_coupling()
             self._synchronize_water_aperture()
             self.steps += 1
+        if num_steps:
+            self._read_control()  # raises a pending device-side check of the last step
 
     def _substeps(self, load, process, dt, outflow=None):
         """simulator.py:342-401."""
@@ -639,16 +695,8 @@ class IceFlow2D:
             raise RuntimeError("melt water has no legal LBM water cell for conservative mass and energy injection")
 
     def _synchronize_water_aperture(self):
-        """simulator.py:511-548."""
+        """simulator.py:511-548 (the capacity check runs on the device)."""
         self._call("measure_aperture")
-        control = self._read_control()
-        total, full = control.aperture[0], control.aperture[3]
-        tolerance = 1.0e-12 * max(self._g.area, abs(total), abs(full))
-        if total < -tolerance:
-            raise RuntimeError(f"thermal water advection produced a negative total volume: {total:.17g} m^2")
-        if total > full + tolerance:
-            raise RuntimeError("thermal water volume exceeds the current fluid aperture capacity: "
-                               f"volume={total:.17g} m^2, capacity={full:.17g} m^2")
         self._call("build_aperture_targets")
         self._call("apply_aperture_transfer")
         self._call("finish_aperture_transfer")
