@@ -16,7 +16,11 @@ The JSON line adds:
                 step of the same workload after its initialisation;
   d2q9          the coupled D2Q9 step on the 1024x2048 falling-ice case
                 (FP32 populations, FP64 thermal), timed after the headline,
-                with its own CPU baseline (the 2D oracle on the 100x200 case).
+                with its own CPU baseline (the 2D oracle on the 100x200 case);
+  ensemble      64 uneven D2Q9 cases (widths 64-512) stepped as one batch
+                with FP16 populations: cases/hour for 10,000-step cases,
+                the same figure for one-case-at-a-time FP32 runs, and the
+                melt-mass check of the FP16 batch against FP32 runs.
 """
 
 import argparse
@@ -55,6 +59,12 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--d2q9-steps", type=int, default=200,
                    help="steps of the 1024x2048 D2Q9 case timed after the headline (0 skips it)")
+    p.add_argument("--ensemble-cases", type=int, default=64)
+    p.add_argument("--ensemble-steps", type=int, default=400,
+                   help="timed steps of the batched FP16 D2Q9 ensemble (0 skips it)")
+    p.add_argument("--ensemble-gate-steps", type=int, default=10000,
+                   help="steps of the melt-mass check: FP16 batch vs FP32 standalone runs of one case per width "
+                        "(0 skips it)")
     return p.parse_args()
 
 
@@ -93,6 +103,109 @@ def d2q9_measurement(args):
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = d2q9_cpu_baseline()
     return result
+
+
+ENSEMBLE_CASE_STEPS = 10000  # steps of one ensemble case
+
+
+def ensemble_measurement(args):
+    """64 uneven D2Q9 cases, FP16 populations, one launch per kernel for all
+    cases; cases/hour against one-case-at-a-time FP32 runs."""
+    import torch
+
+    from projecticy_amd.iceflow2d import IceFlow2D
+    from projecticy_amd.iceflow2d.ensemble import IceFlowEnsemble2D
+    from projecticy_amd.iceflow2d.workloads import ensemble_cases
+
+    configs = ensemble_cases(args.ensemble_cases, seed=args.seed)
+    t0 = time.perf_counter()
+    ensemble = IceFlowEnsemble2D(configs, population_storage="fp16")
+    init_s = time.perf_counter() - t0
+    tuning = ensemble.autotune()
+    ensemble.step(8)
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    ensemble.step(args.ensemble_steps)
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - start
+    per_step = seconds / args.ensemble_steps
+    cells = ensemble.cells
+    del ensemble
+    torch.cuda.empty_cache()
+    # plain form: each case alone, FP32 populations, one width at a time
+    plain_step = {}
+    for config in configs:
+        if config.nx in plain_step:
+            continue
+        sim = IceFlow2D(config)
+        sim.step(8)
+        torch.cuda.synchronize()
+        begin = time.perf_counter()
+        sim.step(16)
+        torch.cuda.synchronize()
+        plain_step[config.nx] = (time.perf_counter() - begin) / 16
+        del sim
+    plain_case_s = sum(plain_step[c.nx] * ENSEMBLE_CASE_STEPS for c in configs)
+    batch_cases_hour = len(configs) * 3600.0 / (per_step * ENSEMBLE_CASE_STEPS)
+    plain_cases_hour = len(configs) * 3600.0 / plain_case_s
+    print(f"[bench] ensemble {len(configs)} cases, {cells / 1e6:.2f} M cells: init {init_s:.1f} s, "
+          f"{args.ensemble_steps} steps in {seconds:.3f} s, {batch_cases_hour:.1f} cases/h "
+          f"(plain {plain_cases_hour:.1f})", file=sys.stderr, flush=True)
+    result = {
+        "workload": f"{len(configs)} D2Q9 falling-ice cases, widths 64-512 (height 2x), seed {args.seed}, "
+                    f"FP16 populations, {ENSEMBLE_CASE_STEPS} steps per case",
+        "value": round(batch_cases_hour, 2), "unit": "cases/hour", "cells": cells,
+        "steps_timed": args.ensemble_steps, "ms_per_step": round(per_step * 1e3, 4),
+        "mlups": round(cells * args.ensemble_steps / seconds / 1e6, 2),
+        "plain_fp32_cases_per_hour": round(plain_cases_hour, 2),
+        "speedup_vs_plain": round(batch_cases_hour / plain_cases_hour, 3),
+        "threads": {"collide": tuning and min(tuning, key=tuning.get)[0],
+                    "stream": tuning and min(tuning, key=tuning.get)[1]},
+        "dtype": "fp16 populations, fp32 compute, fp64 thermal",
+    }
+    if args.ensemble_gate_steps > 0:
+        result["melt_mass_check"] = ensemble_gate(configs, args.ensemble_gate_steps)
+    return result
+
+
+def ensemble_gate(configs, steps):
+    """Melt mass after ``steps`` steps: one case of each width run in one
+    FP16 batch against the same cases run alone with FP32 populations."""
+    import torch
+
+    from projecticy_amd.iceflow2d import IceFlow2D
+    from projecticy_amd.iceflow2d.ensemble import IceFlowEnsemble2D
+
+    picked, widths = [], set()
+    for config in configs:
+        if config.nx not in widths:
+            widths.add(config.nx)
+            picked.append(config)
+    batch = IceFlowEnsemble2D(picked, population_storage="fp16")
+    start = time.perf_counter()
+    batch.step(steps)
+    torch.cuda.synchronize()
+    batch_s = time.perf_counter() - start
+    fp16 = [t.melted_mass_kg_m for t in batch.totals()]
+    del batch
+    fp32 = []
+    for config in picked:
+        sim = IceFlow2D(config)
+        sim.step(steps)
+        fp32.append(sim.thermal.mass_energy_totals().melted_mass_kg_m)
+        del sim
+    errors = [abs(a - b) / b if b > 0.0 else None for a, b in zip(fp16, fp32)]
+    measured = [e for e in errors if e is not None]
+    print(f"[bench] ensemble melt-mass check: {len(picked)} cases x {steps} steps ({batch_s:.1f} s batched), "
+          f"max relative error {max(measured) if measured else float('nan'):.3e}", file=sys.stderr, flush=True)
+    return {
+        "steps": steps, "cases": [c.nx for c in picked],
+        "melted_kg_m_fp16_batch": fp16, "melted_kg_m_fp32_plain": fp32,
+        "relative_error": errors,
+        "max_relative_error": max(measured) if measured else None,
+        "within_1pct": bool(measured) and max(measured) <= 0.01,
+        "reference": "same cases run alone with FP32 populations (the fine 2x solver is not run here)",
+    }
 
 
 def d2q9_cpu_baseline():
@@ -213,6 +326,9 @@ def main():
     torch.cuda.empty_cache()
     if args.d2q9_steps > 0:
         result["d2q9"] = d2q9_measurement(args)
+        torch.cuda.empty_cache()
+    if args.ensemble_steps > 0:
+        result["ensemble"] = ensemble_measurement(args)
     print(json.dumps(result), flush=True)
 
 

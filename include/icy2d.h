@@ -165,6 +165,89 @@ int icy2d_reduce_totals(const icy2d_grid* g, void* stream);                /* :1
 /* Binding check: sizeof(icy2d_grid), sizeof(icy2d_control). */
 void icy2d_abi_sizes(int32_t* grid, int32_t* control);
 
+/* Ensembles ---------------------------------------------------------------
+ * No counterpart in the reference, which runs one case per process.  An
+ * icy2d_batch covers many independent cases of any sizes (each with its
+ * own icy2d_grid, buffers and control block); every icy2d_batch_* entry
+ * point does what the single-case entry point of the same name does, for
+ * every case, in one launch per kernel: block b of a launch belongs to the
+ * case c with first_block[shape][c] <= b < first_block[shape][c + 1].
+ * A ``const double*`` argument is a device array with one value per case.
+ * Blocks of cases with active[c] == 0 return at once (active may be NULL).
+ * The per-case results equal the single-case entry points' bit for bit
+ * when collide_threads and stream_threads are 256. */
+#define ICY2D_SHAPES 8 /* cells, body, x faces, y faces, one, totals, cells/128, cells/512 */
+typedef struct icy2d_batch {
+  int32_t ncases;
+  int32_t collide_threads;      /* 128, 256 or 512 (chosen by the autotuner) */
+  int32_t stream_threads;       /* 128, 256 or 512 */
+  int32_t reserved;
+  const icy2d_grid* grids;      /* device array of ncases grids */
+  const int32_t* active;        /* device, ncases flags, or NULL for all */
+  const int32_t* first_block;   /* device, ICY2D_SHAPES * (ncases + 1) prefix sums */
+  int32_t total_blocks[ICY2D_SHAPES];
+} icy2d_batch;
+
+/* Host: fills first_block (ICY2D_SHAPES * (ncases + 1)) and total_blocks
+ * (ICY2D_SHAPES) from host copies of the grids, and checks that every
+ * case's partials hold its largest reduction. */
+int icy2d_batch_plan(const icy2d_grid* grids, int32_t ncases, int32_t* first_block, int32_t* total_blocks);
+
+int icy2d_batch_body_fraction(const icy2d_batch* b, int only_if_projected, void* stream);
+int icy2d_batch_commit_raster(const icy2d_batch* b, void* stream);
+int icy2d_batch_contact_support(const icy2d_batch* b, int reduce, int only_if_projected, void* stream);
+int icy2d_batch_integrate_rigid(const icy2d_batch* b, void* stream);
+int icy2d_batch_project_inside(const icy2d_batch* b, void* stream);
+int icy2d_batch_update_solid_mask(const icy2d_batch* b, int init, void* stream);
+int icy2d_batch_reduce_mass_properties(const icy2d_batch* b, void* stream);
+int icy2d_batch_apply_mass_properties(const icy2d_batch* b, int initialize, void* stream);
+int icy2d_batch_collide(const icy2d_batch* b, int with_momentum, void* stream);
+int icy2d_batch_stream(const icy2d_batch* b, int phase_only, int mode, void* stream);
+int icy2d_batch_update_pressure(const icy2d_batch* b, void* stream);
+int icy2d_batch_refill(const icy2d_batch* b, void* stream);
+int icy2d_batch_measure_outflow(const icy2d_batch* b, void* stream);
+int icy2d_batch_clip_phase(const icy2d_batch* b, double max_bulk_l1, void* stream);
+int icy2d_batch_seed_projection(const icy2d_batch* b, void* stream);
+int icy2d_batch_dilate_projection(const icy2d_batch* b, int primary_to_next, void* stream);
+int icy2d_batch_evaluate_projection(const icy2d_batch* b, const double* lambda, void* stream);
+int icy2d_batch_apply_projection(const icy2d_batch* b, const double* lambda, void* stream);
+int icy2d_batch_projection_residual_weight(const icy2d_batch* b, const double* residual, void* stream);
+int icy2d_batch_apply_projection_residual(const icy2d_batch* b, const double* residual, const double* weight,
+                                          void* stream);
+int icy2d_batch_snapshot_melt_momentum(const icy2d_batch* b, void* stream);
+int icy2d_batch_reduce_provisional_momentum(const icy2d_batch* b, void* stream);
+int icy2d_batch_melt_weight_geometry(const icy2d_batch* b, void* stream);
+int icy2d_batch_apply_melt_correction(const icy2d_batch* b, void* stream);
+int icy2d_batch_reduce_final_momentum(const icy2d_batch* b, void* stream);
+int icy2d_batch_local_melt_correction(const icy2d_batch* b, void* stream);
+int icy2d_batch_finish_melt_momentum(const icy2d_batch* b, void* stream);
+int icy2d_batch_measure_aperture(const icy2d_batch* b, void* stream);
+int icy2d_batch_build_aperture_targets(const icy2d_batch* b, void* stream);
+int icy2d_batch_apply_aperture_transfer(const icy2d_batch* b, void* stream);
+int icy2d_batch_finish_aperture_transfer(const icy2d_batch* b, void* stream);
+int icy2d_batch_body_conduction(const icy2d_batch* b, const double* dt, void* stream);
+int icy2d_batch_conduction_flux(const icy2d_batch* b, int axis, void* stream);
+int icy2d_batch_advection_flux(const icy2d_batch* b, int axis, void* stream);
+int icy2d_batch_update_water(const icy2d_batch* b, const double* dt, int volume, void* stream);
+int icy2d_batch_reset_interval_sources(const icy2d_batch* b, void* stream);
+int icy2d_batch_accumulate_boundary_heat(const icy2d_batch* b, const double* dt, void* stream);
+int icy2d_batch_exchange_heat(const icy2d_batch* b, const double* dt, void* stream);
+int icy2d_batch_phase_change(const icy2d_batch* b, void* stream);
+int icy2d_batch_freeze_eligibility(const icy2d_batch* b, void* stream);
+int icy2d_batch_inject_melt(const icy2d_batch* b, void* stream);
+int icy2d_batch_measure_fallback(const icy2d_batch* b, void* stream);
+int icy2d_batch_apply_fallback(const icy2d_batch* b, int use_free, void* stream);
+int icy2d_batch_reduce_totals(const icy2d_batch* b, void* stream);
+void icy2d_batch_abi_size(int32_t* batch);
+
+/* HIP graphs: capture the launches enqueued on ``stream`` between begin and
+ * end into an executable graph (returned as an opaque handle), replay it, and
+ * free it.  No counterpart in the reference (Taichi launches every kernel). */
+int icy2d_graph_begin(void* stream);
+int icy2d_graph_end(void* stream, void** graph);
+int icy2d_graph_launch(void* graph, void* stream);
+int icy2d_graph_destroy(void* graph);
+
 #ifdef __cplusplus
 }
 #endif

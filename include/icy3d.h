@@ -85,6 +85,11 @@ typedef struct icy3d_grid {
   float *energy_flux, *volume_flux;                /* [3][N] */
   int32_t *tank_cells, *ice_cells;                 /* [N] flag-1 / flag-2 cell lists */
   float* partials;                                 /* [partials_capacity] block partial sums */
+  /* Learned interface closure (no reference counterpart; both may be NULL):
+   * exchange_gain[n] multiplies the heat a water cell requests from the ice
+   * in the exchange (thermal.py:308-392); exchange_heat_out[n] accumulates
+   * the heat the cell hands to the ice. */
+  float *exchange_gain, *exchange_heat_out;        /* [N] */
   int32_t partials_capacity, reserved;
   icy3d_control* control;
 } icy3d_grid;

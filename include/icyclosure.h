@@ -1,0 +1,91 @@
+/* icyclosure — C ABI of the learned interface closure on MI355X (gfx950).
+ *
+ * No counterpart in the reference: the reference computes the ice/water
+ * heat exchange of iceflow3d/thermal.py:308-392 with the coarse-grid
+ * formula only.  This library holds the hand-written BF16 MFMA (FP32
+ * accumulate) kernels of a small 3D CNN that predicts a per-cell gain on
+ * that exchange, its backward pass, its loss and its Adam update.  The
+ * Python host (``projecticy_amd.closure``) owns every buffer; the library
+ * allocates nothing.
+ *
+ * Layouts
+ *   - Grid nx * ny * nz, cell n = (i * ny + j) * nz + k (k fastest).
+ *   - Activations: BF16 channels-last, ``act[n * 32 + c]`` (32 channels;
+ *     narrower tensors are zero-padded to 32).
+ *   - Taps t = 9 * (di + 1) + 3 * (dj + 1) + (dk + 1); zero padding outside
+ *     the grid.
+ *   - Forward weights ``wf[(t * 32 + co) * 32 + ci]`` (BF16); the backward
+ *     data pass uses ``wb[(t * 32 + ci) * 32 + co] = wf[((26 - t) * 32 + co)
+ *     * 32 + ci]``; master weights and moments are FP32 in the forward
+ *     layout, 27 * 32 * 32 weights then 32 biases per layer.
+ *   - Return codes: 0 success, >0 hipError_t of a failed launch, <0 an
+ *     ICYC_ERR_* code.  ``stream`` is a hipStream_t passed as void*.
+ */
+#ifndef ICYCLOSURE_H
+#define ICYCLOSURE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ICYC_ERR_ARGUMENT -1
+#define ICYC_CHANNELS 32
+#define ICYC_TAPS 27
+#define ICYC_LAYER_PARAMS (ICYC_TAPS * ICYC_CHANNELS * ICYC_CHANNELS + ICYC_CHANNELS)
+
+/* Convolution epilogues. */
+#define ICYC_RELU_BF16 0   /* out = bf16(relu(conv + bias)) */
+#define ICYC_LINEAR_F32 1  /* out_f32[n] = conv + bias, output channel 0 only */
+#define ICYC_MASKED_BF16 2 /* out = bf16(conv) where mask_act[n, c] > 0, else 0 (no bias) */
+
+/* One 3x3x3 convolution, 32 -> 32 channels, BF16 MFMA with FP32
+ * accumulation.  ``bias`` (FP32) is used by the first two epilogues,
+ * ``mask_act`` by the third. */
+int icyc_conv3d(int32_t nx, int32_t ny, int32_t nz, const uint16_t* in, const uint16_t* weights, const float* bias,
+                int32_t epilogue, const uint16_t* mask_act, uint16_t* out_bf16, float* out_f32, void* stream);
+
+/* Weight and bias gradients of one convolution: dW[t][co][ci] = sum_n
+ * dout[n, co] * in[n + off(t), ci], db[co] = sum_n dout[n, co].  Partial
+ * sums go to ``partials`` (icyc_wgrad_partials_size floats), then fold in a
+ * fixed order into ``grad`` (ICYC_LAYER_PARAMS floats).  nz must be 16, 32,
+ * 48 or 64. */
+int64_t icyc_wgrad_partials_size(int32_t nx, int32_t ny, int32_t nz);
+int icyc_conv3d_wgrad(int32_t nx, int32_t ny, int32_t nz, const uint16_t* in, const uint16_t* dout,
+                      float* partials, float* grad, void* stream);
+
+/* Weighted squared error of channel 0: loss = sum_n w[n] (pred[n] -
+ * target[n])^2; writes dloss/dpred into channel 0 of ``dout`` (BF16, other
+ * channels zero) and the loss into loss[0] (FP32, via ``partials`` of
+ * icyc_loss_partials_size floats). */
+int64_t icyc_loss_partials_size(int64_t cells);
+int icyc_weighted_mse(int64_t cells, const float* pred, const float* target, const float* weight, uint16_t* dout,
+                      float* partials, float* loss, void* stream);
+
+/* Adam step on ``count`` layer parameter blocks (ICYC_LAYER_PARAMS each):
+ * updates master weights and moments, and writes the BF16 forward and
+ * backward-data weight layouts. */
+int icyc_adam(int32_t layers, float* master, float* m, float* v, const float* grad, float lr, float beta1,
+              float beta2, float eps, int32_t step, uint16_t* wf, uint16_t* wb, void* stream);
+
+/* BF16 copies of FP32 master weights (forward and backward layouts). */
+int icyc_pack_weights(int32_t layers, const float* master, uint16_t* wf, uint16_t* wb, void* stream);
+
+/* Features of a 3D solver state for the network: channels-last BF16
+ * (channel 0 phase, 1 water temperature / 100 degC, 2-4 lattice velocity
+ * x 10, 5 ice coverage, 6 wet-cell flag, 7 ice-contact flag, rest zero).
+ * Inputs are the solver's FP32 planes of N cells. */
+int icyc_features(int32_t nx, int32_t ny, int32_t nz, const float* phi, const float* water_volume,
+                  const float* water_energy, const float* u, const float* coverage, const int8_t* cov_id,
+                  float cell_volume, float rho_cp, float tm, uint16_t* features, void* stream);
+
+/* Exchange gain from the network output: gain[n] = clamp(1 + pred[n], 0,
+ * max_gain) on cells that touch ice, 1 elsewhere. */
+int icyc_gain(int32_t nx, int32_t ny, int32_t nz, const float* pred, const float* coverage, const int8_t* cov_id,
+              float max_gain, float* gain, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -38,6 +38,7 @@ GRID_POINTERS = (
     "water_volume", "water_energy", "target_energy", "sampled_ice_t",
     "energy_flux", "volume_flux", "tank_cells", "ice_cells", "partials",
 )
+GRID_CLOSURE_POINTERS = ("exchange_gain", "exchange_heat_out")  # NULL unless a closure is attached
 GRID_TAIL_INTS = ("partials_capacity", "reserved")
 
 
@@ -46,6 +47,7 @@ class Grid(ctypes.Structure):
         [(n, _i) for n in GRID_INTS]
         + [(n, _f) for n in GRID_FLOATS]
         + [(n, _p) for n in GRID_POINTERS]
+        + [(n, _p) for n in GRID_CLOSURE_POINTERS]
         + [(n, _i) for n in GRID_TAIL_INTS]
         + [("control", _p)]
     )

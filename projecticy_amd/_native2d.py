@@ -76,6 +76,27 @@ CALLS = {
     "apply_fallback": [_i], "reduce_totals": [],
 }
 
+SHAPES = 8  # ICY2D_SHAPES
+
+
+class Batch2D(ctypes.Structure):
+    _fields_ = [
+        ("ncases", _i), ("collide_threads", _i), ("stream_threads", _i), ("reserved", _i),
+        ("grids", _p), ("active", _p), ("first_block", _p), ("total_blocks", _i * SHAPES),
+    ]
+
+
+# Batched entry points: the single-case argument lists, with every per-case
+# FP64 argument a device pointer to one value per case.
+PER_CASE = {"evaluate_projection", "apply_projection", "projection_residual_weight", "apply_projection_residual",
+            "body_conduction", "update_water", "accumulate_boundary_heat", "exchange_heat"}
+SINGLE_ONLY = {"initialize_fluid", "hydrostatic_equilibrium", "initialize_water_volume", "recover_initial_moments",
+               "initialize_body_state", "initialize_water_state", "prepare_cut_links"}
+BATCH_CALLS = {
+    name: [_p if (name in PER_CASE and t is _d) else t for t in args]
+    for name, args in CALLS.items() if name not in SINGLE_ONLY
+}
+
 _LIB = None
 
 
@@ -93,10 +114,24 @@ def lib():
         fn = getattr(handle, "icy2d_" + name)
         fn.argtypes = [_GRID, *args, _p]
         fn.restype = ctypes.c_int
-    sizes = (_i(), _i())
+    for name, args in BATCH_CALLS.items():
+        fn = getattr(handle, "icy2d_batch_" + name)
+        fn.argtypes = [ctypes.POINTER(Batch2D), *args, _p]
+        fn.restype = ctypes.c_int
+    handle.icy2d_batch_plan.argtypes = [_GRID, _i, ctypes.POINTER(_i), ctypes.POINTER(_i)]
+    handle.icy2d_batch_plan.restype = ctypes.c_int
+    handle.icy2d_graph_begin.argtypes = [_p]
+    handle.icy2d_graph_end.argtypes = [_p, ctypes.POINTER(_p)]
+    handle.icy2d_graph_launch.argtypes = [_p, _p]
+    handle.icy2d_graph_destroy.argtypes = [_p]
+    for name in ("graph_begin", "graph_end", "graph_launch", "graph_destroy"):
+        getattr(handle, "icy2d_" + name).restype = ctypes.c_int
+    sizes = (_i(), _i(), _i())
     handle.icy2d_abi_sizes(ctypes.byref(sizes[0]), ctypes.byref(sizes[1]))
-    if (sizes[0].value, sizes[1].value) != (ctypes.sizeof(Grid2D), ctypes.sizeof(Control2D)):
-        raise RuntimeError(f"icy2d ABI mismatch: library grid/control {sizes[0].value}/{sizes[1].value} bytes, "
-                           f"binding {ctypes.sizeof(Grid2D)}/{ctypes.sizeof(Control2D)}")
+    handle.icy2d_batch_abi_size(ctypes.byref(sizes[2]))
+    expected = (ctypes.sizeof(Grid2D), ctypes.sizeof(Control2D), ctypes.sizeof(Batch2D))
+    if tuple(v.value for v in sizes) != expected:
+        raise RuntimeError(f"icy2d ABI mismatch: library grid/control/batch {[v.value for v in sizes]} bytes, "
+                           f"binding {list(expected)}")
     _LIB = handle
     return _LIB

@@ -1,0 +1,53 @@
+"""ctypes binding of ``libicyclosure.so`` (the C ABI declared in
+include/icyclosure.h).  Loaded after ``import torch`` so the library shares
+PyTorch's HIP runtime; a missing library is an error (no fallback path)."""
+
+import ctypes
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libicyclosure.so"
+
+CHANNELS = 32
+TAPS = 27
+LAYER_PARAMS = TAPS * CHANNELS * CHANNELS + CHANNELS
+WEIGHTS = TAPS * CHANNELS * CHANNELS
+RELU_BF16, LINEAR_F32, MASKED_BF16 = 0, 1, 2
+
+_i, _i64, _f, _p = ctypes.c_int32, ctypes.c_int64, ctypes.c_float, ctypes.c_void_p
+
+SIGNATURES = {
+    "icyc_conv3d": ([_i, _i, _i, _p, _p, _p, _i, _p, _p, _p, _p], ctypes.c_int),
+    "icyc_wgrad_partials_size": ([_i, _i, _i], _i64),
+    "icyc_conv3d_wgrad": ([_i, _i, _i, _p, _p, _p, _p, _p], ctypes.c_int),
+    "icyc_loss_partials_size": ([_i64], _i64),
+    "icyc_weighted_mse": ([_i64, _p, _p, _p, _p, _p, _p, _p], ctypes.c_int),
+    "icyc_adam": ([_i, _p, _p, _p, _p, _f, _f, _f, _f, _i, _p, _p, _p], ctypes.c_int),
+    "icyc_pack_weights": ([_i, _p, _p, _p, _p], ctypes.c_int),
+    "icyc_features": ([_i, _i, _i, _p, _p, _p, _p, _p, _p, _f, _f, _f, _p, _p], ctypes.c_int),
+    "icyc_gain": ([_i, _i, _i, _p, _p, _p, _f, _p, _p], ctypes.c_int),
+}
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    import torch  # noqa: F401  -- share PyTorch's HIP runtime
+
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH.name} is not built; run `python -m projecticy_amd.build` first")
+    handle = ctypes.CDLL(str(LIB_PATH))
+    for name, (args, restype) in SIGNATURES.items():
+        fn = getattr(handle, name)
+        fn.argtypes = args
+        fn.restype = restype
+    _LIB = handle
+    return _LIB
+
+
+def check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with icyclosure code {rc}")

@@ -2,7 +2,8 @@
 
 ``libicy3d.so`` holds the 3D kernels, the C ABI of include/icy3d.h and the
 host FP64 rigid-body solver; ``libicy2d.so`` holds the 2D kernels and the C
-ABI of include/icy2d.h.  Called by ``__graft_entry__.build()``; the Python
+ABI of include/icy2d.h; ``libicyclosure.so`` holds the learned closure's
+BF16 MFMA kernels and the C ABI of include/icyclosure.h.  Called by ``__graft_entry__.build()``; the Python
 package only loads the results and never builds at import time.
 """
 
@@ -17,6 +18,7 @@ INCLUDE = HERE.parent / "include"
 LIBRARIES = {
     "libicy3d.so": ["icy3d_flow.hip", "icy3d_geometry.hip", "icy3d_thermal.hip", "rigid3d.cpp"],
     "libicy2d.so": ["icy2d.hip"],
+    "libicyclosure.so": ["icy_closure.hip"],
 }
 LIB = HERE / "libicy3d.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,0 +1,150 @@
+"""The closure network on the GPU: parameters, activations and one training
+step, every operation a ``libicyclosure.so`` kernel."""
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from .. import _native_closure as native
+
+LAYERS = 4
+INPUT_CHANNELS = 8  # features written by icyc_features
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class ClosureNet:
+    """Four 3x3x3 convolutions 8(->32) -> 32 -> 32 -> 32 -> 1 on one grid shape.
+
+    Master weights, Adam moments and gradients are FP32 in the forward
+    layout of include/icyclosure.h; the kernels read BF16 copies."""
+
+    def __init__(self, shape, *, seed=0, device=None, last_layer_scale=1e-3):
+        if not torch.cuda.is_available():
+            raise RuntimeError("the closure network requires a ROCm GPU; no CPU fallback is provided")
+        self.lib = native.lib()
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None else device
+        self.shape = tuple(int(v) for v in shape)
+        nx, ny, nz = self.shape
+        if nz % 16 != 0 or nz > 64:
+            raise ValueError("the closure grid needs nz in {16, 32, 48, 64}")
+        self.cells = nx * ny * nz
+        dev, P, C = self.device, native.LAYER_PARAMS, native.CHANNELS
+        self.master = torch.from_numpy(self.initial_parameters(seed, last_layer_scale)).to(dev)
+        self.m = torch.zeros_like(self.master)
+        self.v = torch.zeros_like(self.master)
+        self.grad = torch.zeros_like(self.master)
+        self.wf = torch.zeros(LAYERS * native.WEIGHTS, dtype=torch.int16, device=dev)
+        self.wb = torch.zeros_like(self.wf)
+        act = lambda: torch.zeros(self.cells * C, dtype=torch.int16, device=dev)  # noqa: E731
+        self.acts = [act() for _ in range(LAYERS)]  # input features, a1, a2, a3
+        self.dpre = [act() for _ in range(LAYERS)]  # gradients of each layer's pre-activation
+        self.pred = torch.zeros(self.cells, dtype=torch.float32, device=dev)
+        self.loss = torch.zeros(1, dtype=torch.float32, device=dev)
+        self.wgrad_partials = torch.zeros(int(self.lib.icyc_wgrad_partials_size(nx, ny, nz)), dtype=torch.float32,
+                                          device=dev)
+        self.loss_partials = torch.zeros(int(self.lib.icyc_loss_partials_size(self.cells)), dtype=torch.float32,
+                                         device=dev)
+        self.step_count = 0
+        self._pack()
+
+    @staticmethod
+    def initial_parameters(seed, last_layer_scale):
+        """He-normal hidden layers (fan-in 27 x real input channels), a small
+        last layer (the initial gain is close to 1), zero biases; the padded
+        input channels of layer 0 and output channels of layer 3 are zero."""
+        rng = np.random.default_rng(seed)
+        P, C, T = native.LAYER_PARAMS, native.CHANNELS, native.TAPS
+        params = np.zeros(LAYERS * P, dtype=np.float32)
+        for layer in range(LAYERS):
+            fan_in = T * (INPUT_CHANNELS if layer == 0 else C)
+            w = rng.normal(scale=math.sqrt(2.0 / fan_in), size=(T, C, C)).astype(np.float32)
+            if layer == 0:
+                w[:, :, INPUT_CHANNELS:] = 0.0
+            if layer == LAYERS - 1:
+                w *= last_layer_scale
+                w[:, 1:, :] = 0.0
+            params[layer * P:layer * P + T * C * C] = w.ravel()
+        return params
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _pack(self):
+        native.check(self.lib.icyc_pack_weights(LAYERS, _ptr(self.master), _ptr(self.wf), _ptr(self.wb),
+                                                self._stream()), "icyc_pack_weights")
+
+    def _layer(self, tensor, layer, size):
+        return ctypes.c_void_p(tensor.data_ptr() + layer * size * tensor.element_size())
+
+    def _bias(self, layer):
+        return ctypes.c_void_p(self.master.data_ptr() + (layer * native.LAYER_PARAMS + native.WEIGHTS) * 4)
+
+    # ------------------------------------------------------------ passes
+    def forward(self, features=None):
+        """Prediction (FP32 per cell) of the features in ``acts[0]`` (or the
+        given BF16 channels-last tensor, copied there)."""
+        if features is not None:
+            self.acts[0].copy_(features.view(torch.int16).reshape(-1))
+        nx, ny, nz = self.shape
+        s = self._stream()
+        for layer in range(LAYERS):
+            w = self._layer(self.wf, layer, native.WEIGHTS)
+            if layer + 1 < LAYERS:
+                rc = self.lib.icyc_conv3d(nx, ny, nz, _ptr(self.acts[layer]), w, self._bias(layer), native.RELU_BF16,
+                                          None, _ptr(self.acts[layer + 1]), None, s)
+            else:
+                rc = self.lib.icyc_conv3d(nx, ny, nz, _ptr(self.acts[layer]), w, self._bias(layer), native.LINEAR_F32,
+                                          None, None, _ptr(self.pred), s)
+            native.check(rc, "icyc_conv3d")
+        return self.pred
+
+    def backward(self, target, weight):
+        """Weighted squared error of the last prediction and every parameter
+        gradient (into ``grad``); returns the loss tensor (device, FP32)."""
+        nx, ny, nz = self.shape
+        s = self._stream()
+        native.check(self.lib.icyc_weighted_mse(self.cells, _ptr(self.pred), _ptr(target), _ptr(weight),
+                                                _ptr(self.dpre[LAYERS - 1]), _ptr(self.loss_partials),
+                                                _ptr(self.loss), s), "icyc_weighted_mse")
+        for layer in reversed(range(LAYERS)):
+            native.check(self.lib.icyc_conv3d_wgrad(nx, ny, nz, _ptr(self.acts[layer]), _ptr(self.dpre[layer]),
+                                                    _ptr(self.wgrad_partials),
+                                                    self._layer(self.grad, layer, native.LAYER_PARAMS), s),
+                         "icyc_conv3d_wgrad")
+            if layer > 0:
+                native.check(self.lib.icyc_conv3d(nx, ny, nz, _ptr(self.dpre[layer]),
+                                                  self._layer(self.wb, layer, native.WEIGHTS), None,
+                                                  native.MASKED_BF16, _ptr(self.acts[layer]),
+                                                  _ptr(self.dpre[layer - 1]), None, s), "icyc_conv3d")
+        return self.loss
+
+    def adam_step(self, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8):
+        self.step_count += 1
+        native.check(self.lib.icyc_adam(LAYERS, _ptr(self.master), _ptr(self.m), _ptr(self.v), _ptr(self.grad),
+                                        lr, beta1, beta2, eps, self.step_count, _ptr(self.wf), _ptr(self.wb),
+                                        self._stream()), "icyc_adam")
+
+    def train_step(self, features, target, weight, lr=1e-3):
+        self.forward(features)
+        loss = self.backward(target, weight)
+        self.adam_step(lr)
+        return loss
+
+    def evaluate(self, features, target, weight):
+        """Loss of the current weights on one sample (no update)."""
+        self.forward(features)
+        return float(self.backward(target, weight).item())
+
+    # ------------------------------------------------------------ state
+    def state_dict(self):
+        return {"shape": self.shape, "master": self.master.cpu().numpy(), "step": self.step_count}
+
+    def load_state_dict(self, state):
+        self.master.copy_(torch.from_numpy(np.asarray(state["master"], dtype=np.float32)))
+        self.step_count = int(state.get("step", 0))
+        self._pack()

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstddef>
 
 #include "icy2d.h"
 
@@ -220,10 +221,11 @@ __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   return v;
 }
-// Writes the block's sums of v[0..K) to partials[blockIdx.x * K + k].
+// Writes the block's sums of v[0..K) to partials[blk * K + k] (blocks of at
+// most 1024 threads).
 template <int K>
-__device__ __forceinline__ void block_partials(double (&v)[K], double* partials) {
-  __shared__ double part[K][kBlock / 64];
+__device__ __forceinline__ void block_partials(double (&v)[K], double* partials, int blk) {
+  __shared__ double part[K][16];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -234,7 +236,7 @@ __device__ __forceinline__ void block_partials(double (&v)[K], double* partials)
   if (threadIdx.x < K) {
     double total = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) total += part[threadIdx.x][w];
-    partials[blockIdx.x * K + threadIdx.x] = total;
+    partials[blk * K + threadIdx.x] = total;
   }
 }
 // Folds partials[b * K + k] over b in a fixed order: one block per k,
@@ -243,11 +245,9 @@ __device__ __forceinline__ void block_partials(double (&v)[K], double* partials)
 // max_mask selects a minimum / maximum instead of a sum; bit k of add_mask
 // adds the result to out[k] instead of replacing it.
 constexpr int kFinishThreads = 1024;
-__global__ void __launch_bounds__(kFinishThreads) k2_finish(const double* __restrict__ partials, int blocks, int K,
-                                                            double* out, unsigned min_mask, unsigned max_mask,
-                                                            unsigned add_mask) {
+__device__ __forceinline__ void finish_fold(const double* __restrict__ partials, int blocks, int K, int k,
+                                            double* out, unsigned min_mask, unsigned max_mask, unsigned add_mask) {
   __shared__ double part[kFinishThreads];
-  const int k = blockIdx.x;
   const bool is_min = (min_mask >> k) & 1u, is_max = (max_mask >> k) & 1u;
   const double identity = is_min ? 1.0e300 : is_max ? -1.0e300 : 0.0;
   auto combine = [&](double a, double b) { return is_min ? fmin(a, b) : is_max ? fmax(a, b) : a + b; };
@@ -268,14 +268,20 @@ __global__ void __launch_bounds__(kFinishThreads) k2_finish(const double* __rest
   if (threadIdx.x == 0) out[k] = ((add_mask >> k) & 1u) ? out[k] + part[0] : part[0];
 }
 
-__device__ __forceinline__ int cell_index() { return blockIdx.x * blockDim.x + threadIdx.x; }
+__global__ void __launch_bounds__(kFinishThreads) k2_finish(const double* __restrict__ partials, int blocks, int K,
+                                                            double* out, unsigned min_mask, unsigned max_mask,
+                                                            unsigned add_mask) {
+  finish_fold(partials, blocks, K, (int)blockIdx.x, out, min_mask, max_mask, add_mask);
+}
+
+__device__ __forceinline__ int cell_index(int blk) { return blk * (int)blockDim.x + (int)threadIdx.x; }
 
 // ---------------------------------------------------- geometry kernels
 // simulator.py:2532-2537; only_if_projected repeats the pass after a
 // contact projection (simulator.py:4109-4111) without a host round trip.
-__global__ void k2_body_fraction(icy2d_grid g, int only_if_projected) {
+__device__ __forceinline__ void k2_body_fraction(const icy2d_grid& g, const int blk, int only_if_projected) {
   if (only_if_projected && g.control->projected == 0) return;
-  const int n = cell_index();
+  const int n = cell_index(blk);
   if (n >= cells(g)) return;
   const Pose p = load_pose(g);
   const int i = n / g.ny, j = n - i * g.ny;
@@ -284,16 +290,16 @@ __global__ void k2_body_fraction(icy2d_grid g, int only_if_projected) {
   g.coverage[n] = sample_fraction(g, lx, ly);
 }
 // simulator.py:2539-2544
-__global__ void k2_commit_raster(icy2d_grid g) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_commit_raster(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk);
   if (n >= cells(g)) return;
   const int i = n / g.ny, j = n - i * g.ny;
   if (wall_at(g, i, j)) g.coverage[n] = 0.0f;
 }
 // simulator.py:3940-3965: per-block extrema (min x, max x, min y, max y) as partials
-__global__ void k2_support_partials(icy2d_grid g, int only_if_projected) {
+__device__ __forceinline__ void k2_support_partials(const icy2d_grid& g, const int blk, int only_if_projected) {
   if (only_if_projected && g.control->projected == 0) return;
-  const int n = cell_index();
+  const int n = cell_index(blk);
   const Pose p = load_pose(g);
   double v[4] = {1.0e30, -1.0e30, 1.0e30, -1.0e30};
   if (n < cells(g)) {
@@ -313,11 +319,11 @@ __global__ void k2_support_partials(icy2d_grid g, int only_if_projected) {
     const int k = threadIdx.x;
     double r = part[k][0];
     for (int t = 1; t < (int)blockDim.x; ++t) r = (k & 1) ? fmax(r, part[k][t]) : fmin(r, part[k][t]);
-    g.partials[blockIdx.x * 4 + k] = r;
+    g.partials[blk * 4 + k] = r;
   }
 }
 // simulator.py:3932-3938 and :3966-3971: store the extrema
-__global__ void k2_support_store(icy2d_grid g, int reduce, int only_if_projected) {
+__device__ __forceinline__ void k2_support_store(const icy2d_grid& g, const int blk, int reduce, int only_if_projected) {
   icy2d_control* ctl = g.control;
   if (only_if_projected && ctl->projected == 0) return;
   double* e = g.partials + g.partials_capacity - 4;
@@ -333,7 +339,7 @@ __global__ void k2_support_store(icy2d_grid g, int reduce, int only_if_projected
   for (int k = 0; k < 4; ++k) ctl->support[k] = (float)e[k];
 }
 // simulator.py:3973-4017
-__global__ void k2_integrate_rigid(icy2d_grid g) {
+__device__ __forceinline__ void k2_integrate_rigid(const icy2d_grid& g, const int blk) {
   icy2d_control* ctl = g.control;
   if (ctl->inertia > 0.0) {
     const float mass = (float)ctl->mass, inertia = (float)ctl->inertia;
@@ -360,7 +366,7 @@ __global__ void k2_integrate_rigid(icy2d_grid g) {
   ctl->impulse[0] = ctl->impulse[1] = ctl->impulse[2] = 0.0;
 }
 // simulator.py:4019-4080
-__global__ void k2_project_inside(icy2d_grid g) {
+__device__ __forceinline__ void k2_project_inside(const icy2d_grid& g, const int blk) {
   icy2d_control* ctl = g.control;
   int projected = 0;
   const Pose p = load_pose(g);
@@ -399,8 +405,8 @@ __global__ void k2_project_inside(icy2d_grid g) {
   }
 }
 // simulator.py:2699-2711 (init) and :2713-2727
-__global__ void k2_update_solid_mask(icy2d_grid g, int init) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_update_solid_mask(const icy2d_grid& g, const int blk, int init) {
+  const int n = cell_index(blk);
   if (n >= cells(g)) return;
   const Pose p = load_pose(g);
   const int i = n / g.ny, j = n - i * g.ny;
@@ -412,8 +418,8 @@ __global__ void k2_update_solid_mask(icy2d_grid g, int init) {
 
 // -------------------------------------------------------- initialisation
 // simulator.py:2595-2624
-__global__ void k2_initialize_fluid(icy2d_grid g, int water_width, int water_height) {
-  const int n = cell_index(), N = cells(g);
+__device__ __forceinline__ void k2_initialize_fluid(const icy2d_grid& g, const int blk, int water_width, int water_height) {
+  const int n = cell_index(blk), N = cells(g);
   if (n >= N) return;
   const Pose p = load_pose(g);
   const int i = n / g.ny, j = n - i * g.ny;
@@ -435,8 +441,8 @@ __global__ void k2_initialize_fluid(icy2d_grid g, int water_width, int water_hei
   }
 }
 // simulator.py:2664-2697
-__global__ void k2_hydrostatic_equilibrium(icy2d_grid g) {
-  const int n = cell_index(), N = cells(g);
+__device__ __forceinline__ void k2_hydrostatic_equilibrium(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk), N = cells(g);
   if (n >= N) return;
   const int i = n / g.ny, j = n - i * g.ny;
   if (fluid_at(g, i, j)) {
@@ -457,19 +463,19 @@ __global__ void k2_hydrostatic_equilibrium(icy2d_grid g) {
   }
 }
 // simulator.py:4125-4133 (partials of the fluid phase sum)
-__global__ void k2_water_volume_partials(icy2d_grid g) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_water_volume_partials(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk);
   double v[1] = {0.0};
   if (n < cells(g)) {
     const int i = n / g.ny, j = n - i * g.ny;
     if (fluid_at(g, i, j)) v[0] = (double)g.phi[n];
   }
-  block_partials<1>(v, g.partials);
+  block_partials<1>(v, g.partials, blk);
 }
-__global__ void k2_copy_initial_volume(icy2d_grid g) { g.control->water[1] = g.control->water[0]; }
+__device__ __forceinline__ void k2_copy_initial_volume(const icy2d_grid& g, const int blk) { g.control->water[1] = g.control->water[0]; }
 // simulator.py:3833-3858
-__global__ void k2_recover_initial_moments(icy2d_grid g) {
-  const int n = cell_index(), N = cells(g);
+__device__ __forceinline__ void k2_recover_initial_moments(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk), N = cells(g);
   if (n >= N) return;
   const int i = n / g.ny, j = n - i * g.ny;
   if (fluid_at(g, i, j)) {
@@ -576,8 +582,8 @@ __device__ __forceinline__ bool prepare_cut_links(const icy2d_grid& g, const Pos
 }
 
 // simulator.py:3241-3552 (momentum) then :3554-3697 (phase) for one cell.
-__global__ void __launch_bounds__(kBlock) k2_collide(icy2d_grid g, int with_momentum) {
-  const int n = cell_index(), N = cells(g);
+__device__ __forceinline__ void k2_collide(const icy2d_grid& g, const int blk, int with_momentum) {
+  const int n = cell_index(blk), N = cells(g);
   double load[3] = {0.0, 0.0, 0.0};
   if (n < N) {
     const int i = n / g.ny, j = n - i * g.ny;
@@ -728,12 +734,12 @@ __global__ void __launch_bounds__(kBlock) k2_collide(icy2d_grid g, int with_mome
       pop_set(g.hbuf, g, 8 * N + n, reconstruct_cm<8>(ux, uy, post));
     }
   }
-  if (with_momentum) block_partials<3>(load, g.partials);
+  if (with_momentum) block_partials<3>(load, g.partials, blk);
 }
 // The cut-link preparation alone, in place on fbuf (test helper of the
 // reference's own tests around simulator.py:4798-4866).
-__global__ void k2_prepare_cut_links(icy2d_grid g) {
-  const int n = cell_index(), N = cells(g);
+__device__ __forceinline__ void k2_prepare_cut_links(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk), N = cells(g);
   double load[3] = {0.0, 0.0, 0.0};
   if (n < N) {
     const int i = n / g.ny, j = n - i * g.ny;
@@ -746,7 +752,7 @@ __global__ void k2_prepare_cut_links(icy2d_grid g) {
       for (int q = 0; q < 9; ++q) pop_set(g.fbuf, g, q * N + n, raw[q]);
     }
   }
-  block_partials<3>(load, g.partials);
+  block_partials<3>(load, g.partials, blk);
 }
 
 // Pull streaming + macroscopic update (simulator.py:3699-3825).
@@ -776,8 +782,8 @@ __device__ __forceinline__ void pull(const icy2d_grid& g, const Pose& pose, int 
 // population sum) from the cell's populations, as the reference's separate
 // macroscopic kernel does after streaming.  The step runs both in one pass.
 constexpr int kStreamPull = 1, kStreamUpdate = 2;
-__global__ void __launch_bounds__(kBlock) k2_stream(icy2d_grid g, int phase_only, int mode) {
-  const int n = cell_index(), N = cells(g);
+__device__ __forceinline__ void k2_stream(const icy2d_grid& g, const int blk, int phase_only, int mode) {
+  const int n = cell_index(blk), N = cells(g);
   if (n >= N) return;
   const int i = n / g.ny, j = n - i * g.ny;
   const Pose pose = load_pose(g);
@@ -837,8 +843,8 @@ __global__ void __launch_bounds__(kBlock) k2_stream(icy2d_grid g, int phase_only
   }
 }
 // simulator.py:3860-3897
-__global__ void k2_update_pressure(icy2d_grid g) {
-  const int n = cell_index(), N = cells(g);
+__device__ __forceinline__ void k2_update_pressure(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk), N = cells(g);
   if (n >= N) return;
   const int i = n / g.ny, j = n - i * g.ny;
   if (fluid_at(g, i, j)) {
@@ -860,8 +866,8 @@ __global__ void k2_update_pressure(icy2d_grid g) {
   }
 }
 // simulator.py:3163-3236
-__global__ void k2_refill(icy2d_grid g) {
-  const int n = cell_index(), N = cells(g);
+__device__ __forceinline__ void k2_refill(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk), N = cells(g);
   if (n >= N) return;
   const int i = n / g.ny, j = n - i * g.ny;
   if (wall_at(g, i, j)) return;
@@ -915,8 +921,8 @@ __global__ void k2_refill(icy2d_grid g) {
   }
 }
 // simulator.py:2303-2338 (per-block maxima as partials)
-__global__ void k2_outflow_partials(icy2d_grid g) {
-  const int n = cell_index(), N = cells(g);
+__device__ __forceinline__ void k2_outflow_partials(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk), N = cells(g);
   double best = 0.0;
   if (n < N) {
     const int i = n / g.ny, j = n - i * g.ny;
@@ -944,10 +950,10 @@ __global__ void k2_outflow_partials(icy2d_grid g) {
   if (threadIdx.x == 0) {
     double r = part[0];
     for (int t = 1; t < (int)blockDim.x; ++t) r = fmax(r, part[t]);
-    g.partials[blockIdx.x] = r;
+    g.partials[blk] = r;
   }
 }
-__global__ void k2_store_outflow(icy2d_grid g) {
+__device__ __forceinline__ void k2_store_outflow(const icy2d_grid& g, const int blk) {
   g.control->max_outflow = (float)g.partials[g.partials_capacity - 1];
 }
 
@@ -970,8 +976,8 @@ __device__ __forceinline__ void lift_phase(const icy2d_grid& g, int n, int j, fl
   g.phi[n] = new_phase;
 }
 // simulator.py:4150-4229
-__global__ void k2_clip_phase(icy2d_grid g, double max_bulk_l1) {
-  const int n = cell_index(), N = cells(g);
+__device__ __forceinline__ void k2_clip_phase(const icy2d_grid& g, const int blk, double max_bulk_l1) {
+  const int n = cell_index(blk), N = cells(g);
   double v[1] = {0.0};
   if (n < N) {
     const int i = n / g.ny, j = n - i * g.ny;
@@ -1016,11 +1022,11 @@ __global__ void k2_clip_phase(icy2d_grid g, double max_bulk_l1) {
       v[0] = (double)bounded;
     }
   }
-  block_partials<1>(v, g.partials);
+  block_partials<1>(v, g.partials, blk);
 }
 // simulator.py:4231-4257; the two masks are hbuf planes 0 and 1
-__global__ void k2_seed_projection(icy2d_grid g) {
-  const int n = cell_index(), N = cells(g);
+__device__ __forceinline__ void k2_seed_projection(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk), N = cells(g);
   if (n >= N) return;
   const int i = n / g.ny, j = n - i * g.ny;
   int seed = 0;
@@ -1042,8 +1048,8 @@ __global__ void k2_seed_projection(icy2d_grid g) {
   pop_set(g.hbuf, g, N + n, 0.0f);
 }
 // simulator.py:4259-4289
-__global__ void k2_dilate_projection(icy2d_grid g, int primary_to_next) {
-  const int n = cell_index(), N = cells(g);
+__device__ __forceinline__ void k2_dilate_projection(const icy2d_grid& g, const int blk, int primary_to_next) {
+  const int n = cell_index(blk), N = cells(g);
   if (n >= N) return;
   const int src = primary_to_next ? 0 : N, dst = primary_to_next ? N : 0;
   const int i = n / g.ny, j = n - i * g.ny;
@@ -1061,8 +1067,9 @@ __global__ void k2_dilate_projection(icy2d_grid g, int primary_to_next) {
   pop_set(g.hbuf, g, dst + n, (float)value);
 }
 // simulator.py:4291-4351
-__global__ void k2_evaluate_projection(icy2d_grid g, double e) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_evaluate_projection(const icy2d_grid& g, const int blk, double lambda) {
+  const double e = exp(lambda);
+  const int n = cell_index(blk);
   double v[2] = {0.0, 0.0};
   if (n < cells(g)) {
     const int i = n / g.ny, j = n - i * g.ny;
@@ -1078,11 +1085,12 @@ __global__ void k2_evaluate_projection(icy2d_grid g, double e) {
       if (adjustable && g.cutoff < m32 && m32 < 1.0f - g.cutoff) v[1] = (double)m32 * (1.0 - (double)m32);
     }
   }
-  block_partials<2>(v, g.partials);
+  block_partials<2>(v, g.partials, blk);
 }
 // simulator.py:4353-4406
-__global__ void k2_apply_projection(icy2d_grid g, double e) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_apply_projection(const icy2d_grid& g, const int blk, double lambda) {
+  const double e = exp(lambda);
+  const int n = cell_index(blk);
   double v[1] = {0.0};
   if (n < cells(g)) {
     const int i = n / g.ny, j = n - i * g.ny;
@@ -1099,7 +1107,7 @@ __global__ void k2_apply_projection(icy2d_grid g, double e) {
       v[0] = (double)new_phase;
     }
   }
-  block_partials<1>(v, g.partials);
+  block_partials<1>(v, g.partials, blk);
 }
 __device__ __forceinline__ bool residual_free(const icy2d_grid& g, int n, double phase, double residual) {
   const double margin = 4.0 * 1.1920928955078125e-07;
@@ -1110,19 +1118,19 @@ __device__ __forceinline__ bool residual_free(const icy2d_grid& g, int n, double
   return phase - cutoff > magnitude + margin;
 }
 // simulator.py:4408-4431
-__global__ void k2_residual_weight(icy2d_grid g, double residual) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_residual_weight(const icy2d_grid& g, const int blk, double residual) {
+  const int n = cell_index(blk);
   double v[1] = {0.0};
   if (n < cells(g)) {
     const int i = n / g.ny, j = n - i * g.ny;
     const double phase = g.phi[n];
     if (fluid_at(g, i, j) && residual_free(g, n, phase, residual)) v[0] = phase * (1.0 - phase);
   }
-  block_partials<1>(v, g.partials);
+  block_partials<1>(v, g.partials, blk);
 }
 // simulator.py:4433-4492
-__global__ void k2_apply_residual(icy2d_grid g, double residual, double total_weight) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_apply_residual(const icy2d_grid& g, const int blk, double residual, double total_weight) {
+  const int n = cell_index(blk);
   double v[1] = {0.0};
   if (n < cells(g)) {
     const int i = n / g.ny, j = n - i * g.ny;
@@ -1136,13 +1144,13 @@ __global__ void k2_apply_residual(icy2d_grid g, double residual, double total_we
       v[0] = (double)new_phase;
     }
   }
-  block_partials<1>(v, g.partials);
+  block_partials<1>(v, g.partials, blk);
 }
 
 // ------------------------------------------------------- mass properties
 // simulator.py:3014-3044 over the material grid
-__global__ void k2_mass_partials(icy2d_grid g) {
-  const int k = cell_index();
+__device__ __forceinline__ void k2_mass_partials(const icy2d_grid& g, const int blk) {
+  const int k = cell_index(blk);
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (k < g.body_nx * g.body_ny) {
     const int bi = k / g.body_ny, bj = k - bi * g.body_ny;
@@ -1154,10 +1162,10 @@ __global__ void k2_mass_partials(icy2d_grid g) {
     v[2] = mass * ly;
     v[3] = mass * (lx * lx + ly * ly + fraction / 6.0);
   }
-  block_partials<4>(v, g.partials);
+  block_partials<4>(v, g.partials, blk);
 }
 // simulator.py:3046-3161
-__global__ void k2_apply_mass_properties(icy2d_grid g, int initialize) {
+__device__ __forceinline__ void k2_apply_mass_properties(const icy2d_grid& g, const int blk, int initialize) {
   icy2d_control* ctl = g.control;
   const double reduced = ctl->reduced[0], old_mass = ctl->mass, initial = g.initial_mass;
   const double tol = 1.0e-12 * fmax(initial, 1.0);
@@ -1223,8 +1231,8 @@ __global__ void k2_apply_mass_properties(icy2d_grid g, int initialize) {
 
 // -------------------------------------------------------- melt momentum
 // Fluid momentum (x, y, angular about the origin) of simulator.py:2735-2760.
-__global__ void k2_momentum_partials(icy2d_grid g) {
-  const int n = cell_index(), N = cells(g);
+__device__ __forceinline__ void k2_momentum_partials(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk), N = cells(g);
   double v[3] = {0.0, 0.0, 0.0};
   if (n < N) {
     const int i = n / g.ny, j = n - i * g.ny;
@@ -1241,15 +1249,15 @@ __global__ void k2_momentum_partials(icy2d_grid g) {
       v[2] = (i + 0.5) * my - (j + 0.5) * mx;
     }
   }
-  block_partials<3>(v, g.partials);
+  block_partials<3>(v, g.partials, blk);
 }
-__global__ void k2_snapshot_body(icy2d_grid g) {
+__device__ __forceinline__ void k2_snapshot_body(const icy2d_grid& g, const int blk) {
   icy2d_control* ctl = g.control;
   for (int k = 0; k < 3; ++k) ctl->body_before[k] = ctl->melted[k];
 }
 // simulator.py:2762-2797: wet weight and the min/max wet cell codes
-__global__ void k2_wet_partials(icy2d_grid g) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_wet_partials(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk);
   double v[1] = {0.0};
   int lo = INT32_MAX, hi = -1;
   if (n < cells(g)) {
@@ -1260,7 +1268,7 @@ __global__ void k2_wet_partials(icy2d_grid g) {
       lo = hi = n;
     }
   }
-  block_partials<1>(v, g.partials);
+  block_partials<1>(v, g.partials, blk);
   // One atomic per block for the first and last wet cell (order-independent).
   __shared__ int wave_lo[kBlock / 64], wave_hi[kBlock / 64];
 #pragma unroll
@@ -1284,11 +1292,11 @@ __global__ void k2_wet_partials(icy2d_grid g) {
     }
   }
 }
-__global__ void k2_reset_wet(icy2d_grid g) {
+__device__ __forceinline__ void k2_reset_wet(const icy2d_grid& g, const int blk) {
   g.control->wet_target = g.nx * g.ny;
   g.control->wet_target_max = -1;
 }
-__global__ void k2_copy_provisional(icy2d_grid g) {
+__device__ __forceinline__ void k2_copy_provisional(const icy2d_grid& g, const int blk) {
   icy2d_control* ctl = g.control;
   for (int k = 0; k < 3; ++k) ctl->after[k] = ctl->provisional[k];
 }
@@ -1296,8 +1304,8 @@ __device__ __forceinline__ void melt_correction(const icy2d_control* ctl, double
   for (int k = 0; k < 3; ++k) c[k] = ctl->melted[k] - ctl->body_before[k] - (ctl->after[k] - ctl->before[k]);
 }
 // simulator.py:2799-2840: first and second moments of the wet weights
-__global__ void k2_weight_partials(icy2d_grid g) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_weight_partials(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk);
   double v[3] = {0.0, 0.0, 0.0};
   if (n < cells(g)) {
     const int i = n / g.ny, j = n - i * g.ny;
@@ -1309,7 +1317,7 @@ __global__ void k2_weight_partials(icy2d_grid g) {
       v[2] = w * (px * px + py * py);
     }
   }
-  block_partials<3>(v, g.partials);
+  block_partials<3>(v, g.partials, blk);
 }
 __device__ __forceinline__ void wet_geometry(const icy2d_control* ctl, double cen[2], double& polar) {
   cen[0] = cen[1] = 0.0;
@@ -1319,7 +1327,7 @@ __device__ __forceinline__ void wet_geometry(const icy2d_control* ctl, double ce
   }
   polar = fmax(0.0, ctl->wet[3] - ctl->wet[0] * (cen[0] * cen[0] + cen[1] * cen[1]));
 }
-__global__ void k2_melt_status(icy2d_grid g) {
+__device__ __forceinline__ void k2_melt_status(const icy2d_grid& g, const int blk) {
   icy2d_control* ctl = g.control;
   double c[3], cen[2], polar;
   melt_correction(ctl, c);
@@ -1345,8 +1353,8 @@ __device__ __forceinline__ void add_momentum(const icy2d_grid& g, int n, double 
   for (int q = 0; q < 9; ++q) pop_add(g.f, g, q * N + n, 3.0f * wq(q) * ((float)cxq(q) * mx + (float)cyq(q) * my));
 }
 // simulator.py:2842-2884
-__global__ void k2_apply_melt_correction(icy2d_grid g) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_apply_melt_correction(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk);
   if (n >= cells(g)) return;
   const icy2d_control* ctl = g.control;
   const int i = n / g.ny, j = n - i * g.ny;
@@ -1361,7 +1369,7 @@ __global__ void k2_apply_melt_correction(icy2d_grid g) {
   add_momentum(g, n, w * (c[0] / den - scale * ry), w * (c[1] / den + scale * rx));
 }
 // simulator.py:2905-2958
-__global__ void k2_local_melt_correction(icy2d_grid g) {
+__device__ __forceinline__ void k2_local_melt_correction(const icy2d_grid& g, const int blk) {
   const icy2d_control* ctl = g.control;
   const int e1 = ctl->wet_target, e2 = ctl->wet_target_max, total = g.nx * g.ny;
   if (!(e1 >= 0 && e1 < total && e2 >= 0 && e2 < total && e1 != e2)) return;
@@ -1377,7 +1385,7 @@ __global__ void k2_local_melt_correction(icy2d_grid g) {
   add_momentum(g, e2, 0.5 * c[0] - qx, 0.5 * c[1] - qy);
 }
 // simulator.py:2960-2986
-__global__ void k2_finish_melt_momentum(icy2d_grid g) {
+__device__ __forceinline__ void k2_finish_melt_momentum(const icy2d_grid& g, const int blk) {
   icy2d_control* ctl = g.control;
   for (int k = 0; k < 3; ++k) {
     ctl->carrier[k] += ctl->provisional[k] - ctl->before[k];
@@ -1391,15 +1399,15 @@ __device__ __forceinline__ bool water_cell(const icy2d_grid& g, int i, int j) {
   return !wall_at(g, i, j) && g.solid[n] == 0 && g.phi[n] > g.cutoff;
 }
 // simulator.py:579-588
-__global__ void k2_initialize_body_state(icy2d_grid g, double ice_temperature) {
-  const int k = cell_index();
+__device__ __forceinline__ void k2_initialize_body_state(const icy2d_grid& g, const int blk, double ice_temperature) {
+  const int k = cell_index(blk);
   if (k >= g.body_nx * g.body_ny) return;
   g.body_mass[k] = g.cell_mass;
   g.body_energy[k] = g.cell_mass * (g.cp_i * (ice_temperature - g.t_melt));
 }
 // simulator.py:589-617
-__global__ void k2_initialize_water_state(icy2d_grid g, double water_temperature_c) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_initialize_water_state(const icy2d_grid& g, const int blk, double water_temperature_c) {
+  const int n = cell_index(blk);
   if (n >= cells(g)) return;
   const int i = n / g.ny, j = n - i * g.ny;
   const float phase = clamp01(g.phi[n]);
@@ -1408,14 +1416,14 @@ __global__ void k2_initialize_water_state(icy2d_grid g, double water_temperature
   g.water_v[n] = v;
   g.water_e[n] = (g.rho_w_si * g.cp_w) * v * (water_temperature_c - g.t_melt);
 }
-__global__ void k2_reset_thermal_diagnostics(icy2d_grid g) {
+__device__ __forceinline__ void k2_reset_thermal_diagnostics(const icy2d_grid& g, const int blk) {
   icy2d_control* ctl = g.control;
   ctl->boundary_power = ctl->boundary_heat_input = ctl->correction_abs = 0.0;
 }
 // simulator.py:633-714 in control order: volume, energy, base and full
 // capacity (sums), specific energy min, max.
-__global__ void k2_aperture_partials(icy2d_grid g) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_aperture_partials(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk);
   double v[6] = {0.0, 0.0, 0.0, 0.0, 1.0e30, -1.0e30};
   if (n < cells(g)) {
     const int i = n / g.ny, j = n - i * g.ny;
@@ -1447,11 +1455,11 @@ __global__ void k2_aperture_partials(icy2d_grid g) {
     double r = part[k][0];
     for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
       r = k == 4 ? fmin(r, part[k][w]) : k == 5 ? fmax(r, part[k][w]) : r + part[k][w];
-    g.partials[blockIdx.x * 6 + k] = r;
+    g.partials[blk * 6 + k] = r;
   }
 }
 // simulator.py:522-540: the thermal volume must stay within [0, capacity].
-__global__ void k2_check_aperture(icy2d_grid g) {
+__device__ __forceinline__ void k2_check_aperture(const icy2d_grid& g, const int blk) {
   icy2d_control* ctl = g.control;
   const double total = ctl->aperture[0], full = ctl->aperture[3];
   const double tol = 1.0e-12 * fmax(g.area, fmax(fabs(total), fabs(full)));
@@ -1486,8 +1494,8 @@ __device__ __forceinline__ double aperture_target(const icy2d_grid& g, int i, in
   return fmin(g.area, fmax(0.0, target));
 }
 // simulator.py:716-827 (control aperture = {volume, energy, base, full, min, max})
-__global__ void k2_build_aperture_targets(icy2d_grid g) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_build_aperture_targets(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk);
   const icy2d_control* ctl = g.control;
   double v[3] = {0.0, 0.0, 0.0};
   if (n < cells(g)) {
@@ -1527,11 +1535,11 @@ __global__ void k2_build_aperture_targets(icy2d_grid g) {
     g.target_e[n] = energy;
     v[0] = energy;
   }
-  block_partials<3>(v, g.partials);
+  block_partials<3>(v, g.partials, blk);
 }
 // simulator.py:829-859
-__global__ void k2_apply_aperture_transfer(icy2d_grid g) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_apply_aperture_transfer(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk);
   if (n >= cells(g)) return;
   const icy2d_control* ctl = g.control;
   const int i = n / g.ny, j = n - i * g.ny;
@@ -1548,23 +1556,23 @@ __global__ void k2_apply_aperture_transfer(icy2d_grid g) {
   g.water_v[n] = target;
   g.water_e[n] = energy;
 }
-__global__ void k2_add_correction_abs(icy2d_grid g) {
+__device__ __forceinline__ void k2_add_correction_abs(const icy2d_grid& g, const int blk) {
   icy2d_control* ctl = g.control;
   ctl->correction_abs += fabs(ctl->aperture[1] - ctl->targets[0]);
 }
 // simulator.py:861-877
-__global__ void k2_water_totals_partials(icy2d_grid g) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_water_totals_partials(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk);
   double v[2] = {0.0, 0.0};
   if (n < cells(g)) {
     v[0] = g.water_v[n];
     v[1] = g.water_e[n];
   }
-  block_partials<2>(v, g.partials);
+  block_partials<2>(v, g.partials, blk);
 }
 // simulator.py:879-907
-__global__ void k2_body_conduction(icy2d_grid g, double dt) {
-  const int k = cell_index();
+__device__ __forceinline__ void k2_body_conduction(const icy2d_grid& g, const int blk, double dt) {
+  const int k = cell_index(blk);
   if (k == 0) g.control->unassigned[0] = g.control->unassigned[1] = 0.0;
   if (k >= g.body_nx * g.body_ny) return;
   const int bi = k / g.body_ny, bj = k - bi * g.body_ny;
@@ -1585,8 +1593,8 @@ __global__ void k2_body_conduction(icy2d_grid g, double dt) {
 }
 __device__ __forceinline__ double cell_aperture(const icy2d_grid& g, int n) { return fmin(1.0, fmax(0.0, g.water_v[n] / g.area)); }
 // simulator.py:909-1010 (axis 0) and :1063-1164 (axis 1); boundary power as partials
-__global__ void k2_conduction_flux(icy2d_grid g, int axis) {
-  const int t = cell_index();
+__device__ __forceinline__ void k2_conduction_flux(const icy2d_grid& g, const int blk, int axis) {
+  const int t = cell_index(blk);
   const int fx = axis == 0 ? g.nx + 1 : g.nx, fy = axis == 0 ? g.ny : g.ny + 1, limit = axis == 0 ? g.nx : g.ny;
   double inward[1] = {0.0};
   if (t < fx * fy) {
@@ -1611,11 +1619,11 @@ __global__ void k2_conduction_flux(icy2d_grid g, int axis) {
     }
     (axis == 0 ? g.flux_ex : g.flux_ey)[t] = out;
   }
-  block_partials<1>(inward, g.partials);
+  block_partials<1>(inward, g.partials, blk);
 }
 // simulator.py:1012-1061 (axis 0) and :1166-1215 (axis 1)
-__global__ void k2_advection_flux(icy2d_grid g, int axis) {
-  const int t = cell_index(), N = cells(g);
+__device__ __forceinline__ void k2_advection_flux(const icy2d_grid& g, const int blk, int axis) {
+  const int t = cell_index(blk), N = cells(g);
   const int fx = axis == 0 ? g.nx + 1 : g.nx, fy = axis == 0 ? g.ny : g.ny + 1, limit = axis == 0 ? g.nx : g.ny;
   if (t >= fx * fy) return;
   const int a = t / fy, b = t - a * fy, face = axis == 0 ? a : b;
@@ -1637,8 +1645,8 @@ __global__ void k2_advection_flux(icy2d_grid g, int axis) {
   (axis == 0 ? g.flux_vx : g.flux_vy)[t] = v;
 }
 // simulator.py:1217-1246
-__global__ void k2_update_water(icy2d_grid g, double dt, int volume) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_update_water(const icy2d_grid& g, const int blk, double dt, int volume) {
+  const int n = cell_index(blk);
   if (n >= cells(g)) return;
   const int i = n / g.ny, j = n - i * g.ny, ny = g.ny;
   if (wall_at(g, i, j)) return;
@@ -1652,13 +1660,13 @@ __global__ void k2_update_water(icy2d_grid g, double dt, int volume) {
   }
 }
 // simulator.py:624-627
-__global__ void k2_reset_interval_sources(icy2d_grid g) { g.control->interval[0] = g.control->interval[1] = 0.0; }
-__global__ void k2_accumulate_boundary_heat(icy2d_grid g, double dt) {
+__device__ __forceinline__ void k2_reset_interval_sources(const icy2d_grid& g, const int blk) { g.control->interval[0] = g.control->interval[1] = 0.0; }
+__device__ __forceinline__ void k2_accumulate_boundary_heat(const icy2d_grid& g, const int blk, double dt) {
   g.control->boundary_heat_input += dt * g.control->boundary_power;
 }
 // simulator.py:1252-1372
-__global__ void __launch_bounds__(kBlock) k2_exchange_heat(icy2d_grid g, double dt) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_exchange_heat(const icy2d_grid& g, const int blk, double dt) {
+  const int n = cell_index(blk);
   if (n >= cells(g)) return;
   const int i = n / g.ny, j = n - i * g.ny;
   if (!(water_cell(g, i, j) && g.water_v[n] > 1.0e-30)) return;
@@ -1719,8 +1727,8 @@ __global__ void __launch_bounds__(kBlock) k2_exchange_heat(icy2d_grid g, double 
   g.water_e[n] -= transferred;
 }
 // simulator.py:1374-1406 (interval melt as partials)
-__global__ void k2_phase_change(icy2d_grid g) {
-  const int k = cell_index();
+__device__ __forceinline__ void k2_phase_change(const icy2d_grid& g, const int blk) {
+  const int k = cell_index(blk);
   double v[1] = {0.0};
   if (k < g.body_nx * g.body_ny) {
     const double heat = g.body_heat[k];
@@ -1745,18 +1753,18 @@ __global__ void k2_phase_change(icy2d_grid g) {
     g.melt_sensible[k] = melt_sensible;
     if (melt > 0.0) v[0] = melt;
   }
-  block_partials<1>(v, g.partials);
+  block_partials<1>(v, g.partials, blk);
 }
 // simulator.py:1408-1422
-__global__ void k2_freeze_eligibility(icy2d_grid g) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_freeze_eligibility(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk);
   if (n >= cells(g)) return;
   const int i = n / g.ny, j = n - i * g.ny;
   g.eligible[n] = (int8_t)(water_cell(g, i, j) && g.water_v[n] > 1.0e-30);
 }
 // simulator.py:1424-1475 (assigned melt mass and the unassigned pool as partials)
-__global__ void k2_inject_melt(icy2d_grid g) {
-  const int k = cell_index();
+__device__ __forceinline__ void k2_inject_melt(const icy2d_grid& g, const int blk) {
+  const int k = cell_index(blk);
   double v[3] = {0.0, 0.0, 0.0};
   if (k < g.body_nx * g.body_ny && g.melt_mass[k] > 0.0) {
     const icy2d_control* ctl = g.control;
@@ -1797,9 +1805,9 @@ __global__ void k2_inject_melt(icy2d_grid g) {
       v[2] = energy;
     }
   }
-  block_partials<3>(v, g.partials);
+  block_partials<3>(v, g.partials, blk);
 }
-__global__ void k2_store_injection(icy2d_grid g) {
+__device__ __forceinline__ void k2_store_injection(const icy2d_grid& g, const int blk) {
   const double* t = g.partials + g.partials_capacity - 3;
   icy2d_control* ctl = g.control;
   ctl->interval[1] += t[0];
@@ -1807,8 +1815,8 @@ __global__ void k2_store_injection(icy2d_grid g) {
   ctl->unassigned[1] += t[2];
 }
 // simulator.py:1477-1502
-__global__ void k2_fallback_partials(icy2d_grid g) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_fallback_partials(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk);
   double v[2] = {0.0, 0.0};
   if (n < cells(g)) {
     const int i = n / g.ny, j = n - i * g.ny;
@@ -1817,11 +1825,11 @@ __global__ void k2_fallback_partials(icy2d_grid g) {
       v[1] = g.water_v[n];
     }
   }
-  block_partials<2>(v, g.partials);
+  block_partials<2>(v, g.partials, blk);
 }
 // simulator.py:1504-1540
-__global__ void k2_apply_fallback(icy2d_grid g, int use_free) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_apply_fallback(const icy2d_grid& g, const int blk, int use_free) {
+  const int n = cell_index(blk);
   if (n >= cells(g)) return;
   const icy2d_control* ctl = g.control;
   const double den = use_free ? ctl->fallback[0] : ctl->fallback[1];
@@ -1835,14 +1843,14 @@ __global__ void k2_apply_fallback(icy2d_grid g, int use_free) {
     g.water_e[n] += fr * energy;
   }
 }
-__global__ void k2_count_fallback(icy2d_grid g, int use_free) {
+__device__ __forceinline__ void k2_count_fallback(const icy2d_grid& g, const int blk, int use_free) {
   icy2d_control* ctl = g.control;
   const double den = use_free ? ctl->fallback[0] : ctl->fallback[1];
   if (den > 1.0e-30 && ctl->unassigned[0] > 0.0) ctl->interval[1] += ctl->unassigned[0];
 }
 // simulator.py:1565-1583
-__global__ void k2_totals_partials(icy2d_grid g) {
-  const int n = cell_index();
+__device__ __forceinline__ void k2_totals_partials(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk);
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   if (n < g.body_nx * g.body_ny) {
     v[0] = g.body_mass[n];
@@ -1852,397 +1860,522 @@ __global__ void k2_totals_partials(icy2d_grid g) {
     v[1] = g.rho_w_si * g.water_v[n];
     v[3] = g.water_e[n];
   }
-  block_partials<4>(v, g.partials);
+  block_partials<4>(v, g.partials, blk);
 }
 
 // ---------------------------------------------------------------- host
-unsigned blocks_for(long count) { return (unsigned)((count + kBlock - 1) / kBlock); }
+// Index spaces a kernel covers per case, and the launch wrappers.  Every
+// kernel body above takes (grid, block index within its case); one_case
+// runs it for one grid passed by value, many_cases for the case that owns
+// the block in a batch's prefix table.
+enum Shape : int { kCells = 0, kBody, kFaceX, kFaceY, kOne, kTotals, kCells128, kCells512, kShapes };
+static_assert(kShapes == ICY2D_SHAPES, "shape count");
+
+long shape_count(const icy2d_grid& g, int shape) {
+  const long cells = (long)g.nx * g.ny, body = (long)g.body_nx * g.body_ny;
+  switch (shape) {
+    case kBody: return body;
+    case kFaceX: return (long)(g.nx + 1) * g.ny;
+    case kFaceY: return (long)g.nx * (g.ny + 1);
+    case kOne: return 1;
+    case kTotals: return cells > body ? cells : body;
+    default: return cells;
+  }
+}
+int shape_threads(int shape) { return shape == kOne ? 1 : shape == kCells128 ? 128 : shape == kCells512 ? 512 : kBlock; }
+unsigned shape_blocks(const icy2d_grid& g, int shape) {
+  const long t = shape_threads(shape);
+  return (unsigned)((shape_count(g, shape) + t - 1) / t);
+}
+int cells_shape(int threads) { return threads == 128 ? kCells128 : threads == 512 ? kCells512 : kCells; }
+
+struct PerCase {
+  const double* v;
+};
+__device__ __forceinline__ double resolve(PerCase a, int c) { return a.v[c]; }
+template <class T>
+__device__ __forceinline__ T resolve(T a, int) {
+  return a;
+}
+
+template <int T, auto F, class... A>
+__global__ void __launch_bounds__(T) one_case(icy2d_grid g, A... a) {
+  F(g, (int)blockIdx.x, a...);
+}
+// Largest c with first[c] <= x (first is non-decreasing, first[0] = 0).
+__device__ __forceinline__ int owner(const int32_t* first, int ncases, int x) {
+  int lo = 0, hi = ncases;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (first[mid] <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+template <int T, auto F, class... A>
+__global__ void __launch_bounds__(T) many_cases(icy2d_batch b, int shape, A... a) {
+  const int32_t* first = b.first_block + shape * (b.ncases + 1);
+  const int c = owner(first, b.ncases, (int)blockIdx.x);
+  if (b.active && !b.active[c]) return;
+  const icy2d_grid g = b.grids[c];
+  F(g, (int)blockIdx.x - first[c], resolve(a, c)...);
+}
+
+// Destination of a finished reduction: a byte offset into the control block,
+// or the last `tail` slots of the case's partials.
+struct Dest {
+  long control_offset;
+  int tail;
+};
+__host__ __device__ inline double* dest_ptr(const icy2d_grid& g, Dest d) {
+  return d.tail ? g.partials + g.partials_capacity - d.tail : (double*)((char*)g.control + d.control_offset);
+}
+#define CTL(field, k) (Dest{(long)offsetof(icy2d_control, field) + 8 * (k), 0})
+#define TAIL(k) (Dest{0, (k)})
+
+__global__ void __launch_bounds__(kFinishThreads) k2_finish_many(icy2d_batch b, int shape, int K, Dest d,
+                                                                 unsigned min_mask, unsigned max_mask,
+                                                                 unsigned add_mask) {
+  const int c = (int)blockIdx.x / K, k = (int)blockIdx.x - c * K;
+  if (b.active && !b.active[c]) return;
+  const icy2d_grid g = b.grids[c];
+  const int32_t* first = b.first_block + shape * (b.ncases + 1);
+  finish_fold(g.partials, first[c + 1] - first[c], K, k, dest_ptr(g, d), min_mask, max_mask, add_mask);
+}
+
+int launch_status() {
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (int)e;
+}
+
+// One case (the single-case C ABI).
+struct One {
+  const icy2d_grid& g;
+  hipStream_t s;
+  using Real = double;
+  bool fits(int shape, int K) const { return (long)shape_blocks(g, shape) * K + 8 <= (long)g.partials_capacity; }
+  int tuned_shape(int) const { return kCells; }
+  template <auto F, class... A>
+  int launch(int shape, A... a) const {
+    one_case<kBlock, F><<<shape_blocks(g, shape), shape_threads(shape), 0, s>>>(g, a...);
+    return launch_status();
+  }
+  template <auto F, class... A>
+  int launch_tuned(int shape, A... a) const {
+    return launch<F>(shape, a...);
+  }
+  int finish(int shape, int K, Dest d, unsigned mn, unsigned mx, unsigned add) const {
+    k2_finish<<<K, kFinishThreads, 0, s>>>(g.partials, (int)shape_blocks(g, shape), K, dest_ptr(g, d), mn, mx, add);
+    return launch_status();
+  }
+};
+// Every case of a batch (capacities are checked once by icy2d_batch_plan).
+struct Many {
+  const icy2d_batch& b;
+  hipStream_t s;
+  using Real = PerCase;
+  bool fits(int, int) const { return true; }
+  int tuned_shape(int which) const { return cells_shape(which == 0 ? b.collide_threads : b.stream_threads); }
+  template <auto F, class... A>
+  int launch(int shape, A... a) const {
+    many_cases<kBlock, F><<<b.total_blocks[shape], shape_threads(shape), 0, s>>>(b, shape, a...);
+    return launch_status();
+  }
+  template <auto F, class... A>
+  int launch_tuned(int shape, A... a) const {
+    const unsigned blocks = b.total_blocks[shape];
+    if (shape == kCells128) many_cases<128, F><<<blocks, 128, 0, s>>>(b, shape, a...);
+    else if (shape == kCells512) many_cases<512, F><<<blocks, 512, 0, s>>>(b, shape, a...);
+    else many_cases<kBlock, F><<<blocks, kBlock, 0, s>>>(b, shape, a...);
+    return launch_status();
+  }
+  int finish(int shape, int K, Dest d, unsigned mn, unsigned mx, unsigned add) const {
+    k2_finish_many<<<b.ncases * K, kFinishThreads, 0, s>>>(b, shape, K, d, mn, mx, add);
+    return launch_status();
+  }
+};
+
+#define TRY(expr)                \
+  do {                           \
+    if (int rc_ = (expr)) return rc_; \
+  } while (0)
+#define NEED(d, shape, K)                              \
+  do {                                                 \
+    if (!(d).fits((shape), (K))) return ICY2D_ERR_ARGUMENT; \
+  } while (0)
+
+// ------------------------------------------------------------ operations
+template <class D> int op_body_fraction(const D& d, int only_if_projected) {
+  return d.template launch<k2_body_fraction>(kCells, only_if_projected);
+}
+template <class D> int op_commit_raster(const D& d) { return d.template launch<k2_commit_raster>(kCells); }
+template <class D> int op_contact_support(const D& d, int reduce, int only_if_projected) {
+  NEED(d, kCells, 4);
+  if (reduce) {
+    TRY(d.template launch<k2_support_partials>(kCells, only_if_projected));
+    // min x, max x, min y, max y into the last four partials slots
+    TRY(d.finish(kCells, 4, TAIL(4), 0x5u, 0xAu, 0u));
+  }
+  return d.template launch<k2_support_store>(kOne, reduce, only_if_projected);
+}
+template <class D> int op_integrate_rigid(const D& d) { return d.template launch<k2_integrate_rigid>(kOne); }
+template <class D> int op_project_inside(const D& d) { return d.template launch<k2_project_inside>(kOne); }
+template <class D> int op_update_solid_mask(const D& d, int init) {
+  return d.template launch<k2_update_solid_mask>(kCells, init);
+}
+template <class D> int op_reduce_mass_properties(const D& d) {
+  NEED(d, kBody, 4);
+  TRY(d.template launch<k2_mass_partials>(kBody));
+  return d.finish(kBody, 4, CTL(reduced, 0), 0u, 0u, 0u);
+}
+template <class D> int op_apply_mass_properties(const D& d, int initialize) {
+  return d.template launch<k2_apply_mass_properties>(kOne, initialize);
+}
+template <class D> int op_collide(const D& d, int with_momentum) {
+  const int shape = d.tuned_shape(0);
+  NEED(d, shape, 3);
+  TRY(d.template launch_tuned<k2_collide>(shape, with_momentum));
+  if (!with_momentum) return 0;
+  return d.finish(shape, 3, CTL(impulse, 0), 0u, 0u, 0x7u);
+}
+template <class D> int op_prepare_cut_links(const D& d) {
+  NEED(d, kCells, 3);
+  TRY(d.template launch<k2_prepare_cut_links>(kCells));
+  return d.finish(kCells, 3, CTL(impulse, 0), 0u, 0u, 0x7u);
+}
+template <class D> int op_stream(const D& d, int phase_only, int mode) {
+  if (mode < 1 || mode > 3) return ICY2D_ERR_ARGUMENT;
+  return d.template launch_tuned<k2_stream>(d.tuned_shape(1), phase_only, mode);
+}
+template <class D> int op_update_pressure(const D& d) { return d.template launch<k2_update_pressure>(kCells); }
+template <class D> int op_refill(const D& d) { return d.template launch<k2_refill>(kCells); }
+template <class D> int op_measure_outflow(const D& d) {
+  NEED(d, kCells, 1);
+  TRY(d.template launch<k2_outflow_partials>(kCells));
+  TRY(d.finish(kCells, 1, TAIL(1), 0u, 0x1u, 0u));
+  return d.template launch<k2_store_outflow>(kOne);
+}
+template <class D> int op_clip_phase(const D& d, double max_bulk_l1) {
+  NEED(d, kCells, 1);
+  TRY(d.template launch<k2_clip_phase>(kCells, max_bulk_l1));
+  return d.finish(kCells, 1, CTL(water, 1), 0u, 0u, 0u);
+}
+template <class D> int op_seed_projection(const D& d) { return d.template launch<k2_seed_projection>(kCells); }
+template <class D> int op_dilate_projection(const D& d, int primary_to_next) {
+  return d.template launch<k2_dilate_projection>(kCells, primary_to_next);
+}
+template <class D> int op_evaluate_projection(const D& d, typename D::Real lambda) {
+  NEED(d, kCells, 2);
+  TRY(d.template launch<k2_evaluate_projection>(kCells, lambda));
+  return d.finish(kCells, 2, CTL(water, 1), 0u, 0u, 0u);
+}
+template <class D> int op_apply_projection(const D& d, typename D::Real lambda) {
+  NEED(d, kCells, 1);
+  TRY(d.template launch<k2_apply_projection>(kCells, lambda));
+  return d.finish(kCells, 1, CTL(water, 1), 0u, 0u, 0u);
+}
+template <class D> int op_projection_residual_weight(const D& d, typename D::Real residual) {
+  NEED(d, kCells, 1);
+  TRY(d.template launch<k2_residual_weight>(kCells, residual));
+  return d.finish(kCells, 1, CTL(water, 2), 0u, 0u, 0u);
+}
+template <class D>
+int op_apply_projection_residual(const D& d, typename D::Real residual, typename D::Real weight) {
+  NEED(d, kCells, 1);
+  TRY(d.template launch<k2_apply_residual>(kCells, residual, weight));
+  return d.finish(kCells, 1, CTL(water, 1), 0u, 0u, 0u);
+}
+template <class D> int fluid_momentum(const D& d, Dest out) {
+  NEED(d, kCells, 3);
+  TRY(d.template launch<k2_momentum_partials>(kCells));
+  return d.finish(kCells, 3, out, 0u, 0u, 0u);
+}
+template <class D> int op_snapshot_melt_momentum(const D& d) {
+  TRY(d.template launch<k2_snapshot_body>(kOne));
+  return fluid_momentum(d, CTL(before, 0));
+}
+template <class D> int op_reduce_provisional_momentum(const D& d) {
+  TRY(fluid_momentum(d, CTL(provisional, 0)));
+  TRY(d.template launch<k2_reset_wet>(kOne));
+  TRY(d.template launch<k2_wet_partials>(kCells));
+  TRY(d.finish(kCells, 1, CTL(wet, 0), 0u, 0u, 0u));
+  return d.template launch<k2_copy_provisional>(kOne);
+}
+template <class D> int op_melt_weight_geometry(const D& d) {
+  NEED(d, kCells, 3);
+  TRY(d.template launch<k2_weight_partials>(kCells));
+  TRY(d.finish(kCells, 3, CTL(wet, 1), 0u, 0u, 0u));
+  return d.template launch<k2_melt_status>(kOne);
+}
+template <class D> int op_apply_melt_correction(const D& d) {
+  return d.template launch<k2_apply_melt_correction>(kCells);
+}
+template <class D> int op_reduce_final_momentum(const D& d) { return fluid_momentum(d, CTL(after, 0)); }
+template <class D> int op_local_melt_correction(const D& d) {
+  return d.template launch<k2_local_melt_correction>(kOne);
+}
+template <class D> int op_finish_melt_momentum(const D& d) {
+  return d.template launch<k2_finish_melt_momentum>(kOne);
+}
+template <class D> int op_measure_aperture(const D& d) {
+  NEED(d, kCells, 6);
+  TRY(d.template launch<k2_aperture_partials>(kCells));
+  TRY(d.finish(kCells, 6, CTL(aperture, 0), 0x10u, 0x20u, 0u));
+  return d.template launch<k2_check_aperture>(kOne);
+}
+template <class D> int op_build_aperture_targets(const D& d) {
+  NEED(d, kCells, 3);
+  TRY(d.template launch<k2_build_aperture_targets>(kCells));
+  return d.finish(kCells, 3, CTL(targets, 0), 0u, 0u, 0u);
+}
+template <class D> int op_apply_aperture_transfer(const D& d) {
+  TRY(d.template launch<k2_add_correction_abs>(kOne));
+  return d.template launch<k2_apply_aperture_transfer>(kCells);
+}
+template <class D> int op_finish_aperture_transfer(const D& d) {
+  NEED(d, kCells, 2);
+  TRY(d.template launch<k2_water_totals_partials>(kCells));
+  return d.finish(kCells, 2, CTL(transfer, 0), 0u, 0u, 0u);
+}
+template <class D> int op_body_conduction(const D& d, typename D::Real dt) {
+  return d.template launch<k2_body_conduction>(kBody, dt);
+}
+template <class D> int op_conduction_flux(const D& d, int axis) {
+  if (axis != 0 && axis != 1) return ICY2D_ERR_ARGUMENT;
+  const int shape = axis == 0 ? kFaceX : kFaceY;
+  NEED(d, shape, 1);
+  TRY(d.template launch<k2_conduction_flux>(shape, axis));
+  // simulator.py:630-631: the power is reset before the x faces and summed over both passes
+  return d.finish(shape, 1, CTL(boundary_power, 0), 0u, 0u, axis == 0 ? 0u : 0x1u);
+}
+template <class D> int op_advection_flux(const D& d, int axis) {
+  if (axis != 0 && axis != 1) return ICY2D_ERR_ARGUMENT;
+  return d.template launch<k2_advection_flux>(axis == 0 ? kFaceX : kFaceY, axis);
+}
+template <class D> int op_update_water(const D& d, typename D::Real dt, int volume) {
+  return d.template launch<k2_update_water>(kCells, dt, volume);
+}
+template <class D> int op_reset_interval_sources(const D& d) {
+  return d.template launch<k2_reset_interval_sources>(kOne);
+}
+template <class D> int op_accumulate_boundary_heat(const D& d, typename D::Real dt) {
+  return d.template launch<k2_accumulate_boundary_heat>(kOne, dt);
+}
+template <class D> int op_exchange_heat(const D& d, typename D::Real dt) {
+  return d.template launch<k2_exchange_heat>(kCells, dt);
+}
+template <class D> int op_phase_change(const D& d) {
+  NEED(d, kBody, 1);
+  TRY(d.template launch<k2_phase_change>(kBody));
+  return d.finish(kBody, 1, CTL(interval, 0), 0u, 0u, 0x1u);
+}
+template <class D> int op_freeze_eligibility(const D& d) {
+  return d.template launch<k2_freeze_eligibility>(kCells);
+}
+template <class D> int op_inject_melt(const D& d) {
+  NEED(d, kBody, 3);
+  TRY(d.template launch<k2_inject_melt>(kBody));
+  // partial columns: assigned melt, unassigned mass, unassigned energy
+  TRY(d.finish(kBody, 3, TAIL(3), 0u, 0u, 0u));
+  return d.template launch<k2_store_injection>(kOne);
+}
+template <class D> int op_measure_fallback(const D& d) {
+  NEED(d, kCells, 2);
+  TRY(d.template launch<k2_fallback_partials>(kCells));
+  return d.finish(kCells, 2, CTL(fallback, 0), 0u, 0u, 0u);
+}
+template <class D> int op_apply_fallback(const D& d, int use_free) {
+  TRY(d.template launch<k2_count_fallback>(kOne, use_free));
+  return d.template launch<k2_apply_fallback>(kCells, use_free);
+}
+template <class D> int op_reduce_totals(const D& d) {
+  NEED(d, kTotals, 4);
+  TRY(d.template launch<k2_totals_partials>(kTotals));
+  return d.finish(kTotals, 4, CTL(totals, 0), 0u, 0u, 0u);
+}
 
 bool valid(const icy2d_grid* g) {
   return g && g->nx > 0 && g->ny > 0 && g->body_nx > 0 && g->body_ny > 0 && g->control && g->partials &&
          g->partials_capacity >= 8;
 }
-int launch_status() {
-  const hipError_t e = hipGetLastError();
-  return e == hipSuccess ? 0 : (int)e;
+bool valid_batch(const icy2d_batch* b) {
+  if (!b || b->ncases <= 0 || !b->grids || !b->first_block) return false;
+  for (int t : {b->collide_threads, b->stream_threads})
+    if (t != 128 && t != 256 && t != 512) return false;
+  return true;
 }
-// Launch a partials-producing kernel's finish into out[0..K).
-int finish(const icy2d_grid& g, unsigned blocks, int K, double* out, unsigned min_mask, unsigned max_mask,
-           unsigned add_mask, hipStream_t s) {
-  k2_finish<<<K, kFinishThreads, 0, s>>>(g.partials, (int)blocks, K, out, min_mask, max_mask, add_mask);
-  return launch_status();
+// Partials a case needs for its largest reduction (K columns per block plus
+// the eight tail slots).
+long partials_needed(const icy2d_grid& g) {
+  long need = 0;
+  auto take = [&](int shape, int K) {
+    const long v = (long)shape_blocks(g, shape) * K;
+    if (v > need) need = v;
+  };
+  take(kCells, 6);
+  take(kCells128, 3);
+  take(kCells512, 3);
+  take(kBody, 4);
+  take(kFaceX, 1);
+  take(kFaceY, 1);
+  take(kTotals, 4);
+  return need + 8;
 }
-bool fits(const icy2d_grid& g, unsigned blocks, int K) { return (long)blocks * K + 8 <= (long)g.partials_capacity; }
 
 }  // namespace icy2
 
 using namespace icy2;
 
-#define ICY2D_GRID_CHECK(g)                     \
-  do {                                          \
-    if (!valid(g)) return ICY2D_ERR_ARGUMENT;   \
-  } while (0)
 #define S(stream) ((hipStream_t)(stream))
+#define SINGLE(name, params, ...)                                 \
+  int icy2d_##name(const icy2d_grid* g ICY2D_EXPAND params, void* stream) { \
+    if (!valid(g)) return ICY2D_ERR_ARGUMENT;                      \
+    return op_##name(One{*g, S(stream)} __VA_ARGS__);              \
+  }
+#define BATCH(name, params, ...)                                         \
+  int icy2d_batch_##name(const icy2d_batch* b ICY2D_EXPAND params, void* stream) { \
+    if (!valid_batch(b)) return ICY2D_ERR_ARGUMENT;                       \
+    return op_##name(Many{*b, S(stream)} __VA_ARGS__);                    \
+  }
+#define ICY2D_EXPAND(...) __VA_ARGS__
+#define BOTH(name, params, ...) SINGLE(name, params, __VA_ARGS__) BATCH(name, params, __VA_ARGS__)
+#define PC(x) PerCase{x}
 
 extern "C" {
 
-int icy2d_body_fraction(const icy2d_grid* g, int only_if_projected, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_body_fraction<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, only_if_projected);
-  return launch_status();
-}
-int icy2d_commit_raster(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_commit_raster<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
-  return launch_status();
-}
-int icy2d_contact_support(const icy2d_grid* g, int reduce, int only_if_projected, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 4)) return ICY2D_ERR_ARGUMENT;
-  if (reduce) {
-    k2_support_partials<<<blocks, kBlock, 0, S(stream)>>>(*g, only_if_projected);
-    if (int rc = launch_status()) return rc;
-    // min x, max x, min y, max y into the last four partials slots
-    if (int rc = finish(*g, blocks, 4, g->partials + g->partials_capacity - 4, 0x5u, 0xAu, 0u, S(stream))) return rc;
-  }
-  k2_support_store<<<1, 1, 0, S(stream)>>>(*g, reduce, only_if_projected);
-  return launch_status();
-}
-int icy2d_integrate_rigid(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_integrate_rigid<<<1, 1, 0, S(stream)>>>(*g);
-  return launch_status();
-}
-int icy2d_project_inside(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_project_inside<<<1, 1, 0, S(stream)>>>(*g);
-  return launch_status();
-}
-int icy2d_update_solid_mask(const icy2d_grid* g, int init, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_update_solid_mask<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, init);
-  return launch_status();
-}
-int icy2d_reduce_mass_properties(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->body_nx * g->body_ny);
-  if (!fits(*g, blocks, 4)) return ICY2D_ERR_ARGUMENT;
-  k2_mass_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
-  if (int rc = launch_status()) return rc;
-  return finish(*g, blocks, 4, g->control->reduced, 0u, 0u, 0u, S(stream));
-}
-int icy2d_apply_mass_properties(const icy2d_grid* g, int initialize, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_apply_mass_properties<<<1, 1, 0, S(stream)>>>(*g, initialize);
-  return launch_status();
-}
+BOTH(body_fraction, (, int only_if_projected), , only_if_projected)
+BOTH(commit_raster, ())
+BOTH(contact_support, (, int reduce, int only_if_projected), , reduce, only_if_projected)
+BOTH(integrate_rigid, ())
+BOTH(project_inside, ())
+BOTH(update_solid_mask, (, int init), , init)
+BOTH(reduce_mass_properties, ())
+BOTH(apply_mass_properties, (, int initialize), , initialize)
+BOTH(collide, (, int with_momentum), , with_momentum)
+BOTH(stream, (, int phase_only, int mode), , phase_only, mode)
+BOTH(update_pressure, ())
+BOTH(refill, ())
+BOTH(measure_outflow, ())
+BOTH(clip_phase, (, double max_bulk_l1), , max_bulk_l1)
+BOTH(seed_projection, ())
+BOTH(dilate_projection, (, int primary_to_next), , primary_to_next)
+BOTH(snapshot_melt_momentum, ())
+BOTH(reduce_provisional_momentum, ())
+BOTH(melt_weight_geometry, ())
+BOTH(apply_melt_correction, ())
+BOTH(reduce_final_momentum, ())
+BOTH(local_melt_correction, ())
+BOTH(finish_melt_momentum, ())
+BOTH(measure_aperture, ())
+BOTH(build_aperture_targets, ())
+BOTH(apply_aperture_transfer, ())
+BOTH(finish_aperture_transfer, ())
+BOTH(conduction_flux, (, int axis), , axis)
+BOTH(advection_flux, (, int axis), , axis)
+BOTH(reset_interval_sources, ())
+BOTH(phase_change, ())
+BOTH(freeze_eligibility, ())
+BOTH(inject_melt, ())
+BOTH(measure_fallback, ())
+BOTH(apply_fallback, (, int use_free), , use_free)
+BOTH(reduce_totals, ())
+SINGLE(prepare_cut_links, ())
+
+// Entry points whose double arguments are one value per case in a batch.
+SINGLE(evaluate_projection, (, double lambda), , lambda)
+BATCH(evaluate_projection, (, const double* lambda), , PC(lambda))
+SINGLE(apply_projection, (, double lambda), , lambda)
+BATCH(apply_projection, (, const double* lambda), , PC(lambda))
+SINGLE(projection_residual_weight, (, double residual), , residual)
+BATCH(projection_residual_weight, (, const double* residual), , PC(residual))
+SINGLE(apply_projection_residual, (, double residual, double weight), , residual, weight)
+BATCH(apply_projection_residual, (, const double* residual, const double* weight), , PC(residual), PC(weight))
+SINGLE(body_conduction, (, double dt), , dt)
+BATCH(body_conduction, (, const double* dt), , PC(dt))
+SINGLE(update_water, (, double dt, int volume), , dt, volume)
+BATCH(update_water, (, const double* dt, int volume), , PC(dt), volume)
+SINGLE(accumulate_boundary_heat, (, double dt), , dt)
+BATCH(accumulate_boundary_heat, (, const double* dt), , PC(dt))
+SINGLE(exchange_heat, (, double dt), , dt)
+BATCH(exchange_heat, (, const double* dt), , PC(dt))
+
+// Initialisation (single case).
 int icy2d_initialize_fluid(const icy2d_grid* g, int water_width, int water_height, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_initialize_fluid<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, water_width, water_height);
-  return launch_status();
+  if (!valid(g)) return ICY2D_ERR_ARGUMENT;
+  return One{*g, S(stream)}.launch<k2_initialize_fluid>(kCells, water_width, water_height);
 }
 int icy2d_hydrostatic_equilibrium(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_hydrostatic_equilibrium<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
-  return launch_status();
+  if (!valid(g)) return ICY2D_ERR_ARGUMENT;
+  return One{*g, S(stream)}.launch<k2_hydrostatic_equilibrium>(kCells);
 }
 int icy2d_initialize_water_volume(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
-  k2_water_volume_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
-  if (int rc = launch_status()) return rc;
-  if (int rc = finish(*g, blocks, 1, g->control->water, 0u, 0u, 0u, S(stream))) return rc;
-  k2_copy_initial_volume<<<1, 1, 0, S(stream)>>>(*g);
-  return launch_status();
+  if (!valid(g)) return ICY2D_ERR_ARGUMENT;
+  const One d{*g, S(stream)};
+  NEED(d, kCells, 1);
+  TRY(d.launch<k2_water_volume_partials>(kCells));
+  TRY(d.finish(kCells, 1, CTL(water, 0), 0u, 0u, 0u));
+  return d.launch<k2_copy_initial_volume>(kOne);
 }
 int icy2d_recover_initial_moments(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_recover_initial_moments<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
-  return launch_status();
+  if (!valid(g)) return ICY2D_ERR_ARGUMENT;
+  return One{*g, S(stream)}.launch<k2_recover_initial_moments>(kCells);
 }
 int icy2d_initialize_body_state(const icy2d_grid* g, double ice_temperature, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_initialize_body_state<<<blocks_for((long)g->body_nx * g->body_ny), kBlock, 0, S(stream)>>>(*g, ice_temperature);
-  return launch_status();
+  if (!valid(g)) return ICY2D_ERR_ARGUMENT;
+  return One{*g, S(stream)}.launch<k2_initialize_body_state>(kBody, ice_temperature);
 }
 int icy2d_initialize_water_state(const icy2d_grid* g, double water_temperature_c, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_initialize_water_state<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, water_temperature_c);
-  if (int rc = launch_status()) return rc;
-  k2_reset_thermal_diagnostics<<<1, 1, 0, S(stream)>>>(*g);
-  return launch_status();
+  if (!valid(g)) return ICY2D_ERR_ARGUMENT;
+  const One d{*g, S(stream)};
+  TRY(d.launch<k2_initialize_water_state>(kCells, water_temperature_c));
+  return d.launch<k2_reset_thermal_diagnostics>(kOne);
 }
-int icy2d_collide(const icy2d_grid* g, int with_momentum, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 3)) return ICY2D_ERR_ARGUMENT;
-  k2_collide<<<blocks, kBlock, 0, S(stream)>>>(*g, with_momentum);
-  if (int rc = launch_status()) return rc;
-  if (!with_momentum) return 0;
-  return finish(*g, blocks, 3, g->control->impulse, 0u, 0u, 0x7u, S(stream));
+
+int icy2d_batch_plan(const icy2d_grid* grids, int32_t ncases, int32_t* first_block, int32_t* total_blocks) {
+  if (!grids || ncases <= 0 || !first_block || !total_blocks) return ICY2D_ERR_ARGUMENT;
+  for (int c = 0; c < ncases; ++c)
+    if (!valid(&grids[c]) || partials_needed(grids[c]) > (long)grids[c].partials_capacity) return ICY2D_ERR_ARGUMENT;
+  for (int shape = 0; shape < kShapes; ++shape) {
+    int32_t* first = first_block + shape * (ncases + 1);
+    long total = 0;
+    for (int c = 0; c < ncases; ++c) {
+      first[c] = (int32_t)total;
+      total += shape_blocks(grids[c], shape);
+    }
+    if (total > 0x7fffffffL) return ICY2D_ERR_ARGUMENT;
+    first[ncases] = (int32_t)total;
+    total_blocks[shape] = (int32_t)total;
+  }
+  return 0;
 }
-int icy2d_prepare_cut_links(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 3)) return ICY2D_ERR_ARGUMENT;
-  k2_prepare_cut_links<<<blocks, kBlock, 0, S(stream)>>>(*g);
-  if (int rc = launch_status()) return rc;
-  return finish(*g, blocks, 3, g->control->impulse, 0u, 0u, 0x7u, S(stream));
+void icy2d_batch_abi_size(int32_t* batch) {
+  if (batch) *batch = (int32_t)sizeof(icy2d_batch);
 }
-int icy2d_stream(const icy2d_grid* g, int phase_only, int mode, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  if (mode < 1 || mode > 3) return ICY2D_ERR_ARGUMENT;
-  k2_stream<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, phase_only, mode);
-  return launch_status();
+
+int icy2d_graph_begin(void* stream) {
+  return (int)hipStreamBeginCapture(S(stream), hipStreamCaptureModeThreadLocal);
 }
-int icy2d_update_pressure(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_update_pressure<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
-  return launch_status();
+int icy2d_graph_end(void* stream, void** graph) {
+  if (!graph) return ICY2D_ERR_ARGUMENT;
+  *graph = nullptr;
+  hipGraph_t captured = nullptr;
+  if (hipError_t e = hipStreamEndCapture(S(stream), &captured)) return (int)e;
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, captured, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(captured);
+  if (e != hipSuccess) return (int)e;
+  *graph = (void*)exec;
+  return 0;
 }
-int icy2d_refill(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_refill<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
-  return launch_status();
+int icy2d_graph_launch(void* graph, void* stream) {
+  if (!graph) return ICY2D_ERR_ARGUMENT;
+  return (int)hipGraphLaunch((hipGraphExec_t)graph, S(stream));
 }
-int icy2d_measure_outflow(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
-  k2_outflow_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
-  if (int rc = launch_status()) return rc;
-  if (int rc = finish(*g, blocks, 1, g->partials + g->partials_capacity - 1, 0u, 0x1u, 0u, S(stream))) return rc;
-  k2_store_outflow<<<1, 1, 0, S(stream)>>>(*g);
-  return launch_status();
+int icy2d_graph_destroy(void* graph) {
+  if (!graph) return 0;
+  return (int)hipGraphExecDestroy((hipGraphExec_t)graph);
 }
-int icy2d_clip_phase(const icy2d_grid* g, double max_bulk_l1, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
-  k2_clip_phase<<<blocks, kBlock, 0, S(stream)>>>(*g, max_bulk_l1);
-  if (int rc = launch_status()) return rc;
-  return finish(*g, blocks, 1, g->control->water + 1, 0u, 0u, 0u, S(stream));
-}
-int icy2d_seed_projection(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_seed_projection<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
-  return launch_status();
-}
-int icy2d_dilate_projection(const icy2d_grid* g, int primary_to_next, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_dilate_projection<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, primary_to_next);
-  return launch_status();
-}
-int icy2d_evaluate_projection(const icy2d_grid* g, double lambda, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 2)) return ICY2D_ERR_ARGUMENT;
-  k2_evaluate_projection<<<blocks, kBlock, 0, S(stream)>>>(*g, std::exp(lambda));
-  if (int rc = launch_status()) return rc;
-  return finish(*g, blocks, 2, g->control->water + 1, 0u, 0u, 0u, S(stream));
-}
-int icy2d_apply_projection(const icy2d_grid* g, double lambda, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
-  k2_apply_projection<<<blocks, kBlock, 0, S(stream)>>>(*g, std::exp(lambda));
-  if (int rc = launch_status()) return rc;
-  return finish(*g, blocks, 1, g->control->water + 1, 0u, 0u, 0u, S(stream));
-}
-int icy2d_projection_residual_weight(const icy2d_grid* g, double residual, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
-  k2_residual_weight<<<blocks, kBlock, 0, S(stream)>>>(*g, residual);
-  if (int rc = launch_status()) return rc;
-  return finish(*g, blocks, 1, g->control->water + 2, 0u, 0u, 0u, S(stream));
-}
-int icy2d_apply_projection_residual(const icy2d_grid* g, double residual, double weight, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
-  k2_apply_residual<<<blocks, kBlock, 0, S(stream)>>>(*g, residual, weight);
-  if (int rc = launch_status()) return rc;
-  return finish(*g, blocks, 1, g->control->water + 1, 0u, 0u, 0u, S(stream));
-}
-static int fluid_momentum(const icy2d_grid* g, double* out, hipStream_t s) {
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 3)) return ICY2D_ERR_ARGUMENT;
-  k2_momentum_partials<<<blocks, kBlock, 0, s>>>(*g);
-  if (int rc = launch_status()) return rc;
-  return finish(*g, blocks, 3, out, 0u, 0u, 0u, s);
-}
-int icy2d_snapshot_melt_momentum(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_snapshot_body<<<1, 1, 0, S(stream)>>>(*g);
-  if (int rc = launch_status()) return rc;
-  return fluid_momentum(g, g->control->before, S(stream));
-}
-int icy2d_reduce_provisional_momentum(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  if (int rc = fluid_momentum(g, g->control->provisional, S(stream))) return rc;
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  k2_reset_wet<<<1, 1, 0, S(stream)>>>(*g);
-  k2_wet_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
-  if (int rc = launch_status()) return rc;
-  if (int rc = finish(*g, blocks, 1, g->control->wet, 0u, 0u, 0u, S(stream))) return rc;
-  k2_copy_provisional<<<1, 1, 0, S(stream)>>>(*g);
-  return launch_status();
-}
-int icy2d_melt_weight_geometry(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 3)) return ICY2D_ERR_ARGUMENT;
-  k2_weight_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
-  if (int rc = launch_status()) return rc;
-  if (int rc = finish(*g, blocks, 3, g->control->wet + 1, 0u, 0u, 0u, S(stream))) return rc;
-  k2_melt_status<<<1, 1, 0, S(stream)>>>(*g);
-  return launch_status();
-}
-int icy2d_apply_melt_correction(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_apply_melt_correction<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
-  return launch_status();
-}
-int icy2d_reduce_final_momentum(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  return fluid_momentum(g, g->control->after, S(stream));
-}
-int icy2d_local_melt_correction(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_local_melt_correction<<<1, 1, 0, S(stream)>>>(*g);
-  return launch_status();
-}
-int icy2d_finish_melt_momentum(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_finish_melt_momentum<<<1, 1, 0, S(stream)>>>(*g);
-  return launch_status();
-}
-int icy2d_measure_aperture(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 6)) return ICY2D_ERR_ARGUMENT;
-  k2_aperture_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
-  if (int rc = launch_status()) return rc;
-  if (int rc = finish(*g, blocks, 6, g->control->aperture, 0x10u, 0x20u, 0u, S(stream))) return rc;
-  k2_check_aperture<<<1, 1, 0, S(stream)>>>(*g);
-  return launch_status();
-}
-int icy2d_build_aperture_targets(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 3)) return ICY2D_ERR_ARGUMENT;
-  k2_build_aperture_targets<<<blocks, kBlock, 0, S(stream)>>>(*g);
-  if (int rc = launch_status()) return rc;
-  return finish(*g, blocks, 3, g->control->targets, 0u, 0u, 0u, S(stream));
-}
-int icy2d_apply_aperture_transfer(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_add_correction_abs<<<1, 1, 0, S(stream)>>>(*g);
-  k2_apply_aperture_transfer<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
-  return launch_status();
-}
-int icy2d_finish_aperture_transfer(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 2)) return ICY2D_ERR_ARGUMENT;
-  k2_water_totals_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
-  if (int rc = launch_status()) return rc;
-  return finish(*g, blocks, 2, g->control->transfer, 0u, 0u, 0u, S(stream));
-}
-int icy2d_body_conduction(const icy2d_grid* g, double dt, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_body_conduction<<<blocks_for((long)g->body_nx * g->body_ny), kBlock, 0, S(stream)>>>(*g, dt);
-  return launch_status();
-}
-int icy2d_conduction_flux(const icy2d_grid* g, int axis, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  if (axis != 0 && axis != 1) return ICY2D_ERR_ARGUMENT;
-  const long faces = axis == 0 ? (long)(g->nx + 1) * g->ny : (long)g->nx * (g->ny + 1);
-  const unsigned blocks = blocks_for(faces);
-  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
-  k2_conduction_flux<<<blocks, kBlock, 0, S(stream)>>>(*g, axis);
-  if (int rc = launch_status()) return rc;
-  // simulator.py:630-631: the power is reset before the x faces and summed over both passes
-  return finish(*g, blocks, 1, &g->control->boundary_power, 0u, 0u, axis == 0 ? 0u : 0x1u, S(stream));
-}
-int icy2d_advection_flux(const icy2d_grid* g, int axis, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  if (axis != 0 && axis != 1) return ICY2D_ERR_ARGUMENT;
-  const long faces = axis == 0 ? (long)(g->nx + 1) * g->ny : (long)g->nx * (g->ny + 1);
-  k2_advection_flux<<<blocks_for(faces), kBlock, 0, S(stream)>>>(*g, axis);
-  return launch_status();
-}
-int icy2d_update_water(const icy2d_grid* g, double dt, int volume, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_update_water<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, dt, volume);
-  return launch_status();
-}
-int icy2d_reset_interval_sources(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_reset_interval_sources<<<1, 1, 0, S(stream)>>>(*g);
-  return launch_status();
-}
-int icy2d_accumulate_boundary_heat(const icy2d_grid* g, double dt, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_accumulate_boundary_heat<<<1, 1, 0, S(stream)>>>(*g, dt);
-  return launch_status();
-}
-int icy2d_exchange_heat(const icy2d_grid* g, double dt, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_exchange_heat<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, dt);
-  return launch_status();
-}
-int icy2d_phase_change(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->body_nx * g->body_ny);
-  if (!fits(*g, blocks, 1)) return ICY2D_ERR_ARGUMENT;
-  k2_phase_change<<<blocks, kBlock, 0, S(stream)>>>(*g);
-  if (int rc = launch_status()) return rc;
-  return finish(*g, blocks, 1, g->control->interval, 0u, 0u, 0x1u, S(stream));
-}
-int icy2d_freeze_eligibility(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_freeze_eligibility<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g);
-  return launch_status();
-}
-int icy2d_inject_melt(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->body_nx * g->body_ny);
-  if (!fits(*g, blocks, 3)) return ICY2D_ERR_ARGUMENT;
-  k2_inject_melt<<<blocks, kBlock, 0, S(stream)>>>(*g);
-  if (int rc = launch_status()) return rc;
-  // partial columns: assigned melt, unassigned mass, unassigned energy
-  if (int rc = finish(*g, blocks, 3, g->partials + g->partials_capacity - 3, 0u, 0u, 0u, S(stream))) return rc;
-  k2_store_injection<<<1, 1, 0, S(stream)>>>(*g);
-  return launch_status();
-}
-int icy2d_measure_fallback(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const unsigned blocks = blocks_for((long)g->nx * g->ny);
-  if (!fits(*g, blocks, 2)) return ICY2D_ERR_ARGUMENT;
-  k2_fallback_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
-  if (int rc = launch_status()) return rc;
-  return finish(*g, blocks, 2, g->control->fallback, 0u, 0u, 0u, S(stream));
-}
-int icy2d_apply_fallback(const icy2d_grid* g, int use_free, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  k2_count_fallback<<<1, 1, 0, S(stream)>>>(*g, use_free);
-  k2_apply_fallback<<<blocks_for((long)g->nx * g->ny), kBlock, 0, S(stream)>>>(*g, use_free);
-  return launch_status();
-}
-int icy2d_reduce_totals(const icy2d_grid* g, void* stream) {
-  ICY2D_GRID_CHECK(g);
-  const long count = (long)g->nx * g->ny > (long)g->body_nx * g->body_ny ? (long)g->nx * g->ny
-                                                                           : (long)g->body_nx * g->body_ny;
-  const unsigned blocks = blocks_for(count);
-  if (!fits(*g, blocks, 4)) return ICY2D_ERR_ARGUMENT;
-  k2_totals_partials<<<blocks, kBlock, 0, S(stream)>>>(*g);
-  if (int rc = launch_status()) return rc;
-  return finish(*g, blocks, 4, g->control->totals, 0u, 0u, 0u, S(stream));
-}
+
 void icy2d_abi_sizes(int32_t* grid, int32_t* control) {
   if (grid) *grid = (int32_t)sizeof(icy2d_grid);
   if (control) *control = (int32_t)sizeof(icy2d_control);

@@ -205,6 +205,7 @@ __device__ void exchange_heat_cell(const icy3d_grid& g, const Cell& c, float dt)
   float requested = 0.0f;
   const float tw = water_temperature(g, n);
   const float aperture = g.water_volume[n] / g.cell_volume;
+  const float gain = g.exchange_gain ? g.exchange_gain[n] : 1.0f;
   const float kh = 2.0f * g.k_water * g.k_ice / (g.k_water + g.k_ice);
   for (int f = 0; f < 7; ++f) {
     body[f] = 0;
@@ -232,7 +233,7 @@ __device__ void exchange_heat_cell(const icy3d_grid& g, const Cell& c, float dt)
       if (!(mass > 0.0f)) continue;
       const float w = (a ? part[0] : 1.0f - part[0]) * (cc ? part[1] : 1.0f - part[1]) * (d ? part[2] : 1.0f - part[2]);
       const float share = w * mass / (g.initial_cell_mass * fraction);
-      const float value = dt * kh * g.dx * contact * share * fmaxf(0.0f, tw - ice_temperature(g, b, ki, kj, kk));
+      const float value = gain * dt * kh * g.dx * contact * share * fmaxf(0.0f, tw - ice_temperature(g, b, ki, kj, kk));
       heat[8 * f + s] = value;
       requested += value;
     }
@@ -247,6 +248,7 @@ __device__ void exchange_heat_cell(const icy3d_grid& g, const Cell& c, float dt)
       }
     }
   g.water_energy[n] -= scale * requested;
+  if (g.exchange_heat_out) g.exchange_heat_out[n] += scale * requested;
 }
 
 __global__ void k_exchange_heat(icy3d_grid g, int lo0, int lo1, int lo2, int ex, int ey, int ez, float dt) {

@@ -1,0 +1,417 @@
+// MI355X (gfx950) kernels of the learned interface closure: a 3x3x3, 32
+// channel 3D CNN in BF16 with FP32 accumulation on the matrix cores, its
+// backward pass (data and weight gradients), a weighted squared-error loss,
+// Adam, and the feature / gain maps that connect it to the 3D solver's heat
+// exchange.
+//
+// Convolution as an implicit GEMM on v_mfma_f32_16x16x32_bf16: the GEMM row
+// is the output cell, the column the output channel and the reduction runs
+// over (tap, input channel) in chunks of 32 input channels, one tap per MFMA.
+// Channels-last activations make each lane's A operand one 16-byte load
+// (8 channels of one neighbour cell).  Operand lane layout (16x16x32):
+//   A (16 x 32): lane l holds row l % 16, reduction elements 8 * (l / 16) + e;
+//   B (32 x 16): lane l holds column l % 16, the same reduction elements;
+//   D (16 x 16): lane l holds rows 4 * (l / 16) + r, column l % 16.
+// A and B use the same reduction index map, so the product does not depend
+// on how the hardware orders the reduction elements within a lane.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "icyclosure.h"
+
+namespace icyc {
+
+constexpr int kC = ICYC_CHANNELS;
+constexpr int kTaps = ICYC_TAPS;
+constexpr int kLayer = ICYC_LAYER_PARAMS;
+constexpr int kWeights = kTaps * kC * kC;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;  // NaN stays NaN
+  u += 0x7fffu + ((u >> 16) & 1u);                      // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+__device__ __forceinline__ floatx4 mfma(bf16x8 a, bf16x8 b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// ------------------------------------------------------------ convolution
+// One wave: 8 row tiles of 16 consecutive cells x 32 output channels.
+constexpr int kConvThreads = 256;
+constexpr int kRowTiles = 8;
+constexpr int kCellsPerWave = 16 * kRowTiles;
+constexpr int kCellsPerBlock = kCellsPerWave * (kConvThreads / 64);
+
+template <int Epi>
+__global__ void __launch_bounds__(kConvThreads)
+    k_conv3d(int nx, int ny, int nz, const uint16_t* __restrict__ in, const uint16_t* __restrict__ w,
+             const float* __restrict__ bias, const uint16_t* __restrict__ mask, uint16_t* __restrict__ out,
+             float* __restrict__ out_f32) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long N = (long)nx * ny * nz;
+  const long base = (long)blockIdx.x * kCellsPerBlock + (long)wave * kCellsPerWave;
+  const int row = lane & 15, group = lane >> 4;
+  int ci[kRowTiles], cj[kRowTiles], ck[kRowTiles];
+#pragma unroll
+  for (int mt = 0; mt < kRowTiles; ++mt) {
+    const long m = base + mt * 16 + row;
+    ck[mt] = (int)(m % nz);
+    cj[mt] = (int)((m / nz) % ny);
+    ci[mt] = m < N ? (int)(m / ((long)nz * ny)) : -1000;  // past the end: never inside
+  }
+  floatx4 acc[kRowTiles][2];
+#pragma unroll
+  for (int mt = 0; mt < kRowTiles; ++mt) acc[mt][0] = acc[mt][1] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int t = 0; t < kTaps; ++t) {
+    const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
+    const uint4* wt = reinterpret_cast<const uint4*>(w + (long)t * kC * kC);
+    const bf16x8 b0 = as_bf16x8(wt[(row * kC + group * 8) / 8]);
+    const bf16x8 b1 = as_bf16x8(wt[((16 + row) * kC + group * 8) / 8]);
+#pragma unroll
+    for (int mt = 0; mt < kRowTiles; ++mt) {
+      const int i = ci[mt] + di, j = cj[mt] + dj, k = ck[mt] + dk;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz)
+        v = *reinterpret_cast<const uint4*>(in + (((long)i * ny + j) * nz + k) * kC + group * 8);
+      const bf16x8 a = as_bf16x8(v);
+      acc[mt][0] = mfma(a, b0, acc[mt][0]);
+      acc[mt][1] = mfma(a, b1, acc[mt][1]);
+    }
+  }
+#pragma unroll
+  for (int mt = 0; mt < kRowTiles; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+      const int co = nt * 16 + row;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long cell = base + mt * 16 + group * 4 + r;
+        if (cell >= N) continue;
+        const float x = acc[mt][nt][r];
+        if (Epi == ICYC_RELU_BF16) {
+          out[cell * kC + co] = f2bf(fmaxf(0.0f, x + bias[co]));
+        } else if (Epi == ICYC_LINEAR_F32) {
+          if (co == 0) out_f32[cell] = x + bias[0];
+        } else {
+          out[cell * kC + co] = bf2f(mask[cell * kC + co]) > 0.0f ? f2bf(x) : (uint16_t)0;
+        }
+      }
+    }
+}
+
+// ----------------------------------------------------------- weight grad
+// One block per group of z-columns (i, j): the input's 3x3 neighbour
+// columns (transposed to [channel][k]) and three k-shifted copies of the
+// output gradient ([channel][k]) are staged in LDS, so both MFMA operands are
+// 16-byte LDS reads: D[co][ci] += sum_k dout[k - dk][co] * in_col(di, dj)[k][ci].
+// Wave w accumulates taps w, w + 4, ... (at most 7 taps x 32 x 32 in registers).
+constexpr int kGradThreads = 256;
+constexpr int kColumn = 64;          // cells of a staged z-column (nz <= 64)
+constexpr int kPitch = kColumn + 8;  // padded LDS row (breaks 128-byte bank strides)
+constexpr int kTapsPerWave = 7;
+constexpr int kMaxGradBlocks = 512;
+
+__global__ void __launch_bounds__(kGradThreads)
+    k_conv3d_wgrad(int nx, int ny, int nz, const uint16_t* __restrict__ in, const uint16_t* __restrict__ dout,
+                   float* __restrict__ partials) {
+  __shared__ __attribute__((aligned(16))) uint16_t xs[9][kC][kPitch];
+  __shared__ __attribute__((aligned(16))) uint16_t ds[3][kC][kPitch];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int row = lane & 15, group = lane >> 4;
+  const int chunks = (nz + 31) / 32;
+  floatx4 acc[kTapsPerWave][2][2];
+#pragma unroll
+  for (int tl = 0; tl < kTapsPerWave; ++tl)
+#pragma unroll
+    for (int a = 0; a < 2; ++a) acc[tl][a][0] = acc[tl][a][1] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  float bias_acc = 0.0f;
+  const int cell = tid >> 2, quarter = tid & 3;  // staging: one cell's 8 channels per thread
+  for (int column = blockIdx.x; column < nx * ny; column += gridDim.x) {
+    const int i = column / ny, j = column - i * ny;
+    // the shifted gradient copies start from zero (k - dk outside the column)
+    for (int e = tid; e < 3 * kC * kPitch / 8; e += kGradThreads)
+      reinterpret_cast<uint4*>(&ds[0][0][0])[e] = make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
+#pragma unroll 1
+    for (int rr = 0; rr < 9; ++rr) {
+      const int ii = i + rr / 3 - 1, jj = j + rr % 3 - 1;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (cell < nz && ii >= 0 && ii < nx && jj >= 0 && jj < ny)
+        v = *reinterpret_cast<const uint4*>(in + (((long)ii * ny + jj) * nz + cell) * kC + quarter * 8);
+      const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) xs[rr][quarter * 8 + e][cell] = h[e];
+    }
+    {
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (cell < nz) v = *reinterpret_cast<const uint4*>(dout + ((long)column * nz + cell) * kC + quarter * 8);
+      const uint16_t* h = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) {
+        const int k = cell + s - 1;  // ds[s][.][k] = dout[k - (s - 1)]
+        if (cell < nz && k >= 0 && k < kColumn)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) ds[s][quarter * 8 + e][k] = h[e];
+      }
+    }
+    __syncthreads();
+    if (tid < kC)
+      for (int k = 0; k < nz; ++k) bias_acc += bf2f(ds[1][tid][k]);
+#pragma unroll
+    for (int tl = 0; tl < kTapsPerWave; ++tl) {
+      const int t = wave + 4 * tl;
+      if (t >= kTaps) continue;
+      const int rr = (t / 9) * 3 + (t / 3) % 3, s = t % 3;
+      for (int c = 0; c < chunks; ++c) {
+        const int k0 = c * 32 + group * 8;
+        const bf16x8 a0 = as_bf16x8(*reinterpret_cast<const uint4*>(&ds[s][row][k0]));
+        const bf16x8 a1 = as_bf16x8(*reinterpret_cast<const uint4*>(&ds[s][16 + row][k0]));
+        const bf16x8 b0 = as_bf16x8(*reinterpret_cast<const uint4*>(&xs[rr][row][k0]));
+        const bf16x8 b1 = as_bf16x8(*reinterpret_cast<const uint4*>(&xs[rr][16 + row][k0]));
+        acc[tl][0][0] = mfma(a0, b0, acc[tl][0][0]);
+        acc[tl][0][1] = mfma(a0, b1, acc[tl][0][1]);
+        acc[tl][1][0] = mfma(a1, b0, acc[tl][1][0]);
+        acc[tl][1][1] = mfma(a1, b1, acc[tl][1][1]);
+      }
+    }
+    __syncthreads();
+  }
+  float* part = partials + (long)blockIdx.x * kLayer;
+#pragma unroll
+  for (int tl = 0; tl < kTapsPerWave; ++tl) {
+    const int t = wave + 4 * tl;
+    if (t >= kTaps) continue;
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int co = mt * 16 + group * 4 + r, ci = nt * 16 + row;
+          part[(t * kC + co) * kC + ci] = acc[tl][mt][nt][r];
+        }
+  }
+  if (tid < kC) part[kWeights + tid] = bias_acc;
+}
+
+// grad[p] = sum over blocks of partials[b][p], in block order.
+__global__ void k_fold(const float* __restrict__ partials, int blocks, int count, float* __restrict__ out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= count) return;
+  float s = 0.0f;
+  for (int b = 0; b < blocks; ++b) s += partials[(long)b * count + p];
+  out[p] = s;
+}
+
+// ------------------------------------------------------------------ loss
+constexpr int kLossThreads = 256;
+__global__ void __launch_bounds__(kLossThreads)
+    k_weighted_mse(long cells, const float* __restrict__ pred, const float* __restrict__ target,
+                   const float* __restrict__ weight, uint16_t* __restrict__ dout, float* __restrict__ partials) {
+  __shared__ float part[kLossThreads / 64];
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  float loss = 0.0f;
+  if (n < cells) {
+    const float d = pred[n] - target[n], w = weight[n];
+    loss = w * d * d;
+    uint4* o = reinterpret_cast<uint4*>(dout + n * kC);
+    const uint32_t g = f2bf(2.0f * w * d);
+    o[0] = make_uint4(g, 0u, 0u, 0u);
+    o[1] = o[2] = o[3] = make_uint4(0u, 0u, 0u, 0u);
+  }
+  for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = loss;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float s = 0.0f;
+    for (int w = 0; w < kLossThreads / 64; ++w) s += part[w];
+    partials[blockIdx.x] = s;
+  }
+}
+
+// ------------------------------------------------------------------ Adam
+__device__ __forceinline__ void store_weight(int layer, int p, float value, uint16_t* wf, uint16_t* wb) {
+  if (p >= kWeights) return;
+  const int t = p / (kC * kC), co = (p / kC) % kC, ci = p % kC;
+  const uint16_t b = f2bf(value);
+  wf[(long)layer * kWeights + p] = b;
+  wb[(long)layer * kWeights + ((kTaps - 1 - t) * kC + ci) * kC + co] = b;
+}
+__global__ void k_adam(int layers, float* __restrict__ master, float* __restrict__ m, float* __restrict__ v,
+                       const float* __restrict__ grad, float lr, float beta1, float beta2, float eps, float c1,
+                       float c2, uint16_t* __restrict__ wf, uint16_t* __restrict__ wb) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)layers * kLayer) return;
+  const float g = grad[q];
+  const float mq = beta1 * m[q] + (1.0f - beta1) * g;
+  const float vq = beta2 * v[q] + (1.0f - beta2) * g * g;
+  m[q] = mq;
+  v[q] = vq;
+  const float w = master[q] - lr * (mq / c1) / (sqrtf(vq / c2) + eps);
+  master[q] = w;
+  store_weight((int)(q / kLayer), (int)(q % kLayer), w, wf, wb);
+}
+__global__ void k_pack(int layers, const float* __restrict__ master, uint16_t* __restrict__ wf,
+                       uint16_t* __restrict__ wb) {
+  const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (long)layers * kLayer) return;
+  store_weight((int)(q / kLayer), (int)(q % kLayer), master[q], wf, wb);
+}
+
+// ------------------------------------------------- solver state interface
+__device__ __forceinline__ bool touches_ice(int nx, int ny, int nz, int i, int j, int k, const float* coverage,
+                                            const int8_t* cov_id) {
+  const int offs[7][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}, {0, 0, 0}};
+#pragma unroll
+  for (int f = 0; f < 7; ++f) {
+    const int a = i + offs[f][0], b = j + offs[f][1], c = k + offs[f][2];
+    if (a < 0 || a >= nx || b < 0 || b >= ny || c < 0 || c >= nz) continue;
+    const long m = ((long)a * ny + b) * nz + c;
+    if (cov_id[m] >= 0 && coverage[m] > 1e-8f) return true;
+  }
+  return false;
+}
+__global__ void k_features(int nx, int ny, int nz, const float* __restrict__ phi, const float* __restrict__ volume,
+                           const float* __restrict__ energy, const float* __restrict__ u,
+                           const float* __restrict__ coverage, const int8_t* __restrict__ cov_id, float cell_volume,
+                           float rho_cp, float tm, uint16_t* __restrict__ features) {
+  const long N = (long)nx * ny * nz;
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const int k = (int)(n % nz), j = (int)((n / nz) % ny), i = (int)(n / ((long)nz * ny));
+  const float v = volume[n];
+  const bool wet = v > 1e-30f * cell_volume;
+  const float temperature = wet ? tm + energy[n] / (rho_cp * v) : 0.0f;
+  float f[8] = {phi[n], 0.01f * temperature, 10.0f * u[n], 10.0f * u[N + n], 10.0f * u[2 * N + n],
+                cov_id[n] >= 0 ? coverage[n] : 0.0f, wet ? 1.0f : 0.0f,
+                touches_ice(nx, ny, nz, i, j, k, coverage, cov_id) ? 1.0f : 0.0f};
+  uint32_t packed[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) packed[e] = (uint32_t)f2bf(f[2 * e]) | ((uint32_t)f2bf(f[2 * e + 1]) << 16);
+  uint4* o = reinterpret_cast<uint4*>(features + n * kC);
+  o[0] = make_uint4(packed[0], packed[1], packed[2], packed[3]);
+  o[1] = o[2] = o[3] = make_uint4(0u, 0u, 0u, 0u);
+}
+__global__ void k_gain(int nx, int ny, int nz, const float* __restrict__ pred, const float* __restrict__ coverage,
+                       const int8_t* __restrict__ cov_id, float max_gain, float* __restrict__ gain) {
+  const long N = (long)nx * ny * nz;
+  const long n = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const int k = (int)(n % nz), j = (int)((n / nz) % ny), i = (int)(n / ((long)nz * ny));
+  float g = 1.0f;
+  if (touches_ice(nx, ny, nz, i, j, k, coverage, cov_id)) g = fminf(max_gain, fmaxf(0.0f, 1.0f + pred[n]));
+  gain[n] = g;
+}
+
+int status() {
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : (int)e;
+}
+bool valid_grid(int nx, int ny, int nz) { return nx > 0 && ny > 0 && nz > 0 && (long)nx * ny * nz < (1L << 31); }
+unsigned blocks_of(long count, int threads) { return (unsigned)((count + threads - 1) / threads); }
+int wgrad_blocks(int nx, int ny) { return nx * ny < kMaxGradBlocks ? nx * ny : kMaxGradBlocks; }
+
+}  // namespace icyc
+
+using namespace icyc;
+
+extern "C" {
+
+int icyc_conv3d(int32_t nx, int32_t ny, int32_t nz, const uint16_t* in, const uint16_t* weights, const float* bias,
+                int32_t epilogue, const uint16_t* mask_act, uint16_t* out_bf16, float* out_f32, void* stream) {
+  if (!valid_grid(nx, ny, nz) || !in || !weights) return ICYC_ERR_ARGUMENT;
+  const unsigned blocks = blocks_of((long)nx * ny * nz, kCellsPerBlock);
+  hipStream_t s = (hipStream_t)stream;
+  switch (epilogue) {
+    case ICYC_RELU_BF16:
+      if (!bias || !out_bf16) return ICYC_ERR_ARGUMENT;
+      k_conv3d<ICYC_RELU_BF16><<<blocks, kConvThreads, 0, s>>>(nx, ny, nz, in, weights, bias, nullptr, out_bf16,
+                                                              nullptr);
+      break;
+    case ICYC_LINEAR_F32:
+      if (!bias || !out_f32) return ICYC_ERR_ARGUMENT;
+      k_conv3d<ICYC_LINEAR_F32><<<blocks, kConvThreads, 0, s>>>(nx, ny, nz, in, weights, bias, nullptr, nullptr,
+                                                               out_f32);
+      break;
+    case ICYC_MASKED_BF16:
+      if (!mask_act || !out_bf16) return ICYC_ERR_ARGUMENT;
+      k_conv3d<ICYC_MASKED_BF16><<<blocks, kConvThreads, 0, s>>>(nx, ny, nz, in, weights, nullptr, mask_act,
+                                                                out_bf16, nullptr);
+      break;
+    default:
+      return ICYC_ERR_ARGUMENT;
+  }
+  return status();
+}
+
+int64_t icyc_wgrad_partials_size(int32_t nx, int32_t ny, int32_t nz) {
+  if (!valid_grid(nx, ny, nz)) return 0;
+  return (int64_t)wgrad_blocks(nx, ny) * kLayer;
+}
+
+int icyc_conv3d_wgrad(int32_t nx, int32_t ny, int32_t nz, const uint16_t* in, const uint16_t* dout,
+                      float* partials, float* grad, void* stream) {
+  if (!valid_grid(nx, ny, nz) || nz % 16 != 0 || nz > kColumn || !in || !dout || !partials || !grad)
+    return ICYC_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  const int blocks = wgrad_blocks(nx, ny);
+  k_conv3d_wgrad<<<blocks, kGradThreads, 0, s>>>(nx, ny, nz, in, dout, partials);
+  if (int rc = status()) return rc;
+  k_fold<<<blocks_of(kLayer, 256), 256, 0, s>>>(partials, blocks, kLayer, grad);
+  return status();
+}
+
+int64_t icyc_loss_partials_size(int64_t cells) { return cells > 0 ? (int64_t)blocks_of(cells, kLossThreads) : 0; }
+
+int icyc_weighted_mse(int64_t cells, const float* pred, const float* target, const float* weight, uint16_t* dout,
+                      float* partials, float* loss, void* stream) {
+  if (cells <= 0 || !pred || !target || !weight || !dout || !partials || !loss) return ICYC_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  const unsigned blocks = blocks_of(cells, kLossThreads);
+  k_weighted_mse<<<blocks, kLossThreads, 0, s>>>(cells, pred, target, weight, dout, partials);
+  if (int rc = status()) return rc;
+  k_fold<<<1, 1, 0, s>>>(partials, (int)blocks, 1, loss);
+  return status();
+}
+
+int icyc_adam(int32_t layers, float* master, float* m, float* v, const float* grad, float lr, float beta1,
+              float beta2, float eps, int32_t step, uint16_t* wf, uint16_t* wb, void* stream) {
+  if (layers <= 0 || step <= 0 || !master || !m || !v || !grad || !wf || !wb) return ICYC_ERR_ARGUMENT;
+  const float c1 = 1.0f - powf(beta1, (float)step), c2 = 1.0f - powf(beta2, (float)step);
+  k_adam<<<blocks_of((long)layers * kLayer, 256), 256, 0, (hipStream_t)stream>>>(layers, master, m, v, grad, lr, beta1,
+                                                                                beta2, eps, c1, c2, wf, wb);
+  return status();
+}
+
+int icyc_pack_weights(int32_t layers, const float* master, uint16_t* wf, uint16_t* wb, void* stream) {
+  if (layers <= 0 || !master || !wf || !wb) return ICYC_ERR_ARGUMENT;
+  k_pack<<<blocks_of((long)layers * kLayer, 256), 256, 0, (hipStream_t)stream>>>(layers, master, wf, wb);
+  return status();
+}
+
+int icyc_features(int32_t nx, int32_t ny, int32_t nz, const float* phi, const float* water_volume,
+                  const float* water_energy, const float* u, const float* coverage, const int8_t* cov_id,
+                  float cell_volume, float rho_cp, float tm, uint16_t* features, void* stream) {
+  if (!valid_grid(nx, ny, nz) || !phi || !water_volume || !water_energy || !u || !coverage || !cov_id || !features)
+    return ICYC_ERR_ARGUMENT;
+  k_features<<<blocks_of((long)nx * ny * nz, 256), 256, 0, (hipStream_t)stream>>>(
+      nx, ny, nz, phi, water_volume, water_energy, u, coverage, cov_id, cell_volume, rho_cp, tm, features);
+  return status();
+}
+
+int icyc_gain(int32_t nx, int32_t ny, int32_t nz, const float* pred, const float* coverage, const int8_t* cov_id,
+              float max_gain, float* gain, void* stream) {
+  if (!valid_grid(nx, ny, nz) || !pred || !coverage || !cov_id || !gain) return ICYC_ERR_ARGUMENT;
+  k_gain<<<blocks_of((long)nx * ny * nz, 256), 256, 0, (hipStream_t)stream>>>(nx, ny, nz, pred, coverage, cov_id,
+                                                                             max_gain, gain);
+  return status();
+}
+
+}  // extern "C"

@@ -1,0 +1,527 @@
+"""Batched ensembles of independent 2D cases on one MI355X.
+
+The reference runs one ``IceFlow2D`` case per process.  ``IceFlowEnsemble2D``
+advances many cases of uneven sizes together: every case keeps its own
+``IceFlow2D`` state (fields, control block, host counters), and each kernel
+of the step runs for all cases in ONE launch through the ``icy2d_batch_*``
+entry points, whose blocks find their case in a prefix table of block counts.
+The step is the reference's ``IceFlow2D.step`` (iceflow2d/simulator.py:
+2194-2227) applied to every case, with the host decisions taken per case
+from one device-to-host copy of all control blocks:
+
+  - the fixed launch sequence from the collision to the outflow measurement
+    is captured once into a HIP graph and replayed every step;
+  - data-dependent work (advection substeps, melt fallback, the volume
+    projection's Newton iterations, the melt-momentum status) launches for
+    the cases that need it, the others' blocks returning at once (an
+    ``active`` mask), with per-case FP64 arguments in a device array.
+
+With the default 256-thread collide and stream launches every case's result
+equals a standalone ``IceFlow2D`` run of that case bit for bit; the
+autotuner may pick other block sizes, which only reorders the per-block
+partial sums of the hydrodynamic impulse.
+"""
+
+import ctypes
+import math
+import time
+
+import numpy as np
+import torch
+
+from .. import _native2d as native
+from .simulator import MAX_BULK_PHASE_L1, IceFlow2D, _require_device
+
+THREAD_CHOICES = (128, 256, 512)
+
+
+class IceFlowEnsemble2D:
+    """Many ``IceFlow2D`` cases advanced in lockstep, one launch per kernel."""
+
+    def __init__(self, configs, *, population_storage="fp16", use_graph=True):
+        configs = list(configs)
+        if not configs:
+            raise ValueError("an ensemble needs at least one case")
+        intervals = {c.thermal.update_interval_lbm_steps for c in configs if c.thermal is not None}
+        if len(intervals) > 1:
+            raise ValueError("every case of an ensemble must use the same thermal update interval")
+        self.device = _require_device()
+        self._lib = native.lib()
+        self._stream = torch.cuda.Stream(self.device)
+        self._s = ctypes.c_void_p(self._stream.cuda_stream)
+        n = len(configs)
+        self.ncases = n
+        size = ctypes.sizeof(native.Control2D)
+        self._controls = torch.zeros((n, size), dtype=torch.uint8, device=self.device)
+        self._controls_host = torch.zeros((n, size), dtype=torch.uint8, pin_memory=True)
+        with torch.cuda.stream(self._stream):
+            self.cases = [IceFlow2D(c, population_storage=population_storage, _control_storage=self._controls[i])
+                          for i, c in enumerate(configs)]
+        self._stream.synchronize()
+        self.cells = sum(case.nx * case.ny for case in self.cases)
+        host_grids = (native.Grid2D * n)(*[case._g for case in self.cases])
+        first = (ctypes.c_int32 * (native.SHAPES * (n + 1)))()
+        totals = (ctypes.c_int32 * native.SHAPES)()
+        rc = self._lib.icy2d_batch_plan(ctypes.cast(host_grids, ctypes.POINTER(native.Grid2D)), n, first, totals)
+        if rc != 0:
+            raise native.IcyError(rc, "icy2d_batch_plan")
+        raw = torch.frombuffer(bytearray(bytes(host_grids)), dtype=torch.uint8)
+        self._grids = raw.to(self.device)
+        self._first = torch.tensor(list(first), dtype=torch.int32).to(self.device)
+        # Rings of per-case argument slots (FP64 values, int32 active masks),
+        # reused after each host synchronisation.
+        self._values = torch.zeros((64, n), dtype=torch.float64, device=self.device)
+        self._values_host = torch.zeros((64, n), dtype=torch.float64, pin_memory=True)
+        self._masks = torch.zeros((32, n), dtype=torch.int32, device=self.device)
+        self._masks_host = torch.zeros((32, n), dtype=torch.int32, pin_memory=True)
+        self._value_slot = self._mask_slot = 0
+        self._batch = native.Batch2D()
+        b = self._batch
+        b.ncases = n
+        b.collide_threads = b.stream_threads = 256
+        b.grids = self._grids.data_ptr()
+        b.active = None
+        b.first_block = self._first.data_ptr()
+        for k in range(native.SHAPES):
+            b.total_blocks[k] = totals[k]
+        self._batch_ptr = ctypes.pointer(self._batch)
+        self.use_graph = bool(use_graph)
+        self._graph = None
+        self.steps = 0
+        self._pending = self.cases[0]._thermal_lbm_steps_pending
+        self._interval = configs[0].thermal.update_interval_lbm_steps
+
+    def __del__(self):
+        self._drop_graph()
+
+    # ---------------------------------------------------------- plumbing
+    def _drop_graph(self):
+        graph = getattr(self, "_graph", None)
+        if graph is not None:
+            self._lib.icy2d_graph_destroy(graph)
+            self._graph = None
+
+    @property
+    def threads(self):
+        return int(self._batch.collide_threads), int(self._batch.stream_threads)
+
+    def set_threads(self, collide, stream):
+        """Block sizes of the batched collide and stream launches."""
+        if collide not in THREAD_CHOICES or stream not in THREAD_CHOICES:
+            raise ValueError(f"block sizes must be in {THREAD_CHOICES}")
+        self._batch.collide_threads, self._batch.stream_threads = int(collide), int(stream)
+        self._drop_graph()
+
+    def _call(self, name, *args, active=None):
+        if active is None:
+            batch = self._batch_ptr
+        else:
+            masked = native.Batch2D.from_buffer_copy(self._batch)
+            masked.active = self._upload_mask(active)
+            batch = ctypes.byref(masked)
+        rc = getattr(self._lib, "icy2d_batch_" + name)(batch, *args, self._s)
+        if rc != 0:
+            raise native.IcyError(rc, "icy2d_batch_" + name)
+
+    def _upload_values(self, values):
+        """Device pointer to one FP64 value per case (stream ordered)."""
+        if self._value_slot == self._values.shape[0]:
+            self._stream.synchronize()
+            self._value_slot = 0
+        k = self._value_slot
+        self._value_slot += 1
+        self._values_host[k].copy_(torch.as_tensor(np.asarray(values, dtype=np.float64)))
+        with torch.cuda.stream(self._stream):
+            self._values[k].copy_(self._values_host[k], non_blocking=True)
+        return ctypes.c_void_p(self._values[k].data_ptr())
+
+    def _upload_mask(self, active):
+        if self._mask_slot == self._masks.shape[0]:
+            self._stream.synchronize()
+            self._mask_slot = 0
+        k = self._mask_slot
+        self._mask_slot += 1
+        self._masks_host[k].copy_(torch.as_tensor(np.asarray(active, dtype=np.int32)))
+        with torch.cuda.stream(self._stream):
+            self._masks[k].copy_(self._masks_host[k], non_blocking=True)
+        return self._masks[k].data_ptr()
+
+    def _read_controls(self):
+        """All control blocks with one copy; raises a pending device-side check."""
+        with torch.cuda.stream(self._stream):
+            self._controls_host.copy_(self._controls, non_blocking=True)
+        self._stream.synchronize()
+        self._value_slot = self._mask_slot = 0
+        raw = self._controls_host.numpy()
+        controls = [native.Control2D.from_buffer_copy(raw[i].tobytes()) for i in range(self.ncases)]
+        for i, control in enumerate(controls):
+            if control.error_flags:
+                try:
+                    with torch.cuda.stream(self._stream):
+                        self.cases[i]._raise_aperture_error(control)
+                except RuntimeError as error:
+                    raise RuntimeError(f"ensemble case {i}: {error}") from error
+        return controls
+
+    # ---------------------------------------------------------- step
+    def _flow_segment(self):
+        """simulator.py:2203-2210 and the outflow measurement of :2233-2247:
+        the fixed launch sequence of every step."""
+        self._call("collide", 1)
+        self._call("stream", 0, 3)
+        self._call("update_pressure")
+        self._call("integrate_rigid")
+        self._contact_and_rasterize(resolve_contact=True)
+        self._call("update_solid_mask", 0)
+        self._call("refill")
+        self._call("measure_outflow")
+
+    def _run_flow_segment(self):
+        if not self.use_graph:
+            self._flow_segment()
+            return
+        if self._graph is None:
+            rc = self._lib.icy2d_graph_begin(self._s)
+            if rc != 0:
+                raise native.IcyError(rc, "icy2d_graph_begin")
+            try:
+                self._flow_segment()
+            finally:
+                graph = ctypes.c_void_p()
+                rc = self._lib.icy2d_graph_end(self._s, ctypes.byref(graph))
+            if rc != 0:
+                raise native.IcyError(rc, "icy2d_graph_end")
+            self._graph = graph
+        rc = self._lib.icy2d_graph_launch(self._graph, self._s)
+        if rc != 0:
+            raise native.IcyError(rc, "icy2d_graph_launch")
+
+    def _contact_and_rasterize(self, *, resolve_contact, active=None):
+        """simulator.py:4082-4123 (the projected re-pass is decided on the device)."""
+        self._call("body_fraction", 0, active=active)
+        if resolve_contact:
+            self._call("contact_support", 1, 0, active=active)
+            self._call("project_inside", active=active)
+            self._call("body_fraction", 1, active=active)
+            self._call("contact_support", 1, 1, active=active)
+        else:
+            self._call("contact_support", 0, 0, active=active)
+        self._call("commit_raster", active=active)
+
+    def _synchronize_water_aperture(self, active=None):
+        """simulator.py:511-548."""
+        self._call("measure_aperture", active=active)
+        self._call("build_aperture_targets", active=active)
+        self._call("apply_aperture_transfer", active=active)
+        self._call("finish_aperture_transfer", active=active)
+
+    def step(self, num_steps=1):
+        """Advance every case by ``num_steps`` steps (simulator.py:2194-2227 per case)."""
+        if isinstance(num_steps, bool) or not isinstance(num_steps, (int, np.integer)) or num_steps < 0:
+            raise ValueError("num_steps must be a non-negative integer")
+        for _ in range(int(num_steps)):
+            self._run_flow_segment()
+            self._advance_thermal_fast(self._read_controls())
+            self._pending += 1
+            if self._pending >= self._interval:
+                self._advance_thermal_slow(self._pending)
+                self._pending = 0
+            self._correct_water_volume()
+            self._finalize_melt_momentum()
+            self._synchronize_water_aperture()
+            self.steps += 1
+            for case in self.cases:
+                case.steps += 1
+                case._thermal_lbm_steps_pending = self._pending
+        if num_steps:
+            self._read_controls()
+
+    def _advance_thermal_fast(self, controls):
+        """simulator.py:2233-2247 and :418-453 per case."""
+        counts, subs = [], []
+        for i, (case, control) in enumerate(zip(self.cases, controls)):
+            speed = float(control.max_outflow)
+            if not speed >= 0.0:
+                raise ValueError(f"ensemble case {i}: maximum_outflow_rate_lattice must be finite")
+            th, dt = case.config.thermal, case._time_step_s
+            load = 0.0
+            if th.advection_enabled and speed != 0.0:
+                limit = th.max_courant_number * dt / speed
+                if math.isfinite(limit):
+                    load = dt / limit
+            count = case._substeps(load, "advection", dt, speed)
+            counts.append(count)
+            subs.append(dt / count)
+        self._synchronize_water_aperture()
+        sub = self._upload_values(subs)
+        for k in range(max(counts)):
+            active = None if min(counts) > k else [int(c > k) for c in counts]
+            self._call("advection_flux", 0, active=active)
+            self._call("advection_flux", 1, active=active)
+            self._call("update_water", sub, 1, active=active)
+            more = [int(c > k + 1) for c in counts]
+            if any(more):
+                self._synchronize_water_aperture(active=None if all(more) else more)
+
+    def _advance_thermal_slow(self, lbm_steps):
+        """simulator.py:2249-2286 and :455-509 per case."""
+        count = int(lbm_steps)
+        for i, case in enumerate(self.cases):
+            if case._moving_melt_momentum_pending:
+                raise RuntimeError(f"ensemble case {i}: a moving thermal momentum interval is still pending "
+                                   "phase projection")
+        self._synchronize_water_aperture()
+        self._call("snapshot_melt_momentum")
+        dts = [count * case._time_step_s for case in self.cases]
+        counts = [case._substeps(dt / case._maximum_diffusion_time_step_s, "diffusion", dt)
+                  for case, dt in zip(self.cases, dts)]
+        sub = self._upload_values([dt / c for dt, c in zip(dts, counts)])
+        self._call("reset_interval_sources")
+        for k in range(max(counts)):
+            active = None if min(counts) > k else [int(c > k) for c in counts]
+            self._call("body_conduction", sub, active=active)
+            self._call("conduction_flux", 0, active=active)
+            self._call("conduction_flux", 1, active=active)
+            self._call("update_water", sub, 0, active=active)
+            self._call("accumulate_boundary_heat", sub, active=active)
+            self._call("exchange_heat", sub, active=active)
+            self._call("phase_change", active=active)
+            self._call("freeze_eligibility", active=active)
+            self._call("inject_melt", active=active)
+            self._distribute_unassigned_melt(active)
+            self._synchronize_water_aperture(active=active)
+            more = [int(c > k + 1) for c in counts]
+            if any(more):
+                self._contact_and_rasterize(resolve_contact=False, active=None if all(more) else more)
+            sub = self._upload_values([dt / c for dt, c in zip(dts, counts)])
+        self._contact_and_rasterize(resolve_contact=False)
+        for case, dt, c in zip(self.cases, dts, counts):
+            case.thermal.steps += c
+            case.thermal.time_s += dt
+        self._call("reduce_mass_properties")
+        self._call("apply_mass_properties", 0)
+        self._call("update_solid_mask", 0)
+        self._call("refill")
+        for case in self.cases:
+            case._moving_melt_momentum_pending = True
+
+    def _distribute_unassigned_melt(self, active):
+        """simulator.py:1542-1563 per case."""
+        controls = self._read_controls()
+        live = [1 if active is None else active[i] for i in range(self.ncases)]
+        need = [int(live[i] and controls[i].unassigned[0] > 0.0) for i in range(self.ncases)]
+        if not any(need):
+            return
+        self._call("measure_fallback", active=need)
+        controls = self._read_controls()
+        free, wet = [0] * self.ncases, [0] * self.ncases
+        for i in range(self.ncases):
+            if not need[i]:
+                continue
+            if controls[i].fallback[0] > 1.0e-30:
+                free[i] = 1
+            elif controls[i].fallback[1] > 1.0e-30:
+                wet[i] = 1
+            else:
+                raise RuntimeError(f"ensemble case {i}: melt water has no legal LBM water cell for conservative "
+                                   "mass and energy injection")
+        if any(free):
+            self._call("apply_fallback", 1, active=free)
+        if any(wet):
+            self._call("apply_fallback", 0, active=wet)
+
+    def _finalize_melt_momentum(self):
+        """simulator.py:2989-3012 per case (every case shares the schedule)."""
+        pending = [int(case._moving_melt_momentum_pending) for case in self.cases]
+        if not any(pending):
+            return
+        mask = None if all(pending) else pending
+        self._call("reduce_provisional_momentum", active=mask)
+        self._call("melt_weight_geometry", active=mask)
+        controls = self._read_controls()
+        apply = [0] * self.ncases
+        for i, control in enumerate(controls):
+            if not pending[i]:
+                continue
+            status = int(control.melt_status)
+            if status == -1:
+                raise RuntimeError(f"ensemble case {i}: melt momentum has no legal wet LBM cell for conservative "
+                                   "linear/angular-momentum closure")
+            if status == -2:
+                raise RuntimeError(f"ensemble case {i}: melt momentum correction requires two distinct wet cells")
+            apply[i] = int(status == 1)
+        if any(apply):
+            self._call("apply_melt_correction", active=apply)
+        self._call("reduce_final_momentum", active=mask)
+        for _ in range(2):
+            self._call("local_melt_correction", active=mask)
+            self._call("reduce_final_momentum", active=mask)
+        self._call("finish_melt_momentum", active=mask)
+        for case in self.cases:
+            case._moving_melt_momentum_pending = False
+
+    def _correct_water_volume(self):
+        """simulator.py:4494-4560 per case, every Newton evaluation batched."""
+        n = self.ncases
+        self._call("clip_phase", MAX_BULK_PHASE_L1)
+        controls = self._read_controls()
+        target, tolerance, need = [0.0] * n, [0.0] * n, [0] * n
+        for i, (case, control) in enumerate(zip(self.cases, controls)):
+            current = control.water[1]
+            target[i] = control.water[0] + case._initial_body_mass_lattice - control.mass
+            tolerance[i] = case.config.volume_projection_tolerance * max(1.0, abs(target[i]))
+            need[i] = int(abs(target[i] - current) > tolerance[i])
+        if not any(need):
+            return
+        mask = None if all(need) else need
+        self._call("seed_projection", active=mask)
+        radii = [case._volume_projection_band_radius for case in self.cases]
+        for k in range(max(radii[i] for i in range(n) if need[i])):
+            live = [int(need[i] and radii[i] > k) for i in range(n)]
+            self._call("dilate_projection", 1 if k % 2 == 0 else 0, active=None if all(live) else live)
+        limit = [4.0 * case.config.volume_projection_max_shift / case.config.interface_width for case in self.cases]
+
+        def evaluate(lams, live):
+            self._call("evaluate_projection", self._upload_values(lams), active=None if all(live) else live)
+            controls = self._read_controls()
+            return [c.water[1] for c in controls], [c.water[2] for c in controls]
+
+        lower = evaluate([-v for v in limit], need)[0]
+        upper = evaluate(limit, need)[0]
+        for i in range(n):
+            if not need[i]:
+                continue
+            if abs(upper[i] - lower[i]) <= tolerance[i]:
+                raise RuntimeError(f"ensemble case {i}: water-volume projection is infeasible: no adjustable "
+                                   "phi=0.5-connected interface is available")
+            if target[i] < lower[i] - tolerance[i] or target[i] > upper[i] + tolerance[i]:
+                raise RuntimeError(f"ensemble case {i}: water-volume projection exceeds the permitted interface "
+                                   f"shift: target={target[i]:.17g}, reachable=[{lower[i]:.17g}, {upper[i]:.17g}]")
+        initial = [abs(target[i] - controls[i].water[1]) for i in range(n)]
+        lo, hi = [-v for v in limit], list(limit)
+        lam, best_lam, best = [0.0] * n, [0.0] * n, initial
+        searching = list(need)
+        iterations = [case.config.volume_projection_max_iterations for case in self.cases]
+        for k in range(max(iterations)):
+            searching = [int(searching[i] and k < iterations[i]) for i in range(n)]
+            if not any(searching):
+                break
+            mass, derivative = evaluate(lam, searching)
+            for i in range(n):
+                if not searching[i]:
+                    continue
+                residual = mass[i] - target[i]
+                if abs(residual) < best[i]:
+                    best[i], best_lam[i] = abs(residual), lam[i]
+                if abs(residual) <= tolerance[i]:
+                    best_lam[i] = lam[i]
+                    searching[i] = 0
+                    continue
+                if residual < 0.0:
+                    lo[i] = lam[i]
+                else:
+                    hi[i] = lam[i]
+                proposal = math.nan
+                if derivative[i] > 0.0 and math.isfinite(derivative[i]):
+                    proposal = lam[i] - residual / derivative[i]
+                if not math.isfinite(proposal) or not lo[i] < proposal < hi[i]:
+                    proposal = 0.5 * (lo[i] + hi[i])
+                lam[i] = proposal
+        self._call("apply_projection", self._upload_values(best_lam), active=mask)
+        current = [c.water[1] for c in self._read_controls()]
+        closing = list(need)
+        for _ in range(8):
+            residual = [target[i] - current[i] for i in range(n)]
+            closing = [int(closing[i] and abs(residual[i]) > tolerance[i]) for i in range(n)]
+            if not any(closing):
+                break
+            live = None if all(closing) else closing
+            res_ptr = self._upload_values(residual)
+            self._call("projection_residual_weight", res_ptr, active=live)
+            weight = [c.water[2] for c in self._read_controls()]
+            for i in range(n):
+                if closing[i] and (not math.isfinite(weight[i]) or weight[i] <= 0.0):
+                    closing[i] = 0
+            if not any(closing):
+                break
+            live = None if all(closing) else closing
+            self._call("apply_projection_residual", self._upload_values(residual), self._upload_values(weight),
+                       active=live)
+            controls = self._read_controls()
+            current = [controls[i].water[1] if closing[i] else current[i] for i in range(n)]
+        for i in range(n):
+            if need[i]:
+                error = abs(target[i] - current[i])
+                if error > tolerance[i]:
+                    raise RuntimeError(f"ensemble case {i}: water-volume projection cannot close its endpoint "
+                                       f"active set: residual={error:.6e}, tolerance={tolerance[i]:.6e}")
+
+    # ---------------------------------------------------------- tuning
+    def _state_tensors(self):
+        tensors = [self._controls]
+        for case in self.cases:
+            tensors.extend(case._t.values())
+        return tensors
+
+    def _snapshot(self):
+        with torch.cuda.stream(self._stream):
+            saved = [t.clone() for t in self._state_tensors()]
+        host = (self.steps, self._pending, [(c.steps, c._thermal_lbm_steps_pending, c._moving_melt_momentum_pending,
+                                            c.thermal.steps, c.thermal.time_s) for c in self.cases])
+        return saved, host
+
+    def _restore(self, snapshot):
+        saved, host = snapshot
+        with torch.cuda.stream(self._stream):
+            for t, s in zip(self._state_tensors(), saved):
+                t.copy_(s)
+        self._stream.synchronize()
+        self.steps, self._pending, counters = host
+        for c, (steps, pending, melt, tsteps, ttime) in zip(self.cases, counters):
+            c.steps, c._thermal_lbm_steps_pending, c._moving_melt_momentum_pending = steps, pending, melt
+            c.thermal.steps, c.thermal.time_s = tsteps, ttime
+
+    def autotune(self, steps=None, repeats=1):
+        """Time whole steps (one thermal interval by default) for every pair
+        of collide/stream block sizes on the live ensemble, keep the fastest
+        and restore the state.  Returns {(collide, stream): seconds per step}."""
+        steps = int(self._interval if steps is None else steps)
+        snapshot = self._snapshot()
+        timings = {}
+        try:
+            for collide in THREAD_CHOICES:
+                for stream in THREAD_CHOICES:
+                    self.set_threads(collide, stream)
+                    self.step(1)  # captures the graph for these sizes
+                    best = math.inf
+                    for _ in range(int(repeats)):
+                        self._restore(snapshot)
+                        start = time.perf_counter()
+                        self.step(steps)
+                        best = min(best, (time.perf_counter() - start) / steps)
+                    timings[(collide, stream)] = best
+                    self._restore(snapshot)
+        except BaseException:
+            self._restore(snapshot)
+            raise
+        choice = min(timings, key=timings.get)
+        self.set_threads(*choice)
+        return timings
+
+    # ---------------------------------------------------------- results
+    def totals(self):
+        """Per-case ``mass_energy_totals()`` (simulator.py:550-571)."""
+        with torch.cuda.stream(self._stream):
+            return [case.thermal.mass_energy_totals() for case in self.cases]
+
+    def bath_temperatures_c(self):
+        """Mean water temperature of every case: sensible energy over
+        rho_w * c_w * volume, above the melting point."""
+        out = []
+        with torch.cuda.stream(self._stream):
+            for case in self.cases:
+                v = case._t["water_v"].sum().item()
+                e = case._t["water_e"].sum().item()
+                g = case._g
+                out.append(g.t_melt + e / (g.rho_w_si * g.cp_w * v) if v > 0.0 else math.nan)
+        return out

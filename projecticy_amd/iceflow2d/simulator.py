@@ -160,10 +160,12 @@ class Thermal2D:
 class IceFlow2D:
     """MI355X solver for the coupled falling-ice melting scenario."""
 
-    def __init__(self, config, *, population_storage="fp32"):
+    def __init__(self, config, *, population_storage="fp32", _control_storage=None):
         """``population_storage="fp16"`` keeps the four population buffers in
         FP16 (scaled by 2^10) with FP32 arithmetic; it has no reference
-        counterpart.  Every other argument and check is the reference's."""
+        counterpart.  Every other argument and check is the reference's.
+        ``_control_storage`` (a device uint8 tensor of the control block's
+        size) lets an ensemble keep every case's control block in one array."""
         # simulator.py:1829-1843
         if not isinstance(config, IceFlowConfig):
             raise TypeError("config must be an IceFlowConfig")
@@ -180,6 +182,7 @@ class IceFlow2D:
         if population_storage not in ("fp32", "fp16"):
             raise ValueError("population_storage must be 'fp32' or 'fp16'")
         self.population_storage = population_storage
+        self._control_storage = _control_storage
         self.device = _require_device()
         self._lib = native.lib()
         self.config = config
@@ -268,7 +271,15 @@ class IceFlow2D:
         blocks = (max(N, M, (nx + 1) * (ny + 1)) + 255) // 256
         capacity = blocks * 6 + 16
         self._t["partials"] = torch.zeros(capacity, dtype=f64, device=dev)
-        self._control_t = torch.zeros(ctypes.sizeof(native.Control2D), dtype=torch.uint8, device=dev)
+        size = ctypes.sizeof(native.Control2D)
+        if self._control_storage is not None:
+            storage = self._control_storage
+            if storage.dtype != torch.uint8 or storage.numel() != size or storage.device != dev:
+                raise ValueError(f"control storage must be {size} uint8 bytes on {dev}")
+            self._control_t = storage.view(size)
+            self._control_t.zero_()
+        else:
+            self._control_t = torch.zeros(size, dtype=torch.uint8, device=dev)
         self._control_host = torch.zeros(ctypes.sizeof(native.Control2D), dtype=torch.uint8, pin_memory=True)
         g = self._g
         for name in native.GRID_POINTERS:

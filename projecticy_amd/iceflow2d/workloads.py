@@ -15,6 +15,12 @@ Cases:
                       328-cell ice, the water level 0.030 m = 1228.8 cells
                       rounded to 1229, the angle drawn uniformly in
                       [0, 15] degrees by numpy.random.default_rng(seed).
+  ``ensemble_cases(count, seed)`` uneven cases for a batched ensemble: width
+                      uniform in {64, 128, ..., 512} cells, height twice the
+                      width, dx = 2.5e-4 m (the default case's), and per case
+                      an ice edge of 20-40% of the width, an angle in
+                      [0, 15] degrees, a bath depth of 50-70% of the height
+                      and a bath temperature in [60, 95] degC.
 """
 
 import math
@@ -61,3 +67,42 @@ def scaled_case(seed=2026, **overrides):
 
     angle = float(np.random.default_rng(seed).uniform(0.0, 15.0))
     return falling_ice(nx=1024, ny=2048, water_cells=1229, ice_cells=328, angle_degrees=angle, **overrides)
+
+
+ENSEMBLE_DX_M = 2.5e-4
+ENSEMBLE_DROP_M = 0.0015
+
+
+def ensemble_draws(count=64, seed=2026):
+    """Per-case parameters, drawn from numpy.random.default_rng(seed) in
+    this order for each case: width index, ice fraction, angle, bath
+    fraction, bath temperature.  The bath depth is capped so that the
+    rotated ice, the drop height and two free cells fit below the top wall."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    draws = []
+    for _ in range(int(count)):
+        nx = 64 * int(rng.integers(1, 9))
+        ice_fraction = float(rng.uniform(0.2, 0.4))
+        angle = float(rng.uniform(0.0, 15.0))
+        bath_fraction = float(rng.uniform(0.5, 0.7))
+        temperature = float(rng.uniform(60.0, 95.0))
+        ny, boundary = 2 * nx, 3
+        ice = max(4, int(round(ice_fraction * nx)))
+        theta = math.radians(angle)
+        height = ice * (abs(math.sin(theta)) + abs(math.cos(theta)))
+        drop = ENSEMBLE_DROP_M / ENSEMBLE_DX_M
+        water = int(round(bath_fraction * ny))
+        water = min(water, int(math.floor(ny - boundary - 2 - drop - height)))
+        draws.append(dict(nx=nx, ny=ny, ice_cells=ice, angle_degrees=angle, water_cells=water,
+                          water_temperature_c=temperature))
+    return draws
+
+
+def ensemble_cases(count=64, seed=2026, **overrides):
+    """IceFlowConfig for every drawn case (same physics as the default case)."""
+    return [falling_ice(nx=d["nx"], ny=d["ny"], domain_width_m=d["nx"] * ENSEMBLE_DX_M, water_cells=d["water_cells"],
+                        ice_cells=d["ice_cells"], angle_degrees=d["angle_degrees"],
+                        water_temperature_c=d["water_temperature_c"], drop_height_m=ENSEMBLE_DROP_M, **overrides)
+            for d in ensemble_draws(count, seed)]

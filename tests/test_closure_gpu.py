@@ -1,0 +1,180 @@
+"""The closure's HIP kernels (BF16 MFMA, FP32 accumulation) against the FP64
+CPU oracle on the same BF16 inputs.
+
+Tolerances: BF16 outputs within 1 BF16 ulp (2^-8 relative) of the oracle's
+value, plus 1e-6 of the output's largest magnitude for the FP32
+accumulation order; FP32 outputs within 2e-5 of the largest magnitude;
+gradients, which pass through BF16-rounded intermediate tensors, within 2e-2
+of the largest gradient of their layer."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import closure_oracle as co
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(5, 6, 16), (3, 2, 32), (2, 3, 64)]
+
+
+def bits_tensor(values):
+    return torch.from_numpy(co.bf16_bits(values).view(np.int16)).cuda()
+
+
+def values_of(t, shape):
+    return co.from_bf16_bits(t.cpu().numpy().view(np.uint16)).reshape(shape).astype(np.float64)
+
+
+def ptr(t):
+    import ctypes
+
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def stream():
+    import ctypes
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def random_problem(shape, seed):
+    rng = np.random.default_rng(seed)
+    x = co.bf16_round(rng.normal(size=shape + (co.C,)))
+    wf = co.bf16_round(rng.normal(scale=0.1, size=(co.TAPS, co.C, co.C)))
+    bias = rng.normal(scale=0.1, size=co.C).astype(np.float32)
+    return x, wf, bias
+
+
+def assert_bf16_close(got, expected):
+    expected_r = co.bf16_round(expected).astype(np.float64)
+    scale = max(np.abs(expected).max(), 1e-30)
+    err = np.abs(got - expected_r) - (2.0 ** -8 * np.abs(expected_r) + 1e-6 * scale)
+    assert err.max() <= 0.0, f"max excess {err.max():.3e}"
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_conv_forward_epilogues(shape):
+    from projecticy_amd import _native_closure as nc
+
+    lib = nc.lib()
+    x, wf, bias = random_problem(shape, sum(shape))
+    nx, ny, nz = shape
+    cells = nx * ny * nz
+    xt, wt, bt = bits_tensor(x), bits_tensor(wf), torch.from_numpy(bias).cuda()
+    out = torch.zeros(cells * co.C, dtype=torch.int16, device="cuda")
+    out32 = torch.zeros(cells, dtype=torch.float32, device="cuda")
+    expected = co.conv3d(x, wf, bias)
+    nc.check(lib.icyc_conv3d(nx, ny, nz, ptr(xt), ptr(wt), ptr(bt), nc.RELU_BF16, None, ptr(out), None, stream()), "relu")
+    torch.cuda.synchronize()
+    assert_bf16_close(values_of(out, shape + (co.C,)), np.maximum(expected, 0.0))
+    nc.check(lib.icyc_conv3d(nx, ny, nz, ptr(xt), ptr(wt), ptr(bt), nc.LINEAR_F32, None, None, ptr(out32), stream()),
+             "linear")
+    got = out32.cpu().numpy().reshape(shape)
+    assert np.abs(got - expected[..., 0]).max() <= 2e-5 * np.abs(expected[..., 0]).max()
+    # masked epilogue: conv without bias, zero where the mask activation is not positive
+    mask = co.bf16_round(np.random.default_rng(1).normal(size=shape + (co.C,)))
+    nc.check(lib.icyc_conv3d(nx, ny, nz, ptr(xt), ptr(wt), None, nc.MASKED_BF16, ptr(bits_tensor(mask)), ptr(out),
+                             None, stream()), "masked")
+    assert_bf16_close(values_of(out, shape + (co.C,)), np.where(mask > 0, co.conv3d(x, wf), 0.0))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_weight_gradient(shape):
+    from projecticy_amd import _native_closure as nc
+
+    lib = nc.lib()
+    rng = np.random.default_rng(7 + sum(shape))
+    nx, ny, nz = shape
+    x = co.bf16_round(rng.normal(size=shape + (co.C,)))
+    dout = co.bf16_round(rng.normal(size=shape + (co.C,)))
+    partials = torch.zeros(int(lib.icyc_wgrad_partials_size(nx, ny, nz)), dtype=torch.float32, device="cuda")
+    grad = torch.zeros(co.LAYER_PARAMS, dtype=torch.float32, device="cuda")
+    nc.check(lib.icyc_conv3d_wgrad(nx, ny, nz, ptr(bits_tensor(x)), ptr(bits_tensor(dout)), ptr(partials), ptr(grad),
+                                   stream()), "wgrad")
+    dw, db = co.conv3d_weight_grad(x, dout)
+    got = grad.cpu().numpy().astype(np.float64)
+    scale = np.abs(dw).max()
+    assert np.abs(got[:co.TAPS * co.C * co.C] - dw.ravel()).max() <= 2e-6 * scale * max(1.0, nx * ny * nz / 64)
+    assert np.abs(got[co.TAPS * co.C * co.C:] - db).max() <= 1e-5 * max(np.abs(db).max(), 1.0)
+
+
+def test_wgrad_rejects_unsupported_depth():
+    from projecticy_amd import _native_closure as nc
+
+    lib = nc.lib()
+    t = torch.zeros(64, dtype=torch.int16, device="cuda")
+    f = torch.zeros(co.LAYER_PARAMS, dtype=torch.float32, device="cuda")
+    assert lib.icyc_conv3d_wgrad(2, 2, 20, ptr(t), ptr(t), ptr(f), ptr(f), stream()) < 0
+    assert lib.icyc_conv3d_wgrad(2, 2, 80, ptr(t), ptr(t), ptr(f), ptr(f), stream()) < 0
+
+
+def test_network_loss_and_gradients_match_oracle():
+    from projecticy_amd.closure import ClosureNet
+
+    shape = (6, 5, 16)
+    net = ClosureNet(shape, seed=4, last_layer_scale=0.5)
+    rng = np.random.default_rng(12)
+    features = np.zeros(shape + (co.C,), dtype=np.float32)
+    features[..., :8] = rng.normal(size=shape + (8,))
+    target = rng.normal(scale=0.1, size=shape).astype(np.float32)
+    weight = (rng.random(shape) / np.prod(shape)).astype(np.float32)
+    params = net.master.cpu().numpy().astype(np.float64)
+    loss, grad, pred = co.network_gradient(features, params, target, weight, emulate_bf16=True)
+    net.forward(bits_tensor(features))
+    got_loss = net.backward(torch.from_numpy(target.ravel()).cuda(), torch.from_numpy(weight.ravel()).cuda())
+    torch.cuda.synchronize()
+    got_pred = net.pred.cpu().numpy().reshape(shape)
+    assert np.abs(got_pred - pred).max() <= 1e-2 * np.abs(pred).max()
+    assert float(got_loss.item()) == pytest.approx(loss, rel=2e-2)
+    got_grad = net.grad.cpu().numpy().astype(np.float64)
+    for layer in range(4):
+        block = slice(layer * co.LAYER_PARAMS, (layer + 1) * co.LAYER_PARAMS)
+        g, e = got_grad[block], grad[block]
+        assert np.abs(g - e).max() <= 2e-2 * np.abs(e).max(), layer
+        assert np.dot(g, e) / (np.linalg.norm(g) * np.linalg.norm(e)) > 0.999, layer
+
+
+def test_adam_matches_oracle_and_packs_weights():
+    from projecticy_amd.closure import ClosureNet
+
+    net = ClosureNet((2, 2, 16), seed=1)
+    rng = np.random.default_rng(2)
+    master = net.master.cpu().numpy().astype(np.float64)
+    m = np.zeros_like(master)
+    v = np.zeros_like(master)
+    for step in range(1, 4):
+        g = rng.normal(size=master.shape).astype(np.float32)
+        net.grad.copy_(torch.from_numpy(g))
+        net.adam_step(lr=1e-2)
+        master, m, v = co.adam(master, m, v, g.astype(np.float64), 1e-2, 0.9, 0.999, 1e-8, step)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(net.master.cpu().numpy(), master, rtol=1e-5, atol=1e-6)
+    # BF16 forward / backward-data layouts of the updated master weights
+    w = net.master.cpu().numpy()[:co.TAPS * co.C * co.C].reshape(co.TAPS, co.C, co.C)
+    wf = co.from_bf16_bits(net.wf.cpu().numpy()[:w.size].view(np.uint16)).reshape(w.shape)
+    wb = co.from_bf16_bits(net.wb.cpu().numpy()[:w.size].view(np.uint16)).reshape(w.shape)
+    np.testing.assert_array_equal(wf, co.bf16_round(w))
+    np.testing.assert_array_equal(wb, co.bf16_round(w[::-1].transpose(0, 2, 1)))
+
+
+def test_training_reduces_a_smooth_regression_loss():
+    from projecticy_amd.closure import ClosureNet
+
+    shape = (8, 8, 16)
+    rng = np.random.default_rng(5)
+    net = ClosureNet(shape, seed=3)
+    samples = []
+    for _ in range(4):
+        features = np.zeros(shape + (co.C,), dtype=np.float32)
+        features[..., :8] = rng.normal(size=shape + (8,))
+        target = 0.3 * np.tanh(features[..., 0] - 0.5 * features[..., 3]).astype(np.float32)
+        weight = np.full(shape, 1.0 / np.prod(shape), dtype=np.float32)
+        samples.append((bits_tensor(features), torch.from_numpy(target.ravel()).cuda(),
+                        torch.from_numpy(weight.ravel()).cuda()))
+    start = np.mean([net.evaluate(*s) for s in samples])
+    for _ in range(60):
+        for s in samples:
+            net.train_step(*s, lr=3e-3)
+    end = np.mean([net.evaluate(*s) for s in samples])
+    assert end < 0.2 * start, (start, end)

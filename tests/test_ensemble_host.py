@@ -1,0 +1,110 @@
+"""Host side of the batched 2D ensemble (no GPU): the seeded case draws,
+the C ABI's batch planner and the exported batch entry points."""
+
+import ctypes
+import math
+import re
+from pathlib import Path
+
+import pytest
+
+from projecticy_amd import _native2d as native
+from projecticy_amd.iceflow2d.workloads import ensemble_cases, ensemble_draws
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "icy2d.h"
+
+
+def declared_functions():
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(icy2d_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def library():
+    if not native.LIB_PATH.exists():
+        pytest.fail(f"{native.LIB_PATH} is not built (run __graft_entry__.build())")
+    return native.lib()
+
+
+def test_draws_follow_the_ensemble_ranges():
+    draws = ensemble_draws(64, seed=7)
+    assert draws == ensemble_draws(64, seed=7)
+    assert draws != ensemble_draws(64, seed=8)
+    for d in draws:
+        nx, ny = d["nx"], d["ny"]
+        assert nx in range(64, 513, 64) and ny == 2 * nx
+        assert 0.2 * nx - 0.5 <= d["ice_cells"] <= 0.4 * nx + 0.5
+        assert 0.0 <= d["angle_degrees"] <= 15.0
+        assert 60.0 <= d["water_temperature_c"] <= 95.0
+        assert d["water_cells"] <= round(0.7 * ny)
+        # the rotated ice and the 6-cell drop fit below the top wall
+        theta = math.radians(d["angle_degrees"])
+        top = d["water_cells"] + 6.0 + d["ice_cells"] * (math.sin(theta) + math.cos(theta))
+        assert top <= ny - 3 - 2 + 1e-9
+    assert len({d["nx"] for d in draws}) > 4
+
+
+def test_cases_are_valid_configs():
+    configs = ensemble_cases(6, seed=3)
+    for config, d in zip(configs, ensemble_draws(6, seed=3)):
+        assert (config.nx, config.ny) == (d["nx"], d["ny"])
+        assert config.ice_width == d["ice_cells"]
+        assert config.water_height == d["water_cells"]
+        assert config.dx == pytest.approx(2.5e-4)
+        assert config.thermal.initial_water_temperature_c == pytest.approx(d["water_temperature_c"])
+
+
+def test_header_batch_entry_points_are_exported(library):
+    names = declared_functions()
+    batch = [n for n in names if n.startswith("icy2d_batch_")]
+    assert len(batch) >= 40
+    for name in ("icy2d_graph_begin", "icy2d_graph_end", "icy2d_graph_launch", "icy2d_graph_destroy"):
+        assert name in names
+    missing = [n for n in names if not hasattr(library, n)]
+    assert not missing, f"declared but not exported: {missing}"
+    for name in native.BATCH_CALLS:
+        assert "icy2d_batch_" + name in names
+
+
+def _grid(nx, ny, bx, by, capacity):
+    g = native.Grid2D()
+    g.nx, g.ny, g.body_nx, g.body_ny = nx, ny, bx, by
+    g.control = g.partials = 0x1000  # never dereferenced by the planner
+    g.partials_capacity = capacity
+    return g
+
+
+def _blocks(count, threads=256):
+    return (count + threads - 1) // threads
+
+
+def test_batch_plan_prefix_tables(library):
+    sizes = [(64, 128, 20, 20), (37, 11, 5, 7), (512, 1024, 150, 150)]
+    grids = (native.Grid2D * 3)(*[_grid(nx, ny, bx, by, 6 * _blocks(nx * ny) + 64) for nx, ny, bx, by in sizes])
+    first = (ctypes.c_int32 * (native.SHAPES * 4))()
+    totals = (ctypes.c_int32 * native.SHAPES)()
+    assert library.icy2d_batch_plan(grids, 3, first, totals) == 0
+    counts = {
+        0: [_blocks(nx * ny) for nx, ny, _, _ in sizes],
+        1: [_blocks(bx * by) for _, _, bx, by in sizes],
+        2: [_blocks((nx + 1) * ny) for nx, ny, _, _ in sizes],
+        3: [_blocks(nx * (ny + 1)) for nx, ny, _, _ in sizes],
+        4: [1, 1, 1],
+        5: [_blocks(max(nx * ny, bx * by)) for nx, ny, bx, by in sizes],
+        6: [_blocks(nx * ny, 128) for nx, ny, _, _ in sizes],
+        7: [_blocks(nx * ny, 512) for nx, ny, _, _ in sizes],
+    }
+    for shape, per_case in counts.items():
+        table = list(first[shape * 4:(shape + 1) * 4])
+        assert table == [0, per_case[0], per_case[0] + per_case[1], sum(per_case)], shape
+        assert totals[shape] == sum(per_case)
+
+
+def test_batch_plan_rejects_short_partials(library):
+    ok = _grid(64, 64, 8, 8, 6 * _blocks(64 * 64) + 8)
+    short = _grid(64, 64, 8, 8, 6 * _blocks(64 * 64) + 7)
+    first = (ctypes.c_int32 * (native.SHAPES * 3))()
+    totals = (ctypes.c_int32 * native.SHAPES)()
+    assert library.icy2d_batch_plan((native.Grid2D * 2)(ok, ok), 2, first, totals) == 0
+    assert library.icy2d_batch_plan((native.Grid2D * 2)(ok, short), 2, first, totals) < 0
+    assert library.icy2d_batch_plan((native.Grid2D * 1)(ok), 0, first, totals) < 0
