@@ -1,0 +1,189 @@
+"""Training data for the closure from paired coarse / 2x-fine solver runs.
+
+A pair is the same physical tank at two resolutions: the fine grid halves
+dx (and with it the lattice time step), doubles every cell count, and places
+the same four rotated ice cubes (twice the cells) at the same physical
+positions.  The cubes start submerged in an 80 degC water column so the
+ice/water exchange acts from the first thermal update.  Both runs are cut
+into windows of equal physical time (W coarse steps = 2W fine steps); each
+window gives one sample:
+
+  features  the coarse state at the window start (icyc_features),
+  target    the per-cell flux ratio - 1, where the ratio is the
+            fine run's exchanged heat restricted to the coarse cells over the
+            coarse run's, both smoothed over 3x3x3 coarse cells (the two
+            grids place the interface up to one coarse cell apart),
+  weight    the coarse cell's share of the smoothed coarse exchange.
+
+Draw order (numpy.random.default_rng(seed)), per cube until four fit
+without overlap: centre x, y, z (uniform over the submerged box), rotation
+axis (three normals), angle (uniform [0, 30] degrees).
+"""
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from .. import _native_closure as native
+from ..iceflow3d.config import IceBodyConfig, IceFlowConfig, ThermalConfig
+from ..iceflow3d.rigid import rotation_matrix, separating_axis
+
+
+def closure_pair(seed, coarse_shape=(64, 32, 32), coarse_dx_m=2.0 / 3000.0, cube_cells=8, warmup_steps=100,
+                 water_temperature_c=80.0):
+    """(coarse IceFlowConfig, fine IceFlowConfig) of one seeded pair."""
+    nx, ny, nz = coarse_shape
+    boundary = 1
+    water_width, water_height = int(round(0.6 * nx)), int(round(0.7 * ny))
+    rng = np.random.default_rng(seed)
+    half = np.full(3, cube_cells / 2.0)
+    centres, rotations, quats = [], [], []
+    lo = np.array([boundary + 1.0, boundary + 1.0, boundary + 1.0])
+    hi = np.array([water_width - 1.0, water_height - 1.0, nz - boundary - 1.0])
+    attempts = 0
+    while len(centres) < 4:
+        attempts += 1
+        if attempts > 20000:
+            raise RuntimeError("could not place four non-overlapping submerged cubes")
+        centre = lo + (hi - lo) * rng.random(3)
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        angle = math.radians(rng.uniform(0.0, 30.0))
+        q = (math.cos(angle / 2), *(math.sin(angle / 2) * axis))
+        r = rotation_matrix(q)
+        extent = np.abs(r) @ half
+        if np.any(centre - extent < lo) or np.any(centre + extent > hi):
+            continue
+        if any(separating_axis(centre, r, half, c, rr, half) is not None for c, rr in zip(centres, rotations)):
+            continue
+        centres.append(centre)
+        rotations.append(r)
+        quats.append(tuple(float(v) for v in q))
+
+    def build(scale):
+        bodies = tuple(IceBodyConfig(tuple(float(v) * scale for v in c), (cube_cells * scale,) * 3, q)
+                       for c, q in zip(centres, quats))
+        dx = coarse_dx_m / scale
+        return IceFlowConfig(
+            nx=nx * scale, ny=ny * scale, nz=nz * scale, domain_size_m=(nx * coarse_dx_m, ny * coarse_dx_m,
+                                                                        nz * coarse_dx_m),
+            boundary_cells=boundary * scale, water_width=water_width * scale, water_height=water_height * scale,
+            floor_water_depth_cells=0, ice_bodies=bodies, interface_width=4.0,
+            phase_warmup_steps=warmup_steps * scale,
+            thermal=ThermalConfig(initial_water_temperature_c=water_temperature_c))
+
+    return build(1), build(2)
+
+
+def ice_mass_kg(sim):
+    """Total ice mass of a 3D simulator (sum of its material cells)."""
+    return float(sim._t["material_mass"].sum(dtype=torch.float64).item())
+
+
+class HeatRecorder:
+    """Accumulates the heat each water cell hands to the ice (exchange_heat_out)."""
+
+    def __init__(self, sim):
+        self.sim = sim
+        self.heat = torch.zeros(sim.nx * sim.ny * sim.nz, dtype=torch.float32, device=sim.device)
+        sim._grid.exchange_heat_out = self.heat.data_ptr()
+
+    def reset(self):
+        self.heat.zero_()
+
+    def detach(self):
+        self.sim._grid.exchange_heat_out = None
+
+
+def features_of(sim, out=None):
+    """icyc_features of a 3D simulator's state: BF16 channels-last [N * 32]."""
+    lib = native.lib()
+    t = sim._t
+    g = sim._grid
+    cells = sim.nx * sim.ny * sim.nz
+    if out is None:
+        out = torch.zeros(cells * native.CHANNELS, dtype=torch.int16, device=sim.device)
+    p = lambda name: ctypes.c_void_p(t[name].data_ptr())  # noqa: E731
+    native.check(lib.icyc_features(sim.nx, sim.ny, sim.nz, p("phi"), p("water_volume"), p("water_energy"), p("u"),
+                                   p("coverage"), p("cov_id"), g.cell_volume, g.rho_cp, g.tm,
+                                   ctypes.c_void_p(out.data_ptr()),
+                                   ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "icyc_features")
+    return out
+
+
+def _smooth(h, shape):
+    x = h.view(1, 1, *shape)
+    return torch.nn.functional.avg_pool3d(x, 3, stride=1, padding=1, count_include_pad=True).reshape(-1)
+
+
+def targets(coarse_heat, fine_heat, coarse_shape, clamp=(-0.9, 4.0), floor=1e-3):
+    """(target, weight) of one window (see the module docstring)."""
+    nx, ny, nz = coarse_shape
+    restricted = fine_heat.view(nx, 2, ny, 2, nz, 2).sum(dim=(1, 3, 5)).reshape(-1)
+    hc, hf = _smooth(coarse_heat.double(), coarse_shape), _smooth(restricted.double(), coarse_shape)
+    total = float(hc.max().item())
+    valid = hc > floor * total if total > 0.0 else torch.zeros_like(hc, dtype=torch.bool)
+    ratio = torch.where(valid, hf / torch.where(valid, hc, torch.ones_like(hc)), torch.ones_like(hc))
+    target = (ratio - 1.0).clamp(*clamp)
+    weight = torch.where(valid, coarse_heat.double().clamp(min=0.0) + hc, torch.zeros_like(hc))
+    norm = float(weight.sum().item())
+    if norm > 0.0:
+        weight = weight / norm
+    return target.float(), weight.float()
+
+
+def record_pair(coarse_cfg, fine_cfg, windows=4, window_steps=16):
+    """Samples [(features, target, weight)] and the melt histories of one pair.
+
+    The fine run takes 2 * window_steps steps per window (half the time step)."""
+    from ..iceflow3d import IceFlow3D
+
+    coarse, fine = IceFlow3D(coarse_cfg), IceFlow3D(fine_cfg)
+    shape = (coarse.nx, coarse.ny, coarse.nz)
+    rc, rf = HeatRecorder(coarse), HeatRecorder(fine)
+    samples, history = [], [(0, ice_mass_kg(coarse), ice_mass_kg(fine))]
+    for w in range(windows):
+        features = features_of(coarse)
+        rc.reset()
+        rf.reset()
+        coarse.step(window_steps)
+        fine.step(2 * window_steps)
+        target, weight = targets(rc.heat, rf.heat, shape)
+        samples.append((features, target, weight))
+        history.append(((w + 1) * window_steps, ice_mass_kg(coarse), ice_mass_kg(fine)))
+    rc.detach()
+    rf.detach()
+    return samples, history
+
+
+class ClosureCoupling:
+    """Runs a coarse simulator with the network's exchange gain, refreshed at
+    the start of every window from the current state."""
+
+    def __init__(self, sim, net, window_steps=16, max_gain=8.0):
+        if (sim.nx, sim.ny, sim.nz) != net.shape:
+            raise ValueError("the network was built for another grid shape")
+        self.sim, self.net, self.window_steps, self.max_gain = sim, net, int(window_steps), float(max_gain)
+        self.gain = torch.ones(sim.nx * sim.ny * sim.nz, dtype=torch.float32, device=sim.device)
+        self.features = torch.zeros(self.gain.numel() * native.CHANNELS, dtype=torch.int16, device=sim.device)
+        sim._grid.exchange_gain = self.gain.data_ptr()
+
+    def refresh(self):
+        lib, sim = native.lib(), self.sim
+        features_of(sim, self.features)
+        pred = self.net.forward(self.features)
+        native.check(lib.icyc_gain(sim.nx, sim.ny, sim.nz, ctypes.c_void_p(pred.data_ptr()),
+                                   ctypes.c_void_p(sim._t["coverage"].data_ptr()),
+                                   ctypes.c_void_p(sim._t["cov_id"].data_ptr()), self.max_gain,
+                                   ctypes.c_void_p(self.gain.data_ptr()),
+                                   ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "icyc_gain")
+
+    def run(self, windows):
+        for _ in range(int(windows)):
+            self.refresh()
+            self.sim.step(self.window_steps)
+
+    def detach(self):
+        self.sim._grid.exchange_gain = None

@@ -130,8 +130,10 @@ class ClosureNet:
                                         self._stream()), "icyc_adam")
 
     def train_step(self, features, target, weight, lr=1e-3):
+        """One Adam step on one sample; returns the sample's loss before the
+        update (a device tensor)."""
         self.forward(features)
-        loss = self.backward(target, weight)
+        loss = self.backward(target, weight).clone()
         self.adam_step(lr)
         return loss
 

@@ -297,7 +297,8 @@ class IceFlow2D:
             raise native.IcyError(rc, "icy2d_" + name)
 
     def _read_control(self):
-        self._control_host.copy_(self._control_t, non_blocking=True)
+        with torch.cuda.stream(self._stream_handle):
+            self._control_host.copy_(self._control_t, non_blocking=True)
         self._stream_handle.synchronize()
         control = native.Control2D.from_buffer_copy(self._control_host.numpy().tobytes())
         if control.error_flags:
@@ -316,7 +317,8 @@ class IceFlow2D:
 
     def _write_control(self, control):
         raw = torch.frombuffer(bytearray(bytes(control)), dtype=torch.uint8)
-        self._control_t.copy_(raw)
+        with torch.cuda.stream(self._stream_handle):
+            self._control_t.copy_(raw)
 
     def _make_fields(self):
         nx, ny, t = self.nx, self.ny, self._t

@@ -1,8 +1,9 @@
 """The closure's HIP kernels (BF16 MFMA, FP32 accumulation) against the FP64
 CPU oracle on the same BF16 inputs.
 
-Tolerances: BF16 outputs within 1 BF16 ulp (2^-8 relative) of the oracle's
-value, plus 1e-6 of the output's largest magnitude for the FP32
+Tolerances: BF16 outputs within 1 BF16 ulp (at most 2^-7 relative) of the
+oracle's rounded value (the FP32 accumulation order can move a value across a
+rounding boundary), plus 1e-6 of the output's largest magnitude for the FP32
 accumulation order; FP32 outputs within 2e-5 of the largest magnitude;
 gradients, which pass through BF16-rounded intermediate tensors, within 2e-2
 of the largest gradient of their layer."""
@@ -49,7 +50,7 @@ def random_problem(shape, seed):
 def assert_bf16_close(got, expected):
     expected_r = co.bf16_round(expected).astype(np.float64)
     scale = max(np.abs(expected).max(), 1e-30)
-    err = np.abs(got - expected_r) - (2.0 ** -8 * np.abs(expected_r) + 1e-6 * scale)
+    err = np.abs(got - expected_r) - (2.0 ** -7 * np.abs(expected_r) + 1e-6 * scale)
     assert err.max() <= 0.0, f"max excess {err.max():.3e}"
 
 
@@ -74,8 +75,9 @@ def test_conv_forward_epilogues(shape):
     assert np.abs(got - expected[..., 0]).max() <= 2e-5 * np.abs(expected[..., 0]).max()
     # masked epilogue: conv without bias, zero where the mask activation is not positive
     mask = co.bf16_round(np.random.default_rng(1).normal(size=shape + (co.C,)))
-    nc.check(lib.icyc_conv3d(nx, ny, nz, ptr(xt), ptr(wt), None, nc.MASKED_BF16, ptr(bits_tensor(mask)), ptr(out),
-                             None, stream()), "masked")
+    mt = bits_tensor(mask)
+    nc.check(lib.icyc_conv3d(nx, ny, nz, ptr(xt), ptr(wt), None, nc.MASKED_BF16, ptr(mt), ptr(out), None, stream()),
+             "masked")
     assert_bf16_close(values_of(out, shape + (co.C,)), np.where(mask > 0, co.conv3d(x, wf), 0.0))
 
 
@@ -90,8 +92,8 @@ def test_weight_gradient(shape):
     dout = co.bf16_round(rng.normal(size=shape + (co.C,)))
     partials = torch.zeros(int(lib.icyc_wgrad_partials_size(nx, ny, nz)), dtype=torch.float32, device="cuda")
     grad = torch.zeros(co.LAYER_PARAMS, dtype=torch.float32, device="cuda")
-    nc.check(lib.icyc_conv3d_wgrad(nx, ny, nz, ptr(bits_tensor(x)), ptr(bits_tensor(dout)), ptr(partials), ptr(grad),
-                                   stream()), "wgrad")
+    xt, dt = bits_tensor(x), bits_tensor(dout)  # both held until the kernel has run
+    nc.check(lib.icyc_conv3d_wgrad(nx, ny, nz, ptr(xt), ptr(dt), ptr(partials), ptr(grad), stream()), "wgrad")
     dw, db = co.conv3d_weight_grad(x, dout)
     got = grad.cpu().numpy().astype(np.float64)
     scale = np.abs(dw).max()
