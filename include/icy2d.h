@@ -80,8 +80,10 @@ typedef struct icy2d_grid {
   double *flux_ex, *flux_vx, *flux_ey, *flux_vy;
   double* partials;               /* reduction scratch: partials_capacity doubles */
   int32_t partials_capacity;
-  /* Population storage: 0 = FP32; 1 = FP16 holding population * population_scale
-   * (a power of two; FP32 arithmetic in registers).  No reference counterpart. */
+  /* Population storage, a bit mask: bit 0 set = f and fbuf hold FP16
+   * population * population_scale, bit 1 set = h and hbuf do (a power-of-two
+   * scale; FP32 arithmetic in registers); clear bits mean FP32.  No reference
+   * counterpart. */
   int32_t half_populations;
   float population_scale, population_inverse_scale;
   icy2d_control* control;

@@ -129,6 +129,13 @@ int icy3d_stream_home(const icy3d_grid* g, void* stream);               /* home.
 /* classify != 0 recomputes the link classes and boundary lists first
  * (needed after any external edit of the labels). */
 int icy3d_flow_step(const icy3d_grid* g, int parity, int refresh_gradients, int classify, void* stream);
+/* icy3d_flow_step in two parts (same launches, same results): _begin runs up
+ * to the HOME stream, after which the body loads in the control block are
+ * final; _end runs recovery, the stability scan and the gradients.  A host
+ * that copies the loads out between the two can integrate the rigid bodies
+ * while _end runs. */
+int icy3d_flow_step_begin(const icy3d_grid* g, int parity, int refresh_gradients, int classify, void* stream);
+int icy3d_flow_step_end(const icy3d_grid* g, int parity, void* stream);
 
 /* Kernel-group timing of the fused flow step with HIP events on the same
  * stream (no reference counterpart; the reference measures with Taichi's

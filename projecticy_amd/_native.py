@@ -89,7 +89,7 @@ _GRID_CALLS = {
     "classify": [_I3, _I3], "gradients": [_I3, _I3], "refill_gradients": [_I3, _I3],
     "collide_phase": [_i, _i], "recover_phase": [_i, _i], "recover_pressure": [],
     "scan_stability": [], "forces": [_i, _i], "collide_momentum": [], "stream_home": [],
-    "flow_step": [_i, _i, _i], "repair_phase_layout": [_i, _I3, _I3], "export_phase": [_i, _p],
+    "flow_step": [_i, _i, _i], "flow_step_begin": [_i, _i, _i], "flow_step_end": [_i], "repair_phase_layout": [_i, _I3, _I3], "export_phase": [_i, _p],
     "rasterize": [_I3, _I3], "find_refill_changes": [_I3, _I3], "seed_refill_region": [_I3, _I3],
     "propagate_refill": [_I3, _I3], "accept_refill": [_I3, _I3], "commit_refill": [_I3, _I3],
     "build_boundary_lists": [],

@@ -51,24 +51,34 @@ __device__ __forceinline__ bool fluid_at(const icy2d_grid& g, int i, int j) { re
 __device__ __forceinline__ float clamp01(float v) { return fminf(1.0f, fmaxf(0.0f, v)); }
 __device__ __forceinline__ double clamp01d(double v) { return fmin(1.0, fmax(0.0, v)); }
 
-// Population storage (f, fbuf, h, hbuf): FP32, or FP16 holding the
-// population times population_scale (a power of two, so the scaling is exact)
-// with FP32 arithmetic in registers.  The Liang populations are small
-// (O(u) and O(p)); the scale keeps them in the FP16 normal range.
+// Population storage: FP32, or FP16 holding the population times
+// population_scale (a power of two, so the scaling is exact) with FP32
+// arithmetic in registers.  Bit 0 of half_populations selects FP16 for the
+// momentum populations (f, fbuf), bit 1 for the phase populations (h, hbuf).
+// The Liang populations are small (O(u) and O(p)); the scale keeps them in
+// the FP16 normal range.
+template <int Kind>
+__device__ __forceinline__ bool half_storage(const icy2d_grid& g) {
+  return (g.half_populations >> Kind) & 1;
+}
+template <int Kind>
 __device__ __forceinline__ float pop_get(const float* base, const icy2d_grid& g, int idx) {
-  if (g.half_populations) return __half2float(reinterpret_cast<const __half*>(base)[idx]) * g.population_inverse_scale;
+  if (half_storage<Kind>(g)) return __half2float(reinterpret_cast<const __half*>(base)[idx]) * g.population_inverse_scale;
   return base[idx];
 }
+template <int Kind>
 __device__ __forceinline__ void pop_set(float* base, const icy2d_grid& g, int idx, float v) {
-  if (g.half_populations) {
+  if (half_storage<Kind>(g)) {
     reinterpret_cast<__half*>(base)[idx] = __float2half(v * g.population_scale);
   } else {
     base[idx] = v;
   }
 }
+template <int Kind>
 __device__ __forceinline__ void pop_add(float* base, const icy2d_grid& g, int idx, float v) {
-  pop_set(base, g, idx, pop_get(base, g, idx) + v);
+  pop_set<Kind>(base, g, idx, pop_get<Kind>(base, g, idx) + v);
 }
+constexpr int kMomentum = 0, kPhase = 1;
 
 // lattice.py:64-104
 template <int Q>
@@ -436,8 +446,8 @@ __device__ __forceinline__ void k2_initialize_fluid(const icy2d_grid& g, const i
   const float rho = mixture(g, phi0);
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
-    pop_set(g.f, g, q * N + n, momentum_eq_rt(q, 0.0f, rho, v[0], v[1]));
-    pop_set(g.h, g, q * N + n, active ? phase_eq_rt(q, phi0, v[0], v[1]) : 0.0f);
+    pop_set<kMomentum>(g.f, g, q * N + n, momentum_eq_rt(q, 0.0f, rho, v[0], v[1]));
+    pop_set<kPhase>(g.h, g, q * N + n, active ? phase_eq_rt(q, phi0, v[0], v[1]) : 0.0f);
   }
 }
 // simulator.py:2664-2697
@@ -451,8 +461,8 @@ __device__ __forceinline__ void k2_hydrostatic_equilibrium(const icy2d_grid& g, 
     g.p[n] = g.p_ref[j];
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-      pop_set(g.f, g, q * N + n, momentum_eq_rt(q, 0.0f, rho, 0.0f, 0.0f));
-      pop_set(g.h, g, q * N + n, phase_eq_rt(q, phi, 0.0f, 0.0f));
+      pop_set<kMomentum>(g.f, g, q * N + n, momentum_eq_rt(q, 0.0f, rho, 0.0f, 0.0f));
+      pop_set<kPhase>(g.h, g, q * N + n, phase_eq_rt(q, phi, 0.0f, 0.0f));
     }
   } else {
     float v[2] = {0.0f, 0.0f};
@@ -482,8 +492,8 @@ __device__ __forceinline__ void k2_recover_initial_moments(const icy2d_grid& g, 
     float phi = 0.0f, mx = 0.0f, my = 0.0f;
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-      phi += pop_get(g.h, g, q * N + n);
-      const float fv = pop_get(g.f, g, q * N + n);
+      phi += pop_get<kPhase>(g.h, g, q * N + n);
+      const float fv = pop_get<kMomentum>(g.f, g, q * N + n);
       if (cxq(q)) mx += (float)cxq(q) * fv;
       if (cyq(q)) my += (float)cyq(q) * fv;
     }
@@ -561,7 +571,7 @@ __device__ __forceinline__ bool prepare_cut_links(const icy2d_grid& g, const Pos
     float ub[2];
     body_velocity(p, bx, by, ub);
     const int o = oppq(q);
-    const float noneq = pop_get(g.f, g, q * N + n) - momentum_eq_rt(q, pdyn, rho, ux, uy);
+    const float noneq = pop_get<kMomentum>(g.f, g, q * N + n) - momentum_eq_rt(q, pdyn, rho, ux, uy);
     const float wall = momentum_eq_rt(o, pdyn, rho, ub[0], ub[1]);
     const float reflected = (wall + noneq + eta * raw[o]) / (1.0f + eta);
     prepared[q] = reflected;
@@ -654,7 +664,7 @@ __device__ __forceinline__ void k2_collide(const icy2d_grid& g, const int blk, i
         float raw[6] = {0, 0, 0, 0, 0, 0}, eq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, src[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
-          const float cx = (float)cxq(q), cy = (float)cyq(q), v = pop_get(g.f, g, q * N + n);
+          const float cx = (float)cxq(q), cy = (float)cyq(q), v = pop_get<kMomentum>(g.f, g, q * N + n);
           const float e = momentum_eq_rt(q, pdyn, rho, ux, uy);
           const float b = wq(q) * 3.0f * (cx * Fx + cy * Fy + drho * (cx * ux + cy * uy) * (cx * gsx + cy * gsy));
           const float k[9] = {1.0f, cx, cy, cx * cx, cy * cy, cx * cy, cx * cx * cy, cx * cy * cy, cx * cx * cy * cy};
@@ -691,7 +701,7 @@ __device__ __forceinline__ void k2_collide(const icy2d_grid& g, const int blk, i
         out[8] = reconstruct_cm<8>(0.0f, 0.0f, post);
         prepare_cut_links(g, pose, i, j, out, load);
 #pragma unroll
-        for (int q = 0; q < 9; ++q) pop_set(g.fbuf, g, q * N + n, out[q]);
+        for (int q = 0; q < 9; ++q) pop_set<kMomentum>(g.fbuf, g, q * N + n, out[q]);
       } else {
         ax = g.a[n];
         ay = g.a[N + n];
@@ -705,7 +715,7 @@ __device__ __forceinline__ void k2_collide(const icy2d_grid& g, const int blk, i
       float r10 = 0.0f, r01 = 0.0f, eq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, src[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
       for (int q = 0; q < 9; ++q) {
-        const float rx = (float)cxq(q) - ux, ry = (float)cyq(q) - uy, hv = pop_get(g.h, g, q * N + n);
+        const float rx = (float)cxq(q) - ux, ry = (float)cyq(q) - uy, hv = pop_get<kPhase>(g.h, g, q * N + n);
         const float he = phase_eq_rt(q, phi, ux, uy);
         const float force = wq(q) * compression * ((float)cxq(q) * nx_ + (float)cyq(q) * ny_);
         const float rx2 = rx * rx, ry2 = ry * ry;
@@ -723,15 +733,15 @@ __device__ __forceinline__ void k2_collide(const icy2d_grid& g, const int blk, i
       for (int m = 0; m < 9; ++m) post[m] = eq[m] + 0.5f * src[m];
       post[1] = r10 - rate * (r10 - eq[1]) + (1.0f - 0.5f * rate) * src[1];
       post[2] = r01 - rate * (r01 - eq[2]) + (1.0f - 0.5f * rate) * src[2];
-      pop_set(g.hbuf, g, 0 * N + n, reconstruct_cm<0>(ux, uy, post));
-      pop_set(g.hbuf, g, 1 * N + n, reconstruct_cm<1>(ux, uy, post));
-      pop_set(g.hbuf, g, 2 * N + n, reconstruct_cm<2>(ux, uy, post));
-      pop_set(g.hbuf, g, 3 * N + n, reconstruct_cm<3>(ux, uy, post));
-      pop_set(g.hbuf, g, 4 * N + n, reconstruct_cm<4>(ux, uy, post));
-      pop_set(g.hbuf, g, 5 * N + n, reconstruct_cm<5>(ux, uy, post));
-      pop_set(g.hbuf, g, 6 * N + n, reconstruct_cm<6>(ux, uy, post));
-      pop_set(g.hbuf, g, 7 * N + n, reconstruct_cm<7>(ux, uy, post));
-      pop_set(g.hbuf, g, 8 * N + n, reconstruct_cm<8>(ux, uy, post));
+      pop_set<kPhase>(g.hbuf, g, 0 * N + n, reconstruct_cm<0>(ux, uy, post));
+      pop_set<kPhase>(g.hbuf, g, 1 * N + n, reconstruct_cm<1>(ux, uy, post));
+      pop_set<kPhase>(g.hbuf, g, 2 * N + n, reconstruct_cm<2>(ux, uy, post));
+      pop_set<kPhase>(g.hbuf, g, 3 * N + n, reconstruct_cm<3>(ux, uy, post));
+      pop_set<kPhase>(g.hbuf, g, 4 * N + n, reconstruct_cm<4>(ux, uy, post));
+      pop_set<kPhase>(g.hbuf, g, 5 * N + n, reconstruct_cm<5>(ux, uy, post));
+      pop_set<kPhase>(g.hbuf, g, 6 * N + n, reconstruct_cm<6>(ux, uy, post));
+      pop_set<kPhase>(g.hbuf, g, 7 * N + n, reconstruct_cm<7>(ux, uy, post));
+      pop_set<kPhase>(g.hbuf, g, 8 * N + n, reconstruct_cm<8>(ux, uy, post));
     }
   }
   if (with_momentum) block_partials<3>(load, g.partials, blk);
@@ -746,10 +756,10 @@ __device__ __forceinline__ void k2_prepare_cut_links(const icy2d_grid& g, const 
     if (fluid_at(g, i, j)) {
       float raw[9];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) raw[q] = pop_get(g.fbuf, g, q * N + n);
+      for (int q = 0; q < 9; ++q) raw[q] = pop_get<kMomentum>(g.fbuf, g, q * N + n);
       prepare_cut_links(g, load_pose(g), i, j, raw, load);
 #pragma unroll
-      for (int q = 0; q < 9; ++q) pop_set(g.fbuf, g, q * N + n, raw[q]);
+      for (int q = 0; q < 9; ++q) pop_set<kMomentum>(g.fbuf, g, q * N + n, raw[q]);
     }
   }
   block_partials<3>(load, g.partials, blk);
@@ -763,12 +773,12 @@ __device__ __forceinline__ void pull(const icy2d_grid& g, const Pose& pose, int 
   const int si = i - cxq(Q), sj = j - cyq(Q);  // source of population Q
   if (Q == 0 || (inside(g, si, sj) && fluid_at(g, si, sj))) {
     const int m = Q == 0 ? n : at(g, si, sj);
-    if (!phase_only) fq = pop_get(g.fbuf, g, Q * N + m);
-    hq = pop_get(g.hbuf, g, Q * N + m);
+    if (!phase_only) fq = pop_get<kMomentum>(g.fbuf, g, Q * N + m);
+    hq = pop_get<kPhase>(g.hbuf, g, Q * N + m);
   } else {
     constexpr int out = oppq(Q);  // own population sent towards (si, sj)
-    if (!phase_only) fq = pop_get(g.fbuf, g, out * N + n);
-    float ho = pop_get(g.hbuf, g, out * N + n);
+    if (!phase_only) fq = pop_get<kMomentum>(g.fbuf, g, out * N + n);
+    float ho = pop_get<kPhase>(g.hbuf, g, out * N + n);
     if (inside(g, si, sj) && g.solid[at(g, si, sj)] == 1) {
       const float eta = cut_fraction(g, pose, i, j, si, sj);
       float ub[2];
@@ -803,14 +813,14 @@ __device__ __forceinline__ void k2_stream(const icy2d_grid& g, const int blk, in
       pull<8>(g, pose, i, j, n, phi_old, po, f[8], h[8]);
 #pragma unroll
       for (int q = 0; q < 9; ++q) {
-        pop_set(g.h, g, q * N + n, h[q]);
-        if (!po) pop_set(g.f, g, q * N + n, f[q]);
+        pop_set<kPhase>(g.h, g, q * N + n, h[q]);
+        if (!po) pop_set<kMomentum>(g.f, g, q * N + n, f[q]);
       }
     } else {
 #pragma unroll
       for (int q = 0; q < 9; ++q) {
-        h[q] = pop_get(g.h, g, q * N + n);
-        if (!po) f[q] = pop_get(g.f, g, q * N + n);
+        h[q] = pop_get<kPhase>(g.h, g, q * N + n);
+        if (!po) f[q] = pop_get<kMomentum>(g.f, g, q * N + n);
       }
     }
     if (!update) return;
@@ -916,8 +926,8 @@ __device__ __forceinline__ void k2_refill(const icy2d_grid& g, const int blk) {
   const float pdyn = p0 - g.p_ref[j];
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
-    pop_set(g.f, g, q * N + n, momentum_eq_rt(q, pdyn, rho, v[0], v[1]));
-    pop_set(g.h, g, q * N + n, phase_eq_rt(q, phi0, v[0], v[1]));
+    pop_set<kMomentum>(g.f, g, q * N + n, momentum_eq_rt(q, pdyn, rho, v[0], v[1]));
+    pop_set<kPhase>(g.h, g, q * N + n, phase_eq_rt(q, phi0, v[0], v[1]));
   }
 }
 // simulator.py:2303-2338 (per-block maxima as partials)
@@ -964,15 +974,15 @@ __device__ __forceinline__ void lift_phase(const icy2d_grid& g, int n, int j, fl
   const float ro = mixture(g, old_phase), rn = mixture(g, new_phase);
 #pragma unroll
   for (int q = 0; q < 9; ++q)
-    pop_add(g.f, g, q * N + n, momentum_eq_rt(q, pdyn, rn, ux, uy) - momentum_eq_rt(q, pdyn, ro, ux, uy));
+    pop_add<kMomentum>(g.f, g, q * N + n, momentum_eq_rt(q, pdyn, rn, ux, uy) - momentum_eq_rt(q, pdyn, ro, ux, uy));
   float sum = 0.0f;
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
-    const float lifted = pop_get(g.h, g, q * N + n) + phase_eq_rt(q, new_phase, ux, uy) - phase_eq_rt(q, old_phase, ux, uy);
-    pop_set(g.h, g, q * N + n, lifted);
+    const float lifted = pop_get<kPhase>(g.h, g, q * N + n) + phase_eq_rt(q, new_phase, ux, uy) - phase_eq_rt(q, old_phase, ux, uy);
+    pop_set<kPhase>(g.h, g, q * N + n, lifted);
     sum += lifted;
   }
-  pop_add(g.h, g, n, new_phase - sum);
+  pop_add<kPhase>(g.h, g, n, new_phase - sum);
   g.phi[n] = new_phase;
 }
 // simulator.py:4150-4229
@@ -992,11 +1002,11 @@ __device__ __forceinline__ void k2_clip_phase(const icy2d_grid& g, const int blk
         const float ro = mixture(g, old_phase), rn = mixture(g, bounded);
 #pragma unroll
         for (int q = 0; q < 9; ++q)
-          pop_add(g.f, g, q * N + n, momentum_eq_rt(q, pdyn, rn, ux, uy) - momentum_eq_rt(q, pdyn, ro, ux, uy));
+          pop_add<kMomentum>(g.f, g, q * N + n, momentum_eq_rt(q, pdyn, rn, ux, uy) - momentum_eq_rt(q, pdyn, ro, ux, uy));
       }
       float l1 = 0.0f;
 #pragma unroll
-      for (int q = 0; q < 9; ++q) l1 += fabsf(pop_get(g.h, g, q * N + n));
+      for (int q = 0; q < 9; ++q) l1 += fabsf(pop_get<kPhase>(g.h, g, q * N + n));
       const bool regularize = (bounded == 0.0f || bounded == 1.0f) && (double)l1 > max_bulk_l1;
       if (regularize) {
         const float px = g.u[n] + 0.5f * g.a[n], py = g.u[N + n] + 0.5f * g.a[N + n];
@@ -1004,19 +1014,19 @@ __device__ __forceinline__ void k2_clip_phase(const icy2d_grid& g, const int blk
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
           const float r = phase_eq_rt(q, bounded, px, py);
-          pop_set(g.h, g, q * N + n, r);
+          pop_set<kPhase>(g.h, g, q * N + n, r);
           sum += r;
         }
-        pop_add(g.h, g, n, bounded - sum);
+        pop_add<kPhase>(g.h, g, n, bounded - sum);
       } else if (bounded != old_phase) {
         float sum = 0.0f;
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
-          const float lifted = pop_get(g.h, g, q * N + n) + phase_eq_rt(q, bounded, ux, uy) - phase_eq_rt(q, old_phase, ux, uy);
-          pop_set(g.h, g, q * N + n, lifted);
+          const float lifted = pop_get<kPhase>(g.h, g, q * N + n) + phase_eq_rt(q, bounded, ux, uy) - phase_eq_rt(q, old_phase, ux, uy);
+          pop_set<kPhase>(g.h, g, q * N + n, lifted);
           sum += lifted;
         }
-        pop_add(g.h, g, n, bounded - sum);
+        pop_add<kPhase>(g.h, g, n, bounded - sum);
       }
       g.phi[n] = bounded;
       v[0] = (double)bounded;
@@ -1044,8 +1054,8 @@ __device__ __forceinline__ void k2_seed_projection(const icy2d_grid& g, const in
         }
     }
   }
-  pop_set(g.hbuf, g, n, (float)seed);
-  pop_set(g.hbuf, g, N + n, 0.0f);
+  pop_set<kPhase>(g.hbuf, g, n, (float)seed);
+  pop_set<kPhase>(g.hbuf, g, N + n, 0.0f);
 }
 // simulator.py:4259-4289
 __device__ __forceinline__ void k2_dilate_projection(const icy2d_grid& g, const int blk, int primary_to_next) {
@@ -1056,15 +1066,15 @@ __device__ __forceinline__ void k2_dilate_projection(const icy2d_grid& g, const 
   int value = 0;
   const float phase = g.phi[n];
   if (fluid_at(g, i, j) && g.cutoff < phase && phase < 1.0f - g.cutoff) {
-    value = (int)pop_get(g.hbuf, g, src + n);
+    value = (int)pop_get<kPhase>(g.hbuf, g, src + n);
     if (value == 0)
       for (int di = -1; di <= 1; ++di)
         for (int dj = -1; dj <= 1; ++dj) {
           const int ni = i + di, nj = j + dj;
-          if ((di || dj) && inside(g, ni, nj) && pop_get(g.hbuf, g, src + at(g, ni, nj)) == 1.0f) value = 1;
+          if ((di || dj) && inside(g, ni, nj) && pop_get<kPhase>(g.hbuf, g, src + at(g, ni, nj)) == 1.0f) value = 1;
         }
   }
-  pop_set(g.hbuf, g, dst + n, (float)value);
+  pop_set<kPhase>(g.hbuf, g, dst + n, (float)value);
 }
 // simulator.py:4291-4351
 __device__ __forceinline__ void k2_evaluate_projection(const icy2d_grid& g, const int blk, double lambda) {
@@ -1076,7 +1086,7 @@ __device__ __forceinline__ void k2_evaluate_projection(const icy2d_grid& g, cons
     if (fluid_at(g, i, j)) {
       const double phase = g.phi[n];
       double mapped = phase;
-      const bool adjustable = pop_get(g.hbuf, g, n) == 1.0f;
+      const bool adjustable = pop_get<kPhase>(g.hbuf, g, n) == 1.0f;
       if (adjustable) mapped = phase * e / (1.0 - phase + phase * e);
       float m32 = (float)mapped;
       if (m32 <= g.cutoff) m32 = 0.0f;
@@ -1097,7 +1107,7 @@ __device__ __forceinline__ void k2_apply_projection(const icy2d_grid& g, const i
     if (fluid_at(g, i, j)) {
       const float old_phase = g.phi[n];
       float new_phase = old_phase;
-      if (pop_get(g.hbuf, g, n) == 1.0f) {
+      if (pop_get<kPhase>(g.hbuf, g, n) == 1.0f) {
         const double p = old_phase;
         new_phase = (float)(p * e / (1.0 - p + p * e));
       }
@@ -1112,7 +1122,7 @@ __device__ __forceinline__ void k2_apply_projection(const icy2d_grid& g, const i
 __device__ __forceinline__ bool residual_free(const icy2d_grid& g, int n, double phase, double residual) {
   const double margin = 4.0 * 1.1920928955078125e-07;
   const double cutoff = g.cutoff;
-  if (!(pop_get(g.hbuf, g, n) == 1.0f && cutoff < phase && phase < 1.0 - cutoff)) return false;
+  if (!(pop_get<kPhase>(g.hbuf, g, n) == 1.0f && cutoff < phase && phase < 1.0 - cutoff)) return false;
   const double magnitude = fabs(residual);
   if (residual > 0.0) return 1.0 - cutoff - phase > magnitude + margin;
   return phase - cutoff > magnitude + margin;
@@ -1240,7 +1250,7 @@ __device__ __forceinline__ void k2_momentum_partials(const icy2d_grid& g, const 
       double mx = 0.0, my = 0.0;
 #pragma unroll
       for (int q = 0; q < 9; ++q) {
-        const double fv = pop_get(g.f, g, q * N + n);
+        const double fv = pop_get<kMomentum>(g.f, g, q * N + n);
         if (cxq(q)) mx += (double)cxq(q) * fv;
         if (cyq(q)) my += (double)cyq(q) * fv;
       }
@@ -1350,7 +1360,7 @@ __device__ __forceinline__ void add_momentum(const icy2d_grid& g, int n, double 
   g.u[n] += mx / rho;
   g.u[N + n] += my / rho;
 #pragma unroll
-  for (int q = 0; q < 9; ++q) pop_add(g.f, g, q * N + n, 3.0f * wq(q) * ((float)cxq(q) * mx + (float)cyq(q) * my));
+  for (int q = 0; q < 9; ++q) pop_add<kMomentum>(g.f, g, q * N + n, 3.0f * wq(q) * ((float)cxq(q) * mx + (float)cyq(q) * my));
 }
 // simulator.py:2842-2884
 __device__ __forceinline__ void k2_apply_melt_correction(const icy2d_grid& g, const int blk) {

@@ -2013,6 +2013,38 @@ int icy3d_flow_step_profiled(const icy3d_grid* g, int parity, int refresh_gradie
 int icy3d_flow_step(const icy3d_grid* g, int parity, int refresh_gradients, int classify, void* stream) {
   return icy3d_flow_step_profiled(g, parity, refresh_gradients, classify, nullptr, stream);
 }
+// The same launches in two parts: everything up to the stream (after which
+// the body loads in the control block are final), then recovery and the
+// gradients.  A caller copies the loads out between the two and integrates
+// the rigid bodies on the host while the second part runs.
+int icy3d_flow_step_begin(const icy3d_grid* g, int parity, int refresh_gradients, int classify, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  const icy3d_grid& G = *g;
+  const unsigned blocks = icy_blocks(grid_cells(G));
+  int rc = 0;
+  if (classify || refresh_gradients) {
+    rc = launch_classify_all(G, s);
+    if (rc) return rc;
+  }
+  if (refresh_gradients) {
+    rc = launch_gradients(G, full_box(G), true, s);
+    if (rc) return rc;
+  }
+  k_collide_phase<<<blocks, kBlock, 0, s>>>(G, parity, 1);
+  rc = launch_forces(G, 1, 1, s);
+  if (rc) return rc;
+  k_collide_momentum<<<blocks, kBlock, 0, s>>>(G);
+  k_zero_loads<<<1, ICY3D_MAX_BODIES * 6, 0, s>>>(G);
+  return launch_stream(G, s);
+}
+int icy3d_flow_step_end(const icy3d_grid* g, int parity, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = launch_recover(*g, 1 - parity, 1, s);
+  if (!rc) rc = launch_gradients(*g, full_box(*g), true, s);
+  return rc;
+}
 int icy3d_profile_create(icy3d_profile** out) {
   if (!out) return ICY3D_ERR_ARGUMENT;
   icy3d_profile* p = new icy3d_profile();

@@ -38,7 +38,7 @@ THREAD_CHOICES = (128, 256, 512)
 class IceFlowEnsemble2D:
     """Many ``IceFlow2D`` cases advanced in lockstep, one launch per kernel."""
 
-    def __init__(self, configs, *, population_storage="fp16", use_graph=True):
+    def __init__(self, configs, *, population_storage="fp16", population_scale=None, use_graph=True):
         configs = list(configs)
         if not configs:
             raise ValueError("an ensemble needs at least one case")
@@ -55,8 +55,9 @@ class IceFlowEnsemble2D:
         self._controls = torch.zeros((n, size), dtype=torch.uint8, device=self.device)
         self._controls_host = torch.zeros((n, size), dtype=torch.uint8, pin_memory=True)
         with torch.cuda.stream(self._stream):
-            self.cases = [IceFlow2D(c, population_storage=population_storage, _control_storage=self._controls[i])
-                          for i, c in enumerate(configs)]
+            extra = {} if population_scale is None else {"population_scale": population_scale}
+            self.cases = [IceFlow2D(c, population_storage=population_storage, _control_storage=self._controls[i],
+                                    **extra) for i, c in enumerate(configs)]
         self._stream.synchronize()
         self.cells = sum(case.nx * case.ny for case in self.cases)
         host_grids = (native.Grid2D * n)(*[case._g for case in self.cases])

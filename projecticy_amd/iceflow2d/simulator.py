@@ -21,6 +21,8 @@ from .config import BOUNDARY_CODE, IceFlowConfig, LatticeScales, MovingBodyTherm
 
 MAX_BULK_PHASE_L1 = 2.0          # simulator.py:45
 POPULATION_SCALE = 1024.0         # FP16 population storage holds population * 2^10
+# storage name -> bit mask of FP16 buffers (bit 0 momentum f/fbuf, bit 1 phase h/hbuf)
+POPULATION_STORAGE = {"fp32": 0, "fp16": 3, "fp16-momentum": 1}
 MAX_LATTICE_VELOCITY_L1 = 1.0    # simulator.py:44
 
 
@@ -160,10 +162,14 @@ class Thermal2D:
 class IceFlow2D:
     """MI355X solver for the coupled falling-ice melting scenario."""
 
-    def __init__(self, config, *, population_storage="fp32", _control_storage=None):
-        """``population_storage="fp16"`` keeps the four population buffers in
-        FP16 (scaled by 2^10) with FP32 arithmetic; it has no reference
-        counterpart.  Every other argument and check is the reference's.
+    def __init__(self, config, *, population_storage="fp32", population_scale=POPULATION_SCALE,
+                 _control_storage=None):
+        """``population_storage`` has no reference counterpart: "fp32" (the
+        reference's), "fp16" (all four population buffers in FP16 scaled by
+        2^10, FP32 arithmetic) or "fp16-momentum" (the momentum populations in
+        FP16, the phase populations in FP32); FP16 buffers hold the population
+        times ``population_scale`` (a power of two).  Every other argument and
+        check is the reference's.
         ``_control_storage`` (a device uint8 tensor of the control block's
         size) lets an ensemble keep every case's control block in one array."""
         # simulator.py:1829-1843
@@ -179,9 +185,13 @@ class IceFlow2D:
             raise ValueError("IceFlow2D requires well-balanced hydrostatics")
         if config.thermal.water_buoyancy_model != "linear":
             raise ValueError("IceFlow2D supports only the linear buoyancy model")
-        if population_storage not in ("fp32", "fp16"):
-            raise ValueError("population_storage must be 'fp32' or 'fp16'")
+        if population_storage not in POPULATION_STORAGE:
+            raise ValueError(f"population_storage must be one of {sorted(POPULATION_STORAGE)}")
         self.population_storage = population_storage
+        scale = float(population_scale)
+        if not (scale > 0.0 and math.frexp(scale)[0] == 0.5):
+            raise ValueError("population_scale must be a positive power of two")
+        self.population_scale = scale
         self._control_storage = _control_storage
         self.device = _require_device()
         self._lib = native.lib()
@@ -256,9 +266,11 @@ class IceFlow2D:
         M = self.config.ice_width * self.config.ice_height
         dev = self.device
         f32, f64, i8, i32 = torch.float32, torch.float64, torch.int8, torch.int32
-        pop = torch.float16 if self.population_storage == "fp16" else f32
+        bits = POPULATION_STORAGE[self.population_storage]
+        fpop = torch.float16 if bits & 1 else f32
+        hpop = torch.float16 if bits & 2 else f32
         shapes = {
-            "f": (9 * N, pop), "fbuf": (9 * N, pop), "h": (9 * N, pop), "hbuf": (9 * N, pop),
+            "f": (9 * N, fpop), "fbuf": (9 * N, fpop), "h": (9 * N, hpop), "hbuf": (9 * N, hpop),
             "phi": (N, f32), "u": (2 * N, f32), "a": (2 * N, f32), "p": (N, f32), "moving": (N, f32),
             "p_ref": (ny, f32), "rho_ref": (ny, f32), "solid": (N, i8), "prev_solid": (N, i8), "eligible": (N, i8),
             "coverage": (N, f32), "body_mass": (M, f64), "body_energy": (M, f64), "body_heat": (M, f64),
@@ -285,9 +297,9 @@ class IceFlow2D:
         for name in native.GRID_POINTERS:
             setattr(g, name, self._t[name].data_ptr())
         g.partials_capacity = capacity
-        g.half_populations = int(self.population_storage == "fp16")
-        g.population_scale = POPULATION_SCALE
-        g.population_inverse_scale = 1.0 / POPULATION_SCALE
+        g.half_populations = POPULATION_STORAGE[self.population_storage]
+        g.population_scale = self.population_scale
+        g.population_inverse_scale = 1.0 / self.population_scale
         g.control = self._control_t.data_ptr()
         self._gp = ctypes.pointer(g)
 
@@ -322,12 +334,10 @@ class IceFlow2D:
 
     def _make_fields(self):
         nx, ny, t = self.nx, self.ny, self._t
-        if self.population_storage == "fp16":
-            def pop_field(name):
-                return ScaledPopulationField(t[name].view(9, nx, ny), POPULATION_SCALE, components=9)
-        else:
-            def pop_field(name):
-                return DeviceField(t[name].view(9, nx, ny), components=9)
+        def pop_field(name):
+            if t[name].dtype == torch.float16:
+                return ScaledPopulationField(t[name].view(9, nx, ny), self.population_scale, components=9)
+            return DeviceField(t[name].view(9, nx, ny), components=9)
         self.momentum_populations = pop_field("f")
         self.momentum_stream_buffer = pop_field("fbuf")
         self.phase_populations = pop_field("h")

@@ -125,6 +125,8 @@ class IceFlow3D:
             "control": torch.zeros(ctypes.sizeof(Control), dtype=torch.uint8, device=dev),
         }
         self._control_host = torch.zeros(ctypes.sizeof(Control), dtype=torch.uint8).pin_memory()
+        self._loads_host = torch.zeros(ctypes.sizeof(Control), dtype=torch.uint8).pin_memory()
+        self._loads_ready = torch.cuda.Event()
         g = Grid()
         g.nx, g.ny, g.nz = self.shape
         g.boundary, g.bodies = cfg.boundary_cells, B
@@ -552,6 +554,7 @@ class IceFlow3D:
             self._call("project_refill", self._parity, int3(lower), int3(upper + 1))
         if phase_changed or has_openings or covered:
             self._synchronize_water()
+        return control
 
     def _advance_advection(self, speed_bound=None):
         """thermal.py:208-226."""
@@ -628,26 +631,35 @@ class IceFlow3D:
             refresh = not self._gradient_cache_valid
             classify = int(not self._flags_valid)
             if profile is None:
-                self._call("flow_step", self._parity, int(refresh), classify)
+                # The loads are final after the stream: copy them out, queue the
+                # rest of the flow step, and integrate the bodies on the host
+                # while the GPU runs recovery and the gradients.
+                self._call("flow_step_begin", self._parity, int(refresh), classify)
+                self._loads_host.copy_(self._t["control"], non_blocking=True)
+                self._loads_ready.record(torch.cuda.current_stream(self.device))
+                self._call("flow_step_end", self._parity)
+                self._loads_ready.synchronize()
+                control = Control.from_buffer_copy(self._loads_host.numpy().tobytes())
             else:
                 check(self._lib.icy3d_flow_step_profiled(self._gp, self._parity, int(refresh), classify, profile,
                                                          self._stream()), "flow_step")
+                control = self._read_control()
+                check(self._lib.icy3d_profile_collect(profile), "profile_collect")
             self._gradient_cache_valid = True
             self._flags_valid = True
             self._parity = 1 - self._parity
-            control = self._read_control()
-            if profile is not None:
-                check(self._lib.icy3d_profile_collect(profile), "profile_collect")
+            loads = np.array(control.loads, dtype=np.float32)[: self.body_count]
+            self.rigid.integrate(loads)
+            self.rigid.resolve_contacts()
+            self.rigid.upload()
+            # The geometry refresh's first control read follows the recovery's
+            # stability scan in stream order.
+            control = self._refresh_geometry()
             speed_bound = None
             if self.config.check_stability:
                 if control.invalid_index < math.prod(self.shape):
                     self._raise_unstable(control.invalid_index)
                 speed_bound = float(control.maximum_speed)
-            loads = np.array(control.loads, dtype=np.float32)[: self.body_count]
-            self.rigid.integrate(loads)
-            self.rigid.resolve_contacts()
-            self.rigid.upload()
-            self._refresh_geometry()
             self._advance_advection(speed_bound)
             self._thermal_pending += 1
             if self._thermal_pending >= self.config.thermal.update_interval_lbm_steps:
