@@ -23,7 +23,10 @@ The JSON line adds:
                 the same figure for one-case-at-a-time FP32 runs, and each
                 storage's melted-mass error after 10,000 steps against the
                 FP32 standalone runs; ``value`` is the best cases/hour among
-                the storages within 1%.
+                the storages within 1%;
+  closure       the learned closure's BF16 MFMA forward and backward kernels
+                on the 128x64x64 coarse grid, in TFLOP/s against the 2.5
+                PFLOP/s dense BF16 peak.
 """
 
 import argparse
@@ -69,6 +72,8 @@ def parse():
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="after the measurements, run five FP32 scale-by-one copies of this many MiB (the "
                         "reference kernel for calibrating HBM byte counters)")
+    p.add_argument("--closure-repeats", type=int, default=20,
+                   help="timed forward+backward passes of the closure network at 128x64x64 (0 skips it)")
     p.add_argument("--ensemble-cases", type=int, default=64)
     p.add_argument("--ensemble-steps", type=int, default=400,
                    help="timed steps of the batched FP16 D2Q9 ensemble (0 skips it)")
@@ -116,6 +121,59 @@ def d2q9_measurement(args):
 
 
 ENSEMBLE_CASE_STEPS = 10000  # steps of one ensemble case
+
+
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # dense BF16 matrix-core peak of MI355X
+
+
+def closure_measurement(args):
+    """The learned closure's BF16 MFMA kernels on the coarse 128x64x64 grid:
+    forward (4 convolutions) and backward (weight gradients of all four,
+    data gradients of the last three) of one sample, timed with HIP events."""
+    import numpy as np
+    import torch
+
+    from projecticy_amd.closure import ClosureNet
+
+    shape = (128, 64, 64)
+    cells = shape[0] * shape[1] * shape[2]
+    net = ClosureNet(shape, seed=args.seed)
+    rng = np.random.default_rng(args.seed)
+    feats = torch.from_numpy(rng.normal(size=(cells, 32)).astype(np.float32)).to(torch.bfloat16).cuda()
+    feats[:, 8:] = 0
+    features = feats.view(torch.int16)
+    target = torch.zeros(cells, dtype=torch.float32, device="cuda")
+    weight = torch.full((cells,), 1.0 / cells, dtype=torch.float32, device="cuda")
+    net.forward(features)
+    net.backward(target, weight)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    events = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    fwd = bwd = 0.0
+    for _ in range(args.closure_repeats):
+        events[0].record(stream)
+        net.forward(features)
+        events[1].record(stream)
+        net.backward(target, weight)
+        events[2].record(stream)
+        events[2].synchronize()
+        fwd += events[0].elapsed_time(events[1])
+        bwd += events[1].elapsed_time(events[2])
+    fwd /= args.closure_repeats
+    bwd /= args.closure_repeats
+    conv = 2.0 * 27 * 32 * 32 * cells  # executed MFMA FLOPs of one 32 -> 32 convolution
+    fwd_flops, bwd_flops = 4 * conv, 7 * conv  # 4 forward; 4 weight + 3 data gradients
+    algorithmic = 2.0 * 27 * cells * (8 * 32 + 32 * 32 + 32 * 32 + 32 * 1)
+    print(f"[bench] closure 128x64x64: forward {fwd:.3f} ms ({fwd_flops / fwd / 1e9:.1f} TFLOP/s), backward "
+          f"{bwd:.3f} ms ({bwd_flops / bwd / 1e9:.1f} TFLOP/s)", file=sys.stderr, flush=True)
+    return {
+        "workload": "closure CNN, 4 x (3x3x3 conv, 32 channels), BF16 MFMA / FP32 accumulate, 128x64x64 coarse grid",
+        "forward_ms": round(fwd, 4), "backward_ms": round(bwd, 4),
+        "forward_tflops": round(fwd_flops / fwd / 1e9, 1), "backward_tflops": round(bwd_flops / bwd / 1e9, 1),
+        "forward_frac_of_bf16_peak": round(fwd_flops / fwd / 1e9 / BF16_MFMA_PEAK_TFLOPS, 4),
+        "forward_algorithmic_tflops": round(algorithmic / fwd / 1e9, 1),
+        "note": "executed FLOPs count the padded channels (8 real input channels padded to 32, 1 output to 32)",
+    }
 
 
 ENSEMBLE_STORAGE = ("fp32", "fp16-momentum", "fp16")
@@ -362,6 +420,8 @@ def main():
         torch.cuda.empty_cache()
     if args.ensemble_steps > 0:
         result["ensemble"] = ensemble_measurement(args)
+    if args.closure_repeats > 0:
+        result["closure"] = closure_measurement(args)
     if args.copy_probe_mib > 0:
         src = torch.ones(args.copy_probe_mib * (1 << 18), dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
