@@ -129,6 +129,10 @@ int icy2d_measure_outflow(const icy2d_grid* g, void* stream);   /* :2303-2338 */
 int icy2d_clip_phase(const icy2d_grid* g, double max_bulk_l1, void* stream);
 int icy2d_seed_projection(const icy2d_grid* g, void* stream);
 int icy2d_dilate_projection(const icy2d_grid* g, int primary_to_next, void* stream);
+/* All `radius` sweeps of icy2d_dilate_projection (alternating planes from
+ * primary_to_next = 1) in one launch; radius even and <= 16; the band ends
+ * in the primary plane, as after the sweeps. */
+int icy2d_dilate_band(const icy2d_grid* g, int radius, void* stream);
 int icy2d_evaluate_projection(const icy2d_grid* g, double lambda, void* stream);
 int icy2d_apply_projection(const icy2d_grid* g, double lambda, void* stream);
 int icy2d_projection_residual_weight(const icy2d_grid* g, double residual, void* stream);
@@ -178,7 +182,7 @@ void icy2d_abi_sizes(int32_t* grid, int32_t* control);
  * Blocks of cases with active[c] == 0 return at once (active may be NULL).
  * The per-case results equal the single-case entry points' bit for bit
  * when collide_threads and stream_threads are 256. */
-#define ICY2D_SHAPES 8 /* cells, body, x faces, y faces, one, totals, cells/128, cells/512 */
+#define ICY2D_SHAPES 9 /* cells, body, x faces, y faces, one, totals, cells/128, cells/512, band tiles */
 typedef struct icy2d_batch {
   int32_t ncases;
   int32_t collide_threads;      /* 128, 256 or 512 (chosen by the autotuner) */
@@ -211,6 +215,7 @@ int icy2d_batch_measure_outflow(const icy2d_batch* b, void* stream);
 int icy2d_batch_clip_phase(const icy2d_batch* b, double max_bulk_l1, void* stream);
 int icy2d_batch_seed_projection(const icy2d_batch* b, void* stream);
 int icy2d_batch_dilate_projection(const icy2d_batch* b, int primary_to_next, void* stream);
+int icy2d_batch_dilate_band(const icy2d_batch* b, int radius, void* stream);
 int icy2d_batch_evaluate_projection(const icy2d_batch* b, const double* lambda, void* stream);
 int icy2d_batch_apply_projection(const icy2d_batch* b, const double* lambda, void* stream);
 int icy2d_batch_projection_residual_weight(const icy2d_batch* b, const double* residual, void* stream);

@@ -65,7 +65,7 @@ CALLS = {
     "initialize_fluid": [_i, _i], "hydrostatic_equilibrium": [], "initialize_water_volume": [],
     "recover_initial_moments": [], "initialize_body_state": [_d], "initialize_water_state": [_d],
     "collide": [_i], "prepare_cut_links": [], "stream": [_i, _i], "update_pressure": [], "refill": [],
-    "measure_outflow": [], "clip_phase": [_d], "seed_projection": [], "dilate_projection": [_i],
+    "measure_outflow": [], "clip_phase": [_d], "seed_projection": [], "dilate_projection": [_i], "dilate_band": [_i],
     "evaluate_projection": [_d], "apply_projection": [_d], "projection_residual_weight": [_d],
     "apply_projection_residual": [_d, _d], "snapshot_melt_momentum": [], "reduce_provisional_momentum": [],
     "melt_weight_geometry": [], "apply_melt_correction": [], "reduce_final_momentum": [],
@@ -76,7 +76,7 @@ CALLS = {
     "apply_fallback": [_i], "reduce_totals": [],
 }
 
-SHAPES = 8  # ICY2D_SHAPES
+SHAPES = 9  # ICY2D_SHAPES
 
 
 class Batch2D(ctypes.Structure):

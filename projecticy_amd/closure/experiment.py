@@ -1,0 +1,134 @@
+"""Train and score the closure on paired coarse / 2x-fine solver runs.
+
+    python -m projecticy_amd.closure.experiment --pairs 200 --held-out 8 \
+        --coarse 128 64 64 --out closure.json
+
+Makes ``pairs`` seeded training pairs and ``held_out`` further pairs, trains
+the network on the training windows with the project's kernels, and scores
+it on the held-out pairs: the held-out loss against the plain method's, and
+the melted ice mass and body-centroid trajectories of the coarse run with
+and without the closure against the fine run (physical units).
+"""
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+import torch
+
+from .data import ClosureCoupling, closure_pair, ice_mass_kg, record_pair
+from .network import ClosureNet
+from .train import baseline_loss, mean_loss, train
+
+
+def _centres_m(sim):
+    return sim.body_center.to_numpy().astype(np.float64) * sim.scales.dx_m
+
+
+def trajectories(net, seed, windows, window_steps, pair_options):
+    """Per window: melted mass and body centres of the fine, plain coarse and
+    closure coarse runs."""
+    from ..iceflow3d import IceFlow3D
+
+    coarse_cfg, fine_cfg = closure_pair(seed, **pair_options)
+    runs = {"fine": IceFlow3D(fine_cfg), "plain": IceFlow3D(coarse_cfg), "closure": IceFlow3D(coarse_cfg)}
+    coupling = ClosureCoupling(runs["closure"], net, window_steps=window_steps)
+    start = {k: ice_mass_kg(v) for k, v in runs.items()}
+    out = {k: {"melted_kg": [], "centres_m": []} for k in runs}
+    for _ in range(windows):
+        runs["fine"].step(2 * window_steps)
+        runs["plain"].step(window_steps)
+        coupling.run(1)
+        for k, sim in runs.items():
+            out[k]["melted_kg"].append(start[k] - ice_mass_kg(sim))
+            out[k]["centres_m"].append(_centres_m(sim).tolist())
+    coupling.detach()
+    del runs
+    torch.cuda.empty_cache()
+    return out
+
+
+def score(traj):
+    fine_melt = np.array(traj["fine"]["melted_kg"])
+    fine_c = np.array(traj["fine"]["centres_m"])
+    result = {}
+    for k in ("plain", "closure"):
+        melt = np.array(traj[k]["melted_kg"])
+        cen = np.array(traj[k]["centres_m"])
+        result[k] = {
+            "final_melt_error": float(abs(melt[-1] - fine_melt[-1]) / max(abs(fine_melt[-1]), 1e-30)),
+            "mean_melt_error": float(np.mean(np.abs(melt - fine_melt) / np.maximum(np.abs(fine_melt), 1e-30))),
+            "max_centroid_error_m": float(np.max(np.linalg.norm(cen - fine_c, axis=-1))),
+        }
+    return result
+
+
+def main(argv=None):
+    p = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    p.add_argument("--pairs", type=int, default=200)
+    p.add_argument("--held-out", type=int, default=8)
+    p.add_argument("--windows", type=int, default=4)
+    p.add_argument("--window-steps", type=int, default=16)
+    p.add_argument("--coarse", type=int, nargs=3, default=(128, 64, 64))
+    p.add_argument("--cube", type=int, default=8, help="coarse cube edge in cells")
+    p.add_argument("--epochs", type=int, default=20)
+    p.add_argument("--lr", type=float, default=2e-3)
+    p.add_argument("--seed", type=int, default=2026)
+    p.add_argument("--out", default=None)
+    args = p.parse_args(argv)
+    options = dict(coarse_shape=tuple(args.coarse), cube_cells=args.cube)
+    train_seeds = [args.seed + k for k in range(args.pairs)]
+    held_seeds = [args.seed + 100000 + k for k in range(args.held_out)]
+    t0 = time.perf_counter()
+    samples = []
+    for k, seed in enumerate(train_seeds):
+        coarse, fine = closure_pair(seed, **options)
+        pair_samples, _ = record_pair(coarse, fine, windows=args.windows, window_steps=args.window_steps)
+        samples.extend(pair_samples)
+        torch.cuda.empty_cache()
+        if (k + 1) % 10 == 0 or k + 1 == len(train_seeds):
+            print(f"[closure] {k + 1}/{len(train_seeds)} training pairs, {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+    held = []
+    for seed in held_seeds:
+        coarse, fine = closure_pair(seed, **options)
+        held.extend(record_pair(coarse, fine, windows=args.windows, window_steps=args.window_steps)[0])
+    data_s = time.perf_counter() - t0
+    net = ClosureNet(tuple(args.coarse), seed=args.seed)
+    plain_loss = baseline_loss(held)
+    t1 = time.perf_counter()
+    curve = []
+    for epoch in range(args.epochs):
+        curve.extend(train(net, samples, epochs=1, lr=args.lr, seed=args.seed + epoch))
+        print(f"[closure] epoch {epoch + 1}/{args.epochs}: training loss {curve[-1]:.4e}", file=sys.stderr,
+              flush=True)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t1
+    held_loss = mean_loss(net, held)
+    scores = []
+    for seed in held_seeds:
+        scores.append(score(trajectories(net, seed, args.windows, args.window_steps, options)))
+    summary = {
+        k: {m: float(np.mean([s[k][m] for s in scores])) for m in scores[0][k]} for k in ("plain", "closure")
+    }
+    result = {
+        "coarse": list(args.coarse), "fine": [2 * v for v in args.coarse], "training_pairs": args.pairs,
+        "held_out_pairs": args.held_out, "windows": args.windows, "window_coarse_steps": args.window_steps,
+        "samples": len(samples), "epochs": args.epochs, "data_seconds": round(data_s, 1),
+        "training_seconds": round(train_s, 1), "training_curve": curve,
+        "held_out_loss": {"plain": plain_loss, "closure": held_loss},
+        "held_out_mean": summary, "held_out_per_pair": scores,
+    }
+    text = json.dumps(result)
+    if args.out:
+        with open(args.out, "w") as f:
+            f.write(text)
+    print(json.dumps({k: result[k] for k in ("held_out_loss", "held_out_mean", "samples", "data_seconds",
+                                              "training_seconds")}), flush=True)
+    return result
+
+
+if __name__ == "__main__":
+    main()

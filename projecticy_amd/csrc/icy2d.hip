@@ -1076,6 +1076,80 @@ __device__ __forceinline__ void k2_dilate_projection(const icy2d_grid& g, const 
   }
   pop_set<kPhase>(g.hbuf, g, dst + n, (float)value);
 }
+// All R dilation sweeps of simulator.py:4259-4289 in one pass: a block stages
+// a (16 + 2R) x (64 + 2R) window of the seed flags and the sweep mask
+// (fluid, cutoff < phi < 1 - cutoff) in LDS, runs the R sweeps there
+// (new = mask & (old | any of the 8 neighbours)), and writes the exact
+// central 16 x 64 cells: after R sweeps a cell depends only on flags within
+// R cells.  Blocks read the seeds from the primary plane and write the band
+// to the second plane (no block reads what another writes); k2_copy_band then
+// moves it to the primary plane, where the reference's ping-pong (R even)
+// ends.
+constexpr int kBandI = 16, kBandJ = 64, kMaxBand = 16;
+constexpr int kBandRows = kBandI + 2 * kMaxBand, kBandWords = (kBandJ + 2 * kMaxBand + 31) / 32;
+// Rows of the window are bit sets (bit b of word b / 32 = column b): one
+// sweep is new = mask & dilate3x3(old), the 3x3 dilation a horizontal
+// shift-or followed by an or of the row above and below.
+__device__ __forceinline__ void k2_dilate_band(const icy2d_grid& g, const int blk, int radius) {
+  __shared__ uint32_t flag[kBandRows][kBandWords], sweep[kBandRows][kBandWords], horiz[kBandRows][kBandWords];
+  const int tiles_j = (g.ny + kBandJ - 1) / kBandJ;
+  const int i0 = (blk / tiles_j) * kBandI - radius, j0 = (blk % tiles_j) * kBandJ - radius;
+  const int H = kBandI + 2 * radius, W = kBandJ + 2 * radius, words = (W + 31) / 32;
+  for (int e = threadIdx.x; e < kBandRows * kBandWords; e += blockDim.x) {
+    (&flag[0][0])[e] = 0u;
+    (&sweep[0][0])[e] = 0u;
+  }
+  __syncthreads();
+  int any = 0;
+  for (int e = threadIdx.x; e < H * W; e += blockDim.x) {
+    const int a = e / W, b = e - a * W, i = i0 + a, j = j0 + b;
+    if (inside(g, i, j) && fluid_at(g, i, j)) {
+      const float phase = g.phi[at(g, i, j)];
+      if (g.cutoff < phase && phase < 1.0f - g.cutoff) {
+        const uint32_t bit = 1u << (b & 31);
+        atomicOr(&sweep[a][b >> 5], bit);
+        if (pop_get<kPhase>(g.hbuf, g, at(g, i, j)) == 1.0f) {
+          atomicOr(&flag[a][b >> 5], bit);
+          any = 1;
+        }
+      }
+    }
+  }
+  // A window without seeds leaves no band: all of its central cells are 0.
+  if (__syncthreads_or(any)) {
+    for (int it = 0; it < radius; ++it) {
+      for (int e = threadIdx.x; e < H * words; e += blockDim.x) {
+        const int a = e / words, w = e - a * words;
+        const uint32_t f = flag[a][w];
+        const uint32_t left = (f << 1) | (w > 0 ? flag[a][w - 1] >> 31 : 0u);
+        const uint32_t right = (f >> 1) | (w + 1 < words ? flag[a][w + 1] << 31 : 0u);
+        horiz[a][w] = f | left | right;
+      }
+      __syncthreads();
+      for (int e = threadIdx.x; e < H * words; e += blockDim.x) {
+        const int a = e / words, w = e - a * words;
+        uint32_t v = horiz[a][w];
+        if (a > 0) v |= horiz[a - 1][w];
+        if (a + 1 < H) v |= horiz[a + 1][w];
+        flag[a][w] = v & sweep[a][w];
+      }
+      __syncthreads();
+    }
+  } else {
+    for (int e = threadIdx.x; e < kBandRows * kBandWords; e += blockDim.x) (&flag[0][0])[e] = 0u;
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < kBandI * kBandJ; e += blockDim.x) {
+    const int a = e / kBandJ, b = e - a * kBandJ, i = i0 + radius + a, j = j0 + radius + b;
+    const int col = radius + b;
+    if (i < g.nx && j < g.ny)
+      pop_set<kPhase>(g.hbuf, g, cells(g) + at(g, i, j), (float)((flag[radius + a][col >> 5] >> (col & 31)) & 1u));
+  }
+}
+__device__ __forceinline__ void k2_copy_band(const icy2d_grid& g, const int blk) {
+  const int n = cell_index(blk), N = cells(g);
+  if (n < N) pop_set<kPhase>(g.hbuf, g, n, pop_get<kPhase>(g.hbuf, g, N + n));
+}
 // simulator.py:4291-4351
 __device__ __forceinline__ void k2_evaluate_projection(const icy2d_grid& g, const int blk, double lambda) {
   const double e = exp(lambda);
@@ -1878,7 +1952,7 @@ __device__ __forceinline__ void k2_totals_partials(const icy2d_grid& g, const in
 // kernel body above takes (grid, block index within its case); one_case
 // runs it for one grid passed by value, many_cases for the case that owns
 // the block in a batch's prefix table.
-enum Shape : int { kCells = 0, kBody, kFaceX, kFaceY, kOne, kTotals, kCells128, kCells512, kShapes };
+enum Shape : int { kCells = 0, kBody, kFaceX, kFaceY, kOne, kTotals, kCells128, kCells512, kBandTiles, kShapes };
 static_assert(kShapes == ICY2D_SHAPES, "shape count");
 
 long shape_count(const icy2d_grid& g, int shape) {
@@ -1889,6 +1963,8 @@ long shape_count(const icy2d_grid& g, int shape) {
     case kFaceY: return (long)g.nx * (g.ny + 1);
     case kOne: return 1;
     case kTotals: return cells > body ? cells : body;
+    case kBandTiles:  // one block of kBlock threads per 16 x 64 band tile
+      return (long)((g.nx + kBandI - 1) / kBandI) * ((g.ny + kBandJ - 1) / kBandJ) * kBlock;
     default: return cells;
   }
 }
@@ -2073,6 +2149,12 @@ template <class D> int op_clip_phase(const D& d, double max_bulk_l1) {
 template <class D> int op_seed_projection(const D& d) { return d.template launch<k2_seed_projection>(kCells); }
 template <class D> int op_dilate_projection(const D& d, int primary_to_next) {
   return d.template launch<k2_dilate_projection>(kCells, primary_to_next);
+}
+template <class D> int op_dilate_band(const D& d, int radius) {
+  if (radius < 0 || radius > kMaxBand || radius % 2) return ICY2D_ERR_ARGUMENT;
+  if (radius == 0) return 0;
+  TRY(d.template launch<k2_dilate_band>(kBandTiles, radius));
+  return d.template launch<k2_copy_band>(kCells);
 }
 template <class D> int op_evaluate_projection(const D& d, typename D::Real lambda) {
   NEED(d, kCells, 2);
@@ -2269,6 +2351,7 @@ BOTH(measure_outflow, ())
 BOTH(clip_phase, (, double max_bulk_l1), , max_bulk_l1)
 BOTH(seed_projection, ())
 BOTH(dilate_projection, (, int primary_to_next), , primary_to_next)
+BOTH(dilate_band, (, int radius), , radius)
 BOTH(snapshot_melt_momentum, ())
 BOTH(reduce_provisional_momentum, ())
 BOTH(melt_weight_geometry, ())

@@ -924,19 +924,30 @@ constexpr int kForceJ = 4, kForceK = 64, kForceX = 16;
 constexpr int kForceHJ = kForceJ + 2, kForceHK = kForceK + 2, kForceSlab = kForceHJ * kForceHK;
 constexpr int kForceRounds = (kForceSlab + kForceJ * kForceK - 1) / (kForceJ * kForceK);
 // Staged per cell: Ax, Ay, Az, Bxy, Bxz, Byz, X1, X2 (stencil sums), then
-// u (3), phi and the mass flux (3) for the face-neighbour terms.
-constexpr int kForceQ = 15;
-constexpr int kStencilQ = 8, kU = 8, kPhi = 11, kFlux = 12;
+// u (3), phi and the mass flux (3) for the face-neighbour terms, and a wall
+// marker.
+constexpr int kForceQ = 16;
+constexpr int kStencilQ = 8, kU = 8, kPhi = 11, kFlux = 12, kWall = 15;
 
-// Raw values prefetched for one staged cell: grad (3), phi, pressure field, u (3), flux (3).
+// Raw values prefetched for one staged cell: grad (3), phi, pressure field,
+// u (3), flux (3).  A position inside the tank walls is fetched from its
+// mirror, the nearest cell of the fluid box (simulator.py:418-519, tank
+// links), with `wall` holding the mirrored axes; staging then turns it into
+// what the tank formulas use: the gradient with the mirrored components
+// negated (capillary), no pressure term, the velocity -u of the mirror cell
+// (halfway no-slip), and no donor flux.
 struct ForceFetch {
   float grad[3], phi, p, u[3], flux[3];
+  int wall;
 };
 __device__ __forceinline__ void fetch_force_cell(const icy3d_grid& g, int i, int j, int k, int moment_pressure,
                                                  ForceFetch& f) {
-  i = min(max(i, 0), g.nx - 1);
-  j = min(max(j, 0), g.ny - 1);
-  k = min(max(k, 0), g.nz - 1);
+  const int b = g.boundary;
+  const int ci = min(max(i, b), g.nx - b - 1), cj = min(max(j, b), g.ny - b - 1), ck = min(max(k, b), g.nz - b - 1);
+  f.wall = (ci != i) | ((cj != j) << 1) | ((ck != k) << 2);
+  i = ci;
+  j = cj;
+  k = ck;
   const size_t N = cells(g);
   const int n = lin(g, i, j, k);
 #pragma unroll
@@ -950,7 +961,9 @@ __device__ __forceinline__ void fetch_force_cell(const icy3d_grid& g, int i, int
 }
 __device__ __forceinline__ void stage_force_cell(const icy3d_grid& g, float (*sq)[kForceSlab], int t, int moment_pressure,
                                                  const ForceFetch& f) {
-  const float gx = f.grad[0], gy = f.grad[1], gz = f.grad[2];
+  const float gx = (f.wall & 1) ? -f.grad[0] : f.grad[0];
+  const float gy = (f.wall & 2) ? -f.grad[1] : f.grad[1];
+  const float gz = (f.wall & 4) ? -f.grad[2] : f.grad[2];
   sq[0][t] = gy * gy + gz * gz;
   sq[1][t] = gx * gx + gz * gz;
   sq[2][t] = gx * gx + gy * gy;
@@ -958,7 +971,10 @@ __device__ __forceinline__ void stage_force_cell(const icy3d_grid& g, float (*sq
   sq[4][t] = gx * gz;
   sq[5][t] = gy * gz;
   const float rho = density(g, f.phi);
-  if (moment_pressure) {
+  if (f.wall) {
+    sq[6][t] = 0.0f;
+    sq[7][t] = 0.0f;
+  } else if (moment_pressure) {
     sq[6][t] = f.p;
     sq[7][t] = f.p * rho;
   } else {
@@ -967,10 +983,11 @@ __device__ __forceinline__ void stage_force_cell(const icy3d_grid& g, float (*sq
   }
 #pragma unroll
   for (int e = 0; e < 3; ++e) {
-    sq[kU + e][t] = f.u[e];
+    sq[kU + e][t] = f.wall ? -f.u[e] : f.u[e];
     sq[kFlux + e][t] = f.flux[e];
   }
   sq[kPhi][t] = f.phi;
+  sq[kWall][t] = f.wall ? 1.0f : 0.0f;
 }
 struct SlabParts {
   float px[5], qy[5], qz[5];  // D_x parts of Ax Bxy Bxz X1 X2; D_y of Ay Bxy Byz X1 X2; D_z of Az Bxz Byz X1 X2
@@ -1018,7 +1035,7 @@ __device__ __forceinline__ SlabParts slab_parts(const float (*sq)[kForceSlab], i
 }
 // Own-column values carried between slabs.
 struct ColumnCell {
-  float u[3], flux_x, phi;
+  float u[3], flux_x, phi, wall;
 };
 // Terms of the y and z faces, formed while the cell's own slab is in LDS:
 // the donor-flux force of those four faces and the y/z derivatives of u.
@@ -1061,7 +1078,7 @@ __device__ __forceinline__ FaceTerms face_terms(const icy3d_grid& g, const float
       const int m = centre + off[2 * a + side];
 #pragma unroll
       for (int e = 0; e < 3; ++e) un[side][e] = sq[kU + e][m];
-      if (with_flux)
+      if (with_flux && sq[kWall][m] == 0.0f)
         face_flux(g, side == 0 ? -1.0f : 1.0f, flux_own, sq[kFlux + comp][m], rho, sq[kPhi][m], u_own, un[side], t.f);
     }
 #pragma unroll
@@ -1116,11 +1133,13 @@ __global__ void __launch_bounds__(kForceJ* kForceK) k_forces_tiled(icy3d_grid g,
     for (int e = 0; e < 3; ++e) col_p1.u[e] = sq[kU + e][centre];
     col_p1.flux_x = sq[kFlux][centre];
     col_p1.phi = sq[kPhi][centre];
+    col_p1.wall = sq[kWall][centre];
     const FaceTerms faces_p1 = face_terms(g, sq, centre, with_flux);
     const int x = s - 1;
     if (x >= x0 && mine) {
       const int n = lin(g, x, j, k);
-      if (g.flag[n] == 0) {
+      const int8_t flag = g.flag[n];
+      if (flag == 0 || flag == 1) {  // bulk and tank-adjacent cells
         const float sixth = 1.0f / 6.0f;
         const float phi = col_0.phi, rho = density(g, phi);
         float f[3] = {faces_0.f[0], (rho - g.ref_density[j]) * g.gravity_y + faces_0.f[1], faces_0.f[2]};
@@ -1139,8 +1158,8 @@ __global__ void __launch_bounds__(kForceJ* kForceK) k_forces_tiled(icy3d_grid g,
           f[2] += kcap * (dz[0] - dx[2] - dy[2]);
         }
         if (with_flux) {
-          face_flux(g, -1.0f, col_0.flux_x, col_m1.flux_x, rho, col_m1.phi, col_0.u, col_m1.u, f);
-          face_flux(g, 1.0f, col_0.flux_x, col_p1.flux_x, rho, col_p1.phi, col_0.u, col_p1.u, f);
+          if (col_m1.wall == 0.0f) face_flux(g, -1.0f, col_0.flux_x, col_m1.flux_x, rho, col_m1.phi, col_0.u, col_m1.u, f);
+          if (col_p1.wall == 0.0f) face_flux(g, 1.0f, col_0.flux_x, col_p1.flux_x, rho, col_p1.phi, col_0.u, col_p1.u, f);
         }
         // home.py:127-197
         const float inv_rho = 1.0f / rho;
@@ -1854,8 +1873,7 @@ int launch_gradients(const icy3d_grid& g, const Box& b, bool, hipStream_t s) {
 }
 int launch_forces(const icy3d_grid& g, int with_flux, int moment_pressure, hipStream_t s) {
   const dim3 tiles((g.nz + kForceK - 1) / kForceK, (g.ny + kForceJ - 1) / kForceJ, (g.nx + kForceX - 1) / kForceX);
-  k_forces_tiled<<<tiles, kForceJ * kForceK, 0, s>>>(g, with_flux, moment_pressure);
-  k_forces_tank<<<kListBlocks, kBlock, 0, s>>>(g, with_flux, moment_pressure);
+  k_forces_tiled<<<tiles, kForceJ * kForceK, 0, s>>>(g, with_flux, moment_pressure);  // bulk and tank cells
   k_forces_general<<<kListBlocks, kBlock, 0, s>>>(g, with_flux, moment_pressure);
   ICY_LAUNCH_CHECK();
   return 0;

@@ -30,9 +30,10 @@ import numpy as np
 import torch
 
 from .. import _native2d as native
-from .simulator import MAX_BULK_PHASE_L1, IceFlow2D, _require_device
+from .simulator import MAX_BULK_PHASE_L1, MAX_FUSED_BAND, IceFlow2D, _require_device
 
 THREAD_CHOICES = (128, 256, 512)
+CONTROL_DTYPE = np.dtype(native.Control2D)
 
 
 class IceFlowEnsemble2D:
@@ -148,15 +149,18 @@ class IceFlowEnsemble2D:
         return self._masks[k].data_ptr()
 
     def _read_controls(self):
-        """All control blocks with one copy; raises a pending device-side check."""
+        """All control blocks with one copy (a record array, one record per
+        case, fields named as in icy2d_control); raises a pending
+        device-side check."""
         with torch.cuda.stream(self._stream):
             self._controls_host.copy_(self._controls, non_blocking=True)
         self._stream.synchronize()
         self._value_slot = self._mask_slot = 0
         raw = self._controls_host.numpy()
-        controls = [native.Control2D.from_buffer_copy(raw[i].tobytes()) for i in range(self.ncases)]
-        for i, control in enumerate(controls):
-            if control.error_flags:
+        controls = raw.view(CONTROL_DTYPE).reshape(self.ncases).copy().view(np.recarray)
+        if controls.error_flags.any():
+            for i in np.flatnonzero(controls.error_flags):
+                control = native.Control2D.from_buffer_copy(raw[i].tobytes())
                 try:
                     with torch.cuda.stream(self._stream):
                         self.cases[i]._raise_aperture_error(control)
@@ -240,8 +244,7 @@ class IceFlowEnsemble2D:
     def _advance_thermal_fast(self, controls):
         """simulator.py:2233-2247 and :418-453 per case."""
         counts, subs = [], []
-        for i, (case, control) in enumerate(zip(self.cases, controls)):
-            speed = float(control.max_outflow)
+        for i, (case, speed) in enumerate(zip(self.cases, controls.max_outflow.tolist())):
             if not speed >= 0.0:
                 raise ValueError(f"ensemble case {i}: maximum_outflow_rate_lattice must be finite")
             th, dt = case.config.thermal, case._time_step_s
@@ -310,18 +313,19 @@ class IceFlowEnsemble2D:
         """simulator.py:1542-1563 per case."""
         controls = self._read_controls()
         live = [1 if active is None else active[i] for i in range(self.ncases)]
-        need = [int(live[i] and controls[i].unassigned[0] > 0.0) for i in range(self.ncases)]
+        unassigned = controls.unassigned[:, 0].tolist()
+        need = [int(live[i] and unassigned[i] > 0.0) for i in range(self.ncases)]
         if not any(need):
             return
         self._call("measure_fallback", active=need)
-        controls = self._read_controls()
+        fallback = self._read_controls().fallback.tolist()
         free, wet = [0] * self.ncases, [0] * self.ncases
         for i in range(self.ncases):
             if not need[i]:
                 continue
-            if controls[i].fallback[0] > 1.0e-30:
+            if fallback[i][0] > 1.0e-30:
                 free[i] = 1
-            elif controls[i].fallback[1] > 1.0e-30:
+            elif fallback[i][1] > 1.0e-30:
                 wet[i] = 1
             else:
                 raise RuntimeError(f"ensemble case {i}: melt water has no legal LBM water cell for conservative "
@@ -341,10 +345,9 @@ class IceFlowEnsemble2D:
         self._call("melt_weight_geometry", active=mask)
         controls = self._read_controls()
         apply = [0] * self.ncases
-        for i, control in enumerate(controls):
+        for i, status in enumerate(controls.melt_status.tolist()):
             if not pending[i]:
                 continue
-            status = int(control.melt_status)
             if status == -1:
                 raise RuntimeError(f"ensemble case {i}: melt momentum has no legal wet LBM cell for conservative "
                                    "linear/angular-momentum closure")
@@ -367,9 +370,10 @@ class IceFlowEnsemble2D:
         self._call("clip_phase", MAX_BULK_PHASE_L1)
         controls = self._read_controls()
         target, tolerance, need = [0.0] * n, [0.0] * n, [0] * n
-        for i, (case, control) in enumerate(zip(self.cases, controls)):
-            current = control.water[1]
-            target[i] = control.water[0] + case._initial_body_mass_lattice - control.mass
+        water, mass = controls.water.tolist(), controls.mass.tolist()
+        for i, case in enumerate(self.cases):
+            current = water[i][1]
+            target[i] = water[i][0] + case._initial_body_mass_lattice - mass[i]
             tolerance[i] = case.config.volume_projection_tolerance * max(1.0, abs(target[i]))
             need[i] = int(abs(target[i] - current) > tolerance[i])
         if not any(need):
@@ -377,15 +381,20 @@ class IceFlowEnsemble2D:
         mask = None if all(need) else need
         self._call("seed_projection", active=mask)
         radii = [case._volume_projection_band_radius for case in self.cases]
-        for k in range(max(radii[i] for i in range(n) if need[i])):
-            live = [int(need[i] and radii[i] > k) for i in range(n)]
-            self._call("dilate_projection", 1 if k % 2 == 0 else 0, active=None if all(live) else live)
+        for radius in sorted({radii[i] for i in range(n) if need[i]}):
+            group = [int(need[i] and radii[i] == radius) for i in range(n)]
+            live = None if all(group) else group
+            if radius <= MAX_FUSED_BAND:
+                self._call("dilate_band", radius, active=live)
+            else:
+                for k in range(radius):
+                    self._call("dilate_projection", 1 if k % 2 == 0 else 0, active=live)
         limit = [4.0 * case.config.volume_projection_max_shift / case.config.interface_width for case in self.cases]
 
         def evaluate(lams, live):
             self._call("evaluate_projection", self._upload_values(lams), active=None if all(live) else live)
             controls = self._read_controls()
-            return [c.water[1] for c in controls], [c.water[2] for c in controls]
+            return controls.water[:, 1].tolist(), controls.water[:, 2].tolist()
 
         lower = evaluate([-v for v in limit], need)[0]
         upper = evaluate(limit, need)[0]
@@ -398,7 +407,7 @@ class IceFlowEnsemble2D:
             if target[i] < lower[i] - tolerance[i] or target[i] > upper[i] + tolerance[i]:
                 raise RuntimeError(f"ensemble case {i}: water-volume projection exceeds the permitted interface "
                                    f"shift: target={target[i]:.17g}, reachable=[{lower[i]:.17g}, {upper[i]:.17g}]")
-        initial = [abs(target[i] - controls[i].water[1]) for i in range(n)]
+        initial = [abs(target[i] - water[i][1]) for i in range(n)]
         lo, hi = [-v for v in limit], list(limit)
         lam, best_lam, best = [0.0] * n, [0.0] * n, initial
         searching = list(need)
@@ -429,7 +438,7 @@ class IceFlowEnsemble2D:
                     proposal = 0.5 * (lo[i] + hi[i])
                 lam[i] = proposal
         self._call("apply_projection", self._upload_values(best_lam), active=mask)
-        current = [c.water[1] for c in self._read_controls()]
+        current = self._read_controls().water[:, 1].tolist()
         closing = list(need)
         for _ in range(8):
             residual = [target[i] - current[i] for i in range(n)]
@@ -439,7 +448,7 @@ class IceFlowEnsemble2D:
             live = None if all(closing) else closing
             res_ptr = self._upload_values(residual)
             self._call("projection_residual_weight", res_ptr, active=live)
-            weight = [c.water[2] for c in self._read_controls()]
+            weight = self._read_controls().water[:, 2].tolist()
             for i in range(n):
                 if closing[i] and (not math.isfinite(weight[i]) or weight[i] <= 0.0):
                     closing[i] = 0
@@ -449,7 +458,8 @@ class IceFlowEnsemble2D:
             self._call("apply_projection_residual", self._upload_values(residual), self._upload_values(weight),
                        active=live)
             controls = self._read_controls()
-            current = [controls[i].water[1] if closing[i] else current[i] for i in range(n)]
+            latest = controls.water[:, 1].tolist()
+            current = [latest[i] if closing[i] else current[i] for i in range(n)]
         for i in range(n):
             if need[i]:
                 error = abs(target[i] - current[i])

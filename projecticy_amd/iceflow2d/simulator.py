@@ -24,6 +24,7 @@ POPULATION_SCALE = 1024.0         # FP16 population storage holds population * 2
 # storage name -> bit mask of FP16 buffers (bit 0 momentum f/fbuf, bit 1 phase h/hbuf)
 POPULATION_STORAGE = {"fp32": 0, "fp16": 3, "fp16-momentum": 1}
 MAX_LATTICE_VELOCITY_L1 = 1.0    # simulator.py:44
+MAX_FUSED_BAND = 16              # widest band icy2d_dilate_band builds in one launch
 
 
 def _require_device():
@@ -756,8 +757,12 @@ class IceFlow2D:
         if abs(initial_error) <= tolerance:
             return
         self._call("seed_projection")
-        for k in range(self._volume_projection_band_radius):
-            self._call("dilate_projection", 1 if k % 2 == 0 else 0)
+        radius = self._volume_projection_band_radius
+        if radius <= MAX_FUSED_BAND:
+            self._call("dilate_band", radius)
+        else:
+            for k in range(radius):
+                self._call("dilate_projection", 1 if k % 2 == 0 else 0)
         limit = 4.0 * c.volume_projection_max_shift / c.interface_width
 
         def evaluate(lam):

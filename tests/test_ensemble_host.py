@@ -93,6 +93,7 @@ def test_batch_plan_prefix_tables(library):
         5: [_blocks(max(nx * ny, bx * by)) for nx, ny, bx, by in sizes],
         6: [_blocks(nx * ny, 128) for nx, ny, _, _ in sizes],
         7: [_blocks(nx * ny, 512) for nx, ny, _, _ in sizes],
+        8: [_blocks(nx, 16) * _blocks(ny, 64) for nx, ny, _, _ in sizes],
     }
     for shape, per_case in counts.items():
         table = list(first[shape * 4:(shape + 1) * 4])

@@ -91,3 +91,30 @@ def test_library_is_the_hip_build():
     lib = _native2d.lib()
     assert str(_native2d.LIB_PATH).endswith("libicy2d.so")
     assert hasattr(lib, "icy2d_collide")
+
+
+@pytest.mark.parametrize("radius", [2, 10, 16])
+def test_fused_band_equals_the_reference_sweeps(radius):
+    """icy2d_dilate_band(R) leaves the same primary-plane band as R sweeps of
+    icy2d_dilate_projection (simulator.py:4259-4289) from the same seeds."""
+    import torch
+
+    from projecticy_amd.iceflow2d import IceFlow2D
+    from projecticy_amd.iceflow2d.simulator import MAX_BULK_PHASE_L1
+
+    sim = IceFlow2D(default_run(20))
+    sim.step(3)
+    sim._call("clip_phase", MAX_BULK_PHASE_L1)
+    sim._call("seed_projection")
+    N = sim.nx * sim.ny
+    seeds = sim._t["hbuf"].clone()
+    assert float(seeds[:N].sum()) > 0
+    for k in range(radius):
+        sim._call("dilate_projection", 1 if k % 2 == 0 else 0)
+    swept = sim._t["hbuf"][:N].clone()
+    sim._t["hbuf"].copy_(seeds)
+    sim._call("dilate_band", radius)
+    torch.cuda.synchronize()
+    fused = sim._t["hbuf"][:N]
+    assert torch.equal(fused, swept)
+    assert float(fused.sum()) > float(seeds[:N].sum())
