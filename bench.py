@@ -55,7 +55,10 @@ STEP_BYTES = 161  # minimal persistent state per cell-update of the whole step
 # HBM bytes per launch of a kernel group, from the FETCH_SIZE (x2.0) and
 # WRITE_SIZE (x1.0) counters calibrated on a copy kernel of known size
 # (rocprofv3 --pmc runs of this script; the summary is kept under profiles/).
-MEASURED_TRAFFIC_BYTES = {"forces": 1.031e9}
+MEASURED_TRAFFIC_BYTES = {
+    "phase_collide": 3.652e8, "forces": 6.284e8, "home_collide": 5.248e8,
+    "home_stream": 9.060e8, "phase_recover": 1.489e8, "gradients": 1.056e8,
+}
 
 
 def parse():
@@ -374,6 +377,8 @@ def main():
         mean_ms = ms / max(count, 1)
         gbs = KERNEL_BYTES[name] * cells / (mean_ms * 1e-3) / 1e9 if mean_ms > 0 else 0.0
         groups[name] = {"mean_ms": round(mean_ms, 5), "algorithmic_GBs": round(gbs, 1)}
+        if name in MEASURED_TRAFFIC_BYTES and mean_ms > 0:
+            groups[name]["hbm_GBs_counted"] = round(MEASURED_TRAFFIC_BYTES[name] / (mean_ms * 1e-3) / 1e9, 1)
     dominant = max(groups, key=lambda k: groups[k]["mean_ms"])
     achieved = groups[dominant]["algorithmic_GBs"]
     roofline = {

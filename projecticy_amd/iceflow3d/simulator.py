@@ -127,6 +127,8 @@ class IceFlow3D:
         self._control_host = torch.zeros(ctypes.sizeof(Control), dtype=torch.uint8).pin_memory()
         self._loads_host = torch.zeros(ctypes.sizeof(Control), dtype=torch.uint8).pin_memory()
         self._loads_ready = torch.cuda.Event()
+        self._upload_staging = torch.zeros((B, _native.STATE_STRIDE), dtype=torch.float32).pin_memory()
+        self._upload_done = torch.cuda.Event()
         g = Grid()
         g.nx, g.ny, g.nz = self.shape
         g.boundary, g.bodies = cfg.boundary_cells, B
@@ -308,10 +310,15 @@ class IceFlow3D:
         self._t["control"].copy_(torch.from_numpy(raw.copy()))
 
     def _upload_rigid_state(self, state):
-        """rigid.py:136-151 device side: one small host-to-device copy."""
+        """rigid.py:136-151 device side: one small host-to-device copy,
+        queued from pinned memory so that the host does not wait for the
+        kernels still ahead of it on the stream."""
         state = np.ascontiguousarray(state, dtype=np.float32)
         self._body_state[...] = state
-        self._t["body"].copy_(torch.from_numpy(state.copy()))
+        self._upload_done.synchronize()  # the previous upload has left the staging buffer
+        self._upload_staging.numpy()[...] = state
+        self._t["body"].copy_(self._upload_staging, non_blocking=True)
+        self._upload_done.record(torch.cuda.current_stream(self.device))
 
     def _reduce_mass_state(self):
         self._call("reduce_mass")
