@@ -87,8 +87,15 @@ class IceFlowEnsemble2D:
         for k in range(native.SHAPES):
             b.total_blocks[k] = totals[k]
         self._batch_ptr = ctypes.pointer(self._batch)
+        self._masked = native.Batch2D()
+        self._masked_ptr = ctypes.pointer(self._masked)
+        self._fns = {name: getattr(self._lib, "icy2d_batch_" + name) for name in native.BATCH_CALLS}
+        # per-case time step of a single advection substep (the common case), read by a graph
+        self._fast_dt = torch.zeros(n, dtype=torch.float64, device=self.device)
+        self._fast_dt_host = torch.zeros(n, dtype=torch.float64, pin_memory=True)
+        self._fast_dt_ptr = ctypes.c_void_p(self._fast_dt.data_ptr())
         self.use_graph = bool(use_graph)
-        self._graph = None
+        self._graphs = {}
         self.steps = 0
         self._pending = self.cases[0]._thermal_lbm_steps_pending
         self._interval = configs[0].thermal.update_interval_lbm_steps
@@ -98,10 +105,11 @@ class IceFlowEnsemble2D:
 
     # ---------------------------------------------------------- plumbing
     def _drop_graph(self):
-        graph = getattr(self, "_graph", None)
-        if graph is not None:
+        graphs = getattr(self, "_graphs", None) or {}
+        for graph in graphs.values():
             self._lib.icy2d_graph_destroy(graph)
-            self._graph = None
+        if graphs:
+            graphs.clear()
 
     @property
     def threads(self):
@@ -118,12 +126,35 @@ class IceFlowEnsemble2D:
         if active is None:
             batch = self._batch_ptr
         else:
-            masked = native.Batch2D.from_buffer_copy(self._batch)
-            masked.active = self._upload_mask(active)
-            batch = ctypes.byref(masked)
-        rc = getattr(self._lib, "icy2d_batch_" + name)(batch, *args, self._s)
+            ctypes.memmove(self._masked_ptr, self._batch_ptr, ctypes.sizeof(native.Batch2D))
+            self._masked.active = self._upload_mask(active)
+            batch = self._masked_ptr
+        rc = self._fns[name](batch, *args, self._s)
         if rc != 0:
             raise native.IcyError(rc, "icy2d_batch_" + name)
+
+    def _replay(self, name, enqueue):
+        """Run ``enqueue`` (launches only, no host reads), captured into a HIP
+        graph on first use and replayed afterwards."""
+        if not self.use_graph:
+            enqueue()
+            return
+        graph = self._graphs.get(name)
+        if graph is None:
+            rc = self._lib.icy2d_graph_begin(self._s)
+            if rc != 0:
+                raise native.IcyError(rc, "icy2d_graph_begin")
+            try:
+                enqueue()
+            finally:
+                graph = ctypes.c_void_p()
+                rc = self._lib.icy2d_graph_end(self._s, ctypes.byref(graph))
+            if rc != 0:
+                raise native.IcyError(rc, "icy2d_graph_end")
+            self._graphs[name] = graph
+        rc = self._lib.icy2d_graph_launch(graph, self._s)
+        if rc != 0:
+            raise native.IcyError(rc, "icy2d_graph_launch")
 
     def _upload_values(self, values):
         """Device pointer to one FP64 value per case (stream ordered)."""
@@ -182,24 +213,7 @@ class IceFlowEnsemble2D:
         self._call("measure_outflow")
 
     def _run_flow_segment(self):
-        if not self.use_graph:
-            self._flow_segment()
-            return
-        if self._graph is None:
-            rc = self._lib.icy2d_graph_begin(self._s)
-            if rc != 0:
-                raise native.IcyError(rc, "icy2d_graph_begin")
-            try:
-                self._flow_segment()
-            finally:
-                graph = ctypes.c_void_p()
-                rc = self._lib.icy2d_graph_end(self._s, ctypes.byref(graph))
-            if rc != 0:
-                raise native.IcyError(rc, "icy2d_graph_end")
-            self._graph = graph
-        rc = self._lib.icy2d_graph_launch(self._graph, self._s)
-        if rc != 0:
-            raise native.IcyError(rc, "icy2d_graph_launch")
+        self._replay("flow", self._flow_segment)
 
     def _contact_and_rasterize(self, *, resolve_contact, active=None):
         """simulator.py:4082-4123 (the projected re-pass is decided on the device)."""
@@ -233,7 +247,7 @@ class IceFlowEnsemble2D:
                 self._pending = 0
             self._correct_water_volume()
             self._finalize_melt_momentum()
-            self._synchronize_water_aperture()
+            self._replay("aperture", self._synchronize_water_aperture)
             self.steps += 1
             for case in self.cases:
                 case.steps += 1
@@ -256,6 +270,14 @@ class IceFlowEnsemble2D:
             count = case._substeps(load, "advection", dt, speed)
             counts.append(count)
             subs.append(dt / count)
+        if max(counts) == 1:
+            # every case takes one substep: the fixed sequence runs as a graph
+            # that reads the per-case time step from a persistent array
+            self._fast_dt_host.copy_(torch.as_tensor(subs, dtype=torch.float64))
+            with torch.cuda.stream(self._stream):
+                self._fast_dt.copy_(self._fast_dt_host, non_blocking=True)
+            self._replay("fast", self._one_substep)
+            return
         self._synchronize_water_aperture()
         sub = self._upload_values(subs)
         for k in range(max(counts)):
@@ -266,6 +288,13 @@ class IceFlowEnsemble2D:
             more = [int(c > k + 1) for c in counts]
             if any(more):
                 self._synchronize_water_aperture(active=None if all(more) else more)
+
+    def _one_substep(self):
+        """simulator.py:418-453 with one substep for every case."""
+        self._synchronize_water_aperture()
+        self._call("advection_flux", 0)
+        self._call("advection_flux", 1)
+        self._call("update_water", self._fast_dt_ptr, 1)
 
     def _advance_thermal_slow(self, lbm_steps):
         """simulator.py:2249-2286 and :455-509 per case."""
