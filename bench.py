@@ -18,8 +18,9 @@ The JSON line adds:
                 (FP32 populations, FP64 thermal), timed after the headline,
                 with its own CPU baseline (the 2D oracle on the 100x200 case);
   ensemble      64 uneven D2Q9 cases (widths 64-512) stepped as one batch
-                (one launch per kernel for all cases) with FP32, FP16-momentum
-                and FP16 population storage: cases/hour for 10,000-step cases,
+                (one launch per kernel for all cases) with FP32, FP16 momentum
+                (stochastic rounding) and FP16 (stochastic rounding) population
+                storage: cases/hour for 10,000-step cases,
                 the same figure for one-case-at-a-time FP32 runs, and each
                 storage's melted-mass error after 10,000 steps against the
                 FP32 standalone runs; ``value`` is the best cases/hour among
@@ -179,7 +180,7 @@ def closure_measurement(args):
     }
 
 
-ENSEMBLE_STORAGE = ("fp32", "fp16-momentum", "fp16")
+ENSEMBLE_STORAGE = ("fp32", "fp16-momentum-sr", "fp16-sr")
 
 
 def ensemble_measurement(args):

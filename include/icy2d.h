@@ -82,8 +82,9 @@ typedef struct icy2d_grid {
   int32_t partials_capacity;
   /* Population storage, a bit mask: bit 0 set = f and fbuf hold FP16
    * population * population_scale, bit 1 set = h and hbuf do (a power-of-two
-   * scale; FP32 arithmetic in registers); clear bits mean FP32.  No reference
-   * counterpart. */
+   * scale; FP32 arithmetic in registers); clear bits mean FP32; bit 2 rounds
+   * FP16 stores stochastically (hash of value and index) instead of to
+   * nearest.  No reference counterpart. */
   int32_t half_populations;
   float population_scale, population_inverse_scale;
   icy2d_control* control;

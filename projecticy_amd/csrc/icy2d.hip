@@ -54,12 +54,35 @@ __device__ __forceinline__ double clamp01d(double v) { return fmin(1.0, fmax(0.0
 // Population storage: FP32, or FP16 holding the population times
 // population_scale (a power of two, so the scaling is exact) with FP32
 // arithmetic in registers.  Bit 0 of half_populations selects FP16 for the
-// momentum populations (f, fbuf), bit 1 for the phase populations (h, hbuf).
+// momentum populations (f, fbuf), bit 1 for the phase populations (h, hbuf);
+// bit 2 rounds FP16 stores stochastically instead of to nearest, so that
+// updates smaller than half an FP16 step (the equilibrium lifts of the
+// volume projection and the melt-momentum closure) survive in expectation
+// instead of being dropped.  The random number is a hash of the stored
+// value's FP32 bits and its index: deterministic for a given state.
 // The Liang populations are small (O(u) and O(p)); the scale keeps them in
 // the FP16 normal range.
 template <int Kind>
 __device__ __forceinline__ bool half_storage(const icy2d_grid& g) {
   return (g.half_populations >> Kind) & 1;
+}
+__device__ __forceinline__ uint32_t mix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  h *= 0x846ca68bu;
+  h ^= h >> 16;
+  return h;
+}
+__device__ __forceinline__ __half half_stochastic(float x, uint32_t salt) {
+  const __half lo = __float2half_rz(x);
+  const float flo = __half2float(lo);
+  if (flo == x || !(fabsf(x) < 65504.0f)) return __float2half(x);
+  const __half hi = __ushort_as_half((unsigned short)(__half_as_ushort(lo) + 1));  // next step away from zero
+  const float fhi = __half2float(hi);
+  const float p = (x - flo) / (fhi - flo);
+  const float r = (float)(mix32(__float_as_uint(x) ^ mix32(salt)) >> 8) * (1.0f / 16777216.0f);
+  return r < p ? hi : lo;
 }
 template <int Kind>
 __device__ __forceinline__ float pop_get(const float* base, const icy2d_grid& g, int idx) {
@@ -69,7 +92,9 @@ __device__ __forceinline__ float pop_get(const float* base, const icy2d_grid& g,
 template <int Kind>
 __device__ __forceinline__ void pop_set(float* base, const icy2d_grid& g, int idx, float v) {
   if (half_storage<Kind>(g)) {
-    reinterpret_cast<__half*>(base)[idx] = __float2half(v * g.population_scale);
+    const float x = v * g.population_scale;
+    reinterpret_cast<__half*>(base)[idx] =
+        (g.half_populations & 4) ? half_stochastic(x, (uint32_t)idx * 2u + (uint32_t)Kind) : __float2half(x);
   } else {
     base[idx] = v;
   }

@@ -39,7 +39,7 @@ CONTROL_DTYPE = np.dtype(native.Control2D)
 class IceFlowEnsemble2D:
     """Many ``IceFlow2D`` cases advanced in lockstep, one launch per kernel."""
 
-    def __init__(self, configs, *, population_storage="fp16", population_scale=None, use_graph=True):
+    def __init__(self, configs, *, population_storage="fp16-momentum-sr", population_scale=None, use_graph=True):
         configs = list(configs)
         if not configs:
             raise ValueError("an ensemble needs at least one case")

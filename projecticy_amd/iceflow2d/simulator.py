@@ -22,7 +22,8 @@ from .config import BOUNDARY_CODE, IceFlowConfig, LatticeScales, MovingBodyTherm
 MAX_BULK_PHASE_L1 = 2.0          # simulator.py:45
 POPULATION_SCALE = 1024.0         # FP16 population storage holds population * 2^10
 # storage name -> bit mask of FP16 buffers (bit 0 momentum f/fbuf, bit 1 phase h/hbuf)
-POPULATION_STORAGE = {"fp32": 0, "fp16": 3, "fp16-momentum": 1}
+# (bit 2: stochastic rounding of FP16 stores)
+POPULATION_STORAGE = {"fp32": 0, "fp16": 3, "fp16-momentum": 1, "fp16-sr": 7, "fp16-momentum-sr": 5}
 MAX_LATTICE_VELOCITY_L1 = 1.0    # simulator.py:44
 MAX_FUSED_BAND = 16              # widest band icy2d_dilate_band builds in one launch
 
@@ -168,8 +169,10 @@ class IceFlow2D:
         """``population_storage`` has no reference counterpart: "fp32" (the
         reference's), "fp16" (all four population buffers in FP16 scaled by
         2^10, FP32 arithmetic) or "fp16-momentum" (the momentum populations in
-        FP16, the phase populations in FP32); FP16 buffers hold the population
-        times ``population_scale`` (a power of two).  Every other argument and
+        FP16, the phase populations in FP32), and "fp16-sr" /
+        "fp16-momentum-sr", the same with stochastic rounding of FP16 stores;
+        FP16 buffers hold the population times ``population_scale`` (a power
+        of two).  Every other argument and
         check is the reference's.
         ``_control_storage`` (a device uint8 tensor of the control block's
         size) lets an ensemble keep every case's control block in one array."""
