@@ -183,7 +183,8 @@ void icy2d_abi_sizes(int32_t* grid, int32_t* control);
  * Blocks of cases with active[c] == 0 return at once (active may be NULL).
  * The per-case results equal the single-case entry points' bit for bit
  * when collide_threads and stream_threads are 256. */
-#define ICY2D_SHAPES 9 /* cells, body, x faces, y faces, one, totals, cells/128, cells/512, band tiles */
+#define ICY2D_SHAPES 12 /* cells, body, x faces, y faces, one, totals, cells/128, cells/512, band tiles,
+                           collision tiles of 128, 256 and 512 threads */
 typedef struct icy2d_batch {
   int32_t ncases;
   int32_t collide_threads;      /* 128, 256 or 512 (chosen by the autotuner) */

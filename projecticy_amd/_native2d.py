@@ -76,7 +76,7 @@ CALLS = {
     "apply_fallback": [_i], "reduce_totals": [],
 }
 
-SHAPES = 9  # ICY2D_SHAPES
+SHAPES = 12  # ICY2D_SHAPES
 
 
 class Batch2D(ctypes.Structure):

@@ -616,16 +616,91 @@ __device__ __forceinline__ bool prepare_cut_links(const icy2d_grid& g, const Pos
   return cut;
 }
 
+// The collision reads a 5x5 phase stencil (the blended gradient uses
+// neighbours at distance 2) and a 3x3 velocity stencil.  A block covers a
+// tile of (blockDim.x / 64) x 64 cells and stages the tile's phase and
+// fluid flags with a two-cell halo, and its velocities and neighbour kinds
+// with a one-cell halo, in LDS once.
+constexpr int kTileCols = 64, kTileMaxRows = 8;
+struct CollideTile {
+  float phi[kTileMaxRows + 4][kTileCols + 4];
+  float u[2][kTileMaxRows + 2][kTileCols + 2];
+  uint8_t fluid[kTileMaxRows + 4][kTileCols + 4];
+  uint8_t kind[kTileMaxRows + 2][kTileCols + 2];  // 0 fluid, 1 ice (solid == 1), 2 wall or outside
+};
+// simulator.py:2459-2477 on the staged tile (a, b: local row/column of the cell in the phase window).
+__device__ __forceinline__ float phase_neighbor_tile(const CollideTile& t, int a, int b, int sx, int sy, int distance,
+                                                     float centre) {
+  float value = centre;
+  if (t.fluid[a + sx][b + sy]) {
+    value = t.phi[a + sx][b + sy];
+    if (distance == 2) value = t.fluid[a + 2 * sx][b + 2 * sy] ? t.phi[a + 2 * sx][b + 2 * sy] : centre;
+  }
+  return value;
+}
+__device__ __forceinline__ PhaseStencil phase_stencil_tile(const CollideTile& t, int a, int b, float phi) {
+  PhaseStencil s = {{0.0f, 0.0f}, {0.0f, 0.0f}, 0.0f};
+  const float alpha = 1.0f / 3.0f;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    const float p1 = phase_neighbor_tile(t, a, b, cxq(q), cyq(q), 1, phi);
+    const float p2 = phase_neighbor_tile(t, a, b, cxq(q), cyq(q), 2, phi);
+    const float c[2] = {(float)cxq(q), (float)cyq(q)};
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      s.source[e] += 3.0f * wq(q) * c[e] * (p1 - phi);
+      s.g[e] += (1.0f - alpha) * 3.0f * wq(q) * c[e] * (p1 - phi);
+      s.g[e] += alpha * 1.5f * wq(q) * c[e] * (4.0f * p1 - p2 - 3.0f * phi);
+    }
+    s.laplacian += 6.0f * wq(q) * (p1 - phi);
+  }
+  return s;
+}
+__device__ __forceinline__ void stage_collide_tile(const icy2d_grid& g, CollideTile& t, int i0, int j0, int rows) {
+  const int N = cells(g);
+  for (int e = threadIdx.x; e < (rows + 4) * (kTileCols + 4); e += blockDim.x) {
+    const int a = e / (kTileCols + 4), b = e - a * (kTileCols + 4), i = i0 - 2 + a, j = j0 - 2 + b;
+    const bool fluid = inside(g, i, j) && fluid_at(g, i, j);
+    t.fluid[a][b] = fluid;
+    t.phi[a][b] = fluid ? g.phi[at(g, i, j)] : 0.0f;
+  }
+  for (int e = threadIdx.x; e < (rows + 2) * (kTileCols + 2); e += blockDim.x) {
+    const int a = e / (kTileCols + 2), b = e - a * (kTileCols + 2), i = i0 - 1 + a, j = j0 - 1 + b;
+    uint8_t kind = 2;
+    float ux = 0.0f, uy = 0.0f;
+    if (inside(g, i, j)) {
+      const int m = at(g, i, j);
+      if (fluid_at(g, i, j)) {
+        kind = 0;
+        ux = g.u[m];
+        uy = g.u[N + m];
+      } else if (g.solid[m] == 1) {
+        kind = 1;
+      }
+    }
+    t.kind[a][b] = kind;
+    t.u[0][a][b] = ux;
+    t.u[1][a][b] = uy;
+  }
+}
+
 // simulator.py:3241-3552 (momentum) then :3554-3697 (phase) for one cell.
 __device__ __forceinline__ void k2_collide(const icy2d_grid& g, const int blk, int with_momentum) {
-  const int n = cell_index(blk), N = cells(g);
+  __shared__ CollideTile tile;
+  const int N = cells(g);
+  const int rows = (int)blockDim.x / kTileCols, tiles_j = (g.ny + kTileCols - 1) / kTileCols;
+  const int i0 = (blk / tiles_j) * rows, j0 = (blk % tiles_j) * kTileCols;
+  const int li = (int)threadIdx.x / kTileCols, lj = (int)threadIdx.x % kTileCols;
+  const int i = i0 + li, j = j0 + lj;
+  stage_collide_tile(g, tile, i0, j0, rows);
+  __syncthreads();
   double load[3] = {0.0, 0.0, 0.0};
-  if (n < N) {
-    const int i = n / g.ny, j = n - i * g.ny;
-    if (fluid_at(g, i, j)) {
+  if (i < g.nx && j < g.ny) {
+    const int n = at(g, i, j);
+    if (tile.fluid[li + 2][lj + 2]) {
       const Pose pose = load_pose(g);
-      const float phi = g.phi[n];
-      const PhaseStencil st = phase_stencil(g, i, j, phi);
+      const float phi = tile.phi[li + 2][lj + 2];
+      const PhaseStencil st = phase_stencil_tile(tile, li + 2, lj + 2, phi);
       float ax, ay;
       if (with_momentum) {
         // _fluid_force_viscosity_and_gradient (simulator.py:3241-3368)
@@ -645,18 +720,19 @@ __device__ __forceinline__ void k2_collide(const icy2d_grid& g, const int blk, i
         fy += chemical * st.g[1] / rho_b;
         const float ux0 = g.u[n], uy0 = g.u[N + n];
         float gxx = 0.0f, gxy = 0.0f, gyx = 0.0f, gyy = 0.0f;
+        float near_solid = 0.0f;
 #pragma unroll
         for (int q = 1; q < 9; ++q) {
           const int ni = i + cxq(q), nj = j + cyq(q);
+          const int a = li + 1 + cxq(q), b = lj + 1 + cyq(q);
           float v[2] = {0.0f, 0.0f};
-          if (inside(g, ni, nj)) {
-            const int m = at(g, ni, nj);
-            if (fluid_at(g, ni, nj)) {
-              v[0] = g.u[m];
-              v[1] = g.u[N + m];
-            } else if (g.solid[m] == 1) {
-              body_velocity(pose, ni + 0.5f, nj + 0.5f, v);
-            }
+          const uint8_t kind = tile.kind[a][b];
+          if (kind == 0) {
+            v[0] = tile.u[0][a][b];
+            v[1] = tile.u[1][a][b];
+          } else if (kind == 1) {
+            body_velocity(pose, ni + 0.5f, nj + 0.5f, v);
+            near_solid = 1.0f;
           }
           const float dxv = v[0] - ux0, dyv = v[1] - uy0, w3 = 3.0f * wq(q);
           if (cxq(q)) gxx += w3 * (float)cxq(q) * dxv;
@@ -675,12 +751,6 @@ __device__ __forceinline__ void k2_collide(const icy2d_grid& g, const int blk, i
         const float fraction = clamp01(phi);
         const float nu = fraction * g.nu_w + (1.0f - fraction) * g.nu_a + nu_t;
         const float tau = 0.5f + 3.0f * nu;
-        float near_solid = 0.0f;
-#pragma unroll
-        for (int q = 1; q < 9; ++q) {
-          const int ni = i + cxq(q), nj = j + cyq(q);
-          if (inside(g, ni, nj) && g.solid[at(g, ni, nj)] == 1) near_solid = 1.0f;
-        }
         const float indicator = fmaxf(1.0f - fraction, near_solid);
         const float shear_tau = fmaxf(tau, 0.5f + (g.interface_tau - 0.5f) * indicator);
         const float rho = mixture(g, phi), pdyn = g.p[n] - g.p_ref[j];
@@ -1977,7 +2047,10 @@ __device__ __forceinline__ void k2_totals_partials(const icy2d_grid& g, const in
 // kernel body above takes (grid, block index within its case); one_case
 // runs it for one grid passed by value, many_cases for the case that owns
 // the block in a batch's prefix table.
-enum Shape : int { kCells = 0, kBody, kFaceX, kFaceY, kOne, kTotals, kCells128, kCells512, kBandTiles, kShapes };
+enum Shape : int {
+  kCells = 0, kBody, kFaceX, kFaceY, kOne, kTotals, kCells128, kCells512, kBandTiles,
+  kTiles128, kTiles256, kTiles512, kShapes
+};
 static_assert(kShapes == ICY2D_SHAPES, "shape count");
 
 long shape_count(const icy2d_grid& g, int shape) {
@@ -1990,15 +2063,27 @@ long shape_count(const icy2d_grid& g, int shape) {
     case kTotals: return cells > body ? cells : body;
     case kBandTiles:  // one block of kBlock threads per 16 x 64 band tile
       return (long)((g.nx + kBandI - 1) / kBandI) * ((g.ny + kBandJ - 1) / kBandJ) * kBlock;
+    case kTiles128:  // one block per (threads / 64) x 64 collision tile
+    case kTiles256:
+    case kTiles512: {
+      const int threads = shape == kTiles128 ? 128 : shape == kTiles512 ? 512 : 256, rows = threads / kTileCols;
+      return (long)((g.nx + rows - 1) / rows) * ((g.ny + kTileCols - 1) / kTileCols) * threads;
+    }
     default: return cells;
   }
 }
-int shape_threads(int shape) { return shape == kOne ? 1 : shape == kCells128 ? 128 : shape == kCells512 ? 512 : kBlock; }
+int shape_threads(int shape) {
+  if (shape == kOne) return 1;
+  if (shape == kCells128 || shape == kTiles128) return 128;
+  if (shape == kCells512 || shape == kTiles512) return 512;
+  return kBlock;
+}
 unsigned shape_blocks(const icy2d_grid& g, int shape) {
   const long t = shape_threads(shape);
   return (unsigned)((shape_count(g, shape) + t - 1) / t);
 }
 int cells_shape(int threads) { return threads == 128 ? kCells128 : threads == 512 ? kCells512 : kCells; }
+int tiles_shape(int threads) { return threads == 128 ? kTiles128 : threads == 512 ? kTiles512 : kTiles256; }
 
 struct PerCase {
   const double* v;
@@ -2065,7 +2150,7 @@ struct One {
   hipStream_t s;
   using Real = double;
   bool fits(int shape, int K) const { return (long)shape_blocks(g, shape) * K + 8 <= (long)g.partials_capacity; }
-  int tuned_shape(int) const { return kCells; }
+  int tuned_shape(int which) const { return which == 0 ? kTiles256 : kCells; }
   template <auto F, class... A>
   int launch(int shape, A... a) const {
     one_case<kBlock, F><<<shape_blocks(g, shape), shape_threads(shape), 0, s>>>(g, a...);
@@ -2086,7 +2171,9 @@ struct Many {
   hipStream_t s;
   using Real = PerCase;
   bool fits(int, int) const { return true; }
-  int tuned_shape(int which) const { return cells_shape(which == 0 ? b.collide_threads : b.stream_threads); }
+  int tuned_shape(int which) const {
+    return which == 0 ? tiles_shape(b.collide_threads) : cells_shape(b.stream_threads);
+  }
   template <auto F, class... A>
   int launch(int shape, A... a) const {
     many_cases<kBlock, F><<<b.total_blocks[shape], shape_threads(shape), 0, s>>>(b, shape, a...);
@@ -2095,8 +2182,8 @@ struct Many {
   template <auto F, class... A>
   int launch_tuned(int shape, A... a) const {
     const unsigned blocks = b.total_blocks[shape];
-    if (shape == kCells128) many_cases<128, F><<<blocks, 128, 0, s>>>(b, shape, a...);
-    else if (shape == kCells512) many_cases<512, F><<<blocks, 512, 0, s>>>(b, shape, a...);
+    if (shape_threads(shape) == 128) many_cases<128, F><<<blocks, 128, 0, s>>>(b, shape, a...);
+    else if (shape_threads(shape) == 512) many_cases<512, F><<<blocks, 512, 0, s>>>(b, shape, a...);
     else many_cases<kBlock, F><<<blocks, kBlock, 0, s>>>(b, shape, a...);
     return launch_status();
   }
@@ -2332,6 +2419,9 @@ long partials_needed(const icy2d_grid& g) {
   take(kCells, 6);
   take(kCells128, 3);
   take(kCells512, 3);
+  take(kTiles128, 3);
+  take(kTiles256, 3);
+  take(kTiles512, 3);
   take(kBody, 4);
   take(kFaceX, 1);
   take(kFaceY, 1);

@@ -285,7 +285,8 @@ class IceFlow2D:
         }
         self._t = {name: torch.zeros(size, dtype=dtype, device=dev) for name, (size, dtype) in shapes.items()}
         blocks = (max(N, M, (nx + 1) * (ny + 1)) + 255) // 256
-        capacity = blocks * 6 + 16
+        tiles = ((nx + 1) // 2) * ((ny + 63) // 64)  # collision tiles of 2 x 64 cells, the most numerous
+        capacity = max(blocks, tiles) * 6 + 16
         self._t["partials"] = torch.zeros(capacity, dtype=f64, device=dev)
         size = ctypes.sizeof(native.Control2D)
         if self._control_storage is not None:

@@ -80,7 +80,8 @@ def _blocks(count, threads=256):
 
 def test_batch_plan_prefix_tables(library):
     sizes = [(64, 128, 20, 20), (37, 11, 5, 7), (512, 1024, 150, 150)]
-    grids = (native.Grid2D * 3)(*[_grid(nx, ny, bx, by, 6 * _blocks(nx * ny) + 64) for nx, ny, bx, by in sizes])
+    grids = (native.Grid2D * 3)(*[_grid(nx, ny, bx, by, 6 * _blocks(nx * ny) + 3 * _blocks(nx, 2) * _blocks(ny, 64) + 64)
+                                  for nx, ny, bx, by in sizes])
     first = (ctypes.c_int32 * (native.SHAPES * 4))()
     totals = (ctypes.c_int32 * native.SHAPES)()
     assert library.icy2d_batch_plan(grids, 3, first, totals) == 0
@@ -94,6 +95,9 @@ def test_batch_plan_prefix_tables(library):
         6: [_blocks(nx * ny, 128) for nx, ny, _, _ in sizes],
         7: [_blocks(nx * ny, 512) for nx, ny, _, _ in sizes],
         8: [_blocks(nx, 16) * _blocks(ny, 64) for nx, ny, _, _ in sizes],
+        9: [_blocks(nx, 2) * _blocks(ny, 64) for nx, ny, _, _ in sizes],
+        10: [_blocks(nx, 4) * _blocks(ny, 64) for nx, ny, _, _ in sizes],
+        11: [_blocks(nx, 8) * _blocks(ny, 64) for nx, ny, _, _ in sizes],
     }
     for shape, per_case in counts.items():
         table = list(first[shape * 4:(shape + 1) * 4])
