@@ -106,6 +106,78 @@ __global__ void __launch_bounds__(kConvThreads)
     }
 }
 
+// LDS-tiled form for z-columns of at most 64 cells (nz a multiple of 16): a
+// block of four waves computes a 2 x 2 group of whole z-columns, one per
+// wave.  The 4 x 4 input columns around them (with one cell of z halo) are
+// staged in LDS once, 16 bytes (8 channels) per slot with the slot index
+// swizzled by the cell's low bits so that the 16 lanes of an operand read
+// spread over the banks; every tap's A operands then come from LDS.
+constexpr int kColumnTile = 2, kStageCols = (kColumnTile + 2) * (kColumnTile + 2), kMaxColumn = 64;
+template <int Epi>
+__global__ void __launch_bounds__(kConvThreads)
+    k_conv3d_columns(int nx, int ny, int nz, const uint16_t* __restrict__ in, const uint16_t* __restrict__ w,
+                     const float* __restrict__ bias, const uint16_t* __restrict__ mask, uint16_t* __restrict__ out,
+                     float* __restrict__ out_f32) {
+  __shared__ uint4 xs[kStageCols][kMaxColumn + 2][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 15, group = lane >> 4;
+  const int tiles_j = (ny + kColumnTile - 1) / kColumnTile;
+  const int i0 = (int)(blockIdx.x / tiles_j) * kColumnTile, j0 = (int)(blockIdx.x % tiles_j) * kColumnTile;
+  const int depth = nz + 2, per_col = depth * 4;
+  for (int e = threadIdx.x; e < kStageCols * per_col; e += blockDim.x) {
+    const int col = e / per_col, rem = e - col * per_col, cell = rem >> 2, chunk = rem & 3;
+    const int i = i0 - 1 + col / 4, j = j0 - 1 + col % 4, k = cell - 1;
+    uint4 v = make_uint4(0u, 0u, 0u, 0u);
+    if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz)
+      v = *reinterpret_cast<const uint4*>(in + (((long)i * ny + j) * nz + k) * kC + chunk * 8);
+    xs[col][cell][chunk ^ (cell & 3)] = v;
+  }
+  __syncthreads();
+  const int wi = wave >> 1, wj = wave & 1, i = i0 + wi, j = j0 + wj;
+  if (i >= nx || j >= ny) return;
+  const int mtiles = nz / 16;
+  floatx4 acc[4][2];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) acc[mt][0] = acc[mt][1] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int t = 0; t < kTaps; ++t) {
+    const int di = t / 9 - 1, dj = (t / 3) % 3 - 1, dk = t % 3 - 1;
+    const uint4* wt = reinterpret_cast<const uint4*>(w + (long)t * kC * kC);
+    const bf16x8 b0 = as_bf16x8(wt[row * 4 + group]);
+    const bf16x8 b1 = as_bf16x8(wt[(16 + row) * 4 + group]);
+    const int col = (wi + 1 + di) * 4 + (wj + 1 + dj);
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+      if (mt < mtiles) {
+        const int cell = mt * 16 + row + 1 + dk;
+        const bf16x8 a = as_bf16x8(xs[col][cell][group ^ (cell & 3)]);
+        acc[mt][0] = mfma(a, b0, acc[mt][0]);
+        acc[mt][1] = mfma(a, b1, acc[mt][1]);
+      }
+    }
+  }
+  const long base = ((long)i * ny + j) * nz;
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    if (mt >= mtiles) continue;
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt) {
+      const int co = nt * 16 + row;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long cell = base + mt * 16 + group * 4 + r;
+        const float x = acc[mt][nt][r];
+        if (Epi == ICYC_RELU_BF16) {
+          out[cell * kC + co] = f2bf(fmaxf(0.0f, x + bias[co]));
+        } else if (Epi == ICYC_LINEAR_F32) {
+          if (co == 0) out_f32[cell] = x + bias[0];
+        } else {
+          out[cell * kC + co] = bf2f(mask[cell * kC + co]) > 0.0f ? f2bf(x) : (uint16_t)0;
+        }
+      }
+    }
+  }
+}
+
 // ----------------------------------------------------------- weight grad
 // One block per group of z-columns (i, j): the input's 3x3 neighbour
 // columns (transposed to [channel][k]) and three k-shifted copies of the
@@ -327,8 +399,31 @@ extern "C" {
 int icyc_conv3d(int32_t nx, int32_t ny, int32_t nz, const uint16_t* in, const uint16_t* weights, const float* bias,
                 int32_t epilogue, const uint16_t* mask_act, uint16_t* out_bf16, float* out_f32, void* stream) {
   if (!valid_grid(nx, ny, nz) || !in || !weights) return ICYC_ERR_ARGUMENT;
-  const unsigned blocks = blocks_of((long)nx * ny * nz, kCellsPerBlock);
   hipStream_t s = (hipStream_t)stream;
+  if (nz % 16 == 0 && nz <= kMaxColumn) {
+    const unsigned tiles = (unsigned)(((nx + kColumnTile - 1) / kColumnTile) * ((ny + kColumnTile - 1) / kColumnTile));
+    switch (epilogue) {
+      case ICYC_RELU_BF16:
+        if (!bias || !out_bf16) return ICYC_ERR_ARGUMENT;
+        k_conv3d_columns<ICYC_RELU_BF16><<<tiles, kConvThreads, 0, s>>>(nx, ny, nz, in, weights, bias, nullptr,
+                                                                       out_bf16, nullptr);
+        break;
+      case ICYC_LINEAR_F32:
+        if (!bias || !out_f32) return ICYC_ERR_ARGUMENT;
+        k_conv3d_columns<ICYC_LINEAR_F32><<<tiles, kConvThreads, 0, s>>>(nx, ny, nz, in, weights, bias, nullptr,
+                                                                        nullptr, out_f32);
+        break;
+      case ICYC_MASKED_BF16:
+        if (!mask_act || !out_bf16) return ICYC_ERR_ARGUMENT;
+        k_conv3d_columns<ICYC_MASKED_BF16><<<tiles, kConvThreads, 0, s>>>(nx, ny, nz, in, weights, nullptr,
+                                                                         mask_act, out_bf16, nullptr);
+        break;
+      default:
+        return ICYC_ERR_ARGUMENT;
+    }
+    return status();
+  }
+  const unsigned blocks = blocks_of((long)nx * ny * nz, kCellsPerBlock);
   switch (epilogue) {
     case ICYC_RELU_BF16:
       if (!bias || !out_bf16) return ICYC_ERR_ARGUMENT;
