@@ -19,8 +19,9 @@ The JSON line adds:
                 with its own CPU baseline (the 2D oracle on the 100x200 case);
   ensemble      64 uneven D2Q9 cases (widths 64-512) stepped as one batch
                 (one launch per kernel for all cases) with FP32, FP16 momentum
-                (stochastic rounding) and FP16 (stochastic rounding) population
-                storage: cases/hour for 10,000-step cases,
+                (stochastic rounding) and all-FP16 (phase populations stored
+                relative to w_q*phi, stochastic rounding) population storage:
+                cases/hour for 10,000-step cases,
                 the same figure for one-case-at-a-time FP32 runs, and each
                 storage's melted-mass error after 10,000 steps against the
                 FP32 standalone runs; ``value`` is the best cases/hour among
@@ -180,7 +181,7 @@ def closure_measurement(args):
     }
 
 
-ENSEMBLE_STORAGE = ("fp32", "fp16-momentum-sr", "fp16-sr")
+ENSEMBLE_STORAGE = ("fp32", "fp16-momentum-sr", "fp16-rel-sr")
 
 
 def ensemble_measurement(args):

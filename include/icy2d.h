@@ -73,6 +73,7 @@ typedef struct icy2d_grid {
   float *p_ref, *rho_ref;         /* [ny] */
   int8_t *solid, *prev_solid, *eligible;
   float* coverage;                /* world body solid fraction [N] */
+  float* phase_ref;               /* phase each cell's hbuf is stored against [N] (bit 3 below) */
   /* thermal fields (FP64) */
   double *body_mass, *body_energy, *body_heat, *melt_mass, *melt_sensible;
   int32_t* melt_target;
@@ -84,7 +85,9 @@ typedef struct icy2d_grid {
    * population * population_scale, bit 1 set = h and hbuf do (a power-of-two
    * scale; FP32 arithmetic in registers); clear bits mean FP32; bit 2 rounds
    * FP16 stores stochastically (hash of value and index) instead of to
-   * nearest.  No reference counterpart. */
+   * nearest; bit 3 stores each phase population as its deviation from
+   * w_q * phi (h against phi, hbuf against phase_ref, the phase at the
+   * collision).  No reference counterpart. */
   int32_t half_populations;
   float population_scale, population_inverse_scale;
   icy2d_control* control;

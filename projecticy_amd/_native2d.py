@@ -20,7 +20,7 @@ GRID_DOUBLES = ("initial_mass", "initial_inertia", "t_melt", "cp_w", "cp_i", "k_
                 "area", "velocity_scale", "cell_mass", "t_air", "left_value", "right_value", "bottom_value",
                 "top_value")
 GRID_POINTERS = ("f", "fbuf", "h", "hbuf", "phi", "u", "a", "p", "moving", "p_ref", "rho_ref", "solid",
-                 "prev_solid", "eligible", "coverage", "body_mass", "body_energy", "body_heat", "melt_mass",
+                 "prev_solid", "eligible", "coverage", "phase_ref", "body_mass", "body_energy", "body_heat", "melt_mass",
                  "melt_sensible", "melt_target", "water_v", "water_e", "target_e", "flux_ex", "flux_vx",
                  "flux_ey", "flux_vy", "partials")
 

@@ -105,6 +105,24 @@ __device__ __forceinline__ void pop_add(float* base, const icy2d_grid& g, int id
 }
 constexpr int kMomentum = 0, kPhase = 1;
 
+// Bit 3 of half_populations stores each phase population as its deviation
+// from w_q * ref, where ref is the cell's phase: phi[n] for h (every kernel
+// that changes a fluid cell's phi re-encodes its h), and phase_ref[n], the
+// phase the collision saw, for hbuf (the stream rewrites phi while it pulls
+// hbuf from the neighbours).  Near the bulk the deviation is O(u) instead of
+// the O(1) equilibrium part, so FP16 keeps ~100x finer absolute steps.
+// The projection's flag planes in hbuf use pop_get/pop_set directly.
+__device__ __forceinline__ bool phase_relative(const icy2d_grid& g) { return (g.half_populations & 8) != 0; }
+__device__ __forceinline__ float phase_get(const float* base, const icy2d_grid& g, int q, int n, float ref) {
+  const float v = pop_get<kPhase>(base, g, q * cells(g) + n);
+  return phase_relative(g) ? v + wq(q) * ref : v;
+}
+__device__ __forceinline__ void phase_set(float* base, const icy2d_grid& g, int q, int n, float ref, float v) {
+  pop_set<kPhase>(base, g, q * cells(g) + n, phase_relative(g) ? v - wq(q) * ref : v);
+}
+// The saved phase of a cell whose hbuf is read (0 when unused).
+__device__ __forceinline__ float hbuf_ref(const icy2d_grid& g, int n) { return phase_relative(g) ? g.phase_ref[n] : 0.0f; }
+
 // lattice.py:64-104
 template <int Q>
 __device__ __forceinline__ float momentum_eq(float p, float rho, float ux, float uy) {
@@ -463,7 +481,8 @@ __device__ __forceinline__ void k2_initialize_fluid(const icy2d_grid& g, const i
   const float phi0 = active ? indicator : 0.0f;
   float v[2] = {0.0f, 0.0f};
   if (g.solid[n] == 1) body_velocity(p, i + 0.5f, j + 0.5f, v);
-  g.phi[n] = g.solid[n] == 1 ? indicator : phi0;
+  const float phase = g.solid[n] == 1 ? indicator : phi0;
+  g.phi[n] = phase;
   g.u[n] = v[0];
   g.u[N + n] = v[1];
   g.a[n] = g.a[N + n] = 0.0f;
@@ -472,7 +491,7 @@ __device__ __forceinline__ void k2_initialize_fluid(const icy2d_grid& g, const i
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
     pop_set<kMomentum>(g.f, g, q * N + n, momentum_eq_rt(q, 0.0f, rho, v[0], v[1]));
-    pop_set<kPhase>(g.h, g, q * N + n, active ? phase_eq_rt(q, phi0, v[0], v[1]) : 0.0f);
+    phase_set(g.h, g, q, n, phase, active ? phase_eq_rt(q, phi0, v[0], v[1]) : 0.0f);
   }
 }
 // simulator.py:2664-2697
@@ -487,7 +506,7 @@ __device__ __forceinline__ void k2_hydrostatic_equilibrium(const icy2d_grid& g, 
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
       pop_set<kMomentum>(g.f, g, q * N + n, momentum_eq_rt(q, 0.0f, rho, 0.0f, 0.0f));
-      pop_set<kPhase>(g.h, g, q * N + n, phase_eq_rt(q, phi, 0.0f, 0.0f));
+      phase_set(g.h, g, q, n, phi, phase_eq_rt(q, phi, 0.0f, 0.0f));
     }
   } else {
     float v[2] = {0.0f, 0.0f};
@@ -514,15 +533,21 @@ __device__ __forceinline__ void k2_recover_initial_moments(const icy2d_grid& g, 
   if (n >= N) return;
   const int i = n / g.ny, j = n - i * g.ny;
   if (fluid_at(g, i, j)) {
-    float phi = 0.0f, mx = 0.0f, my = 0.0f;
+    float phi = 0.0f, mx = 0.0f, my = 0.0f, h[9];
+    const float old_phase = g.phi[n];
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-      phi += pop_get<kPhase>(g.h, g, q * N + n);
+      h[q] = phase_get(g.h, g, q, n, old_phase);
+      phi += h[q];
       const float fv = pop_get<kMomentum>(g.f, g, q * N + n);
       if (cxq(q)) mx += (float)cxq(q) * fv;
       if (cyq(q)) my += (float)cyq(q) * fv;
     }
     g.phi[n] = phi;
+    if (phase_relative(g)) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) phase_set(g.h, g, q, n, phi, h[q]);
+    }
     const float rho = fmaxf(mixture(g, phi), 1.0e-12f);
     g.u[n] = mx / rho;
     g.u[N + n] = my / rho;
@@ -810,7 +835,7 @@ __device__ __forceinline__ void k2_collide(const icy2d_grid& g, const int blk, i
       float r10 = 0.0f, r01 = 0.0f, eq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, src[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
       for (int q = 0; q < 9; ++q) {
-        const float rx = (float)cxq(q) - ux, ry = (float)cyq(q) - uy, hv = pop_get<kPhase>(g.h, g, q * N + n);
+        const float rx = (float)cxq(q) - ux, ry = (float)cyq(q) - uy, hv = phase_get(g.h, g, q, n, phi);
         const float he = phase_eq_rt(q, phi, ux, uy);
         const float force = wq(q) * compression * ((float)cxq(q) * nx_ + (float)cyq(q) * ny_);
         const float rx2 = rx * rx, ry2 = ry * ry;
@@ -828,15 +853,16 @@ __device__ __forceinline__ void k2_collide(const icy2d_grid& g, const int blk, i
       for (int m = 0; m < 9; ++m) post[m] = eq[m] + 0.5f * src[m];
       post[1] = r10 - rate * (r10 - eq[1]) + (1.0f - 0.5f * rate) * src[1];
       post[2] = r01 - rate * (r01 - eq[2]) + (1.0f - 0.5f * rate) * src[2];
-      pop_set<kPhase>(g.hbuf, g, 0 * N + n, reconstruct_cm<0>(ux, uy, post));
-      pop_set<kPhase>(g.hbuf, g, 1 * N + n, reconstruct_cm<1>(ux, uy, post));
-      pop_set<kPhase>(g.hbuf, g, 2 * N + n, reconstruct_cm<2>(ux, uy, post));
-      pop_set<kPhase>(g.hbuf, g, 3 * N + n, reconstruct_cm<3>(ux, uy, post));
-      pop_set<kPhase>(g.hbuf, g, 4 * N + n, reconstruct_cm<4>(ux, uy, post));
-      pop_set<kPhase>(g.hbuf, g, 5 * N + n, reconstruct_cm<5>(ux, uy, post));
-      pop_set<kPhase>(g.hbuf, g, 6 * N + n, reconstruct_cm<6>(ux, uy, post));
-      pop_set<kPhase>(g.hbuf, g, 7 * N + n, reconstruct_cm<7>(ux, uy, post));
-      pop_set<kPhase>(g.hbuf, g, 8 * N + n, reconstruct_cm<8>(ux, uy, post));
+      phase_set(g.hbuf, g, 0, n, phi, reconstruct_cm<0>(ux, uy, post));
+      phase_set(g.hbuf, g, 1, n, phi, reconstruct_cm<1>(ux, uy, post));
+      phase_set(g.hbuf, g, 2, n, phi, reconstruct_cm<2>(ux, uy, post));
+      phase_set(g.hbuf, g, 3, n, phi, reconstruct_cm<3>(ux, uy, post));
+      phase_set(g.hbuf, g, 4, n, phi, reconstruct_cm<4>(ux, uy, post));
+      phase_set(g.hbuf, g, 5, n, phi, reconstruct_cm<5>(ux, uy, post));
+      phase_set(g.hbuf, g, 6, n, phi, reconstruct_cm<6>(ux, uy, post));
+      phase_set(g.hbuf, g, 7, n, phi, reconstruct_cm<7>(ux, uy, post));
+      phase_set(g.hbuf, g, 8, n, phi, reconstruct_cm<8>(ux, uy, post));
+      if (phase_relative(g)) g.phase_ref[n] = phi;
     }
   }
   if (with_momentum) block_partials<3>(load, g.partials, blk);
@@ -869,11 +895,11 @@ __device__ __forceinline__ void pull(const icy2d_grid& g, const Pose& pose, int 
   if (Q == 0 || (inside(g, si, sj) && fluid_at(g, si, sj))) {
     const int m = Q == 0 ? n : at(g, si, sj);
     if (!phase_only) fq = pop_get<kMomentum>(g.fbuf, g, Q * N + m);
-    hq = pop_get<kPhase>(g.hbuf, g, Q * N + m);
+    hq = phase_get(g.hbuf, g, Q, m, hbuf_ref(g, m));
   } else {
     constexpr int out = oppq(Q);  // own population sent towards (si, sj)
     if (!phase_only) fq = pop_get<kMomentum>(g.fbuf, g, out * N + n);
-    float ho = pop_get<kPhase>(g.hbuf, g, out * N + n);
+    float ho = phase_get(g.hbuf, g, out, n, hbuf_ref(g, n));
     if (inside(g, si, sj) && g.solid[at(g, si, sj)] == 1) {
       const float eta = cut_fraction(g, pose, i, j, si, sj);
       float ub[2];
@@ -906,22 +932,28 @@ __device__ __forceinline__ void k2_stream(const icy2d_grid& g, const int blk, in
       pull<6>(g, pose, i, j, n, phi_old, po, f[6], h[6]);
       pull<7>(g, pose, i, j, n, phi_old, po, f[7], h[7]);
       pull<8>(g, pose, i, j, n, phi_old, po, f[8], h[8]);
-#pragma unroll
-      for (int q = 0; q < 9; ++q) {
-        pop_set<kPhase>(g.h, g, q * N + n, h[q]);
-        if (!po) pop_set<kMomentum>(g.f, g, q * N + n, f[q]);
-      }
     } else {
 #pragma unroll
       for (int q = 0; q < 9; ++q) {
-        h[q] = pop_get<kPhase>(g.h, g, q * N + n);
+        h[q] = phase_get(g.h, g, q, n, phi_old);
         if (!po) f[q] = pop_get<kMomentum>(g.f, g, q * N + n);
       }
     }
-    if (!update) return;
-    float phi = 0.0f;
+    float phi = phi_old;
+    if (update) {
+      phi = 0.0f;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) phi += h[q];
+      for (int q = 0; q < 9; ++q) phi += h[q];
+    }
+    // h is stored against the phase this pass leaves in phi[n]
+    if (pulling || (update && phase_relative(g))) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        phase_set(g.h, g, q, n, phi, h[q]);
+        if (pulling && !po) pop_set<kMomentum>(g.f, g, q * N + n, f[q]);
+      }
+    }
+    if (!update) return;
     g.phi[n] = phi;
     if (!po) {
       float mx = 0.0f, my = 0.0f, moving = 0.0f;
@@ -1022,7 +1054,7 @@ __device__ __forceinline__ void k2_refill(const icy2d_grid& g, const int blk) {
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
     pop_set<kMomentum>(g.f, g, q * N + n, momentum_eq_rt(q, pdyn, rho, v[0], v[1]));
-    pop_set<kPhase>(g.h, g, q * N + n, phase_eq_rt(q, phi0, v[0], v[1]));
+    phase_set(g.h, g, q, n, phi0, phase_eq_rt(q, phi0, v[0], v[1]));
   }
 }
 // simulator.py:2303-2338 (per-block maxima as partials)
@@ -1070,14 +1102,16 @@ __device__ __forceinline__ void lift_phase(const icy2d_grid& g, int n, int j, fl
 #pragma unroll
   for (int q = 0; q < 9; ++q)
     pop_add<kMomentum>(g.f, g, q * N + n, momentum_eq_rt(q, pdyn, rn, ux, uy) - momentum_eq_rt(q, pdyn, ro, ux, uy));
-  float sum = 0.0f;
+  const float stored = g.phi[n];
+  float lifted[9], sum = 0.0f;
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
-    const float lifted = pop_get<kPhase>(g.h, g, q * N + n) + phase_eq_rt(q, new_phase, ux, uy) - phase_eq_rt(q, old_phase, ux, uy);
-    pop_set<kPhase>(g.h, g, q * N + n, lifted);
-    sum += lifted;
+    lifted[q] = phase_get(g.h, g, q, n, stored) + phase_eq_rt(q, new_phase, ux, uy) - phase_eq_rt(q, old_phase, ux, uy);
+    sum += lifted[q];
   }
-  pop_add<kPhase>(g.h, g, n, new_phase - sum);
+  lifted[0] += new_phase - sum;
+#pragma unroll
+  for (int q = 0; q < 9; ++q) phase_set(g.h, g, q, n, new_phase, lifted[q]);
   g.phi[n] = new_phase;
 }
 // simulator.py:4150-4229
@@ -1099,29 +1133,25 @@ __device__ __forceinline__ void k2_clip_phase(const icy2d_grid& g, const int blk
         for (int q = 0; q < 9; ++q)
           pop_add<kMomentum>(g.f, g, q * N + n, momentum_eq_rt(q, pdyn, rn, ux, uy) - momentum_eq_rt(q, pdyn, ro, ux, uy));
       }
-      float l1 = 0.0f;
+      float h[9], l1 = 0.0f;
 #pragma unroll
-      for (int q = 0; q < 9; ++q) l1 += fabsf(pop_get<kPhase>(g.h, g, q * N + n));
+      for (int q = 0; q < 9; ++q) {
+        h[q] = phase_get(g.h, g, q, n, old_phase);
+        l1 += fabsf(h[q]);
+      }
       const bool regularize = (bounded == 0.0f || bounded == 1.0f) && (double)l1 > max_bulk_l1;
-      if (regularize) {
+      if (regularize || bounded != old_phase) {
         const float px = g.u[n] + 0.5f * g.a[n], py = g.u[N + n] + 0.5f * g.a[N + n];
         float sum = 0.0f;
 #pragma unroll
         for (int q = 0; q < 9; ++q) {
-          const float r = phase_eq_rt(q, bounded, px, py);
-          pop_set<kPhase>(g.h, g, q * N + n, r);
-          sum += r;
+          h[q] = regularize ? phase_eq_rt(q, bounded, px, py)
+                            : h[q] + phase_eq_rt(q, bounded, ux, uy) - phase_eq_rt(q, old_phase, ux, uy);
+          sum += h[q];
         }
-        pop_add<kPhase>(g.h, g, n, bounded - sum);
-      } else if (bounded != old_phase) {
-        float sum = 0.0f;
+        h[0] += bounded - sum;
 #pragma unroll
-        for (int q = 0; q < 9; ++q) {
-          const float lifted = pop_get<kPhase>(g.h, g, q * N + n) + phase_eq_rt(q, bounded, ux, uy) - phase_eq_rt(q, old_phase, ux, uy);
-          pop_set<kPhase>(g.h, g, q * N + n, lifted);
-          sum += lifted;
-        }
-        pop_add<kPhase>(g.h, g, n, bounded - sum);
+        for (int q = 0; q < 9; ++q) phase_set(g.h, g, q, n, bounded, h[q]);
       }
       g.phi[n] = bounded;
       v[0] = (double)bounded;

@@ -22,8 +22,11 @@ from .config import BOUNDARY_CODE, IceFlowConfig, LatticeScales, MovingBodyTherm
 MAX_BULK_PHASE_L1 = 2.0          # simulator.py:45
 POPULATION_SCALE = 1024.0         # FP16 population storage holds population * 2^10
 # storage name -> bit mask of FP16 buffers (bit 0 momentum f/fbuf, bit 1 phase h/hbuf)
-# (bit 2: stochastic rounding of FP16 stores)
-POPULATION_STORAGE = {"fp32": 0, "fp16": 3, "fp16-momentum": 1, "fp16-sr": 7, "fp16-momentum-sr": 5}
+# (bit 2: stochastic rounding of FP16 stores; bit 3: phase populations stored
+# as their deviation from w_q * phi)
+POPULATION_STORAGE = {"fp32": 0, "fp16": 3, "fp16-momentum": 1, "fp16-sr": 7, "fp16-momentum-sr": 5,
+                      "fp16-rel": 11, "fp16-rel-sr": 15}
+D2Q9_WEIGHTS = np.array([4 / 9] + [1 / 9] * 4 + [1 / 36] * 4, dtype=np.float32)
 MAX_LATTICE_VELOCITY_L1 = 1.0    # simulator.py:44
 MAX_FUSED_BAND = 16              # widest band icy2d_dilate_band builds in one launch
 
@@ -76,24 +79,52 @@ class ControlValue:
 
 
 class ScaledPopulationField(DeviceField):
-    """View of FP16 population planes that hold population * scale."""
+    """View of FP16 population planes that hold population * scale, or, with
+    a ``reference`` [nx, ny] tensor, (population - w_q * reference) * scale."""
 
-    def __init__(self, tensor, scale, **kwargs):
+    def __init__(self, tensor, scale, reference=None, **kwargs):
         super().__init__(tensor, logical_dtype=np.float32, **kwargs)
         self.scale = scale
+        self.reference = reference
+
+    def _anchor(self):
+        ref = self.reference.detach().cpu().numpy().astype(np.float32)
+        return D2Q9_WEIGHTS[:, None, None] * ref[None]
 
     def to_numpy(self):
         array = self.t.detach().float().cpu().numpy() / np.float32(self.scale)
+        if self.reference is not None:
+            array = array + self._anchor()
         array = np.moveaxis(array, 0, -1)
         return np.ascontiguousarray(array.astype(np.float32))
 
     def from_numpy(self, value):
         value = np.broadcast_to(np.asarray(value, dtype=np.float32), self.shape)
-        physical = np.moveaxis(value, -1, 0) * np.float32(self.scale)
+        physical = np.moveaxis(value, -1, 0)
+        if self.reference is not None:
+            physical = physical - self._anchor()
+        physical = physical * np.float32(self.scale)
         self.t.copy_(torch.from_numpy(np.ascontiguousarray(physical)).to(torch.float16))
 
     def fill(self, value):
         self.from_numpy(np.full(self.shape, float(value), dtype=np.float32))
+
+
+class AnchoredPhaseField(DeviceField):
+    """``water_phase`` when the phase populations are stored against it: a
+    write keeps their values by re-encoding them against the new phase."""
+
+    def __init__(self, tensor, anchored, **kwargs):
+        super().__init__(tensor, **kwargs)
+        self.anchored = anchored
+
+    def from_numpy(self, value):
+        saved = self.anchored.to_numpy()
+        super().from_numpy(value)
+        self.anchored.from_numpy(saved)
+
+    def fill(self, value):
+        self.from_numpy(np.broadcast_to(np.asarray(value, dtype=np.float32), self.shape))
 
 
 class DerivedValue:
@@ -171,9 +202,11 @@ class IceFlow2D:
         2^10, FP32 arithmetic) or "fp16-momentum" (the momentum populations in
         FP16, the phase populations in FP32), and "fp16-sr" /
         "fp16-momentum-sr", the same with stochastic rounding of FP16 stores;
-        FP16 buffers hold the population times ``population_scale`` (a power
-        of two).  Every other argument and
-        check is the reference's.
+        "fp16-rel" / "fp16-rel-sr" are "fp16" / "fp16-sr" with each phase
+        population stored as its deviation from w_q * phi (the equilibrium
+        part at rest), which FP16 resolves ~100x more finely.  FP16 buffers
+        hold the population times ``population_scale`` (a power of two).
+        Every other argument and check is the reference's.
         ``_control_storage`` (a device uint8 tensor of the control block's
         size) lets an ensemble keep every case's control block in one array."""
         # simulator.py:1829-1843
@@ -277,7 +310,7 @@ class IceFlow2D:
             "f": (9 * N, fpop), "fbuf": (9 * N, fpop), "h": (9 * N, hpop), "hbuf": (9 * N, hpop),
             "phi": (N, f32), "u": (2 * N, f32), "a": (2 * N, f32), "p": (N, f32), "moving": (N, f32),
             "p_ref": (ny, f32), "rho_ref": (ny, f32), "solid": (N, i8), "prev_solid": (N, i8), "eligible": (N, i8),
-            "coverage": (N, f32), "body_mass": (M, f64), "body_energy": (M, f64), "body_heat": (M, f64),
+            "coverage": (N, f32), "phase_ref": (N, f32), "body_mass": (M, f64), "body_energy": (M, f64), "body_heat": (M, f64),
             "melt_mass": (M, f64), "melt_sensible": (M, f64), "melt_target": (M, i32),
             "water_v": (N, f64), "water_e": (N, f64), "target_e": (N, f64),
             "flux_ex": ((nx + 1) * ny, f64), "flux_vx": ((nx + 1) * ny, f64),
@@ -339,15 +372,20 @@ class IceFlow2D:
 
     def _make_fields(self):
         nx, ny, t = self.nx, self.ny, self._t
-        def pop_field(name):
+        relative = POPULATION_STORAGE[self.population_storage] & 8
+        def pop_field(name, reference=None):
             if t[name].dtype == torch.float16:
-                return ScaledPopulationField(t[name].view(9, nx, ny), self.population_scale, components=9)
+                ref = t[reference].view(nx, ny) if relative and reference else None
+                return ScaledPopulationField(t[name].view(9, nx, ny), self.population_scale, ref, components=9)
             return DeviceField(t[name].view(9, nx, ny), components=9)
         self.momentum_populations = pop_field("f")
         self.momentum_stream_buffer = pop_field("fbuf")
-        self.phase_populations = pop_field("h")
-        self.phase_stream_buffer = pop_field("hbuf")
-        self.water_phase = DeviceField(t["phi"].view(nx, ny))
+        self.phase_populations = pop_field("h", "phi")
+        self.phase_stream_buffer = pop_field("hbuf", "phase_ref")
+        if relative:
+            self.water_phase = AnchoredPhaseField(t["phi"].view(nx, ny), self.phase_populations)
+        else:
+            self.water_phase = DeviceField(t["phi"].view(nx, ny))
         self.momentum_velocity_lattice = DeviceField(t["u"].view(2, nx, ny), components=2)
         self.fluid_acceleration_lattice = DeviceField(t["a"].view(2, nx, ny), components=2)
         self.pressure_lattice = DeviceField(t["p"].view(nx, ny))

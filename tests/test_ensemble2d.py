@@ -48,7 +48,8 @@ def run_pair(storage, use_graph, steps):
     return ensemble, singles
 
 
-@pytest.mark.parametrize("storage,use_graph", [("fp32", True), ("fp16", True), ("fp32", False)])
+@pytest.mark.parametrize("storage,use_graph", [("fp32", True), ("fp16", True), ("fp16-rel-sr", True),
+                                               ("fp32", False)])
 def test_ensemble_cases_equal_standalone_runs(storage, use_graph):
     steps = 9  # four slow thermal updates (interval 2) and the melt-momentum closures
     ensemble, singles = run_pair(storage, use_graph, steps)
