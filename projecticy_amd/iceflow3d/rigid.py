@@ -184,7 +184,14 @@ class RigidBodies:
 
     # -- native plumbing -------------------------------------------------
     def _arrays(self):
-        """Contiguous FP64 views of the state for the native solver."""
+        """Contiguous FP64 views of the state for the native solver.  The
+        pointer struct is kept while every attribute is still the same
+        array object with its checked shape (in-place edits keep it valid)."""
+        cached = getattr(self, "_native_cache", None)
+        if cached is not None:
+            arrays, held = cached
+            if all(getattr(self, name) is value and value.shape == shape for name, value, shape in held):
+                return arrays
         count = len(np.atleast_1d(self.mass))
         shapes = {
             "origin": (count, 3), "q": (count, 4), "velocity": (count, 3),
@@ -205,6 +212,7 @@ class RigidBodies:
         arrays = _RigidArrays(count)
         for name in _RIGID_FIELDS:
             setattr(arrays, name, getattr(self, name).ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        self._native_cache = (arrays, [(name, getattr(self, name), shapes[name]) for name in _RIGID_FIELDS])
         return arrays
 
     def _limits(self):

@@ -45,6 +45,7 @@ class IceFlow3D:
             raise ValueError(f"at most {_native.MAX_BODIES} ice bodies are supported")
         self.device = _require_device()
         self._lib = lib()
+        self._step_stream = None
         self.config, self.scales = config, LatticeScales.from_iceflow_config(config)
         self.shape = (config.nx, config.ny, config.nz)
         self.nx, self.ny, self.nz = self.shape
@@ -182,15 +183,15 @@ class IceFlow3D:
     def _bodies_box(self):
         """Cells whose coverage can be nonzero for the uploaded body state:
         the rotated material box plus its trilinear support, with a margin."""
-        state = self._body_state
-        sizes = np.array([b.size_cells for b in self.config.bodies], dtype=np.float64)
+        state = self._body_state[: self.body_count].astype(np.float64)
         shape = np.array(self.shape)
-        lo, hi = shape.copy(), np.zeros(3, dtype=np.int64)
-        for b in range(self.body_count):
-            origin = state[b, 0:3].astype(np.float64)
-            extent = np.abs(state[b, 3:12].astype(np.float64).reshape(3, 3)) @ (0.5 * sizes[b] + 1.0)
-            lo = np.minimum(lo, np.floor(origin - extent).astype(np.int64) - 1)
-            hi = np.maximum(hi, np.ceil(origin + extent).astype(np.int64) + 2)
+        if not self.body_count:
+            return np.clip(shape, 0, shape), np.zeros(3, dtype=np.int64)
+        half = 0.5 * np.array([b.size_cells for b in self.config.bodies], dtype=np.float64) + 1.0
+        origin = state[:, 0:3]
+        extent = np.einsum("bij,bj->bi", np.abs(state[:, 3:12].reshape(-1, 3, 3)), half)
+        lo = (np.floor(origin - extent).astype(np.int64) - 1).min(axis=0)
+        hi = (np.ceil(origin + extent).astype(np.int64) + 2).max(axis=0)
         return np.clip(lo, 0, shape), np.clip(hi, 0, shape)
 
     def _raster_box(self):
@@ -295,8 +296,15 @@ class IceFlow3D:
         return out
 
     # --------------------------------------------------------------- plumbing
+    def _torch_stream(self):
+        """The stream every launch goes to: torch's current stream, fixed for
+        the length of one ``step`` call (so the loop asks torch once)."""
+        held = self._step_stream
+        return held[0] if held is not None else torch.cuda.current_stream(self.device)
+
     def _stream(self):
-        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        held = self._step_stream
+        return held[1] if held is not None else ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
     def _call(self, name, *args):
         check(getattr(self._lib, "icy3d_" + name)(self._gp, *args, self._stream()), name)
@@ -318,7 +326,7 @@ class IceFlow3D:
         self._upload_done.synchronize()  # the previous upload has left the staging buffer
         self._upload_staging.numpy()[...] = state
         self._t["body"].copy_(self._upload_staging, non_blocking=True)
-        self._upload_done.record(torch.cuda.current_stream(self.device))
+        self._upload_done.record(self._torch_stream())
 
     def _reduce_mass_state(self):
         self._call("reduce_mass")
@@ -633,6 +641,14 @@ class IceFlow3D:
         """simulator.py:823-850: one fused device launch sequence for the
         flow part, then host rigid bodies, geometry refresh and heat."""
         num_steps = integer("num_steps", num_steps, 0)
+        stream = torch.cuda.current_stream(self.device)
+        self._step_stream = (stream, ctypes.c_void_p(stream.cuda_stream))
+        try:
+            self._run_steps(num_steps)
+        finally:
+            self._step_stream = None
+
+    def _run_steps(self, num_steps):
         profile = getattr(self, "_profile", None)
         for _ in range(num_steps):
             refresh = not self._gradient_cache_valid
@@ -643,7 +659,7 @@ class IceFlow3D:
                 # while the GPU runs recovery and the gradients.
                 self._call("flow_step_begin", self._parity, int(refresh), classify)
                 self._loads_host.copy_(self._t["control"], non_blocking=True)
-                self._loads_ready.record(torch.cuda.current_stream(self.device))
+                self._loads_ready.record(self._torch_stream())
                 self._call("flow_step_end", self._parity)
                 self._loads_ready.synchronize()
                 control = Control.from_buffer_copy(self._loads_host.numpy().tobytes())
