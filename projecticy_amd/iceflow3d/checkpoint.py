@@ -1,0 +1,164 @@
+"""Version-2 restart files of ``IceFlow3D`` (iceflow3d/checkpoint.py:1-107).
+
+A checkpoint is a ZIP archive of ``.npy`` members, written to a temporary
+name and renamed into place, so a reader never sees a partial file:
+
+- ``metadata.npy``: a JSON string with ``format_version`` 2, the
+  configuration (``IceFlowConfig.to_dict()``), the SHA-256 of every source
+  file that defines the solver state, the step counters, the pending refill
+  cells, the initial water and phase volumes, and the lists of field and
+  rigid-state names;
+- ``initial_ice_mass_kg.npy``;
+- ``field/<name>.npy`` for every restart field (``IceFlow3D.fields()``:
+  device grids and per-body rows in their logical layouts, the control
+  scalars and the AA parity), read and written through the fields'
+  ``to_numpy``/``from_numpy``;
+- ``rigid/<name>.npy`` for every FP64 array of the host rigid state.
+
+Loading refuses another format version, a field schema that differs from the
+solver's, and (unless ``allow_source_change``) files written by other
+sources.  Every field and the rigid state come back bit for bit; the
+restored run then follows the uninterrupted one to rounding (per-body loads
+and the exchange-heat scatter add FP32 values with atomics, as in the
+reference).  The member names, metadata keys and gates are the reference's,
+so tools that list or read its restart files read these too.
+"""
+
+from dataclasses import asdict
+import hashlib
+import json
+from pathlib import Path
+import zipfile
+
+import numpy as np
+
+from .config import IceBodyConfig, IceFlowConfig, PhaseChangeProperties, ThermalConfig, _set
+
+FORMAT_VERSION = 2
+_PACKAGE = Path(__file__).resolve().parent
+# Sources that define what the state means: the Python layer and the kernels.
+_SOURCE_GLOBS = ((_PACKAGE, "*.py"), (_PACKAGE.parent / "csrc", "icy3d*"), (_PACKAGE.parent / "csrc", "rigid3d.cpp"),
+                 (_PACKAGE.parent.parent / "include", "icy3d.h"))
+
+
+def source_hashes():
+    """SHA-256 of each solver source file, by file name (checkpoint.py:14-18)."""
+    hashes = {}
+    for folder, pattern in _SOURCE_GLOBS:
+        for path in sorted(folder.glob(pattern)):
+            if path.is_file():
+                hashes[path.name] = hashlib.sha256(path.read_bytes()).hexdigest()
+    return hashes
+
+
+def _fields(simulation):
+    """Restart fields by name: every entry of ``simulation.fields()`` that can
+    be read and written (checkpoint.py:21-28)."""
+    return {
+        name: value
+        for name, value in simulation.fields().items()
+        if callable(getattr(value, "to_numpy", None)) and callable(getattr(value, "from_numpy", None))
+    }
+
+
+def _rigid_arrays(simulation):
+    return {name: value for name, value in vars(simulation.rigid).items() if isinstance(value, np.ndarray)}
+
+
+def save_checkpoint(simulation, path):
+    """Write every restart field and the host rigid state into an atomic ZIP
+    (checkpoint.py:31-70)."""
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    temporary = path.with_suffix(path.suffix + ".tmp")
+    fields = _fields(simulation)
+    rigid = _rigid_arrays(simulation)
+    metadata = {
+        "format_version": FORMAT_VERSION,
+        "configuration": simulation.config.to_dict(),
+        "source_sha256": source_hashes(),
+        "steps": int(simulation.steps),
+        "thermal_pending": int(simulation._thermal_pending),
+        "pending_cells": int(simulation.pending_cells),
+        "initial_water_volume_m3": float(simulation.initial_water_volume_m3),
+        "initial_phase_volume_m3": float(simulation.initial_phase_volume_m3),
+        "fields": list(fields),
+        "rigid": list(rigid),
+    }
+    with zipfile.ZipFile(temporary, "w", compression=zipfile.ZIP_STORED, allowZip64=True) as archive:
+
+        def member(name, array):
+            with archive.open(name + ".npy", "w", force_zip64=True) as stream:
+                np.lib.format.write_array(stream, np.asarray(array), allow_pickle=False)
+
+        member("metadata", np.array(json.dumps(metadata)))
+        member("initial_ice_mass_kg", simulation.initial_ice_mass_kg)
+        for name, value in fields.items():
+            member("field/" + name, value.to_numpy())
+        for name, value in rigid.items():
+            member("rigid/" + name, value)
+    temporary.replace(path)
+
+
+def config_from_dict(values, *, backend=None):
+    """The ``IceFlowConfig`` that ``to_dict`` described.  Stored floats are
+    kept exactly: the body quaternions are not normalised a second time."""
+    values = dict(values)
+    thermal = dict(values["thermal"])
+    thermal["properties"] = PhaseChangeProperties(**thermal["properties"])
+    values["thermal"] = ThermalConfig(**thermal)
+    bodies = []
+    for stored in values["ice_bodies"]:
+        body = IceBodyConfig(**stored)
+        _set(body, "quaternion_wxyz", tuple(float(v) for v in stored["quaternion_wxyz"]))
+        bodies.append(body)
+    values["ice_bodies"] = tuple(bodies)
+    if backend is not None:
+        values["backend"] = backend
+    return IceFlowConfig(**values)
+
+
+def read_metadata(path):
+    """The metadata of a checkpoint, after the format-version gate."""
+    with np.load(path, allow_pickle=False) as archive:
+        metadata = json.loads(str(archive["metadata"]))
+    if metadata.get("format_version") != FORMAT_VERSION:
+        raise ValueError("unsupported checkpoint format: HOME requires a version-2 moment/AA checkpoint")
+    return metadata
+
+
+def load_checkpoint(path, *, backend=None, allow_source_change=False):
+    """A new ``IceFlow3D`` holding the saved state (checkpoint.py:73-107)."""
+    from .simulator import IceFlow3D
+
+    metadata = read_metadata(path)
+    same_source = metadata["source_sha256"] == source_hashes()
+    if not same_source and not allow_source_change:
+        raise ValueError("checkpoint source differs; explicit allow_source_change required")
+    config = config_from_dict(metadata["configuration"], backend=backend)
+    with np.load(path, allow_pickle=False) as archive:
+        sim = IceFlow3D(_without_warmup(config))
+        sim.config = config
+        if set(metadata["fields"]) != set(_fields(sim)):
+            raise ValueError("checkpoint field schema differs from the solver")
+        fields = _fields(sim)
+        for name in metadata["fields"]:
+            fields[name].from_numpy(archive["field/" + name])
+        for name in metadata["rigid"]:
+            setattr(sim.rigid, name, archive["rigid/" + name].copy())
+        sim.steps = int(metadata["steps"])
+        sim._thermal_pending = int(metadata["thermal_pending"])
+        sim.pending_cells = int(metadata["pending_cells"])
+        sim.initial_water_volume_m3 = float(metadata["initial_water_volume_m3"])
+        sim.initial_phase_volume_m3 = float(metadata["initial_phase_volume_m3"])
+        sim.initial_ice_mass_kg = archive["initial_ice_mass_kg"].copy()
+    # A file from other sources may hold a gradient that precedes its final
+    # geometry update: rebuild it once (checkpoint.py:103-106).
+    sim._restored(keep_gradients=same_source)
+    return sim
+
+
+def _without_warmup(config):
+    values = asdict(config)
+    values["phase_warmup_steps"] = 0
+    return config_from_dict(values)

@@ -345,6 +345,22 @@ class IceFlow3D:
         """Call after externally editing phase, coverage, or fluid/solid masks."""
         self._gradient_cache_valid = False
 
+    def _restored(self, keep_gradients):
+        """After every field was written from a restart file: the host copy of
+        the uploaded body rows follows the device; labels, coverage and
+        previous labels were saved together with those rows, so the next
+        refresh covers the union of the bodies' boxes as in the saved run
+        (whose last refresh used the same rows), and the link classes are
+        rebuilt from the labels.  The saved phase gradient is the cache the
+        saved run would have used next (it may precede the last geometry
+        update outside the refreshed box), so it is kept when the file comes
+        from these sources and rebuilt otherwise."""
+        self._body_state[...] = self._t["body"].cpu().numpy()
+        self._flags_valid = False
+        self._raster_full = False
+        self._raster_region = self._bodies_box()
+        self._gradient_cache_valid = bool(keep_gradients)
+
     @property
     def physical_time_s(self):
         return self.steps * self.scales.dt_s

@@ -1,0 +1,90 @@
+"""3D restart files (iceflow3d/checkpoint.py; the reference's round trip in
+iceflow3d/tests/test_checkpoint.py:13-61): every restart field and the host
+rigid state survive a save/load bit for bit, and the restored run continues
+across a thermal update, with melting and a geometry remap, like the
+uninterrupted one.  As in the reference, per-body loads, the exchange-heat
+scatter and the mass reduction add FP32 values with atomics, so two runs
+agree to rounding, not bit for bit: the continuation is held to the
+reference's tolerance (rtol 2e-6, atol 1e-9)."""
+
+import json
+import zipfile
+
+import numpy as np
+import pytest
+
+from projecticy_amd.iceflow3d import IceBodyConfig, IceFlowConfig
+from projecticy_amd.iceflow3d.checkpoint import (FORMAT_VERSION, _fields, config_from_dict, load_checkpoint,
+                                                 read_metadata, save_checkpoint, source_hashes)
+from projecticy_amd.iceflow3d.workloads import dam_break_four_ice
+
+
+def small_config():
+    return IceFlowConfig(nx=20, ny=16, nz=12, domain_size_m=(0.02, 0.016, 0.012), boundary_cells=1,
+                         water_width=8, water_height=12, floor_water_depth_cells=5,
+                         ice_bodies=(IceBodyConfig((14, 5, 6), (4, 4, 4)),), phase_warmup_steps=0)
+
+
+def write_archive(path, metadata):
+    with zipfile.ZipFile(path, "w") as archive:
+        with archive.open("metadata.npy", "w") as stream:
+            np.lib.format.write_array(stream, np.array(json.dumps(metadata)), allow_pickle=False)
+
+
+@pytest.mark.parametrize("config", [small_config(), dam_break_four_ice(64, 32, 32, cube_cells=4, seed=7)])
+def test_configuration_survives_json(config):
+    assert config_from_dict(json.loads(json.dumps(config.to_dict()))) == config
+
+
+def test_source_hashes_cover_python_and_kernels():
+    hashes = source_hashes()
+    for name in ("simulator.py", "checkpoint.py", "icy3d_flow.hip", "rigid3d.cpp", "icy3d.h"):
+        assert len(hashes[name]) == 64
+    assert hashes == source_hashes()
+
+
+def test_other_format_versions_are_refused(tmp_path):
+    path = tmp_path / "old.npz"
+    write_archive(path, {"format_version": 1})
+    with pytest.raises(ValueError, match="version-2"):
+        read_metadata(path)
+    with pytest.raises(ValueError, match="version-2"):
+        load_checkpoint(path)
+
+
+def test_changed_sources_need_explicit_permission(tmp_path):
+    path = tmp_path / "foreign.npz"
+    hashes = dict(source_hashes(), **{"simulator.py": "0" * 64})
+    write_archive(path, {"format_version": FORMAT_VERSION, "source_sha256": hashes,
+                         "configuration": small_config().to_dict()})
+    with pytest.raises(ValueError, match="allow_source_change"):
+        load_checkpoint(path)
+
+
+@pytest.mark.gpu
+def test_roundtrip_includes_pending_heat_and_host_body_state(tmp_path):
+    from projecticy_amd.iceflow3d import IceFlow3D
+
+    sim = IceFlow3D(small_config())
+    sim.step(5)
+    path = tmp_path / "checkpoint.npz"
+    save_checkpoint(sim, path)
+    restored = load_checkpoint(path)
+    assert restored.steps == 5
+    assert restored._thermal_pending == 5
+    assert restored.config == sim.config
+    for name, field in _fields(sim).items():
+        np.testing.assert_array_equal(_fields(restored)[name].to_numpy(), field.to_numpy(), err_msg=name)
+    for name, value in vars(sim.rigid).items():
+        if isinstance(value, np.ndarray):
+            np.testing.assert_array_equal(getattr(restored.rigid, name), value, err_msg=name)
+    # Cross a thermal update (interval 8), with melting and a geometry remap.
+    sim.step(4)
+    restored.step(4)
+    assert restored._thermal_pending == sim._thermal_pending
+    for name in ("water_phase", "velocity_lattice", "water_energy", "material_mass", "body_center", "home", "h"):
+        np.testing.assert_allclose(getattr(restored, name).to_numpy(), getattr(sim, name).to_numpy(), rtol=2e-6,
+                                   atol=1e-9, err_msg=name)
+    for name in ("origin", "q", "velocity", "omega", "mass"):
+        np.testing.assert_allclose(getattr(restored.rigid, name), getattr(sim.rigid, name), rtol=2e-6, atol=1e-9,
+                                   err_msg=name)
