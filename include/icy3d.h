@@ -201,6 +201,19 @@ int icy3d_rigid_contact_residuals(const icy3d_rigid_arrays* s, const int shape[3
                                   double residual[2]);
 int icy3d_rigid_update_mass(const icy3d_rigid_arrays* s, const float* reduced);
 int icy3d_rigid_pack_state(const icy3d_rigid_arrays* s, float* out);
+/* Cells whose coverage can be nonzero for packed rows `state` (simulator.py:
+ * 288-323's bounding box: the rotated material box plus its trilinear
+ * support, a margin of one cell below and two above), clamped to the grid;
+ * sizes: [body_count][3] material cells.  lo = shape, hi = 0 without bodies. */
+int icy3d_rigid_coverage_box(const float* state, int body_count, const int32_t* sizes, const int shape[3],
+                             int32_t lo[3], int32_t hi[3]);
+/* One rigid update of the flow loop in a single host call (rigid.py:178-347
+ * then simulator.py:259-286): icy3d_rigid_integrate, _resolve_contacts,
+ * _pack_state into `state` and _coverage_box of the packed rows.  Returns the
+ * first failing stage's code (failed_body / residual as those stages set). */
+int icy3d_rigid_step(const icy3d_rigid_arrays* s, const float* loads, const double gravity[3], double linear_damping,
+                     double angular_damping, const int shape[3], int boundary, int iterations, const int32_t* sizes,
+                     float* state, int32_t box_lo[3], int32_t box_hi[3], int* failed_body, double residual[2]);
 
 /* Size of the structs this library was built with (ABI check). */
 int icy3d_abi_sizes(int32_t* grid_bytes, int32_t* control_bytes);

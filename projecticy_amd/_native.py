@@ -147,8 +147,11 @@ def lib():
                                                      ctypes.POINTER(ctypes.c_double)]
     handle.icy3d_rigid_update_mass.argtypes = [rigid, _p]
     handle.icy3d_rigid_pack_state.argtypes = [rigid, _p]
+    handle.icy3d_rigid_coverage_box.argtypes = [_p, _i, _p, ctypes.POINTER(_i), _p, _p]
+    handle.icy3d_rigid_step.argtypes = [rigid, _p, _p, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_i), _i, _i,
+                                        _p, _p, _p, _p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
     for name in ("rigid_integrate", "rigid_resolve_contacts", "rigid_contact_residuals", "rigid_update_mass",
-                 "rigid_pack_state"):
+                 "rigid_pack_state", "rigid_coverage_box", "rigid_step"):
         getattr(handle, "icy3d_" + name).restype = ctypes.c_int
     sizes = (_i(), _i())
     handle.icy3d_abi_sizes(ctypes.byref(sizes[0]), ctypes.byref(sizes[1]))

@@ -616,4 +616,36 @@ int icy3d_rigid_pack_state(const icy3d_rigid_arrays* s, float* out) {
   return 0;
 }
 
+int icy3d_rigid_coverage_box(const float* state, int body_count, const int32_t* sizes, const int shape[3],
+                             int32_t lo[3], int32_t hi[3]) {
+  if (!state || !sizes || body_count < 0 || body_count > ICY3D_MAX_BODIES) return ICY3D_ERR_ARGUMENT;
+  int64_t low[3] = {shape[0], shape[1], shape[2]}, high[3] = {0, 0, 0};
+  for (int b = 0; b < body_count; ++b) {
+    const float* row = state + ICY3D_STATE_STRIDE * b;
+    for (int a = 0; a < 3; ++a) {
+      // |R| (0.5 size + 1), the rotated half extent plus the trilinear support
+      double extent = 0.0;
+      for (int c = 0; c < 3; ++c) extent += std::fabs(double(row[3 + 3 * a + c])) * (0.5 * double(sizes[3 * b + c]) + 1.0);
+      const double origin = double(row[a]);
+      low[a] = std::min<int64_t>(low[a], int64_t(std::floor(origin - extent)) - 1);
+      high[a] = std::max<int64_t>(high[a], int64_t(std::ceil(origin + extent)) + 2);
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = int32_t(std::min<int64_t>(std::max<int64_t>(low[a], 0), shape[a]));
+    hi[a] = int32_t(std::min<int64_t>(std::max<int64_t>(high[a], 0), shape[a]));
+  }
+  return 0;
+}
+
+int icy3d_rigid_step(const icy3d_rigid_arrays* s, const float* loads, const double gravity[3], double linear_damping,
+                     double angular_damping, const int shape[3], int boundary, int iterations, const int32_t* sizes,
+                     float* state, int32_t box_lo[3], int32_t box_hi[3], int* failed_body, double residual[2]) {
+  int rc = icy3d_rigid_integrate(s, loads, gravity, linear_damping, angular_damping, failed_body);
+  if (!rc) rc = icy3d_rigid_resolve_contacts(s, shape, boundary, iterations, residual);
+  if (!rc) rc = icy3d_rigid_pack_state(s, state);
+  if (!rc) rc = icy3d_rigid_coverage_box(state, s->body_count, sizes, shape, box_lo, box_hi);
+  return rc;
+}
+
 }  // extern "C"

@@ -273,6 +273,51 @@ class RigidBodies:
             )
         check(code, "rigid integration")
 
+    def _step_arguments(self):
+        """ctypes arguments of the fused native step that depend only on the
+        configuration, built once per configuration object."""
+        s = self.s
+        cached = getattr(self, "_step_cache", None)
+        if cached is not None and cached[0] is s.config:
+            return cached[1]
+        gravity = np.array(s.config.gravity, dtype=float) * s.scales.dt_s**2 / s.scales.dx_m
+        shape, boundary, iterations = self._limits()
+        sizes = np.ascontiguousarray([b.size_cells for b in s.config.bodies], dtype=np.int32).reshape(-1, 3)
+        lo, hi = np.zeros(3, np.int32), np.zeros(3, np.int32)
+        args = {
+            "gravity": gravity, "sizes": sizes, "lo": lo, "hi": hi, "shape": shape, "boundary": boundary,
+            "iterations": iterations, "linear": ctypes.c_double(s.config.linear_damping),
+            "angular": ctypes.c_double(s.config.angular_damping), "failed": ctypes.c_int(-1),
+            "residual": (ctypes.c_double * 2)(),
+            "pointers": [ctypes.c_void_p(a.ctypes.data) for a in (gravity, sizes, lo, hi)],
+        }
+        self._step_cache = (s.config, args)
+        return args
+
+    def advance_packed(self, loads, state):
+        """``integrate(loads)`` and ``resolve_contacts()`` in one native call
+        that also packs the FP32 device rows into ``state`` (float32
+        [B, 21], C-contiguous) and returns the cells whose coverage can be
+        nonzero for them, ``(lo, hi)`` (int32, clamped to the grid)."""
+        from .._native import check, lib
+
+        args = self._step_arguments()
+        arrays = self._arrays()
+        gravity, sizes, lo, hi = args["pointers"]
+        loads = np.ascontiguousarray(loads, dtype=np.float32)
+        code = lib().icy3d_rigid_step(
+            ctypes.byref(arrays), ctypes.c_void_p(loads.ctypes.data), gravity, args["linear"], args["angular"],
+            args["shape"], args["boundary"], args["iterations"], sizes, ctypes.c_void_p(state.ctypes.data),
+            lo, hi, ctypes.byref(args["failed"]), args["residual"])
+        if code == -4:
+            raise RuntimeError(
+                f"rigid displacement exceeds half a cell at step {self.s.steps + 1}, body {args['failed'].value}"
+            )
+        if code in (-5, -6):
+            _contact_error(code, self.s.steps, args["residual"])
+        check(code, "rigid step")
+        return args["lo"].copy(), args["hi"].copy()
+
     def resolve_contacts(self):
         from .._native import lib
 

@@ -129,7 +129,12 @@ class IceFlow3D:
         self._loads_host = torch.zeros(ctypes.sizeof(Control), dtype=torch.uint8).pin_memory()
         self._loads_ready = torch.cuda.Event()
         self._upload_staging = torch.zeros((B, _native.STATE_STRIDE), dtype=torch.float32).pin_memory()
+        self._upload_staging_view = self._upload_staging.numpy()
         self._upload_done = torch.cuda.Event()
+        # The per-body loads inside the pinned copy of the control block.
+        loads0 = Control.loads.offset
+        self._loads_view = self._loads_host.numpy()[loads0 : loads0 + 24 * _native.MAX_BODIES].view(
+            np.float32).reshape(_native.MAX_BODIES, 6)[:B]
         g = Grid()
         g.nx, g.ny, g.nz = self.shape
         g.boundary, g.bodies = cfg.boundary_cells, B
@@ -194,10 +199,11 @@ class IceFlow3D:
         hi = (np.ceil(origin + extent).astype(np.int64) + 2).max(axis=0)
         return np.clip(lo, 0, shape), np.clip(hi, 0, shape)
 
-    def _raster_box(self):
+    def _raster_box(self, current=None):
         """Union of the bodies' current and previous boxes: every cell whose
         label, coverage or previous label can change in this refresh."""
-        current = self._bodies_box()
+        if current is None:
+            current = self._bodies_box()
         if self._raster_full or self._raster_region is None:
             box = (np.zeros(3, dtype=np.int64), np.array(self.shape))
             self._raster_full = False
@@ -316,6 +322,18 @@ class IceFlow3D:
     def _write_control(self, control):
         raw = np.frombuffer(bytes(control), dtype=np.uint8)
         self._t["control"].copy_(torch.from_numpy(raw.copy()))
+
+    def _rigid_step(self, loads):
+        """rigid.py:178-347 and the upload of simulator.py:259-286 in one
+        native call: the packed rows go straight into the pinned staging
+        buffer and are queued for upload; returns their coverage box."""
+        self._upload_done.synchronize()  # the previous upload has left the staging buffer
+        staging = self._upload_staging_view
+        box = self.rigid.advance_packed(loads, staging)
+        self._body_state[...] = staging
+        self._t["body"].copy_(self._upload_staging, non_blocking=True)
+        self._upload_done.record(self._torch_stream())
+        return box
 
     def _upload_rigid_state(self, state):
         """rigid.py:136-151 device side: one small host-to-device copy,
@@ -549,9 +567,10 @@ class IceFlow3D:
             self._raise_unstable(control.invalid_index)
         return float(control.maximum_speed)
 
-    def _refresh_geometry(self, phase_changed=True):
-        """simulator.py:708-736; the refill loop reads one counter per sweep."""
-        box = self._raster_box()
+    def _refresh_geometry(self, phase_changed=True, current=None):
+        """simulator.py:708-736; the refill loop reads one counter per sweep.
+        ``current``: the bodies' coverage box when the caller has it."""
+        box = self._raster_box(current)
         self._rasterize(box)
         self._repair_phase_layout(box)
         self._call("find_refill_changes", int3(box[0]), int3(box[1]))
@@ -678,22 +697,19 @@ class IceFlow3D:
                 self._loads_ready.record(self._torch_stream())
                 self._call("flow_step_end", self._parity)
                 self._loads_ready.synchronize()
-                control = Control.from_buffer_copy(self._loads_host.numpy().tobytes())
+                loads = self._loads_view
             else:
                 check(self._lib.icy3d_flow_step_profiled(self._gp, self._parity, int(refresh), classify, profile,
                                                          self._stream()), "flow_step")
-                control = self._read_control()
+                loads = np.array(self._read_control().loads, dtype=np.float32)[: self.body_count]
                 check(self._lib.icy3d_profile_collect(profile), "profile_collect")
             self._gradient_cache_valid = True
             self._flags_valid = True
             self._parity = 1 - self._parity
-            loads = np.array(control.loads, dtype=np.float32)[: self.body_count]
-            self.rigid.integrate(loads)
-            self.rigid.resolve_contacts()
-            self.rigid.upload()
+            current = self._rigid_step(loads)
             # The geometry refresh's first control read follows the recovery's
             # stability scan in stream order.
-            control = self._refresh_geometry()
+            control = self._refresh_geometry(current=current)
             speed_bound = None
             if self.config.check_stability:
                 if control.invalid_index < math.prod(self.shape):

@@ -265,3 +265,61 @@ def test_native_integration_matches_oracle_euler_update():
     r.integrate(loads)
     for name, value in expected.items():
         np.testing.assert_allclose(getattr(r, name), value, rtol=1e-12, atol=1e-14, err_msg=name)
+
+
+def seeded_bodies(seed, count=4):
+    """Bodies near the walls and each other of a 48^3 grid, so that the
+    contact stage has work to do."""
+    rng = np.random.default_rng(seed)
+    r = RigidBodies.__new__(RigidBodies)
+    sizes = rng.integers(4, 10, (count, 3))
+    r.s = SimpleNamespace(shape=(48, 40, 36), body_count=count, steps=0, scales=SimpleNamespace(dt_s=1e-5, dx_m=1e-3),
+                          config=SimpleNamespace(gravity=(0, -9.8, 0), linear_damping=0.99, angular_damping=0.98,
+                                                 boundary_cells=1, body_contact_iterations=32,
+                                                 bodies=[SimpleNamespace(size_cells=tuple(int(v) for v in s))
+                                                         for s in sizes]))
+    r.origin = rng.uniform(4, 32, (count, 3))
+    r.q = rng.normal(size=(count, 4))
+    r.q /= np.linalg.norm(r.q, axis=1)[:, None]
+    r.velocity = rng.normal(0, 0.02, (count, 3))
+    r.omega = rng.normal(0, 0.004, (count, 3))
+    r.com = rng.normal(0, 0.2, (count, 3))
+    r.mass = sizes.prod(axis=1).astype(float)
+    r.inertia = np.einsum("b,ij->bij", r.mass * 4.0, np.eye(3))
+    r.lower, r.upper = -0.5 * sizes.astype(float), 0.5 * sizes.astype(float)
+    return r, sizes, rng.normal(0, 0.05, (count, 6)).astype(np.float32)
+
+
+def coverage_box(state, sizes, shape):
+    """The bounding box of simulator.py:288-323 from packed rows, restated."""
+    lo, hi = np.array(shape, dtype=np.int64), np.zeros(3, dtype=np.int64)
+    for row, size in zip(state.astype(np.float64), sizes):
+        extent = np.abs(row[3:12].reshape(3, 3)) @ (0.5 * size + 1.0)
+        lo = np.minimum(lo, np.floor(row[0:3] - extent).astype(np.int64) - 1)
+        hi = np.maximum(hi, np.ceil(row[0:3] + extent).astype(np.int64) + 2)
+    return np.clip(lo, 0, shape), np.clip(hi, 0, shape)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4, 5, 6])
+def test_fused_rigid_step_equals_the_separate_calls(seed):
+    """icy3d_rigid_step = integrate, resolve_contacts, pack_state (bit for
+    bit) plus the coverage box of the packed rows."""
+    fused, sizes, loads = seeded_bodies(seed)
+    separate, _, _ = seeded_bodies(seed)
+    state = np.zeros((len(sizes), 21), np.float32)
+    lo, hi = fused.advance_packed(loads, state)
+    separate.integrate(loads)
+    separate.resolve_contacts()
+    np.testing.assert_array_equal(state, separate.device_state())
+    for name in ("origin", "q", "velocity", "omega"):
+        np.testing.assert_array_equal(getattr(fused, name), getattr(separate, name), err_msg=name)
+    want_lo, want_hi = coverage_box(state, sizes, fused.s.shape)
+    np.testing.assert_array_equal(lo, want_lo)
+    np.testing.assert_array_equal(hi, want_hi)
+
+
+def test_fused_rigid_step_reports_travel_like_integrate():
+    r, sizes, loads = seeded_bodies(9)
+    r.velocity[2] = [0.6, 0.0, 0.0]
+    with pytest.raises(RuntimeError, match="rigid displacement exceeds half a cell at step 1, body 2"):
+        r.advance_packed(loads, np.zeros((len(sizes), 21), np.float32))
