@@ -11,7 +11,8 @@ The JSON line adds:
   roofline      the dominant kernel group of the flow step: algorithmic HBM
                 bytes per launch / its mean duration (HIP events on the
                 solver stream, a profiled pass after the timed region),
-                against the 8 TB/s MI355X HBM peak;
+                against the 8 TB/s MI355X HBM peak, with the counted HBM
+                bytes and the counted VALU busy fraction beside it;
   cpu_baseline  the FP64 CPU oracle (sequential C, one core) timed on one
                 step of the same workload after its initialisation;
   d2q9          the coupled D2Q9 step on the 1024x2048 falling-ice case
@@ -57,6 +58,13 @@ STEP_BYTES = 161  # minimal persistent state per cell-update of the whole step
 # HBM bytes per launch of a kernel group, from the FETCH_SIZE (x2.0) and
 # WRITE_SIZE (x1.0) counters calibrated on a copy kernel of known size
 # (rocprofv3 --pmc runs of this script; the summary is kept under profiles/).
+# Fraction of the chip's unpacked FP32 VALU issue slots each kernel group uses,
+# from SQ_ACTIVE_INST_VALU and GRBM_GUI_ACTIVE (rocprofv3 --pmc runs of this
+# script; the summary is kept under profiles/).
+MEASURED_VALU_BUSY = {
+    "phase_collide": 0.378, "forces": 0.367, "home_collide": 0.166,
+    "home_stream": 0.413, "phase_recover": 0.490, "gradients": 0.223,
+}
 MEASURED_TRAFFIC_BYTES = {
     "phase_collide": 3.652e8, "forces": 6.284e8, "home_collide": 5.248e8,
     "home_stream": 9.060e8, "phase_recover": 1.489e8, "gradients": 1.056e8,
@@ -381,6 +389,8 @@ def main():
         groups[name] = {"mean_ms": round(mean_ms, 5), "algorithmic_GBs": round(gbs, 1)}
         if name in MEASURED_TRAFFIC_BYTES and mean_ms > 0:
             groups[name]["hbm_GBs_counted"] = round(MEASURED_TRAFFIC_BYTES[name] / (mean_ms * 1e-3) / 1e9, 1)
+        if name in MEASURED_VALU_BUSY:
+            groups[name]["valu_busy_counted"] = MEASURED_VALU_BUSY[name]
     dominant = max(groups, key=lambda k: groups[k]["mean_ms"])
     achieved = groups[dominant]["algorithmic_GBs"]
     roofline = {
@@ -393,6 +403,7 @@ def main():
         "kernel": dominant,
         "bytes_per_cell": KERNEL_BYTES[dominant],
         "cells_per_launch": cells,
+        "valu_busy": MEASURED_VALU_BUSY.get(dominant),
     }
     result = {
         "metric": "D3Q27 MLUPS + % HBM roofline; ensemble cases/hour at <=1% melt-mass error",
