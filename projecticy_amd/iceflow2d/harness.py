@@ -1,0 +1,429 @@
+"""The 2D falling-and-melting ice harness
+(iceflow2d/examples/coupled_falling_ice_melting_2d.py:98-1460).
+
+    python -m projecticy_amd.iceflow2d --end-time-s 1.0 --save-npz
+
+A square ice block, rotated, dropped above a full-width hot-water pool.  The
+flags, their defaults and checks, the geometry they derive and the
+configuration, the sample schedule (whole thermal updates), and the output
+files are the reference's, so a script that drives ``python -m iceflow2d``
+drives this module unchanged:
+
+- ``history.csv``: one row per sample, the reference's columns;
+- ``fields.npz`` (``--save-npz``): every sample's fields and diagnostics;
+- ``metadata.json``: model, closed-bath reference (the equilibrium
+  temperature if all the ice melts), requested values, geometry, lattice
+  scaling, configuration, results, sequences and outputs;
+- unless ``--no-plot``: velocity, vorticity and temperature frame sequences
+  with GIFs, and the final plot ``coupled_falling_ice_melting_2d.png``.
+"""
+
+import argparse
+import csv
+import json
+import math
+import sys
+from pathlib import Path
+
+from . import reporting
+from .config import (DEFAULT_REFERENCE_VELOCITY_M_S, LatticeScales, PhaseChangeProperties, ThermalBoundarySet,
+                     ThermalConfig, create_iceflow_config)
+
+DEFAULT_OUTPUT_DIR = "iceflow2d/outputs/iceflow2d/coupled_falling_ice_melting_2d"
+DEFAULT_BOUNDARY_CELLS = 3
+DEFAULT_THERMAL_UPDATE_INTERVAL = 8
+DEFAULT_VELOCITY_VISUALIZATION_MAX_M_S = 0.1
+DEFAULT_VORTICITY_VISUALIZATION_MAX_S_1 = 250.0
+PLOT_NAME = "coupled_falling_ice_melting_2d.png"
+
+
+def build_parser():
+    """The reference's flags (coupled_falling_ice_melting_2d.py:98-219)."""
+    p = argparse.ArgumentParser(
+        description="Air-water LBM with a freely falling and melting ice block in a hot-water bath")
+    p.add_argument("--domain-width-m", type=float, default=0.025)
+    p.add_argument("--domain-height-m", type=float, default=0.050)
+    p.add_argument("--resolution-x", type=int, default=100)
+    p.add_argument("--resolution-y", type=int, default=200)
+    p.add_argument("--water-level-m", type=float, default=0.030)
+    p.add_argument("--ice-size-m", type=float, default=0.008)
+    p.add_argument("--ice-angle-degrees", type=float, default=5.0)
+    p.add_argument("--drop-height-m", type=float, default=0.0015,
+                   help="air gap between the water surface and the lowest rotated ice corner")
+    p.add_argument("--boundary-cells", type=int, default=DEFAULT_BOUNDARY_CELLS)
+    p.add_argument("--initial-horizontal-speed-lattice", type=float, default=0.0,
+                   help="initial horizontal rigid-body speed in cells per LBM step")
+    p.add_argument("--initial-vertical-speed-lattice", type=float, default=0.0,
+                   help="initial vertical rigid-body speed in cells per LBM step")
+    p.add_argument("--reference-velocity-m-s", type=float, default=DEFAULT_REFERENCE_VELOCITY_M_S,
+                   help="physical speed represented by the fixed reference speed 0.1 LU")
+    p.add_argument("--end-time-s", type=float, default=10.0)
+    p.add_argument("--output-interval-s", type=float, default=0.01,
+                   help="physical-time spacing of history rows and field-sequence frames")
+    p.add_argument("--velocity-visualization-max-m-s", type=float, default=DEFAULT_VELOCITY_VISUALIZATION_MAX_M_S)
+    p.add_argument("--vorticity-visualization-max-s-1", type=float, default=DEFAULT_VORTICITY_VISUALIZATION_MAX_S_1)
+    p.add_argument("--water-temperature-c", type=float, default=90.0)
+    p.add_argument("--ice-temperature-c", type=float, default=0.0)
+    p.add_argument("--air-temperature-c", type=float, default=20.0)
+    p.add_argument("--melting-temperature-c", type=float, default=0.0)
+    p.add_argument("--gravity-m-s2", type=float, default=9.8)
+    p.add_argument("--thermal-expansion-water-1-k", type=float, default=2.1e-4,
+                   help="constant expansion coefficient in the simplified linear Boussinesq law")
+    p.add_argument("--density-water", type=float, default=1000.0)
+    p.add_argument("--density-ice", type=float, default=917.0)
+    p.add_argument("--density-air", type=float, default=1.25)
+    p.add_argument("--kinematic-viscosity-water", type=float, default=1.0e-6)
+    p.add_argument("--kinematic-viscosity-air", type=float, default=1.5e-5)
+    p.add_argument("--surface-tension", type=float, default=0.072)
+    p.add_argument("--water-specific-heat", type=float, default=4186.0)
+    p.add_argument("--ice-specific-heat", type=float, default=2100.0)
+    p.add_argument("--air-specific-heat", type=float, default=1005.0)
+    p.add_argument("--water-conductivity", type=float, default=0.60)
+    p.add_argument("--ice-conductivity", type=float, default=2.20)
+    p.add_argument("--air-conductivity", type=float, default=0.026)
+    p.add_argument("--latent-heat", type=float, default=334000.0)
+    p.add_argument("--phase-warmup-steps", type=int, default=500)
+    p.add_argument("--thermal-update-interval", type=int, default=DEFAULT_THERMAL_UPDATE_INTERVAL,
+                   help="LBM steps per conduction/phase-change update; pose ALE and water advection remain "
+                        "coupled on every LBM step")
+    p.add_argument("--interface-width", type=float, default=5.0)
+    p.add_argument("--mobility", type=float, default=0.1)
+    p.add_argument("--no-advection", action="store_true")
+    p.add_argument("--output-dir", default=DEFAULT_OUTPUT_DIR)
+    p.add_argument("--no-plot", action="store_true",
+                   help="disable final plot and velocity/vorticity/temperature sequences")
+    p.add_argument("--save-npz", action=argparse.BooleanOptionalAction, default=False,
+                   help="retain sampled arrays and write fields.npz")
+    p.add_argument("--progress", action=argparse.BooleanOptionalAction, default=True,
+                   help="show the default LBM-step progress bar")
+    p.add_argument("--quiet", action="store_true", help="suppress progress and run summaries")
+    return p
+
+
+def parse_args(argv=None):
+    """Arguments that describe a valid case; otherwise the parser exits with
+    the error (coupled_falling_ice_melting_2d.py:222-247)."""
+    parser = build_parser()
+    args = parser.parse_args(argv)
+    try:
+        if args.resolution_x <= 0 or args.resolution_y <= 0:
+            raise ValueError("resolution components must be positive")
+        if args.boundary_cells < 1:
+            raise ValueError("boundary_cells must be positive")
+        if args.phase_warmup_steps < 0:
+            raise ValueError("phase_warmup_steps must be non-negative")
+        if args.thermal_update_interval < 1:
+            raise ValueError("thermal_update_interval must be positive")
+        reporting._positive("end_time_s", args.end_time_s)
+        reporting._positive("output_interval_s", args.output_interval_s)
+        reporting._positive("velocity_visualization_max_m_s", args.velocity_visualization_max_m_s)
+        reporting._positive("vorticity_visualization_max_s_1", args.vorticity_visualization_max_s_1)
+        derive_geometry(args)
+    except ValueError as exc:
+        parser.error(str(exc))
+    return args
+
+
+def _cells(name, length_m, spacing_m):
+    cells = reporting._positive(name, length_m) / spacing_m
+    if not math.isclose(cells, round(cells), rel_tol=0.0, abs_tol=1.0e-9):
+        raise ValueError(f"{name} must align with lattice-cell faces")
+    return int(round(cells))
+
+
+def _bin_fraction(count, total):
+    """The fraction whose cell rounding gives ``count`` of ``total`` cells."""
+    return (int(count) + 0.5) / int(total)
+
+
+def derive_geometry(args):
+    """The full-width pool and the rotated block's placement
+    (coupled_falling_ice_melting_2d.py:262-342)."""
+    nx, ny, boundary = int(args.resolution_x), int(args.resolution_y), int(args.boundary_cells)
+    if nx <= 2 * boundary + 4 or ny <= 2 * boundary + 4:
+        raise ValueError("resolution is too small for the requested boundary layers")
+    dx = reporting._positive("domain_width_m", args.domain_width_m) / nx
+    dy = reporting._positive("domain_height_m", args.domain_height_m) / ny
+    if not math.isclose(dx, dy, rel_tol=1.0e-12, abs_tol=1.0e-15):
+        raise ValueError("the falling/melting example requires square cells")
+    water = _cells("water_level_m", args.water_level_m, dx)
+    ice_w, ice_h = _cells("ice_size_m", args.ice_size_m, dx), _cells("ice_size_m", args.ice_size_m, dy)
+    drop = reporting._non_negative("drop_height_m", args.drop_height_m) / dx
+    angle = math.radians(float(args.ice_angle_degrees))
+    if not math.isfinite(angle):
+        raise ValueError("ice_angle_degrees must be finite")
+    if ice_w < 4 or ice_h < 4:
+        raise ValueError("ice must span at least four cells in each direction")
+    if water <= boundary + 1 or water >= ny - boundary:
+        raise ValueError("water level must leave both an active pool and an air cap")
+    clearance = max(2, int(math.ceil(float(args.interface_width))))
+    if drop < clearance:
+        raise ValueError("drop height must keep the ice outside the diffuse water interface "
+                         f"(at least {clearance} cells)")
+    half_w, half_h = 0.5 * ice_w, 0.5 * ice_h
+    cos_a, sin_a = abs(math.cos(angle)), abs(math.sin(angle))
+    extent_x, extent_y = cos_a * half_w + sin_a * half_h, sin_a * half_w + cos_a * half_h
+    center_x = (nx - ice_w) // 2 + half_w
+    lowest = water + drop
+    center_y = lowest + extent_y
+    if center_x - extent_x < boundary or center_x + extent_x > nx - boundary:
+        raise ValueError("rotated ice overlaps a side wall")
+    if center_y + extent_y > ny - boundary:
+        raise ValueError("ice and drop height overlap the top wall")
+    velocity = (float(args.initial_horizontal_speed_lattice), float(args.initial_vertical_speed_lattice))
+    if not all(math.isfinite(v) for v in velocity):
+        raise ValueError("initial ice velocity must be finite")
+    return {
+        "dx_m": dx, "water_width": nx - boundary, "water_height": water,
+        "water_width_fraction": _bin_fraction(nx - boundary, nx), "water_height_fraction": _bin_fraction(water, ny),
+        "ice_width": ice_w, "ice_height": ice_h, "ice_width_fraction": _bin_fraction(ice_w, nx),
+        "ice_height_fraction": _bin_fraction(ice_h, ny), "ice_angle_rad": angle, "ice_base_y_cells": center_y - half_h,
+        "ice_center_x_cells": center_x, "ice_center_y_cells": center_y, "ice_lowest_y_cells": lowest,
+        "ice_highest_y_cells": center_y + extent_y, "drop_height_cells": drop, "interface_clearance_cells": clearance,
+    }
+
+
+def create_config(args=None):
+    """The moving body-ALE thermal configuration (coupled_falling_ice_melting_2d.py:345-423)."""
+    args = parse_args([]) if args is None else args
+    geometry = derive_geometry(args)
+    properties = PhaseChangeProperties(
+        melting_temperature_c=args.melting_temperature_c, specific_heat_water_j_kg_k=args.water_specific_heat,
+        specific_heat_ice_j_kg_k=args.ice_specific_heat, specific_heat_air_j_kg_k=args.air_specific_heat,
+        conductivity_water_w_m_k=args.water_conductivity, conductivity_ice_w_m_k=args.ice_conductivity,
+        conductivity_air_w_m_k=args.air_conductivity, latent_heat_j_kg=args.latent_heat)
+    thermal = ThermalConfig(
+        properties=properties, boundaries=ThermalBoundarySet(), initial_water_temperature_c=args.water_temperature_c,
+        initial_ice_temperature_c=args.ice_temperature_c, initial_air_temperature_c=args.air_temperature_c,
+        advection_enabled=not args.no_advection, water_air_interface_adiabatic=True,
+        update_interval_lbm_steps=int(args.thermal_update_interval), solid_liquid_threshold=0.5,
+        max_fourier_number=0.15, max_courant_number=0.50, water_buoyancy_model="linear",
+        thermal_expansion_water_1_k=args.thermal_expansion_water_1_k,
+        buoyancy_reference_temperature_c=args.water_temperature_c, moving_body_scheme="body_ale")
+    config = create_iceflow_config(
+        resolution=(int(args.resolution_x), int(args.resolution_y)), dx=float(geometry["dx_m"]),
+        reference_velocity=reporting._positive("reference_velocity_m_s", args.reference_velocity_m_s),
+        rho_water=args.density_water, rho_air=args.density_air, rho_ice=args.density_ice,
+        viscosity_water=args.kinematic_viscosity_water, viscosity_air=args.kinematic_viscosity_air,
+        sigma=args.surface_tension, gravity=(0.0, -reporting._non_negative("gravity_m_s2", args.gravity_m_s2)),
+        interface_width=args.interface_width, mobility=args.mobility, phase_warmup_steps=args.phase_warmup_steps,
+        boundary_cells=int(args.boundary_cells), thermal=thermal, well_balanced_hydrostatics=True,
+        water_width_fraction=float(geometry["water_width_fraction"]),
+        water_height_fraction=float(geometry["water_height_fraction"]),
+        ice_width_fraction=float(geometry["ice_width_fraction"]),
+        ice_height_fraction=float(geometry["ice_height_fraction"]),
+        ice_base_y_cells=float(geometry["ice_base_y_cells"]),
+        ice_initial_velocity=(args.initial_horizontal_speed_lattice, args.initial_vertical_speed_lattice),
+        ice_initial_angle=float(geometry["ice_angle_rad"]), ice_fixed=False, volume_projection_tolerance=1.0e-8,
+        volume_projection_max_shift=0.50, air_interface_relaxation_time=0.8, output_dir=str(args.output_dir),
+        show_gui=False, save_npz=False)
+    for name in ("water_width", "water_height", "ice_width", "ice_height"):
+        if getattr(config, name) != geometry[name]:
+            raise RuntimeError("derived geometry disagrees with config discretization")
+    return config
+
+
+def visualization_limits(args):
+    """Colour scales shared by every frame (reporting.py:122-155)."""
+    low = float(args.melting_temperature_c)
+    if not math.isfinite(low) or not math.isfinite(float(args.water_temperature_c)):
+        raise ValueError("visualization temperatures must be finite")
+    high = float(args.water_temperature_c)
+    if high < low:
+        raise ValueError("temperature visualization maximum must not be below the melting temperature")
+    return {"velocity_max_m_s": reporting._positive("velocity_visualization_max_m_s",
+                                                    args.velocity_visualization_max_m_s),
+            "vorticity_abs_max_s_1": reporting._positive("vorticity_visualization_max_s_1",
+                                                         args.vorticity_visualization_max_s_1),
+            "temperature_min_c": low, "temperature_max_c": high}
+
+
+def progress_chunk_steps(total_lbm_steps):
+    """About two thousand progress updates for a whole run."""
+    return max(1, math.ceil(max(0, int(total_lbm_steps)) / 2000))
+
+
+def advance_to(sim, target, progress, *, chunk_steps):
+    """Step to ``target`` in bounded chunks (coupled_falling_ice_melting_2d.py:1239-1260)."""
+    current = int(sim.steps)
+    if int(target) < current:
+        raise ValueError("target_step must not precede the current simulation step")
+    while current < target:
+        count = min(max(1, int(chunk_steps)), int(target) - current)
+        sim.step(count)
+        if int(sim.steps) != current + count:
+            raise RuntimeError("simulation step counter did not advance as requested")
+        progress.advance(count)
+        current += count
+
+
+def _sequence_entry(summary, units_key):
+    if summary is None:
+        return None
+    entry = {"directory": summary["directory"], "pattern": summary["pattern"], "frames": summary["frames"],
+             "animation": summary["animation"], "units": summary["units"]}
+    if units_key == "velocity":
+        entry.update(observed_max=summary["observed"][1], color_max=summary["color"][1])
+    elif units_key == "vorticity":
+        observed = [v for v in summary["observed"] if v is not None]
+        entry.update(observed_abs_max=max((abs(v) for v in observed), default=None), color_abs_max=summary["color"][1])
+    else:
+        entry.update(observed_range=summary["observed"], color_range=summary["color"])
+    return entry
+
+
+def metadata(args, config, scales, targets, initial, final, count, sequences):
+    """The ``metadata.json`` document (coupled_falling_ice_melting_2d.py:1040-1222)."""
+    props = config.thermal.properties
+    after_full_melt = initial.total_enthalpy_j_m - initial.body_mass_kg_m * props.latent_heat_j_kg
+    equilibrium = None
+    if after_full_melt >= 0.0 and initial.thermal_total_mass_kg_m > 0.0:
+        equilibrium = props.melting_temperature_c + after_full_melt / (
+            initial.thermal_total_mass_kg_m * props.specific_heat_water_j_kg_k)
+    pair = lambda a, b: [getattr(final, a), getattr(final, b)]  # noqa: E731
+    results = {
+        "snapshots": count, "lbm_steps": final.lbm_steps, "physical_time_s": final.physical_time_s,
+        "melted_fraction": final.melted_fraction, "initial_body_mass_lattice": initial.body_mass_lattice,
+        "final_body_mass_lattice": final.body_mass_lattice, "final_body_active": final.body_active,
+        "final_body_center_m": pair("body_center_x_m", "body_center_y_m"),
+        "final_body_velocity_m_s": pair("body_velocity_x_m_s", "body_velocity_y_m_s"),
+        "final_body_angle_rad": final.body_angle_rad,
+        "body_solid_center_local_cells": pair("body_solid_center_local_x_cells", "body_solid_center_local_y_cells"),
+        "cumulative_melted_mass_lattice": final.cumulative_melted_mass_lattice,
+    }
+    for name in ("cumulative_melted_momentum", "cumulative_fluid_melt_momentum", "melt_momentum_residual"):
+        results[name + "_lattice"] = pair(name + "_x_lattice", name + "_y_lattice")
+    for name in ("cumulative_melted_angular_momentum_lattice", "cumulative_fluid_melt_angular_momentum_lattice",
+                 "melt_angular_momentum_residual_lattice", "water_volume_residual_cells", "energy_residual_j_m",
+                 "ale_water_residual_cells", "ale_energy_residual_j_m", "phase_aperture_water_residual_cells",
+                 "phase_aperture_energy_residual_j_m", "phase_aperture_capacity_margin_cells",
+                 "melt_injection_mass_residual_kg_m", "thermal_initial_total_mass_kg_m", "thermal_total_mass_kg_m",
+                 "thermal_total_mass_residual_kg_m", "water_mean_temperature_c", "aperture_energy_correction_abs_j_m"):
+        results[name] = getattr(final, name)
+    sequence = {kind: _sequence_entry(sequences.get(kind), kind) for kind in ("velocity", "vorticity", "temperature")}
+    return {
+        "model": "coupled two-dimensional falling-ice melting",
+        "backend": "hip-mi355x",
+        "hydrodynamics": "air-water phase-field D2Q9 LBM",
+        "thermal_solver": "conservative finite-volume enthalpy in water and body coordinates",
+        "mechanics": "freely translating and rotating, thermally eroding rigid ice body",
+        "moving_body_scheme": config.thermal.moving_body_scheme,
+        "phase_change_mass_conversion": "delta_V_water=(rho_ice/rho_water)*(V_solid_initial-V_solid_current)",
+        "closed_bath_reference": {
+            "all_walls_adiabatic": all(getattr(config.thermal.boundaries, side).kind == "adiabatic"
+                                       for side in ("left", "right", "bottom", "top")),
+            "initial_energy_j_m": initial.total_enthalpy_j_m,
+            "full_melt_added_water_area_m2": initial.body_mass_kg_m / config.rho_water,
+            "melt_water_to_ice_volume_ratio": config.rho_ice / config.rho_water,
+            "enough_heat_for_full_melt": equilibrium is not None,
+            "fully_melted_equilibrium_temperature_c": equilibrium,
+        },
+        "assumptions": [
+            "liquid water remains liquid at 90 degC",
+            "no evaporation, boiling, or water-vapour transport",
+            "water-air thermal interface is adiabatic",
+            "local temperature reconstruction with bounded global sensible-energy correction",
+            "linear Boussinesq water buoyancy over the 0--90 degC range",
+            "two-dimensional quantities are reported per unit out-of-plane depth",
+        ],
+        "requested": {"end_time_s": float(args.end_time_s), "output_interval_s": float(args.output_interval_s),
+                      "water_temperature_c": float(args.water_temperature_c), "save_npz": bool(args.save_npz),
+                      "progress_enabled": bool(args.progress and not args.quiet)},
+        "geometry": {"domain_width_m": config.nx * config.dx, "domain_height_m": config.ny * config.dx,
+                     "full_width_pool": config.water_width == config.nx - config.boundary_cells,
+                     "water_level_m": config.water_height * config.dx, "ice_width_m": config.ice_width * config.dx,
+                     "ice_height_m": config.ice_height * config.dx, "drop_height_m": float(args.drop_height_m),
+                     "ice_initial_angle_rad": config.ice_initial_angle},
+        "lattice_scaling": {"dx_m": scales.dx_m, "dt_s": scales.dt_s, "velocity_scale_m_s": scales.velocity_scale_m_s,
+                            "scheduled_snapshot_steps": targets,
+                            "thermal_update_interval_lbm_steps": config.thermal.update_interval_lbm_steps},
+        "config": config.to_dict(),
+        "results": results,
+        "field_sequences": sequence,
+        "field_output": {"write_mode": "disabled" if args.no_plot else "streaming_png_gif",
+                         "raw_fields_retained": bool(args.save_npz),
+                         "snapshot_arrays_retained_in_memory": bool(args.save_npz)},
+        "outputs": {
+            "history": "history.csv", "fields": "fields.npz" if args.save_npz else None,
+            "plot": None if args.no_plot else PLOT_NAME,
+            **{f"{kind}_{what}": (None if sequence[kind] is None else sequence[kind][key])
+               for kind in ("velocity", "vorticity", "temperature")
+               for what, key in (("frames", "directory"), ("animation", "animation"))},
+        },
+    }
+
+
+def run_case(args, config, targets):
+    """Run the case and write its outputs (coupled_falling_ice_melting_2d.py:1263-1452)."""
+    from .simulator import IceFlow2D
+
+    output = Path(args.output_dir)
+    output.mkdir(parents=True, exist_ok=True)
+    show = bool(args.progress and not args.quiet)
+    if not args.quiet:
+        print("Initializing the MI355X solver state...", file=sys.stderr, flush=True)
+    sim = IceFlow2D(config)
+    scales = LatticeScales.from_iceflow_config(config)
+    initial_enthalpy = float(sim.thermal.total_enthalpy_j_m(sim.wall_mask))
+    initial_solid = float(sim.phase_change_solid_volume_cells())
+    initial = reporting.capture_sample(sim, initial_total_enthalpy_j_m=initial_enthalpy,
+                                       initial_solid_volume_cells=initial_solid)
+    final, count = initial, 0
+    kept = [] if args.save_npz else None
+    limits = visualization_limits(args)
+    streams = {} if args.no_plot else reporting.sequences(output, config, limits)
+    summaries = {}
+    total = targets[-1] if targets else 0
+    chunk = progress_chunk_steps(total)
+    try:
+        with (output / "history.csv").open("w", newline="", encoding="utf-8") as stream, \
+                reporting.Progress(total, enabled=show, label="Coupled falling-ice melting", unit="LBM steps") as progress:
+            writer = csv.writer(stream)
+            writer.writerow(reporting.HISTORY_COLUMNS)
+            for frame, target in enumerate(targets):
+                if frame == 0:
+                    sample = initial
+                else:
+                    advance_to(sim, target, progress, chunk_steps=chunk)
+                    sample = reporting.capture_sample(
+                        sim, initial_total_enthalpy_j_m=initial_enthalpy, initial_solid_volume_cells=initial_solid,
+                        initial_total_mass_kg_m=initial.thermal_initial_total_mass_kg_m)
+                writer.writerow(reporting.history_row(sample))
+                stream.flush()
+                if kept is not None:
+                    kept.append(sample)
+                for writer_ in streams.values():
+                    writer_.append(sample)
+                count += 1
+                final = sample
+                progress.set_status(f"snapshot {count}/{len(targets)} | t={sample.physical_time_s:.4f} s | "
+                                    f"melted={100.0 * sample.melted_fraction:.2f}%")
+                if frame > 0 and not args.quiet and not show:
+                    print(f"t={sample.physical_time_s:.4f} s | steps={sample.lbm_steps} | "
+                          f"y_body={sample.body_center_y_m:.5f} m | melted={100.0 * sample.melted_fraction:6.2f}%",
+                          flush=True)
+    finally:
+        for kind, writer_ in streams.items():
+            summaries[kind] = writer_.close()
+    if kept is not None:
+        reporting.write_fields_npz(output / "fields.npz", config, kept)
+    if not args.no_plot:
+        reporting.write_final_plot(output / PLOT_NAME, config, final, temperature_max_c=limits["temperature_max_c"])
+    (output / "metadata.json").write_text(
+        json.dumps(metadata(args, config, scales, targets, initial, final, count, summaries), indent=2),
+        encoding="utf-8")
+    if not args.quiet:
+        print(f"Coupled falling-ice melting example complete (T_inf={args.water_temperature_c:g} degC)")
+        print(f"  simulated: {final.physical_time_s:.6f} s in {final.lbm_steps} LBM steps")
+        print(f"  ice melted: {100.0 * final.melted_fraction:.3f}%")
+        print(f"  output: {output}")
+    return {"output_directory": str(output), "snapshots": count, "lbm_steps": final.lbm_steps,
+            "physical_time_s": final.physical_time_s, "melted_fraction": final.melted_fraction,
+            **{f"{kind}_frames": (summaries[kind]["frames"] if kind in summaries else 0)
+               for kind in ("velocity", "vorticity", "temperature")}}
+
+
+def main(argv=None):
+    args = parse_args(argv)
+    config = create_config(args)
+    return run_case(args, config, reporting.snapshot_step_targets(config, args.end_time_s, args.output_interval_s))

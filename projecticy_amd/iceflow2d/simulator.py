@@ -171,6 +171,9 @@ class Thermal2D:
             lambda: sim._read_control().transfer[1] - sim._read_control().aperture[1])
         self.melt_injection_mass_residual_kg_m = DerivedValue(
             lambda: sim._read_control().interval[1] - sim._read_control().interval[0])
+        # simulator.py:1775-1776: full aperture capacity minus the volume after the transfer
+        self.phase_aperture_capacity_margin_m2 = DerivedValue(
+            lambda: sim._read_control().aperture[3] - sim._read_control().transfer[0])
 
     def synchronize_water_aperture(self, *args):
         self._sim._synchronize_water_aperture()
@@ -437,6 +440,13 @@ class IceFlow2D:
             - self._read_control().melted[2])
         self.thermal = Thermal2D(self)
         self._moving_body_fraction = self.thermal.world_body_solid_fraction
+        # simulator.py:1997-2018: the aperture remap's residuals in cells
+        area = self.config.dx * self.config.dx
+        self.ale_water_residual_cells = DerivedValue(lambda: self.thermal.phase_aperture_volume_residual_m2[None] / area)
+        self.ale_energy_residual_j_m = DerivedValue(lambda: self.thermal.phase_aperture_energy_residual_j_m[None])
+        self.phase_aperture_water_residual_cells = self.ale_water_residual_cells
+        self.phase_aperture_capacity_margin_cells = DerivedValue(
+            lambda: self.thermal.phase_aperture_capacity_margin_m2[None] / area)
         self.physical_velocity_lattice = DerivedValue(self._physical_velocity_numpy)
 
     # ------------------------------------------------------ views
@@ -502,6 +512,14 @@ class IceFlow2D:
     @property
     def physical_time_s(self):
         return self.steps * self._time_step_s
+
+    def phase_change_solid_volume_cells(self):
+        """simulator.py:4144-4145, 4734-4738: the body's solid volume in cells."""
+        return float(self._read_control().mass * (self.config.rho_water / self.config.rho_ice))
+
+    def phase_change_geometry_volume_cells(self):
+        """simulator.py:4147-4148, 4135-4141: the cells the solid mask marks."""
+        return float(torch.count_nonzero(self._t["solid"]).item())
 
     def sample_thermal_fields(self):
         """Host snapshot of temperature, enthalpy and liquid fraction (simulator.py:1694-1765)."""
