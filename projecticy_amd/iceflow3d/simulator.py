@@ -204,12 +204,13 @@ class IceFlow3D:
         label, coverage or previous label can change in this refresh."""
         if current is None:
             current = self._bodies_box()
+        current = (tuple(int(v) for v in current[0]), tuple(int(v) for v in current[1]))
         if self._raster_full or self._raster_region is None:
-            box = (np.zeros(3, dtype=np.int64), np.array(self.shape))
+            box = ((0, 0, 0), tuple(self.shape))
             self._raster_full = False
         else:
             last = self._raster_region
-            box = (np.minimum(current[0], last[0]), np.maximum(current[1], last[1]))
+            box = (tuple(map(min, current[0], last[0])), tuple(map(max, current[1], last[1])))
         self._raster_region = current
         return box
 
@@ -376,7 +377,7 @@ class IceFlow3D:
         self._body_state[...] = self._t["body"].cpu().numpy()
         self._flags_valid = False
         self._raster_full = False
-        self._raster_region = self._bodies_box()
+        self._raster_region = tuple(tuple(int(v) for v in side) for side in self._bodies_box())
         self._gradient_cache_valid = bool(keep_gradients)
 
     @property
@@ -436,9 +437,9 @@ class IceFlow3D:
         self._flags_valid = False
 
     def _repair_phase_layout(self, box=None):
-        lo, hi = (np.zeros(3, dtype=int), np.array(self.shape)) if box is None else box
+        lo, hi = ((0, 0, 0), self.shape) if box is None else box
         # A link (I, I+e) is swapped by the thread of I: extend one cell down.
-        self._call("repair_phase_layout", self._parity, int3(np.maximum(np.asarray(lo) - 1, 0)), int3(hi))
+        self._call("repair_phase_layout", self._parity, int3(max(int(v) - 1, 0) for v in lo), int3(hi))
 
     def _synchronize_water(self):
         self._call("synchronize_water")
@@ -575,10 +576,13 @@ class IceFlow3D:
         self._repair_phase_layout(box)
         self._call("find_refill_changes", int3(box[0]), int3(box[1]))
         control = self._read_control()
+        # After this read the GPU waits for the host: the box arithmetic is on
+        # plain ints (three components), not small NumPy arrays.
         remaining, covered = int(control.openings), int(control.covered)
-        shape = np.array(self.shape)
-        lower, upper = np.array(control.refill_lower), np.array(control.refill_upper)
-        box_lo, box_hi = np.maximum(lower - 1, 0), np.minimum(upper + 2, shape)
+        shape = self.shape
+        lower, upper = tuple(control.refill_lower), tuple(control.refill_upper)
+        box_lo = tuple(max(v - 1, 0) for v in lower)
+        box_hi = tuple(min(v + 2, n) for v, n in zip(upper, shape))
         if remaining > 0:
             self._call("seed_refill_region", int3(box_lo), int3(box_hi))
         has_openings = remaining > 0
@@ -593,15 +597,16 @@ class IceFlow3D:
             remaining = new
         self._call("commit_refill", int3(box[0]), int3(box[1]))
         # gradients.py:388-400: changed box plus the mirror dependency radius.
-        geo_lo, geo_hi = np.array(control.geometry_lower), np.array(control.geometry_upper)
-        lo, hi = np.minimum(lower, geo_lo), np.maximum(upper, geo_hi)
-        if np.all(hi >= lo):
-            lo, hi = np.maximum(lo - 6, 0), np.minimum(hi + 7, shape)
+        lo = tuple(map(min, lower, control.geometry_lower))
+        hi = tuple(map(max, upper, control.geometry_upper))
+        if all(h >= l for l, h in zip(lo, hi)):
+            lo = tuple(max(v - 6, 0) for v in lo)
+            hi = tuple(min(v + 7, n) for v, n in zip(hi, shape))
             self._call("refill_gradients", int3(lo), int3(hi))
         # Labels changed only inside the refreshed box, which was reclassified.
         self._flags_valid = True
         if has_openings or covered:
-            self._call("project_refill", self._parity, int3(lower), int3(upper + 1))
+            self._call("project_refill", self._parity, int3(lower), int3(v + 1 for v in upper))
         if phase_changed or has_openings or covered:
             self._synchronize_water()
         return control
