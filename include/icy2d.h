@@ -25,6 +25,7 @@
 #ifndef ICY2D_H
 #define ICY2D_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -259,6 +260,11 @@ int icy2d_graph_begin(void* stream);
 int icy2d_graph_end(void* stream, void** graph);
 int icy2d_graph_launch(void* graph, void* stream);
 int icy2d_graph_destroy(void* graph);
+/* Stream-ordered copy of ``bytes`` between host (pinned) and device memory in
+ * either direction: the ensemble's per-case value uploads and control-block
+ * readbacks, one call each instead of a framework copy.  No counterpart in
+ * the reference. */
+int icy2d_copy_async(void* dst, const void* src, size_t bytes, void* stream);
 
 #ifdef __cplusplus
 }

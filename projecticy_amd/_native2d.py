@@ -124,6 +124,8 @@ def lib():
     handle.icy2d_graph_end.argtypes = [_p, ctypes.POINTER(_p)]
     handle.icy2d_graph_launch.argtypes = [_p, _p]
     handle.icy2d_graph_destroy.argtypes = [_p]
+    handle.icy2d_copy_async.argtypes = [_p, _p, ctypes.c_size_t, _p]
+    handle.icy2d_copy_async.restype = ctypes.c_int
     for name in ("graph_begin", "graph_end", "graph_launch", "graph_destroy"):
         getattr(handle, "icy2d_" + name).restype = ctypes.c_int
     sizes = (_i(), _i(), _i())

@@ -2613,6 +2613,11 @@ int icy2d_graph_destroy(void* graph) {
   if (!graph) return 0;
   return (int)hipGraphExecDestroy((hipGraphExec_t)graph);
 }
+int icy2d_copy_async(void* dst, const void* src, size_t bytes, void* stream) {
+  if (!bytes) return 0;
+  if (!dst || !src) return ICY2D_ERR_ARGUMENT;
+  return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, S(stream));
+}
 
 void icy2d_abi_sizes(int32_t* grid, int32_t* control) {
   if (grid) *grid = (int32_t)sizeof(icy2d_grid);

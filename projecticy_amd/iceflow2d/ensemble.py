@@ -94,6 +94,12 @@ class IceFlowEnsemble2D:
         self._fast_dt = torch.zeros(n, dtype=torch.float64, device=self.device)
         self._fast_dt_host = torch.zeros(n, dtype=torch.float64, pin_memory=True)
         self._fast_dt_ptr = ctypes.c_void_p(self._fast_dt.data_ptr())
+        # Uploads and readbacks go through icy2d_copy_async on NumPy views of
+        # the pinned buffers: after a host read the GPU waits for the host,
+        # and a framework copy costs tens of microseconds of Python each.
+        self._copy_async = self._lib.icy2d_copy_async
+        self._values_np, self._masks_np = self._values_host.numpy(), self._masks_host.numpy()
+        self._controls_np, self._fast_dt_np = self._controls_host.numpy(), self._fast_dt_host.numpy()
         self.use_graph = bool(use_graph)
         self._graphs = {}
         self.steps = 0
@@ -163,10 +169,10 @@ class IceFlowEnsemble2D:
             self._value_slot = 0
         k = self._value_slot
         self._value_slot += 1
-        self._values_host[k].copy_(torch.as_tensor(np.asarray(values, dtype=np.float64)))
-        with torch.cuda.stream(self._stream):
-            self._values[k].copy_(self._values_host[k], non_blocking=True)
-        return ctypes.c_void_p(self._values[k].data_ptr())
+        self._values_np[k] = values
+        device = self._values[k].data_ptr()
+        self._copy(device, self._values_host[k].data_ptr(), self._values_np[k].nbytes)
+        return ctypes.c_void_p(device)
 
     def _upload_mask(self, active):
         if self._mask_slot == self._masks.shape[0]:
@@ -174,20 +180,25 @@ class IceFlowEnsemble2D:
             self._mask_slot = 0
         k = self._mask_slot
         self._mask_slot += 1
-        self._masks_host[k].copy_(torch.as_tensor(np.asarray(active, dtype=np.int32)))
-        with torch.cuda.stream(self._stream):
-            self._masks[k].copy_(self._masks_host[k], non_blocking=True)
-        return self._masks[k].data_ptr()
+        self._masks_np[k] = active
+        device = self._masks[k].data_ptr()
+        self._copy(device, self._masks_host[k].data_ptr(), self._masks_np[k].nbytes)
+        return device
+
+    def _copy(self, dst, src, nbytes):
+        """Stream-ordered copy between pinned host and device memory."""
+        rc = self._copy_async(ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes, self._s)
+        if rc != 0:
+            raise native.IcyError(rc, "icy2d_copy_async")
 
     def _read_controls(self):
         """All control blocks with one copy (a record array, one record per
         case, fields named as in icy2d_control); raises a pending
         device-side check."""
-        with torch.cuda.stream(self._stream):
-            self._controls_host.copy_(self._controls, non_blocking=True)
+        self._copy(self._controls_host.data_ptr(), self._controls.data_ptr(), self._controls_np.nbytes)
         self._stream.synchronize()
         self._value_slot = self._mask_slot = 0
-        raw = self._controls_host.numpy()
+        raw = self._controls_np
         controls = raw.view(CONTROL_DTYPE).reshape(self.ncases).copy().view(np.recarray)
         if controls.error_flags.any():
             for i in np.flatnonzero(controls.error_flags):
@@ -273,9 +284,8 @@ class IceFlowEnsemble2D:
         if max(counts) == 1:
             # every case takes one substep: the fixed sequence runs as a graph
             # that reads the per-case time step from a persistent array
-            self._fast_dt_host.copy_(torch.as_tensor(subs, dtype=torch.float64))
-            with torch.cuda.stream(self._stream):
-                self._fast_dt.copy_(self._fast_dt_host, non_blocking=True)
+            self._fast_dt_np[:] = subs
+            self._copy(self._fast_dt.data_ptr(), self._fast_dt_host.data_ptr(), self._fast_dt_np.nbytes)
             self._replay("fast", self._one_substep)
             return
         self._synchronize_water_aperture()
