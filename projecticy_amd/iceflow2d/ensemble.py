@@ -100,6 +100,11 @@ class IceFlowEnsemble2D:
         self._copy_async = self._lib.icy2d_copy_async
         self._values_np, self._masks_np = self._values_host.numpy(), self._masks_host.numpy()
         self._controls_np, self._fast_dt_np = self._controls_host.numpy(), self._fast_dt_host.numpy()
+        # Copies of the control blocks taken between queued projection
+        # evaluations, so several evaluations share one host read.
+        self._snapshots = torch.zeros((3, n, size), dtype=torch.uint8, device=self.device)
+        self._snapshots_host = torch.zeros((3, n, size), dtype=torch.uint8, pin_memory=True)
+        self._snapshots_np = self._snapshots_host.numpy()
         self.use_graph = bool(use_graph)
         self._graphs = {}
         self.steps = 0
@@ -435,8 +440,34 @@ class IceFlowEnsemble2D:
             controls = self._read_controls()
             return controls.water[:, 1].tolist(), controls.water[:, 2].tolist()
 
-        lower = evaluate([-v for v in limit], need)[0]
-        upper = evaluate(limit, need)[0]
+        def evaluate_queued(requests):
+            """Evaluations that do not depend on each other's results, one host
+            read for all: each one's control blocks are copied aside on the
+            device before the next is launched."""
+            nbytes = self._controls_np.nbytes
+            for slot, (lams, live) in enumerate(requests):
+                self._call("evaluate_projection", self._upload_values(lams), active=None if all(live) else live)
+                self._copy(self._snapshots[slot].data_ptr(), self._controls.data_ptr(), nbytes)
+            self._copy(self._snapshots_host.data_ptr(), self._snapshots.data_ptr(), nbytes * len(requests))
+            self._stream.synchronize()
+            self._value_slot = self._mask_slot = 0
+            out = []
+            for slot in range(len(requests)):
+                water = self._snapshots_np[slot].view(CONTROL_DTYPE).reshape(n).view(np.recarray).water
+                out.append((water[:, 1].tolist(), water[:, 2].tolist()))
+            return out
+
+        iterations = [case.config.volume_projection_max_iterations for case in self.cases]
+        # The Newton search starts at lambda = 0 for every case it searches;
+        # that evaluation needs nothing from the bracket ones, so all three go
+        # to the device before the host reads any of them.
+        first_search = [int(need[i] and 0 < iterations[i]) for i in range(n)]
+        requests = [([-v for v in limit], need), (list(limit), need)]
+        if any(first_search):
+            requests.append(([0.0] * n, first_search))
+        results = evaluate_queued(requests)
+        lower, upper = results[0][0], results[1][0]
+        first_evaluation = results[2] if any(first_search) else None
         for i in range(n):
             if not need[i]:
                 continue
@@ -450,12 +481,11 @@ class IceFlowEnsemble2D:
         lo, hi = [-v for v in limit], list(limit)
         lam, best_lam, best = [0.0] * n, [0.0] * n, initial
         searching = list(need)
-        iterations = [case.config.volume_projection_max_iterations for case in self.cases]
         for k in range(max(iterations)):
             searching = [int(searching[i] and k < iterations[i]) for i in range(n)]
             if not any(searching):
                 break
-            mass, derivative = evaluate(lam, searching)
+            mass, derivative = first_evaluation if k == 0 else evaluate(lam, searching)
             for i in range(n):
                 if not searching[i]:
                     continue
