@@ -105,6 +105,17 @@ class IceFlowEnsemble2D:
         self._snapshots = torch.zeros((3, n, size), dtype=torch.uint8, device=self.device)
         self._snapshots_host = torch.zeros((3, n, size), dtype=torch.uint8, pin_memory=True)
         self._snapshots_np = self._snapshots_host.numpy()
+        # The projection's bracket and starting multipliers are per-case
+        # constants: on the device once, not uploaded every step.
+        limit = [4.0 * c.volume_projection_max_shift / c.interface_width for c in configs]
+        self._lambda_constants = torch.tensor([[-v for v in limit], limit, [0.0] * n], dtype=torch.float64,
+                                              device=self.device)
+        self._mask_cache = {}
+        # per-case constants of the advection substep rule, as arrays
+        self._dt = np.array([case._time_step_s for case in self.cases], dtype=np.float64)
+        self._courant = np.array([c.thermal.max_courant_number for c in configs], dtype=np.float64)
+        self._advective = np.array([bool(c.thermal.advection_enabled) for c in configs])
+        self._max_substeps = np.array([c.thermal.max_substeps_per_update for c in configs], dtype=np.int64)
         self.use_graph = bool(use_graph)
         self._graphs = {}
         self.steps = 0
@@ -180,14 +191,20 @@ class IceFlowEnsemble2D:
         return ctypes.c_void_p(device)
 
     def _upload_mask(self, active):
+        key = bytes(np.asarray(active, dtype=np.int8))
+        cached = self._mask_cache.get(key)
+        if cached is not None:  # the same mask since the last host read: its slot still holds it
+            return cached
         if self._mask_slot == self._masks.shape[0]:
             self._stream.synchronize()
             self._mask_slot = 0
+            self._mask_cache.clear()
         k = self._mask_slot
         self._mask_slot += 1
         self._masks_np[k] = active
         device = self._masks[k].data_ptr()
         self._copy(device, self._masks_host[k].data_ptr(), self._masks_np[k].nbytes)
+        self._mask_cache[key] = device
         return device
 
     def _copy(self, dst, src, nbytes):
@@ -203,6 +220,7 @@ class IceFlowEnsemble2D:
         self._copy(self._controls_host.data_ptr(), self._controls.data_ptr(), self._controls_np.nbytes)
         self._stream.synchronize()
         self._value_slot = self._mask_slot = 0
+        self._mask_cache.clear()
         raw = self._controls_np
         controls = raw.view(CONTROL_DTYPE).reshape(self.ncases).copy().view(np.recarray)
         if controls.error_flags.any():
@@ -273,19 +291,25 @@ class IceFlowEnsemble2D:
 
     def _advance_thermal_fast(self, controls):
         """simulator.py:2233-2247 and :418-453 per case."""
-        counts, subs = [], []
-        for i, (case, speed) in enumerate(zip(self.cases, controls.max_outflow.tolist())):
-            if not speed >= 0.0:
-                raise ValueError(f"ensemble case {i}: maximum_outflow_rate_lattice must be finite")
-            th, dt = case.config.thermal, case._time_step_s
-            load = 0.0
-            if th.advection_enabled and speed != 0.0:
-                limit = th.max_courant_number * dt / speed
-                if math.isfinite(limit):
-                    load = dt / limit
-            count = case._substeps(load, "advection", dt, speed)
-            counts.append(count)
-            subs.append(dt / count)
+        # The per-case rule of IceFlow2D (courant * dt / speed, then dt / limit,
+        # then ceil), elementwise in float64: the same IEEE operations in the
+        # same order, so each case takes exactly its standalone substeps.
+        speed = np.asarray(controls.max_outflow, dtype=np.float64)
+        bad = ~(speed >= 0.0)
+        if bad.any():
+            i = int(np.flatnonzero(bad)[0])
+            raise ValueError(f"ensemble case {i}: maximum_outflow_rate_lattice must be finite")
+        dt = self._dt
+        with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+            limit = self._courant * dt / speed
+            load = np.where(self._advective & (speed != 0.0) & np.isfinite(limit), dt / limit, 0.0)
+        counts_array = np.maximum(1, np.ceil(load - 1.0e-14)).astype(np.int64)
+        over = counts_array > self._max_substeps
+        if over.any():
+            i = int(np.flatnonzero(over)[0])
+            self.cases[i]._substeps(float(load[i]), "advection", float(dt[i]), float(speed[i]))
+        counts = counts_array.tolist()
+        subs = (dt / counts_array).tolist()
         if max(counts) == 1:
             # every case takes one substep: the fixed sequence runs as a graph
             # that reads the per-case time step from a persistent array
@@ -446,11 +470,12 @@ class IceFlowEnsemble2D:
             device before the next is launched."""
             nbytes = self._controls_np.nbytes
             for slot, (lams, live) in enumerate(requests):
-                self._call("evaluate_projection", self._upload_values(lams), active=None if all(live) else live)
+                self._call("evaluate_projection", lams, active=None if all(live) else live)
                 self._copy(self._snapshots[slot].data_ptr(), self._controls.data_ptr(), nbytes)
             self._copy(self._snapshots_host.data_ptr(), self._snapshots.data_ptr(), nbytes * len(requests))
             self._stream.synchronize()
             self._value_slot = self._mask_slot = 0
+            self._mask_cache.clear()
             out = []
             for slot in range(len(requests)):
                 water = self._snapshots_np[slot].view(CONTROL_DTYPE).reshape(n).view(np.recarray).water
@@ -462,9 +487,10 @@ class IceFlowEnsemble2D:
         # that evaluation needs nothing from the bracket ones, so all three go
         # to the device before the host reads any of them.
         first_search = [int(need[i] and 0 < iterations[i]) for i in range(n)]
-        requests = [([-v for v in limit], need), (list(limit), need)]
+        constants = [ctypes.c_void_p(self._lambda_constants[k].data_ptr()) for k in range(3)]
+        requests = [(constants[0], need), (constants[1], need)]
         if any(first_search):
-            requests.append(([0.0] * n, first_search))
+            requests.append((constants[2], first_search))
         results = evaluate_queued(requests)
         lower, upper = results[0][0], results[1][0]
         first_evaluation = results[2] if any(first_search) else None
