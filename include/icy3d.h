@@ -51,6 +51,9 @@ typedef struct icy3d_control {
   int32_t repair_count;
   int32_t refill_lower[3], refill_upper[3];
   int32_t geometry_lower[3], geometry_upper[3];
+  /* Cells labelled unresolved (-3) after the last refill commit or before
+   * it; a refresh of part of the grid must relabel them (lower > upper: none). */
+  int32_t cavity_lower[3], cavity_upper[3];
   int32_t tank_count, ice_count; /* lengths of the boundary-cell lists */
   float maximum_speed, peak_speed, phase_clip_volume;
   float remap_energy, discarded_heat, max_outflow;
@@ -116,6 +119,11 @@ int icy3d_phase_warmup(const icy3d_grid* g, int steps, int* parity, void* stream
 int icy3d_classify(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream); /* home.py:280-284 */
 int icy3d_gradients(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream); /* gradients.py:352-386 */
 int icy3d_refill_gradients(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream); /* gradients.py:402-431 */
+/* icy3d_refill_gradients over `count` boxes (lo/hi: [count][3], at most
+ * ICY3D_MAX_BODIES + 1): classification and gradients inside each box, the
+ * boundary lists rebuilt once.  Boxes may overlap (both passes only write
+ * values computed from unchanged inputs). */
+int icy3d_refill_gradients_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi, void* stream);
 int icy3d_collide_phase(const icy3d_grid* g, int parity, int moving_walls, void* stream); /* phase.py:90-127 */
 int icy3d_recover_phase(const icy3d_grid* g, int parity, int recover_flow, void* stream); /* phase.py:129-161 */
 int icy3d_recover_pressure(const icy3d_grid* g, void* stream);          /* home.py:347-353 */
@@ -152,6 +160,9 @@ int icy3d_profile_collect(icy3d_profile* profile); /* waits for the last step; a
 int icy3d_profile_read(const icy3d_profile* profile, double* total_ms, int64_t* count);
 int icy3d_repair_phase_layout(const icy3d_grid* g, int parity, const int32_t lo[3], const int32_t hi[3],
                               void* stream); /* phase.py:163-184 */
+/* icy3d_repair_phase_layout over `count` disjoint boxes ([count][3]). */
+int icy3d_repair_phase_layout_boxes(const icy3d_grid* g, int parity, int count, const int32_t* lo, const int32_t* hi,
+                                    void* stream);
 int icy3d_export_phase(const icy3d_grid* g, int parity, float* out, void* stream); /* phase.py:186-191 */
 
 /* ---- geometry and refill (simulator.py:288-323, 589-736) ---------------- */
@@ -163,6 +174,15 @@ int icy3d_seed_refill_region(const icy3d_grid* g, const int32_t lo[3], const int
 int icy3d_propagate_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream);
 int icy3d_accept_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream);
 int icy3d_commit_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream);
+/* The same passes over `count` disjoint boxes (lo/hi: [count][3], at most
+ * ICY3D_MAX_BODIES + 1; simulator.py:708-736 runs them over the whole grid,
+ * and only cells inside the boxes can change).  A box list lets a refresh
+ * cover each body's region instead of one box spanning every body.
+ * commit_refill also writes cavity_lower/cavity_upper. */
+int icy3d_rasterize_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi, void* stream);
+int icy3d_find_refill_changes_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi,
+                                    void* stream);
+int icy3d_commit_refill_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi, void* stream);
 /* Rebuild the tank (flag 1) and ice (flag 2) cell lists from the flags. */
 int icy3d_build_boundary_lists(const icy3d_grid* g, void* stream);
 int icy3d_project_refill(const icy3d_grid* g, int parity, const int32_t lo[3], const int32_t hi[3], void* stream);
@@ -207,6 +227,17 @@ int icy3d_rigid_pack_state(const icy3d_rigid_arrays* s, float* out);
  * sizes: [body_count][3] material cells.  lo = shape, hi = 0 without bodies. */
 int icy3d_rigid_coverage_box(const float* state, int body_count, const int32_t* sizes, const int shape[3],
                              int32_t lo[3], int32_t hi[3]);
+/* The same box for each body separately (lo/hi: [body_count][3]). */
+int icy3d_rigid_coverage_boxes(const float* state, int body_count, const int32_t* sizes, const int shape[3],
+                               int32_t* lo, int32_t* hi);
+/* Merge `count` boxes ([count][3], lo inclusive, hi exclusive; empty boxes
+ * dropped) into disjoint ones: any two whose copies extended one cell down
+ * on every axis overlap are replaced by their bounding box, until none do.
+ * Writes at most `capacity` boxes to out_lo/out_hi and their number to
+ * *out_count (no reference counterpart: the reference refreshes the whole
+ * grid, simulator.py:708-736). */
+int icy3d_merge_boxes(int count, const int32_t* lo, const int32_t* hi, int capacity, int32_t* out_lo,
+                      int32_t* out_hi, int* out_count);
 /* One rigid update of the flow loop in a single host call (rigid.py:178-347
  * then simulator.py:259-286): icy3d_rigid_integrate, _resolve_contacts,
  * _pack_state into `state` and _coverage_box of the packed rows.  Returns the

@@ -59,6 +59,7 @@ class Control(ctypes.Structure):
         ("remaining", _i), ("repair_count", _i),
         ("refill_lower", _i * 3), ("refill_upper", _i * 3),
         ("geometry_lower", _i * 3), ("geometry_upper", _i * 3),
+        ("cavity_lower", _i * 3), ("cavity_upper", _i * 3),
         ("tank_count", _i), ("ice_count", _i),
         ("maximum_speed", _f), ("peak_speed", _f), ("phase_clip_volume", _f),
         ("remap_energy", _f), ("discarded_heat", _f), ("max_outflow", _f),
@@ -93,6 +94,8 @@ _GRID_CALLS = {
     "rasterize": [_I3, _I3], "find_refill_changes": [_I3, _I3], "seed_refill_region": [_I3, _I3],
     "propagate_refill": [_I3, _I3], "accept_refill": [_I3, _I3], "commit_refill": [_I3, _I3],
     "build_boundary_lists": [],
+    "rasterize_boxes": [_i, _p, _p], "find_refill_changes_boxes": [_i, _p, _p], "commit_refill_boxes": [_i, _p, _p],
+    "repair_phase_layout_boxes": [_i, _i, _p, _p], "refill_gradients_boxes": [_i, _p, _p],
     "project_refill": [_i, _I3, _I3],
     "synchronize_water": [], "thermal_fluxes": [_i], "apply_thermal_fluxes": [_f, _i],
     "measure_outflow": [], "body_conduction": [_f], "exchange_heat": [_f, _I3, _I3],
@@ -148,10 +151,12 @@ def lib():
     handle.icy3d_rigid_update_mass.argtypes = [rigid, _p]
     handle.icy3d_rigid_pack_state.argtypes = [rigid, _p]
     handle.icy3d_rigid_coverage_box.argtypes = [_p, _i, _p, ctypes.POINTER(_i), _p, _p]
+    handle.icy3d_rigid_coverage_boxes.argtypes = [_p, _i, _p, ctypes.POINTER(_i), _p, _p]
+    handle.icy3d_merge_boxes.argtypes = [_i, _p, _p, _i, _p, _p, ctypes.POINTER(ctypes.c_int)]
     handle.icy3d_rigid_step.argtypes = [rigid, _p, _p, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_i), _i, _i,
                                         _p, _p, _p, _p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
     for name in ("rigid_integrate", "rigid_resolve_contacts", "rigid_contact_residuals", "rigid_update_mass",
-                 "rigid_pack_state", "rigid_coverage_box", "rigid_step"):
+                 "rigid_pack_state", "rigid_coverage_box", "rigid_coverage_boxes", "merge_boxes", "rigid_step"):
         getattr(handle, "icy3d_" + name).restype = ctypes.c_int
     sizes = (_i(), _i())
     handle.icy3d_abi_sizes(ctypes.byref(sizes[0]), ctypes.byref(sizes[1]))

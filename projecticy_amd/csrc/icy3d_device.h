@@ -395,3 +395,58 @@ __device__ __forceinline__ void wave_append(bool take, int value, int32_t* list,
 inline unsigned icy_blocks(size_t count) {
   return static_cast<unsigned>((count + icy::kBlock - 1) / icy::kBlock);
 }
+
+namespace icy {
+// Disjoint cell boxes covered by one launch of kBlock-thread blocks: box b
+// owns blocks [first[b], first[b + 1]), so a block's box is uniform.
+constexpr int kMaxBoxes = ICY3D_MAX_BODIES + 1;
+struct BoxList {
+  int count;
+  int first[kMaxBoxes + 1];
+  int lo[kMaxBoxes][3], hi[kMaxBoxes][3];
+};
+__device__ __forceinline__ bool list_cell(const icy3d_grid& g, const BoxList& L, Cell& c) {
+  int b = 0;
+  while (b + 1 < L.count && (int)blockIdx.x >= L.first[b + 1]) ++b;
+  const int t = ((int)blockIdx.x - L.first[b]) * (int)blockDim.x + (int)threadIdx.x;
+  const int ex = L.hi[b][0] - L.lo[b][0], ey = L.hi[b][1] - L.lo[b][1], ez = L.hi[b][2] - L.lo[b][2];
+  if (t >= ex * ey * ez) return false;
+  c.k = L.lo[b][2] + t % ez;
+  const int r = t / ez;
+  c.j = L.lo[b][1] + r % ey;
+  c.i = L.lo[b][0] + r / ey;
+  c.n = lin(g, c.i, c.j, c.k);
+  return true;
+}
+__device__ __forceinline__ bool list_contains(const BoxList& L, int i, int j, int k) {
+  for (int b = 0; b < L.count; ++b)
+    if (i >= L.lo[b][0] && i < L.hi[b][0] && j >= L.lo[b][1] && j < L.hi[b][1] && k >= L.lo[b][2] && k < L.hi[b][2])
+      return true;
+  return false;
+}
+// Host side: the list of `count` boxes [lo, hi) (clamped to the grid, empty
+// boxes dropped) with its block table; false if count is out of range or a
+// box leaves the grid.
+inline bool make_box_list(const icy3d_grid& g, int count, const int32_t* lo, const int32_t* hi, BoxList& L) {
+  if (count < 0 || count > kMaxBoxes || (count && (!lo || !hi))) return false;
+  const int shape[3] = {g.nx, g.ny, g.nz};
+  L.count = 0;
+  L.first[0] = 0;
+  for (int b = 0; b < count; ++b) {
+    long cells = 1;
+    for (int a = 0; a < 3; ++a) {
+      if (lo[3 * b + a] < 0 || hi[3 * b + a] > shape[a]) return false;
+      cells *= hi[3 * b + a] > lo[3 * b + a] ? hi[3 * b + a] - lo[3 * b + a] : 0;
+    }
+    if (!cells) continue;
+    const int m = L.count++;
+    for (int a = 0; a < 3; ++a) {
+      L.lo[m][a] = lo[3 * b + a];
+      L.hi[m][a] = hi[3 * b + a];
+    }
+    L.first[m + 1] = L.first[m] + (int)((cells + kBlock - 1) / kBlock);
+  }
+  return true;
+}
+inline unsigned list_blocks(const BoxList& L) { return (unsigned)L.first[L.count]; }
+}  // namespace icy

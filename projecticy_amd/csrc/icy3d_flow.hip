@@ -84,27 +84,59 @@ __device__ __forceinline__ bool box_cell(const icy3d_grid& g, const Box& b, int 
   return true;
 }
 
-__global__ void k_classify(icy3d_grid g, Box box) {
+__global__ void k_classify(icy3d_grid g, BoxList boxes) {
   Cell c;
-  if (!box_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
+  if (!list_cell(g, boxes, c)) return;
   g.flag[c.n] = (int8_t)boundary_flag(g, c);
 }
 // Compact the tank (flag 1) and ice (flag 2) cells into lists so that the
 // boundary kernels launch over the boundary only.  Three passes (count per
 // block, scan of block counts, scatter) give lists in cell order without a
 // contended global counter.  Block counts live in the partials buffer.
-constexpr int kCompactCells = 4096;  // cells per compaction block (16 per thread)
+constexpr int kCompactCells = 4096;  // cells per compaction block
+constexpr int kCompactPerThread = kCompactCells / 256;  // 16 consecutive cells per thread
+// Bit t of *tank / *ice: cell n0 + t has flag 1 / 2.
+__device__ __forceinline__ void flag_bits(const icy3d_grid& g, int n0, int N, unsigned& tank, unsigned& ice) {
+  int8_t f[kCompactPerThread];
+  if (n0 + kCompactPerThread <= N && (reinterpret_cast<uintptr_t>(g.flag) & 15) == 0) {
+    const int4 v = *reinterpret_cast<const int4*>(g.flag + n0);
+    const int w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int t = 0; t < kCompactPerThread; ++t) f[t] = (int8_t)(w[t >> 2] >> (8 * (t & 3)));
+  } else {
+#pragma unroll
+    for (int t = 0; t < kCompactPerThread; ++t) f[t] = n0 + t < N ? g.flag[n0 + t] : (int8_t)-1;
+  }
+  tank = 0u;
+  ice = 0u;
+#pragma unroll
+  for (int t = 0; t < kCompactPerThread; ++t) {
+    tank |= (f[t] == 1 ? 1u : 0u) << t;
+    ice |= (f[t] == 2 ? 1u : 0u) << t;
+  }
+}
+// Exclusive prefix sum over the block (blockDim.x a multiple of 64).
+__device__ __forceinline__ int block_exclusive_scan(int v) {
+  __shared__ int wave_total[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wave_total[wave] = x;
+  __syncthreads();
+  int base = 0;
+  for (int w = 0; w < wave; ++w) base += wave_total[w];
+  return base + x - v;
+}
 __global__ void __launch_bounds__(256) k_count_flags(icy3d_grid g) {
   const int N = (int)cells(g);
-  int tank = 0, ice = 0;
-  for (int it = 0; it < kCompactCells / 256; ++it) {
-    const int n = blockIdx.x * kCompactCells + it * 256 + threadIdx.x;
-    const int flag = n < N ? g.flag[n] : -1;
-    tank += flag == 1;
-    ice += flag == 2;
-  }
-  const int tanks = block_sum_int(tank);
-  const int ices = block_sum_int(ice);
+  unsigned tank, ice;
+  flag_bits(g, blockIdx.x * kCompactCells + threadIdx.x * kCompactPerThread, N, tank, ice);
+  const int tanks = block_sum_int(__popc(tank));
+  const int ices = block_sum_int(__popc(ice));
   int* counts = reinterpret_cast<int*>(g.partials);
   if (threadIdx.x == 0) {
     counts[2 * blockIdx.x] = tanks;
@@ -146,38 +178,20 @@ __global__ void __launch_bounds__(1024) k_scan_flag_counts(icy3d_grid g, int blo
     g.control->ice_count = ice_sum[threadIdx.x];
   }
 }
-__device__ __forceinline__ void block_scatter(bool take, int value, int base, int32_t* list) {
-  __shared__ int wave_base[16];
-  const unsigned long long mask = __ballot(take);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) wave_base[wave] = __popcll(mask);
-  __syncthreads();
-  int offset = base;
-  for (int w = 0; w < wave; ++w) offset += wave_base[w];
-  if (take) list[offset + __popcll(mask & ((1ull << lane) - 1ull))] = value;
-  __syncthreads();
-}
+// Each thread owns 16 consecutive cells, so one block-wide scan of the
+// per-thread counts places every cell in cell order.  Both counts (at most
+// 4096 per block) share one scan as the halves of an int.
 __global__ void __launch_bounds__(256) k_scatter_flags(icy3d_grid g) {
   const int N = (int)cells(g);
   const int* offsets = reinterpret_cast<const int*>(g.partials);
-  int tank_base = offsets[2 * blockIdx.x], ice_base = offsets[2 * blockIdx.x + 1];
-  __shared__ int totals[2];
-  for (int it = 0; it < kCompactCells / 256; ++it) {
-    const int n = blockIdx.x * kCompactCells + it * 256 + threadIdx.x;
-    const int flag = n < N ? g.flag[n] : -1;
-    block_scatter(flag == 1, n, tank_base, g.tank_cells);
-    block_scatter(flag == 2, n, ice_base, g.ice_cells);
-    const int tanks = block_sum_int(flag == 1 ? 1 : 0);
-    const int ices = block_sum_int(flag == 2 ? 1 : 0);
-    if (threadIdx.x == 0) {
-      totals[0] = tanks;
-      totals[1] = ices;
-    }
-    __syncthreads();
-    tank_base += totals[0];
-    ice_base += totals[1];
-    __syncthreads();
-  }
+  const int n0 = blockIdx.x * kCompactCells + threadIdx.x * kCompactPerThread;
+  unsigned tank, ice;
+  flag_bits(g, n0, N, tank, ice);
+  const int before = block_exclusive_scan(__popc(tank) | (__popc(ice) << 16));
+  int tank_at = offsets[2 * blockIdx.x] + (before & 0xffff);
+  int ice_at = offsets[2 * blockIdx.x + 1] + (before >> 16);
+  for (; tank; tank &= tank - 1u) g.tank_cells[tank_at++] = n0 + __ffs(tank) - 1;
+  for (; ice; ice &= ice - 1u) g.ice_cells[ice_at++] = n0 + __ffs(ice) - 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -392,16 +406,14 @@ __global__ void __launch_bounds__(kGradJ* kGradK) k_gradients_tiled(icy3d_grid g
 
 // gradients.py:116-145 with one lane per direction (wall links sample the
 // mirror), summed within the cell's 32-lane group.
-__global__ void __launch_bounds__(256) k_gradients_general_list(icy3d_grid g, Box box) {
+__global__ void __launch_bounds__(256) k_gradients_general_list(icy3d_grid g, BoxList boxes) {
   const int lane = threadIdx.x & (kCellLanes - 1);
   const int groups = blockDim.x / kCellLanes;
   const int count = g.control->ice_count;
   for (int t = blockIdx.x * groups + threadIdx.x / kCellLanes; t < count; t += gridDim.x * groups) {
     const int n = g.ice_cells[t];
     const Cell c = cell_of(g, n);
-    if (c.i < box.lo[0] || c.i >= box.hi[0] || c.j < box.lo[1] || c.j >= box.hi[1] || c.k < box.lo[2] ||
-        c.k >= box.hi[2])
-      continue;
+    if (!list_contains(boxes, c.i, c.j, c.k)) continue;
     const float centre = g.phi[n];
     float rx = 0.0f, ry = 0.0f, rz = 0.0f;
     if (lane < 27 && lane != 13) {
@@ -599,9 +611,9 @@ __global__ void k_fold_peak(icy3d_grid g) {
 }
 
 // phase.py:163-184
-__global__ void k_repair_layout(icy3d_grid g, Box box) {
+__global__ void k_repair_layout(icy3d_grid g, BoxList boxes) {
   Cell c;
-  if (!box_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c)) return;
+  if (!list_cell(g, boxes, c)) return;
   const size_t N = cells(g);
   const int n = c.n;
 #pragma unroll
@@ -1838,11 +1850,21 @@ bool valid(const icy3d_grid* g) {
          grid_cells(*g) < (size_t)INT32_MAX / 16 && g->control;
 }
 
-int launch_classify(const icy3d_grid& g, const Box& b, hipStream_t s) {
-  size_t n = box_cells(b);
-  if (n) k_classify<<<icy_blocks(n), kBlock, 0, s>>>(g, b);
+// A clamped box as a one-entry list (empty when the box is).
+BoxList one_box(const icy3d_grid& g, const Box& b) {
+  BoxList L;
+  L.count = 0;
+  L.first[0] = 0;
+  if (box_cells(b)) (void)make_box_list(g, 1, b.lo, b.hi, L);
+  return L;
+}
+int launch_classify_list(const icy3d_grid& g, const BoxList& L, hipStream_t s) {
+  if (list_blocks(L)) k_classify<<<list_blocks(L), kBlock, 0, s>>>(g, L);
   ICY_LAUNCH_CHECK();
   return 0;
+}
+int launch_classify(const icy3d_grid& g, const Box& b, hipStream_t s) {
+  return launch_classify_list(g, one_box(g, b), s);
 }
 int launch_lists(const icy3d_grid& g, hipStream_t s) {
   const unsigned blocks = (unsigned)((grid_cells(g) + kCompactCells - 1) / kCompactCells);
@@ -1861,15 +1883,26 @@ int launch_classify_all(const icy3d_grid& g, hipStream_t s) {
 // refreshes test the flag per cell.
 // Mirror gradients always run over the ice list (one code path, so a
 // box refresh reproduces a full pass bit for bit); the box only filters.
-int launch_gradients(const icy3d_grid& g, const Box& b, bool, hipStream_t s) {
-  size_t n = box_cells(b);
-  if (!n) return 0;
-  const dim3 tiles((b.hi[2] - b.lo[2] + kGradK - 1) / kGradK, (b.hi[1] - b.lo[1] + kGradJ - 1) / kGradJ,
-                   (b.hi[0] - b.lo[0] + kGradX - 1) / kGradX);
-  k_gradients_tiled<<<tiles, kGradJ * kGradK, 0, s>>>(g, b);
-  k_gradients_general_list<<<kListBlocks, kBlock, 0, s>>>(g, b);
+// The tiled pass per box, then one pass over the ice list filtered by the
+// boxes.  Overlapping boxes write the same values twice.
+int launch_gradients_list(const icy3d_grid& g, const BoxList& L, hipStream_t s) {
+  if (!L.count) return 0;
+  for (int m = 0; m < L.count; ++m) {
+    Box b;
+    for (int a = 0; a < 3; ++a) {
+      b.lo[a] = L.lo[m][a];
+      b.hi[a] = L.hi[m][a];
+    }
+    const dim3 tiles((b.hi[2] - b.lo[2] + kGradK - 1) / kGradK, (b.hi[1] - b.lo[1] + kGradJ - 1) / kGradJ,
+                     (b.hi[0] - b.lo[0] + kGradX - 1) / kGradX);
+    k_gradients_tiled<<<tiles, kGradJ * kGradK, 0, s>>>(g, b);
+  }
+  k_gradients_general_list<<<kListBlocks, kBlock, 0, s>>>(g, L);
   ICY_LAUNCH_CHECK();
   return 0;
+}
+int launch_gradients(const icy3d_grid& g, const Box& b, bool, hipStream_t s) {
+  return launch_gradients_list(g, one_box(g, b), s);
 }
 int launch_forces(const icy3d_grid& g, int with_flux, int moment_pressure, hipStream_t s) {
   const dim3 tiles((g.nz + kForceK - 1) / kForceK, (g.ny + kForceJ - 1) / kForceJ, (g.nx + kForceX - 1) / kForceX);
@@ -1918,6 +1951,23 @@ int icy3d_classify(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3]
 int icy3d_gradients(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;
   return launch_gradients(*g, lo ? clamp_box(*g, make_box(lo, hi)) : full_box(*g), lo == nullptr, (hipStream_t)stream);
+}
+int icy3d_refill_gradients_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi, void* stream) {
+  if (!valid(g) || count < 0 || count > kMaxBoxes || (count && (!lo || !hi))) return ICY3D_ERR_ARGUMENT;
+  int32_t clo[kMaxBoxes * 3], chi[kMaxBoxes * 3];
+  for (int m = 0; m < count; ++m) {
+    const Box b = clamp_box(*g, make_box(lo + 3 * m, hi + 3 * m));
+    for (int a = 0; a < 3; ++a) {
+      clo[3 * m + a] = b.lo[a];
+      chi[3 * m + a] = b.hi[a];
+    }
+  }
+  BoxList L;
+  if (!make_box_list(*g, count, clo, chi, L)) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = launch_classify_list(*g, L, s);
+  if (!rc) rc = launch_lists(*g, s);
+  return rc ? rc : launch_gradients_list(*g, L, s);
 }
 int icy3d_refill_gradients(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;
@@ -2104,15 +2154,21 @@ int icy3d_profile_read(const icy3d_profile* p, double* total_ms, int64_t* count)
   *count = p->count;
   return 0;
 }
+int icy3d_repair_phase_layout_boxes(const icy3d_grid* g, int parity, int count, const int32_t* lo, const int32_t* hi,
+                                    void* stream) {
+  BoxList L;
+  if (!valid(g) || !make_box_list(*g, count, lo, hi, L)) return ICY3D_ERR_ARGUMENT;
+  if (!parity) return 0;
+  if (list_blocks(L)) k_repair_layout<<<list_blocks(L), kBlock, 0, (hipStream_t)stream>>>(*g, L);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
 int icy3d_repair_phase_layout(const icy3d_grid* g, int parity, const int32_t lo[3], const int32_t hi[3],
                               void* stream) {
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;
-  if (!parity) return 0;
-  Box b = lo ? clamp_box(*g, make_box(lo, hi)) : full_box(*g);
-  size_t n = box_cells(b);
-  if (n) k_repair_layout<<<icy_blocks(n), kBlock, 0, (hipStream_t)stream>>>(*g, b);
-  ICY_LAUNCH_CHECK();
-  return 0;
+  const Box b = lo ? clamp_box(*g, make_box(lo, hi)) : full_box(*g);
+  if (!box_cells(b)) return 0;
+  return icy3d_repair_phase_layout_boxes(g, parity, 1, b.lo, b.hi, stream);
 }
 int icy3d_export_phase(const icy3d_grid* g, int parity, float* out, void* stream) {
   if (!valid(g) || !out) return ICY3D_ERR_ARGUMENT;

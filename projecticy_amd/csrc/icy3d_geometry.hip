@@ -76,11 +76,12 @@ __global__ void k_reset_geometry_bounds(icy3d_grid g) {
   }
 }
 
-// simulator.py:288-323 over a box that contains every cell whose label or
-// coverage can change (the union of the bodies' current and previous boxes).
-__global__ void k_rasterize(icy3d_grid g, GBox box) {
+// simulator.py:288-323 over disjoint boxes that contain every cell whose
+// label or coverage can change (each body's current and previous boxes and
+// the cells the last refill left unresolved).
+__global__ void k_rasterize(icy3d_grid g, BoxList boxes) {
   Cell c = {0, 0, 0, 0};
-  bool active = gbox_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c);
+  bool active = list_cell(g, boxes, c);
   bool changed = false;
   if (active) {
     const int n = c.n;
@@ -119,9 +120,9 @@ __global__ void k_reset_refill_counts(icy3d_grid g) {
 }
 
 // simulator.py:595-613
-__global__ void k_find_refill_changes(icy3d_grid g, GBox box) {
+__global__ void k_find_refill_changes(icy3d_grid g, BoxList boxes) {
   Cell c = {0, 0, 0, 0};
-  bool active = gbox_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, c);
+  bool active = list_cell(g, boxes, c);
   bool opening = false, covered = false;
   if (active) {
     const int n = c.n;
@@ -230,33 +231,49 @@ __global__ void k_accept_refill(icy3d_grid g, GBox box) {
   g.refill_valid[c.n] = g.refill_next_valid[c.n];
 }
 
-// simulator.py:662-678
-__global__ void k_commit_refill(icy3d_grid g, GBox box) {
-  Cell cb;
-  if (!gbox_cell(g, box, blockIdx.x * blockDim.x + threadIdx.x, cb)) return;
-  const size_t N = cells(g);
-  const int n = cb.n;
-  int id = g.id[n];
-  if (id == -1 && g.prev_id[n] != -1) {
-    if (g.refill_valid[n]) {
-      g.phi[n] = g.refill_state[n];
-      g.pressure[n] = g.refill_state[N + n];
-      g.u[n] = g.refill_state[2 * N + n];
-      g.u[N + n] = g.refill_state[3 * N + n];
-      g.u[2 * N + n] = g.refill_state[4 * N + n];
-    } else {
-      id = -3;
-      g.id[n] = -3;
+__global__ void k_reset_cavity_bounds(icy3d_grid g) {
+  const int shape[3] = {g.nx, g.ny, g.nz};
+  for (int a = 0; a < 3; ++a) {
+    g.control->cavity_lower[a] = shape[a];
+    g.control->cavity_upper[a] = -1;
+  }
+}
+
+// simulator.py:662-678.  Also bounds the cells that are unresolved after
+// the commit or were unresolved before it (label or previous label -3): the
+// next refresh must relabel them, as a refresh of the whole grid does.
+__global__ void k_commit_refill(icy3d_grid g, BoxList boxes) {
+  Cell cb = {0, 0, 0, 0};
+  const bool active = list_cell(g, boxes, cb);
+  bool cavity = false;
+  if (active) {
+    const size_t N = cells(g);
+    const int n = cb.n;
+    int id = g.id[n];
+    const int before = g.prev_id[n];
+    if (id == -1 && before != -1) {
+      if (g.refill_valid[n]) {
+        g.phi[n] = g.refill_state[n];
+        g.pressure[n] = g.refill_state[N + n];
+        g.u[n] = g.refill_state[2 * N + n];
+        g.u[N + n] = g.refill_state[3 * N + n];
+        g.u[2 * N + n] = g.refill_state[4 * N + n];
+      } else {
+        id = -3;
+        g.id[n] = -3;
+      }
     }
+    if (id != -1) {
+      float v[3] = {0.0f, 0.0f, 0.0f};
+      if (id >= 0) body_velocity_at(g, cb.i + 0.5f, cb.j + 0.5f, cb.k + 0.5f, id, v);
+      g.phi[n] = 0.0f;
+      g.u[n] = v[0];
+      g.u[N + n] = v[1];
+      g.u[2 * N + n] = v[2];
+    }
+    cavity = id == -3 || before == -3;
   }
-  if (id != -1) {
-    float v[3] = {0.0f, 0.0f, 0.0f};
-    if (id >= 0) body_velocity_at(g, cb.i + 0.5f, cb.j + 0.5f, cb.k + 0.5f, id, v);
-    g.phi[n] = 0.0f;
-    g.u[n] = v[0];
-    g.u[N + n] = v[1];
-    g.u[2 * N + n] = v[2];
-  }
+  reduce_bounds(g.control->cavity_lower, g.control->cavity_upper, cavity, cb);
 }
 
 }  // namespace icy
@@ -295,27 +312,34 @@ bool gbox_inside(const icy3d_grid& g, const GBox& b) {
 
 extern "C" {
 
-int icy3d_rasterize(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
-  if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
-  GBox b = lo ? make_gbox(lo, hi) : full_gbox(*g);
-  if (!gbox_inside(*g, b)) return ICY3D_ERR_ARGUMENT;
+int icy3d_rasterize_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi, void* stream) {
+  BoxList L;
+  if (!valid_grid(g) || !make_box_list(*g, count, lo, hi, L)) return ICY3D_ERR_ARGUMENT;
   hipStream_t s = (hipStream_t)stream;
   k_reset_geometry_bounds<<<1, 1, 0, s>>>(*g);
-  size_t n = gbox_cells(b);
-  if (n) k_rasterize<<<icy_blocks(n), kBlock, 0, s>>>(*g, b);
+  if (list_blocks(L)) k_rasterize<<<list_blocks(L), kBlock, 0, s>>>(*g, L);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_rasterize(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
+  if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
+  const GBox b = lo ? make_gbox(lo, hi) : full_gbox(*g);
+  return icy3d_rasterize_boxes(g, 1, b.lo, b.hi, stream);
+}
+int icy3d_find_refill_changes_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi,
+                                    void* stream) {
+  BoxList L;
+  if (!valid_grid(g) || !make_box_list(*g, count, lo, hi, L)) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  k_reset_refill_counts<<<1, 1, 0, s>>>(*g);
+  if (list_blocks(L)) k_find_refill_changes<<<list_blocks(L), kBlock, 0, s>>>(*g, L);
   ICY_LAUNCH_CHECK();
   return 0;
 }
 int icy3d_find_refill_changes(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
   if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
-  GBox b = lo ? make_gbox(lo, hi) : full_gbox(*g);
-  if (!gbox_inside(*g, b)) return ICY3D_ERR_ARGUMENT;
-  hipStream_t s = (hipStream_t)stream;
-  k_reset_refill_counts<<<1, 1, 0, s>>>(*g);
-  size_t n = gbox_cells(b);
-  if (n) k_find_refill_changes<<<icy_blocks(n), kBlock, 0, s>>>(*g, b);
-  ICY_LAUNCH_CHECK();
-  return 0;
+  const GBox b = lo ? make_gbox(lo, hi) : full_gbox(*g);
+  return icy3d_find_refill_changes_boxes(g, 1, b.lo, b.hi, stream);
 }
 int icy3d_seed_refill_region(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
   if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
@@ -346,14 +370,19 @@ int icy3d_accept_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t 
   ICY_LAUNCH_CHECK();
   return 0;
 }
-int icy3d_commit_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
-  if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
-  GBox b = lo ? make_gbox(lo, hi) : full_gbox(*g);
-  if (!gbox_inside(*g, b)) return ICY3D_ERR_ARGUMENT;
-  size_t n = gbox_cells(b);
-  if (n) k_commit_refill<<<icy_blocks(n), kBlock, 0, (hipStream_t)stream>>>(*g, b);
+int icy3d_commit_refill_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi, void* stream) {
+  BoxList L;
+  if (!valid_grid(g) || !make_box_list(*g, count, lo, hi, L)) return ICY3D_ERR_ARGUMENT;
+  hipStream_t s = (hipStream_t)stream;
+  k_reset_cavity_bounds<<<1, 1, 0, s>>>(*g);
+  if (list_blocks(L)) k_commit_refill<<<list_blocks(L), kBlock, 0, s>>>(*g, L);
   ICY_LAUNCH_CHECK();
   return 0;
+}
+int icy3d_commit_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
+  if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
+  const GBox b = lo ? make_gbox(lo, hi) : full_gbox(*g);
+  return icy3d_commit_refill_boxes(g, 1, b.lo, b.hi, stream);
 }
 
 }  // extern "C"

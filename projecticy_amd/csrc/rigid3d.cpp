@@ -8,9 +8,11 @@
 // fallback for low-inertia remnants.  Everything here is plain C++ so one
 // step costs microseconds instead of the ~0.5 ms of small-array NumPy calls.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <vector>
 
 #include "../../include/icy3d.h"
 
@@ -616,10 +618,9 @@ int icy3d_rigid_pack_state(const icy3d_rigid_arrays* s, float* out) {
   return 0;
 }
 
-int icy3d_rigid_coverage_box(const float* state, int body_count, const int32_t* sizes, const int shape[3],
-                             int32_t lo[3], int32_t hi[3]) {
-  if (!state || !sizes || body_count < 0 || body_count > ICY3D_MAX_BODIES) return ICY3D_ERR_ARGUMENT;
-  int64_t low[3] = {shape[0], shape[1], shape[2]}, high[3] = {0, 0, 0};
+int icy3d_rigid_coverage_boxes(const float* state, int body_count, const int32_t* sizes, const int shape[3],
+                               int32_t* lo, int32_t* hi) {
+  if (!state || !sizes || !lo || !hi || body_count < 0 || body_count > ICY3D_MAX_BODIES) return ICY3D_ERR_ARGUMENT;
   for (int b = 0; b < body_count; ++b) {
     const float* row = state + ICY3D_STATE_STRIDE * b;
     for (int a = 0; a < 3; ++a) {
@@ -627,14 +628,69 @@ int icy3d_rigid_coverage_box(const float* state, int body_count, const int32_t* 
       double extent = 0.0;
       for (int c = 0; c < 3; ++c) extent += std::fabs(double(row[3 + 3 * a + c])) * (0.5 * double(sizes[3 * b + c]) + 1.0);
       const double origin = double(row[a]);
-      low[a] = std::min<int64_t>(low[a], int64_t(std::floor(origin - extent)) - 1);
-      high[a] = std::max<int64_t>(high[a], int64_t(std::ceil(origin + extent)) + 2);
+      const int64_t low = int64_t(std::floor(origin - extent)) - 1, high = int64_t(std::ceil(origin + extent)) + 2;
+      lo[3 * b + a] = int32_t(std::min<int64_t>(std::max<int64_t>(low, 0), shape[a]));
+      hi[3 * b + a] = int32_t(std::min<int64_t>(std::max<int64_t>(high, 0), shape[a]));
     }
   }
+  return 0;
+}
+
+int icy3d_rigid_coverage_box(const float* state, int body_count, const int32_t* sizes, const int shape[3],
+                             int32_t lo[3], int32_t hi[3]) {
+  int32_t blo[3 * ICY3D_MAX_BODIES], bhi[3 * ICY3D_MAX_BODIES];
+  const int rc = icy3d_rigid_coverage_boxes(state, body_count, sizes, shape, blo, bhi);
+  if (rc) return rc;
   for (int a = 0; a < 3; ++a) {
-    lo[a] = int32_t(std::min<int64_t>(std::max<int64_t>(low[a], 0), shape[a]));
-    hi[a] = int32_t(std::min<int64_t>(std::max<int64_t>(high[a], 0), shape[a]));
+    lo[a] = shape[a];
+    hi[a] = 0;
+    for (int b = 0; b < body_count; ++b) {
+      lo[a] = std::min(lo[a], blo[3 * b + a]);
+      hi[a] = std::max(hi[a], bhi[3 * b + a]);
+    }
   }
+  return 0;
+}
+
+int icy3d_merge_boxes(int count, const int32_t* lo, const int32_t* hi, int capacity, int32_t* out_lo,
+                      int32_t* out_hi, int* out_count) {
+  if (count < 0 || capacity < 0 || !out_count || (count && (!lo || !hi)) || (capacity && (!out_lo || !out_hi)))
+    return ICY3D_ERR_ARGUMENT;
+  std::vector<std::array<int32_t, 6>> boxes;
+  boxes.reserve(count);
+  for (int m = 0; m < count; ++m) {
+    bool empty = false;
+    for (int a = 0; a < 3; ++a) empty = empty || hi[3 * m + a] <= lo[3 * m + a];
+    if (!empty)
+      boxes.push_back({lo[3 * m], lo[3 * m + 1], lo[3 * m + 2], hi[3 * m], hi[3 * m + 1], hi[3 * m + 2]});
+  }
+  // Copies extended one cell down overlap on every axis: [lo - 1, hi).
+  auto touch = [](const std::array<int32_t, 6>& p, const std::array<int32_t, 6>& q) {
+    for (int a = 0; a < 3; ++a)
+      if (p[a] - 1 >= q[3 + a] || q[a] - 1 >= p[3 + a]) return false;
+    return true;
+  };
+  for (bool merged = true; merged;) {
+    merged = false;
+    for (size_t m = 0; m < boxes.size() && !merged; ++m)
+      for (size_t k = m + 1; k < boxes.size(); ++k)
+        if (touch(boxes[m], boxes[k])) {
+          for (int a = 0; a < 3; ++a) {
+            boxes[m][a] = std::min(boxes[m][a], boxes[k][a]);
+            boxes[m][3 + a] = std::max(boxes[m][3 + a], boxes[k][3 + a]);
+          }
+          boxes.erase(boxes.begin() + k);
+          merged = true;
+          break;
+        }
+  }
+  if ((int)boxes.size() > capacity) return ICY3D_ERR_ARGUMENT;
+  for (size_t m = 0; m < boxes.size(); ++m)
+    for (int a = 0; a < 3; ++a) {
+      out_lo[3 * m + a] = boxes[m][a];
+      out_hi[3 * m + a] = boxes[m][3 + a];
+    }
+  *out_count = (int)boxes.size();
   return 0;
 }
 

@@ -161,7 +161,11 @@ class IceFlow3D:
         # grid until the first geometry refresh establishes the ice region.
         self._flags_valid = False
         self._raster_full = True
-        self._raster_region = None
+        self._box_history = None
+        self._full_boxes = (np.zeros((1, 3), np.int32), np.array([self.shape], np.int32))
+        self._merged = (np.zeros((_native.MAX_BODIES + 1, 3), np.int32), np.zeros((_native.MAX_BODIES + 1, 3), np.int32))
+        cavity0 = Control.cavity_lower.offset
+        self._cavity_view = self._loads_host.numpy()[cavity0 : cavity0 + 24].view(np.int32)
         self._grid = g
         self._gp = ctypes.pointer(g)
         self._body_state = np.zeros((B, _native.STATE_STRIDE), np.float32)
@@ -185,34 +189,59 @@ class IceFlow3D:
         self._raster_full = True
         self._flags_valid = False
 
-    def _bodies_box(self):
-        """Cells whose coverage can be nonzero for the uploaded body state:
-        the rotated material box plus its trilinear support, with a margin."""
-        state = self._body_state[: self.body_count].astype(np.float64)
-        shape = np.array(self.shape)
-        if not self.body_count:
-            return np.clip(shape, 0, shape), np.zeros(3, dtype=np.int64)
-        half = 0.5 * np.array([b.size_cells for b in self.config.bodies], dtype=np.float64) + 1.0
-        origin = state[:, 0:3]
-        extent = np.einsum("bij,bj->bi", np.abs(state[:, 3:12].reshape(-1, 3, 3)), half)
-        lo = (np.floor(origin - extent).astype(np.int64) - 1).min(axis=0)
-        hi = (np.ceil(origin + extent).astype(np.int64) + 2).max(axis=0)
-        return np.clip(lo, 0, shape), np.clip(hi, 0, shape)
+    # gradients.py:15-20: every phase and coverage dependency of a gradient
+    # lies within this many cells.
+    GRADIENT_DEPENDENCY_RADIUS = 6
 
-    def _raster_box(self, current=None):
-        """Union of the bodies' current and previous boxes: every cell whose
-        label, coverage or previous label can change in this refresh."""
+    def _coverage_boxes(self, state=None):
+        """Per body, the cells whose coverage can be nonzero for FP32 rows
+        ``state`` (the uploaded rows by default): int32 (lo, hi), [B, 3]."""
+        state = np.ascontiguousarray(self._body_state if state is None else state, dtype=np.float32)
+        B = self.body_count
+        lo, hi = np.zeros((B, 3), np.int32), np.zeros((B, 3), np.int32)
+        sizes = self.rigid._step_arguments()["sizes"]
+        check(self._lib.icy3d_rigid_coverage_boxes(state.ctypes.data, B, sizes.ctypes.data, int3(self.shape),
+                                                   lo.ctypes.data, hi.ctypes.data), "coverage boxes")
+        return lo, hi
+
+    @staticmethod
+    def _cavity_box(lower, upper):
+        """(lo, hi) of the cells the last refill commit left unresolved or
+        resolved (inclusive bounds from the device), or None."""
+        lo, hi = np.array(lower, np.int32).reshape(1, 3), np.array(upper, np.int32).reshape(1, 3) + 1
+        return None if np.any(hi <= lo) else (lo, hi)
+
+    def _raster_boxes(self, current=None, cavity=()):
+        """Disjoint boxes (int32 lo, hi: [count, 3]) holding every cell whose
+        label, coverage or previous label can change in this refresh: each
+        body's box for the uploaded rows and for the rows of the two
+        refreshes before (the last refresh relabelled cells only inside its
+        own boxes), plus ``cavity`` (the last commit's unresolved cells;
+        read from the device when not given).  The reference refreshes the
+        whole grid; every other cell keeps its label, coverage and previous
+        label there.  The whole grid on the first refresh and after edits."""
         if current is None:
-            current = self._bodies_box()
-        current = (tuple(int(v) for v in current[0]), tuple(int(v) for v in current[1]))
-        if self._raster_full or self._raster_region is None:
-            box = ((0, 0, 0), tuple(self.shape))
+            current = self._coverage_boxes()
+        history = self._box_history
+        self._box_history = (current, None if history is None else history[0])
+        if self._raster_full or history is None:
             self._raster_full = False
-        else:
-            last = self._raster_region
-            box = (tuple(map(min, current[0], last[0])), tuple(map(max, current[1], last[1])))
-        self._raster_region = current
-        return box
+            return self._full_boxes
+        if cavity == ():
+            control = self._read_control()
+            cavity = self._cavity_box(control.cavity_lower, control.cavity_upper)
+        parts = [current, history[0]] + [part for part in (history[1], cavity) if part is not None]
+        lo = np.ascontiguousarray(np.concatenate([part[0] for part in parts]), dtype=np.int32)
+        hi = np.ascontiguousarray(np.concatenate([part[1] for part in parts]), dtype=np.int32)
+        out_lo, out_hi = self._merged
+        count = ctypes.c_int(0)
+        code = self._lib.icy3d_merge_boxes(len(lo), lo.ctypes.data, hi.ctypes.data, len(out_lo), out_lo.ctypes.data,
+                                           out_hi.ctypes.data, ctypes.byref(count))
+        if code == 0:
+            return out_lo[: count.value].copy(), out_hi[: count.value].copy()
+        # More disjoint regions than one launch takes: their bounding box.
+        keep = np.all(hi > lo, axis=1)
+        return lo[keep].min(axis=0, keepdims=True), hi[keep].max(axis=0, keepdims=True)
 
     def _control_offsets(self):
         floats = self._t["control"].view(torch.float32)
@@ -327,14 +356,15 @@ class IceFlow3D:
     def _rigid_step(self, loads):
         """rigid.py:178-347 and the upload of simulator.py:259-286 in one
         native call: the packed rows go straight into the pinned staging
-        buffer and are queued for upload; returns their coverage box."""
+        buffer and are queued for upload; returns their per-body coverage
+        boxes."""
         self._upload_done.synchronize()  # the previous upload has left the staging buffer
         staging = self._upload_staging_view
-        box = self.rigid.advance_packed(loads, staging)
+        self.rigid.advance_packed(loads, staging)
         self._body_state[...] = staging
         self._t["body"].copy_(self._upload_staging, non_blocking=True)
         self._upload_done.record(self._torch_stream())
-        return box
+        return self._coverage_boxes(staging)
 
     def _upload_rigid_state(self, state):
         """rigid.py:136-151 device side: one small host-to-device copy,
@@ -368,16 +398,28 @@ class IceFlow3D:
         """After every field was written from a restart file: the host copy of
         the uploaded body rows follows the device; labels, coverage and
         previous labels were saved together with those rows, so the next
-        refresh covers the union of the bodies' boxes as in the saved run
-        (whose last refresh used the same rows), and the link classes are
-        rebuilt from the labels.  The saved phase gradient is the cache the
-        saved run would have used next (it may precede the last geometry
-        update outside the refreshed box), so it is kept when the file comes
-        from these sources and rebuilt otherwise."""
+        refresh covers the bodies' boxes for those rows (as the saved run's
+        did) and the box of every cell whose label differs from its previous
+        label or is unresolved (which the saved run's box history covered),
+        and the link classes are rebuilt from the labels.  The saved phase
+        gradient is the cache the saved run would have used next (it may
+        precede the last geometry update outside the refreshed boxes), so it
+        is kept when the file comes from these sources and rebuilt
+        otherwise."""
         self._body_state[...] = self._t["body"].cpu().numpy()
         self._flags_valid = False
         self._raster_full = False
-        self._raster_region = tuple(tuple(int(v) for v in side) for side in self._bodies_box())
+        ids, previous = self._t["id"], self._t["prev_id"]
+        unsettled = ((ids != previous) | (ids == -3) | (previous == -3)).nonzero()
+        stale = None
+        if len(unsettled):
+            stale = (unsettled.min(dim=0).values.cpu().numpy().astype(np.int32).reshape(1, 3),
+                     unsettled.max(dim=0).values.cpu().numpy().astype(np.int32).reshape(1, 3) + 1)
+        self._box_history = (self._coverage_boxes(), stale)
+        control = self._read_control()
+        control.cavity_lower[:] = self.shape
+        control.cavity_upper[:] = (-1, -1, -1)
+        self._write_control(control)
         self._gradient_cache_valid = bool(keep_gradients)
 
     @property
@@ -430,16 +472,21 @@ class IceFlow3D:
         use = pressure is not None
         self._call("set_equilibrium_all", self._parity, int(with_phase), int(use), float(pressure or 0.0))
 
-    def _rasterize(self, box=None):
-        if box is None:
-            box = self._raster_box()
-        self._call("rasterize", int3(box[0]), int3(box[1]))
+    def _rasterize(self, boxes=None):
+        """simulator.py:288-323 over ``boxes`` (int32 lo, hi: [count, 3];
+        by default those of the next refresh)."""
+        lo, hi = self._raster_boxes() if boxes is None else boxes
+        self._call("rasterize_boxes", len(lo), lo.ctypes.data, hi.ctypes.data)
         self._flags_valid = False
 
-    def _repair_phase_layout(self, box=None):
-        lo, hi = ((0, 0, 0), self.shape) if box is None else box
-        # A link (I, I+e) is swapped by the thread of I: extend one cell down.
-        self._call("repair_phase_layout", self._parity, int3(max(int(v) - 1, 0) for v in lo), int3(hi))
+    def _repair_phase_layout(self, boxes=None):
+        """phase.py:163-184 over ``boxes`` (int32 lo, hi: [count, 3], whose
+        copies extended one cell down are disjoint; the whole grid by
+        default).  A link (I, I+e) is swapped by the thread of I, so each box
+        is extended one cell down."""
+        lo, hi = self._full_boxes if boxes is None else boxes
+        lo = np.ascontiguousarray(np.maximum(lo - 1, 0), dtype=np.int32)
+        self._call("repair_phase_layout_boxes", self._parity, len(lo), lo.ctypes.data, hi.ctypes.data)
 
     def _synchronize_water(self):
         self._call("synchronize_water")
@@ -568,13 +615,17 @@ class IceFlow3D:
             self._raise_unstable(control.invalid_index)
         return float(control.maximum_speed)
 
-    def _refresh_geometry(self, phase_changed=True, current=None):
-        """simulator.py:708-736; the refill loop reads one counter per sweep.
-        ``current``: the bodies' coverage box when the caller has it."""
-        box = self._raster_box(current)
-        self._rasterize(box)
-        self._repair_phase_layout(box)
-        self._call("find_refill_changes", int3(box[0]), int3(box[1]))
+    def _refresh_geometry(self, phase_changed=True, current=None, cavity=()):
+        """simulator.py:708-736 over the boxes of ``_raster_boxes`` (outside
+        them the reference's passes change nothing); the refill loop reads
+        one counter per sweep.  ``current``: the bodies' coverage boxes and
+        ``cavity``: the last commit's unresolved cells, when the caller has
+        them."""
+        boxes = self._raster_boxes(current, cavity)
+        count, lo_p, hi_p = len(boxes[0]), boxes[0].ctypes.data, boxes[1].ctypes.data
+        self._rasterize(boxes)
+        self._repair_phase_layout(boxes)
+        self._call("find_refill_changes_boxes", count, lo_p, hi_p)
         control = self._read_control()
         # After this read the GPU waits for the host: the box arithmetic is on
         # plain ints (three components), not small NumPy arrays.
@@ -595,15 +646,21 @@ class IceFlow3D:
                 self.pending_cells = new
                 break
             remaining = new
-        self._call("commit_refill", int3(box[0]), int3(box[1]))
-        # gradients.py:388-400: changed box plus the mirror dependency radius.
+        self._call("commit_refill_boxes", count, lo_p, hi_p)
+        # gradients.py:198-209: changed box plus the dependency radius.  Every
+        # change lies in a raster box, so the part of that box within the
+        # radius of some raster box holds every gradient and class the
+        # reference's pass can change; elsewhere it rewrites the same values.
         lo = tuple(map(min, lower, control.geometry_lower))
         hi = tuple(map(max, upper, control.geometry_upper))
         if all(h >= l for l, h in zip(lo, hi)):
-            lo = tuple(max(v - 6, 0) for v in lo)
-            hi = tuple(min(v + 7, n) for v, n in zip(hi, shape))
-            self._call("refill_gradients", int3(lo), int3(hi))
-        # Labels changed only inside the refreshed box, which was reclassified.
+            radius = self.GRADIENT_DEPENDENCY_RADIUS
+            lo = np.array([max(v - radius, 0) for v in lo], np.int32)
+            hi = np.array([min(v + radius + 1, n) for v, n in zip(hi, shape)], np.int32)
+            region_lo = np.ascontiguousarray(np.maximum(boxes[0] - radius, lo), dtype=np.int32)
+            region_hi = np.ascontiguousarray(np.minimum(boxes[1] + radius, hi), dtype=np.int32)
+            self._call("refill_gradients_boxes", len(region_lo), region_lo.ctypes.data, region_hi.ctypes.data)
+        # Labels changed only inside the refreshed boxes, which were reclassified.
         self._flags_valid = True
         if has_openings or covered:
             self._call("project_refill", self._parity, int3(lower), int3(v + 1 for v in upper))
@@ -703,10 +760,13 @@ class IceFlow3D:
                 self._call("flow_step_end", self._parity)
                 self._loads_ready.synchronize()
                 loads = self._loads_view
+                cavity = self._cavity_box(self._cavity_view[:3], self._cavity_view[3:])
             else:
                 check(self._lib.icy3d_flow_step_profiled(self._gp, self._parity, int(refresh), classify, profile,
                                                          self._stream()), "flow_step")
-                loads = np.array(self._read_control().loads, dtype=np.float32)[: self.body_count]
+                control = self._read_control()
+                loads = np.array(control.loads, dtype=np.float32)[: self.body_count]
+                cavity = self._cavity_box(control.cavity_lower, control.cavity_upper)
                 check(self._lib.icy3d_profile_collect(profile), "profile_collect")
             self._gradient_cache_valid = True
             self._flags_valid = True
@@ -714,7 +774,7 @@ class IceFlow3D:
             current = self._rigid_step(loads)
             # The geometry refresh's first control read follows the recovery's
             # stability scan in stream order.
-            control = self._refresh_geometry(current=current)
+            control = self._refresh_geometry(current=current, cavity=cavity)
             speed_bound = None
             if self.config.check_stability:
                 if control.invalid_index < math.prod(self.shape):
