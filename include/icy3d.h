@@ -119,11 +119,14 @@ int icy3d_phase_warmup(const icy3d_grid* g, int steps, int* parity, void* stream
 int icy3d_classify(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream); /* home.py:280-284 */
 int icy3d_gradients(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream); /* gradients.py:352-386 */
 int icy3d_refill_gradients(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream); /* gradients.py:402-431 */
-/* icy3d_refill_gradients over `count` boxes (lo/hi: [count][3], at most
- * ICY3D_MAX_BODIES + 1): classification and gradients inside each box, the
- * boundary lists rebuilt once.  Boxes may overlap (both passes only write
- * values computed from unchanged inputs). */
-int icy3d_refill_gradients_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi, void* stream);
+/* icy3d_refill_gradients of the box [ref_lo, ref_hi) when every changed
+ * cell lies in one of `count` boxes (lo/hi: [count][3], at most
+ * ICY3D_MAX_BODIES + 1): classification within one cell and gradients within
+ * the dependency radius (6, gradients.py:15-20) of those boxes, inside the
+ * reference box, where the reference's pass can change them; the boundary
+ * lists rebuilt once. */
+int icy3d_refill_gradients_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi,
+                                 const int32_t ref_lo[3], const int32_t ref_hi[3], void* stream);
 int icy3d_collide_phase(const icy3d_grid* g, int parity, int moving_walls, void* stream); /* phase.py:90-127 */
 int icy3d_recover_phase(const icy3d_grid* g, int parity, int recover_flow, void* stream); /* phase.py:129-161 */
 int icy3d_recover_pressure(const icy3d_grid* g, void* stream);          /* home.py:347-353 */

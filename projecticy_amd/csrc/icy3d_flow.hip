@@ -343,11 +343,30 @@ __device__ __forceinline__ float staged_phi(const icy3d_grid& g, int i, int j, i
   return g.phi[lin(g, i, j, k)];
 }
 
-__global__ void __launch_bounds__(kGradJ* kGradK) k_gradients_tiled(icy3d_grid g, Box box) {
+// gradients.py:15-20: every phase and coverage dependency of a gradient lies
+// within this many cells.
+constexpr int kGradientRadius = 6;
+// Tiles of box m: ceil(ez / kGradK) x ceil(ey / kGradJ) x ceil(ex / kGradX),
+// k fastest; the boxes' tiles share one launch.
+__host__ __device__ inline int grad_tiles(const int lo[3], const int hi[3]) {
+  return ((hi[2] - lo[2] + kGradK - 1) / kGradK) * ((hi[1] - lo[1] + kGradJ - 1) / kGradJ) *
+         ((hi[0] - lo[0] + kGradX - 1) / kGradX);
+}
+__global__ void __launch_bounds__(kGradJ* kGradK) k_gradients_tiled(icy3d_grid g, BoxList tiles) {
   __shared__ float sp[kGradSlab];
+  int m = 0;
+  while (m + 1 < tiles.count && (int)blockIdx.x >= tiles.first[m + 1]) ++m;
+  Box box;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    box.lo[a] = tiles.lo[m][a];
+    box.hi[a] = tiles.hi[m][a];
+  }
+  const int tile = (int)blockIdx.x - tiles.first[m];
+  const int nk = (box.hi[2] - box.lo[2] + kGradK - 1) / kGradK, nj = (box.hi[1] - box.lo[1] + kGradJ - 1) / kGradJ;
   const int tk = threadIdx.x % kGradK, tj = threadIdx.x / kGradK;
-  const int k0 = box.lo[2] + blockIdx.x * kGradK, j0 = box.lo[1] + blockIdx.y * kGradJ;
-  const int x0 = box.lo[0] + blockIdx.z * kGradX, x_end = min(x0 + kGradX, box.hi[0]);
+  const int k0 = box.lo[2] + (tile % nk) * kGradK, j0 = box.lo[1] + (tile / nk % nj) * kGradJ;
+  const int x0 = box.lo[0] + tile / nk / nj * kGradX, x_end = min(x0 + kGradX, box.hi[0]);
   const int j = j0 + tj, k = k0 + tk;
   const bool mine = j < box.hi[1] && k < box.hi[2];
   float fetched[kGradRounds];
@@ -1887,16 +1906,9 @@ int launch_classify_all(const icy3d_grid& g, hipStream_t s) {
 // boxes.  Overlapping boxes write the same values twice.
 int launch_gradients_list(const icy3d_grid& g, const BoxList& L, hipStream_t s) {
   if (!L.count) return 0;
-  for (int m = 0; m < L.count; ++m) {
-    Box b;
-    for (int a = 0; a < 3; ++a) {
-      b.lo[a] = L.lo[m][a];
-      b.hi[a] = L.hi[m][a];
-    }
-    const dim3 tiles((b.hi[2] - b.lo[2] + kGradK - 1) / kGradK, (b.hi[1] - b.lo[1] + kGradJ - 1) / kGradJ,
-                     (b.hi[0] - b.lo[0] + kGradX - 1) / kGradX);
-    k_gradients_tiled<<<tiles, kGradJ * kGradK, 0, s>>>(g, b);
-  }
+  BoxList tiles = L;
+  for (int m = 0; m < L.count; ++m) tiles.first[m + 1] = tiles.first[m] + grad_tiles(L.lo[m], L.hi[m]);
+  k_gradients_tiled<<<(unsigned)tiles.first[L.count], kGradJ * kGradK, 0, s>>>(g, tiles);
   k_gradients_general_list<<<kListBlocks, kBlock, 0, s>>>(g, L);
   ICY_LAUNCH_CHECK();
   return 0;
@@ -1952,22 +1964,29 @@ int icy3d_gradients(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;
   return launch_gradients(*g, lo ? clamp_box(*g, make_box(lo, hi)) : full_box(*g), lo == nullptr, (hipStream_t)stream);
 }
-int icy3d_refill_gradients_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi, void* stream) {
-  if (!valid(g) || count < 0 || count > kMaxBoxes || (count && (!lo || !hi))) return ICY3D_ERR_ARGUMENT;
-  int32_t clo[kMaxBoxes * 3], chi[kMaxBoxes * 3];
-  for (int m = 0; m < count; ++m) {
-    const Box b = clamp_box(*g, make_box(lo + 3 * m, hi + 3 * m));
+int icy3d_refill_gradients_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi,
+                                 const int32_t ref_lo[3], const int32_t ref_hi[3], void* stream) {
+  if (!valid(g) || !ref_lo || !ref_hi || count < 0 || count > kMaxBoxes || (count && (!lo || !hi)))
+    return ICY3D_ERR_ARGUMENT;
+  // Classes depend on labels one cell away, gradients on cells within the
+  // dependency radius (gradients.py:15-20): only there, inside the reference
+  // box, can the refreshed values differ from the stored ones.
+  const Box ref = clamp_box(*g, make_box(ref_lo, ref_hi));
+  int32_t clo[kMaxBoxes * 3], chi[kMaxBoxes * 3], glo[kMaxBoxes * 3], ghi[kMaxBoxes * 3];
+  for (int m = 0; m < count; ++m)
     for (int a = 0; a < 3; ++a) {
-      clo[3 * m + a] = b.lo[a];
-      chi[3 * m + a] = b.hi[a];
+      clo[3 * m + a] = max(lo[3 * m + a] - 1, ref.lo[a]);
+      chi[3 * m + a] = min(hi[3 * m + a] + 1, ref.hi[a]);
+      glo[3 * m + a] = max(lo[3 * m + a] - kGradientRadius, ref.lo[a]);
+      ghi[3 * m + a] = min(hi[3 * m + a] + kGradientRadius, ref.hi[a]);
     }
-  }
-  BoxList L;
-  if (!make_box_list(*g, count, clo, chi, L)) return ICY3D_ERR_ARGUMENT;
+  BoxList classes, gradients;
+  if (!make_box_list(*g, count, clo, chi, classes) || !make_box_list(*g, count, glo, ghi, gradients))
+    return ICY3D_ERR_ARGUMENT;
   hipStream_t s = (hipStream_t)stream;
-  int rc = launch_classify_list(*g, L, s);
+  int rc = launch_classify_list(*g, classes, s);
   if (!rc) rc = launch_lists(*g, s);
-  return rc ? rc : launch_gradients_list(*g, L, s);
+  return rc ? rc : launch_gradients_list(*g, gradients, s);
 }
 int icy3d_refill_gradients(const icy3d_grid* g, const int32_t lo[3], const int32_t hi[3], void* stream) {
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;

@@ -22,11 +22,16 @@ __device__ __forceinline__ bool gbox_cell(const icy3d_grid& g, const GBox& b, in
   return true;
 }
 
-// simulator.py:233-251: trilinear material coverage of body b at a point.
-__device__ float body_fraction(const icy3d_grid& g, float px, float py, float pz, int b) {
+// simulator.py:233-251: trilinear material coverage of body b at a point;
+// `pose` is the body's uploaded row (origin, rotation) and `size` its
+// material size, both staged in LDS by the caller (material_grid's formula).
+__device__ float body_fraction(const icy3d_grid& g, float px, float py, float pz, int b, const float* pose,
+                               const int* size) {
   float grid[3];
-  material_grid(g, px, py, pz, b, grid);
-  const int* size = g.material_size + 3 * b;
+  const float dx = px - pose[0], dy = py - pose[1], dz = pz - pose[2];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+    grid[a] = (pose[3 + a] * dx + pose[6 + a] * dy + pose[9 + a] * dz) + 0.5f * (float)size[a] - 0.5f;
   // Outside [-1, size] on any axis no trilinear corner is inside the body.
   if (grid[0] <= -1.0f || grid[1] <= -1.0f || grid[2] <= -1.0f || grid[0] >= (float)size[0] ||
       grid[1] >= (float)size[1] || grid[2] >= (float)size[2])
@@ -43,14 +48,18 @@ __device__ float body_fraction(const icy3d_grid& g, float px, float py, float pz
   for (int t = 0; t < 8; ++t) {
     int a = t >> 2, c = (t >> 1) & 1, d = t & 1;
     int i = base[0] + a, j = base[1] + c, k = base[2] + d;
-    if (!mat_inside(g, b, i, j, k)) continue;
+    if ((unsigned)i >= (unsigned)size[0] || (unsigned)j >= (unsigned)size[1] || (unsigned)k >= (unsigned)size[2]) continue;
     float w = (a ? part[0] : 1.0f - part[0]) * (c ? part[1] : 1.0f - part[1]) * (d ? part[2] : 1.0f - part[2]);
     value += w * (g.material_mass[mat_index(g, b, i, j, k)] / g.initial_cell_mass);
   }
   return value;
 }
 
+// Bounds of the block's hit cells, folded into [lower, upper] by one thread
+// per block (same-address atomics serialise, so one set per block rather
+// than per wave).  Every thread of the block must call it.
 __device__ __forceinline__ void reduce_bounds(int32_t* lower, int32_t* upper, bool hit, const Cell& c) {
+  __shared__ int part[16][6];
   const int big = INT32_MAX, small = INT32_MIN;
   int lo[3] = {hit ? c.i : big, hit ? c.j : big, hit ? c.k : big};
   int hi[3] = {hit ? c.i : small, hit ? c.j : small, hit ? c.k : small};
@@ -59,13 +68,29 @@ __device__ __forceinline__ void reduce_bounds(int32_t* lower, int32_t* upper, bo
     lo[a] = wave_min_int(lo[a]);
     hi[a] = wave_max_int(hi[a]);
   }
-  if (wave_leader() && lo[0] != big) {
+  if (wave_leader()) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      atomicMin(lower + a, lo[a]);
-      atomicMax(upper + a, hi[a]);
+      part[threadIdx.x >> 6][a] = lo[a];
+      part[threadIdx.x >> 6][3 + a] = hi[a];
     }
   }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = min(lo[a], part[w][a]);
+        hi[a] = max(hi[a], part[w][3 + a]);
+      }
+    if (lo[0] != big)
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        atomicMin(lower + a, lo[a]);
+        atomicMax(upper + a, hi[a]);
+      }
+  }
+  __syncthreads();
 }
 
 __global__ void k_reset_geometry_bounds(icy3d_grid g) {
@@ -80,6 +105,15 @@ __global__ void k_reset_geometry_bounds(icy3d_grid g) {
 // label or coverage can change (each body's current and previous boxes and
 // the cells the last refill left unresolved).
 __global__ void k_rasterize(icy3d_grid g, BoxList boxes) {
+  // Every body's pose and material size, read once per block.
+  __shared__ float pose[ICY3D_MAX_BODIES][12];
+  __shared__ int size[ICY3D_MAX_BODIES][3];
+  for (int t = threadIdx.x; t < g.bodies * 15; t += blockDim.x) {
+    const int b = t / 15, e = t % 15;
+    if (e < 12) pose[b][e] = body_row(g, b)[e];
+    else size[b][e - 12] = g.material_size[3 * b + e - 12];
+  }
+  __syncthreads();
   Cell c = {0, 0, 0, 0};
   bool active = list_cell(g, boxes, c);
   bool changed = false;
@@ -93,7 +127,7 @@ __global__ void k_rasterize(icy3d_grid g, BoxList boxes) {
     int owner = -1;
     if (!wall)
       for (int body = 0; body < g.bodies; ++body) {
-        float f = body_fraction(g, c.i + 0.5f, c.j + 0.5f, c.k + 0.5f, body);
+        float f = body_fraction(g, c.i + 0.5f, c.j + 0.5f, c.k + 0.5f, body, pose[body], size[body]);
         if (f > fraction) {
           fraction = f;
           owner = body;

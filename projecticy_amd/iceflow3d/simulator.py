@@ -648,18 +648,16 @@ class IceFlow3D:
             remaining = new
         self._call("commit_refill_boxes", count, lo_p, hi_p)
         # gradients.py:198-209: changed box plus the dependency radius.  Every
-        # change lies in a raster box, so the part of that box within the
-        # radius of some raster box holds every gradient and class the
-        # reference's pass can change; elsewhere it rewrites the same values.
+        # change lies in a raster box, so the library refreshes only the parts
+        # of that box near the raster boxes; elsewhere the reference's pass
+        # rewrites the same values.
         lo = tuple(map(min, lower, control.geometry_lower))
         hi = tuple(map(max, upper, control.geometry_upper))
         if all(h >= l for l, h in zip(lo, hi)):
             radius = self.GRADIENT_DEPENDENCY_RADIUS
-            lo = np.array([max(v - radius, 0) for v in lo], np.int32)
-            hi = np.array([min(v + radius + 1, n) for v, n in zip(hi, shape)], np.int32)
-            region_lo = np.ascontiguousarray(np.maximum(boxes[0] - radius, lo), dtype=np.int32)
-            region_hi = np.ascontiguousarray(np.minimum(boxes[1] + radius, hi), dtype=np.int32)
-            self._call("refill_gradients_boxes", len(region_lo), region_lo.ctypes.data, region_hi.ctypes.data)
+            lo = tuple(max(v - radius, 0) for v in lo)
+            hi = tuple(min(v + radius + 1, n) for v, n in zip(hi, shape))
+            self._call("refill_gradients_boxes", count, lo_p, hi_p, int3(lo), int3(hi))
         # Labels changed only inside the refreshed boxes, which were reclassified.
         self._flags_valid = True
         if has_openings or covered:
