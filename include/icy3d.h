@@ -186,6 +186,16 @@ int icy3d_rasterize_boxes(const icy3d_grid* g, int count, const int32_t* lo, con
 int icy3d_find_refill_changes_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi,
                                     void* stream);
 int icy3d_commit_refill_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi, void* stream);
+/* simulator.py:708-736 over such boxes in two calls around the host's refill
+ * loop.  _begin: one reset of the geometry and refill bounds, the refill
+ * counters and the cavity bounds, then rasterize, the phase-layout repair
+ * (parity odd; boxes extended one cell down) and find_refill_changes.
+ * _end: commit_refill (its cavity bounds reset by _begin), then
+ * icy3d_refill_gradients_boxes of [ref_lo, ref_hi) unless ref_lo is NULL. */
+int icy3d_refresh_begin_boxes(const icy3d_grid* g, int parity, int count, const int32_t* lo, const int32_t* hi,
+                              void* stream);
+int icy3d_refresh_end_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi,
+                            const int32_t ref_lo[3], const int32_t ref_hi[3], void* stream);
 /* Rebuild the tank (flag 1) and ice (flag 2) cell lists from the flags. */
 int icy3d_build_boundary_lists(const icy3d_grid* g, void* stream);
 int icy3d_project_refill(const icy3d_grid* g, int parity, const int32_t lo[3], const int32_t hi[3], void* stream);

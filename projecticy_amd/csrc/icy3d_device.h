@@ -325,8 +325,9 @@ namespace {
 // partials in a fixed order and adds the total to *target.
 // Four loads are in flight per thread per round.
 constexpr int kFinishThreads = 1024;
-__global__ void __launch_bounds__(kFinishThreads) k_finish_sum(const float* __restrict__ partials, int count,
-                                                              float* target) {
+// The fold itself (blockDim.x == kFinishThreads); the total is valid in
+// thread 0.
+__device__ __forceinline__ float finish_total(const float* __restrict__ partials, int count) {
   float v = 0.0f;
   int t = threadIdx.x;
   for (; t + 3 * kFinishThreads < count; t += 4 * kFinishThreads) {
@@ -335,7 +336,11 @@ __global__ void __launch_bounds__(kFinishThreads) k_finish_sum(const float* __re
     v += (a + b) + (c + d);
   }
   for (; t < count; t += kFinishThreads) v += partials[t];
-  const float total = icy::block_sum(v);
+  return icy::block_sum(v);
+}
+__global__ void __launch_bounds__(kFinishThreads) k_finish_sum(const float* __restrict__ partials, int count,
+                                                              float* target) {
+  const float total = finish_total(partials, count);
   if (threadIdx.x == 0) *target += total;
 }
 }  // namespace

@@ -579,34 +579,60 @@ __global__ void k_recover_phase(icy3d_grid g, int parity, int recover_flow) {
     }
   }
   if (!isfinite(clip)) clip = 0.0f;
-  const float block_clip = block_sum(clip);
-  if (threadIdx.x == 0) g.partials[blockIdx.x] = block_clip;
-  if (recover_flow && g.check_stability) {
-    // Per-block maximum speed and first invalid index (exact in FP32 below
-    // 2^24 cells), folded by k_fold_stability.
-    const float speed = block_max(st.speed);
-    const int invalid = block_min_int(st.invalid);
-    if (threadIdx.x == 0) {
+  // Per-block clip sum, maximum speed and first invalid index (exact in FP32
+  // below 2^24 cells), reduced per wave and then across the block's waves
+  // with one barrier; folded by k_fold_recovery.
+  const bool scan = recover_flow && g.check_stability;
+  __shared__ float wave_clip[kBlock / 64], wave_speed[kBlock / 64];
+  __shared__ int wave_invalid[kBlock / 64];
+  const float clip_w = wave_sum(clip);
+  const float speed_w = scan ? wave_max(st.speed) : 0.0f;
+  const int invalid_w = scan ? wave_min_int(st.invalid) : INT32_MAX;
+  if (wave_leader()) {
+    wave_clip[threadIdx.x >> 6] = clip_w;
+    wave_speed[threadIdx.x >> 6] = speed_w;
+    wave_invalid[threadIdx.x >> 6] = invalid_w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float total = 0.0f, speed = wave_speed[0];
+    int invalid = wave_invalid[0];
+    for (int w = 0; w < kBlock / 64; ++w) total += wave_clip[w];
+    for (int w = 1; w < kBlock / 64; ++w) {
+      speed = fmaxf(speed, wave_speed[w]);
+      invalid = min(invalid, wave_invalid[w]);
+    }
+    g.partials[blockIdx.x] = total;
+    if (scan) {
       g.partials[gridDim.x + blockIdx.x] = speed;
       g.partials[2 * gridDim.x + blockIdx.x] = invalid == INT32_MAX ? -1.0f : (float)invalid;
     }
   }
 }
-// Folds the per-block stability partials of k_recover_phase into the control block.
-__global__ void __launch_bounds__(1024) k_fold_stability(icy3d_grid g, int blocks) {
+// Folds the per-block partials of k_recover_phase: the clip volume (the
+// fixed-order sum of k_finish_sum) and, with `scan`, the maximum speed and
+// first invalid index (simulator.py:778-805), which replace the previous
+// step's values.
+__global__ void __launch_bounds__(kFinishThreads) k_fold_recovery(icy3d_grid g, int blocks, int scan) {
+  const float clip = finish_total(g.partials, blocks);
   float speed = 0.0f;
   int invalid = INT32_MAX;
-  for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
-    speed = fmaxf(speed, g.partials[blocks + b]);
-    const float v = g.partials[2 * blocks + b];
-    if (v >= 0.0f) invalid = min(invalid, (int)v);
+  if (scan) {
+    for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+      speed = fmaxf(speed, g.partials[blocks + b]);
+      const float v = g.partials[2 * blocks + b];
+      if (v >= 0.0f) invalid = min(invalid, (int)v);
+    }
+    speed = block_max(speed);
+    invalid = block_min_int(invalid);
   }
-  speed = block_max(speed);
-  invalid = block_min_int(invalid);
   if (threadIdx.x == 0) {
-    g.control->maximum_speed = speed;
-    if (invalid != INT32_MAX) g.control->invalid_index = min(g.control->invalid_index, invalid);
-    g.control->peak_speed = fmaxf(g.control->peak_speed, speed);
+    g.control->phase_clip_volume += clip;
+    if (scan) {
+      g.control->maximum_speed = speed;
+      g.control->invalid_index = invalid == INT32_MAX ? (int32_t)cells(g) : invalid;
+      g.control->peak_speed = fmaxf(g.control->peak_speed, speed);
+    }
   }
 }
 
@@ -1436,9 +1462,13 @@ __global__ void k_set_equilibrium(icy3d_grid g, int parity, int with_phase, int 
 
 // ---------------------------------------------------------------------------
 // HOME collision (home.py:199-211).
-__global__ void k_collide_momentum(icy3d_grid g) {
+// zero_loads != 0: the first block also clears the body loads that the
+// stream kernels accumulate next (one launch fewer per step).
+__global__ void k_collide_momentum(icy3d_grid g, int zero_loads) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   const size_t N = cells(g);
+  if (zero_loads && blockIdx.x == 0)
+    for (int t = threadIdx.x; t < ICY3D_MAX_BODIES * 6; t += blockDim.x) (&g.control->loads[0][0])[t] = 0.0f;
   if (n >= (int)N || g.id[n] != -1) return;
 #pragma unroll
   for (int e = 0; e < 3; ++e) g.old_u[e * N + n] = g.u[e * N + n];
@@ -1928,10 +1958,8 @@ int launch_recover(const icy3d_grid& g, int parity, int recover_flow, hipStream_
   const unsigned blocks = icy_blocks(grid_cells(g));
   if ((int)blocks > g.partials_capacity) return ICY3D_ERR_ARGUMENT;
   if (scan && 3 * (int)blocks > g.partials_capacity) return ICY3D_ERR_ARGUMENT;
-  if (scan) k_reset_stability<<<1, 1, 0, s>>>(g);
   k_recover_phase<<<blocks, kBlock, 0, s>>>(g, parity, recover_flow);
-  k_finish_sum<<<1, kFinishThreads, 0, s>>>(g.partials, (int)blocks, &g.control->phase_clip_volume);
-  if (scan) k_fold_stability<<<1, 1024, 0, s>>>(g, (int)blocks);
+  k_fold_recovery<<<1, kFinishThreads, 0, s>>>(g, (int)blocks, scan ? 1 : 0);
   ICY_LAUNCH_CHECK();
   return 0;
 }
@@ -2051,7 +2079,7 @@ int icy3d_forces(const icy3d_grid* g, int with_flux, int moment_pressure, void* 
 }
 int icy3d_collide_momentum(const icy3d_grid* g, void* stream) {
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;
-  k_collide_momentum<<<icy_blocks(grid_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g);
+  k_collide_momentum<<<icy_blocks(grid_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g, 0);
   ICY_LAUNCH_CHECK();
   return 0;
 }
@@ -2085,8 +2113,7 @@ int icy3d_flow_step_profiled(const icy3d_grid* g, int parity, int refresh_gradie
   rc = launch_forces(G, 1, 1, s);
   if (rc) return rc;
   mark(2);
-  k_collide_momentum<<<blocks, kBlock, 0, s>>>(G);
-  k_zero_loads<<<1, ICY3D_MAX_BODIES * 6, 0, s>>>(G);
+  k_collide_momentum<<<blocks, kBlock, 0, s>>>(G, 1);
   mark(3);
   rc = launch_stream(G, s);
   mark(4);
@@ -2121,8 +2148,7 @@ int icy3d_flow_step_begin(const icy3d_grid* g, int parity, int refresh_gradients
   k_collide_phase<<<blocks, kBlock, 0, s>>>(G, parity, 1);
   rc = launch_forces(G, 1, 1, s);
   if (rc) return rc;
-  k_collide_momentum<<<blocks, kBlock, 0, s>>>(G);
-  k_zero_loads<<<1, ICY3D_MAX_BODIES * 6, 0, s>>>(G);
+  k_collide_momentum<<<blocks, kBlock, 0, s>>>(G, 1);
   return launch_stream(G, s);
 }
 int icy3d_flow_step_end(const icy3d_grid* g, int parity, void* stream) {

@@ -265,6 +265,22 @@ __global__ void k_accept_refill(icy3d_grid g, GBox box) {
   g.refill_valid[c.n] = g.refill_next_valid[c.n];
 }
 
+// The three resets below in one launch, for a refresh run by
+// icy3d_refresh_begin_boxes / icy3d_refresh_end_boxes.
+__global__ void k_reset_refresh(icy3d_grid g) {
+  const int shape[3] = {g.nx, g.ny, g.nz};
+  icy3d_control* ctl = g.control;
+  ctl->openings = 0;
+  ctl->covered = 0;
+  for (int a = 0; a < 3; ++a) {
+    ctl->geometry_lower[a] = shape[a];
+    ctl->geometry_upper[a] = -1;
+    ctl->refill_lower[a] = shape[a];
+    ctl->refill_upper[a] = -1;
+    ctl->cavity_lower[a] = shape[a];
+    ctl->cavity_upper[a] = -1;
+  }
+}
 __global__ void k_reset_cavity_bounds(icy3d_grid g) {
   const int shape[3] = {g.nx, g.ny, g.nz};
   for (int a = 0; a < 3; ++a) {
@@ -417,6 +433,34 @@ int icy3d_commit_refill(const icy3d_grid* g, const int32_t lo[3], const int32_t 
   if (!valid_grid(g)) return ICY3D_ERR_ARGUMENT;
   const GBox b = lo ? make_gbox(lo, hi) : full_gbox(*g);
   return icy3d_commit_refill_boxes(g, 1, b.lo, b.hi, stream);
+}
+
+int icy3d_refresh_begin_boxes(const icy3d_grid* g, int parity, int count, const int32_t* lo, const int32_t* hi,
+                              void* stream) {
+  BoxList L;
+  if (!valid_grid(g) || !make_box_list(*g, count, lo, hi, L)) return ICY3D_ERR_ARGUMENT;
+  // A link (I, I+e) is swapped by the thread of I: each box extended one cell down.
+  int32_t down[kMaxBoxes * 3];
+  for (int m = 0; m < count; ++m)
+    for (int a = 0; a < 3; ++a) down[3 * m + a] = lo[3 * m + a] > 0 ? lo[3 * m + a] - 1 : 0;
+  hipStream_t s = (hipStream_t)stream;
+  k_reset_refresh<<<1, 1, 0, s>>>(*g);
+  if (list_blocks(L)) k_rasterize<<<list_blocks(L), kBlock, 0, s>>>(*g, L);
+  ICY_LAUNCH_CHECK();
+  const int rc = icy3d_repair_phase_layout_boxes(g, parity, count, down, hi, stream);
+  if (rc) return rc;
+  if (list_blocks(L)) k_find_refill_changes<<<list_blocks(L), kBlock, 0, s>>>(*g, L);
+  ICY_LAUNCH_CHECK();
+  return 0;
+}
+int icy3d_refresh_end_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi,
+                            const int32_t ref_lo[3], const int32_t ref_hi[3], void* stream) {
+  BoxList L;
+  if (!valid_grid(g) || !make_box_list(*g, count, lo, hi, L)) return ICY3D_ERR_ARGUMENT;
+  if (list_blocks(L)) k_commit_refill<<<list_blocks(L), kBlock, 0, (hipStream_t)stream>>>(*g, L);
+  ICY_LAUNCH_CHECK();
+  if (!ref_lo || !ref_hi) return 0;
+  return icy3d_refill_gradients_boxes(g, count, lo, hi, ref_lo, ref_hi, stream);
 }
 
 }  // extern "C"

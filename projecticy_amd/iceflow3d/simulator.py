@@ -127,6 +127,7 @@ class IceFlow3D:
         }
         self._control_host = torch.zeros(ctypes.sizeof(Control), dtype=torch.uint8).pin_memory()
         self._loads_host = torch.zeros(ctypes.sizeof(Control), dtype=torch.uint8).pin_memory()
+        self._control_alias = Control.from_buffer(self._control_host.numpy())
         self._loads_ready = torch.cuda.Event()
         self._upload_staging = torch.zeros((B, _native.STATE_STRIDE), dtype=torch.float32).pin_memory()
         self._upload_staging_view = self._upload_staging.numpy()
@@ -348,6 +349,12 @@ class IceFlow3D:
     def _read_control(self):
         self._control_host.copy_(self._t["control"])
         return Control.from_buffer_copy(self._control_host.numpy().tobytes())
+
+    def _sync_control(self):
+        """The control block copied into the pinned host buffer, returned as a
+        struct that aliases that buffer (valid until the next copy)."""
+        self._control_host.copy_(self._t["control"])
+        return self._control_alias
 
     def _write_control(self, control):
         raw = np.frombuffer(bytes(control), dtype=np.uint8)
@@ -623,48 +630,48 @@ class IceFlow3D:
         them."""
         boxes = self._raster_boxes(current, cavity)
         count, lo_p, hi_p = len(boxes[0]), boxes[0].ctypes.data, boxes[1].ctypes.data
-        self._rasterize(boxes)
-        self._repair_phase_layout(boxes)
-        self._call("find_refill_changes_boxes", count, lo_p, hi_p)
-        control = self._read_control()
-        # After this read the GPU waits for the host: the box arithmetic is on
-        # plain ints (three components), not small NumPy arrays.
+        self._call("refresh_begin_boxes", self._parity, count, lo_p, hi_p)
+        self._flags_valid = False
+        # After this read the GPU waits for the host until the next launch:
+        # the struct aliases the pinned copy and the box arithmetic is on
+        # plain ints (three components).
+        control = self._sync_control()
         remaining, covered = int(control.openings), int(control.covered)
         shape = self.shape
         lower, upper = tuple(control.refill_lower), tuple(control.refill_upper)
-        box_lo = tuple(max(v - 1, 0) for v in lower)
-        box_hi = tuple(min(v + 2, n) for v, n in zip(upper, shape))
-        if remaining > 0:
-            self._call("seed_refill_region", int3(box_lo), int3(box_hi))
-        has_openings = remaining > 0
-        self.pending_cells = 0
-        while remaining:
-            self._call("propagate_refill", int3(box_lo), int3(box_hi))
-            new = int(self._read_control().remaining)
-            self._call("accept_refill", int3(box_lo), int3(box_hi))
-            if new == remaining:
-                self.pending_cells = new
-                break
-            remaining = new
-        self._call("commit_refill_boxes", count, lo_p, hi_p)
         # gradients.py:198-209: changed box plus the dependency radius.  Every
         # change lies in a raster box, so the library refreshes only the parts
         # of that box near the raster boxes; elsewhere the reference's pass
         # rewrites the same values.
         lo = tuple(map(min, lower, control.geometry_lower))
         hi = tuple(map(max, upper, control.geometry_upper))
+        reference_box = (None, None)
         if all(h >= l for l, h in zip(lo, hi)):
             radius = self.GRADIENT_DEPENDENCY_RADIUS
-            lo = tuple(max(v - radius, 0) for v in lo)
-            hi = tuple(min(v + radius + 1, n) for v, n in zip(hi, shape))
-            self._call("refill_gradients_boxes", count, lo_p, hi_p, int3(lo), int3(hi))
+            reference_box = (int3(max(v - radius, 0) for v in lo),
+                             int3(min(v + radius + 1, n) for v, n in zip(hi, shape)))
+        if remaining > 0:
+            box_lo = int3(max(v - 1, 0) for v in lower)
+            box_hi = int3(min(v + 2, n) for v, n in zip(upper, shape))
+            self._call("seed_refill_region", box_lo, box_hi)
+        has_openings = remaining > 0
+        self.pending_cells = 0
+        while remaining:
+            self._call("propagate_refill", box_lo, box_hi)
+            new = int(self._sync_control().remaining)
+            self._call("accept_refill", box_lo, box_hi)
+            if new == remaining:
+                self.pending_cells = new
+                break
+            remaining = new
+        self._call("refresh_end_boxes", count, lo_p, hi_p, *reference_box)
         # Labels changed only inside the refreshed boxes, which were reclassified.
         self._flags_valid = True
         if has_openings or covered:
             self._call("project_refill", self._parity, int3(lower), int3(v + 1 for v in upper))
         if phase_changed or has_openings or covered:
             self._synchronize_water()
-        return control
+        return Control.from_buffer_copy(control)
 
     def _advance_advection(self, speed_bound=None):
         """thermal.py:208-226."""
@@ -674,7 +681,7 @@ class IceFlow3D:
         rate = 3 * speed_bound if speed_bound is not None else math.inf
         if rate > th.max_courant_number:
             self._measure_outflow()
-            rate = float(self._read_control().max_outflow)
+            rate = float(self._sync_control().max_outflow)
         if not math.isfinite(rate):
             raise RuntimeError(f"nonfinite thermal velocity at step {self.steps + 1}")
         substeps = max(1, math.ceil(rate / th.max_courant_number))
