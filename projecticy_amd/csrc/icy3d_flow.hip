@@ -21,8 +21,10 @@ struct icy3d_profile {
 namespace icy {
 
 // ---------------------------------------------------------------------------
-// Runtime-direction reconstruction (boundary paths only).
-__device__ __noinline__ float reconstruct_rt(int q, const Moments& M) {
+// Runtime-direction reconstruction, inlined where every lane of a boundary
+// group uses it; reconstruct_rt is the same code out of line for the rare
+// reflection path.
+__device__ __forceinline__ float reconstruct_dyn(int q, const Moments& M) {
   const float fx = (float)cx(q), fy = (float)cy(q), fz = (float)cz(q);
   const float p = M.m[0], ux = M.m[1], uy = M.m[2], uz = M.m[3];
   const float sxx = M.m[4], syy = M.m[5], szz = M.m[6], sxy = M.m[7], sxz = M.m[8], syz = M.m[9];
@@ -40,6 +42,7 @@ __device__ __noinline__ float reconstruct_rt(int q, const Moments& M) {
   third += 2.0f * fx * fy * fz * (sxz * uy + syz * ux + sxy * uz - 2.0f * ux * uy * uz);
   return w27(q) * (value + 13.5f * third);
 }
+__device__ __noinline__ float reconstruct_rt(int q, const Moments& M) { return reconstruct_dyn(q, M); }
 
 // ---------------------------------------------------------------------------
 // AA addressing of D3Q7 (phase.py:69-82): plane offset of logical slot q.
@@ -1773,7 +1776,7 @@ __global__ void __launch_bounds__(256) k_stream_boundary(icy3d_grid g) {
     if (lane < 27) {
       const int i = c.i - cx(q), j = c.j - cy(q), k = c.k - cz(q);
       if (fluid_at(g, i, j, k)) {
-        v = reconstruct_rt(q, load_moments(g.home_post, N, lin(g, i, j, k)));
+        v = reconstruct_dyn(q, load_moments(g.home_post, N, lin(g, i, j, k)));
       } else {
         int body;
         float load[6];
