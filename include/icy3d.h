@@ -147,6 +147,12 @@ int icy3d_flow_step(const icy3d_grid* g, int parity, int refresh_gradients, int 
  * while _end runs. */
 int icy3d_flow_step_begin(const icy3d_grid* g, int parity, int refresh_gradients, int classify, void* stream);
 int icy3d_flow_step_end(const icy3d_grid* g, int parity, void* stream);
+/* _end without the gradient pass, for a caller that refreshes the geometry
+ * next and runs that pass after it (icy3d_refresh_end_boxes with
+ * full_gradients): outside the reference's refresh box nothing within the
+ * gradients' dependency radius changes in between, so one pass after the
+ * refresh gives the field the reference has after its refresh. */
+int icy3d_flow_step_recover(const icy3d_grid* g, int parity, void* stream);
 
 /* Kernel-group timing of the fused flow step with HIP events on the same
  * stream (no reference counterpart; the reference measures with Taichi's
@@ -191,11 +197,13 @@ int icy3d_commit_refill_boxes(const icy3d_grid* g, int count, const int32_t* lo,
  * counters and the cavity bounds, then rasterize, the phase-layout repair
  * (parity odd; boxes extended one cell down) and find_refill_changes.
  * _end: commit_refill (its cavity bounds reset by _begin), then
- * icy3d_refill_gradients_boxes of [ref_lo, ref_hi) unless ref_lo is NULL. */
+ * icy3d_refill_gradients_boxes of [ref_lo, ref_hi) unless ref_lo is NULL;
+ * with full_gradients, the same class refresh and one gradient pass over the
+ * whole grid (after icy3d_flow_step_recover). */
 int icy3d_refresh_begin_boxes(const icy3d_grid* g, int parity, int count, const int32_t* lo, const int32_t* hi,
                               void* stream);
 int icy3d_refresh_end_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi,
-                            const int32_t ref_lo[3], const int32_t ref_hi[3], void* stream);
+                            const int32_t ref_lo[3], const int32_t ref_hi[3], int full_gradients, void* stream);
 /* Rebuild the tank (flag 1) and ice (flag 2) cell lists from the flags. */
 int icy3d_build_boundary_lists(const icy3d_grid* g, void* stream);
 int icy3d_project_refill(const icy3d_grid* g, int parity, const int32_t lo[3], const int32_t hi[3], void* stream);

@@ -96,7 +96,7 @@ _GRID_CALLS = {
     "build_boundary_lists": [],
     "rasterize_boxes": [_i, _p, _p], "find_refill_changes_boxes": [_i, _p, _p], "commit_refill_boxes": [_i, _p, _p],
     "repair_phase_layout_boxes": [_i, _i, _p, _p], "refill_gradients_boxes": [_i, _p, _p, _I3, _I3],
-    "refresh_begin_boxes": [_i, _i, _p, _p], "refresh_end_boxes": [_i, _p, _p, _I3, _I3],
+    "refresh_begin_boxes": [_i, _i, _p, _p], "refresh_end_boxes": [_i, _p, _p, _I3, _I3, _i], "flow_step_recover": [_i],
     "project_refill": [_i, _I3, _I3],
     "synchronize_water": [], "thermal_fluxes": [_i], "apply_thermal_fluxes": [_f, _i],
     "measure_outflow": [], "body_conduction": [_f], "exchange_heat": [_f, _I3, _I3],

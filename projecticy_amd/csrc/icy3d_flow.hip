@@ -1977,6 +1977,29 @@ int launch_stream(const icy3d_grid& g, hipStream_t s) {
 }
 }  // namespace
 
+namespace icy {
+// The class refresh of icy3d_refill_gradients_boxes (none when ref_lo is
+// NULL), then one gradient pass over the whole grid.
+int refresh_classes_full_gradients(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi,
+                                   const int32_t ref_lo[3], const int32_t ref_hi[3], hipStream_t s) {
+  if (ref_lo && ref_hi) {
+    const Box ref = clamp_box(*g, make_box(ref_lo, ref_hi));
+    int32_t clo[kMaxBoxes * 3], chi[kMaxBoxes * 3];
+    for (int m = 0; m < count; ++m)
+      for (int a = 0; a < 3; ++a) {
+        clo[3 * m + a] = max(lo[3 * m + a] - 1, ref.lo[a]);
+        chi[3 * m + a] = min(hi[3 * m + a] + 1, ref.hi[a]);
+      }
+    BoxList classes;
+    if (!make_box_list(*g, count, clo, chi, classes)) return ICY3D_ERR_ARGUMENT;
+    int rc = launch_classify_list(*g, classes, s);
+    if (!rc) rc = launch_lists(*g, s);
+    if (rc) return rc;
+  }
+  return launch_gradients(*g, full_box(*g), true, s);
+}
+}  // namespace icy
+
 extern "C" {
 
 int icy3d_abi_sizes(int32_t* grid_bytes, int32_t* control_bytes) {
@@ -2160,6 +2183,10 @@ int icy3d_flow_step_end(const icy3d_grid* g, int parity, void* stream) {
   int rc = launch_recover(*g, 1 - parity, 1, s);
   if (!rc) rc = launch_gradients(*g, full_box(*g), true, s);
   return rc;
+}
+int icy3d_flow_step_recover(const icy3d_grid* g, int parity, void* stream) {
+  if (!valid(g)) return ICY3D_ERR_ARGUMENT;
+  return launch_recover(*g, 1 - parity, 1, (hipStream_t)stream);
 }
 int icy3d_profile_create(icy3d_profile** out) {
   if (!out) return ICY3D_ERR_ARGUMENT;

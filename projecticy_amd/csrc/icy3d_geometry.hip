@@ -330,6 +330,12 @@ __global__ void k_commit_refill(icy3d_grid g, BoxList boxes) {
 
 using namespace icy;
 
+namespace icy {
+// Defined with the flow kernels.
+int refresh_classes_full_gradients(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi,
+                                   const int32_t ref_lo[3], const int32_t ref_hi[3], hipStream_t s);
+}  // namespace icy
+
 namespace {
 bool valid_grid(const icy3d_grid* g) {
   return g && g->nx > 0 && g->ny > 0 && g->nz > 0 && g->bodies >= 0 && g->bodies <= ICY3D_MAX_BODIES && g->control &&
@@ -454,11 +460,12 @@ int icy3d_refresh_begin_boxes(const icy3d_grid* g, int parity, int count, const 
   return 0;
 }
 int icy3d_refresh_end_boxes(const icy3d_grid* g, int count, const int32_t* lo, const int32_t* hi,
-                            const int32_t ref_lo[3], const int32_t ref_hi[3], void* stream) {
+                            const int32_t ref_lo[3], const int32_t ref_hi[3], int full_gradients, void* stream) {
   BoxList L;
   if (!valid_grid(g) || !make_box_list(*g, count, lo, hi, L)) return ICY3D_ERR_ARGUMENT;
   if (list_blocks(L)) k_commit_refill<<<list_blocks(L), kBlock, 0, (hipStream_t)stream>>>(*g, L);
   ICY_LAUNCH_CHECK();
+  if (full_gradients) return refresh_classes_full_gradients(g, count, lo, hi, ref_lo, ref_hi, (hipStream_t)stream);
   if (!ref_lo || !ref_hi) return 0;
   return icy3d_refill_gradients_boxes(g, count, lo, hi, ref_lo, ref_hi, stream);
 }

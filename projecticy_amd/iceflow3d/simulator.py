@@ -622,12 +622,13 @@ class IceFlow3D:
             self._raise_unstable(control.invalid_index)
         return float(control.maximum_speed)
 
-    def _refresh_geometry(self, phase_changed=True, current=None, cavity=()):
+    def _refresh_geometry(self, phase_changed=True, current=None, cavity=(), full_gradients=False):
         """simulator.py:708-736 over the boxes of ``_raster_boxes`` (outside
         them the reference's passes change nothing); the refill loop reads
         one counter per sweep.  ``current``: the bodies' coverage boxes and
         ``cavity``: the last commit's unresolved cells, when the caller has
-        them."""
+        them.  ``full_gradients``: the flow step left out its gradient pass,
+        which runs over the whole grid here after the class refresh."""
         boxes = self._raster_boxes(current, cavity)
         count, lo_p, hi_p = len(boxes[0]), boxes[0].ctypes.data, boxes[1].ctypes.data
         self._call("refresh_begin_boxes", self._parity, count, lo_p, hi_p)
@@ -664,7 +665,7 @@ class IceFlow3D:
                 self.pending_cells = new
                 break
             remaining = new
-        self._call("refresh_end_boxes", count, lo_p, hi_p, *reference_box)
+        self._call("refresh_end_boxes", count, lo_p, hi_p, *reference_box, int(full_gradients))
         # Labels changed only inside the refreshed boxes, which were reclassified.
         self._flags_valid = True
         if has_openings or covered:
@@ -757,12 +758,15 @@ class IceFlow3D:
             classify = int(not self._flags_valid)
             if profile is None:
                 # The loads are final after the stream: copy them out, queue the
-                # rest of the flow step, and integrate the bodies on the host
-                # while the GPU runs recovery and the gradients.
+                # recovery, and integrate the bodies on the host while the GPU
+                # runs it.  The step's gradient pass runs once, after the
+                # geometry refresh (the field the reference has after its
+                # refresh: outside its refresh box nothing the gradients
+                # depend on changes in between).
                 self._call("flow_step_begin", self._parity, int(refresh), classify)
                 self._loads_host.copy_(self._t["control"], non_blocking=True)
                 self._loads_ready.record(self._torch_stream())
-                self._call("flow_step_end", self._parity)
+                self._call("flow_step_recover", self._parity)
                 self._loads_ready.synchronize()
                 loads = self._loads_view
                 cavity = self._cavity_box(self._cavity_view[:3], self._cavity_view[3:])
@@ -779,7 +783,7 @@ class IceFlow3D:
             current = self._rigid_step(loads)
             # The geometry refresh's first control read follows the recovery's
             # stability scan in stream order.
-            control = self._refresh_geometry(current=current, cavity=cavity)
+            control = self._refresh_geometry(current=current, cavity=cavity, full_gradients=profile is None)
             speed_bound = None
             if self.config.check_stability:
                 if control.invalid_index < math.prod(self.shape):

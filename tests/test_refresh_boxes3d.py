@@ -194,3 +194,22 @@ def test_unresolved_cell_far_from_bodies_is_relabelled(tmp_path):
     _refresh(whole, True)
     _assert_same(boxed, whole)
     assert int(boxed.previous_body_id.to_numpy()[i, j, k]) == -1
+
+
+@pytest.mark.gpu
+def test_one_gradient_pass_after_the_refresh_matches_the_reference_order(tmp_path):
+    """IceFlow3D.step leaves the gradient pass out of the flow step and runs
+    it over the whole grid after the geometry refresh.  The reference runs it
+    at the end of the flow step and refreshes it inside the changed box after
+    the refresh (gradients.py:198-240); both give the same field."""
+    reference_order, deferred = _pair(tmp_path)
+    rng = np.random.default_rng(8)
+    velocity = rng.uniform(-0.3, 0.3, (deferred.body_count, 3))
+    for sim in (reference_order, deferred):
+        sim.rigid.velocity[...] = velocity
+    loads = np.zeros((deferred.body_count, 6), np.float32)
+    for _ in range(4):
+        reference_order._call("gradients", None, None)
+        reference_order._refresh_geometry(current=reference_order._rigid_step(loads))
+        deferred._refresh_geometry(current=deferred._rigid_step(loads), full_gradients=True)
+        _assert_same(reference_order, deferred)
