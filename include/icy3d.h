@@ -267,6 +267,11 @@ int icy3d_rigid_step(const icy3d_rigid_arrays* s, const float* loads, const doub
                      double angular_damping, const int shape[3], int boundary, int iterations, const int32_t* sizes,
                      float* state, int32_t box_lo[3], int32_t box_hi[3], int* failed_body, double residual[2]);
 
+/* Stream-ordered copy between any two of device, pinned host and host
+ * memory (hipMemcpyAsync); no reference counterpart: the host's per-step
+ * upload of body rows without a framework tensor copy. */
+int icy3d_copy_async(void* dst, const void* src, size_t bytes, void* stream);
+
 /* Size of the structs this library was built with (ABI check). */
 int icy3d_abi_sizes(int32_t* grid_bytes, int32_t* control_bytes);
 

@@ -134,6 +134,8 @@ def lib():
     handle.icy3d_probe_viscosity.argtypes = [_p, _i, _p, _i, _f, _f, _f, _p, _p]
     for name in ("probe_home", "probe_phase", "probe_viscosity"):
         getattr(handle, "icy3d_" + name).restype = ctypes.c_int
+    handle.icy3d_copy_async.argtypes = [_p, _p, ctypes.c_size_t, _p]
+    handle.icy3d_copy_async.restype = ctypes.c_int
     handle.icy3d_profile_create.argtypes = [ctypes.POINTER(_p)]
     handle.icy3d_profile_destroy.argtypes = [_p]
     handle.icy3d_profile_collect.argtypes = [_p]

@@ -2002,6 +2002,11 @@ int refresh_classes_full_gradients(const icy3d_grid* g, int count, const int32_t
 
 extern "C" {
 
+int icy3d_copy_async(void* dst, const void* src, size_t bytes, void* stream) {
+  if (!bytes) return 0;
+  if (!dst || !src) return ICY3D_ERR_ARGUMENT;
+  return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, (hipStream_t)stream);
+}
 int icy3d_abi_sizes(int32_t* grid_bytes, int32_t* control_bytes) {
   *grid_bytes = (int32_t)sizeof(icy3d_grid);
   *control_bytes = (int32_t)sizeof(icy3d_control);

@@ -45,6 +45,7 @@ class IceFlow3D:
             raise ValueError(f"at most {_native.MAX_BODIES} ice bodies are supported")
         self.device = _require_device()
         self._lib = lib()
+        self._functions = {}
         self._step_stream = None
         self.config, self.scales = config, LatticeScales.from_iceflow_config(config)
         self.shape = (config.nx, config.ny, config.nz)
@@ -344,7 +345,10 @@ class IceFlow3D:
         return held[1] if held is not None else ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
     def _call(self, name, *args):
-        check(getattr(self._lib, "icy3d_" + name)(self._gp, *args, self._stream()), name)
+        fn = self._functions.get(name)
+        if fn is None:
+            fn = self._functions[name] = getattr(self._lib, "icy3d_" + name)
+        check(fn(self._gp, *args, self._stream()), name)
 
     def _read_control(self):
         self._control_host.copy_(self._t["control"])
@@ -369,7 +373,8 @@ class IceFlow3D:
         staging = self._upload_staging_view
         self.rigid.advance_packed(loads, staging)
         self._body_state[...] = staging
-        self._t["body"].copy_(self._upload_staging, non_blocking=True)
+        check(self._lib.icy3d_copy_async(self._t["body"].data_ptr(), staging.ctypes.data, staging.nbytes,
+                                         self._stream()), "body upload")
         self._upload_done.record(self._torch_stream())
         return self._coverage_boxes(staging)
 
