@@ -346,6 +346,10 @@ __device__ __forceinline__ float staged_phi(const icy3d_grid& g, int i, int j, i
   return g.phi[lin(g, i, j, k)];
 }
 
+// Grids of the per-cell ice-list kernels (no cross-block sums): one resident
+// round of 256-thread blocks on 256 CUs at each kernel's occupancy (8 and 5
+// waves per SIMD), so ~10^4 listed cells run in one pass rather than two.
+constexpr unsigned kGradientListBlocks = 2048, kForceListBlocks = 1280;
 // gradients.py:15-20: every phase and coverage dependency of a gradient lies
 // within this many cells.
 constexpr int kGradientRadius = 6;
@@ -621,9 +625,26 @@ __global__ void __launch_bounds__(kFinishThreads) k_fold_recovery(icy3d_grid g, 
   float speed = 0.0f;
   int invalid = INT32_MAX;
   if (scan) {
-    for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
-      speed = fmaxf(speed, g.partials[blocks + b]);
-      const float v = g.partials[2 * blocks + b];
+    // Order-free maximum and minimum, four partials in flight per thread.
+    const float* speeds = g.partials + blocks;
+    const float* firsts = g.partials + 2 * blocks;
+    int b = threadIdx.x;
+    for (; b + 3 * kFinishThreads < blocks; b += 4 * kFinishThreads) {
+      float s4[4], v4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        s4[u] = speeds[b + u * kFinishThreads];
+        v4[u] = firsts[b + u * kFinishThreads];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        speed = fmaxf(speed, s4[u]);
+        if (v4[u] >= 0.0f) invalid = min(invalid, (int)v4[u]);
+      }
+    }
+    for (; b < blocks; b += kFinishThreads) {
+      speed = fmaxf(speed, speeds[b]);
+      const float v = firsts[b];
       if (v >= 0.0f) invalid = min(invalid, (int)v);
     }
     speed = block_max(speed);
@@ -1942,7 +1963,7 @@ int launch_gradients_list(const icy3d_grid& g, const BoxList& L, hipStream_t s) 
   BoxList tiles = L;
   for (int m = 0; m < L.count; ++m) tiles.first[m + 1] = tiles.first[m] + grad_tiles(L.lo[m], L.hi[m]);
   k_gradients_tiled<<<(unsigned)tiles.first[L.count], kGradJ * kGradK, 0, s>>>(g, tiles);
-  k_gradients_general_list<<<kListBlocks, kBlock, 0, s>>>(g, L);
+  k_gradients_general_list<<<kGradientListBlocks, kBlock, 0, s>>>(g, L);
   ICY_LAUNCH_CHECK();
   return 0;
 }
@@ -1952,7 +1973,7 @@ int launch_gradients(const icy3d_grid& g, const Box& b, bool, hipStream_t s) {
 int launch_forces(const icy3d_grid& g, int with_flux, int moment_pressure, hipStream_t s) {
   const dim3 tiles((g.nz + kForceK - 1) / kForceK, (g.ny + kForceJ - 1) / kForceJ, (g.nx + kForceX - 1) / kForceX);
   k_forces_tiled<<<tiles, kForceJ * kForceK, 0, s>>>(g, with_flux, moment_pressure);  // bulk and tank cells
-  k_forces_general<<<kListBlocks, kBlock, 0, s>>>(g, with_flux, moment_pressure);
+  k_forces_general<<<kForceListBlocks, kBlock, 0, s>>>(g, with_flux, moment_pressure);
   ICY_LAUNCH_CHECK();
   return 0;
 }
