@@ -61,10 +61,7 @@ STEP_BYTES = 161  # minimal persistent state per cell-update of the whole step
 # Fraction of the chip's unpacked FP32 VALU issue slots each kernel group uses,
 # from SQ_ACTIVE_INST_VALU and GRBM_GUI_ACTIVE (rocprofv3 --pmc runs of this
 # script; the summary is kept under profiles/).
-MEASURED_VALU_BUSY = {
-    "phase_collide": 0.378, "forces": 0.367, "home_collide": 0.166,
-    "home_stream": 0.413, "phase_recover": 0.490, "gradients": 0.223,
-}
+MEASURED_VALU_BUSY = {"phase_collide": 0.376, "forces": 0.376, "home_collide": 0.167, "home_stream": 0.422, "phase_recover": 0.512, "gradients": 0.245}
 MEASURED_TRAFFIC_BYTES = {
     "phase_collide": 3.652e8, "forces": 6.284e8, "home_collide": 5.248e8,
     "home_stream": 9.060e8, "phase_recover": 1.489e8, "gradients": 1.056e8,
