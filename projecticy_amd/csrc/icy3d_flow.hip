@@ -46,13 +46,14 @@ __device__ __noinline__ float reconstruct_rt(int q, const Moments& M) { return r
 
 // ---------------------------------------------------------------------------
 // AA addressing of D3Q7 (phase.py:69-82): plane offset of logical slot q.
-__device__ __forceinline__ size_t phase_addr(const icy3d_grid& g, const Cell& c, int q, int parity) {
-  const size_t N = cells(g);
+// 32-bit offsets: a valid grid has fewer than INT32_MAX / 16 cells, so 7 N fits.
+__device__ __forceinline__ int phase_addr(const icy3d_grid& g, const Cell& c, int q, int parity) {
+  const int N = (int)cells(g);
   if (parity != 0 && q != 0) {
     int i = c.i - kC7[q][0], j = c.j - kC7[q][1], k = c.k - kC7[q][2];
-    if (fluid_at(g, i, j, k)) return (size_t)opp7(q) * N + lin(g, i, j, k);
+    if (fluid_at(g, i, j, k)) return opp7(q) * N + lin(g, i, j, k);
   }
-  return (size_t)q * N + c.n;
+  return q * N + c.n;
 }
 
 // home.py:262-278
@@ -503,7 +504,7 @@ __global__ void k_collide_phase(icy3d_grid g, int parity, int moving_walls) {
   load_grad(g, n, gr);
   const float norm = sqrtf(gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2] + g.normal_eps * g.normal_eps + 1e-28f);
   const float nrm[3] = {gr[0] / norm, gr[1] / norm, gr[2] / norm};
-  size_t addr[7];
+  int addr[7];
   float in[7], post[7], fl[3];
 #pragma unroll
   for (int q = 0; q < 7; ++q) {
