@@ -493,10 +493,14 @@ __global__ void k_capillary_stress(icy3d_grid g, float* __restrict__ out) {
 
 // ---------------------------------------------------------------------------
 // D3Q7 AA collision (phase.py:93-127).
+// moving_walls == 2: as 1, but the link classes are current, so cells of
+// class 0 or 1 (no ice or cavity cell among their 26 neighbours) skip the
+// neighbour labels.
 __global__ void k_collide_phase(icy3d_grid g, int parity, int moving_walls) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   const size_t N = cells(g);
   if (n >= (int)N || g.id[n] != -1) return;
+  if (moving_walls == 2) moving_walls = g.flag[n] == 2;
   const Cell c = cell_of(g, n);
   const float phi = g.phi[n];
   const float u[3] = {g.u[n], g.u[N + n], g.u[2 * N + n]};
@@ -2083,7 +2087,7 @@ int icy3d_build_boundary_lists(const icy3d_grid* g, void* stream) {
 }
 int icy3d_collide_phase(const icy3d_grid* g, int parity, int moving_walls, void* stream) {
   if (!valid(g)) return ICY3D_ERR_ARGUMENT;
-  k_collide_phase<<<icy_blocks(grid_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g, parity, moving_walls);
+  k_collide_phase<<<icy_blocks(grid_cells(*g)), kBlock, 0, (hipStream_t)stream>>>(*g, parity, moving_walls ? 1 : 0);
   ICY_LAUNCH_CHECK();
   return 0;
 }
@@ -2160,7 +2164,7 @@ int icy3d_flow_step_profiled(const icy3d_grid* g, int parity, int refresh_gradie
     if (rc) return rc;
   }
   mark(0);
-  k_collide_phase<<<blocks, kBlock, 0, s>>>(G, parity, 1);
+  k_collide_phase<<<blocks, kBlock, 0, s>>>(G, parity, 2);
   parity = 1 - parity;
   mark(1);
   rc = launch_forces(G, 1, 1, s);
@@ -2198,7 +2202,7 @@ int icy3d_flow_step_begin(const icy3d_grid* g, int parity, int refresh_gradients
     rc = launch_gradients(G, full_box(G), true, s);
     if (rc) return rc;
   }
-  k_collide_phase<<<blocks, kBlock, 0, s>>>(G, parity, 1);
+  k_collide_phase<<<blocks, kBlock, 0, s>>>(G, parity, 2);
   rc = launch_forces(G, 1, 1, s);
   if (rc) return rc;
   k_collide_momentum<<<blocks, kBlock, 0, s>>>(G, 1);
