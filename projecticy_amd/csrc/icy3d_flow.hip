@@ -1006,7 +1006,7 @@ __device__ __forceinline__ void forces_cell(const icy3d_grid& g, int n, int with
 // three slabs.  The six face-neighbour terms (donor phase flux, viscous
 // velocity gradient) read u, flux and phi directly, the x neighbours from the
 // same register pipeline.
-constexpr int kForceJ = 4, kForceK = 64, kForceX = 16;
+constexpr int kForceJ = 4, kForceK = 64, kForceX = 32;
 constexpr int kForceHJ = kForceJ + 2, kForceHK = kForceK + 2, kForceSlab = kForceHJ * kForceHK;
 constexpr int kForceRounds = (kForceSlab + kForceJ * kForceK - 1) / (kForceJ * kForceK);
 // Staged per cell: Ax, Ay, Az, Bxy, Bxz, Byz, X1, X2 (stencil sums), then
